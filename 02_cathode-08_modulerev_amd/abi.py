@@ -1,0 +1,150 @@
+"""ctypes mirror of include/cathode_hip.h and the loader of libcathode_hip.so.
+
+The library is the product: this module only binds its C ABI for Python
+callers (tests, bench.py, __graft_entry__).  Loading fails loudly when the
+shared object is missing -- there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "csrc", "libcathode_hip.so")
+
+CATHODE_OK = 0
+CATHODE_MODE_FP64 = 0
+CATHODE_MODE_MIXED = 1
+SEE_MAX = 256
+
+c_double_p = ctypes.POINTER(ctypes.c_double)
+c_int_p = ctypes.POINTER(ctypes.c_int)
+
+
+class Status(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int), ("converged", ctypes.c_int),
+                ("rel_residual", ctypes.c_double), ("seconds", ctypes.c_double)]
+
+
+class NddParams(ctypes.Structure):
+    _fields_ = [
+        ("ni", ctypes.c_int), ("nj", ctypes.c_int),
+        ("dx", ctypes.c_double), ("dr", ctypes.c_double), ("dt", ctypes.c_double),
+        ("rmin", ctypes.c_double), ("V_bias", ctypes.c_double),
+        ("mu_e", ctypes.c_double), ("mu_i", ctypes.c_double), ("Ti", ctypes.c_double),
+        ("masse", ctypes.c_double), ("massi", ctypes.c_double),
+        ("coefIISEE_ts", ctypes.c_double), ("coefMISEE_ts", ctypes.c_double),
+        ("Te_emitSEE_eV", ctypes.c_double), ("ratioEngy_EISEE_rd", ctypes.c_double),
+        ("dTe_SEE", ctypes.c_double), ("n_see", ctypes.c_int), ("energy_on", ctypes.c_int),
+        ("maxITR_HES_phi", ctypes.c_int), ("maxITR_HES_rhoe", ctypes.c_int),
+        ("maxITR_HES_rhoeps", ctypes.c_int), ("maxITR_HES_rhoi", ctypes.c_int),
+        ("error_cnv_HES_phi", ctypes.c_double), ("error_cnv_HES_rhoe", ctypes.c_double),
+        ("error_cnv_HES_rhoeps", ctypes.c_double), ("error_cnv_HES_rhoi", ctypes.c_double),
+        ("flag_error", ctypes.c_int), ("maxITR", ctypes.c_int), ("tol", ctypes.c_double),
+        ("mode", ctypes.c_int), ("learned", ctypes.c_int),
+        ("see_eb", ctypes.c_double * SEE_MAX), ("see_rd", ctypes.c_double * SEE_MAX),
+        ("see_ts", ctypes.c_double * SEE_MAX),
+    ]
+
+
+FIELD_NAMES = ["phi", "rhoe", "rhoi", "Te", "rhoeps", "rhoe_old",
+               "Ex", "rhoUex", "nUex", "Gx", "rhoUix",
+               "Er", "rhoUer", "nUer", "Gr", "rhoUir",
+               "rate_ionize", "Pabs", "rate_eloss"]
+STATE_NAMES = FIELD_NAMES[:16]
+
+
+class NddFields(ctypes.Structure):
+    _fields_ = [(n, c_double_p) for n in FIELD_NAMES]
+
+
+class NddReport(ctypes.Structure):
+    _fields_ = [
+        ("steps", ctypes.c_int), ("solves", ctypes.c_int), ("total_iterations", ctypes.c_int),
+        ("max_iterations", ctypes.c_int), ("nonconverged", ctypes.c_int),
+        ("max_rel_residual", ctypes.c_double),
+        ("error_rhoi", ctypes.c_double), ("error_phi", ctypes.c_double),
+        ("error_nUex", ctypes.c_double), ("error_nUer", ctypes.c_double),
+        ("error_rhoe", ctypes.c_double), ("error_rhoUex", ctypes.c_double),
+        ("error_rhoUer", ctypes.c_double), ("error_rhoeps", ctypes.c_double),
+        ("error_Gx", ctypes.c_double), ("error_Gr", ctypes.c_double),
+        ("I_anode", ctypes.c_double), ("Ii_anode", ctypes.c_double), ("Ie_anode", ctypes.c_double),
+        ("seconds", ctypes.c_double),
+    ]
+
+
+def dptr(a):
+    """Pointer to a C-contiguous float64 numpy array (None -> NULL)."""
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"], "need contiguous float64"
+    return a.ctypes.data_as(c_double_p)
+
+
+def fields_struct(state):
+    """NddFields pointing into a dict of numpy arrays (missing keys -> NULL)."""
+    f = NddFields()
+    for n in FIELD_NAMES:
+        setattr(f, n, dptr(state.get(n)))
+    return f
+
+
+# exported symbols and their signatures: (name, restype, argtypes)
+_D = c_double_p
+_SOLVE5 = [_D, _D, _D, _D, _D, _D, c_int_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+SIGNATURES = [
+    ("cathode_init", ctypes.c_int, [ctypes.c_int]),
+    ("cathode_finalize", ctypes.c_int, []),
+    ("cathode_abi_version", ctypes.c_int, []),
+    ("cathode_last_error", ctypes.c_char_p, []),
+    ("cathode_last_status", None, [ctypes.POINTER(Status)]),
+    ("cathode_struct5_solve", ctypes.c_int,
+     _SOLVE5 + [ctypes.c_int, ctypes.c_double, ctypes.c_int, _D, ctypes.POINTER(Status)]),
+    ("cathode_struct5_solve_dev", ctypes.c_int,
+     _SOLVE5 + [ctypes.c_int, ctypes.c_double, ctypes.c_int, _D, ctypes.c_void_p,
+                ctypes.POINTER(Status)]),
+    ("cathode_sor_solve", ctypes.c_int,
+     _SOLVE5 + [ctypes.c_double, ctypes.c_int, ctypes.c_double, _D, ctypes.POINTER(Status)]),
+    ("cathode_lu9_solve", ctypes.c_int,
+     [_D] * 10 + [c_int_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _D,
+                  ctypes.POINTER(Status)]),
+    ("cathode_helm5_solve", ctypes.c_int,
+     _SOLVE5 + [ctypes.c_int, ctypes.c_double, ctypes.c_int, _D, ctypes.POINTER(Status)]),
+    ("cathode_ndd_create", ctypes.c_int, [ctypes.POINTER(NddParams), ctypes.POINTER(ctypes.c_void_p)]),
+    ("cathode_ndd_set_state", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(NddFields)]),
+    ("cathode_ndd_get_state", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(NddFields)]),
+    ("cathode_ndd_step", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(NddReport)]),
+    ("cathode_ndd_destroy", ctypes.c_int, [ctypes.c_void_p]),
+    ("cathode_ndd_profile", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("cathode_ndd_profile_read", ctypes.c_int,
+     [ctypes.c_void_p, c_int_p, c_double_p, c_double_p]),
+]
+
+
+def bind(lib):
+    for name, res, args in SIGNATURES:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+_LIB = None
+
+
+def load(path=LIB_PATH):
+    """Load and bind libcathode_hip.so; raises if it has not been built."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(path):
+            raise RuntimeError(
+                f"libcathode_hip.so not built ({path}); run __graft_entry__.build() first")
+        _LIB = bind(ctypes.CDLL(path))
+    return _LIB
+
+
+def check(rc, lib=None):
+    if rc != CATHODE_OK:
+        lib = lib or _LIB
+        msg = lib.cathode_last_error().decode() if lib is not None else ""
+        raise RuntimeError(f"cathode call failed ({rc}): {msg}")

@@ -1,0 +1,148 @@
+"""Synthetic NDD / microwave workloads (host numpy arrays, float64).
+
+Each builder returns (params, state) where params is an abi.NddParams and
+state a dict of (ni+2, nj+2) arrays named as in include/cathode_hip.h.  The
+same arrays feed the HIP library and the CPU oracle.  Random draws use
+numpy's PCG64 with the seed given; physical constants are those of the
+reference (include/constants.hpp:5-8) and the material constants of its
+input/setup.yaml (masse, massi, Ti).
+"""
+import numpy as np
+
+from . import abi
+
+BOLTZ = 1.380649e-23
+E0 = 1.60217662e-19
+EPS0 = 8.8541878128e-12
+MU0 = 1.2566370614e-6
+MASSE = 9.10938356e-31
+MASSI = 2.18017156e-25
+TI = 300.0
+
+
+def default_params(ni, nj, dx, dr, dt=1e-10, V_bias=30.0, energy_on=False,
+                   mode=abi.CATHODE_MODE_FP64):
+    p = abi.NddParams()
+    p.ni, p.nj = ni, nj
+    p.dx, p.dr, p.dt = dx, dr, dt
+    p.rmin = 0.0
+    p.V_bias = V_bias
+    p.mu_e, p.mu_i = 10.0, 0.1
+    p.Ti = TI
+    p.masse, p.massi = MASSE, MASSI
+    # secondary emission off for the synthetic configs (no SEE in their spec)
+    p.coefIISEE_ts = 0.0
+    p.coefMISEE_ts = 0.0
+    p.Te_emitSEE_eV = 2.5
+    p.ratioEngy_EISEE_rd = 0.5
+    p.dTe_SEE = 0.0
+    p.n_see = 0
+    p.energy_on = 1 if energy_on else 0
+    # input/setup.yaml solver_setting
+    p.maxITR_HES_phi = p.maxITR_HES_rhoe = p.maxITR_HES_rhoeps = p.maxITR_HES_rhoi = 2
+    p.error_cnv_HES_phi = p.error_cnv_HES_rhoe = p.error_cnv_HES_rhoeps = 1e-21
+    p.error_cnv_HES_rhoi = 1e-21
+    p.flag_error = 3
+    p.maxITR = 30
+    p.tol = 1e-10
+    p.mode = mode
+    p.learned = 0
+    return p
+
+
+def radii(nj, dr, rmin=0.0):
+    return rmin + (np.arange(nj + 2) - 0.5) * dr
+
+
+def axial(ni, dx, xL=0.0):
+    return xL + (np.arange(ni + 2) - 0.5) * dx
+
+
+def consistent_state(p, n_e, n_i, Te_K, phi=None):
+    """Fill every state array from cell densities/temperature: ghost cells,
+    the potential's ghost mirrors (phi = 0 left, V_bias right, grad 0 top)
+    and the face fields E = -grad phi on every face, fluxes zero."""
+    ni, nj = p.ni, p.nj
+    shp = (ni + 2, nj + 2)
+    st = {k: np.zeros(shp) for k in abi.STATE_NAMES}
+
+    def ghosts(a):
+        a[0, :] = a[1, :]
+        a[ni + 1, :] = a[ni, :]
+        a[:, 0] = a[:, 1]
+        a[:, nj + 1] = a[:, nj]
+        return a
+
+    st["rhoe"][...] = ghosts(n_e.copy())
+    st["rhoi"][...] = ghosts(n_i.copy())
+    st["Te"][...] = ghosts(Te_K.copy())
+    st["rhoe_old"][...] = st["rhoe"]
+    st["rhoeps"][...] = 1.5 * st["rhoe"] * BOLTZ * st["Te"]
+    ph = np.zeros(shp) if phi is None else phi.copy()
+    ph[0, 1:nj + 1] = -ph[1, 1:nj + 1]
+    ph[ni + 1, 1:nj + 1] = 2.0 * p.V_bias - ph[ni, 1:nj + 1]
+    ph[1:ni + 1, nj + 1] = ph[1:ni + 1, nj]
+    st["phi"][...] = ph
+    Ex = st["Ex"]
+    Ex[1:ni + 2, 1:nj + 1] = -(ph[1:ni + 2, 1:nj + 1] - ph[0:ni + 1, 1:nj + 1]) / p.dx
+    Er = st["Er"]
+    Er[1:ni + 1, 2:nj + 1] = -(ph[1:ni + 1, 2:nj + 1] - ph[1:ni + 1, 1:nj]) / p.dr
+    Er[1:ni + 1, nj + 1] = 0.0
+    return st
+
+
+def config0(seed=0, ni=96, nj=64, h=1e-4, V_bias=30.0, mode=abi.CATHODE_MODE_FP64):
+    """Plain NDD step: Gaussian ne = ni of 1e17 peak (sigma 1 mm at z = 4.8 mm,
+    r = 0) on a 1e14 floor with a shared +-1% uniform noise, Te = 5 eV."""
+    p = default_params(ni, nj, h, h, V_bias=V_bias, energy_on=False, mode=mode)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    z = axial(ni, h)[:, None]
+    r = radii(nj, h)[None, :]
+    zc = 0.5 * ni * h  # 4.8 mm at 96 cells of 0.1 mm
+    n = 1e14 + 1e17 * np.exp(-((z - zc) ** 2 + r ** 2) / (2.0 * (1e-3) ** 2))
+    u = rng.uniform(-1.0, 1.0, size=n.shape)
+    n = n * (1.0 + 0.01 * u)
+    Te = np.full(n.shape, 5.0 * E0 / BOLTZ)
+    return p, consistent_state(p, n, n.copy(), Te)
+
+
+def multi_gaussian(rng, ni, nj, h, count=4):
+    z = axial(ni, h)[:, None]
+    r = radii(nj, h)[None, :]
+    n = np.full((ni + 2, nj + 2), 1e14)
+    Lz, Lr = ni * h, nj * h
+    for _ in range(count):
+        amp = 10.0 ** rng.uniform(16.0, 18.0)
+        sig = rng.uniform(0.5e-3, 2.0e-3)
+        zc = rng.uniform(0.0, Lz)
+        rc = rng.uniform(0.0, Lr)
+        n = n + amp * np.exp(-((z - zc) ** 2 + (r - rc) ** 2) / (2.0 * sig * sig))
+    return n
+
+
+def smooth_field(rng, ni, nj, h):
+    """Normalised sum of three low-order cosine modes, values in [0, 1]."""
+    z = axial(ni, h)[:, None] / (ni * h)
+    r = radii(nj, h)[None, :] / (nj * h)
+    s = np.zeros((ni + 2, nj + 2))
+    for _ in range(3):
+        kz, kr = rng.integers(0, 3, size=2)
+        ph = rng.uniform(0, 2 * np.pi, size=2)
+        s = s + rng.uniform(0.5, 1.0) * np.cos(np.pi * kz * z + ph[0]) * np.cos(np.pi * kr * r + ph[1])
+    s = s - s.min()
+    return s / s.max()
+
+
+def config1(seed=1, ni=512, nj=256, h=5e-5, V_bias=30.0, mode=abi.CATHODE_MODE_FP64):
+    """Mixed-precision single instance: ne = ni = 1e14 + four Gaussians
+    (peaks log-uniform 1e16..1e18, sigma 0.5..2 mm), Te = 2 + 8 s(x) eV with
+    s a smooth seeded field, electron energy on."""
+    p = default_params(ni, nj, h, h, V_bias=V_bias, energy_on=True, mode=mode)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    n = multi_gaussian(rng, ni, nj, h)
+    Te = (2.0 + 8.0 * smooth_field(rng, ni, nj, h)) * E0 / BOLTZ
+    return p, consistent_state(p, n, n.copy(), Te)
+
+
+def copy_state(st):
+    return {k: v.copy() for k, v in st.items()}

@@ -1,0 +1,196 @@
+/*
+ * cathode_hip.h -- C ABI of the MI355X (gfx950) implicit-solve library.
+ *
+ * Every entry point is extern "C" with plain pointers and sizes.  Arrays are
+ * row-major (ni+2) x ld doubles indexed [i][j] with i = z (axial, slow) and
+ * j = r (radial, contiguous); ld >= nj+2.  This is the layout of the
+ * reference's vector<vector<double>>(ni+2, vector<double>(nj+2)) fields
+ * (include/arrays.hpp:144-278), flattened.  Complex arrays are interleaved
+ * (re, im) pairs of doubles with the same indexing.
+ *
+ * Matrix convention (include/solverFuncs.hpp:77-82): A x = b with
+ * A[P][P] = aP and A[P][nb] = -a_nb for nb in {E(i+1), W(i-1), N(j+1), S(j-1)}.
+ * Unknowns exist only on the union of n_bl inclusive boxes
+ * boxes[4*b + 0..3] = {ilo, ihi, jlo, jhi}; couplings to cells outside the
+ * union are dropped; x is written only on in-box cells.
+ *
+ * Return codes: 0 = CATHODE_OK, negative = error (message from
+ * cathode_last_error()).  Non-convergence is NOT an error (it is silent in the
+ * reference, include/solverFuncs.hpp:124-141); it is reported through
+ * cathode_status.converged and cathode_last_status().
+ */
+#ifndef CATHODE_HIP_H
+#define CATHODE_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CATHODE_ABI_VERSION 1
+
+enum {
+    CATHODE_OK = 0,
+    CATHODE_ERR_ARG = -1,     /* invalid argument (shape, box, null pointer) */
+    CATHODE_ERR_HIP = -2,     /* HIP runtime error */
+    CATHODE_ERR_NOINIT = -3,  /* cathode_init() not called */
+    CATHODE_ERR_NOMEM = -4,   /* device allocation failed */
+    CATHODE_ERR_NUMERIC = -5  /* non-finite value met inside a solve */
+};
+
+/* precision modes of the multigrid solves */
+enum {
+    CATHODE_MODE_FP64 = 0,   /* FP64 everywhere */
+    CATHODE_MODE_MIXED = 1   /* FP32 V-cycles, FP64 residual / refinement / norms */
+};
+
+typedef struct cathode_status {
+    int iterations;       /* outer iterations taken (V-cycles or SOR sweeps) */
+    int converged;        /* 1 when the stopping test was met */
+    double rel_residual;  /* ||b - A x||_2 / ||b||_2 in FP64 (SOR: last max relative change) */
+    double seconds;       /* host wall time of the call */
+} cathode_status;
+
+/* ------------------------------------------------------------------ */
+/* lifecycle                                                          */
+/* ------------------------------------------------------------------ */
+int cathode_init(int device);     /* selects the device, creates the library stream */
+int cathode_finalize(void);       /* frees every cached device buffer */
+int cathode_abi_version(void);
+const char* cathode_last_error(void);
+void cathode_last_status(cathode_status* st);
+
+/* ------------------------------------------------------------------ */
+/* structured 5-point solves (drop-in for include/solverFuncs.hpp)    */
+/* ------------------------------------------------------------------ */
+
+/* Replaces solver_SMG (include/solverFuncs.hpp:26-162): solves A x = b on
+ * the box union to ||b-Ax||_2/||b||_2 <= tol or maxITR iterations, from a
+ * zero initial guess (the reference never sets xvec, :106-109).  Host
+ * arrays; x in/out (only in-box cells are written).  mode = CATHODE_MODE_*. */
+int cathode_struct5_solve(const double* aP, const double* aE, const double* aW,
+                          const double* aN, const double* aS, const double* b,
+                          const int* boxes, int n_bl, int ni2, int nj2, int ld,
+                          int maxITR, double tol, int mode,
+                          double* x, cathode_status* st);
+
+/* Same solve on device-resident arrays (pointers from hipMalloc, same
+ * layout); stream may be NULL (library stream).  Used by callers that keep
+ * their fields on the GPU. */
+int cathode_struct5_solve_dev(const double* aP, const double* aE, const double* aW,
+                              const double* aN, const double* aS, const double* b,
+                              const int* boxes, int n_bl, int ni2, int nj2, int ld,
+                              int maxITR, double tol, int mode,
+                              double* x, void* stream, cathode_status* st);
+
+/* Replaces solver_SOR (include/solverFuncs.hpp:169-216): lexicographic
+ * point SOR (blocks in order, i then j), x_new = (1-w) x + w (sum a_nb x_nb
+ * + b)/(aP + 1e-100); stops after a sweep whose max relative change
+ * 2|dx|/|x_new + x_old + 1e-6| <= tol or after the sweep that makes the
+ * count exceed maxITR.  x is the initial guess (in/out), as in the
+ * reference.  Runs as an anti-diagonal wavefront on the GPU, so the
+ * update order -- and the result -- are those of the sequential sweep. */
+int cathode_sor_solve(const double* aP, const double* aE, const double* aW,
+                      const double* aN, const double* aS, const double* b,
+                      const int* boxes, int n_bl, int ni2, int nj2, int ld,
+                      double alpha, int maxITR, double tol,
+                      double* x, cathode_status* st);
+
+/* Replaces solver_LU_9p (include/solverFuncs.hpp:223-378): 9-point system
+ * (the four diagonal neighbours EN=(i+1,j+1), ES=(i+1,j-1), WN=(i-1,j+1),
+ * WS=(i-1,j-1) added, same sign convention).  The reference factorizes
+ * exactly; this solves to ||b-Ax||/||b|| <= 1e-13 (or maxITR = 200
+ * iterations) with a 5-point multigrid preconditioned BiCGStab. */
+int cathode_lu9_solve(const double* aP, const double* aE, const double* aW,
+                      const double* aN, const double* aS,
+                      const double* aEN, const double* aES, const double* aWN,
+                      const double* aWS, const double* b,
+                      const int* boxes, int n_bl, int ni2, int nj2, int ld,
+                      double* x, cathode_status* st);
+
+/* Complex 5-point solve (E_theta Helmholtz system of
+ * include/emfieldModule.hpp:2414-2454 folded with its BCs :2713-2773, which
+ * the reference hands to Eigen SparseLU :2994-2998).  Interleaved complex128
+ * arrays.  mode FP64 = complex128 multigrid; MIXED = complex64 V-cycles with
+ * complex128 residual refinement. */
+int cathode_helm5_solve(const double* aP, const double* aE, const double* aW,
+                        const double* aN, const double* aS, const double* b,
+                        const int* boxes, int n_bl, int ni2, int nj2, int ld,
+                        int maxITR, double tol, int mode,
+                        double* x, cathode_status* st);
+
+/* ------------------------------------------------------------------ */
+/* device-resident implicit NDD step                                  */
+/* (FluidModule::solve_phi_couple_wdTe_wSEE_PC, solve_rhoe_wdTe_wSEE_PC,
+ *  solve_Te_wdTe_wSEE_PC, include/fluidModule.hpp:2080-4957, with the ion
+ *  continuity solved by the same Scharfetter-Gummel scheme at charge +1)  */
+/* ------------------------------------------------------------------ */
+
+#define CATHODE_SEE_TABLE_MAX 256
+
+typedef struct cathode_ndd_params {
+    int ni, nj;                 /* interior cells in z and r */
+    double dx, dr, dt;          /* Params::dx, dr, dt */
+    double rmin;                /* Params::rmin (0 puts the axis at the lower face) */
+    double V_bias;              /* potential of the right (z = zmax) electrode */
+    double mu_e, mu_i;          /* isotropic mobilities (B = 0) [m^2/Vs] */
+    double Ti;                  /* ion temperature [K] */
+    double masse, massi;        /* [kg] */
+    double coefIISEE_ts, coefMISEE_ts, Te_emitSEE_eV, ratioEngy_EISEE_rd;
+    double dTe_SEE;             /* EISEE table spacing [K] (SeeVec, inputModule.hpp:651-652) */
+    int n_see;                  /* EISEE table length, 0 = no electron-induced emission */
+    int energy_on;              /* 1 = solve the electron energy equation */
+    int maxITR_HES_phi, maxITR_HES_rhoe, maxITR_HES_rhoeps, maxITR_HES_rhoi;
+    double error_cnv_HES_phi, error_cnv_HES_rhoe, error_cnv_HES_rhoeps, error_cnv_HES_rhoi;
+    int flag_error;             /* calcRes mode 0..3 (miniFuncs.hpp:292-375) */
+    int maxITR;                 /* maxITR_SOR_*: linear-solve iteration cap */
+    double tol;                 /* error_cnv_SOR_*: linear-solve relative residual */
+    int mode;                   /* CATHODE_MODE_* for every linear solve */
+    int learned;                /* 1 = learned initial guess for the Poisson correction */
+    double see_eb[CATHODE_SEE_TABLE_MAX];
+    double see_rd[CATHODE_SEE_TABLE_MAX];
+    double see_ts[CATHODE_SEE_TABLE_MAX];
+} cathode_ndd_params;
+
+/* Host field set, each (ni+2)*(nj+2) doubles, ld = nj+2.  Face arrays use
+ * the reference's GridInterfaceX/R convention: x-face i lies between cells
+ * i-1 and i, r-face j between j-1 and j.  Source arrays may be NULL (zero). */
+typedef struct cathode_ndd_fields {
+    double *phi, *rhoe, *rhoi, *Te, *rhoeps, *rhoe_old;   /* cell centred */
+    double *Ex, *rhoUex, *nUex, *Gx, *rhoUix;              /* x faces */
+    double *Er, *rhoUer, *nUer, *Gr, *rhoUir;              /* r faces */
+    double *rate_ionize, *Pabs, *rate_eloss;               /* sources */
+} cathode_ndd_fields;
+
+typedef struct cathode_ndd_report {
+    int steps;                  /* time steps taken by the call */
+    int solves;                 /* linear solves performed */
+    int total_iterations;       /* sum of V-cycle iterations over the solves */
+    int max_iterations;         /* largest iteration count of one solve */
+    int nonconverged;           /* solves that hit maxITR */
+    double max_rel_residual;    /* largest final relative residual */
+    /* step-level calcRes norms of the last step (Params::error_*) */
+    double error_rhoi, error_phi, error_nUex, error_nUer, error_rhoe, error_rhoUex,
+           error_rhoUer, error_rhoeps, error_Gx, error_Gr;
+    double I_anode, Ii_anode, Ie_anode;   /* right-electrode current [A] (fluidModule.hpp:3942-3965) */
+    double seconds;             /* host wall time of the call */
+} cathode_ndd_report;
+
+int cathode_ndd_create(const cathode_ndd_params* p, void** handle);
+int cathode_ndd_set_state(void* handle, const cathode_ndd_fields* host);
+int cathode_ndd_get_state(void* handle, cathode_ndd_fields* host);
+int cathode_ndd_step(void* handle, int nsteps, cathode_ndd_report* rep);
+int cathode_ndd_destroy(void* handle);
+
+/* Kernel timing of the dominant stencil kernel inside cathode_ndd_step:
+ * when enabled, HIP events bracket every launch of the fine-grid
+ * red-black smoothing kernel on the step's stream.  Returns the launch
+ * count, the summed duration [ms] and the algorithmic bytes of one
+ * launch (coefficients + b + x read + x written at storage width). */
+int cathode_ndd_profile(void* handle, int enable);
+int cathode_ndd_profile_read(void* handle, int* launches, double* total_ms,
+                             double* bytes_per_launch);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CATHODE_HIP_H */

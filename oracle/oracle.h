@@ -1,0 +1,143 @@
+/*
+ * oracle.h -- CPU FP64 restatement of the reference's implicit solve path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product links, loads or calls
+ * this code: it is the checker that the HIP library is compared against
+ * (tests/, __graft_entry__.smoke()) and the CPU baseline that bench.py times
+ * beside the GPU.  It is re-derived from the formulas of the reference files
+ * cited at each function (paths relative to the reference root); no reference
+ * source is copied.
+ *
+ * Parity pins (what fixes each function independently of this restatement):
+ *   orc_band_solve / orc_struct5_direct   brute-force dense solve (numpy) on small grids,
+ *                                         residual ||b-Ax||/||b|| at roundoff
+ *   orc_sor_solve                         hand-computed sweeps on 2x2 / 3x3 systems, fixed
+ *                                         point = direct solution
+ *   orc_mg_solve                          agrees with orc_struct5_direct to tolerance
+ *   orc_bern_*                            B(0)=1, B'(0)=-1/2, B(-z)-B(z)=z, large-|z| limits
+ *   orc_ndd_step (Poisson)                exactly linear potential for n_e=n_i, no flux
+ *                                         (closed form for any r-weighting)
+ *   orc_ndd_step (SG continuity)          exponential 1-D steady profile is reproduced
+ *                                         exactly; M-matrix signs; global conservation
+ *                                         identity (sum of volume divergence = wall flux)
+ *   orc_helm_*                            manufactured solution; PEC cavity eigenvalue;
+ *                                         Pabs closed form 0.5 Re(sigma)|E|^2
+ *   orc_abcd_*                            S -> ABCD -> S round trip is the identity
+ * See tests/test_oracle_*.py for each check.
+ */
+#ifndef CATHODE_ORACLE_H
+#define CATHODE_ORACLE_H
+
+#include "../include/cathode_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- physical constants (include/constants.hpp:5-8), same values ---- */
+#define ORC_BOLTZ 1.380649e-23
+#define ORC_E0 1.60217662e-19
+#define ORC_EPS0 8.8541878128e-12
+#define ORC_MU0 1.2566370614e-6
+#define ORC_LIM_EXP 1.0e2   /* fluidModule.hpp:3098 */
+#define ORC_EPS_VD 1.0e-3   /* fluidModule.hpp:3099 */
+
+/* ---- Scharfetter-Gummel weights (fluidModule.hpp:3239-3243, 3641-3649) ----
+ * bp = vd/(1-exp(min(-z,100)))  weight of the upstream (i-1 / j-1) density
+ * bm = vd/(exp(min(z,100))-1)   weight of the downstream (i / j) density
+ * Canonical form: the differences are taken with expm1 and z == 0 returns
+ * the limit D/h.  literal != 0 evaluates the reference's exp()-1 text. */
+double orc_bern_bp(double vd, double z, double D_over_h, int literal);
+double orc_bern_bm(double vd, double z, double D_over_h, int literal);
+void orc_set_literal_bernoulli(int on);
+
+/* ---- 5-point solves with the HYPRE box semantics (solverFuncs.hpp:26-162) ---- */
+/* Exact direct solve (banded LU, no pivoting: the SG/Poisson matrices are
+ * M-matrices).  Returns 0, or -1 on a zero pivot. */
+int orc_struct5_direct(const double* aP, const double* aE, const double* aW,
+                       const double* aN, const double* aS, const double* b,
+                       const int* boxes, int n_bl, int ni2, int nj2, int ld,
+                       double* x);
+/* complex128 variant with partial pivoting (interleaved re/im arrays) */
+int orc_helm5_direct(const double* aP, const double* aE, const double* aW,
+                     const double* aN, const double* aS, const double* b,
+                     const int* boxes, int n_bl, int ni2, int nj2, int ld,
+                     double* x);
+/* 9-point exact solve (solverFuncs.hpp:223-378), partial pivoting */
+int orc_lu9_direct(const double* aP, const double* aE, const double* aW,
+                   const double* aN, const double* aS, const double* aEN,
+                   const double* aES, const double* aWN, const double* aWS,
+                   const double* b, const int* boxes, int n_bl, int ni2,
+                   int nj2, int ld, double* x);
+/* Point SOR exactly as solverFuncs.hpp:169-216; returns the sweep count,
+ * *err_out = last max relative change. */
+int orc_sor_solve(const double* aP, const double* aE, const double* aW,
+                  const double* aN, const double* aS, const double* b,
+                  const int* boxes, int n_bl, int ni2, int nj2, int ld,
+                  double alpha, int maxITR, double tol, double* x,
+                  double* err_out);
+/* FP64 residual norm ||b-Ax||_2/||b||_2 over the box union */
+double orc_rel_residual(const double* aP, const double* aE, const double* aW,
+                        const double* aN, const double* aS, const double* b,
+                        const int* boxes, int n_bl, int ni2, int nj2, int ld,
+                        const double* x);
+
+/* ---- CPU multigrid: the algorithm of the plain-FP64 GPU path, on the host.
+ * Used as the CPU baseline and to cross-check the GPU iteration counts.
+ * nthreads > 1 uses OpenMP.  Returns iterations; *relres = final residual. */
+int orc_mg_solve(const double* aP, const double* aE, const double* aW,
+                 const double* aN, const double* aS, const double* b,
+                 const int* boxes, int n_bl, int ni2, int nj2, int ld,
+                 int maxITR, double tol, double* x, double* relres);
+void orc_mg_set_threads(int nthreads);
+
+/* ---- implicit NDD step (fluidModule.hpp:1405-1755 ion (restated as SG at
+ * charge +1), :2080-3075 Poisson, :3082-4000 electrons, :4006-4957 energy).
+ * solver: 0 = exact direct solves, 1 = CPU multigrid to tol (the baseline). */
+int orc_ndd_step(const cathode_ndd_params* p, cathode_ndd_fields* f,
+                 int nsteps, int solver, cathode_ndd_report* rep);
+
+/* One assembly of a single equation from the current state, for tests:
+ * which = 0 Poisson (first PC iteration), 1 electrons, 2 ions, 3 energy.
+ * Fills aP..aS, b ((ni+2)*(nj+2) each, ld = nj+2) after the BC folds, and
+ * the predicted face fluxes into fx / fr. */
+int orc_ndd_assemble(const cathode_ndd_params* p, const cathode_ndd_fields* f,
+                     int which, double* aP, double* aE, double* aW, double* aN,
+                     double* aS, double* b, double* fx, double* fr);
+
+/* calcRes (miniFuncs.hpp:292-375) over an index box, mode 0..3 */
+double orc_calc_res(const double* vnew, const double* vold, int ld, int i0,
+                    int i1, int j0, int j1, double eps, int mode);
+
+/* EISEE table lerp func_Te (miniFuncs.hpp:35-46) */
+double orc_func_Te(const double* table, int n, double Te, double dTe);
+
+/* ---- E_theta microwave system (emfieldModule.hpp:2414-2454, PEC BCs
+ * :2713-2773, ghosts :3213-3264) and Pabs (:7739-7741) ---- */
+typedef struct orc_helm_params {
+    int ni, nj;
+    double dx, dr, rmin;
+    double omega, nu, epsr;   /* angular frequency, collision frequency, background eps_r */
+    double masse;
+    int pec_z, pec_rmax;      /* 1 = PEC at both z walls / at r = rmax; 0 = open (first-order ABC) */
+} orc_helm_params;
+/* Assembles the folded E_theta system (interleaved complex arrays) from
+ * n_e (real) and the excitation current J (complex). */
+int orc_helm_assemble(const orc_helm_params* p, const double* ne, const double* J,
+                      double* aP, double* aE, double* aW, double* aN, double* aS,
+                      double* b);
+/* Fills ghost cells of E after a solve (PEC mirrors) and computes
+ * Pabs = 0.5 Re(J E*) with J = sigma E per cell; returns the volume
+ * integral sum Pabs 2 pi r dx dr. */
+double orc_helm_post(const orc_helm_params* p, const double* ne, double* E,
+                     double* Pabs);
+
+/* ---- 2x2 ABCD / S-parameter algebra (cmpMat2.hpp:8-34,
+ * initialModule.hpp:1689-1739); complex numbers as (re, im) pairs ---- */
+void orc_s_to_abcd(const double* S /*8: S11,S12,S21,S22*/, double Z0, double* abcd /*8*/);
+void orc_abcd_to_s(const double* abcd, double Z0, double* S);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

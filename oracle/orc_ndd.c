@@ -1,0 +1,750 @@
+/*
+ * orc_ndd.c -- CPU FP64 restatement of one implicit NDD time step.
+ * TEST INFRASTRUCTURE ONLY (see oracle.h).
+ *
+ * Domain: one rectangular block of cells i = 1..ni (z), j = 1..nj (r) with the
+ * boundary set of the reference's block 4 (include/initialModule.hpp:239-267,
+ * fluidModule.hpp:2178-2240):
+ *   left  x-face 1     wall "z4": phi = 0, thermal (+SEE) wall flux
+ *   right x-face ni+1  wall "z5": phi = V_bias, thermal (+SEE) wall flux
+ *   lower r-face 1     axis "x6": zero face coefficient (q_L = 0 there)
+ *   upper r-face nj+1  open "x5": grad phi = 0, electron flux = ion flux
+ * Interior x-faces are i = 2..ni, interior r-faces j = 2..nj (the flx/flr
+ * block extents of initialModule.hpp:270-330 reduced to one block).
+ *
+ * Mobility is isotropic and uniform (B = 0): mu_para = mu_perp = mu_xx =
+ * mu_rr = mu, mu_xr = 0, evaluated through the reference's tensor formulas so
+ * that the predictor keeps its rounding (fluidModule.hpp:2314-2339).
+ *
+ * Ions use the electron Scharfetter-Gummel routine with charge +1
+ * (vd = mu_i E + EPS, D_i = k Ti mu_i / e, no temperature-gradient drift),
+ * the reference's kinetic wall flux (fluidModule.hpp:1502-1526) evaluated at
+ * the drift velocity mu_i E of the wall face, and an open upper face carrying
+ * n_i (mu_i E_r + EPS) (the zero-gradient limit of the SG flux,
+ * fluidModule.hpp:1563-1573, 1738-1748).
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include "oracle.h"
+
+static int g_literal = 0;
+void orc_set_literal_bernoulli(int on) { g_literal = on; }
+
+double orc_bern_bp(double vd, double z, double D_over_h, int literal)
+{
+    if (literal) return vd / (1.0 - exp(fmin(-z, ORC_LIM_EXP)));
+    if (z == 0.0) return D_over_h;
+    return vd / (-expm1(fmin(-z, ORC_LIM_EXP)));
+}
+
+double orc_bern_bm(double vd, double z, double D_over_h, int literal)
+{
+    if (literal) return vd / (exp(fmin(z, ORC_LIM_EXP)) - 1.0);
+    if (z == 0.0) return D_over_h;
+    return vd / expm1(fmin(z, ORC_LIM_EXP));
+}
+
+/* func_Te, miniFuncs.hpp:35-46 */
+double orc_func_Te(const double* v, int n, double Te, double dTe)
+{
+    int iL = (int)(Te / dTe);
+    int iR = (int)(Te / dTe) + 1;
+    double TeL = iL * dTe;
+    if (iR < n - 1) return (v[iR] - v[iL]) / (dTe + 1e-100) * (Te - TeL) + v[iL];
+    return v[n - 1];
+}
+
+/* calcRes, miniFuncs.hpp:292-375 */
+double orc_calc_res(const double* vn, const double* vo, int ld, int i0, int i1,
+                    int j0, int j1, double eps, int mode)
+{
+    double err = 0.0;
+    if (mode == 0) {
+        for (int i = i0; i <= i1; i++)
+            for (int j = j0; j <= j1; j++) {
+                double e = fabs(vn[i * ld + j] - vo[i * ld + j]);
+                if (e > err) err = e;
+            }
+    } else if (mode == 1) {
+        int n = 0;
+        for (int i = i0; i <= i1; i++)
+            for (int j = j0; j <= j1; j++) {
+                double d = vn[i * ld + j] - vo[i * ld + j];
+                err += (d * d) / (vo[i * ld + j] * vo[i * ld + j] + eps);
+                n++;
+            }
+        err = sqrt(err / (double)n);
+    } else if (mode == 2) {
+        double nrm = 0.0;
+        for (int i = i0; i <= i1; i++)
+            for (int j = j0; j <= j1; j++) {
+                double e = fabs(vn[i * ld + j] - vo[i * ld + j]);
+                if (e > err) err = e;
+                double a = fabs(vo[i * ld + j]);
+                if (a > nrm) nrm = a;
+            }
+        err = err / nrm;
+    } else {
+        double nrm = 0.0;
+        for (int i = i0; i <= i1; i++)
+            for (int j = j0; j <= j1; j++) {
+                double d = vn[i * ld + j] - vo[i * ld + j];
+                err += d * d;
+                nrm += vo[i * ld + j] * vo[i * ld + j];
+            }
+        err = sqrt(err / nrm);
+    }
+    return err;
+}
+
+/* ------------------------------------------------------------------ */
+typedef struct {
+    const cathode_ndd_params* p;
+    int ni, nj, ld;
+    size_t n;
+    double* r; /* cell-centre radii r[0..nj+1] */
+    double *aP, *aE, *aW, *aN, *aS, *b;
+    double *x, *tmp_c, *tmp_x, *tmp_r, *Ex_old, *Er_old;
+    double *zero;
+    int solver;
+    cathode_ndd_report* rep;
+} ctx_t;
+
+#define IX(i, j) ((size_t)(i) * (size_t)C->ld + (size_t)(j))
+
+static double* src_or_zero(ctx_t* C, double* a) { return a ? a : C->zero; }
+
+static void ctx_init(ctx_t* C, const cathode_ndd_params* p, int solver, cathode_ndd_report* rep)
+{
+    C->p = p;
+    C->ni = p->ni;
+    C->nj = p->nj;
+    C->ld = p->nj + 2;
+    C->n = (size_t)(p->ni + 2) * (size_t)(p->nj + 2);
+    C->r = (double*)calloc(p->nj + 2, sizeof(double));
+    for (int j = 0; j < p->nj + 2; j++) C->r[j] = p->rmin + (j - 0.5) * p->dr; /* initialModule.hpp:86-88 */
+    double** bufs[] = {&C->aP, &C->aE, &C->aW, &C->aN, &C->aS, &C->b, &C->x,
+                       &C->tmp_c, &C->tmp_x, &C->tmp_r, &C->Ex_old, &C->Er_old, &C->zero};
+    for (size_t k = 0; k < sizeof(bufs) / sizeof(bufs[0]); k++) *bufs[k] = (double*)calloc(C->n, sizeof(double));
+    C->solver = solver;
+    C->rep = rep;
+}
+
+static void ctx_free(ctx_t* C)
+{
+    free(C->r);
+    double* bufs[] = {C->aP, C->aE, C->aW, C->aN, C->aS, C->b, C->x,
+                      C->tmp_c, C->tmp_x, C->tmp_r, C->Ex_old, C->Er_old, C->zero};
+    for (size_t k = 0; k < sizeof(bufs) / sizeof(bufs[0]); k++) free(bufs[k]);
+}
+
+/* the linear solve handed to solver_SMG in the reference */
+static void solve5(ctx_t* C, double* x)
+{
+    const cathode_ndd_params* p = C->p;
+    int box[4] = {1, C->ni, 1, C->nj};
+    double rel = 0.0;
+    int its = 0;
+    if (C->solver == 0) {
+        orc_struct5_direct(C->aP, C->aE, C->aW, C->aN, C->aS, C->b, box, 1, C->ni + 2, C->nj + 2, C->ld, x);
+        rel = orc_rel_residual(C->aP, C->aE, C->aW, C->aN, C->aS, C->b, box, 1, C->ni + 2, C->nj + 2, C->ld, x);
+        its = 1;
+    } else {
+        its = orc_mg_solve(C->aP, C->aE, C->aW, C->aN, C->aS, C->b, box, 1, C->ni + 2, C->nj + 2, C->ld,
+                           p->maxITR, p->tol, x, &rel);
+    }
+    if (C->rep) {
+        C->rep->solves++;
+        C->rep->total_iterations += its;
+        if (its > C->rep->max_iterations) C->rep->max_iterations = its;
+        if (rel > C->rep->max_rel_residual) C->rep->max_rel_residual = rel;
+        if (C->solver != 0 && !(rel <= p->tol)) C->rep->nonconverged++;
+    }
+}
+
+/* isotropic mobility through the reference's tensor algebra */
+typedef struct {
+    double mu_star, inv_par, inv_cross;
+} mob_t;
+static mob_t mobility(double mu)
+{
+    double mu_para = (mu + mu) / 2.0, mu_perp = (mu + mu) / 2.0;
+    double mu_rr = (mu + mu) / 2.0, mu_xr = (0.0 + 0.0) / 2.0;
+    mob_t m;
+    m.mu_star = mu_perp * 1.0; /* c_mu = 1, fluidModule.hpp:2106 */
+    m.inv_par = mu_rr / (mu_para * mu_perp);
+    m.inv_cross = -mu_xr / (mu_para * mu_perp);
+    return m;
+}
+
+/* electron thermal wall flux per density: +-0.25 vth exp(-e dphi / k Te) */
+static double wall_gpn(const cathode_ndd_params* p, double Te, double dphi, double sign)
+{
+    double vth = sqrt(8.0 * ORC_BOLTZ * Te / (M_PI * p->masse));
+    return sign * 0.25 * vth * exp(fmin(-ORC_E0 * dphi / (ORC_BOLTZ * Te + 1e-100), ORC_LIM_EXP));
+}
+
+static double eisee_sum(const cathode_ndd_params* p, double Te)
+{
+    if (p->n_see <= 1) return 0.0;
+    double eb = orc_func_Te(p->see_eb, p->n_see, Te, p->dTe_SEE);
+    double rd = orc_func_Te(p->see_rd, p->n_see, Te, p->dTe_SEE);
+    double ts = orc_func_Te(p->see_ts, p->n_see, Te, p->dTe_SEE);
+    return eb + rd + ts;
+}
+
+/* ion kinetic wall flux per density, fluidModule.hpp:1502-1526; sign -1 left, +1 right */
+static double ion_wall_gpn(const cathode_ndd_params* p, double U, double sign)
+{
+    double vth = sqrt(8.0 * ORC_BOLTZ * p->Ti / (M_PI * p->massi));
+    double um = sqrt(2.0 * ORC_BOLTZ * p->Ti / p->massi);
+    double u = U / (um + 1e-100);
+    return sign * 0.25 * vth * (exp(-u * u) + u * sqrt(M_PI) * (erf(u) + sign));
+}
+
+/* ================================================================== */
+/* Poisson, fluidModule.hpp:2080-3075                                  */
+/* ================================================================== */
+static void poisson_predict_assemble(ctx_t* C, cathode_ndd_fields* f, const double* nUex_t,
+                                     const double* nUer_t)
+{
+    const cathode_ndd_params* p = C->p;
+    const int ni = C->ni, nj = C->nj;
+    const double dx = p->dx, dr = p->dr, dt = p->dt;
+    const double* r = C->r;
+    mob_t m = mobility(p->mu_e);
+    double* S = src_or_zero(C, f->rate_ionize);
+
+    /* predicted x-flux, interior faces (:2266-2342) */
+    for (int i = 2; i <= ni; i++)
+        for (int j = 1; j <= nj; j++) {
+            double rL = (r[j] + r[j - 1]) / 2.0, rR = (r[j] + r[j + 1]) / 2.0;
+            double n_f = (f->rhoe[IX(i, j)] + f->rhoe[IX(i - 1, j)]) / 2.0;
+            double dphidx = (f->phi[IX(i, j)] - f->phi[IX(i - 1, j)]) / dx;
+            double dpedx = ORC_BOLTZ * (f->rhoe[IX(i, j)] * f->Te[IX(i, j)] - f->rhoe[IX(i - 1, j)] * f->Te[IX(i - 1, j)]) / dx;
+            double bLr = (double)(j != 1), bLrD = (double)(j != 1), bRr = 1.0, bRrD = 1.0;
+            double tr = (rR * nUer_t[IX(i, j + 1)] * bRr + rL * nUer_t[IX(i, j)] * bLr +
+                         rR * nUer_t[IX(i - 1, j + 1)] * bRrD + rL * nUer_t[IX(i - 1, j)] * bLrD) /
+                        (4.0 * r[j]);
+            f->nUex[IX(i, j)] = nUex_t[IX(i, j)] - (m.inv_par * nUex_t[IX(i, j)] + m.inv_cross * tr) * m.mu_star +
+                                m.mu_star * (n_f * dphidx - dpedx / ORC_E0);
+        }
+    /* wall x-faces (:2346-2462) */
+    for (int j = 1; j <= nj; j++) {
+        int i = 1;
+        double Te = f->Te[IX(i, j)];
+        double dphi = fmax(-C->Ex_old[IX(i, j)] * dx / 2.0, 0.0);
+        double gpn = wall_gpn(p, Te, dphi, -1.0);
+        double see = -p->coefIISEE_ts * f->rhoUix[IX(i, j)] - eisee_sum(p, Te) * f->rhoe_old[IX(i, j)] * gpn;
+        f->nUex[IX(i, j)] = f->rhoe[IX(i, j)] * gpn + see;
+        i = ni;
+        Te = f->Te[IX(i, j)];
+        dphi = fmax(C->Ex_old[IX(i + 1, j)] * dx / 2.0, 0.0);
+        gpn = wall_gpn(p, Te, dphi, 1.0);
+        see = -p->coefIISEE_ts * f->rhoUix[IX(i + 1, j)] - eisee_sum(p, Te) * f->rhoe_old[IX(i, j)] * gpn;
+        f->nUex[IX(i + 1, j)] = f->rhoe[IX(i, j)] * gpn + see;
+    }
+    /* predicted r-flux, interior faces (:2466-2545) */
+    for (int i = 1; i <= ni; i++)
+        for (int j = 2; j <= nj; j++) {
+            double r_t = (r[j] + r[j - 1]) / 2.0;
+            double n_f = (f->rhoe[IX(i, j)] + f->rhoe[IX(i, j - 1)]) / 2.0;
+            double dphidr = (f->phi[IX(i, j)] - f->phi[IX(i, j - 1)]) / dr;
+            double dpedr = ORC_BOLTZ * (f->rhoe[IX(i, j)] * f->Te[IX(i, j)] - f->rhoe[IX(i, j - 1)] * f->Te[IX(i, j - 1)]) / dr;
+            double bLx = (double)(i != 1), bLxD = (double)(i != 1), bRx = (double)(i != ni), bRxD = (double)(i != ni);
+            double tx = (r[j] * nUex_t[IX(i + 1, j)] * bRx + r[j] * nUex_t[IX(i, j)] * bLx +
+                         r[j - 1] * nUex_t[IX(i + 1, j - 1)] * bRxD + r[j - 1] * nUex_t[IX(i, j - 1)] * bLxD) /
+                        (4.0 * r_t);
+            f->nUer[IX(i, j)] = nUer_t[IX(i, j)] - (m.inv_cross * tx + m.inv_par * nUer_t[IX(i, j)]) * m.mu_star +
+                                m.mu_star * (n_f * dphidr - dpedr / ORC_E0);
+        }
+    /* open upper face (:2614-2621): electron flux = ion flux */
+    for (int i = 1; i <= ni; i++) {
+        double uir = p->mu_i * f->Er[IX(i, nj + 1)] + ORC_EPS_VD;
+        f->nUer[IX(i, nj + 1)] = f->rhoi[IX(i, nj)] * uir;
+    }
+    /* coefficients (:2627-2719) */
+    for (int i = 1; i <= ni; i++)
+        for (int j = 1; j <= nj; j++) {
+            double qR = (r[j] + r[j + 1]) / 2.0 / r[j];
+            double qL = (r[j] + r[j - 1]) / 2.0 / r[j];
+            double mu = m.mu_star; /* (mu_perp[i][j] + mu_perp[nb]) / 2 * c_mu */
+            double nLx = (f->rhoe[IX(i, j)] + f->rhoe[IX(i - 1, j)]) / 2.0;
+            double nRx = (f->rhoe[IX(i, j)] + f->rhoe[IX(i + 1, j)]) / 2.0;
+            double nLr = (f->rhoe[IX(i, j)] + f->rhoe[IX(i, j - 1)]) / 2.0;
+            double nRr = (f->rhoe[IX(i, j)] + f->rhoe[IX(i, j + 1)]) / 2.0;
+            double bLx = (double)(i != 1), bRx = (double)(i != ni);
+            double bLr = (double)(j != 1), bRr = (double)(j != nj);
+            double eRx = ORC_EPS0 + ORC_E0 * dt * nRx * mu * bRx;
+            double eLx = ORC_EPS0 + ORC_E0 * dt * nLx * mu * bLx;
+            double eRr = ORC_EPS0 + ORC_E0 * dt * nRr * mu * bRr;
+            double eLr = ORC_EPS0 + ORC_E0 * dt * nLr * mu * bLr;
+            C->aP[IX(i, j)] = (eRx + eLx) * dr * dr + (qR * eRr + qL * eLr) * dx * dx;
+            C->aE[IX(i, j)] = eRx * dr * dr;
+            C->aW[IX(i, j)] = eLx * dr * dr;
+            C->aN[IX(i, j)] = qR * eRr * dx * dx;
+            C->aS[IX(i, j)] = qL * eLr * dx * dx;
+            double div = (f->nUex[IX(i + 1, j)] - f->nUex[IX(i, j)]) / dx +
+                         (qR * f->nUer[IX(i, j + 1)] - qL * f->nUer[IX(i, j)]) / dr;
+            double lap = -(f->Ex[IX(i + 1, j)] - f->Ex[IX(i, j)]) / dx - (qR * f->Er[IX(i, j + 1)] - qL * f->Er[IX(i, j)]) / dr;
+            C->b[IX(i, j)] = (ORC_E0 * f->rhoi[IX(i, j)] - ORC_E0 * f->rhoe[IX(i, j)] - ORC_E0 * dt * S[IX(i, j)] +
+                              ORC_EPS0 * lap + ORC_E0 * dt * div) *
+                             dx * dx * dr * dr;
+        }
+    /* BC folds (:2721-2824): Dirichlet walls by ghost mirror, open top Neumann */
+    for (int j = 1; j <= nj; j++) {
+        C->aP[IX(1, j)] = C->aP[IX(1, j)] + C->aW[IX(1, j)];
+        C->aW[IX(1, j)] = 0.0;
+    }
+    for (int j = 1; j <= nj; j++) {
+        C->aP[IX(ni, j)] = C->aP[IX(ni, j)] + C->aE[IX(ni, j)];
+        C->aE[IX(ni, j)] = 0.0;
+    }
+    for (int i = 1; i <= ni; i++) {
+        C->aP[IX(i, nj)] = C->aP[IX(i, nj)] - C->aN[IX(i, nj)];
+        C->aN[IX(i, nj)] = 0.0;
+    }
+}
+
+static void poisson_routine(ctx_t* C, cathode_ndd_fields* f, double* phi_old, double* nUex_old,
+                            double* nUer_old)
+{
+    const cathode_ndd_params* p = C->p;
+    const int ni = C->ni, nj = C->nj;
+    const double dx = p->dx, dr = p->dr;
+    mob_t m = mobility(p->mu_e);
+    double* dphi = C->x;
+    memcpy(phi_old, f->phi, C->n * sizeof(double));
+    memcpy(C->Ex_old, f->Ex, C->n * sizeof(double));
+    memcpy(C->Er_old, f->Er, C->n * sizeof(double));
+    memcpy(nUex_old, f->nUex, C->n * sizeof(double));
+    memcpy(nUer_old, f->nUer, C->n * sizeof(double));
+    for (size_t k = 0; k < C->n; k++) dphi[k] = f->phi[k] - phi_old[k];
+    /* ghost potential (:2160-2253) */
+    for (int j = 1; j <= nj; j++) f->phi[IX(0, j)] = -f->phi[IX(1, j)];
+    for (int j = 1; j <= nj; j++) f->phi[IX(ni + 1, j)] = 2.0 * p->V_bias - f->phi[IX(ni, j)];
+    for (int i = 1; i <= ni; i++) f->phi[IX(i, nj + 1)] = f->phi[IX(i, nj)];
+
+    double* phi_t = C->tmp_c;
+    double* nUex_t = C->tmp_x;
+    double* nUer_t = C->tmp_r;
+    int ncount = 0;
+    for (;;) {
+        ncount++;
+        memcpy(phi_t, f->phi, C->n * sizeof(double));
+        memcpy(nUex_t, f->nUex, C->n * sizeof(double));
+        memcpy(nUer_t, f->nUer, C->n * sizeof(double));
+        poisson_predict_assemble(C, f, nUex_t, nUer_t);
+        solve5(C, dphi);
+        /* ghost correction and wall-face field (:2850-2943) */
+        for (int j = 1; j <= nj; j++) {
+            dphi[IX(0, j)] = -dphi[IX(1, j)];
+            f->Ex[IX(1, j)] = f->Ex[IX(1, j)] - (dphi[IX(1, j)] - dphi[IX(0, j)]) / dx;
+        }
+        for (int j = 1; j <= nj; j++) {
+            dphi[IX(ni + 1, j)] = -dphi[IX(ni, j)];
+            f->Ex[IX(ni + 1, j)] = f->Ex[IX(ni + 1, j)] - (dphi[IX(ni + 1, j)] - dphi[IX(ni, j)]) / dx;
+        }
+        for (int i = 1; i <= ni; i++) {
+            dphi[IX(i, nj + 1)] = dphi[IX(i, nj)];
+            f->Er[IX(i, nj + 1)] = 0.0;
+        }
+        for (size_t k = 0; k < C->n; k++) f->phi[k] = f->phi[k] + dphi[k];
+        for (int i = 2; i <= ni; i++)
+            for (int j = 1; j <= nj; j++) f->Ex[IX(i, j)] = -(f->phi[IX(i, j)] - f->phi[IX(i - 1, j)]) / dx;
+        for (int i = 1; i <= ni; i++)
+            for (int j = 2; j <= nj; j++) f->Er[IX(i, j)] = -(f->phi[IX(i, j)] - f->phi[IX(i, j - 1)]) / dr;
+        /* flux correction (:2980-3033) */
+        for (int i = 2; i <= ni; i++)
+            for (int j = 1; j <= nj; j++) {
+                double n_f = (f->rhoe[IX(i, j)] + f->rhoe[IX(i - 1, j)]) / 2.0;
+                f->nUex[IX(i, j)] = f->nUex[IX(i, j)] + m.mu_star * n_f * ((dphi[IX(i, j)] - dphi[IX(i - 1, j)]) / dx);
+            }
+        for (int i = 1; i <= ni; i++)
+            for (int j = 2; j <= nj; j++) {
+                double n_f = (f->rhoe[IX(i, j)] + f->rhoe[IX(i, j - 1)]) / 2.0;
+                f->nUer[IX(i, j)] = f->nUer[IX(i, j)] + m.mu_star * n_f * ((dphi[IX(i, j)] - dphi[IX(i, j - 1)]) / dr);
+            }
+        double e1 = orc_calc_res(f->phi, phi_t, C->ld, 1, ni, 1, nj, 1e-100, p->flag_error);
+        double e2 = orc_calc_res(f->nUex, nUex_t, C->ld, 2, ni, 1, nj, 1e-100, p->flag_error);
+        double e3 = orc_calc_res(f->nUer, nUer_t, C->ld, 1, ni, 2, nj, 1e-100, p->flag_error);
+        double eg = fmax(fmax(e1, e2), e3);
+        if (eg < p->error_cnv_HES_phi || ncount >= p->maxITR_HES_phi) break;
+    }
+    if (C->rep) {
+        C->rep->error_phi = orc_calc_res(f->phi, phi_old, C->ld, 1, ni, 1, nj, 1e-100, p->flag_error);
+        C->rep->error_nUex = orc_calc_res(f->nUex, nUex_old, C->ld, 2, ni, 1, nj, 1e-100, p->flag_error);
+        C->rep->error_nUer = orc_calc_res(f->nUer, nUer_old, C->ld, 1, ni, 2, nj, 1e-100, p->flag_error);
+    }
+}
+
+/* ================================================================== */
+/* Scharfetter-Gummel continuity: electrons (fluidModule.hpp:3082-4000) */
+/* and ions (charge +1), energy (:4006-4957)                          */
+/* ================================================================== */
+enum { SP_ELEC = 0, SP_ION = 1, SP_ENERGY = 2 };
+
+typedef struct {
+    double vd, D, z;
+} face_t;
+
+/* face drift/diffusion of species sp between cells (a) upstream and (c) at
+ * spacing h with face field E; Tup/Tdn: temperatures [K] of the two cells */
+static face_t sg_face(const cathode_ndd_params* p, int sp, double mu_star, double E, double Ta,
+                      double Tc, double h)
+{
+    face_t F;
+    if (sp == SP_ION) {
+        F.vd = mu_star * (E) + ORC_EPS_VD;
+        F.D = ORC_BOLTZ * p->Ti / ORC_E0 * mu_star;
+    } else {
+        double T_f = (Tc + Ta) / 2.0;
+        double dT = (Tc - Ta) / h;
+        F.vd = mu_star * (-E - ORC_BOLTZ / ORC_E0 * dT) + ORC_EPS_VD;
+        F.D = ORC_BOLTZ * T_f / ORC_E0 * mu_star;
+    }
+    F.z = F.vd * h / F.D;
+    return F;
+}
+
+typedef struct {
+    double* n;      /* transported density (rhoe, rhoi, rhoeps) */
+    double* Fx;     /* x flux */
+    double* Fr;     /* r flux */
+    const double* T; /* temperature used by the SG faces (Te or Te_old) */
+    double mu;
+    double cfac;    /* flux factor: 1 or 5/3 */
+    double tfac;    /* time-term factor: 1 or 3/5 */
+    int maxit;
+    double cnv;
+} sg_eq_t;
+
+static void sg_predict_assemble(ctx_t* C, cathode_ndd_fields* f, int sp, const sg_eq_t* q,
+                                const double* Fx_t, const double* Fr_t, const double* n_old)
+{
+    const cathode_ndd_params* p = C->p;
+    const int ni = C->ni, nj = C->nj;
+    const double dx = p->dx, dr = p->dr, dt = p->dt;
+    const double* r = C->r;
+    const int lit = g_literal;
+    mob_t m = mobility(q->mu);
+    const double* T = q->T;
+    double* n = q->n;
+
+    /* interior x faces */
+    for (int i = 2; i <= ni; i++)
+        for (int j = 1; j <= nj; j++) {
+            double rL = (r[j] + r[j - 1]) / 2.0, rR = (r[j] + r[j + 1]) / 2.0;
+            face_t F = sg_face(p, sp, m.mu_star, f->Ex[IX(i, j)], T[IX(i - 1, j)], T[IX(i, j)], dx);
+            double bLr = (double)(j != 1), bLrD = (double)(j != 1), bRr = 1.0, bRrD = 1.0;
+            double tr = (rR * Fr_t[IX(i, j + 1)] * bRr + rL * Fr_t[IX(i, j)] * bLr +
+                         rR * Fr_t[IX(i - 1, j + 1)] * bRrD + rL * Fr_t[IX(i - 1, j)] * bLrD) /
+                        (4.0 * r[j]);
+            double sg = orc_bern_bp(F.vd, F.z, F.D / dx, lit) * n[IX(i - 1, j)] - orc_bern_bm(F.vd, F.z, F.D / dx, lit) * n[IX(i, j)];
+            if (sp == SP_ENERGY) sg = sg * 5.0 / 3.0;
+            q->Fx[IX(i, j)] = sg + Fx_t[IX(i, j)] - m.mu_star * (m.inv_par * Fx_t[IX(i, j)] + m.inv_cross * tr);
+        }
+    /* interior r faces */
+    for (int i = 1; i <= ni; i++)
+        for (int j = 2; j <= nj; j++) {
+            double r_t = (r[j] + r[j - 1]) / 2.0;
+            face_t F = sg_face(p, sp, m.mu_star, f->Er[IX(i, j)], T[IX(i, j - 1)], T[IX(i, j)], dr);
+            double bLx = (double)(i != 1), bLxD = (double)(i != 1), bRx = (double)(i != ni), bRxD = (double)(i != ni);
+            double tx = (r[j] * Fx_t[IX(i + 1, j)] * bRx + r[j] * Fx_t[IX(i, j)] * bLx +
+                         r[j - 1] * Fx_t[IX(i + 1, j - 1)] * bRxD + r[j - 1] * Fx_t[IX(i, j - 1)] * bLxD) /
+                        (4.0 * r_t);
+            double sg = orc_bern_bp(F.vd, F.z, F.D / dr, lit) * n[IX(i, j - 1)] - orc_bern_bm(F.vd, F.z, F.D / dr, lit) * n[IX(i, j)];
+            if (sp == SP_ENERGY) sg = sg * 5.0 / 3.0;
+            q->Fr[IX(i, j)] = sg + Fr_t[IX(i, j)] - m.mu_star * (m.inv_cross * tx + m.inv_par * Fr_t[IX(i, j)]);
+        }
+    /* wall x faces and open top */
+    for (int j = 1; j <= nj; j++) {
+        for (int side = 0; side < 2; side++) {
+            int i = side ? ni : 1;     /* wall cell */
+            int fi = side ? ni + 1 : 1; /* wall face */
+            double sgn = side ? 1.0 : -1.0;
+            if (sp == SP_ION) {
+                double U = p->mu_i * f->Ex[IX(fi, j)];
+                q->Fx[IX(fi, j)] = n[IX(i, j)] * ion_wall_gpn(p, U, sgn);
+            } else if (sp == SP_ELEC) {
+                double Te = f->Te[IX(i, j)];
+                double dphi = fmax(-sgn * f->Ex[IX(fi, j)] * dx / 2.0, 0.0);
+                double gpn = wall_gpn(p, Te, dphi, sgn);
+                double see = -p->coefIISEE_ts * f->rhoUix[IX(fi, j)] - eisee_sum(p, Te) * f->rhoe_old[IX(i, j)] * gpn;
+                q->Fx[IX(fi, j)] = n[IX(i, j)] * gpn + see;
+            } else {
+                double Teo = T[IX(i, j)];
+                double dphi = fmax(-sgn * f->Ex[IX(fi, j)] * dx / 2.0, 0.0);
+                double vth = sqrt(8.0 * ORC_BOLTZ * Teo / (M_PI * p->masse));
+                double ex = exp(fmin(-ORC_E0 * dphi / (ORC_BOLTZ * Teo + 1e-100), ORC_LIM_EXP));
+                double gpne = sgn * vth * ex / 3.0 * 1.0;
+                double eps_emit = 2.0 * ORC_E0 * p->Te_emitSEE_eV;
+                double gpn = sgn * 0.25 * 1.0 * vth * ex;
+                double Te = f->Te[IX(i, j)];
+                double eb = 0.0, rd = 0.0, ts = 0.0;
+                if (p->n_see > 1) {
+                    eb = orc_func_Te(p->see_eb, p->n_see, Te, p->dTe_SEE);
+                    rd = orc_func_Te(p->see_rd, p->n_see, Te, p->dTe_SEE);
+                    ts = orc_func_Te(p->see_ts, p->n_see, Te, p->dTe_SEE);
+                }
+                double see = -eps_emit * p->coefIISEE_ts * f->rhoUix[IX(fi, j)] -
+                             (eb + p->ratioEngy_EISEE_rd * rd) * n_old[IX(i, j)] * gpne -
+                             eps_emit * ts * f->rhoe_old[IX(i, j)] * gpn;
+                q->Fx[IX(fi, j)] = n[IX(i, j)] * gpne + see;
+            }
+        }
+    }
+    for (int i = 1; i <= ni; i++) {
+        if (sp == SP_ENERGY) {
+            q->Fr[IX(i, nj + 1)] = 5.0 / 2.0 * ORC_BOLTZ * T[IX(i, nj)] * fmax(f->rhoUer[IX(i, nj + 1)], 0.0);
+        } else {
+            double uir = p->mu_i * f->Er[IX(i, nj + 1)] + ORC_EPS_VD;
+            q->Fr[IX(i, nj + 1)] = f->rhoi[IX(i, nj)] * uir;
+        }
+    }
+    /* coefficients (:3544-3664, :4508-4642) */
+    double* S = src_or_zero(C, f->rate_ionize);
+    double* Pabs = src_or_zero(C, f->Pabs);
+    double* eloss = src_or_zero(C, f->rate_eloss);
+    for (int i = 1; i <= ni; i++)
+        for (int j = 1; j <= nj; j++) {
+            double rR = (r[j] + r[j + 1]) / 2.0, rL = (r[j] + r[j - 1]) / 2.0;
+            double qR = (r[j] + r[j + 1]) / 2.0 / r[j];
+            double qL = (r[j] + r[j - 1]) / 2.0 / r[j];
+            face_t Lx = sg_face(p, sp, m.mu_star, f->Ex[IX(i, j)], T[IX(i - 1, j)], T[IX(i, j)], dx);
+            face_t Rx = sg_face(p, sp, m.mu_star, f->Ex[IX(i + 1, j)], T[IX(i, j)], T[IX(i + 1, j)], dx);
+            face_t Lr = sg_face(p, sp, m.mu_star, f->Er[IX(i, j)], T[IX(i, j - 1)], T[IX(i, j)], dr);
+            face_t Rr = sg_face(p, sp, m.mu_star, f->Er[IX(i, j + 1)], T[IX(i, j)], T[IX(i, j + 1)], dr);
+            double bLx = (double)(i != 1), bRx = (double)(i != ni);
+            double bLr = (double)(j != 1), bRr = (double)(j != nj);
+            double aP = orc_bern_bp(Rx.vd, Rx.z, Rx.D / dx, lit) * bRx * dr + orc_bern_bm(Lx.vd, Lx.z, Lx.D / dx, lit) * bLx * dr +
+                        qR * orc_bern_bp(Rr.vd, Rr.z, Rr.D / dr, lit) * bRr * dx +
+                        qL * orc_bern_bm(Lr.vd, Lr.z, Lr.D / dr, lit) * bLr * dx;
+            C->aP[IX(i, j)] = aP + q->tfac * dx * dr / dt;
+            C->aE[IX(i, j)] = orc_bern_bm(Rx.vd, Rx.z, Rx.D / dx, lit) * bRx * dr;
+            C->aW[IX(i, j)] = orc_bern_bp(Lx.vd, Lx.z, Lx.D / dx, lit) * bLx * dr;
+            C->aN[IX(i, j)] = qR * orc_bern_bm(Rr.vd, Rr.z, Rr.D / dr, lit) * bRr * dx;
+            C->aS[IX(i, j)] = qL * orc_bern_bp(Lr.vd, Lr.z, Lr.D / dr, lit) * bLr * dx;
+            double div = (q->Fx[IX(i + 1, j)] - q->Fx[IX(i, j)]) / dx + (qR * q->Fr[IX(i, j + 1)] - qL * q->Fr[IX(i, j)]) / dr;
+            if (sp == SP_ENERGY) {
+                double bLxw = (double)(i != 1), bRxw = (double)(i != ni); /* wall flags only */
+                double JEx = -ORC_E0 * (f->rhoUex[IX(i + 1, j)] * f->Ex[IX(i + 1, j)] * bRxw + f->rhoUex[IX(i, j)] * f->Ex[IX(i, j)] * bLxw) / 2.0;
+                double JEr = -ORC_E0 * (rR * f->rhoUer[IX(i, j + 1)] * f->Er[IX(i, j + 1)] * 1.0 + rL * f->rhoUer[IX(i, j)] * f->Er[IX(i, j)] * 1.0) /
+                             (2.0 * r[j]);
+                double R1 = (-n[IX(i, j)] + n_old[IX(i, j)]) / dt;
+                double R2 = JEx + JEr;
+                double R3 = Pabs[IX(i, j)] - f->rhoe[IX(i, j)] * eloss[IX(i, j)];
+                double R4 = -div;
+                C->b[IX(i, j)] = 3.0 / 5.0 * (R1 + R2 + R3 + R4) * dx * dr;
+            } else {
+                C->b[IX(i, j)] = (-div + S[IX(i, j)] - (n[IX(i, j)] - n_old[IX(i, j)]) / dt) * dx * dr;
+            }
+        }
+}
+
+static void sg_routine(ctx_t* C, cathode_ndd_fields* f, int sp, const sg_eq_t* q, double* n_old,
+                       double* Fx_old, double* Fr_old, double* err3)
+{
+    const cathode_ndd_params* p = C->p;
+    const int ni = C->ni, nj = C->nj;
+    const double dx = p->dx, dr = p->dr;
+    const int lit = g_literal;
+    mob_t m = mobility(q->mu);
+    double* dn = C->x;
+    memcpy(n_old, q->n, C->n * sizeof(double));
+    memcpy(Fx_old, q->Fx, C->n * sizeof(double));
+    memcpy(Fr_old, q->Fr, C->n * sizeof(double));
+    for (size_t k = 0; k < C->n; k++) dn[k] = q->n[k] - n_old[k];
+    double* n_t = C->tmp_c;
+    double* Fx_t = C->tmp_x;
+    double* Fr_t = C->tmp_r;
+    int ncount = 0;
+    for (;;) {
+        ncount++;
+        memcpy(n_t, q->n, C->n * sizeof(double));
+        memcpy(Fx_t, q->Fx, C->n * sizeof(double));
+        memcpy(Fr_t, q->Fr, C->n * sizeof(double));
+        sg_predict_assemble(C, f, sp, q, Fx_t, Fr_t, n_old);
+        solve5(C, dn);
+        /* flux correction (:3682-3743) */
+        for (int i = 2; i <= ni; i++)
+            for (int j = 1; j <= nj; j++) {
+                face_t F = sg_face(p, sp, m.mu_star, f->Ex[IX(i, j)], q->T[IX(i - 1, j)], q->T[IX(i, j)], dx);
+                double d = orc_bern_bp(F.vd, F.z, F.D / dx, lit) * dn[IX(i - 1, j)] - orc_bern_bm(F.vd, F.z, F.D / dx, lit) * dn[IX(i, j)];
+                if (sp == SP_ENERGY) d = d * 5.0 / 3.0;
+                q->Fx[IX(i, j)] = q->Fx[IX(i, j)] + d;
+            }
+        for (int i = 1; i <= ni; i++)
+            for (int j = 2; j <= nj; j++) {
+                face_t F = sg_face(p, sp, m.mu_star, f->Er[IX(i, j)], q->T[IX(i, j - 1)], q->T[IX(i, j)], dr);
+                double d = orc_bern_bp(F.vd, F.z, F.D / dr, lit) * dn[IX(i, j - 1)] - orc_bern_bm(F.vd, F.z, F.D / dr, lit) * dn[IX(i, j)];
+                if (sp == SP_ENERGY) d = d * 5.0 / 3.0;
+                q->Fr[IX(i, j)] = q->Fr[IX(i, j)] + d;
+            }
+        /* positivity-guarded update (:3748-3757) */
+        for (int i = 1; i <= ni; i++)
+            for (int j = 1; j <= nj; j++) {
+                double nn = q->n[IX(i, j)] + dn[IX(i, j)];
+                if (nn > 0.0) q->n[IX(i, j)] = nn;
+            }
+        double e1 = orc_calc_res(q->n, n_t, C->ld, 1, ni, 1, nj, 1e-100, p->flag_error);
+        double e2 = orc_calc_res(q->Fx, Fx_t, C->ld, 2, ni, 1, nj, 1e-100, p->flag_error);
+        double e3 = orc_calc_res(q->Fr, Fr_t, C->ld, 1, ni, 2, nj, 1e-100, p->flag_error);
+        double eg = fmax(fmax(e1, e2), e3);
+        if (eg < q->cnv || ncount >= q->maxit) break;
+    }
+    err3[0] = orc_calc_res(q->n, n_old, C->ld, 1, ni, 1, nj, 1e-100, p->flag_error);
+    err3[1] = orc_calc_res(q->Fx, Fx_old, C->ld, 2, ni, 1, nj, 1e-100, p->flag_error);
+    err3[2] = orc_calc_res(q->Fr, Fr_old, C->ld, 1, ni, 2, nj, 1e-100, p->flag_error);
+}
+
+static void set_eq(const cathode_ndd_params* p, cathode_ndd_fields* f, int sp, const double* Te_old, sg_eq_t* q)
+{
+    if (sp == SP_ELEC) {
+        q->n = f->rhoe; q->Fx = f->rhoUex; q->Fr = f->rhoUer; q->T = f->Te; q->mu = p->mu_e;
+        q->cfac = 1.0; q->tfac = 1.0; q->maxit = p->maxITR_HES_rhoe; q->cnv = p->error_cnv_HES_rhoe;
+    } else if (sp == SP_ION) {
+        q->n = f->rhoi; q->Fx = f->rhoUix; q->Fr = f->rhoUir; q->T = f->Te; q->mu = p->mu_i;
+        q->cfac = 1.0; q->tfac = 1.0; q->maxit = p->maxITR_HES_rhoi; q->cnv = p->error_cnv_HES_rhoi;
+    } else {
+        q->n = f->rhoeps; q->Fx = f->Gx; q->Fr = f->Gr; q->T = Te_old; q->mu = p->mu_e;
+        q->cfac = 5.0 / 3.0; q->tfac = 3.0 / 5.0; q->maxit = p->maxITR_HES_rhoeps; q->cnv = p->error_cnv_HES_rhoeps;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+int orc_ndd_step(const cathode_ndd_params* p, cathode_ndd_fields* f, int nsteps, int solver,
+                 cathode_ndd_report* rep)
+{
+    ctx_t Cx, *C = &Cx;
+    if (rep) memset(rep, 0, sizeof(*rep));
+    struct timespec t0, t1;
+    clock_gettime(CLOCK_MONOTONIC, &t0);
+    ctx_init(C, p, solver, rep);
+    const int ni = C->ni, nj = C->nj;
+    double* s_old = (double*)calloc(C->n, sizeof(double));
+    double* sx_old = (double*)calloc(C->n, sizeof(double));
+    double* sr_old = (double*)calloc(C->n, sizeof(double));
+    double* Te_old = (double*)calloc(C->n, sizeof(double));
+    double* rhoi_old = (double*)calloc(C->n, sizeof(double));
+    double err3[3];
+    sg_eq_t q;
+    for (int step = 0; step < nsteps; step++) {
+        /* 1. ions */
+        set_eq(p, f, SP_ION, NULL, &q);
+        sg_routine(C, f, SP_ION, &q, rhoi_old, sx_old, sr_old, err3);
+        for (int i = 1; i <= ni; i++) f->rhoi[IX(i, nj + 1)] = f->rhoi[IX(i, nj)];
+        if (rep) rep->error_rhoi = err3[0];
+        /* 2. Poisson */
+        poisson_routine(C, f, s_old, sx_old, sr_old);
+        /* 3. electrons */
+        set_eq(p, f, SP_ELEC, NULL, &q);
+        sg_routine(C, f, SP_ELEC, &q, f->rhoe_old, sx_old, sr_old, err3);
+        if (rep) {
+            rep->error_rhoe = err3[0];
+            rep->error_rhoUex = err3[1];
+            rep->error_rhoUer = err3[2];
+            double Ii = 0.0, Ie = 0.0;
+            for (int j = 1; j <= nj; j++) {
+                double di = f->rhoUix[IX(ni + 1, j)] * 2.0 * M_PI * C->r[j] * p->dr * (ORC_E0);
+                double de = f->rhoUex[IX(ni + 1, j)] * 2.0 * M_PI * C->r[j] * p->dr * (-ORC_E0);
+                Ii += di;
+                Ie += de;
+            }
+            rep->Ii_anode = Ii;
+            rep->Ie_anode = Ie;
+            rep->I_anode = Ii + Ie;
+        }
+        /* 4. electron energy */
+        if (p->energy_on) {
+            memcpy(Te_old, f->Te, C->n * sizeof(double));
+            set_eq(p, f, SP_ENERGY, Te_old, &q);
+            sg_routine(C, f, SP_ENERGY, &q, s_old, sx_old, sr_old, err3);
+            if (rep) {
+                rep->error_rhoeps = err3[0];
+                rep->error_Gx = err3[1];
+                rep->error_Gr = err3[2];
+            }
+            for (int i = 1; i <= ni; i++)
+                for (int j = 1; j <= nj; j++)
+                    if (f->rhoeps[IX(i, j)] < 0.0) f->rhoeps[IX(i, j)] = fmax(s_old[IX(i, j)], 0.0);
+            double nmax = 0.0;
+            for (int i = 1; i <= ni; i++)
+                for (int j = 1; j <= nj; j++)
+                    if (nmax < f->rhoe[IX(i, j)]) nmax = f->rhoe[IX(i, j)];
+            const double Te_max = 1000.0 * ORC_E0 / ORC_BOLTZ, ratio = 0.001;
+            for (int i = 1; i <= ni; i++)
+                for (int j = 1; j <= nj; j++)
+                    f->Te[IX(i, j)] = fmin(f->rhoeps[IX(i, j)] / (3.0 / 2.0 * (f->rhoe[IX(i, j)] + nmax * ratio) * ORC_BOLTZ + 1e-100), Te_max);
+            for (int j = 1; j <= nj; j++) {
+                f->Te[IX(0, j)] = f->Te[IX(1, j)];
+                f->Te[IX(ni + 1, j)] = f->Te[IX(ni, j)];
+            }
+            for (int i = 1; i <= ni; i++) {
+                f->Te[IX(i, 0)] = f->Te[IX(i, 1)];
+                f->Te[IX(i, nj + 1)] = f->Te[IX(i, nj)];
+            }
+        }
+        if (rep) rep->steps++;
+    }
+    free(s_old);
+    free(sx_old);
+    free(sr_old);
+    free(Te_old);
+    free(rhoi_old);
+    ctx_free(C);
+    clock_gettime(CLOCK_MONOTONIC, &t1);
+    if (rep) rep->seconds = (t1.tv_sec - t0.tv_sec) + 1e-9 * (t1.tv_nsec - t0.tv_nsec);
+    return 0;
+}
+
+int orc_ndd_assemble(const cathode_ndd_params* p, const cathode_ndd_fields* fin, int which,
+                     double* aP, double* aE, double* aW, double* aN, double* aS, double* b,
+                     double* fx, double* fr)
+{
+    ctx_t Cx, *C = &Cx;
+    ctx_init(C, p, 0, NULL);
+    /* work on a shallow copy whose flux arrays are the caller's outputs */
+    cathode_ndd_fields f = *fin;
+    size_t nb = C->n * sizeof(double);
+    double* n_old = (double*)calloc(C->n, sizeof(double));
+    int rc = 0;
+    if (which == 0) {
+        memcpy(fx, fin->nUex, nb);
+        memcpy(fr, fin->nUer, nb);
+        f.nUex = fx;
+        f.nUer = fr;
+        memcpy(C->Ex_old, fin->Ex, nb);
+        memcpy(C->tmp_x, fin->nUex, nb);
+        memcpy(C->tmp_r, fin->nUer, nb);
+        poisson_predict_assemble(C, &f, C->tmp_x, C->tmp_r);
+    } else if (which >= 1 && which <= 3) {
+        int sp = which == 1 ? SP_ELEC : which == 2 ? SP_ION : SP_ENERGY;
+        sg_eq_t q;
+        set_eq(p, &f, sp, fin->Te, &q);
+        memcpy(fx, q.Fx, nb);
+        memcpy(fr, q.Fr, nb);
+        memcpy(C->tmp_x, q.Fx, nb);
+        memcpy(C->tmp_r, q.Fr, nb);
+        memcpy(n_old, q.n, nb);
+        q.Fx = fx;
+        q.Fr = fr;
+        sg_predict_assemble(C, &f, sp, &q, C->tmp_x, C->tmp_r, n_old);
+    } else {
+        rc = -1;
+    }
+    if (!rc) {
+        memcpy(aP, C->aP, nb);
+        memcpy(aE, C->aE, nb);
+        memcpy(aW, C->aW, nb);
+        memcpy(aN, C->aN, nb);
+        memcpy(aS, C->aS, nb);
+        memcpy(b, C->b, nb);
+    }
+    free(n_old);
+    ctx_free(C);
+    return rc;
+}
