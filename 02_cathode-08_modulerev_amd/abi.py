@@ -123,10 +123,17 @@ SIGNATURES = [
 
 def bind(lib):
     for name, res, args in SIGNATURES:
+        if not hasattr(lib, name):
+            continue  # reported by missing_symbols(); calling it raises AttributeError
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
     return lib
+
+
+def missing_symbols(lib):
+    """Names of SIGNATURES the library does not export."""
+    return [n for n, _, _ in SIGNATURES if not hasattr(lib, n)]
 
 
 _LIB = None

@@ -1,0 +1,228 @@
+// capi.hip -- extern "C" entry points of libcathode_hip (include/cathode_hip.h):
+// lifecycle, the drop-in structured solves (solver_SMG / solver_SOR /
+// solver_LU_9p of include/solverFuncs.hpp) and the error/status side channel.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <vector>
+
+#include "mg.hpp"
+#include "sor.hpp"
+
+namespace cathode {
+
+static std::string g_err;
+static cathode_status g_last{};
+static std::mutex g_mu;
+
+void set_error(const std::string& m) { g_err = m; }
+const char* get_error() { return g_err.c_str(); }
+
+Context& ctx()
+{
+    static Context c;
+    return c;
+}
+
+bool require_init()
+{
+    if (!ctx().ready) {
+        set_error("cathode_init() has not been called");
+        return false;
+    }
+    return true;
+}
+
+void record_status(const cathode_status& s) { g_last = s; }
+
+// generic solver instance (one per library; calls are serialised)
+static std::unique_ptr<MGSolver> g_solver;
+static DevBuf<double> g_stage;                 // device staging for host inputs
+static std::vector<double> g_hx;              // host staging for the solution
+static DevBuf<unsigned char> g_actbuf;
+
+// activity mask (interior-only boxes) built on the host
+static bool build_mask(const int* boxes, int n_bl, int ni2, int nj2, int ld_dst,
+                       std::vector<unsigned char>& m)
+{
+    m.assign((size_t)ni2 * ld_dst, 0);
+    if (n_bl <= 0 || !boxes) return false;
+    for (int b = 0; b < n_bl; b++) {
+        const int* bx = boxes + 4 * b;
+        if (bx[0] < 1 || bx[1] > ni2 - 2 || bx[2] < 1 || bx[3] > nj2 - 2) return false;
+        for (int i = bx[0]; i <= bx[1]; i++)
+            for (int j = bx[2]; j <= bx[3]; j++) m[(size_t)i * ld_dst + j] = 1;
+    }
+    return true;
+}
+
+__global__ void k_scatter_solution(int ni2, int nj2, int ld_src, const double* __restrict__ xs,
+                                   const unsigned char* __restrict__ act, int ld_dst,
+                                   double* __restrict__ x)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y;
+    if (i >= ni2 || j >= nj2) return;
+    if (act[(size_t)i * ld_src + j]) x[(size_t)i * ld_dst + j] = xs[(size_t)i * ld_src + j];
+}
+
+// common body of the host and device struct5 entry points
+static int struct5_impl(const double* aP, const double* aE, const double* aW, const double* aN,
+                        const double* aS, const double* b, const int* boxes, int n_bl, int ni2,
+                        int nj2, int ld, int maxITR, double tol, int mode, double* x, bool device,
+                        hipStream_t s, cathode_status* st)
+{
+    auto t0 = std::chrono::steady_clock::now();
+    if (!aP || !aE || !aW || !aN || !aS || !b || !x || ni2 < 3 || nj2 < 3 || ld < nj2 ||
+        (mode != CATHODE_MODE_FP64 && mode != CATHODE_MODE_MIXED)) {
+        set_error("struct5: invalid arguments");
+        return CATHODE_ERR_ARG;
+    }
+    if (!g_solver) g_solver.reset(new MGSolver());
+    MGSolver& S = *g_solver;
+    S.configure(ni2 - 2, nj2 - 2, s);
+    const int ld0 = S.ld0();
+    std::vector<unsigned char> mask;
+    if (!build_mask(boxes, n_bl, ni2, nj2, ld0, mask)) {
+        set_error("struct5: boxes must lie inside the interior cells 1..ni, 1..nj");
+        return CATHODE_ERR_ARG;
+    }
+    CATHODE_HIP(hipMemcpyAsync(S.act(), mask.data(), mask.size(), hipMemcpyHostToDevice, s));
+    const hipMemcpyKind kind = device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    const double* src[6] = {aP, aE, aW, aN, aS, b};
+    double* dst[6] = {S.aP(), S.aE(), S.aW(), S.aN(), S.aS(), S.rhs()};
+    for (int k = 0; k < 6; k++)
+        CATHODE_HIP(hipMemcpy2DAsync(dst[k], ld0 * sizeof(double), src[k], ld * sizeof(double),
+                                     nj2 * sizeof(double), ni2, kind, s));
+    S.set_weights(nullptr);
+    cathode_status local{};
+    S.solve(maxITR, tol, mode, s, &local);
+    if (device) {
+        hipLaunchKernelGGL(k_scatter_solution, dim3(cdiv(nj2, 64), ni2), dim3(64), 0, s, ni2, nj2, ld0,
+                           S.sol(), S.act(), ld, x);
+        CATHODE_LAUNCH_CHECK();
+        CATHODE_HIP(hipStreamSynchronize(s));
+    } else {
+        g_hx.resize((size_t)ni2 * ld0);
+        CATHODE_HIP(hipMemcpyAsync(g_hx.data(), S.sol(), g_hx.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+        CATHODE_HIP(hipStreamSynchronize(s));
+        for (int i = 0; i < ni2; i++)
+            for (int j = 0; j < nj2; j++)
+                if (mask[(size_t)i * ld0 + j]) x[(size_t)i * ld + j] = g_hx[(size_t)i * ld0 + j];
+    }
+    local.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    record_status(local);
+    if (st) *st = local;
+    if (!std::isfinite(local.rel_residual)) {
+        set_error("struct5: non-finite residual (NaN/Inf in the system)");
+        return CATHODE_ERR_NUMERIC;
+    }
+    return CATHODE_OK;
+}
+
+}  // namespace cathode
+
+using namespace cathode;
+
+#define CATHODE_GUARD_BEGIN \
+    std::lock_guard<std::mutex> lock_(g_mu); \
+    try {
+#define CATHODE_GUARD_END                 \
+    }                                     \
+    catch (const hip_failure&) {          \
+        return CATHODE_ERR_HIP;           \
+    }                                     \
+    catch (const std::bad_alloc&) {       \
+        set_error("host allocation failed"); \
+        return CATHODE_ERR_NOMEM;         \
+    }
+
+extern "C" {
+
+int cathode_abi_version(void) { return CATHODE_ABI_VERSION; }
+
+const char* cathode_last_error(void) { return get_error(); }
+
+void cathode_last_status(cathode_status* st)
+{
+    if (st) *st = g_last;
+}
+
+int cathode_init(int device)
+{
+    CATHODE_GUARD_BEGIN
+    Context& c = ctx();
+    if (c.ready && c.device == device) return CATHODE_OK;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+        set_error("no HIP device visible");
+        return CATHODE_ERR_HIP;
+    }
+    if (device < 0 || device >= n) {
+        set_error("device index out of range");
+        return CATHODE_ERR_ARG;
+    }
+    CATHODE_HIP(hipSetDevice(device));
+    if (!c.stream) CATHODE_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+    c.device = device;
+    c.ready = true;
+    return CATHODE_OK;
+    CATHODE_GUARD_END
+}
+
+int cathode_finalize(void)
+{
+    CATHODE_GUARD_BEGIN
+    Context& c = ctx();
+    if (!c.ready) return CATHODE_OK;
+    g_solver.reset();
+    g_stage.release();
+    g_actbuf.release();
+    sor_release();
+    if (c.stream) (void)hipStreamDestroy(c.stream);
+    c.stream = nullptr;
+    c.ready = false;
+    return CATHODE_OK;
+    CATHODE_GUARD_END
+}
+
+int cathode_struct5_solve(const double* aP, const double* aE, const double* aW, const double* aN,
+                          const double* aS, const double* b, const int* boxes, int n_bl, int ni2,
+                          int nj2, int ld, int maxITR, double tol, int mode, double* x,
+                          cathode_status* st)
+{
+    CATHODE_GUARD_BEGIN
+    if (!require_init()) return CATHODE_ERR_NOINIT;
+    return struct5_impl(aP, aE, aW, aN, aS, b, boxes, n_bl, ni2, nj2, ld, maxITR, tol, mode, x, false,
+                        ctx().stream, st);
+    CATHODE_GUARD_END
+}
+
+int cathode_struct5_solve_dev(const double* aP, const double* aE, const double* aW, const double* aN,
+                              const double* aS, const double* b, const int* boxes, int n_bl, int ni2,
+                              int nj2, int ld, int maxITR, double tol, int mode, double* x,
+                              void* stream, cathode_status* st)
+{
+    CATHODE_GUARD_BEGIN
+    if (!require_init()) return CATHODE_ERR_NOINIT;
+    hipStream_t s = stream ? (hipStream_t)stream : ctx().stream;
+    return struct5_impl(aP, aE, aW, aN, aS, b, boxes, n_bl, ni2, nj2, ld, maxITR, tol, mode, x, true, s, st);
+    CATHODE_GUARD_END
+}
+
+int cathode_sor_solve(const double* aP, const double* aE, const double* aW, const double* aN,
+                      const double* aS, const double* b, const int* boxes, int n_bl, int ni2, int nj2,
+                      int ld, double alpha, int maxITR, double tol, double* x, cathode_status* st)
+{
+    CATHODE_GUARD_BEGIN
+    if (!require_init()) return CATHODE_ERR_NOINIT;
+    return sor_solve_host(aP, aE, aW, aN, aS, b, boxes, n_bl, ni2, nj2, ld, alpha, maxITR, tol, x,
+                          ctx().stream, st);
+    CATHODE_GUARD_END
+}
+
+}  // extern "C"

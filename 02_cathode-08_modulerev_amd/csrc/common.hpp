@@ -1,0 +1,135 @@
+// common.hpp -- shared runtime pieces of libcathode_hip: error state, the
+// library stream, launch helpers and deterministic reductions (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/cathode_hip.h"
+
+namespace cathode {
+
+// ------------------------------------------------------------------ errors
+void set_error(const std::string& msg);
+const char* get_error();
+
+struct hip_failure {
+    hipError_t e;
+};
+
+#define CATHODE_HIP(call)                                                                   \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess) {                                                             \
+            ::cathode::set_error(std::string(#call) + " -> " + hipGetErrorString(e_) + " (" + \
+                                 __FILE__ + ":" + std::to_string(__LINE__) + ")");          \
+            throw ::cathode::hip_failure{e_};                                               \
+        }                                                                                   \
+    } while (0)
+
+#define CATHODE_LAUNCH_CHECK() CATHODE_HIP(hipGetLastError())
+
+// ------------------------------------------------------------------ context
+struct Context {
+    int device = -1;
+    hipStream_t stream = nullptr;
+    bool ready = false;
+};
+Context& ctx();
+bool require_init();
+void record_status(const cathode_status& s);
+
+// ------------------------------------------------------------------ device buffers
+// Owned device allocation (freed in the destructor).
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void alloc(size_t count)
+    {
+        if (count <= n && p) return;
+        release();
+        CATHODE_HIP(hipMalloc(&p, count * sizeof(T) + 256));
+        n = count;
+    }
+    void zero(hipStream_t s) { CATHODE_HIP(hipMemsetAsync(p, 0, n * sizeof(T), s)); }
+};
+
+inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
+
+// ------------------------------------------------------------------ reductions
+// Block-level sum of a double held by every thread of a 256-thread block;
+// fixed order (wavefront butterfly then LDS), so results are deterministic.
+__device__ inline double wave_sum(double v)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+__device__ inline double wave_max(double v)
+{
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// linear thread index of a 1-, 2- or 3-D block
+__device__ inline int lin_tid()
+{
+    return (int)(threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z));
+}
+
+// Result valid in the thread with lin_tid() == 0.  NT = threads per block.
+template <int NT>
+__device__ inline double block_sum(double v, double* lds)
+{
+    v = wave_sum(v);
+    const int t = lin_tid();
+    const int lane = t & 63, wid = t >> 6;
+    if (lane == 0) lds[wid] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (t == 0) {
+        for (int k = 0; k < NT / 64; k++) s += lds[k];
+    }
+    __syncthreads();
+    return s;
+}
+
+template <int NT>
+__device__ inline double block_max(double v, double* lds)
+{
+    v = wave_max(v);
+    const int t = lin_tid();
+    const int lane = t & 63, wid = t >> 6;
+    if (lane == 0) lds[wid] = v;
+    __syncthreads();
+    double s = -INFINITY;
+    if (t == 0) {
+        for (int k = 0; k < NT / 64; k++) s = fmax(s, lds[k]);
+    }
+    __syncthreads();
+    return s;
+}
+
+// Sums `nvec` interleaved partial vectors part[b*nvec + k] over blocks b in
+// fixed order; out[k] = total.  One block of 256 threads.
+__global__ void k_reduce_partials(const double* part, int nblocks, int nvec, const int* is_max,
+                                  double* out);
+
+void reduce_partials(const double* part, int nblocks, int nvec, const int* d_is_max, double* out,
+                     hipStream_t s);
+
+}  // namespace cathode

@@ -1,0 +1,938 @@
+// mg.hip -- multigrid kernels and the solve loop (see mg.hpp for the algorithm).
+#include <cmath>
+#include <cstring>
+#include <chrono>
+
+#include "mg.hpp"
+
+namespace cathode {
+
+// 2-D launch shape: x over j (contiguous in memory), y over i
+constexpr int BX = 64, BY = 4;
+
+static inline dim3 grid2(int ni, int nj) { return dim3(cdiv(nj, BX), cdiv(ni, BY)); }
+
+#define CELL_IJ                                            \
+    const int j = 1 + blockIdx.x * BX + threadIdx.x;       \
+    const int i = 1 + blockIdx.y * BY + threadIdx.y;
+
+// ------------------------------------------------------------------ level-0 setup
+// Zero couplings to inactive neighbours (HYPRE's off-grid convention), make
+// inactive cells identity rows, store the row excess s = max(aP - sum a, 0).
+__global__ void k_l0_prepare(int ni, int nj, int ld, const unsigned char* __restrict__ act,
+                             double* aP, double* aE, double* aW, double* aN, double* aS,
+                             double* __restrict__ s_out)
+{
+    CELL_IJ;
+    if (i > ni || j > nj) return;
+    const size_t k = (size_t)i * ld + j;
+    if (!act[k]) {
+        aP[k] = 1.0;
+        aE[k] = aW[k] = aN[k] = aS[k] = 0.0;
+        s_out[k] = INFINITY;
+        return;
+    }
+    double e = act[k + ld] ? aE[k] : 0.0;
+    double w = act[k - ld] ? aW[k] : 0.0;
+    double n = act[k + 1] ? aN[k] : 0.0;
+    double s = act[k - 1] ? aS[k] : 0.0;
+    aE[k] = e;
+    aW[k] = w;
+    aN[k] = n;
+    aS[k] = s;
+    double ex = aP[k] - (e + w + n + s);
+    s_out[k] = ex > 0.0 ? ex : 0.0;
+}
+
+// Radial weights from column sums; one block.
+__global__ void k_l0_weights(int ni, int nj, int ld, const double* __restrict__ aN,
+                             const double* __restrict__ aS, double* __restrict__ w)
+{
+    extern __shared__ double ratio[];
+    for (int j = 1 + threadIdx.x; j < nj; j += blockDim.x) {
+        double sN = 0.0, sS = 0.0;
+        for (int i = 1; i <= ni; i++) {
+            sN += aN[(size_t)i * ld + j];
+            sS += aS[(size_t)i * ld + j + 1];
+        }
+        double r = (sN > 0.0 && sS > 0.0) ? sN / sS : 1.0;
+        ratio[j] = r < 1e-2 ? 1e-2 : (r > 1e2 ? 1e2 : r);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double v = 1.0;
+        w[0] = 1.0;
+        w[1] = 1.0;
+        for (int j = 1; j < nj; j++) {
+            v *= ratio[j];
+            w[j + 1] = v;
+        }
+        w[nj + 1] = v;
+    }
+}
+
+__global__ void k_fill(double* p, int n, double v)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) p[k] = v;
+}
+
+// mass part m = w_j * min(s) over the 5-point neighbourhood
+__global__ void k_l0_mass(int ni, int nj, int ld, const unsigned char* __restrict__ act,
+                          const double* __restrict__ s, const double* __restrict__ w,
+                          double* __restrict__ m)
+{
+    CELL_IJ;
+    if (i > ni || j > nj) return;
+    const size_t k = (size_t)i * ld + j;
+    if (!act[k]) {
+        m[k] = 0.0;
+        return;
+    }
+    double v = s[k];
+    if (act[k + ld]) v = fmin(v, s[k + ld]);
+    if (act[k - ld]) v = fmin(v, s[k - ld]);
+    if (act[k + 1]) v = fmin(v, s[k + 1]);
+    if (act[k - 1]) v = fmin(v, s[k - 1]);
+    m[k] = w[j] * v;
+}
+
+// coarse operator of level F -> C (thread per coarse cell)
+__global__ void k_coarsen(int fni, int fnj, int fld, int fi, int fj,
+                          const double* __restrict__ faP, const double* __restrict__ faE,
+                          const double* __restrict__ faW, const double* __restrict__ faN,
+                          const double* __restrict__ faS, const double* __restrict__ fm,
+                          const unsigned char* __restrict__ fact, const double* __restrict__ w,
+                          int cni, int cnj, int cld, double* caP, double* caE, double* caW,
+                          double* caN, double* caS, double* cb, double* cx, double* cm,
+                          unsigned char* cact, double hx, double hy, double hb)
+{
+    const int J = 1 + blockIdx.x * BX + threadIdx.x;
+    const int I = 1 + blockIdx.y * BY + threadIdx.y;
+    if (I > cni || J > cnj) return;
+    const int i0 = fi * (I - 1) + 1, i1 = min(fi * I, fni);
+    const int j0 = fj * (J - 1) + 1, j1 = min(fj * J, fnj);
+    double S = 0.0, M = 0.0, E = 0.0, W = 0.0, N = 0.0, So = 0.0;
+    int any = 0;
+    for (int i = i0; i <= i1; i++)
+        for (int j = j0; j <= j1; j++) {
+            const size_t k = (size_t)i * fld + j;
+            if (!fact[k]) continue;
+            any = 1;
+            const double wj = w ? w[j] : 1.0;
+            const double e = faE[k], ww = faW[k], n = faN[k], s = faS[k];
+            const double ex = faP[k] - (e + ww + n + s);
+            S += wj * (ex > 0.0 ? ex : 0.0);
+            M += fm[k];
+            if (i == i1) E += wj * e;
+            if (i == i0) W += wj * ww;
+            if (j == j1) N += wj * n;
+            if (j == j0) So += wj * s;
+        }
+    const size_t kc = (size_t)I * cld + J;
+    cact[kc] = (unsigned char)any;
+    cb[kc] = 0.0;
+    cx[kc] = 0.0;
+    if (!any) {
+        caP[kc] = 1.0;
+        caE[kc] = caW[kc] = caN[kc] = caS[kc] = 0.0;
+        cm[kc] = 0.0;
+        return;
+    }
+    E *= hx;
+    W *= hx;
+    N *= hy;
+    So *= hy;
+    caE[kc] = E;
+    caW[kc] = W;
+    caN[kc] = N;
+    caS[kc] = So;
+    cm[kc] = M;
+    caP[kc] = (M + hb * (S - M)) + (E + W + N + So);
+}
+
+// Dense inverse of the coarsest operator (<= 64 unknowns), in-place
+// Gauss-Jordan with partial pivoting; one block of 256 threads.
+__global__ void k_coarse_inverse(int ni, int nj, int ld, const double* __restrict__ aP,
+                                 const double* __restrict__ aE, const double* __restrict__ aW,
+                                 const double* __restrict__ aN, const double* __restrict__ aS,
+                                 const unsigned char* __restrict__ act, int* __restrict__ cidx,
+                                 double* __restrict__ inv64, float* __restrict__ inv32,
+                                 int* __restrict__ n_out, const double* __restrict__ scal, int mixed)
+{
+    __shared__ double A[MG_COARSE_MAX * MG_COARSE_MAX];
+    __shared__ int idx[MG_COARSE_CELLS];
+    __shared__ int perm[MG_COARSE_MAX];
+    __shared__ double fcol[MG_COARSE_MAX];
+    __shared__ int n_sh, piv_sh;
+    const int tid = threadIdx.x;
+    const int ncell = (ni + 2) * ld;
+    if (tid == 0) {
+        int n = 0;
+        for (int k = 0; k < ncell; k++) idx[k] = act[k] ? n++ : -1;
+        n_sh = n;
+        *n_out = n;
+    }
+    __syncthreads();
+    const int n = n_sh;
+    for (int q = tid; q < n * n; q += blockDim.x) A[q] = 0.0;
+    for (int k = tid; k < ncell; k += blockDim.x) cidx[k] = idx[k];
+    __syncthreads();
+    for (int k = tid; k < ncell; k += blockDim.x) {
+        int row = idx[k];
+        if (row < 0) continue;
+        A[row * n + row] = aP[k];
+        int nb[4] = {idx[k + ld], idx[k - ld], idx[k + 1], idx[k - 1]};
+        double a[4] = {aE[k], aW[k], aN[k], aS[k]};
+        for (int q = 0; q < 4; q++)
+            if (nb[q] >= 0) A[row * n + nb[q]] -= a[q];
+    }
+    __syncthreads();
+    for (int k = 0; k < n; k++) {
+        if (tid == 0) {
+            int p = k;
+            double best = fabs(A[k * n + k]);
+            for (int i = k + 1; i < n; i++)
+                if (fabs(A[i * n + k]) > best) {
+                    best = fabs(A[i * n + k]);
+                    p = i;
+                }
+            piv_sh = p;
+            perm[k] = p;
+        }
+        __syncthreads();
+        const int p = piv_sh;
+        if (p != k)
+            for (int c = tid; c < n; c += blockDim.x) {
+                double t = A[k * n + c];
+                A[k * n + c] = A[p * n + c];
+                A[p * n + c] = t;
+            }
+        __syncthreads();
+        double piv = A[k * n + k];
+        if (piv == 0.0) piv = 1e-300;
+        __syncthreads();
+        for (int c = tid; c < n; c += blockDim.x) A[k * n + c] = (c == k ? 1.0 : A[k * n + c]) / piv;
+        for (int r = tid; r < n; r += blockDim.x) fcol[r] = A[r * n + k];
+        __syncthreads();
+        for (int q = tid; q < n * n; q += blockDim.x) {
+            const int i = q / n, c = q % n;
+            if (i == k) continue;
+            const double f = fcol[i];
+            if (c == k)
+                A[q] = -f * A[k * n + k];
+            else
+                A[q] -= f * A[k * n + c];
+        }
+        __syncthreads();
+    }
+    // undo the row exchanges as column exchanges, last first
+    for (int k = n - 1; k >= 0; k--) {
+        const int p = perm[k];
+        if (p != k)
+            for (int r = tid; r < n; r += blockDim.x) {
+                double t = A[r * n + k];
+                A[r * n + k] = A[r * n + p];
+                A[r * n + p] = t;
+            }
+        __syncthreads();
+    }
+    // the FP32 levels hold s32 * A, so their coarsest inverse is A^-1 / s32
+    const double s32 = mixed ? scal[3] : 1.0;
+    for (int q = tid; q < n * n; q += blockDim.x) {
+        inv64[q] = A[q];
+        inv32[q] = (float)(A[q] / s32);
+    }
+}
+
+__global__ void k_level_to_f32(size_t n, const double* __restrict__ src, float* __restrict__ dst,
+                               const double* __restrict__ scal)
+{
+    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    dst[k] = (float)(src[k] * scal[3]);
+}
+
+// ------------------------------------------------------------------ V-cycle kernels
+template <class T>
+__global__ void k_smooth(int ni, int nj, int ld, const T* __restrict__ aP, const T* __restrict__ aE,
+                         const T* __restrict__ aW, const T* __restrict__ aN,
+                         const T* __restrict__ aS, const T* __restrict__ b, T* __restrict__ x,
+                         int color)
+{
+    CELL_IJ;
+    if (i > ni || j > nj || ((i + j) & 1) != color) return;
+    const size_t k = (size_t)i * ld + j;
+    T s = b[k] + aE[k] * x[k + ld] + aW[k] * x[k - ld] + aN[k] * x[k + 1] + aS[k] * x[k - 1];
+    x[k] = s / aP[k];
+}
+
+template <class T>
+__global__ void k_restrict(int fni, int fnj, int fld, int fi, int fj, const T* __restrict__ aP,
+                           const T* __restrict__ aE, const T* __restrict__ aW,
+                           const T* __restrict__ aN, const T* __restrict__ aS,
+                           const T* __restrict__ b, const T* __restrict__ x,
+                           const double* __restrict__ w, int cni, int cnj, int cld,
+                           T* __restrict__ cb, T* __restrict__ cx)
+{
+    const int J = 1 + blockIdx.x * BX + threadIdx.x;
+    const int I = 1 + blockIdx.y * BY + threadIdx.y;
+    if (I > cni || J > cnj) return;
+    const int i0 = fi * (I - 1) + 1, i1 = min(fi * I, fni);
+    const int j0 = fj * (J - 1) + 1, j1 = min(fj * J, fnj);
+    T sum = 0;
+    for (int i = i0; i <= i1; i++)
+        for (int j = j0; j <= j1; j++) {
+            const size_t k = (size_t)i * fld + j;
+            T r = b[k] - aP[k] * x[k] + aE[k] * x[k + fld] + aW[k] * x[k - fld] + aN[k] * x[k + 1] +
+                  aS[k] * x[k - 1];
+            sum += (w ? (T)w[j] : (T)1) * r;
+        }
+    const size_t kc = (size_t)I * cld + J;
+    cb[kc] = sum;
+    cx[kc] = 0;
+}
+
+template <class T>
+__global__ void k_prolong(int fni, int fnj, int fld, int fi, int fj,
+                          const unsigned char* __restrict__ act, T* __restrict__ x, int cld,
+                          const T* __restrict__ cx)
+{
+    CELL_IJ;
+    if (i > fni || j > fnj) return;
+    const size_t k = (size_t)i * fld + j;
+    if (!act[k]) return;
+    const int I = fi == 2 ? (i + 1) >> 1 : i, J = fj == 2 ? (j + 1) >> 1 : j;
+    x[k] += cx[(size_t)I * cld + J];
+}
+
+template <class T>
+__global__ void k_coarse_apply(int ncell, const int* __restrict__ cidx, const int* __restrict__ n_p,
+                               const T* __restrict__ inv, const T* __restrict__ b, T* __restrict__ x)
+{
+    __shared__ T v[MG_COARSE_MAX];
+    __shared__ int map[MG_COARSE_CELLS];
+    const int n = *n_p;
+    for (int k = threadIdx.x; k < ncell; k += blockDim.x) {
+        int q = cidx[k];
+        map[k] = q;
+        if (q >= 0) v[q] = b[k];
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < ncell; k += blockDim.x) {
+        int q = map[k];
+        if (q < 0) continue;
+        T s = 0;
+        for (int c = 0; c < n; c++) s += inv[q * n + c] * v[c];
+        x[k] = s;
+    }
+}
+
+// ------------------------------------------------------------------ outer loop kernels
+// Device scalars (MGSolver::scal):
+enum {
+    SC_BB = 0,    // |b|^2
+    SC_RR = 1,    // |r|^2 (recursive residual)
+    SC_SCALE = 2, // mixed: preconditioner input scale sqrt(n)/|b|
+    SC_S32 = 3,   // mixed: power-of-two matrix scale of the FP32 levels
+    SC_RHO = 4,   // BiCGStab rho_{k-1}
+    SC_RHO1 = 5,  // BiCGStab rho_k = (r0, r)
+    SC_AL = 6,    // alpha
+    SC_OM = 7,    // omega
+    SC_NC = 8,    // (int) coarsest unknowns
+    SC_R0V = 9,
+    SC_TS = 10,
+    SC_TT = 11,
+    SC_COUNT = 16
+};
+
+// sum of squares of the active entries of v (partials per block)
+__global__ void k_sumsq(int ni, int nj, int ld, const double* __restrict__ v,
+                        const unsigned char* __restrict__ act, double* __restrict__ part)
+{
+    __shared__ double lds[8];
+    CELL_IJ;
+    double s = 0.0;
+    if (i <= ni && j <= nj) {
+        const size_t k = (size_t)i * ld + j;
+        if (act[k]) s = v[k] * v[k];
+    }
+    double tot = block_sum<BX * BY>(s, lds);
+    if (lin_tid() == 0) part[blockIdx.y * gridDim.x + blockIdx.x] = tot;
+}
+
+// preconditioner input: vin = T(src * in_scale), vx = 0
+template <class T>
+__device__ inline T to_pc(double v, const double* sc, int mixed)
+{
+    return mixed ? (T)(v * sc[SC_S32] * sc[SC_SCALE]) : (T)v;
+}
+template <class T>
+__device__ inline double from_pc(T y, const double* sc, int mixed)
+{
+    return mixed ? (double)y / sc[SC_SCALE] : (double)y;
+}
+
+// r = b - A x, r0 = r, p = v = 0 (restart); partials |r|^2 (slot 0), (r0,r) (slot 1)
+__global__ void k_bicg_restart(int ni, int nj, int ld, const double* __restrict__ aP,
+                               const double* __restrict__ aE, const double* __restrict__ aW,
+                               const double* __restrict__ aN, const double* __restrict__ aS,
+                               const double* __restrict__ b, const double* __restrict__ x,
+                               const unsigned char* __restrict__ act, double* __restrict__ r,
+                               double* __restrict__ r0, double* __restrict__ p, double* __restrict__ v,
+                               double* __restrict__ part, int nb)
+{
+    __shared__ double lds[8];
+    CELL_IJ;
+    double rr = 0.0;
+    if (i <= ni && j <= nj) {
+        const size_t k = (size_t)i * ld + j;
+        double q = 0.0;
+        if (act[k]) {
+            q = b[k] - aP[k] * x[k] + aE[k] * x[k + ld] + aW[k] * x[k - ld] + aN[k] * x[k + 1] +
+                aS[k] * x[k - 1];
+            rr = q * q;
+        }
+        r[k] = q;
+        r0[k] = q;
+        p[k] = 0.0;
+        v[k] = 0.0;
+    }
+    double t = block_sum<BX * BY>(rr, lds);
+    if (lin_tid() == 0) {
+        const int bid = blockIdx.y * gridDim.x + blockIdx.x;
+        part[bid] = t;
+        part[nb + bid] = t;
+    }
+}
+
+// p = r + beta (p - om v), beta = (rho1/rho)(al/om) (0 after a restart);
+// preconditioner input vin = p, vx = 0
+template <class T>
+__global__ void k_bicg_p(int ni, int nj, int ld, const unsigned char* __restrict__ act,
+                         const double* __restrict__ r, const double* __restrict__ v,
+                         double* __restrict__ p, T* __restrict__ vin, T* __restrict__ vx,
+                         const double* __restrict__ sc, int first, int mixed)
+{
+    CELL_IJ;
+    if (i > ni || j > nj) return;
+    const size_t k = (size_t)i * ld + j;
+    double pk = 0.0;
+    if (act[k]) {
+        if (first) {
+            pk = r[k];
+        } else {
+            const double beta = (sc[SC_RHO1] / sc[SC_RHO]) * (sc[SC_AL] / sc[SC_OM]);
+            pk = r[k] + beta * (p[k] - sc[SC_OM] * v[k]);
+        }
+    }
+    p[k] = pk;
+    vin[k] = to_pc<T>(pk, sc, mixed);
+    vx[k] = 0;
+}
+
+// y = M^-1(in) from the V-cycle output; out = A y; partial dot (dotvec, out)
+// -> slot 0; with want_tt also (out, out) -> slot 1.
+template <class T>
+__global__ void k_bicg_apply(int ni, int nj, int ld, const double* __restrict__ aP,
+                             const double* __restrict__ aE, const double* __restrict__ aW,
+                             const double* __restrict__ aN, const double* __restrict__ aS,
+                             const unsigned char* __restrict__ act, const T* __restrict__ vx,
+                             double* __restrict__ y, double* __restrict__ out,
+                             const double* __restrict__ dotvec, const double* __restrict__ sc, int mixed,
+                             int want_tt, double* __restrict__ part, int nb)
+{
+    __shared__ double lds[8];
+    CELL_IJ;
+    double d1 = 0.0, d2 = 0.0;
+    if (i <= ni && j <= nj) {
+        const size_t k = (size_t)i * ld + j;
+        double o = 0.0;
+        double yk = 0.0;
+        if (act[k]) {
+            // inactive neighbours have zero coupling and a zero V-cycle output
+            yk = from_pc<T>(vx[k], sc, mixed);
+            o = aP[k] * yk - aE[k] * from_pc<T>(vx[k + ld], sc, mixed) - aW[k] * from_pc<T>(vx[k - ld], sc, mixed) -
+                aN[k] * from_pc<T>(vx[k + 1], sc, mixed) - aS[k] * from_pc<T>(vx[k - 1], sc, mixed);
+            d1 = dotvec[k] * o;
+            d2 = o * o;
+        }
+        y[k] = yk;
+        out[k] = o;
+    }
+    double t1 = block_sum<BX * BY>(d1, lds);
+    double t2 = want_tt ? block_sum<BX * BY>(d2, lds) : 0.0;
+    if (lin_tid() == 0) {
+        const int bid = blockIdx.y * gridDim.x + blockIdx.x;
+        part[bid] = t1;
+        if (want_tt) part[nb + bid] = t2;
+    }
+}
+
+// s = r - al v, preconditioner input vin = s, vx = 0
+template <class T>
+__global__ void k_bicg_s(int ni, int nj, int ld, const unsigned char* __restrict__ act,
+                         const double* __restrict__ r, const double* __restrict__ v,
+                         double* __restrict__ sv, T* __restrict__ vin, T* __restrict__ vx,
+                         const double* __restrict__ sc, int mixed)
+{
+    CELL_IJ;
+    if (i > ni || j > nj) return;
+    const size_t k = (size_t)i * ld + j;
+    const double q = act[k] ? r[k] - sc[SC_AL] * v[k] : 0.0;
+    sv[k] = q;
+    vin[k] = to_pc<T>(q, sc, mixed);
+    vx[k] = 0;
+}
+
+// x += al ph + om sh; r = s - om t; partials |r|^2 (slot 0), (r0, r) (slot 1)
+__global__ void k_bicg_x(int ni, int nj, int ld, const unsigned char* __restrict__ act,
+                         double* __restrict__ x, double* __restrict__ r, const double* __restrict__ r0,
+                         const double* __restrict__ ph, const double* __restrict__ sh,
+                         const double* __restrict__ sv, const double* __restrict__ t,
+                         const double* __restrict__ sc, double* __restrict__ part, int nb)
+{
+    __shared__ double lds[8];
+    CELL_IJ;
+    double a = 0.0, c = 0.0;
+    if (i <= ni && j <= nj) {
+        const size_t k = (size_t)i * ld + j;
+        if (act[k]) {
+            const double al = sc[SC_AL], om = sc[SC_OM];
+            x[k] = x[k] + al * ph[k] + om * sh[k];
+            const double q = sv[k] - om * t[k];
+            r[k] = q;
+            a = q * q;
+            c = r0[k] * q;
+        }
+    }
+    double t1 = block_sum<BX * BY>(a, lds);
+    double t2 = block_sum<BX * BY>(c, lds);
+    if (lin_tid() == 0) {
+        const int bid = blockIdx.y * gridDim.x + blockIdx.x;
+        part[bid] = t1;
+        part[nb + bid] = t2;
+    }
+}
+
+// finishing reductions (one block, fixed order)
+__device__ inline double sum_part(const double* part, int nb, double* lds)
+{
+    double s = 0.0;
+    for (int k = threadIdx.x; k < nb; k += blockDim.x) s += part[k];
+    return block_sum<256>(s, lds);
+}
+
+// what: 0 |b|^2 (+ mixed input scale); 1 after restart: rr, rho1 = (r0,r);
+// 2 al = rho1 / (r0, v); 3 om = (t,s)/(t,t); 4 after the x update: rr, rho = rho1, rho1 = (r0,r)
+__global__ void k_bicg_finish(const double* __restrict__ part, int nb, double* __restrict__ sc, int what,
+                              double nact)
+{
+    __shared__ double lds[8];
+    const double a = sum_part(part, nb, lds);
+    const double b = (what == 1 || what == 3 || what == 4) ? sum_part(part + nb, nb, lds) : 0.0;
+    if (threadIdx.x != 0) return;
+    switch (what) {
+    case 0:
+        sc[SC_BB] = a;
+        sc[SC_SCALE] = a > 0.0 ? sqrt(nact / a) : 1.0;
+        break;
+    case 1:
+        sc[SC_RR] = a;
+        sc[SC_RHO1] = b;
+        sc[SC_RHO] = 1.0;
+        sc[SC_AL] = 1.0;
+        sc[SC_OM] = 1.0;
+        break;
+    case 2:
+        sc[SC_R0V] = a;
+        sc[SC_AL] = sc[SC_RHO1] / a;
+        break;
+    case 3:
+        sc[SC_TS] = a;
+        sc[SC_TT] = b;
+        sc[SC_OM] = b > 0.0 ? a / b : 0.0;
+        break;
+    case 4:
+        sc[SC_RR] = a;
+        sc[SC_RHO] = sc[SC_RHO1];
+        sc[SC_RHO1] = b;
+        break;
+    }
+}
+
+// max |aP| of level 0 -> power-of-two scale scal[3] with scal[3] * max|aP| in [0.5, 1)
+__global__ void k_maxabs(int ni, int nj, int ld, const double* __restrict__ aP,
+                         const unsigned char* __restrict__ act, double* __restrict__ part)
+{
+    __shared__ double lds[8];
+    CELL_IJ;
+    double v = 0.0;
+    if (i <= ni && j <= nj) {
+        const size_t k = (size_t)i * ld + j;
+        if (act[k]) v = fabs(aP[k]);
+    }
+    double m = block_max<BX * BY>(v, lds);
+    if (lin_tid() == 0) part[blockIdx.y * gridDim.x + blockIdx.x] = m;
+}
+
+__global__ void k_finish_scale(const double* __restrict__ part, int nb, double* __restrict__ scal)
+{
+    __shared__ double lds[8];
+    double m = 0.0;
+    for (int k = threadIdx.x; k < nb; k += blockDim.x) m = fmax(m, part[k]);
+    m = block_max<256>(m, lds);
+    if (threadIdx.x == 0) {
+        int e = 0;
+        (void)frexp(m > 0.0 ? m : 1.0, &e);
+        scal[3] = ldexp(1.0, -e);
+    }
+}
+
+// ------------------------------------------------------------------ host side
+MGSolver::~MGSolver()
+{
+    for (auto& e : prof.ev) (void)hipEventDestroy(e);
+    if (h_scal) (void)hipHostFree(h_scal);
+}
+
+static int pad_ld(int nj) { return ((nj + 2 + 7) / 8) * 8; }
+
+void MGSolver::configure(int ni, int nj, hipStream_t s)
+{
+    (void)s;
+    if (ni == cfg_ni && nj == cfg_nj) return;
+    cfg_ni = ni;
+    cfg_nj = nj;
+    // shapes
+    int dims[MG_MAX_LEVELS][2];
+    int n = 0;
+    dims[0][0] = ni;
+    dims[0][1] = nj;
+    n = 1;
+    while (n < MG_MAX_LEVELS) {
+        int a = dims[n - 1][0], b = dims[n - 1][1];
+        if ((long)a * b <= MG_COARSE_MAX || (a == 1 && b == 1)) break;
+        dims[n][0] = a > 1 ? (a + 1) / 2 : a;
+        dims[n][1] = b > 1 ? (b + 1) / 2 : b;
+        n++;
+    }
+    nl = n;
+    size_t tot = 0;
+    for (int l = 0; l < nl; l++) {
+        level_off[l] = tot;
+        tot += (size_t)(dims[l][0] + 2) * pad_ld(dims[l][1]);
+    }
+    level_off[nl] = tot;
+    store64.alloc(7 * tot);
+    store32.alloc(7 * tot);
+    mass.alloc(tot);
+    actall.alloc(tot);
+    CATHODE_HIP(hipMemset(store64.p, 0, 7 * tot * sizeof(double)));
+    CATHODE_HIP(hipMemset(store32.p, 0, 7 * tot * sizeof(float)));
+    CATHODE_HIP(hipMemset(mass.p, 0, tot * sizeof(double)));
+    CATHODE_HIP(hipMemset(actall.p, 0, tot));
+    for (int l = 0; l < nl; l++) {
+        const size_t o = level_off[l], sz = level_off[l + 1] - level_off[l];
+        Level<double>& A = L64[l];
+        Level<float>& B = L32[l];
+        A.ni = B.ni = dims[l][0];
+        A.nj = B.nj = dims[l][1];
+        A.ld = B.ld = pad_ld(dims[l][1]);
+        A.cx = B.cx = (l + 1 < nl) && dims[l][0] > 1;
+        A.cy = B.cy = (l + 1 < nl) && dims[l][1] > 1;
+        double* p = store64.p;
+        float* q = store32.p;
+        // b and x of level l; the five coefficient arrays are placed below
+        A.b = p + 5 * tot + o;
+        A.x = p + 6 * tot + o;
+        B.b = q + 5 * tot + o;
+        B.x = q + 6 * tot + o;
+        (void)sz;
+    }
+    // the coefficient block of level l starts at 5*level_off[l], so that the
+    // five arrays of a level are contiguous (one FP32 conversion launch)
+    for (int l = 0; l < nl; l++) {
+        const size_t o = 5 * level_off[l], sz = level_off[l + 1] - level_off[l];
+        L64[l].aP = store64.p + o;
+        L64[l].aE = store64.p + o + sz;
+        L64[l].aW = store64.p + o + 2 * sz;
+        L64[l].aN = store64.p + o + 3 * sz;
+        L64[l].aS = store64.p + o + 4 * sz;
+        L32[l].aP = store32.p + o;
+        L32[l].aE = store32.p + o + sz;
+        L32[l].aW = store32.p + o + 2 * sz;
+        L32[l].aN = store32.p + o + 3 * sz;
+        L32[l].aS = store32.p + o + 4 * sz;
+    }
+    const size_t n0 = level_off[1] - level_off[0];
+    act0.alloc(n0);
+    CATHODE_HIP(hipMemset(act0.p, 0, n0));
+    bsol.alloc(n0);
+    xsol.alloc(n0);
+    CATHODE_HIP(hipMemset(bsol.p, 0, n0 * sizeof(double)));
+    CATHODE_HIP(hipMemset(xsol.p, 0, n0 * sizeof(double)));
+    w0.alloc(nj + 2);
+    const int nc = (dims[nl - 1][0] + 2) * pad_ld(dims[nl - 1][1]);
+    cidx.alloc(nc + 1);
+    cinv64.alloc(MG_COARSE_MAX * MG_COARSE_MAX);
+    cinv32.alloc(MG_COARSE_MAX * MG_COARSE_MAX);
+    part.alloc(2 * (cdiv(nj, BX) * cdiv(ni, BY)) + 64);
+    scal.alloc(16);
+    kv.alloc(8 * n0);
+    CATHODE_HIP(hipMemset(kv.p, 0, 8 * n0 * sizeof(double)));
+    if (!h_scal) CATHODE_HIP(hipHostMalloc(&h_scal, 16 * sizeof(double), hipHostMallocDefault));
+}
+
+void MGSolver::setup(int mode, hipStream_t s)
+{
+    Level<double>& F = L64[0];
+    const size_t n0 = level_off[1] - level_off[0];
+    // level-0 activity lives in act0 (filled by the caller); mirror into actall
+    CATHODE_HIP(hipMemcpyAsync(actall.p, act0.p, n0, hipMemcpyDeviceToDevice, s));
+    double* sbuf = mass.p;  // excess s of level 0 temporarily in the mass slot of level 1.. (reused)
+    // use the level-0 b array as scratch for s (it is overwritten by the outer loop)
+    sbuf = F.b;
+    hipLaunchKernelGGL(k_l0_prepare, grid2(F.ni, F.nj), dim3(BX, BY), 0, s, F.ni, F.nj, F.ld, act0.p,
+                       F.aP, F.aE, F.aW, F.aN, F.aS, sbuf);
+    CATHODE_LAUNCH_CHECK();
+    const double* w = user_w;
+    if (!w) {
+        hipLaunchKernelGGL(k_l0_weights, dim3(1), dim3(256), (F.nj + 2) * sizeof(double), s, F.ni, F.nj,
+                           F.ld, F.aN, F.aS, w0.p);
+        CATHODE_LAUNCH_CHECK();
+        w = w0.p;
+    }
+    hipLaunchKernelGGL(k_l0_mass, grid2(F.ni, F.nj), dim3(BX, BY), 0, s, F.ni, F.nj, F.ld, act0.p, sbuf,
+                       w, mass.p + level_off[0]);
+    CATHODE_LAUNCH_CHECK();
+    for (int l = 0; l + 1 < nl; l++) {
+        Level<double>& A = L64[l];
+        Level<double>& C = L64[l + 1];
+        const int fi = A.cx ? 2 : 1, fj = A.cy ? 2 : 1;
+        const double hx = A.cx ? 0.5 : 1.0, hy = A.cy ? 0.5 : 1.0;
+        const double hb = (A.cx && A.cy) ? 0.5 : ((A.cx || A.cy) ? 0.75 : 1.0);
+        hipLaunchKernelGGL(k_coarsen, grid2(C.ni, C.nj), dim3(BX, BY), 0, s, A.ni, A.nj, A.ld, fi, fj, A.aP,
+                           A.aE, A.aW, A.aN, A.aS, mass.p + level_off[l], actall.p + level_off[l],
+                           l == 0 ? w : (const double*)nullptr, C.ni, C.nj, C.ld, C.aP, C.aE, C.aW, C.aN,
+                           C.aS, C.b, C.x, mass.p + level_off[l + 1], actall.p + level_off[l + 1], hx, hy,
+                           hb);
+        CATHODE_LAUNCH_CHECK();
+    }
+    if (mode == CATHODE_MODE_MIXED) {
+        const int nb = cdiv(F.nj, BX) * cdiv(F.ni, BY);
+        hipLaunchKernelGGL(k_maxabs, grid2(F.ni, F.nj), dim3(BX, BY), 0, s, F.ni, F.nj, F.ld, F.aP, act0.p,
+                           part.p);
+        hipLaunchKernelGGL(k_finish_scale, dim3(1), dim3(256), 0, s, part.p, nb, scal.p);
+        for (int l = 0; l < nl; l++) {
+            const size_t n5 = 5 * (level_off[l + 1] - level_off[l]);
+            hipLaunchKernelGGL(k_level_to_f32, dim3(cdiv((long)n5, 256)), dim3(256), 0, s, n5, L64[l].aP,
+                               L32[l].aP, (const double*)scal.p);
+        }
+        CATHODE_LAUNCH_CHECK();
+    }
+    Level<double>& Cc = L64[nl - 1];
+    hipLaunchKernelGGL(k_coarse_inverse, dim3(1), dim3(256), 0, s, Cc.ni, Cc.nj, Cc.ld, Cc.aP, Cc.aE, Cc.aW,
+                       Cc.aN, Cc.aS, actall.p + level_off[nl - 1], cidx.p, cinv64.p, cinv32.p,
+                       (int*)(scal.p + 8), (const double*)scal.p, mode == CATHODE_MODE_MIXED ? 1 : 0);
+    CATHODE_LAUNCH_CHECK();
+    (void)n0;
+}
+
+template <class T>
+void MGSolver::vcycle(Level<T>* L, const T* cinv, hipStream_t s)
+{
+    for (int l = 0; l + 1 < nl; l++) {
+        Level<T>& A = L[l];
+        Level<T>& C = L[l + 1];
+        for (int k = 0; k < 2; k++)
+            for (int c = 0; c < 2; c++) {
+                const bool rec = prof.enabled && l == 0;
+                if (rec) {
+                    hipEvent_t a, b;
+                    CATHODE_HIP(hipEventCreate(&a));
+                    CATHODE_HIP(hipEventCreate(&b));
+                    prof.ev.push_back(a);
+                    prof.ev.push_back(b);
+                    CATHODE_HIP(hipEventRecord(a, s));
+                }
+                hipLaunchKernelGGL(k_smooth<T>, grid2(A.ni, A.nj), dim3(BX, BY), 0, s, A.ni, A.nj, A.ld, A.aP,
+                                   A.aE, A.aW, A.aN, A.aS, A.b, A.x, c);
+                if (rec) CATHODE_HIP(hipEventRecord(prof.ev.back(), s));
+            }
+        const int fi = A.cx ? 2 : 1, fj = A.cy ? 2 : 1;
+        hipLaunchKernelGGL(k_restrict<T>, grid2(C.ni, C.nj), dim3(BX, BY), 0, s, A.ni, A.nj, A.ld, fi, fj, A.aP,
+                           A.aE, A.aW, A.aN, A.aS, A.b, A.x, l == 0 ? (user_w ? user_w : w0.p) : nullptr, C.ni,
+                           C.nj, C.ld, C.b, C.x);
+    }
+    Level<T>& Cc = L[nl - 1];
+    hipLaunchKernelGGL(k_coarse_apply<T>, dim3(1), dim3(256), 0, s, (Cc.ni + 2) * Cc.ld, cidx.p,
+                       (const int*)(scal.p + 8), cinv, Cc.b, Cc.x);
+    for (int l = nl - 2; l >= 0; l--) {
+        Level<T>& A = L[l];
+        Level<T>& C = L[l + 1];
+        const int fi = A.cx ? 2 : 1, fj = A.cy ? 2 : 1;
+        hipLaunchKernelGGL(k_prolong<T>, grid2(A.ni, A.nj), dim3(BX, BY), 0, s, A.ni, A.nj, A.ld, fi, fj,
+                           actall.p + level_off[l], A.x, C.ld, C.x);
+        for (int k = 0; k < 2; k++)
+            for (int c = 0; c < 2; c++) {
+                const bool rec = prof.enabled && l == 0;
+                if (rec) {
+                    hipEvent_t a, b;
+                    CATHODE_HIP(hipEventCreate(&a));
+                    CATHODE_HIP(hipEventCreate(&b));
+                    prof.ev.push_back(a);
+                    prof.ev.push_back(b);
+                    CATHODE_HIP(hipEventRecord(a, s));
+                }
+                hipLaunchKernelGGL(k_smooth<T>, grid2(A.ni, A.nj), dim3(BX, BY), 0, s, A.ni, A.nj, A.ld, A.aP,
+                                   A.aE, A.aW, A.aN, A.aS, A.b, A.x, c);
+                if (rec) CATHODE_HIP(hipEventRecord(prof.ev.back(), s));
+            }
+    }
+    CATHODE_LAUNCH_CHECK();
+}
+
+double MGSolver::read_scalar(const double* d, hipStream_t s)
+{
+    CATHODE_HIP(hipMemcpyAsync(h_scal, d, sizeof(double), hipMemcpyDeviceToHost, s));
+    CATHODE_HIP(hipStreamSynchronize(s));
+    return h_scal[0];
+}
+
+int MGSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_status* st)
+{
+    auto t0 = std::chrono::steady_clock::now();
+    Level<double>& F = L64[0];
+    const size_t n0 = level_off[1] - level_off[0];
+    const int nb = cdiv(F.nj, BX) * cdiv(F.ni, BY);
+    const int mixed = mode == CATHODE_MODE_MIXED ? 1 : 0;
+    const dim3 g = grid2(F.ni, F.nj), blk(BX, BY);
+    setup(mode, s);
+    hipLaunchKernelGGL(k_sumsq, g, blk, 0, s, F.ni, F.nj, F.ld, bsol.p, act0.p, part.p);
+    hipLaunchKernelGGL(k_bicg_finish, dim3(1), dim3(256), 0, s, part.p, nb, scal.p, 0,
+                       (double)F.ni * (double)F.nj);
+    CATHODE_LAUNCH_CHECK();
+    CATHODE_HIP(hipMemsetAsync(xsol.p, 0, n0 * sizeof(double), s));
+    const double bb = read_scalar(scal.p + SC_BB, s);
+    double* r = kv.p;
+    double* r0 = r + n0;
+    double* p = r0 + n0;
+    double* v = p + n0;
+    double* sv = v + n0;
+    double* t = sv + n0;
+    double* ph = t + n0;
+    double* sh = ph + n0;
+    int it = 0;
+    double rel = 0.0;
+    bool restart = true;
+    if (!(bb > 0.0)) {
+        rel = bb == 0.0 ? 0.0 : NAN;  // b = 0: HYPRE returns x = 0
+    } else {
+        while (it < maxITR) {
+            if (restart) {
+                hipLaunchKernelGGL(k_bicg_restart, g, blk, 0, s, F.ni, F.nj, F.ld, F.aP, F.aE, F.aW, F.aN, F.aS,
+                                   bsol.p, xsol.p, act0.p, r, r0, p, v, part.p, nb);
+                hipLaunchKernelGGL(k_bicg_finish, dim3(1), dim3(256), 0, s, part.p, nb, scal.p, 1, 0.0);
+            }
+            it++;
+            // p, ph = M^-1 p, v = A ph, al
+            if (mixed) {
+                hipLaunchKernelGGL(k_bicg_p<float>, g, blk, 0, s, F.ni, F.nj, F.ld, act0.p, r, v, p, L32[0].b,
+                                   L32[0].x, scal.p, restart ? 1 : 0, 1);
+                vcycle<float>(L32, cinv32.p, s);
+                hipLaunchKernelGGL(k_bicg_apply<float>, g, blk, 0, s, F.ni, F.nj, F.ld, F.aP, F.aE, F.aW, F.aN,
+                                   F.aS, act0.p, L32[0].x, ph, v, r0, scal.p, 1, 0, part.p, nb);
+            } else {
+                hipLaunchKernelGGL(k_bicg_p<double>, g, blk, 0, s, F.ni, F.nj, F.ld, act0.p, r, v, p, F.b, F.x,
+                                   scal.p, restart ? 1 : 0, 0);
+                vcycle<double>(L64, cinv64.p, s);
+                hipLaunchKernelGGL(k_bicg_apply<double>, g, blk, 0, s, F.ni, F.nj, F.ld, F.aP, F.aE, F.aW, F.aN,
+                                   F.aS, act0.p, F.x, ph, v, r0, scal.p, 0, 0, part.p, nb);
+            }
+            hipLaunchKernelGGL(k_bicg_finish, dim3(1), dim3(256), 0, s, part.p, nb, scal.p, 2, 0.0);
+            restart = false;
+            // s, sh = M^-1 s, t = A sh, om
+            if (mixed) {
+                hipLaunchKernelGGL(k_bicg_s<float>, g, blk, 0, s, F.ni, F.nj, F.ld, act0.p, r, v, sv, L32[0].b,
+                                   L32[0].x, scal.p, 1);
+                vcycle<float>(L32, cinv32.p, s);
+                hipLaunchKernelGGL(k_bicg_apply<float>, g, blk, 0, s, F.ni, F.nj, F.ld, F.aP, F.aE, F.aW, F.aN,
+                                   F.aS, act0.p, L32[0].x, sh, t, sv, scal.p, 1, 1, part.p, nb);
+            } else {
+                hipLaunchKernelGGL(k_bicg_s<double>, g, blk, 0, s, F.ni, F.nj, F.ld, act0.p, r, v, sv, F.b, F.x,
+                                   scal.p, 0);
+                vcycle<double>(L64, cinv64.p, s);
+                hipLaunchKernelGGL(k_bicg_apply<double>, g, blk, 0, s, F.ni, F.nj, F.ld, F.aP, F.aE, F.aW, F.aN,
+                                   F.aS, act0.p, F.x, sh, t, sv, scal.p, 0, 1, part.p, nb);
+            }
+            hipLaunchKernelGGL(k_bicg_finish, dim3(1), dim3(256), 0, s, part.p, nb, scal.p, 3, 0.0);
+            hipLaunchKernelGGL(k_bicg_x, g, blk, 0, s, F.ni, F.nj, F.ld, act0.p, xsol.p, r, r0, ph, sh, sv, t,
+                               scal.p, part.p, nb);
+            hipLaunchKernelGGL(k_bicg_finish, dim3(1), dim3(256), 0, s, part.p, nb, scal.p, 4, 0.0);
+            CATHODE_LAUNCH_CHECK();
+            const double rr = read_scalar(scal.p + SC_RR, s);
+            rel = sqrt(rr / bb);
+            if (!std::isfinite(rel)) break;
+            if (rel <= tol) {
+                // confirm with the true residual b - A x (the recursion drifts)
+                hipLaunchKernelGGL(k_bicg_restart, g, blk, 0, s, F.ni, F.nj, F.ld, F.aP, F.aE, F.aW, F.aN, F.aS,
+                                   bsol.p, xsol.p, act0.p, r, r0, p, v, part.p, nb);
+                hipLaunchKernelGGL(k_bicg_finish, dim3(1), dim3(256), 0, s, part.p, nb, scal.p, 1, 0.0);
+                const double rt = read_scalar(scal.p + SC_RR, s);
+                rel = sqrt(rt / bb);
+                if (rel <= tol || !std::isfinite(rel)) break;
+                restart = true;  // continue from the true residual
+            }
+        }
+    }
+    if (prof.enabled) {
+        CATHODE_HIP(hipStreamSynchronize(s));
+        for (size_t k = 0; k + 1 < prof.ev.size(); k += 2) {
+            float ms = 0.f;
+            CATHODE_HIP(hipEventElapsedTime(&ms, prof.ev[k], prof.ev[k + 1]));
+            prof.total_ms += ms;
+            prof.launches++;
+        }
+        for (auto& e : prof.ev) (void)hipEventDestroy(e);
+        prof.ev.clear();
+        // one half sweep: the cells of one colour read 5 coefficients + b and
+        // write x; the other colour's x is read once (4 neighbours, cached)
+        const double w = mixed ? 4.0 : 8.0;
+        prof.bytes_per_launch = 0.5 * (double)F.ni * (double)F.nj * 8.0 * w;
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    if (st) {
+        st->iterations = it;
+        st->rel_residual = rel;
+        st->converged = (rel <= tol) ? 1 : 0;
+        st->seconds = std::chrono::duration<double>(t1 - t0).count();
+    }
+    return it;
+}
+
+// ------------------------------------------------------------------ reductions (common.hpp)
+__global__ void k_reduce_partials(const double* part, int nblocks, int nvec, const int* is_max,
+                                  double* out)
+{
+    __shared__ double lds[8];
+    for (int v = 0; v < nvec; v++) {
+        const bool mx = is_max && is_max[v];
+        double s = mx ? -INFINITY : 0.0;
+        for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
+            double p = part[(size_t)b * nvec + v];
+            s = mx ? fmax(s, p) : s + p;
+        }
+        double t = mx ? block_max<256>(s, lds) : block_sum<256>(s, lds);
+        if (threadIdx.x == 0) out[v] = t;
+    }
+}
+
+void reduce_partials(const double* part, int nblocks, int nvec, const int* d_is_max, double* out,
+                     hipStream_t s)
+{
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, s, part, nblocks, nvec, d_is_max, out);
+    CATHODE_LAUNCH_CHECK();
+}
+
+}  // namespace cathode

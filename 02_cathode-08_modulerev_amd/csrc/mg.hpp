@@ -1,0 +1,114 @@
+// mg.hpp -- geometric multigrid for 5-point cell-centred systems on gfx950.
+//
+// Algorithm (the host copy in the CPU baseline runs the same steps):
+//   weights    fine rows scaled by a radial weight w_j before aggregation
+//              (the conservative form of the r-weighted rows; the reference
+//              divides every row by r_j, fluidModule.hpp:2634-2641).  The NDD
+//              path passes w_j = r_j; the generic solve infers
+//              w_{j+1}/w_j = sum_i aN(i,j) / sum_i aS(i,j+1), clipped to [1e-2, 1e2];
+//   hierarchy  2x2 aggregation, coarse couplings = halved sums of the fine
+//              couplings crossing the coarse face; fine row excess
+//              s = aP - sum(a_nb) >= 0 split into a mass part m (minimum
+//              excess over the 5-point neighbourhood, summed) and a boundary
+//              part s - m (halved); 5-point on every level;
+//   smoother   red-black point Gauss-Seidel, V(2,2);
+//   transfers  w-weighted sum restriction, piecewise-constant prolongation;
+//   coarsest   dense inverse (<= 64 unknowns) applied by one workgroup;
+//   outer      BiCGStab in FP64 right-preconditioned by one V-cycle per
+//              application, from x = 0 (solverFuncs.hpp:106-109), stopping
+//              when the true residual ||b - A x||_2 / ||b||_2 <= tol (HYPRE's
+//              relative residual) or after maxITR iterations.
+//   mixed      the V-cycle runs on FP32 copies of every level (scaled by a
+//              power of two so aP ~ 1); residual, update and norms stay FP64.
+#pragma once
+
+#include "common.hpp"
+
+namespace cathode {
+
+constexpr int MG_MAX_LEVELS = 24;
+constexpr int MG_COARSE_MAX = 64;
+// cells (with ghosts, padded rows) of the coarsest level: (64 + 2) x 72
+constexpr int MG_COARSE_CELLS = 66 * 72;
+
+template <class T>
+struct Level {
+    int ni = 0, nj = 0, ld = 0;
+    int cx = 0, cy = 0;  // coarsen to the next level in i / j
+    T *aP = nullptr, *aE = nullptr, *aW = nullptr, *aN = nullptr, *aS = nullptr;
+    T *b = nullptr, *x = nullptr;
+};
+
+struct MGProfile {
+    bool enabled = false;
+    int launches = 0;
+    double total_ms = 0.0;
+    double bytes_per_launch = 0.0;
+    std::vector<hipEvent_t> ev;  // pairs of events, read after the solve
+};
+
+class MGSolver {
+public:
+    MGSolver() = default;
+    ~MGSolver();
+    MGSolver(const MGSolver&) = delete;
+    MGSolver& operator=(const MGSolver&) = delete;
+
+    // Shapes the hierarchy for ni x nj interior cells (no-op if unchanged).
+    void configure(int ni, int nj, hipStream_t s);
+
+    // Level-0 operator, right-hand side, activity mask and solution, in the
+    // solver's (ni+2) x ld0 layout.  Callers fill aP..aS, rhs and act before
+    // solve(); inactive cells must have act = 0.
+    int ld0() const { return L64[0].ld; }
+    double* aP() { return L64[0].aP; }
+    double* aE() { return L64[0].aE; }
+    double* aW() { return L64[0].aW; }
+    double* aN() { return L64[0].aN; }
+    double* aS() { return L64[0].aS; }
+    double* rhs() { return bsol.p; }
+    double* sol() { return xsol.p; }
+    unsigned char* act() { return act0.p; }
+    // Radial weights w[0..nj+1] on the device; nullptr = infer from the matrix.
+    void set_weights(const double* d_w) { user_w = d_w; }
+
+    // Solves A x = rhs from x = 0 into sol().  Returns iterations; fills st.
+    int solve(int maxITR, double tol, int mode, hipStream_t s, cathode_status* st);
+
+    MGProfile prof;  // events around the level-0 smoothing launches
+
+    int levels() const { return nl; }
+
+private:
+    void setup(int mode, hipStream_t s);
+    template <class T>
+    void vcycle(Level<T>* L, const T* cinv, hipStream_t s);
+    double read_scalar(const double* d, hipStream_t s);
+
+    int nl = 0;
+    int cfg_ni = -1, cfg_nj = -1;
+    Level<double> L64[MG_MAX_LEVELS];
+    Level<float> L32[MG_MAX_LEVELS];
+    DevBuf<double> store64;   // every FP64 level array
+    DevBuf<float> store32;    // every FP32 level array
+    DevBuf<double> mass;      // mass part m per level (same layout as the levels)
+    DevBuf<unsigned char> actall;  // activity per level
+    size_t level_off[MG_MAX_LEVELS + 1] = {0};
+    DevBuf<unsigned char> act0;
+    DevBuf<double> bsol, xsol, w0;
+    const double* user_w = nullptr;
+    DevBuf<int> cidx;          // coarsest cell -> dense row (or -1)
+    DevBuf<double> cinv64;     // dense inverse of the coarsest operator
+    DevBuf<float> cinv32;
+    DevBuf<double> part;       // reduction partials (2 slots)
+    DevBuf<double> kv;         // BiCGStab vectors r, r0, p, v, s, t, ph, sh
+    DevBuf<double> scal;       // device scalars: [0] |b|^2, [1] |r|^2, [2] scale, [3] f32 matrix scale
+    double* h_scal = nullptr;  // pinned mirror
+};
+
+// FP64 residual-norm helper used by tests of the library itself
+double rel_residual_dev(const double* aP, const double* aE, const double* aW, const double* aN,
+                        const double* aS, const double* b, const double* x, const unsigned char* act,
+                        int ni, int nj, int ld, hipStream_t s);
+
+}  // namespace cathode

@@ -21,8 +21,9 @@
  *   transfers  restriction = w-weighted sum of the children's residuals,
  *              prolongation = piecewise constant;
  *   coarsest   dense LU (<= 64 unknowns);
- *   outer      defect correction x += V(b - A x) to ||b-Ax||_2/||b||_2 <= tol
- *              or maxITR cycles, from x = 0 (solverFuncs.hpp:106-109).
+ *   outer      BiCGStab right-preconditioned by one V-cycle per application,
+ *              from x = 0 (solverFuncs.hpp:106-109), stopping when the true
+ *              residual ||b-Ax||_2/||b||_2 <= tol or after maxITR iterations.
  */
 #include <math.h>
 #include <stdlib.h>
@@ -310,6 +311,40 @@ static void vcycle(level_t* Ls, int l, int nl, const dense_t* D)
     }
 }
 
+static double dotp(const double* a, const double* b, size_t n)
+{
+    double s = 0.0;
+    for (size_t k = 0; k < n; k++) s += a[k] * b[k];
+    return s;
+}
+
+/* out = A in on the active cells of level 0 (inactive: 0) */
+static void apply_A(level_t* L, const double* in, double* out)
+{
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+    for (int i = 0; i < L->ni + 2; i++)
+        for (int j = 0; j < L->ld; j++) {
+            size_t k = (size_t)i * L->ld + j;
+            if (i < 1 || i > L->ni || j < 1 || j > L->nj || !L->act[k]) {
+                out[k] = 0.0;
+                continue;
+            }
+            out[k] = L->aP[k] * in[k] - L->aE[k] * in[k + L->ld] - L->aW[k] * in[k - L->ld] -
+                     L->aN[k] * in[k + 1] - L->aS[k] * in[k - 1];
+        }
+}
+
+/* out = V-cycle(in) from a zero initial guess */
+static void precond(level_t* Ls, int nl, const dense_t* D, const double* in, double* out)
+{
+    level_t* L0 = &Ls[0];
+    size_t n0 = (size_t)(L0->ni + 2) * L0->ld;
+    for (size_t k = 0; k < n0; k++) L0->b[k] = L0->act[k] ? in[k] : 0.0;
+    memset(L0->x, 0, n0 * sizeof(double));
+    vcycle(Ls, 0, nl, D);
+    memcpy(out, L0->x, n0 * sizeof(double));
+}
+
 int orc_mg_solve(const double* aP, const double* aE, const double* aW, const double* aN,
                  const double* aS, const double* b, const int* boxes, int n_bl, int ni2,
                  int nj2, int ld, int maxITR, double tol, double* x, double* relres)
@@ -351,35 +386,70 @@ int orc_mg_solve(const double* aP, const double* aE, const double* aW, const dou
     double bnorm = sqrt(bb);
     int it = 0;
     double res = 0.0;
-    /* x = 0 on the box union (the reference's zero initial guess) */
-    double* xs = (double*)calloc((size_t)(L0->ni + 2) * L0->ld, sizeof(double));
+    const size_t n0 = (size_t)(L0->ni + 2) * L0->ld;
+    /* BiCGStab vectors; x = 0 on the box union (the reference's zero guess) */
+    double* xs = (double*)calloc(n0, sizeof(double));
+    double* bs = (double*)malloc(n0 * sizeof(double));
+    double* vec = (double*)calloc(8 * n0, sizeof(double));
+    double *r = vec, *r0 = vec + n0, *p = vec + 2 * n0, *v = vec + 3 * n0, *sv = vec + 4 * n0,
+           *t = vec + 5 * n0, *ph = vec + 6 * n0, *sh = vec + 7 * n0;
+    memcpy(bs, L0->b, n0 * sizeof(double));
+    double rho = 1.0, rho1 = 0.0, al = 1.0, om = 1.0;
+    int restart = 1;
     if (bnorm > 0.0) {
-        memcpy(L0->r, L0->b, (size_t)(L0->ni + 2) * L0->ld * sizeof(double));
-        double* bsave = (double*)malloc((size_t)(L0->ni + 2) * L0->ld * sizeof(double));
-        memcpy(bsave, L0->b, (size_t)(L0->ni + 2) * L0->ld * sizeof(double));
         while (it < maxITR) {
+            if (restart) {
+                memcpy(L0->x, xs, n0 * sizeof(double));
+                memcpy(L0->b, bs, n0 * sizeof(double));
+                residual(L0);
+                memcpy(r, L0->r, n0 * sizeof(double));
+                memcpy(r0, r, n0 * sizeof(double));
+                memset(p, 0, n0 * sizeof(double));
+                memset(v, 0, n0 * sizeof(double));
+                rho1 = dotp(r0, r, n0);
+                rho = al = om = 1.0;
+            }
             it++;
-            /* correction equation A e = r solved by one V-cycle from e = 0 */
-            memcpy(L0->b, L0->r, (size_t)(L0->ni + 2) * L0->ld * sizeof(double));
-            memset(L0->x, 0, (size_t)(L0->ni + 2) * L0->ld * sizeof(double));
-            vcycle(Ls, 0, nl, &D);
-            for (size_t k = 0; k < (size_t)(L0->ni + 2) * L0->ld; k++) xs[k] += L0->x[k];
-            memcpy(L0->x, xs, (size_t)(L0->ni + 2) * L0->ld * sizeof(double));
-            memcpy(L0->b, bsave, (size_t)(L0->ni + 2) * L0->ld * sizeof(double));
-            residual(L0);
-            double rr = 0.0;
-#pragma omp parallel for num_threads(g_threads) reduction(+ : rr) schedule(static)
-            for (size_t k = 0; k < (size_t)(L0->ni + 2) * L0->ld; k++) rr += L0->r[k] * L0->r[k];
-            res = sqrt(rr) / bnorm;
-            if (res <= tol) break;
+            if (restart) {
+                memcpy(p, r, n0 * sizeof(double));
+            } else {
+                double beta = (rho1 / rho) * (al / om);
+                for (size_t k = 0; k < n0; k++) p[k] = r[k] + beta * (p[k] - om * v[k]);
+            }
+            restart = 0;
+            precond(Ls, nl, &D, p, ph);
+            apply_A(L0, ph, v);
+            al = rho1 / dotp(r0, v, n0);
+            for (size_t k = 0; k < n0; k++) sv[k] = r[k] - al * v[k];
+            precond(Ls, nl, &D, sv, sh);
+            apply_A(L0, sh, t);
+            double tt = dotp(t, t, n0);
+            om = tt > 0.0 ? dotp(t, sv, n0) / tt : 0.0;
+            for (size_t k = 0; k < n0; k++) {
+                xs[k] += al * ph[k] + om * sh[k];
+                r[k] = sv[k] - om * t[k];
+            }
+            rho = rho1;
+            rho1 = dotp(r0, r, n0);
+            res = sqrt(dotp(r, r, n0)) / bnorm;
+            if (!isfinite(res)) break;
+            if (res <= tol) {
+                memcpy(L0->x, xs, n0 * sizeof(double));
+                memcpy(L0->b, bs, n0 * sizeof(double));
+                residual(L0);
+                res = sqrt(dotp(L0->r, L0->r, n0)) / bnorm;
+                if (res <= tol || !isfinite(res)) break;
+                restart = 1;
+            }
         }
-        free(bsave);
     }
     for (int i = 1; i <= L0->ni; i++)
         for (int j = 1; j <= L0->nj; j++)
             if (L_AT(L0, act, i, j)) x[(size_t)i * ld + j] = xs[(size_t)i * L0->ld + j];
     if (relres) *relres = res;
     free(xs);
+    free(bs);
+    free(vec);
     free(D.idx);
     free(D.lu);
     free(D.piv);

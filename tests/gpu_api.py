@@ -1,0 +1,34 @@
+"""Thin numpy wrappers over the C ABI for the GPU tests (ctypes only)."""
+import ctypes
+
+import numpy as np
+
+from oracle_lib import abi
+
+_I = abi.c_int_p
+
+
+def _boxes(boxes):
+    b = np.ascontiguousarray(np.asarray(boxes, dtype=np.int32).reshape(-1))
+    return b, b.ctypes.data_as(_I), len(b) // 4
+
+
+def struct5(lib, aP, aE, aW, aN, aS, b, boxes, maxITR=30, tol=1e-10, mode=0, x=None):
+    ni2, ld = aP.shape
+    x = np.zeros_like(aP) if x is None else x
+    bk, bp, nb = _boxes(boxes)
+    st = abi.Status()
+    rc = lib.cathode_struct5_solve(*[abi.dptr(np.ascontiguousarray(a)) for a in (aP, aE, aW, aN, aS, b)],
+                                   bp, nb, ni2, ld, ld, maxITR, tol, mode, abi.dptr(x), ctypes.byref(st))
+    abi.check(rc, lib)
+    return x, st
+
+
+def sor(lib, aP, aE, aW, aN, aS, b, boxes, alpha, maxITR, tol, x):
+    ni2, ld = aP.shape
+    bk, bp, nb = _boxes(boxes)
+    st = abi.Status()
+    rc = lib.cathode_sor_solve(*[abi.dptr(np.ascontiguousarray(a)) for a in (aP, aE, aW, aN, aS, b)],
+                               bp, nb, ni2, ld, ld, alpha, maxITR, tol, abi.dptr(x), ctypes.byref(st))
+    abi.check(rc, lib)
+    return x, st
