@@ -1,0 +1,1082 @@
+// ndd.hip -- device-resident implicit NDD time step.
+//
+// One step = ions, Poisson, electrons, electron energy, each a
+// predictor-corrector (PC) loop of up to maxITR_HES_* iterations with one
+// 5-point solve per iteration (FluidModule::solve_rhoi_constTe,
+// solve_phi_couple_wdTe_wSEE_PC, solve_rhoe_wdTe_wSEE_PC,
+// solve_Te_wdTe_wSEE_PC; include/fluidModule.hpp:1405-4957 and the order of
+// src/main.cpp:196-231).  The domain is one rectangular block with the
+// reference's block-4 boundary set: left/right z walls (phi = 0 / V_bias,
+// thermal + secondary-emission wall flux), the axis below (zero face
+// coefficient) and an open upper face (grad phi = 0, electron flux = ion
+// flux).  Ions use the electron Scharfetter-Gummel scheme at charge +1 with the
+// kinetic wall flux of fluidModule.hpp:1502-1526 evaluated at the drift
+// velocity mu_i E of the wall face.
+//
+// Layout: every field is (ni+2) x (nj+2) doubles, row-major [i][j] (j = r
+// contiguous), exactly the host layout, so state transfers are flat copies.
+// The 5-point systems are written straight into the multigrid's level-0
+// arrays (row pitch ld0).
+//
+// The assembly, prediction and correction kernels evaluate every expression
+// in the source order of the restated reference loops and are compiled
+// without multiply-add contraction, so they reproduce a sequential FP64
+// evaluation up to the last-ulp differences of exp/expm1/erf.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "mg.hpp"
+
+namespace cathode {
+
+constexpr double K_BOLTZ = 1.380649e-23;   // include/constants.hpp:5
+constexpr double K_E0 = 1.60217662e-19;    // :6
+constexpr double K_EPS0 = 8.8541878128e-12;  // :7
+constexpr double K_LIM_EXP = 1.0e2;        // fluidModule.hpp:3098
+constexpr double K_EPS_VD = 1.0e-3;        // fluidModule.hpp:3099
+constexpr int NB_X = 64, NB_Y = 4;
+
+enum { SP_ELEC = 0, SP_ION = 1, SP_ENERGY = 2 };
+
+// kernel-side view of the problem (passed by value)
+struct NddView {
+    int ni, nj, ld, ld0;
+    double dx, dr, dt;
+    double V_bias, mu_i, Ti, masse, massi;
+    double coefIISEE_ts, Te_emitSEE_eV, ratioEngy_EISEE_rd, dTe_SEE;
+    int n_see;
+    const double* r;  // cell-centre radii r[0..nj+1]
+    const double *see_eb, *see_rd, *see_ts;
+};
+
+struct Mob {
+    double mu_star, inv_par, inv_cross;
+};
+
+// isotropic mobility evaluated through the tensor formulas of
+// fluidModule.hpp:2314-2328 (B = 0)
+static Mob mobility(double mu)
+{
+    const double mu_para = (mu + mu) / 2.0, mu_perp = (mu + mu) / 2.0;
+    const double mu_rr = (mu + mu) / 2.0, mu_xr = (0.0 + 0.0) / 2.0;
+    Mob m;
+    m.mu_star = mu_perp * 1.0;
+    m.inv_par = mu_rr / (mu_para * mu_perp);
+    m.inv_cross = -mu_xr / (mu_para * mu_perp);
+    return m;
+}
+
+#define IX(i, j) ((size_t)(i) * (size_t)V.ld + (size_t)(j))
+#define IX0(i, j) ((size_t)(i) * (size_t)V.ld0 + (size_t)(j))
+
+// Scharfetter-Gummel weights: bp = vd/(1 - exp(-z)) (upstream density),
+// bm = vd/(exp(z) - 1) (downstream density), with the exponent clamped at
+// lim_exp as in fluidModule.hpp:3239-3243; differences taken with expm1,
+// z == 0 returns the limit D/h.
+__device__ inline double bern_bp(double vd, double z, double Doh)
+{
+    if (z == 0.0) return Doh;
+    return vd / (-expm1(fmin(-z, K_LIM_EXP)));
+}
+__device__ inline double bern_bm(double vd, double z, double Doh)
+{
+    if (z == 0.0) return Doh;
+    return vd / expm1(fmin(z, K_LIM_EXP));
+}
+
+struct Face {
+    double vd, D, z;
+};
+
+// face drift / diffusion / Peclet number between an upstream cell (Ta) and a
+// downstream cell (Tc) at spacing h with face field E (fluidModule.hpp:3595-3606)
+__device__ inline Face sg_face(const NddView& V, int sp, double mu_star, double E, double Ta, double Tc,
+                               double h)
+{
+#pragma clang fp contract(off)
+    Face F;
+    if (sp == SP_ION) {
+        F.vd = mu_star * (E) + K_EPS_VD;
+        F.D = K_BOLTZ * V.Ti / K_E0 * mu_star;
+    } else {
+        const double T_f = (Tc + Ta) / 2.0;
+        const double dT = (Tc - Ta) / h;
+        F.vd = mu_star * (-E - K_BOLTZ / K_E0 * dT) + K_EPS_VD;
+        F.D = K_BOLTZ * T_f / K_E0 * mu_star;
+    }
+    F.z = F.vd * h / F.D;
+    return F;
+}
+
+// func_Te, miniFuncs.hpp:35-46
+__device__ inline double func_Te(const double* v, int n, double Te, double dTe)
+{
+#pragma clang fp contract(off)
+    const int iL = (int)(Te / dTe);
+    const int iR = (int)(Te / dTe) + 1;
+    const double TeL = iL * dTe;
+    if (iR < n - 1) return (v[iR] - v[iL]) / (dTe + 1e-100) * (Te - TeL) + v[iL];
+    return v[n - 1];
+}
+
+__device__ inline double eisee_sum(const NddView& V, double Te)
+{
+#pragma clang fp contract(off)
+    if (V.n_see <= 1) return 0.0;
+    const double eb = func_Te(V.see_eb, V.n_see, Te, V.dTe_SEE);
+    const double rd = func_Te(V.see_rd, V.n_see, Te, V.dTe_SEE);
+    const double ts = func_Te(V.see_ts, V.n_see, Te, V.dTe_SEE);
+    return eb + rd + ts;
+}
+
+// electron thermal wall flux per density (fluidModule.hpp:2350-2352)
+__device__ inline double wall_gpn(const NddView& V, double Te, double dphi, double sign)
+{
+#pragma clang fp contract(off)
+    const double vth = sqrt(8.0 * K_BOLTZ * Te / (M_PI * V.masse));
+    return sign * 0.25 * vth * exp(fmin(-K_E0 * dphi / (K_BOLTZ * Te + 1e-100), K_LIM_EXP));
+}
+
+// ion kinetic wall flux per density (fluidModule.hpp:1502-1526)
+__device__ inline double ion_wall_gpn(const NddView& V, double U, double sign)
+{
+#pragma clang fp contract(off)
+    const double vth = sqrt(8.0 * K_BOLTZ * V.Ti / (M_PI * V.massi));
+    const double um = sqrt(2.0 * K_BOLTZ * V.Ti / V.massi);
+    const double u = U / (um + 1e-100);
+    return sign * 0.25 * vth * (exp(-u * u) + u * sqrt(M_PI) * (erf(u) + sign));
+}
+
+#define THREAD_IJ(i0, j0)                                        \
+    const int j = (j0) + (int)(blockIdx.x * NB_X + threadIdx.x); \
+    const int i = (i0) + (int)(blockIdx.y * NB_Y + threadIdx.y);
+
+// ================================================================== Poisson
+struct PoissonArgs {
+    const double *phi, *rhoe, *rhoi, *Te, *rhoe_old, *Ex, *Er, *Ex_old, *rhoUix, *S;
+    const double *nUex_t, *nUer_t;
+    double *nUex, *nUer;
+    Mob m;
+};
+
+// predicted x flux on faces i = 1..ni+1 (fluidModule.hpp:2266-2462)
+__global__ void k_poisson_predict_x(NddView V, PoissonArgs A)
+{
+#pragma clang fp contract(off)
+    THREAD_IJ(1, 1);
+    if (i > V.ni + 1 || j > V.nj) return;
+    const double dx = V.dx;
+    const double* r = V.r;
+    if (i >= 2 && i <= V.ni) {
+        const double rL = (r[j] + r[j - 1]) / 2.0, rR = (r[j] + r[j + 1]) / 2.0;
+        const double n_f = (A.rhoe[IX(i, j)] + A.rhoe[IX(i - 1, j)]) / 2.0;
+        const double dphidx = (A.phi[IX(i, j)] - A.phi[IX(i - 1, j)]) / dx;
+        const double dpedx =
+            K_BOLTZ * (A.rhoe[IX(i, j)] * A.Te[IX(i, j)] - A.rhoe[IX(i - 1, j)] * A.Te[IX(i - 1, j)]) / dx;
+        const double bLr = (double)(j != 1), bLrD = (double)(j != 1), bRr = 1.0, bRrD = 1.0;
+        const double tr = (rR * A.nUer_t[IX(i, j + 1)] * bRr + rL * A.nUer_t[IX(i, j)] * bLr +
+                           rR * A.nUer_t[IX(i - 1, j + 1)] * bRrD + rL * A.nUer_t[IX(i - 1, j)] * bLrD) /
+                          (4.0 * r[j]);
+        A.nUex[IX(i, j)] = A.nUex_t[IX(i, j)] - (A.m.inv_par * A.nUex_t[IX(i, j)] + A.m.inv_cross * tr) * A.m.mu_star +
+                           A.m.mu_star * (n_f * dphidx - dpedx / K_E0);
+    } else if (i == 1) {
+        const double Te = A.Te[IX(1, j)];
+        const double dphi = fmax(-A.Ex_old[IX(1, j)] * dx / 2.0, 0.0);
+        const double gpn = wall_gpn(V, Te, dphi, -1.0);
+        const double see = -V.coefIISEE_ts * A.rhoUix[IX(1, j)] - eisee_sum(V, Te) * A.rhoe_old[IX(1, j)] * gpn;
+        A.nUex[IX(1, j)] = A.rhoe[IX(1, j)] * gpn + see;
+    } else {
+        const int c = V.ni;
+        const double Te = A.Te[IX(c, j)];
+        const double dphi = fmax(A.Ex_old[IX(c + 1, j)] * dx / 2.0, 0.0);
+        const double gpn = wall_gpn(V, Te, dphi, 1.0);
+        const double see = -V.coefIISEE_ts * A.rhoUix[IX(c + 1, j)] - eisee_sum(V, Te) * A.rhoe_old[IX(c, j)] * gpn;
+        A.nUex[IX(c + 1, j)] = A.rhoe[IX(c, j)] * gpn + see;
+    }
+}
+
+// predicted r flux on faces j = 2..nj+1 (fluidModule.hpp:2466-2621)
+__global__ void k_poisson_predict_r(NddView V, PoissonArgs A)
+{
+#pragma clang fp contract(off)
+    THREAD_IJ(1, 2);
+    if (i > V.ni || j > V.nj + 1) return;
+    const double dr = V.dr;
+    const double* r = V.r;
+    if (j <= V.nj) {
+        const double r_t = (r[j] + r[j - 1]) / 2.0;
+        const double n_f = (A.rhoe[IX(i, j)] + A.rhoe[IX(i, j - 1)]) / 2.0;
+        const double dphidr = (A.phi[IX(i, j)] - A.phi[IX(i, j - 1)]) / dr;
+        const double dpedr =
+            K_BOLTZ * (A.rhoe[IX(i, j)] * A.Te[IX(i, j)] - A.rhoe[IX(i, j - 1)] * A.Te[IX(i, j - 1)]) / dr;
+        const double bLx = (double)(i != 1), bLxD = (double)(i != 1);
+        const double bRx = (double)(i != V.ni), bRxD = (double)(i != V.ni);
+        const double tx = (r[j] * A.nUex_t[IX(i + 1, j)] * bRx + r[j] * A.nUex_t[IX(i, j)] * bLx +
+                           r[j - 1] * A.nUex_t[IX(i + 1, j - 1)] * bRxD + r[j - 1] * A.nUex_t[IX(i, j - 1)] * bLxD) /
+                          (4.0 * r_t);
+        A.nUer[IX(i, j)] = A.nUer_t[IX(i, j)] - (A.m.inv_cross * tx + A.m.inv_par * A.nUer_t[IX(i, j)]) * A.m.mu_star +
+                           A.m.mu_star * (n_f * dphidr - dpedr / K_E0);
+    } else {
+        const double uir = V.mu_i * A.Er[IX(i, V.nj + 1)] + K_EPS_VD;
+        A.nUer[IX(i, V.nj + 1)] = A.rhoi[IX(i, V.nj)] * uir;
+    }
+}
+
+// 5-point Poisson correction system and its BC folds (fluidModule.hpp:2627-2824)
+__global__ void k_poisson_assemble(NddView V, PoissonArgs A, double* aP, double* aE, double* aW, double* aN,
+                                   double* aS, double* b)
+{
+#pragma clang fp contract(off)
+    THREAD_IJ(1, 1);
+    if (i > V.ni || j > V.nj) return;
+    const double dx = V.dx, dr = V.dr, dt = V.dt;
+    const double* r = V.r;
+    const double qR = (r[j] + r[j + 1]) / 2.0 / r[j];
+    const double qL = (r[j] + r[j - 1]) / 2.0 / r[j];
+    const double mu = A.m.mu_star;
+    const double nLx = (A.rhoe[IX(i, j)] + A.rhoe[IX(i - 1, j)]) / 2.0;
+    const double nRx = (A.rhoe[IX(i, j)] + A.rhoe[IX(i + 1, j)]) / 2.0;
+    const double nLr = (A.rhoe[IX(i, j)] + A.rhoe[IX(i, j - 1)]) / 2.0;
+    const double nRr = (A.rhoe[IX(i, j)] + A.rhoe[IX(i, j + 1)]) / 2.0;
+    const double bLx = (double)(i != 1), bRx = (double)(i != V.ni);
+    const double bLr = (double)(j != 1), bRr = (double)(j != V.nj);
+    const double eRx = K_EPS0 + K_E0 * dt * nRx * mu * bRx;
+    const double eLx = K_EPS0 + K_E0 * dt * nLx * mu * bLx;
+    const double eRr = K_EPS0 + K_E0 * dt * nRr * mu * bRr;
+    const double eLr = K_EPS0 + K_E0 * dt * nLr * mu * bLr;
+    double vP = (eRx + eLx) * dr * dr + (qR * eRr + qL * eLr) * dx * dx;
+    double vE = eRx * dr * dr;
+    double vW = eLx * dr * dr;
+    double vN = qR * eRr * dx * dx;
+    const double vS = qL * eLr * dx * dx;
+    const double div = (A.nUex[IX(i + 1, j)] - A.nUex[IX(i, j)]) / dx + (qR * A.nUer[IX(i, j + 1)] - qL * A.nUer[IX(i, j)]) / dr;
+    const double lap = -(A.Ex[IX(i + 1, j)] - A.Ex[IX(i, j)]) / dx - (qR * A.Er[IX(i, j + 1)] - qL * A.Er[IX(i, j)]) / dr;
+    const double S = A.S ? A.S[IX(i, j)] : 0.0;
+    const double vb = (K_E0 * A.rhoi[IX(i, j)] - K_E0 * A.rhoe[IX(i, j)] - K_E0 * dt * S + K_EPS0 * lap + K_E0 * dt * div) *
+                      dx * dx * dr * dr;
+    if (i == 1) {
+        vP = vP + vW;
+        vW = 0.0;
+    }
+    if (i == V.ni) {
+        vP = vP + vE;
+        vE = 0.0;
+    }
+    if (j == V.nj) {
+        vP = vP - vN;
+        vN = 0.0;
+    }
+    const size_t k = IX0(i, j);
+    aP[k] = vP;
+    aE[k] = vE;
+    aW[k] = vW;
+    aN[k] = vN;
+    aS[k] = vS;
+    b[k] = vb;
+}
+
+// phi ghosts before the PC loop (fluidModule.hpp:2160-2253): thread per edge cell
+__global__ void k_poisson_ghost(NddView V, double* phi)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < V.nj) {
+        const int j = t + 1;
+        phi[IX(0, j)] = -phi[IX(1, j)];
+        phi[IX(V.ni + 1, j)] = 2.0 * V.V_bias - phi[IX(V.ni, j)];
+    }
+    if (t < V.ni) {
+        const int i = t + 1;
+        phi[IX(i, V.nj + 1)] = phi[IX(i, V.nj)];
+    }
+}
+
+// dphi with its ghost reflections, phi += dphi on every cell, wall-face
+// field update (fluidModule.hpp:2850-2958)
+__global__ void k_poisson_update_phi(NddView V, const double* __restrict__ sol, double* __restrict__ dphi,
+                                     double* __restrict__ phi, double* __restrict__ Ex, double* __restrict__ Er)
+{
+#pragma clang fp contract(off)
+    THREAD_IJ(0, 0);
+    if (i > V.ni + 1 || j > V.nj + 1) return;
+    const int ni = V.ni, nj = V.nj;
+    double d = 0.0;
+    const bool in_i = i >= 1 && i <= ni, in_j = j >= 1 && j <= nj;
+    if (in_i && in_j)
+        d = sol[IX0(i, j)];
+    else if (i == 0 && in_j)
+        d = -sol[IX0(1, j)];
+    else if (i == ni + 1 && in_j)
+        d = -sol[IX0(ni, j)];
+    else if (j == nj + 1 && in_i)
+        d = sol[IX0(i, nj)];
+    dphi[IX(i, j)] = d;
+    phi[IX(i, j)] = phi[IX(i, j)] + d;
+    if (in_j && i == 1) {
+        const double d1 = sol[IX0(1, j)];
+        Ex[IX(1, j)] = Ex[IX(1, j)] - (d1 - (-d1)) / V.dx;
+    }
+    if (in_j && i == ni) {
+        const double dn = sol[IX0(ni, j)];
+        Ex[IX(ni + 1, j)] = Ex[IX(ni + 1, j)] - ((-dn) - dn) / V.dx;
+    }
+    if (in_i && j == nj) Er[IX(i, nj + 1)] = 0.0;
+}
+
+// interior fields from phi and the flux correction (fluidModule.hpp:2959-3033)
+__global__ void k_poisson_fields(NddView V, const double* __restrict__ phi, const double* __restrict__ dphi,
+                                 const double* __restrict__ rhoe, double* __restrict__ Ex, double* __restrict__ Er,
+                                 double* __restrict__ nUex, double* __restrict__ nUer, Mob m)
+{
+#pragma clang fp contract(off)
+    THREAD_IJ(1, 1);
+    if (i > V.ni || j > V.nj) return;
+    if (i >= 2) {
+        Ex[IX(i, j)] = -(phi[IX(i, j)] - phi[IX(i - 1, j)]) / V.dx;
+        const double n_f = (rhoe[IX(i, j)] + rhoe[IX(i - 1, j)]) / 2.0;
+        nUex[IX(i, j)] = nUex[IX(i, j)] + m.mu_star * n_f * ((dphi[IX(i, j)] - dphi[IX(i - 1, j)]) / V.dx);
+    }
+    if (j >= 2) {
+        Er[IX(i, j)] = -(phi[IX(i, j)] - phi[IX(i, j - 1)]) / V.dr;
+        const double n_f = (rhoe[IX(i, j)] + rhoe[IX(i, j - 1)]) / 2.0;
+        nUer[IX(i, j)] = nUer[IX(i, j)] + m.mu_star * n_f * ((dphi[IX(i, j)] - dphi[IX(i, j - 1)]) / V.dr);
+    }
+}
+
+// ================================================================== SG equations
+struct SgArgs {
+    int sp;
+    Mob m;
+    double tfac;
+    const double *n_old, *T;  // start-of-routine density, temperature of the SG faces
+    const double *Ex, *Er, *Te, *rhoi, *rhoe, *rhoe_old, *rhoUix, *rhoUex, *rhoUer;
+    const double *S, *Pabs, *eloss;
+    const double *Fx_t, *Fr_t;
+    double *n, *Fx, *Fr;
+};
+
+__global__ void k_sg_predict_x(NddView V, SgArgs A)
+{
+#pragma clang fp contract(off)
+    THREAD_IJ(1, 1);
+    if (i > V.ni + 1 || j > V.nj) return;
+    const double dx = V.dx;
+    const double* r = V.r;
+    const double* T = A.T;
+    if (i >= 2 && i <= V.ni) {
+        const double rL = (r[j] + r[j - 1]) / 2.0, rR = (r[j] + r[j + 1]) / 2.0;
+        const Face F = sg_face(V, A.sp, A.m.mu_star, A.Ex[IX(i, j)], T[IX(i - 1, j)], T[IX(i, j)], dx);
+        const double bLr = (double)(j != 1), bLrD = (double)(j != 1), bRr = 1.0, bRrD = 1.0;
+        const double tr = (rR * A.Fr_t[IX(i, j + 1)] * bRr + rL * A.Fr_t[IX(i, j)] * bLr +
+                           rR * A.Fr_t[IX(i - 1, j + 1)] * bRrD + rL * A.Fr_t[IX(i - 1, j)] * bLrD) /
+                          (4.0 * r[j]);
+        double sg = bern_bp(F.vd, F.z, F.D / dx) * A.n[IX(i - 1, j)] - bern_bm(F.vd, F.z, F.D / dx) * A.n[IX(i, j)];
+        if (A.sp == SP_ENERGY) sg = sg * 5.0 / 3.0;
+        A.Fx[IX(i, j)] = sg + A.Fx_t[IX(i, j)] - A.m.mu_star * (A.m.inv_par * A.Fx_t[IX(i, j)] + A.m.inv_cross * tr);
+        return;
+    }
+    const int side = i == 1 ? 0 : 1;
+    const int c = side ? V.ni : 1;  // wall cell
+    const int fi = side ? V.ni + 1 : 1;
+    const double sgn = side ? 1.0 : -1.0;
+    if (A.sp == SP_ION) {
+        const double U = V.mu_i * A.Ex[IX(fi, j)];
+        A.Fx[IX(fi, j)] = A.n[IX(c, j)] * ion_wall_gpn(V, U, sgn);
+    } else if (A.sp == SP_ELEC) {
+        const double Te = A.Te[IX(c, j)];
+        const double dphi = fmax(-sgn * A.Ex[IX(fi, j)] * dx / 2.0, 0.0);
+        const double gpn = wall_gpn(V, Te, dphi, sgn);
+        const double see = -V.coefIISEE_ts * A.rhoUix[IX(fi, j)] - eisee_sum(V, Te) * A.rhoe_old[IX(c, j)] * gpn;
+        A.Fx[IX(fi, j)] = A.n[IX(c, j)] * gpn + see;
+    } else {
+        const double Teo = T[IX(c, j)];
+        const double dphi = fmax(-sgn * A.Ex[IX(fi, j)] * dx / 2.0, 0.0);
+        const double vth = sqrt(8.0 * K_BOLTZ * Teo / (M_PI * V.masse));
+        const double ex = exp(fmin(-K_E0 * dphi / (K_BOLTZ * Teo + 1e-100), K_LIM_EXP));
+        const double gpne = sgn * vth * ex / 3.0 * 1.0;
+        const double eps_emit = 2.0 * K_E0 * V.Te_emitSEE_eV;
+        const double gpn = sgn * 0.25 * 1.0 * vth * ex;
+        const double Te = A.Te[IX(c, j)];
+        double eb = 0.0, rd = 0.0, ts = 0.0;
+        if (V.n_see > 1) {
+            eb = func_Te(V.see_eb, V.n_see, Te, V.dTe_SEE);
+            rd = func_Te(V.see_rd, V.n_see, Te, V.dTe_SEE);
+            ts = func_Te(V.see_ts, V.n_see, Te, V.dTe_SEE);
+        }
+        const double see = -eps_emit * V.coefIISEE_ts * A.rhoUix[IX(fi, j)] -
+                           (eb + V.ratioEngy_EISEE_rd * rd) * A.n_old[IX(c, j)] * gpne -
+                           eps_emit * ts * A.rhoe_old[IX(c, j)] * gpn;
+        A.Fx[IX(fi, j)] = A.n[IX(c, j)] * gpne + see;
+    }
+}
+
+__global__ void k_sg_predict_r(NddView V, SgArgs A)
+{
+#pragma clang fp contract(off)
+    THREAD_IJ(1, 2);
+    if (i > V.ni || j > V.nj + 1) return;
+    const double dr = V.dr;
+    const double* r = V.r;
+    const double* T = A.T;
+    if (j <= V.nj) {
+        const double r_t = (r[j] + r[j - 1]) / 2.0;
+        const Face F = sg_face(V, A.sp, A.m.mu_star, A.Er[IX(i, j)], T[IX(i, j - 1)], T[IX(i, j)], dr);
+        const double bLx = (double)(i != 1), bLxD = (double)(i != 1);
+        const double bRx = (double)(i != V.ni), bRxD = (double)(i != V.ni);
+        const double tx = (r[j] * A.Fx_t[IX(i + 1, j)] * bRx + r[j] * A.Fx_t[IX(i, j)] * bLx +
+                           r[j - 1] * A.Fx_t[IX(i + 1, j - 1)] * bRxD + r[j - 1] * A.Fx_t[IX(i, j - 1)] * bLxD) /
+                          (4.0 * r_t);
+        double sg = bern_bp(F.vd, F.z, F.D / dr) * A.n[IX(i, j - 1)] - bern_bm(F.vd, F.z, F.D / dr) * A.n[IX(i, j)];
+        if (A.sp == SP_ENERGY) sg = sg * 5.0 / 3.0;
+        A.Fr[IX(i, j)] = sg + A.Fr_t[IX(i, j)] - A.m.mu_star * (A.m.inv_cross * tx + A.m.inv_par * A.Fr_t[IX(i, j)]);
+    } else if (A.sp == SP_ENERGY) {
+        A.Fr[IX(i, V.nj + 1)] = 5.0 / 2.0 * K_BOLTZ * T[IX(i, V.nj)] * fmax(A.rhoUer[IX(i, V.nj + 1)], 0.0);
+    } else {
+        const double uir = V.mu_i * A.Er[IX(i, V.nj + 1)] + K_EPS_VD;
+        A.Fr[IX(i, V.nj + 1)] = A.rhoi[IX(i, V.nj)] * uir;
+    }
+}
+
+__global__ void k_sg_assemble(NddView V, SgArgs A, double* aP, double* aE, double* aW, double* aN, double* aS,
+                              double* b)
+{
+#pragma clang fp contract(off)
+    THREAD_IJ(1, 1);
+    if (i > V.ni || j > V.nj) return;
+    const double dx = V.dx, dr = V.dr, dt = V.dt;
+    const double* r = V.r;
+    const double* T = A.T;
+    const double rR = (r[j] + r[j + 1]) / 2.0, rL = (r[j] + r[j - 1]) / 2.0;
+    const double qR = (r[j] + r[j + 1]) / 2.0 / r[j];
+    const double qL = (r[j] + r[j - 1]) / 2.0 / r[j];
+    const double mu = A.m.mu_star;
+    const Face Lx = sg_face(V, A.sp, mu, A.Ex[IX(i, j)], T[IX(i - 1, j)], T[IX(i, j)], dx);
+    const Face Rx = sg_face(V, A.sp, mu, A.Ex[IX(i + 1, j)], T[IX(i, j)], T[IX(i + 1, j)], dx);
+    const Face Lr = sg_face(V, A.sp, mu, A.Er[IX(i, j)], T[IX(i, j - 1)], T[IX(i, j)], dr);
+    const Face Rr = sg_face(V, A.sp, mu, A.Er[IX(i, j + 1)], T[IX(i, j)], T[IX(i, j + 1)], dr);
+    const double bLx = (double)(i != 1), bRx = (double)(i != V.ni);
+    const double bLr = (double)(j != 1), bRr = (double)(j != V.nj);
+    const double vP = bern_bp(Rx.vd, Rx.z, Rx.D / dx) * bRx * dr + bern_bm(Lx.vd, Lx.z, Lx.D / dx) * bLx * dr +
+                      qR * bern_bp(Rr.vd, Rr.z, Rr.D / dr) * bRr * dx + qL * bern_bm(Lr.vd, Lr.z, Lr.D / dr) * bLr * dx;
+    const size_t k = IX0(i, j);
+    aP[k] = vP + A.tfac * dx * dr / dt;
+    aE[k] = bern_bm(Rx.vd, Rx.z, Rx.D / dx) * bRx * dr;
+    aW[k] = bern_bp(Lx.vd, Lx.z, Lx.D / dx) * bLx * dr;
+    aN[k] = qR * bern_bm(Rr.vd, Rr.z, Rr.D / dr) * bRr * dx;
+    aS[k] = qL * bern_bp(Lr.vd, Lr.z, Lr.D / dr) * bLr * dx;
+    const double div = (A.Fx[IX(i + 1, j)] - A.Fx[IX(i, j)]) / dx + (qR * A.Fr[IX(i, j + 1)] - qL * A.Fr[IX(i, j)]) / dr;
+    if (A.sp == SP_ENERGY) {
+        const double bLxw = (double)(i != 1), bRxw = (double)(i != V.ni);
+        const double JEx = -K_E0 * (A.rhoUex[IX(i + 1, j)] * A.Ex[IX(i + 1, j)] * bRxw + A.rhoUex[IX(i, j)] * A.Ex[IX(i, j)] * bLxw) / 2.0;
+        const double JEr = -K_E0 * (rR * A.rhoUer[IX(i, j + 1)] * A.Er[IX(i, j + 1)] * 1.0 + rL * A.rhoUer[IX(i, j)] * A.Er[IX(i, j)] * 1.0) /
+                           (2.0 * r[j]);
+        const double R1 = (-A.n[IX(i, j)] + A.n_old[IX(i, j)]) / dt;
+        const double R2 = JEx + JEr;
+        const double Pabs = A.Pabs ? A.Pabs[IX(i, j)] : 0.0;
+        const double eloss = A.eloss ? A.eloss[IX(i, j)] : 0.0;
+        const double R3 = Pabs - A.rhoe[IX(i, j)] * eloss;
+        const double R4 = -div;
+        b[k] = 3.0 / 5.0 * (R1 + R2 + R3 + R4) * dx * dr;
+    } else {
+        const double S = A.S ? A.S[IX(i, j)] : 0.0;
+        b[k] = (-div + S - (A.n[IX(i, j)] - A.n_old[IX(i, j)]) / dt) * dx * dr;
+    }
+}
+
+// flux correction on interior faces and the positivity-guarded density
+// update (fluidModule.hpp:3682-3757); thread (i, j) owns x-face (i, j),
+// r-face (i, j) and cell (i, j)
+__global__ void k_sg_correct(NddView V, SgArgs A, const double* __restrict__ dn)
+{
+#pragma clang fp contract(off)
+    THREAD_IJ(1, 1);
+    if (i > V.ni || j > V.nj) return;
+    const double dx = V.dx, dr = V.dr;
+    const double* T = A.T;
+    const double mu = A.m.mu_star;
+    if (i >= 2) {
+        const Face F = sg_face(V, A.sp, mu, A.Ex[IX(i, j)], T[IX(i - 1, j)], T[IX(i, j)], dx);
+        double d = bern_bp(F.vd, F.z, F.D / dx) * dn[IX0(i - 1, j)] - bern_bm(F.vd, F.z, F.D / dx) * dn[IX0(i, j)];
+        if (A.sp == SP_ENERGY) d = d * 5.0 / 3.0;
+        A.Fx[IX(i, j)] = A.Fx[IX(i, j)] + d;
+    }
+    if (j >= 2) {
+        const Face F = sg_face(V, A.sp, mu, A.Er[IX(i, j)], T[IX(i, j - 1)], T[IX(i, j)], dr);
+        double d = bern_bp(F.vd, F.z, F.D / dr) * dn[IX0(i, j - 1)] - bern_bm(F.vd, F.z, F.D / dr) * dn[IX0(i, j)];
+        if (A.sp == SP_ENERGY) d = d * 5.0 / 3.0;
+        A.Fr[IX(i, j)] = A.Fr[IX(i, j)] + d;
+    }
+    const double nn = A.n[IX(i, j)] + dn[IX0(i, j)];
+    if (nn > 0.0) A.n[IX(i, j)] = nn;
+}
+
+// ================================================================== reductions
+// calcRes (miniFuncs.hpp:292-375) of three (new, old) pairs over the cell
+// box [1..ni]x[1..nj], the x-face box [2..ni]x[1..nj] and the r-face box
+// [1..ni]x[2..nj]; six partial values per block.
+struct ResPairs {
+    const double *n0, *o0, *n1, *o1, *n2, *o2;
+};
+
+__global__ void k_calcres(NddView V, ResPairs P, int mode, double eps, double* __restrict__ part, int nb)
+{
+    __shared__ double lds[8];
+    THREAD_IJ(1, 1);
+    double a[3] = {0.0, 0.0, 0.0}, c[3] = {0.0, 0.0, 0.0};
+    if (i <= V.ni && j <= V.nj) {
+        const double* nw[3] = {P.n0, P.n1, P.n2};
+        const double* od[3] = {P.o0, P.o1, P.o2};
+        const bool in[3] = {true, i >= 2, j >= 2};
+        for (int q = 0; q < 3; q++) {
+            if (!in[q]) continue;
+            const double vn = nw[q][IX(i, j)], vo = od[q][IX(i, j)];
+            const double d = vn - vo;
+            if (mode == 0) {
+                a[q] = fabs(d);
+            } else if (mode == 1) {
+                a[q] = (d * d) / (vo * vo + eps);
+                c[q] = 1.0;
+            } else if (mode == 2) {
+                a[q] = fabs(d);
+                c[q] = fabs(vo);
+            } else {
+                a[q] = d * d;
+                c[q] = vo * vo;
+            }
+        }
+    }
+    const bool mx_a = mode == 0 || mode == 2, mx_c = mode == 2;
+    const int bid = blockIdx.y * gridDim.x + blockIdx.x;
+    for (int q = 0; q < 3; q++) {
+        const double ta = mx_a ? block_max<NB_X * NB_Y>(a[q], lds) : block_sum<NB_X * NB_Y>(a[q], lds);
+        const double tc = mx_c ? block_max<NB_X * NB_Y>(c[q], lds) : block_sum<NB_X * NB_Y>(c[q], lds);
+        if (lin_tid() == 0) {
+            part[(2 * q) * nb + bid] = ta;
+            part[(2 * q + 1) * nb + bid] = tc;
+        }
+    }
+}
+
+// errors e[0..2] and their max e[3]
+__global__ void k_calcres_finish(const double* __restrict__ part, int nb, int mode, double* __restrict__ e)
+{
+    __shared__ double lds[8];
+    const bool mx_a = mode == 0 || mode == 2, mx_c = mode == 2;
+    double err[3];
+    for (int q = 0; q < 3; q++) {
+        double sa = mx_a ? 0.0 : 0.0, sc = 0.0;
+        for (int k = threadIdx.x; k < nb; k += blockDim.x) {
+            const double va = part[(2 * q) * nb + k], vc = part[(2 * q + 1) * nb + k];
+            sa = mx_a ? fmax(sa, va) : sa + va;
+            sc = mx_c ? fmax(sc, vc) : sc + vc;
+        }
+        sa = mx_a ? block_max<256>(sa, lds) : block_sum<256>(sa, lds);
+        sc = mx_c ? block_max<256>(sc, lds) : block_sum<256>(sc, lds);
+        if (mode == 0)
+            err[q] = sa;
+        else if (mode == 1)
+            err[q] = sqrt(sa / sc);
+        else if (mode == 2)
+            err[q] = sa / sc;
+        else
+            err[q] = sqrt(sa / sc);
+    }
+    if (threadIdx.x == 0) {
+        e[0] = err[0];
+        e[1] = err[1];
+        e[2] = err[2];
+        e[3] = fmax(fmax(err[0], err[1]), err[2]);
+    }
+}
+
+// energy post-processing (fluidModule.hpp:4868-4905)
+__global__ void k_energy_guard_max(NddView V, double* __restrict__ rhoeps, const double* __restrict__ rhoeps_old,
+                                   const double* __restrict__ rhoe, double* __restrict__ part)
+{
+    __shared__ double lds[8];
+    THREAD_IJ(1, 1);
+    double v = 0.0;
+    if (i <= V.ni && j <= V.nj) {
+        if (rhoeps[IX(i, j)] < 0.0) rhoeps[IX(i, j)] = fmax(rhoeps_old[IX(i, j)], 0.0);
+        v = rhoe[IX(i, j)];
+    }
+    const double m = block_max<NB_X * NB_Y>(v, lds);
+    if (lin_tid() == 0) part[blockIdx.y * gridDim.x + blockIdx.x] = m;
+}
+
+__global__ void k_max_finish(const double* __restrict__ part, int nb, double* __restrict__ out)
+{
+    __shared__ double lds[8];
+    double m = 0.0;
+    for (int k = threadIdx.x; k < nb; k += blockDim.x) m = fmax(m, part[k]);
+    m = block_max<256>(m, lds);
+    if (threadIdx.x == 0) *out = m;
+}
+
+__global__ void k_te_update(NddView V, const double* __restrict__ rhoeps, const double* __restrict__ rhoe,
+                            const double* __restrict__ nmax, double* __restrict__ Te)
+{
+#pragma clang fp contract(off)
+    THREAD_IJ(1, 1);
+    if (i > V.ni || j > V.nj) return;
+    const double Te_max = 1000.0 * K_E0 / K_BOLTZ, ratio = 0.001;
+    Te[IX(i, j)] = fmin(rhoeps[IX(i, j)] / (3.0 / 2.0 * (rhoe[IX(i, j)] + *nmax * ratio) * K_BOLTZ + 1e-100), Te_max);
+}
+
+// zero-gradient ghosts on the four sides (corners untouched)
+__global__ void k_ghost_copy4(NddView V, double* __restrict__ a)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < V.nj) {
+        const int j = t + 1;
+        a[IX(0, j)] = a[IX(1, j)];
+        a[IX(V.ni + 1, j)] = a[IX(V.ni, j)];
+    }
+    if (t < V.ni) {
+        const int i = t + 1;
+        a[IX(i, 0)] = a[IX(i, 1)];
+        a[IX(i, V.nj + 1)] = a[IX(i, V.nj)];
+    }
+}
+
+__global__ void k_top_copy(NddView V, double* __restrict__ a)
+{
+    const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i <= V.ni) a[IX(i, V.nj + 1)] = a[IX(i, V.nj)];
+}
+
+// right-electrode current (fluidModule.hpp:3942-3965): out = {Ii, Ie}
+__global__ void k_anode(NddView V, const double* __restrict__ rhoUix, const double* __restrict__ rhoUex,
+                        double* __restrict__ out)
+{
+#pragma clang fp contract(off)
+    __shared__ double lds[8];
+    double si = 0.0, se = 0.0;
+    for (int j = 1 + threadIdx.x; j <= V.nj; j += blockDim.x) {
+        si += rhoUix[IX(V.ni + 1, j)] * 2.0 * M_PI * V.r[j] * V.dr * (K_E0);
+        se += rhoUex[IX(V.ni + 1, j)] * 2.0 * M_PI * V.r[j] * V.dr * (-K_E0);
+    }
+    const double ti = block_sum<256>(si, lds);
+    const double te = block_sum<256>(se, lds);
+    if (threadIdx.x == 0) {
+        out[0] = ti;
+        out[1] = te;
+    }
+}
+
+__global__ void k_fill_one(int n, unsigned char* a, int ni, int nj, int ld)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int i = k / ld, j = k % ld;
+    a[k] = (i >= 1 && i <= ni && j >= 1 && j <= nj) ? 1 : 0;
+}
+
+// ================================================================== host side
+struct NddHandle {
+    cathode_ndd_params p{};
+    NddView V{};
+    Mob me{}, mi{};
+    hipStream_t s = nullptr;
+    size_t n = 0;
+    DevBuf<double> fields;  // 19 state/source arrays
+    DevBuf<double> work;    // scratch arrays
+    DevBuf<double> radii, see;
+    DevBuf<double> part, scal;
+    double* h_scal = nullptr;
+    double* F[19] = {nullptr};
+    bool has_src[3] = {false, false, false};
+    // work arrays
+    double *phi_old, *Ex_old, *Er_old, *tmp_c, *tmp_x, *tmp_r, *dphi, *s_old, *sx_old, *sr_old, *Te_old, *rhoi_old;
+    MGSolver mg;
+    cathode_ndd_report rep{};
+    int nb = 0;
+};
+
+enum {
+    F_PHI, F_RHOE, F_RHOI, F_TE, F_RHOEPS, F_RHOE_OLD, F_EX, F_RHOUEX, F_NUEX, F_GX, F_RHOUIX,
+    F_ER, F_RHOUER, F_NUER, F_GR, F_RHOUIR, F_S, F_PABS, F_ELOSS
+};
+
+static double** field_slot(cathode_ndd_fields* f, int k)
+{
+    double** a[19] = {&f->phi, &f->rhoe, &f->rhoi, &f->Te, &f->rhoeps, &f->rhoe_old, &f->Ex, &f->rhoUex, &f->nUex,
+                      &f->Gx, &f->rhoUix, &f->Er, &f->rhoUer, &f->nUer, &f->Gr, &f->rhoUir, &f->rate_ionize,
+                      &f->Pabs, &f->rate_eloss};
+    return a[k];
+}
+
+static inline dim3 gblk() { return dim3(NB_X, NB_Y); }
+static inline dim3 ggrid(int ni, int nj) { return dim3(cdiv(nj, NB_X), cdiv(ni, NB_Y)); }
+
+static void d2d(NddHandle& H, double* dst, const double* src)
+{
+    CATHODE_HIP(hipMemcpyAsync(dst, src, H.n * sizeof(double), hipMemcpyDeviceToDevice, H.s));
+}
+
+static double read_dev(NddHandle& H, const double* d, int count = 1)
+{
+    CATHODE_HIP(hipMemcpyAsync(H.h_scal, d, count * sizeof(double), hipMemcpyDeviceToHost, H.s));
+    CATHODE_HIP(hipStreamSynchronize(H.s));
+    return H.h_scal[0];
+}
+
+static void account(NddHandle& H, const cathode_status& st)
+{
+    H.rep.solves++;
+    H.rep.total_iterations += st.iterations;
+    if (st.iterations > H.rep.max_iterations) H.rep.max_iterations = st.iterations;
+    if (st.rel_residual > H.rep.max_rel_residual || std::isnan(st.rel_residual)) H.rep.max_rel_residual = st.rel_residual;
+    if (!st.converged) H.rep.nonconverged++;
+}
+
+static void solve(NddHandle& H)
+{
+    cathode_status st{};
+    H.mg.solve(H.p.maxITR, H.p.tol, H.p.mode, H.s, &st);
+    account(H, st);
+}
+
+// calcRes of three pairs; returns the max error (host sync); errors also in scal[0..3]
+static double calcres3(NddHandle& H, const ResPairs& P, double* err_out)
+{
+    const NddView& V = H.V;
+    hipLaunchKernelGGL(k_calcres, ggrid(V.ni, V.nj), gblk(), 0, H.s, V, P, H.p.flag_error, 1e-100, H.part.p, H.nb);
+    hipLaunchKernelGGL(k_calcres_finish, dim3(1), dim3(256), 0, H.s, H.part.p, H.nb, H.p.flag_error, H.scal.p);
+    CATHODE_LAUNCH_CHECK();
+    CATHODE_HIP(hipMemcpyAsync(H.h_scal, H.scal.p, 4 * sizeof(double), hipMemcpyDeviceToHost, H.s));
+    CATHODE_HIP(hipStreamSynchronize(H.s));
+    if (err_out)
+        for (int q = 0; q < 3; q++) err_out[q] = H.h_scal[q];
+    return H.h_scal[3];
+}
+
+static void poisson_routine(NddHandle& H, bool report)
+{
+    const NddView& V = H.V;
+    double** F = H.F;
+    d2d(H, H.phi_old, F[F_PHI]);
+    d2d(H, H.Ex_old, F[F_EX]);
+    d2d(H, H.Er_old, F[F_ER]);
+    d2d(H, H.sx_old, F[F_NUEX]);
+    d2d(H, H.sr_old, F[F_NUER]);
+    hipLaunchKernelGGL(k_poisson_ghost, dim3(cdiv(std::max(V.ni, V.nj), 256)), dim3(256), 0, H.s, V, F[F_PHI]);
+    PoissonArgs A{};
+    A.phi = F[F_PHI]; A.rhoe = F[F_RHOE]; A.rhoi = F[F_RHOI]; A.Te = F[F_TE]; A.rhoe_old = F[F_RHOE_OLD];
+    A.Ex = F[F_EX]; A.Er = F[F_ER]; A.Ex_old = H.Ex_old; A.rhoUix = F[F_RHOUIX];
+    A.S = H.has_src[0] ? F[F_S] : nullptr;
+    A.nUex_t = H.tmp_x; A.nUer_t = H.tmp_r; A.nUex = F[F_NUEX]; A.nUer = F[F_NUER];
+    A.m = H.me;
+    int ncount = 0;
+    for (;;) {
+        ncount++;
+        d2d(H, H.tmp_c, F[F_PHI]);
+        d2d(H, H.tmp_x, F[F_NUEX]);
+        d2d(H, H.tmp_r, F[F_NUER]);
+        hipLaunchKernelGGL(k_poisson_predict_x, ggrid(V.ni + 1, V.nj), gblk(), 0, H.s, V, A);
+        hipLaunchKernelGGL(k_poisson_predict_r, ggrid(V.ni, V.nj), gblk(), 0, H.s, V, A);
+        hipLaunchKernelGGL(k_poisson_assemble, ggrid(V.ni, V.nj), gblk(), 0, H.s, V, A, H.mg.aP(), H.mg.aE(),
+                           H.mg.aW(), H.mg.aN(), H.mg.aS(), H.mg.rhs());
+        CATHODE_LAUNCH_CHECK();
+        solve(H);
+        hipLaunchKernelGGL(k_poisson_update_phi, ggrid(V.ni + 2, V.nj + 2), gblk(), 0, H.s, V, H.mg.sol(), H.dphi,
+                           F[F_PHI], F[F_EX], F[F_ER]);
+        hipLaunchKernelGGL(k_poisson_fields, ggrid(V.ni, V.nj), gblk(), 0, H.s, V, F[F_PHI], H.dphi, F[F_RHOE],
+                           F[F_EX], F[F_ER], F[F_NUEX], F[F_NUER], H.me);
+        CATHODE_LAUNCH_CHECK();
+        if (ncount >= H.p.maxITR_HES_phi) break;
+        ResPairs P{F[F_PHI], H.tmp_c, F[F_NUEX], H.tmp_x, F[F_NUER], H.tmp_r};
+        if (calcres3(H, P, nullptr) < H.p.error_cnv_HES_phi) break;
+    }
+    if (report) {
+        double e[3];
+        ResPairs P{F[F_PHI], H.phi_old, F[F_NUEX], H.sx_old, F[F_NUER], H.sr_old};
+        calcres3(H, P, e);
+        H.rep.error_phi = e[0];
+        H.rep.error_nUex = e[1];
+        H.rep.error_nUer = e[2];
+    }
+}
+
+static void sg_routine(NddHandle& H, int sp, bool report, double* err3)
+{
+    const NddView& V = H.V;
+    double** F = H.F;
+    SgArgs A{};
+    A.sp = sp;
+    A.Ex = F[F_EX]; A.Er = F[F_ER]; A.Te = F[F_TE]; A.rhoi = F[F_RHOI]; A.rhoe = F[F_RHOE];
+    A.rhoe_old = F[F_RHOE_OLD]; A.rhoUix = F[F_RHOUIX]; A.rhoUex = F[F_RHOUEX]; A.rhoUer = F[F_RHOUER];
+    A.S = H.has_src[0] ? F[F_S] : nullptr;
+    A.Pabs = H.has_src[1] ? F[F_PABS] : nullptr;
+    A.eloss = H.has_src[2] ? F[F_ELOSS] : nullptr;
+    A.Fx_t = H.tmp_x;
+    A.Fr_t = H.tmp_r;
+    int maxit;
+    double cnv;
+    double* n_old;
+    if (sp == SP_ELEC) {
+        A.n = F[F_RHOE]; A.Fx = F[F_RHOUEX]; A.Fr = F[F_RHOUER]; A.T = F[F_TE]; A.m = H.me; A.tfac = 1.0;
+        maxit = H.p.maxITR_HES_rhoe; cnv = H.p.error_cnv_HES_rhoe; n_old = F[F_RHOE_OLD];
+    } else if (sp == SP_ION) {
+        A.n = F[F_RHOI]; A.Fx = F[F_RHOUIX]; A.Fr = F[F_RHOUIR]; A.T = F[F_TE]; A.m = H.mi; A.tfac = 1.0;
+        maxit = H.p.maxITR_HES_rhoi; cnv = H.p.error_cnv_HES_rhoi; n_old = H.rhoi_old;
+    } else {
+        A.n = F[F_RHOEPS]; A.Fx = F[F_GX]; A.Fr = F[F_GR]; A.T = H.Te_old; A.m = H.me; A.tfac = 3.0 / 5.0;
+        maxit = H.p.maxITR_HES_rhoeps; cnv = H.p.error_cnv_HES_rhoeps; n_old = H.s_old;
+    }
+    A.n_old = n_old;
+    d2d(H, n_old, A.n);
+    d2d(H, H.sx_old, A.Fx);
+    d2d(H, H.sr_old, A.Fr);
+    int ncount = 0;
+    for (;;) {
+        ncount++;
+        d2d(H, H.tmp_c, A.n);
+        d2d(H, H.tmp_x, A.Fx);
+        d2d(H, H.tmp_r, A.Fr);
+        hipLaunchKernelGGL(k_sg_predict_x, ggrid(V.ni + 1, V.nj), gblk(), 0, H.s, V, A);
+        hipLaunchKernelGGL(k_sg_predict_r, ggrid(V.ni, V.nj), gblk(), 0, H.s, V, A);
+        hipLaunchKernelGGL(k_sg_assemble, ggrid(V.ni, V.nj), gblk(), 0, H.s, V, A, H.mg.aP(), H.mg.aE(), H.mg.aW(),
+                           H.mg.aN(), H.mg.aS(), H.mg.rhs());
+        CATHODE_LAUNCH_CHECK();
+        solve(H);
+        hipLaunchKernelGGL(k_sg_correct, ggrid(V.ni, V.nj), gblk(), 0, H.s, V, A, H.mg.sol());
+        CATHODE_LAUNCH_CHECK();
+        if (ncount >= maxit) break;
+        ResPairs P{A.n, H.tmp_c, A.Fx, H.tmp_x, A.Fr, H.tmp_r};
+        if (calcres3(H, P, nullptr) < cnv) break;
+    }
+    if (report) {
+        ResPairs P{A.n, n_old, A.Fx, H.sx_old, A.Fr, H.sr_old};
+        calcres3(H, P, err3);
+    }
+}
+
+static void ndd_one_step(NddHandle& H, bool report)
+{
+    const NddView& V = H.V;
+    double** F = H.F;
+    double e3[3] = {0, 0, 0};
+    // 1. ions
+    sg_routine(H, SP_ION, report, e3);
+    if (report) H.rep.error_rhoi = e3[0];
+    hipLaunchKernelGGL(k_top_copy, dim3(cdiv(V.ni, 256)), dim3(256), 0, H.s, V, F[F_RHOI]);
+    // 2. Poisson
+    poisson_routine(H, report);
+    // 3. electrons
+    sg_routine(H, SP_ELEC, report, e3);
+    if (report) {
+        H.rep.error_rhoe = e3[0];
+        H.rep.error_rhoUex = e3[1];
+        H.rep.error_rhoUer = e3[2];
+    }
+    // 4. electron energy
+    if (H.p.energy_on) {
+        d2d(H, H.Te_old, F[F_TE]);
+        sg_routine(H, SP_ENERGY, report, e3);
+        if (report) {
+            H.rep.error_rhoeps = e3[0];
+            H.rep.error_Gx = e3[1];
+            H.rep.error_Gr = e3[2];
+        }
+        hipLaunchKernelGGL(k_energy_guard_max, ggrid(V.ni, V.nj), gblk(), 0, H.s, V, F[F_RHOEPS], H.s_old, F[F_RHOE],
+                           H.part.p);
+        hipLaunchKernelGGL(k_max_finish, dim3(1), dim3(256), 0, H.s, H.part.p, H.nb, H.scal.p + 8);
+        hipLaunchKernelGGL(k_te_update, ggrid(V.ni, V.nj), gblk(), 0, H.s, V, F[F_RHOEPS], F[F_RHOE],
+                           (const double*)(H.scal.p + 8), F[F_TE]);
+        hipLaunchKernelGGL(k_ghost_copy4, dim3(cdiv(std::max(V.ni, V.nj), 256)), dim3(256), 0, H.s, V, F[F_TE]);
+        CATHODE_LAUNCH_CHECK();
+    }
+    H.rep.steps++;
+}
+
+}  // namespace cathode
+
+using namespace cathode;
+
+extern "C" {
+
+int cathode_ndd_create(const cathode_ndd_params* p, void** handle)
+{
+    try {
+        if (!require_init()) return CATHODE_ERR_NOINIT;
+        if (!p || !handle || p->ni < 2 || p->nj < 2 || !(p->dx > 0) || !(p->dr > 0) || !(p->dt > 0) ||
+            p->n_see < 0 || p->n_see > CATHODE_SEE_TABLE_MAX || (p->mode != CATHODE_MODE_FP64 && p->mode != CATHODE_MODE_MIXED) ||
+            p->maxITR_HES_phi < 1 || p->maxITR_HES_rhoe < 1 || p->maxITR_HES_rhoi < 1 ||
+            (p->energy_on && p->maxITR_HES_rhoeps < 1) || p->flag_error < 0 || p->flag_error > 3) {
+            set_error("ndd_create: invalid parameters");
+            return CATHODE_ERR_ARG;
+        }
+        std::unique_ptr<NddHandle> H(new NddHandle());
+        H->p = *p;
+        H->s = ctx().stream;
+        const int ni = p->ni, nj = p->nj;
+        H->n = (size_t)(ni + 2) * (nj + 2);
+        H->fields.alloc(19 * H->n);
+        CATHODE_HIP(hipMemset(H->fields.p, 0, 19 * H->n * sizeof(double)));
+        for (int k = 0; k < 19; k++) H->F[k] = H->fields.p + k * H->n;
+        H->work.alloc(12 * H->n);
+        CATHODE_HIP(hipMemset(H->work.p, 0, 12 * H->n * sizeof(double)));
+        double** w[12] = {&H->phi_old, &H->Ex_old, &H->Er_old, &H->tmp_c, &H->tmp_x, &H->tmp_r,
+                          &H->dphi, &H->s_old, &H->sx_old, &H->sr_old, &H->Te_old, &H->rhoi_old};
+        for (int k = 0; k < 12; k++) *w[k] = H->work.p + k * H->n;
+        std::vector<double> r(nj + 2);
+        for (int j = 0; j < nj + 2; j++) r[j] = p->rmin + (j - 0.5) * p->dr;  // initialModule.hpp:86-88
+        H->radii.alloc(nj + 2);
+        CATHODE_HIP(hipMemcpy(H->radii.p, r.data(), (nj + 2) * sizeof(double), hipMemcpyHostToDevice));
+        H->see.alloc(3 * CATHODE_SEE_TABLE_MAX);
+        std::vector<double> tab(3 * CATHODE_SEE_TABLE_MAX);
+        std::memcpy(tab.data(), p->see_eb, sizeof(p->see_eb));
+        std::memcpy(tab.data() + CATHODE_SEE_TABLE_MAX, p->see_rd, sizeof(p->see_rd));
+        std::memcpy(tab.data() + 2 * CATHODE_SEE_TABLE_MAX, p->see_ts, sizeof(p->see_ts));
+        CATHODE_HIP(hipMemcpy(H->see.p, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+        NddView& V = H->V;
+        V.ni = ni; V.nj = nj; V.ld = nj + 2;
+        V.dx = p->dx; V.dr = p->dr; V.dt = p->dt;
+        V.V_bias = p->V_bias; V.mu_i = p->mu_i; V.Ti = p->Ti; V.masse = p->masse; V.massi = p->massi;
+        V.coefIISEE_ts = p->coefIISEE_ts; V.Te_emitSEE_eV = p->Te_emitSEE_eV;
+        V.ratioEngy_EISEE_rd = p->ratioEngy_EISEE_rd; V.dTe_SEE = p->dTe_SEE; V.n_see = p->n_see;
+        V.r = H->radii.p;
+        V.see_eb = H->see.p; V.see_rd = H->see.p + CATHODE_SEE_TABLE_MAX; V.see_ts = H->see.p + 2 * CATHODE_SEE_TABLE_MAX;
+        H->me = mobility(p->mu_e);
+        H->mi = mobility(p->mu_i);
+        H->mg.configure(ni, nj, H->s);
+        V.ld0 = H->mg.ld0();
+        const int n0 = (ni + 2) * V.ld0;
+        hipLaunchKernelGGL(k_fill_one, dim3(cdiv(n0, 256)), dim3(256), 0, H->s, n0, H->mg.act(), ni, nj, V.ld0);
+        CATHODE_LAUNCH_CHECK();
+        H->mg.set_weights(H->radii.p);  // conservative rows: w_j = r_j
+        H->nb = cdiv(nj, NB_X) * cdiv(ni, NB_Y);
+        H->part.alloc(6 * H->nb + 64);
+        H->scal.alloc(16);
+        CATHODE_HIP(hipHostMalloc(&H->h_scal, 16 * sizeof(double), hipHostMallocDefault));
+        CATHODE_HIP(hipStreamSynchronize(H->s));
+        *handle = H.release();
+        return CATHODE_OK;
+    } catch (const hip_failure&) {
+        return CATHODE_ERR_HIP;
+    } catch (const std::bad_alloc&) {
+        set_error("ndd_create: host allocation failed");
+        return CATHODE_ERR_NOMEM;
+    }
+}
+
+int cathode_ndd_destroy(void* handle)
+{
+    NddHandle* H = (NddHandle*)handle;
+    if (!H) return CATHODE_OK;
+    if (H->h_scal) (void)hipHostFree(H->h_scal);
+    delete H;
+    return CATHODE_OK;
+}
+
+int cathode_ndd_set_state(void* handle, const cathode_ndd_fields* host)
+{
+    try {
+        NddHandle* H = (NddHandle*)handle;
+        if (!H || !host) {
+            set_error("ndd_set_state: null argument");
+            return CATHODE_ERR_ARG;
+        }
+        cathode_ndd_fields f = *host;
+        for (int k = 0; k < 19; k++) {
+            const double* src = *field_slot(&f, k);
+            if (!src) {
+                if (k < F_S) {
+                    set_error("ndd_set_state: state array missing");
+                    return CATHODE_ERR_ARG;
+                }
+                CATHODE_HIP(hipMemsetAsync(H->F[k], 0, H->n * sizeof(double), H->s));
+                H->has_src[k - F_S] = false;
+                continue;
+            }
+            CATHODE_HIP(hipMemcpyAsync(H->F[k], src, H->n * sizeof(double), hipMemcpyHostToDevice, H->s));
+            if (k >= F_S) H->has_src[k - F_S] = true;
+        }
+        CATHODE_HIP(hipStreamSynchronize(H->s));
+        return CATHODE_OK;
+    } catch (const hip_failure&) {
+        return CATHODE_ERR_HIP;
+    }
+}
+
+int cathode_ndd_get_state(void* handle, cathode_ndd_fields* host)
+{
+    try {
+        NddHandle* H = (NddHandle*)handle;
+        if (!H || !host) {
+            set_error("ndd_get_state: null argument");
+            return CATHODE_ERR_ARG;
+        }
+        for (int k = 0; k < 19; k++) {
+            double* dst = *field_slot(host, k);
+            if (!dst) continue;
+            CATHODE_HIP(hipMemcpyAsync(dst, H->F[k], H->n * sizeof(double), hipMemcpyDeviceToHost, H->s));
+        }
+        CATHODE_HIP(hipStreamSynchronize(H->s));
+        return CATHODE_OK;
+    } catch (const hip_failure&) {
+        return CATHODE_ERR_HIP;
+    }
+}
+
+int cathode_ndd_step(void* handle, int nsteps, cathode_ndd_report* rep)
+{
+    try {
+        NddHandle* H = (NddHandle*)handle;
+        if (!H || nsteps < 0) {
+            set_error("ndd_step: invalid argument");
+            return CATHODE_ERR_ARG;
+        }
+        auto t0 = std::chrono::steady_clock::now();
+        std::memset(&H->rep, 0, sizeof(H->rep));
+        for (int s = 0; s < nsteps; s++) ndd_one_step(*H, rep != nullptr && s == nsteps - 1);
+        if (rep && nsteps > 0) {
+            hipLaunchKernelGGL(k_anode, dim3(1), dim3(256), 0, H->s, H->V, H->F[F_RHOUIX], H->F[F_RHOUEX], H->scal.p + 12);
+            CATHODE_LAUNCH_CHECK();
+            CATHODE_HIP(hipMemcpyAsync(H->h_scal, H->scal.p + 12, 2 * sizeof(double), hipMemcpyDeviceToHost, H->s));
+        }
+        CATHODE_HIP(hipStreamSynchronize(H->s));
+        if (rep) {
+            *rep = H->rep;
+            if (nsteps > 0) {
+                rep->Ii_anode = H->h_scal[0];
+                rep->Ie_anode = H->h_scal[1];
+                rep->I_anode = H->h_scal[0] + H->h_scal[1];
+            }
+            rep->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        }
+        if (!(H->rep.max_rel_residual == H->rep.max_rel_residual)) {
+            set_error("ndd_step: non-finite residual in a linear solve");
+            return CATHODE_ERR_NUMERIC;
+        }
+        return CATHODE_OK;
+    } catch (const hip_failure&) {
+        return CATHODE_ERR_HIP;
+    }
+}
+
+int cathode_ndd_profile(void* handle, int enable)
+{
+    NddHandle* H = (NddHandle*)handle;
+    if (!H) return CATHODE_ERR_ARG;
+    H->mg.prof.enabled = enable != 0;
+    H->mg.prof.launches = 0;
+    H->mg.prof.total_ms = 0.0;
+    return CATHODE_OK;
+}
+
+int cathode_ndd_profile_read(void* handle, int* launches, double* total_ms, double* bytes_per_launch)
+{
+    NddHandle* H = (NddHandle*)handle;
+    if (!H) return CATHODE_ERR_ARG;
+    if (launches) *launches = H->mg.prof.launches;
+    if (total_ms) *total_ms = H->mg.prof.total_ms;
+    if (bytes_per_launch) *bytes_per_launch = H->mg.prof.bytes_per_launch;
+    return CATHODE_OK;
+}
+
+}  // extern "C"
