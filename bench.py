@@ -1,0 +1,156 @@
+#!/usr/bin/env python3
+"""Benchmark: implicit NDD steps/s at 512x256 (every linear solve to relative
+residual 1e-10) on one MI355X, mixed precision vs plain FP64.
+
+Workload (one "step" = one implicit time step of the whole equation set):
+512(z) x 256(r) cells of 0.05 mm, ions + Poisson + electrons + electron
+energy, each a two-iteration predictor-corrector loop with one 5-point solve
+per iteration (8 solves per step), densities = 1e14 floor + four seeded
+Gaussians, Te = 2..10 eV smooth seeded field, bias 30 V, dt = 1e-10 s.
+Synthetic state generated on the host (seed 1) and resident on the GPU before
+the timed region.
+
+Prints one JSON line.  `value` is the steps/s of the fastest GPU path
+(mixed precision); the plain-FP64 GPU path is reported beside it.
+"""
+import argparse
+import ctypes
+import importlib
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+PKG = "02_cathode-08_modulerev_amd"
+abi = importlib.import_module(PKG + ".abi")
+W = importlib.import_module(PKG + ".workloads")
+
+METRIC = "NDD steps/s at 512x256 (resid 1e-10): learned+mixed vs plain FP64, 1 GPU"
+HBM_PEAK_GBS = 8000.0
+
+
+def run_path(lib, mode, steps, warmup, seed):
+    p, st = W.config1(seed=seed, mode=mode)
+    h = ctypes.c_void_p()
+    abi.check(lib.cathode_ndd_create(ctypes.byref(p), ctypes.byref(h)), lib)
+    f = abi.fields_struct(st)
+    abi.check(lib.cathode_ndd_set_state(h, ctypes.byref(f)), lib)
+    rep = abi.NddReport()
+    if warmup > 0:
+        abi.check(lib.cathode_ndd_step(h, warmup, ctypes.byref(rep)), lib)
+    # cathode_ndd_step returns after synchronising the library stream
+    t0 = time.perf_counter()
+    abi.check(lib.cathode_ndd_step(h, steps, ctypes.byref(rep)), lib)
+    t1 = time.perf_counter()
+    wall = t1 - t0
+    its = rep.total_iterations
+    solves = rep.solves
+    nonconv = rep.nonconverged
+    maxres = rep.max_rel_residual
+    # roofline pass: HIP events around every fine-grid smoothing launch
+    nprof = min(5, steps)
+    abi.check(lib.cathode_ndd_profile(h, 1), lib)
+    abi.check(lib.cathode_ndd_step(h, nprof, ctypes.byref(rep)), lib)
+    launches = ctypes.c_int()
+    tot_ms = ctypes.c_double()
+    bpl = ctypes.c_double()
+    abi.check(lib.cathode_ndd_profile_read(h, ctypes.byref(launches), ctypes.byref(tot_ms), ctypes.byref(bpl)), lib)
+    abi.check(lib.cathode_ndd_profile(h, 0), lib)
+    lib.cathode_ndd_destroy(h)
+    avg_ms = tot_ms.value / max(launches.value, 1)
+    return {
+        "steps_per_s": steps / wall,
+        "ms_per_step": 1e3 * wall / steps,
+        "iterations_per_solve": its / max(solves, 1),
+        "solves": solves,
+        "nonconverged": nonconv,
+        "max_rel_residual": maxres,
+        "smooth_launches": launches.value,
+        "smooth_avg_us": 1e3 * avg_ms,
+        "smooth_bytes": bpl.value,
+        "smooth_GBps": bpl.value / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0,
+    }
+
+
+def cpu_baseline(seed, budget_s=15.0):
+    """The oracle's host copy of the same algorithm (FP64 multigrid-BiCGStab),
+    one thread, on a bounded number of steps of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O  # the checker, timed only as the CPU baseline
+
+    O.lib().orc_mg_set_threads(1)
+    p, st = W.config1(seed=seed)
+    t0 = time.perf_counter()
+    O.ndd_step(p, st, 1, solver=1)
+    t1 = time.perf_counter()
+    n = max(1, min(30, int(budget_s / max(t1 - t0, 1e-3))))
+    t0 = time.perf_counter()
+    O.ndd_step(p, st, n, solver=1)
+    t1 = time.perf_counter()
+    return {"value": n / (t1 - t0), "unit": "steps/s", "cores": 1, "kind": "port",
+            "sample": f"{n} steps of the same 512x256 workload after 1 untimed step, "
+                      f"oracle/orc_mg.c + orc_ndd.c (gcc -O2), 1 thread"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--paths", default="fp64,mixed")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
+              "the single-instance implicit step does not shard across GPUs here", file=sys.stderr)
+        sys.exit(2)
+    lib = abi.load()
+    abi.check(lib.cathode_init(0), lib)
+    paths = [s.strip() for s in args.paths.split(",") if s.strip()]
+    res = {}
+    for name in paths:
+        mode = abi.CATHODE_MODE_MIXED if name == "mixed" else abi.CATHODE_MODE_FP64
+        res[name] = run_path(lib, mode, args.steps, args.warmup, args.seed)
+        print(f"# {name}: {res[name]['steps_per_s']:.2f} steps/s, "
+              f"{res[name]['iterations_per_solve']:.2f} BiCGStab it/solve, "
+              f"smooth {res[name]['smooth_avg_us']:.2f} us ({res[name]['smooth_GBps']:.0f} GB/s)",
+              file=sys.stderr, flush=True)
+    best = "mixed" if "mixed" in res else paths[0]
+    b = res[best]
+    roof = {"bound": "hbm", "achieved": round(b["smooth_GBps"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(b["smooth_GBps"] / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "k_smooth (finest level red-black half sweep)",
+            "bytes_per_launch": b["smooth_bytes"], "avg_us": round(b["smooth_avg_us"], 3)}
+    out = {
+        "metric": METRIC,
+        "value": round(b["steps_per_s"], 3),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(b["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64" if best == "mixed" else "f64",
+        "data": "synthetic",
+        "config": {"workload": "implicit NDD step, 512x256 cells of 0.05 mm, ions+Poisson+electrons+energy, "
+                               "each solve to relative residual 1e-10", "grid": [512, 256], "seed": args.seed,
+                   "path": best},
+        "roofline": roof,
+        "paths": {k: {kk: (round(vv, 6) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+                  for k, v in res.items()},
+    }
+    if "fp64" in res and best != "fp64":
+        out["speedup_vs_fp64"] = round(b["steps_per_s"] / res["fp64"]["steps_per_s"], 3)
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args.seed)
+    lib.cathode_finalize()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()
