@@ -151,100 +151,6 @@ __global__ void k_coarsen(int fni, int fnj, int fld, int fi, int fj,
     caP[kc] = (M + hb * (S - M)) + (E + W + N + So);
 }
 
-// Dense inverse of the coarsest operator (<= 64 unknowns), in-place
-// Gauss-Jordan with partial pivoting; one block of 256 threads.
-__global__ void k_coarse_inverse(int ni, int nj, int ld, const double* __restrict__ aP,
-                                 const double* __restrict__ aE, const double* __restrict__ aW,
-                                 const double* __restrict__ aN, const double* __restrict__ aS,
-                                 const unsigned char* __restrict__ act, int* __restrict__ cidx,
-                                 double* __restrict__ inv64, float* __restrict__ inv32,
-                                 int* __restrict__ n_out, const double* __restrict__ scal, int mixed)
-{
-    __shared__ double A[MG_COARSE_MAX * MG_COARSE_MAX];
-    __shared__ int idx[MG_COARSE_CELLS];
-    __shared__ int perm[MG_COARSE_MAX];
-    __shared__ double fcol[MG_COARSE_MAX];
-    __shared__ int n_sh, piv_sh;
-    const int tid = threadIdx.x;
-    const int ncell = (ni + 2) * ld;
-    if (tid == 0) {
-        int n = 0;
-        for (int k = 0; k < ncell; k++) idx[k] = act[k] ? n++ : -1;
-        n_sh = n;
-        *n_out = n;
-    }
-    __syncthreads();
-    const int n = n_sh;
-    for (int q = tid; q < n * n; q += blockDim.x) A[q] = 0.0;
-    for (int k = tid; k < ncell; k += blockDim.x) cidx[k] = idx[k];
-    __syncthreads();
-    for (int k = tid; k < ncell; k += blockDim.x) {
-        int row = idx[k];
-        if (row < 0) continue;
-        A[row * n + row] = aP[k];
-        int nb[4] = {idx[k + ld], idx[k - ld], idx[k + 1], idx[k - 1]};
-        double a[4] = {aE[k], aW[k], aN[k], aS[k]};
-        for (int q = 0; q < 4; q++)
-            if (nb[q] >= 0) A[row * n + nb[q]] -= a[q];
-    }
-    __syncthreads();
-    for (int k = 0; k < n; k++) {
-        if (tid == 0) {
-            int p = k;
-            double best = fabs(A[k * n + k]);
-            for (int i = k + 1; i < n; i++)
-                if (fabs(A[i * n + k]) > best) {
-                    best = fabs(A[i * n + k]);
-                    p = i;
-                }
-            piv_sh = p;
-            perm[k] = p;
-        }
-        __syncthreads();
-        const int p = piv_sh;
-        if (p != k)
-            for (int c = tid; c < n; c += blockDim.x) {
-                double t = A[k * n + c];
-                A[k * n + c] = A[p * n + c];
-                A[p * n + c] = t;
-            }
-        __syncthreads();
-        double piv = A[k * n + k];
-        if (piv == 0.0) piv = 1e-300;
-        __syncthreads();
-        for (int c = tid; c < n; c += blockDim.x) A[k * n + c] = (c == k ? 1.0 : A[k * n + c]) / piv;
-        for (int r = tid; r < n; r += blockDim.x) fcol[r] = A[r * n + k];
-        __syncthreads();
-        for (int q = tid; q < n * n; q += blockDim.x) {
-            const int i = q / n, c = q % n;
-            if (i == k) continue;
-            const double f = fcol[i];
-            if (c == k)
-                A[q] = -f * A[k * n + k];
-            else
-                A[q] -= f * A[k * n + c];
-        }
-        __syncthreads();
-    }
-    // undo the row exchanges as column exchanges, last first
-    for (int k = n - 1; k >= 0; k--) {
-        const int p = perm[k];
-        if (p != k)
-            for (int r = tid; r < n; r += blockDim.x) {
-                double t = A[r * n + k];
-                A[r * n + k] = A[r * n + p];
-                A[r * n + p] = t;
-            }
-        __syncthreads();
-    }
-    // the FP32 levels hold s32 * A, so their coarsest inverse is A^-1 / s32
-    const double s32 = mixed ? scal[3] : 1.0;
-    for (int q = tid; q < n * n; q += blockDim.x) {
-        inv64[q] = A[q];
-        inv32[q] = (float)(A[q] / s32);
-    }
-}
-
 __global__ void k_level_to_f32(size_t n, const double* __restrict__ src, float* __restrict__ dst,
                                const double* __restrict__ scal)
 {
@@ -254,79 +160,11 @@ __global__ void k_level_to_f32(size_t n, const double* __restrict__ src, float* 
 }
 
 // ------------------------------------------------------------------ V-cycle kernels
-template <class T>
-__global__ void k_smooth(int ni, int nj, int ld, const T* __restrict__ aP, const T* __restrict__ aE,
-                         const T* __restrict__ aW, const T* __restrict__ aN,
-                         const T* __restrict__ aS, const T* __restrict__ b, T* __restrict__ x,
-                         int color)
-{
-    CELL_IJ;
-    if (i > ni || j > nj || ((i + j) & 1) != color) return;
-    const size_t k = (size_t)i * ld + j;
-    T s = b[k] + aE[k] * x[k + ld] + aW[k] * x[k - ld] + aN[k] * x[k + 1] + aS[k] * x[k - 1];
-    x[k] = s / aP[k];
-}
-
-template <class T>
-__global__ void k_restrict(int fni, int fnj, int fld, int fi, int fj, const T* __restrict__ aP,
-                           const T* __restrict__ aE, const T* __restrict__ aW,
-                           const T* __restrict__ aN, const T* __restrict__ aS,
-                           const T* __restrict__ b, const T* __restrict__ x,
-                           const double* __restrict__ w, int cni, int cnj, int cld,
-                           T* __restrict__ cb, T* __restrict__ cx)
-{
-    const int J = 1 + blockIdx.x * BX + threadIdx.x;
-    const int I = 1 + blockIdx.y * BY + threadIdx.y;
-    if (I > cni || J > cnj) return;
-    const int i0 = fi * (I - 1) + 1, i1 = min(fi * I, fni);
-    const int j0 = fj * (J - 1) + 1, j1 = min(fj * J, fnj);
-    T sum = 0;
-    for (int i = i0; i <= i1; i++)
-        for (int j = j0; j <= j1; j++) {
-            const size_t k = (size_t)i * fld + j;
-            T r = b[k] - aP[k] * x[k] + aE[k] * x[k + fld] + aW[k] * x[k - fld] + aN[k] * x[k + 1] +
-                  aS[k] * x[k - 1];
-            sum += (w ? (T)w[j] : (T)1) * r;
-        }
-    const size_t kc = (size_t)I * cld + J;
-    cb[kc] = sum;
-    cx[kc] = 0;
-}
-
-template <class T>
-__global__ void k_prolong(int fni, int fnj, int fld, int fi, int fj,
-                          const unsigned char* __restrict__ act, T* __restrict__ x, int cld,
-                          const T* __restrict__ cx)
-{
-    CELL_IJ;
-    if (i > fni || j > fnj) return;
-    const size_t k = (size_t)i * fld + j;
-    if (!act[k]) return;
-    const int I = fi == 2 ? (i + 1) >> 1 : i, J = fj == 2 ? (j + 1) >> 1 : j;
-    x[k] += cx[(size_t)I * cld + J];
-}
-
-template <class T>
-__global__ void k_coarse_apply(int ncell, const int* __restrict__ cidx, const int* __restrict__ n_p,
-                               const T* __restrict__ inv, const T* __restrict__ b, T* __restrict__ x)
-{
-    __shared__ T v[MG_COARSE_MAX];
-    __shared__ int map[MG_COARSE_CELLS];
-    const int n = *n_p;
-    for (int k = threadIdx.x; k < ncell; k += blockDim.x) {
-        int q = cidx[k];
-        map[k] = q;
-        if (q >= 0) v[q] = b[k];
-    }
-    __syncthreads();
-    for (int k = threadIdx.x; k < ncell; k += blockDim.x) {
-        int q = map[k];
-        if (q < 0) continue;
-        T s = 0;
-        for (int c = 0; c < n; c++) s += inv[q * n + c] * v[c];
-        x[k] = s;
-    }
-}
+__device__ inline double abs2v(double v) { return v * v; }
+__device__ inline float abs2v(float v) { return v * v; }
+}  // namespace cathode
+#include "vcycle.hpp"
+namespace cathode {
 
 // ------------------------------------------------------------------ outer loop kernels
 // Device scalars (MGSolver::scal):
@@ -612,7 +450,7 @@ void MGSolver::configure(int ni, int nj, hipStream_t s)
     n = 1;
     while (n < MG_MAX_LEVELS) {
         int a = dims[n - 1][0], b = dims[n - 1][1];
-        if ((long)a * b <= MG_COARSE_MAX || (a == 1 && b == 1)) break;
+        if ((long)a * b <= COARSEST_MAX || (a == 1 && b == 1)) break;
         dims[n][0] = a > 1 ? (a + 1) / 2 : a;
         dims[n][1] = b > 1 ? (b + 1) / 2 : b;
         n++;
@@ -673,10 +511,12 @@ void MGSolver::configure(int ni, int nj, hipStream_t s)
     CATHODE_HIP(hipMemset(bsol.p, 0, n0 * sizeof(double)));
     CATHODE_HIP(hipMemset(xsol.p, 0, n0 * sizeof(double)));
     w0.alloc(nj + 2);
-    const int nc = (dims[nl - 1][0] + 2) * pad_ld(dims[nl - 1][1]);
-    cidx.alloc(nc + 1);
-    cinv64.alloc(MG_COARSE_MAX * MG_COARSE_MAX);
-    cinv32.alloc(MG_COARSE_MAX * MG_COARSE_MAX);
+    // levels from lt on run inside the one-workgroup tail kernel
+    lt = nl - 1;
+    while (lt > 0 && (long)dims[lt - 1][0] * dims[lt - 1][1] <= TAIL_MAX_CELLS) lt--;
+    if (nl - lt > TAIL_MAX_LEVELS) lt = nl - TAIL_MAX_LEVELS;
+    tail_cells = 0;
+    for (int l = lt; l < nl; l++) tail_cells += (dims[l][0] + 2) * (dims[l][1] + 2);
     part.alloc(2 * (cdiv(nj, BX) * cdiv(ni, BY)) + 64);
     scal.alloc(16);
     kv.alloc(8 * n0);
@@ -731,67 +571,77 @@ void MGSolver::setup(int mode, hipStream_t s)
         }
         CATHODE_LAUNCH_CHECK();
     }
-    Level<double>& Cc = L64[nl - 1];
-    hipLaunchKernelGGL(k_coarse_inverse, dim3(1), dim3(256), 0, s, Cc.ni, Cc.nj, Cc.ld, Cc.aP, Cc.aE, Cc.aW,
-                       Cc.aN, Cc.aS, actall.p + level_off[nl - 1], cidx.p, cinv64.p, cinv32.p,
-                       (int*)(scal.p + 8), (const double*)scal.p, mode == CATHODE_MODE_MIXED ? 1 : 0);
-    CATHODE_LAUNCH_CHECK();
     (void)n0;
 }
 
 template <class T>
-void MGSolver::vcycle(Level<T>* L, const T* cinv, hipStream_t s)
+void MGSolver::vcycle(Level<T>* L, hipStream_t s)
 {
-    for (int l = 0; l + 1 < nl; l++) {
+    const double* w = user_w ? user_w : w0.p;
+    // fine levels: fused down kernels
+    for (int l = 0; l < lt; l++) {
         Level<T>& A = L[l];
         Level<T>& C = L[l + 1];
-        for (int k = 0; k < 2; k++)
-            for (int c = 0; c < 2; c++) {
-                const bool rec = prof.enabled && l == 0;
-                if (rec) {
-                    hipEvent_t a, b;
-                    CATHODE_HIP(hipEventCreate(&a));
-                    CATHODE_HIP(hipEventCreate(&b));
-                    prof.ev.push_back(a);
-                    prof.ev.push_back(b);
-                    CATHODE_HIP(hipEventRecord(a, s));
-                }
-                hipLaunchKernelGGL(k_smooth<T>, grid2(A.ni, A.nj), dim3(BX, BY), 0, s, A.ni, A.nj, A.ld, A.aP,
-                                   A.aE, A.aW, A.aN, A.aS, A.b, A.x, c);
-                if (rec) CATHODE_HIP(hipEventRecord(prof.ev.back(), s));
-            }
-        const int fi = A.cx ? 2 : 1, fj = A.cy ? 2 : 1;
-        hipLaunchKernelGGL(k_restrict<T>, grid2(C.ni, C.nj), dim3(BX, BY), 0, s, A.ni, A.nj, A.ld, fi, fj, A.aP,
-                           A.aE, A.aW, A.aN, A.aS, A.b, A.x, l == 0 ? (user_w ? user_w : w0.p) : nullptr, C.ni,
-                           C.nj, C.ld, C.b, C.x);
+        VLevel<T> V{A.ni, A.nj, A.ld, A.aP, A.aE, A.aW, A.aN, A.aS, actall.p + level_off[l], A.b, A.x};
+        const dim3 g(cdiv(A.nj, VT_TJ), cdiv(A.ni, VT_TI));
+        const bool rec = prof.enabled && l == 0;
+        if (rec) prof_begin(s);
+        hipLaunchKernelGGL((k_vtile<T, VT_TI, VT_TJ, 5, true>), g, dim3(VT_NT), 0, s, V, A.cx ? 2 : 1, A.cy ? 2 : 1,
+                           l == 0 ? w : (const double*)nullptr, C.ni, C.nj, C.ld, C.b);
+        if (rec) prof_end(s);
     }
-    Level<T>& Cc = L[nl - 1];
-    hipLaunchKernelGGL(k_coarse_apply<T>, dim3(1), dim3(256), 0, s, (Cc.ni + 2) * Cc.ld, cidx.p,
-                       (const int*)(scal.p + 8), cinv, Cc.b, Cc.x);
-    for (int l = nl - 2; l >= 0; l--) {
+    // coarse levels in one workgroup
+    TailArgs<T> ta{};
+    ta.nt = nl - lt;
+    int off = 0;
+    for (int q = 0; q < ta.nt; q++) {
+        Level<T>& A = L[lt + q];
+        ta.ni[q] = A.ni;
+        ta.nj[q] = A.nj;
+        ta.off[q] = off;
+        off += (A.ni + 2) * (A.nj + 2);
+        ta.cx[q] = A.cx;
+        ta.cy[q] = A.cy;
+        ta.gld[q] = A.ld;
+        ta.aP[q] = A.aP;
+        ta.aE[q] = A.aE;
+        ta.aW[q] = A.aW;
+        ta.aN[q] = A.aN;
+        ta.aS[q] = A.aS;
+        ta.act[q] = actall.p + level_off[lt + q];
+    }
+    ta.ncell = off;
+    ta.b_in = L[lt].b;
+    ta.x_out = L[lt].x;
+    ta.w = lt == 0 ? w : nullptr;
+    const size_t lds = (size_t)off * (7 * sizeof(T) + 1) + 16;
+    hipLaunchKernelGGL(k_tail<T>, dim3(1), dim3(TAIL_NT), lds, s, ta);
+    // fine levels: fused up kernels
+    for (int l = lt - 1; l >= 0; l--) {
         Level<T>& A = L[l];
         Level<T>& C = L[l + 1];
-        const int fi = A.cx ? 2 : 1, fj = A.cy ? 2 : 1;
-        hipLaunchKernelGGL(k_prolong<T>, grid2(A.ni, A.nj), dim3(BX, BY), 0, s, A.ni, A.nj, A.ld, fi, fj,
-                           actall.p + level_off[l], A.x, C.ld, C.x);
-        for (int k = 0; k < 2; k++)
-            for (int c = 0; c < 2; c++) {
-                const bool rec = prof.enabled && l == 0;
-                if (rec) {
-                    hipEvent_t a, b;
-                    CATHODE_HIP(hipEventCreate(&a));
-                    CATHODE_HIP(hipEventCreate(&b));
-                    prof.ev.push_back(a);
-                    prof.ev.push_back(b);
-                    CATHODE_HIP(hipEventRecord(a, s));
-                }
-                hipLaunchKernelGGL(k_smooth<T>, grid2(A.ni, A.nj), dim3(BX, BY), 0, s, A.ni, A.nj, A.ld, A.aP,
-                                   A.aE, A.aW, A.aN, A.aS, A.b, A.x, c);
-                if (rec) CATHODE_HIP(hipEventRecord(prof.ev.back(), s));
-            }
+        VLevel<T> V{A.ni, A.nj, A.ld, A.aP, A.aE, A.aW, A.aN, A.aS, actall.p + level_off[l], A.b, A.x};
+        const dim3 g(cdiv(A.nj, VT_TJ), cdiv(A.ni, VT_TI));
+        const bool rec = prof.enabled && l == 0;
+        if (rec) prof_begin(s);
+        hipLaunchKernelGGL((k_vtile<T, VT_TI, VT_TJ, 4, false>), g, dim3(VT_NT), 0, s, V, A.cx ? 2 : 1, A.cy ? 2 : 1,
+                           (const double*)nullptr, C.ni, C.nj, C.ld, C.x);
+        if (rec) prof_end(s);
     }
     CATHODE_LAUNCH_CHECK();
 }
+
+void MGSolver::prof_begin(hipStream_t s)
+{
+    hipEvent_t a, b;
+    CATHODE_HIP(hipEventCreate(&a));
+    CATHODE_HIP(hipEventCreate(&b));
+    prof.ev.push_back(a);
+    prof.ev.push_back(b);
+    CATHODE_HIP(hipEventRecord(a, s));
+}
+
+void MGSolver::prof_end(hipStream_t s) { CATHODE_HIP(hipEventRecord(prof.ev.back(), s)); }
 
 double MGSolver::read_scalar(const double* d, hipStream_t s)
 {
@@ -840,13 +690,13 @@ int MGSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_sta
             if (mixed) {
                 hipLaunchKernelGGL(k_bicg_p<float>, g, blk, 0, s, F.ni, F.nj, F.ld, act0.p, r, v, p, L32[0].b,
                                    L32[0].x, scal.p, restart ? 1 : 0, 1);
-                vcycle<float>(L32, cinv32.p, s);
+                vcycle<float>(L32, s);
                 hipLaunchKernelGGL(k_bicg_apply<float>, g, blk, 0, s, F.ni, F.nj, F.ld, F.aP, F.aE, F.aW, F.aN,
                                    F.aS, act0.p, L32[0].x, ph, v, r0, scal.p, 1, 0, part.p, nb);
             } else {
                 hipLaunchKernelGGL(k_bicg_p<double>, g, blk, 0, s, F.ni, F.nj, F.ld, act0.p, r, v, p, F.b, F.x,
                                    scal.p, restart ? 1 : 0, 0);
-                vcycle<double>(L64, cinv64.p, s);
+                vcycle<double>(L64, s);
                 hipLaunchKernelGGL(k_bicg_apply<double>, g, blk, 0, s, F.ni, F.nj, F.ld, F.aP, F.aE, F.aW, F.aN,
                                    F.aS, act0.p, F.x, ph, v, r0, scal.p, 0, 0, part.p, nb);
             }
@@ -856,13 +706,13 @@ int MGSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_sta
             if (mixed) {
                 hipLaunchKernelGGL(k_bicg_s<float>, g, blk, 0, s, F.ni, F.nj, F.ld, act0.p, r, v, sv, L32[0].b,
                                    L32[0].x, scal.p, 1);
-                vcycle<float>(L32, cinv32.p, s);
+                vcycle<float>(L32, s);
                 hipLaunchKernelGGL(k_bicg_apply<float>, g, blk, 0, s, F.ni, F.nj, F.ld, F.aP, F.aE, F.aW, F.aN,
                                    F.aS, act0.p, L32[0].x, sh, t, sv, scal.p, 1, 1, part.p, nb);
             } else {
                 hipLaunchKernelGGL(k_bicg_s<double>, g, blk, 0, s, F.ni, F.nj, F.ld, act0.p, r, v, sv, F.b, F.x,
                                    scal.p, 0);
-                vcycle<double>(L64, cinv64.p, s);
+                vcycle<double>(L64, s);
                 hipLaunchKernelGGL(k_bicg_apply<double>, g, blk, 0, s, F.ni, F.nj, F.ld, F.aP, F.aE, F.aW, F.aN,
                                    F.aS, act0.p, F.x, sh, t, sv, scal.p, 0, 1, part.p, nb);
             }
@@ -896,10 +746,11 @@ int MGSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_sta
         }
         for (auto& e : prof.ev) (void)hipEventDestroy(e);
         prof.ev.clear();
-        // one half sweep: the cells of one colour read 5 coefficients + b and
-        // write x; the other colour's x is read once (4 neighbours, cached)
-        const double w = mixed ? 4.0 : 8.0;
-        prof.bytes_per_launch = 0.5 * (double)F.ni * (double)F.nj * 8.0 * w;
+        // level-0 fused kernels (down and up alternate): per fine cell the down
+        // kernel reads 5 coefficients + b and writes x and 1/4 coarse b (7.25
+        // values), the up kernel also reads x and 1/4 coarse x (8.25 values)
+        const double vb = mixed ? 4.0 : 8.0;
+        prof.bytes_per_launch = 0.5 * (7.25 + 8.25) * vb * (double)F.ni * (double)F.nj;
     }
     auto t1 = std::chrono::steady_clock::now();
     if (st) {

@@ -13,7 +13,10 @@
 //              part s - m (halved); 5-point on every level;
 //   smoother   red-black point Gauss-Seidel, V(2,2);
 //   transfers  w-weighted sum restriction, piecewise-constant prolongation;
-//   coarsest   dense inverse (<= 64 unknowns) applied by one workgroup;
+//   coarsest   <= 8 unknowns, Gaussian elimination with partial pivoting;
+//   kernels    one fused down kernel (2 sweeps + residual + restriction) and
+//              one fused up kernel (prolongation + 2 sweeps) per fine level,
+//              and one workgroup for all levels of <= 512 cells (vcycle.hpp).
 //   outer      BiCGStab in FP64 right-preconditioned by one V-cycle per
 //              application, from x = 0 (solverFuncs.hpp:106-109), stopping
 //              when the true residual ||b - A x||_2 / ||b||_2 <= tol (HYPRE's
@@ -27,9 +30,6 @@
 namespace cathode {
 
 constexpr int MG_MAX_LEVELS = 24;
-constexpr int MG_COARSE_MAX = 64;
-// cells (with ghosts, padded rows) of the coarsest level: (64 + 2) x 72
-constexpr int MG_COARSE_CELLS = 66 * 72;
 
 template <class T>
 struct Level {
@@ -82,7 +82,9 @@ public:
 private:
     void setup(int mode, hipStream_t s);
     template <class T>
-    void vcycle(Level<T>* L, const T* cinv, hipStream_t s);
+    void vcycle(Level<T>* L, hipStream_t s);
+    void prof_begin(hipStream_t s);
+    void prof_end(hipStream_t s);
     double read_scalar(const double* d, hipStream_t s);
 
     int nl = 0;
@@ -97,9 +99,8 @@ private:
     DevBuf<unsigned char> act0;
     DevBuf<double> bsol, xsol, w0;
     const double* user_w = nullptr;
-    DevBuf<int> cidx;          // coarsest cell -> dense row (or -1)
-    DevBuf<double> cinv64;     // dense inverse of the coarsest operator
-    DevBuf<float> cinv32;
+    int lt = 0;                // first level handled by the one-workgroup tail kernel
+    int tail_cells = 0;        // LDS cells of the tail levels
     DevBuf<double> part;       // reduction partials (2 slots)
     DevBuf<double> kv;         // BiCGStab vectors r, r0, p, v, s, t, ph, sh
     DevBuf<double> scal;       // device scalars: [0] |b|^2, [1] |r|^2, [2] scale, [3] f32 matrix scale

@@ -1,0 +1,305 @@
+// vcycle.hpp -- fused V-cycle kernels (included by mg.hip).
+//
+// Fine levels run two kernels each:
+//   k_down  a TI x TJ tile of cells plus a halo of 5: coefficients held in
+//           registers, x in LDS, starting from x = 0; two red-black
+//           Gauss-Seidel sweeps (4 half sweeps, red first), then the
+//           residual of the tile cells, restricted (w-weighted sum of the
+//           2x2 children) into the coarse right-hand side.  Each half sweep
+//           invalidates one more ring of the halo, so after 4 sweeps and the
+//           residual (5 rings) the tile holds exactly the values the global
+//           red-black sweep produces.
+//   k_up    the same tile with a halo of 4: x = x_down + x_coarse(parent)
+//           (piecewise-constant prolongation), then two red-black sweeps.
+// Coarse levels (<= TAIL_MAX_CELLS cells each) run in one workgroup, all in
+// LDS (k_tail), ending with a direct solve of the coarsest (<= 8 unknowns).
+#pragma once
+
+namespace cathode {
+
+constexpr int VT_TI = 16, VT_TJ = 32;   // tile of the fused fine-level kernels
+constexpr int VT_NT = 256;              // threads of a tile workgroup
+constexpr int TAIL_MAX_CELLS = 512;     // levels with at most this many cells go to the tail
+constexpr int TAIL_NT = 512;            // threads of the tail workgroup
+constexpr int TAIL_MAX_LEVELS = 16;
+constexpr int COARSEST_MAX = 8;         // unknowns of the coarsest direct solve
+
+template <class T>
+struct VLevel {
+    int ni, nj, ld;
+    const T *aP, *aE, *aW, *aN, *aS;
+    const unsigned char* act;
+    T *b, *x;
+};
+
+// ------------------------------------------------------------------ fine-level tiles
+template <class T, int TI, int TJ, int H, bool DOWN>
+__global__ void __launch_bounds__(VT_NT)
+    k_vtile(VLevel<T> F, int fi, int fj, const double* __restrict__ w, int cni, int cnj, int cld,
+            T* __restrict__ cbx)
+{
+    constexpr int RI = TI + 2 * H, RJ = TJ + 2 * H, R = RI * RJ;
+    constexpr int K = (R + VT_NT - 1) / VT_NT;
+    __shared__ T xs[R];
+    __shared__ T rs[DOWN ? TI * TJ : 1];
+    const int tid = threadIdx.x;
+    const int gi0 = (int)blockIdx.y * TI + 1 - H;  // global i of region row 0
+    const int gj0 = (int)blockIdx.x * TJ + 1 - H;
+    T cP[K], cE[K], cW[K], cN[K], cS[K], cB[K];
+    unsigned flags[K];  // bit0 active, bit1 colour, bit2 updatable (not on the region rim), bit3 tile cell
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int c = tid + k * VT_NT;
+        flags[k] = 0;
+        cP[k] = T(1);
+        cE[k] = cW[k] = cN[k] = cS[k] = cB[k] = T(0);
+        if (c >= R) continue;
+        const int ri = c / RJ, rj = c - ri * RJ;
+        const int gi = gi0 + ri, gj = gj0 + rj;
+        T xv = T(0);
+        if (gi >= 1 && gi <= F.ni && gj >= 1 && gj <= F.nj) {
+            const size_t g = (size_t)gi * F.ld + gj;
+            if (F.act[g]) {
+                flags[k] |= 1u;
+                cP[k] = F.aP[g];
+                cE[k] = F.aE[g];
+                cW[k] = F.aW[g];
+                cN[k] = F.aN[g];
+                cS[k] = F.aS[g];
+                cB[k] = F.b[g];
+                if (!DOWN) {
+                    const int I = fi == 2 ? (gi + 1) >> 1 : gi, J = fj == 2 ? (gj + 1) >> 1 : gj;
+                    xv = F.x[g] + cbx[(size_t)I * cld + J];
+                }
+            }
+        }
+        flags[k] |= ((gi + gj) & 1) ? 2u : 0u;
+        if (ri >= 1 && ri <= RI - 2 && rj >= 1 && rj <= RJ - 2) flags[k] |= 4u;
+        if (ri >= H && ri < H + TI && rj >= H && rj < H + TJ && gi <= F.ni && gj <= F.nj) flags[k] |= 8u;
+        xs[c] = xv;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+        const unsigned col = (unsigned)(s & 1) << 1;
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int c = tid + k * VT_NT;
+            if ((flags[k] & 7u) == (5u | col)) {
+                T v = cB[k] + cE[k] * xs[c + RJ] + cW[k] * xs[c - RJ] + cN[k] * xs[c + 1] + cS[k] * xs[c - 1];
+                xs[c] = v / cP[k];
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int c = tid + k * VT_NT;
+        if (!(flags[k] & 8u)) continue;
+        const int ri = c / RJ, rj = c - ri * RJ;
+        const int gi = gi0 + ri, gj = gj0 + rj;
+        F.x[(size_t)gi * F.ld + gj] = xs[c];
+        if (DOWN) {
+            T r = T(0);
+            if (flags[k] & 1u) {
+                r = cB[k] - cP[k] * xs[c] + cE[k] * xs[c + RJ] + cW[k] * xs[c - RJ] + cN[k] * xs[c + 1] +
+                    cS[k] * xs[c - 1];
+                if (w) r = r * (T)w[gj];
+            }
+            rs[(ri - H) * TJ + (rj - H)] = r;
+        }
+    }
+    if (DOWN) {
+        __syncthreads();
+        // coarse cells of this tile
+        const int CI = TI / fi, CJ = TJ / fj;
+        const int I0 = (int)blockIdx.y * CI, J0 = (int)blockIdx.x * CJ;
+        for (int q = tid; q < CI * CJ; q += VT_NT) {
+            const int il = q / CJ, jl = q - il * CJ;
+            const int I = I0 + il + 1, J = J0 + jl + 1;
+            if (I > cni || J > cnj) continue;
+            T sum = T(0);
+            for (int a = 0; a < fi; a++)
+                for (int b = 0; b < fj; b++) {
+                    const int ti = il * fi + a, tj = jl * fj + b;
+                    const int gi = (int)blockIdx.y * TI + ti + 1, gj = (int)blockIdx.x * TJ + tj + 1;
+                    if (gi <= F.ni && gj <= F.nj) sum += rs[ti * TJ + tj];
+                }
+            cbx[(size_t)I * cld + J] = sum;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ coarse levels in one workgroup
+template <class T>
+struct TailArgs {
+    int nt;                          // number of tail levels (the last one is the coarsest)
+    int ni[TAIL_MAX_LEVELS], nj[TAIL_MAX_LEVELS], off[TAIL_MAX_LEVELS];  // LDS cell offset per level
+    int cx[TAIL_MAX_LEVELS], cy[TAIL_MAX_LEVELS];
+    int gld[TAIL_MAX_LEVELS];        // global row pitch per level
+    const T* aP[TAIL_MAX_LEVELS];
+    const T* aE[TAIL_MAX_LEVELS];
+    const T* aW[TAIL_MAX_LEVELS];
+    const T* aN[TAIL_MAX_LEVELS];
+    const T* aS[TAIL_MAX_LEVELS];
+    const unsigned char* act[TAIL_MAX_LEVELS];
+    int ncell;                       // total LDS cells
+    const T* b_in;                   // right-hand side of the first tail level (global)
+    T* x_out;                        // its solution (global)
+    const double* w;                 // restriction weights when the tail starts at level 0
+};
+
+template <class T>
+__device__ inline void tail_smooth(T* P, T* E, T* W, T* N, T* S, T* B, T* X, int ni, int nj, int color)
+{
+    const int ld = nj + 2;
+    for (int q = threadIdx.x; q < ni * nj; q += TAIL_NT) {
+        const int i = 1 + q / nj, j = 1 + q % nj;
+        if (((i + j) & 1) != color) continue;
+        const int c = i * ld + j;
+        X[c] = (B[c] + E[c] * X[c + ld] + W[c] * X[c - ld] + N[c] * X[c + 1] + S[c] * X[c - 1]) / P[c];
+    }
+    __syncthreads();
+}
+
+template <class T>
+__global__ void __launch_bounds__(TAIL_NT) k_tail(TailArgs<T> A)
+{
+    extern __shared__ unsigned char tail_lds[];
+    T* base = reinterpret_cast<T*>(tail_lds);
+    const int nc = A.ncell;
+    T* P = base;
+    T* E = base + nc;
+    T* W = base + 2 * nc;
+    T* N = base + 3 * nc;
+    T* S = base + 4 * nc;
+    T* B = base + 5 * nc;
+    T* X = base + 6 * nc;
+    unsigned char* ACT = reinterpret_cast<unsigned char*>(base + 7 * nc);
+    // load every tail level (ghost ring: identity rows, x = b = 0)
+    for (int l = 0; l < A.nt; l++) {
+        const int ni = A.ni[l], nj = A.nj[l], ld = nj + 2, o = A.off[l];
+        for (int q = threadIdx.x; q < (ni + 2) * ld; q += TAIL_NT) {
+            const int i = q / ld, j = q % ld;
+            const int c = o + q;
+            bool a = false;
+            if (i >= 1 && i <= ni && j >= 1 && j <= nj) {
+                const size_t g = (size_t)i * A.gld[l] + j;
+                a = A.act[l][g] != 0;
+                if (a) {
+                    P[c] = A.aP[l][g];
+                    E[c] = A.aE[l][g];
+                    W[c] = A.aW[l][g];
+                    N[c] = A.aN[l][g];
+                    S[c] = A.aS[l][g];
+                }
+                B[c] = (l == 0 && a) ? A.b_in[g] : T(0);
+            } else {
+                B[c] = T(0);
+            }
+            if (!a) {
+                P[c] = T(1);
+                E[c] = W[c] = N[c] = S[c] = T(0);
+            }
+            X[c] = T(0);
+            ACT[c] = a ? 1 : 0;
+        }
+    }
+    __syncthreads();
+    // down
+    for (int l = 0; l + 1 < A.nt; l++) {
+        const int ni = A.ni[l], nj = A.nj[l], ld = nj + 2, o = A.off[l];
+        for (int sw = 0; sw < 4; sw++) tail_smooth(P + o, E + o, W + o, N + o, S + o, B + o, X + o, ni, nj, sw & 1);
+        const int fi = A.cx[l] ? 2 : 1, fj = A.cy[l] ? 2 : 1;
+        const int cni = A.ni[l + 1], cnj = A.nj[l + 1], cld = cnj + 2, co = A.off[l + 1];
+        for (int q = threadIdx.x; q < cni * cnj; q += TAIL_NT) {
+            const int I = 1 + q / cnj, J = 1 + q % cnj;
+            T sum = T(0);
+            for (int a = 0; a < fi; a++)
+                for (int bb = 0; bb < fj; bb++) {
+                    const int i = fi * (I - 1) + 1 + a, j = fj * (J - 1) + 1 + bb;
+                    if (i > ni || j > nj) continue;
+                    const int c = o + i * ld + j;
+                    T r = B[c] - P[c] * X[c] + E[c] * X[c + ld] + W[c] * X[c - ld] + N[c] * X[c + 1] + S[c] * X[c - 1];
+                    if (l == 0 && A.w) r = r * (T)A.w[j];
+                    sum += r;
+                }
+            B[co + I * cld + J] = sum;
+        }
+        __syncthreads();
+    }
+    // coarsest: dense Gaussian elimination with partial pivoting (<= 8 unknowns)
+    {
+        const int l = A.nt - 1;
+        const int ni = A.ni[l], nj = A.nj[l], ld = nj + 2, o = A.off[l];
+        if (threadIdx.x == 0) {
+            T M[COARSEST_MAX][COARSEST_MAX + 1];
+            int cell[COARSEST_MAX];
+            int n = 0;
+            int id[(COARSEST_MAX + 2) * (COARSEST_MAX + 2)];
+            for (int q = 0; q < (ni + 2) * ld; q++) id[q] = -1;
+            for (int i = 1; i <= ni; i++)
+                for (int j = 1; j <= nj; j++)
+                    if (ACT[o + i * ld + j]) {
+                        id[i * ld + j] = n;
+                        cell[n++] = i * ld + j;
+                    }
+            for (int r = 0; r < n; r++)
+                for (int c = 0; c <= n; c++) M[r][c] = T(0);
+            for (int r = 0; r < n; r++) {
+                const int q = cell[r], c = o + q;
+                M[r][r] = P[c];
+                M[r][n] = B[c];
+                const int nb[4] = {id[q + ld], id[q - ld], id[q + 1], id[q - 1]};
+                const T a[4] = {E[c], W[c], N[c], S[c]};
+                for (int t = 0; t < 4; t++)
+                    if (nb[t] >= 0) M[r][nb[t]] = M[r][nb[t]] - a[t];
+            }
+            for (int k = 0; k < n; k++) {
+                int p = k;
+                for (int r = k + 1; r < n; r++)
+                    if (abs2v(M[r][k]) > abs2v(M[p][k])) p = r;
+                if (p != k)
+                    for (int c = 0; c <= n; c++) {
+                        T t = M[k][c];
+                        M[k][c] = M[p][c];
+                        M[p][c] = t;
+                    }
+                for (int r = k + 1; r < n; r++) {
+                    const T f = M[r][k] / M[k][k];
+                    for (int c = k; c <= n; c++) M[r][c] = M[r][c] - f * M[k][c];
+                }
+            }
+            for (int k = n - 1; k >= 0; k--) {
+                T v = M[k][n];
+                for (int c = k + 1; c < n; c++) v = v - M[k][c] * X[o + cell[c]];
+                X[o + cell[k]] = v / M[k][k];
+            }
+        }
+        __syncthreads();
+    }
+    // up
+    for (int l = A.nt - 2; l >= 0; l--) {
+        const int ni = A.ni[l], nj = A.nj[l], ld = nj + 2, o = A.off[l];
+        const int cld = A.nj[l + 1] + 2, co = A.off[l + 1];
+        for (int q = threadIdx.x; q < ni * nj; q += TAIL_NT) {
+            const int i = 1 + q / nj, j = 1 + q % nj;
+            const int c = o + i * ld + j;
+            if (!ACT[c]) continue;
+            const int I = A.cx[l] ? (i + 1) >> 1 : i, J = A.cy[l] ? (j + 1) >> 1 : j;
+            X[c] = X[c] + X[co + I * cld + J];
+        }
+        __syncthreads();
+        for (int sw = 0; sw < 4; sw++) tail_smooth(P + o, E + o, W + o, N + o, S + o, B + o, X + o, ni, nj, sw & 1);
+    }
+    // write the first tail level
+    {
+        const int ni = A.ni[0], nj = A.nj[0], ld = nj + 2;
+        for (int q = threadIdx.x; q < ni * nj; q += TAIL_NT) {
+            const int i = 1 + q / nj, j = 1 + q % nj;
+            A.x_out[(size_t)i * A.gld[0] + j] = X[i * ld + j];
+        }
+    }
+}
+
+}  // namespace cathode

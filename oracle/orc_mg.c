@@ -20,7 +20,7 @@
  *   smoother   red-black point Gauss-Seidel, nu1 = nu2 = 2 sweeps;
  *   transfers  restriction = w-weighted sum of the children's residuals,
  *              prolongation = piecewise constant;
- *   coarsest   dense LU (<= 64 unknowns);
+ *   coarsest   dense LU (<= 8 unknowns);
  *   outer      BiCGStab right-preconditioned by one V-cycle per application,
  *              from x = 0 (solverFuncs.hpp:106-109), stopping when the true
  *              residual ||b-Ax||_2/||b||_2 <= tol or after maxITR iterations.
@@ -35,7 +35,7 @@
 #include "oracle.h"
 
 #define MG_MAX_LEVELS 24
-#define MG_COARSE_MAX 64
+static int MG_COARSE_MAX = 8;
 
 typedef struct {
     int ni, nj, ld;
@@ -48,6 +48,7 @@ typedef struct {
 
 static int g_threads = 1;
 void orc_mg_set_threads(int n) { g_threads = n > 0 ? n : 1; }
+void orc_mg_set_coarse_max(int n) { MG_COARSE_MAX = n > 0 ? n : 1; }
 
 #define L_AT(L, a, i, j) ((L)->a[(size_t)(i) * (L)->ld + (j)])
 
