@@ -181,6 +181,8 @@ enum {
     SC_R0V = 9,
     SC_TS = 10,
     SC_TT = 11,
+    SC_RT = 12,   // |b - A x|^2 (true residual)
+    SC_SIDE = 15, // caller's value read with the first round trip (see MGSolver::side_slot)
     SC_COUNT = 16
 };
 
@@ -323,33 +325,46 @@ __global__ void k_bicg_s(int ni, int nj, int ld, const unsigned char* __restrict
     vx[k] = 0;
 }
 
-// x += al ph + om sh; r = s - om t; partials |r|^2 (slot 0), (r0, r) (slot 1)
-__global__ void k_bicg_x(int ni, int nj, int ld, const unsigned char* __restrict__ act,
-                         double* __restrict__ x, double* __restrict__ r, const double* __restrict__ r0,
+// x_new = x + al ph + om sh (into the other x buffer); r = s - om t;
+// partials |r|^2 (slot 0), (r0, r) (slot 1) and the true residual
+// |b - A x_new|^2 (slot 2), evaluating the neighbours' x_new from the old buffer
+__global__ void k_bicg_x(int ni, int nj, int ld, const double* __restrict__ aP, const double* __restrict__ aE,
+                         const double* __restrict__ aW, const double* __restrict__ aN,
+                         const double* __restrict__ aS, const double* __restrict__ bv,
+                         const unsigned char* __restrict__ act, const double* __restrict__ x,
+                         double* __restrict__ xn, double* __restrict__ r, const double* __restrict__ r0,
                          const double* __restrict__ ph, const double* __restrict__ sh,
                          const double* __restrict__ sv, const double* __restrict__ t,
                          const double* __restrict__ sc, double* __restrict__ part, int nb)
 {
     __shared__ double lds[8];
     CELL_IJ;
-    double a = 0.0, c = 0.0;
+    double a = 0.0, c = 0.0, e = 0.0;
     if (i <= ni && j <= nj) {
         const size_t k = (size_t)i * ld + j;
+        const double al = sc[SC_AL], om = sc[SC_OM];
+        double xk = 0.0;
         if (act[k]) {
-            const double al = sc[SC_AL], om = sc[SC_OM];
-            x[k] = x[k] + al * ph[k] + om * sh[k];
+            xk = x[k] + al * ph[k] + om * sh[k];
             const double q = sv[k] - om * t[k];
             r[k] = q;
             a = q * q;
             c = r0[k] * q;
+            auto xnew = [&](size_t m) { return act[m] ? x[m] + al * ph[m] + om * sh[m] : 0.0; };
+            const double rt = bv[k] - aP[k] * xk + aE[k] * xnew(k + ld) + aW[k] * xnew(k - ld) +
+                              aN[k] * xnew(k + 1) + aS[k] * xnew(k - 1);
+            e = rt * rt;
         }
+        xn[k] = xk;
     }
     double t1 = block_sum<BX * BY>(a, lds);
     double t2 = block_sum<BX * BY>(c, lds);
+    double t3 = block_sum<BX * BY>(e, lds);
     if (lin_tid() == 0) {
         const int bid = blockIdx.y * gridDim.x + blockIdx.x;
         part[bid] = t1;
         part[nb + bid] = t2;
+        part[2 * nb + bid] = t3;
     }
 }
 
@@ -369,6 +384,7 @@ __global__ void k_bicg_finish(const double* __restrict__ part, int nb, double* _
     __shared__ double lds[8];
     const double a = sum_part(part, nb, lds);
     const double b = (what == 1 || what == 3 || what == 4) ? sum_part(part + nb, nb, lds) : 0.0;
+    const double c = what == 4 ? sum_part(part + 2 * nb, nb, lds) : 0.0;
     if (threadIdx.x != 0) return;
     switch (what) {
     case 0:
@@ -377,24 +393,28 @@ __global__ void k_bicg_finish(const double* __restrict__ part, int nb, double* _
         break;
     case 1:
         sc[SC_RR] = a;
+        sc[SC_RT] = a;
         sc[SC_RHO1] = b;
         sc[SC_RHO] = 1.0;
         sc[SC_AL] = 1.0;
         sc[SC_OM] = 1.0;
         break;
-    case 2:
+    case 2: {
         sc[SC_R0V] = a;
-        sc[SC_AL] = sc[SC_RHO1] / a;
+        const double al = sc[SC_RHO1] / a;
+        sc[SC_AL] = (a != 0.0 && isfinite(al)) ? al : 0.0;  // breakdown: no step
         break;
+    }
     case 3:
         sc[SC_TS] = a;
         sc[SC_TT] = b;
-        sc[SC_OM] = b > 0.0 ? a / b : 0.0;
+        sc[SC_OM] = (b > 0.0 && isfinite(a / b)) ? a / b : 0.0;
         break;
     case 4:
         sc[SC_RR] = a;
         sc[SC_RHO] = sc[SC_RHO1];
         sc[SC_RHO1] = b;
+        sc[SC_RT] = c;
         break;
     }
 }
@@ -508,6 +528,8 @@ void MGSolver::configure(int ni, int nj, hipStream_t s)
     CATHODE_HIP(hipMemset(act0.p, 0, n0));
     bsol.alloc(n0);
     xsol.alloc(n0);
+    xalt.alloc(n0);
+    CATHODE_HIP(hipMemset(xalt.p, 0, n0 * sizeof(double)));
     CATHODE_HIP(hipMemset(bsol.p, 0, n0 * sizeof(double)));
     CATHODE_HIP(hipMemset(xsol.p, 0, n0 * sizeof(double)));
     w0.alloc(nj + 2);
@@ -517,7 +539,7 @@ void MGSolver::configure(int ni, int nj, hipStream_t s)
     if (nl - lt > TAIL_MAX_LEVELS) lt = nl - TAIL_MAX_LEVELS;
     tail_cells = 0;
     for (int l = lt; l < nl; l++) tail_cells += (dims[l][0] + 2) * (dims[l][1] + 2);
-    part.alloc(2 * (cdiv(nj, BX) * cdiv(ni, BY)) + 64);
+    part.alloc(3 * (cdiv(nj, BX) * cdiv(ni, BY)) + 64);
     scal.alloc(16);
     kv.alloc(8 * n0);
     CATHODE_HIP(hipMemset(kv.p, 0, 8 * n0 * sizeof(double)));
@@ -664,7 +686,6 @@ int MGSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_sta
                        (double)F.ni * (double)F.nj);
     CATHODE_LAUNCH_CHECK();
     CATHODE_HIP(hipMemsetAsync(xsol.p, 0, n0 * sizeof(double), s));
-    const double bb = read_scalar(scal.p + SC_BB, s);
     double* r = kv.p;
     double* r0 = r + n0;
     double* p = r0 + n0;
@@ -673,69 +694,79 @@ int MGSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_sta
     double* t = sv + n0;
     double* ph = t + n0;
     double* sh = ph + n0;
+    aborted = false;
+    double* xa = xsol.p;   // current iterate
+    double* xb = xalt.p;   // next iterate (double buffer)
     int it = 0;
-    double rel = 0.0;
+    double rel = 0.0, bb = -1.0;
     bool restart = true;
-    if (!(bb > 0.0)) {
-        rel = bb == 0.0 ? 0.0 : NAN;  // b = 0: HYPRE returns x = 0
-    } else {
-        while (it < maxITR) {
-            if (restart) {
-                hipLaunchKernelGGL(k_bicg_restart, g, blk, 0, s, F.ni, F.nj, F.ld, F.aP, F.aE, F.aW, F.aN, F.aS,
-                                   bsol.p, xsol.p, act0.p, r, r0, p, v, part.p, nb);
-                hipLaunchKernelGGL(k_bicg_finish, dim3(1), dim3(256), 0, s, part.p, nb, scal.p, 1, 0.0);
-            }
-            it++;
-            // p, ph = M^-1 p, v = A ph, al
-            if (mixed) {
-                hipLaunchKernelGGL(k_bicg_p<float>, g, blk, 0, s, F.ni, F.nj, F.ld, act0.p, r, v, p, L32[0].b,
-                                   L32[0].x, scal.p, restart ? 1 : 0, 1);
-                vcycle<float>(L32, s);
-                hipLaunchKernelGGL(k_bicg_apply<float>, g, blk, 0, s, F.ni, F.nj, F.ld, F.aP, F.aE, F.aW, F.aN,
-                                   F.aS, act0.p, L32[0].x, ph, v, r0, scal.p, 1, 0, part.p, nb);
-            } else {
-                hipLaunchKernelGGL(k_bicg_p<double>, g, blk, 0, s, F.ni, F.nj, F.ld, act0.p, r, v, p, F.b, F.x,
-                                   scal.p, restart ? 1 : 0, 0);
-                vcycle<double>(L64, s);
-                hipLaunchKernelGGL(k_bicg_apply<double>, g, blk, 0, s, F.ni, F.nj, F.ld, F.aP, F.aE, F.aW, F.aN,
-                                   F.aS, act0.p, F.x, ph, v, r0, scal.p, 0, 0, part.p, nb);
-            }
-            hipLaunchKernelGGL(k_bicg_finish, dim3(1), dim3(256), 0, s, part.p, nb, scal.p, 2, 0.0);
-            restart = false;
-            // s, sh = M^-1 s, t = A sh, om
-            if (mixed) {
-                hipLaunchKernelGGL(k_bicg_s<float>, g, blk, 0, s, F.ni, F.nj, F.ld, act0.p, r, v, sv, L32[0].b,
-                                   L32[0].x, scal.p, 1);
-                vcycle<float>(L32, s);
-                hipLaunchKernelGGL(k_bicg_apply<float>, g, blk, 0, s, F.ni, F.nj, F.ld, F.aP, F.aE, F.aW, F.aN,
-                                   F.aS, act0.p, L32[0].x, sh, t, sv, scal.p, 1, 1, part.p, nb);
-            } else {
-                hipLaunchKernelGGL(k_bicg_s<double>, g, blk, 0, s, F.ni, F.nj, F.ld, act0.p, r, v, sv, F.b, F.x,
-                                   scal.p, 0);
-                vcycle<double>(L64, s);
-                hipLaunchKernelGGL(k_bicg_apply<double>, g, blk, 0, s, F.ni, F.nj, F.ld, F.aP, F.aE, F.aW, F.aN,
-                                   F.aS, act0.p, F.x, sh, t, sv, scal.p, 0, 1, part.p, nb);
-            }
-            hipLaunchKernelGGL(k_bicg_finish, dim3(1), dim3(256), 0, s, part.p, nb, scal.p, 3, 0.0);
-            hipLaunchKernelGGL(k_bicg_x, g, blk, 0, s, F.ni, F.nj, F.ld, act0.p, xsol.p, r, r0, ph, sh, sv, t,
-                               scal.p, part.p, nb);
-            hipLaunchKernelGGL(k_bicg_finish, dim3(1), dim3(256), 0, s, part.p, nb, scal.p, 4, 0.0);
-            CATHODE_LAUNCH_CHECK();
-            const double rr = read_scalar(scal.p + SC_RR, s);
-            rel = sqrt(rr / bb);
-            if (!std::isfinite(rel)) break;
-            if (rel <= tol) {
-                // confirm with the true residual b - A x (the recursion drifts)
-                hipLaunchKernelGGL(k_bicg_restart, g, blk, 0, s, F.ni, F.nj, F.ld, F.aP, F.aE, F.aW, F.aN, F.aS,
-                                   bsol.p, xsol.p, act0.p, r, r0, p, v, part.p, nb);
-                hipLaunchKernelGGL(k_bicg_finish, dim3(1), dim3(256), 0, s, part.p, nb, scal.p, 1, 0.0);
-                const double rt = read_scalar(scal.p + SC_RR, s);
-                rel = sqrt(rt / bb);
-                if (rel <= tol || !std::isfinite(rel)) break;
-                restart = true;  // continue from the true residual
-            }
+    while (it < maxITR) {
+        if (restart) {
+            hipLaunchKernelGGL(k_bicg_restart, g, blk, 0, s, F.ni, F.nj, F.ld, F.aP, F.aE, F.aW, F.aN, F.aS,
+                               bsol.p, xa, act0.p, r, r0, p, v, part.p, nb);
+            hipLaunchKernelGGL(k_bicg_finish, dim3(1), dim3(256), 0, s, part.p, nb, scal.p, 1, 0.0);
         }
+        it++;
+        // p, ph = M^-1 p, v = A ph, al
+        if (mixed) {
+            hipLaunchKernelGGL(k_bicg_p<float>, g, blk, 0, s, F.ni, F.nj, F.ld, act0.p, r, v, p, L32[0].b,
+                               L32[0].x, scal.p, restart ? 1 : 0, 1);
+            vcycle<float>(L32, s);
+            hipLaunchKernelGGL(k_bicg_apply<float>, g, blk, 0, s, F.ni, F.nj, F.ld, F.aP, F.aE, F.aW, F.aN,
+                               F.aS, act0.p, L32[0].x, ph, v, r0, scal.p, 1, 0, part.p, nb);
+        } else {
+            hipLaunchKernelGGL(k_bicg_p<double>, g, blk, 0, s, F.ni, F.nj, F.ld, act0.p, r, v, p, F.b, F.x,
+                               scal.p, restart ? 1 : 0, 0);
+            vcycle<double>(L64, s);
+            hipLaunchKernelGGL(k_bicg_apply<double>, g, blk, 0, s, F.ni, F.nj, F.ld, F.aP, F.aE, F.aW, F.aN,
+                               F.aS, act0.p, F.x, ph, v, r0, scal.p, 0, 0, part.p, nb);
+        }
+        hipLaunchKernelGGL(k_bicg_finish, dim3(1), dim3(256), 0, s, part.p, nb, scal.p, 2, 0.0);
+        restart = false;
+        // s, sh = M^-1 s, t = A sh, om
+        if (mixed) {
+            hipLaunchKernelGGL(k_bicg_s<float>, g, blk, 0, s, F.ni, F.nj, F.ld, act0.p, r, v, sv, L32[0].b,
+                               L32[0].x, scal.p, 1);
+            vcycle<float>(L32, s);
+            hipLaunchKernelGGL(k_bicg_apply<float>, g, blk, 0, s, F.ni, F.nj, F.ld, F.aP, F.aE, F.aW, F.aN,
+                               F.aS, act0.p, L32[0].x, sh, t, sv, scal.p, 1, 1, part.p, nb);
+        } else {
+            hipLaunchKernelGGL(k_bicg_s<double>, g, blk, 0, s, F.ni, F.nj, F.ld, act0.p, r, v, sv, F.b, F.x,
+                               scal.p, 0);
+            vcycle<double>(L64, s);
+            hipLaunchKernelGGL(k_bicg_apply<double>, g, blk, 0, s, F.ni, F.nj, F.ld, F.aP, F.aE, F.aW, F.aN,
+                               F.aS, act0.p, F.x, sh, t, sv, scal.p, 0, 1, part.p, nb);
+        }
+        hipLaunchKernelGGL(k_bicg_finish, dim3(1), dim3(256), 0, s, part.p, nb, scal.p, 3, 0.0);
+        // x, recursive and true residuals
+        hipLaunchKernelGGL(k_bicg_x, g, blk, 0, s, F.ni, F.nj, F.ld, F.aP, F.aE, F.aW, F.aN, F.aS, bsol.p, act0.p,
+                           xa, xb, r, r0, ph, sh, sv, t, scal.p, part.p, nb);
+        hipLaunchKernelGGL(k_bicg_finish, dim3(1), dim3(256), 0, s, part.p, nb, scal.p, 4, 0.0);
+        CATHODE_LAUNCH_CHECK();
+        std::swap(xa, xb);
+        // one host round trip per iteration: |b|^2, |r|^2, |b - A x|^2
+        CATHODE_HIP(hipMemcpyAsync(h_scal, scal.p, SC_COUNT * sizeof(double), hipMemcpyDeviceToHost, s));
+        CATHODE_HIP(hipStreamSynchronize(s));
+        bb = h_scal[SC_BB];
+        if (it == 1 && h_scal[SC_SIDE] < side_cnv) {
+            // the caller's stopping test (read with this round trip) says the
+            // solve is not wanted: x is left as it is, the caller rolls back
+            aborted = true;
+            break;
+        }
+        if (!(bb > 0.0)) {
+            rel = bb == 0.0 ? 0.0 : NAN;  // b = 0: HYPRE returns x = 0 (the step above added 0)
+            if (bb == 0.0) it = 0;
+            break;
+        }
+        const double rt = sqrt(h_scal[SC_RT] / bb);
+        rel = rt;
+        if (!std::isfinite(rel)) break;
+        if (rt <= tol) break;
+        // the recursion has drifted from the true residual: restart from it
+        if (sqrt(h_scal[SC_RR] / bb) <= tol) restart = true;
     }
+    if (xa != xsol.p) CATHODE_HIP(hipMemcpyAsync(xsol.p, xa, n0 * sizeof(double), hipMemcpyDeviceToDevice, s));
     if (prof.enabled) {
         CATHODE_HIP(hipStreamSynchronize(s));
         for (size_t k = 0; k + 1 < prof.ev.size(); k += 2) {

@@ -13,7 +13,7 @@
 //              part s - m (halved); 5-point on every level;
 //   smoother   red-black point Gauss-Seidel, V(2,2);
 //   transfers  w-weighted sum restriction, piecewise-constant prolongation;
-//   coarsest   <= 8 unknowns, Gaussian elimination with partial pivoting;
+//   coarsest   a single cell (x = b / aP);
 //   kernels    one fused down kernel (2 sweeps + residual + restriction) and
 //              one fused up kernel (prolongation + 2 sweeps) per fine level,
 //              and one workgroup for all levels of <= 512 cells (vcycle.hpp).
@@ -75,7 +75,15 @@ public:
     // Solves A x = rhs from x = 0 into sol().  Returns iterations; fills st.
     int solve(int maxITR, double tol, int mode, hipStream_t s, cathode_status* st);
 
-    MGProfile prof;  // events around the level-0 smoothing launches
+    MGProfile prof;  // events around the level-0 fused V-cycle launches
+
+    // A device double the caller writes before solve(); it travels with the
+    // solve's first host round trip.  When it is below side_cnv the solve
+    // stops after that round trip with aborted = true (NaN never stops it).
+    // side_cnv = -inf disables the test.
+    double* side_slot() { return scal.p + 15; }
+    double side_cnv = -INFINITY;
+    bool aborted = false;
 
     int levels() const { return nl; }
 
@@ -97,7 +105,7 @@ private:
     DevBuf<unsigned char> actall;  // activity per level
     size_t level_off[MG_MAX_LEVELS + 1] = {0};
     DevBuf<unsigned char> act0;
-    DevBuf<double> bsol, xsol, w0;
+    DevBuf<double> bsol, xsol, xalt, w0;
     const double* user_w = nullptr;
     int lt = 0;                // first level handled by the one-workgroup tail kernel
     int tail_cells = 0;        // LDS cells of the tail levels

@@ -733,11 +733,29 @@ static void account(NddHandle& H, const cathode_status& st)
     if (!st.converged) H.rep.nonconverged++;
 }
 
-static void solve(NddHandle& H)
+// gate: the PC stopping threshold the previous iteration's calcRes is
+// tested against (-inf for the first iteration).  Returns false when the
+// loop should have stopped before this iteration.
+static bool solve(NddHandle& H, double gate)
 {
     cathode_status st{};
+    H.mg.side_cnv = gate;
     H.mg.solve(H.p.maxITR, H.p.tol, H.p.mode, H.s, &st);
+    H.mg.side_cnv = -INFINITY;
+    if (H.mg.aborted) return false;
     account(H, st);
+    return true;
+}
+
+// calcRes of three pairs without a host round trip: the max error lands in
+// the multigrid's side slot and is read with the next solve's first sync
+static void calcres3_async(NddHandle& H, const ResPairs& P)
+{
+    const NddView& V = H.V;
+    hipLaunchKernelGGL(k_calcres, ggrid(V.ni, V.nj), gblk(), 0, H.s, V, P, H.p.flag_error, 1e-100, H.part.p, H.nb);
+    hipLaunchKernelGGL(k_calcres_finish, dim3(1), dim3(256), 0, H.s, H.part.p, H.nb, H.p.flag_error, H.scal.p);
+    CATHODE_HIP(hipMemcpyAsync(H.mg.side_slot(), H.scal.p + 3, sizeof(double), hipMemcpyDeviceToDevice, H.s));
+    CATHODE_LAUNCH_CHECK();
 }
 
 // calcRes of three pairs; returns the max error (host sync); errors also in scal[0..3]
@@ -781,7 +799,12 @@ static void poisson_routine(NddHandle& H, bool report)
         hipLaunchKernelGGL(k_poisson_assemble, ggrid(V.ni, V.nj), gblk(), 0, H.s, V, A, H.mg.aP(), H.mg.aE(),
                            H.mg.aW(), H.mg.aN(), H.mg.aS(), H.mg.rhs());
         CATHODE_LAUNCH_CHECK();
-        solve(H);
+        if (!solve(H, ncount > 1 ? H.p.error_cnv_HES_phi : -INFINITY)) {
+            // calcRes of the previous iteration met the PC criterion: undo the prediction
+            d2d(H, F[F_NUEX], H.tmp_x);
+            d2d(H, F[F_NUER], H.tmp_r);
+            break;
+        }
         hipLaunchKernelGGL(k_poisson_update_phi, ggrid(V.ni + 2, V.nj + 2), gblk(), 0, H.s, V, H.mg.sol(), H.dphi,
                            F[F_PHI], F[F_EX], F[F_ER]);
         hipLaunchKernelGGL(k_poisson_fields, ggrid(V.ni, V.nj), gblk(), 0, H.s, V, F[F_PHI], H.dphi, F[F_RHOE],
@@ -789,7 +812,7 @@ static void poisson_routine(NddHandle& H, bool report)
         CATHODE_LAUNCH_CHECK();
         if (ncount >= H.p.maxITR_HES_phi) break;
         ResPairs P{F[F_PHI], H.tmp_c, F[F_NUEX], H.tmp_x, F[F_NUER], H.tmp_r};
-        if (calcres3(H, P, nullptr) < H.p.error_cnv_HES_phi) break;
+        calcres3_async(H, P);
     }
     if (report) {
         double e[3];
@@ -842,12 +865,16 @@ static void sg_routine(NddHandle& H, int sp, bool report, double* err3)
         hipLaunchKernelGGL(k_sg_assemble, ggrid(V.ni, V.nj), gblk(), 0, H.s, V, A, H.mg.aP(), H.mg.aE(), H.mg.aW(),
                            H.mg.aN(), H.mg.aS(), H.mg.rhs());
         CATHODE_LAUNCH_CHECK();
-        solve(H);
+        if (!solve(H, ncount > 1 ? cnv : -INFINITY)) {
+            d2d(H, A.Fx, H.tmp_x);
+            d2d(H, A.Fr, H.tmp_r);
+            break;
+        }
         hipLaunchKernelGGL(k_sg_correct, ggrid(V.ni, V.nj), gblk(), 0, H.s, V, A, H.mg.sol());
         CATHODE_LAUNCH_CHECK();
         if (ncount >= maxit) break;
         ResPairs P{A.n, H.tmp_c, A.Fx, H.tmp_x, A.Fr, H.tmp_r};
-        if (calcres3(H, P, nullptr) < cnv) break;
+        calcres3_async(H, P);
     }
     if (report) {
         ResPairs P{A.n, n_old, A.Fx, H.sx_old, A.Fr, H.sr_old};

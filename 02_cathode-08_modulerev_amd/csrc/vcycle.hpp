@@ -12,7 +12,7 @@
 //   k_up    the same tile with a halo of 4: x = x_down + x_coarse(parent)
 //           (piecewise-constant prolongation), then two red-black sweeps.
 // Coarse levels (<= TAIL_MAX_CELLS cells each) run in one workgroup, all in
-// LDS (k_tail), ending with a direct solve of the coarsest (<= 8 unknowns).
+// LDS (k_tail), down to a single coarsest cell solved by one division.
 #pragma once
 
 namespace cathode {
@@ -22,7 +22,7 @@ constexpr int VT_NT = 256;              // threads of a tile workgroup
 constexpr int TAIL_MAX_CELLS = 512;     // levels with at most this many cells go to the tail
 constexpr int TAIL_NT = 512;            // threads of the tail workgroup
 constexpr int TAIL_MAX_LEVELS = 16;
-constexpr int COARSEST_MAX = 8;         // unknowns of the coarsest direct solve
+constexpr int COARSEST_MAX = 1;         // cells of the coarsest level (solved by one division)
 
 template <class T>
 struct VLevel {
@@ -228,53 +228,13 @@ __global__ void __launch_bounds__(TAIL_NT) k_tail(TailArgs<T> A)
         }
         __syncthreads();
     }
-    // coarsest: dense Gaussian elimination with partial pivoting (<= 8 unknowns)
+    // coarsest level: a single cell, x = b / aP (inactive or zero diagonal: 0)
     {
         const int l = A.nt - 1;
         const int ni = A.ni[l], nj = A.nj[l], ld = nj + 2, o = A.off[l];
-        if (threadIdx.x == 0) {
-            T M[COARSEST_MAX][COARSEST_MAX + 1];
-            int cell[COARSEST_MAX];
-            int n = 0;
-            int id[(COARSEST_MAX + 2) * (COARSEST_MAX + 2)];
-            for (int q = 0; q < (ni + 2) * ld; q++) id[q] = -1;
-            for (int i = 1; i <= ni; i++)
-                for (int j = 1; j <= nj; j++)
-                    if (ACT[o + i * ld + j]) {
-                        id[i * ld + j] = n;
-                        cell[n++] = i * ld + j;
-                    }
-            for (int r = 0; r < n; r++)
-                for (int c = 0; c <= n; c++) M[r][c] = T(0);
-            for (int r = 0; r < n; r++) {
-                const int q = cell[r], c = o + q;
-                M[r][r] = P[c];
-                M[r][n] = B[c];
-                const int nb[4] = {id[q + ld], id[q - ld], id[q + 1], id[q - 1]};
-                const T a[4] = {E[c], W[c], N[c], S[c]};
-                for (int t = 0; t < 4; t++)
-                    if (nb[t] >= 0) M[r][nb[t]] = M[r][nb[t]] - a[t];
-            }
-            for (int k = 0; k < n; k++) {
-                int p = k;
-                for (int r = k + 1; r < n; r++)
-                    if (abs2v(M[r][k]) > abs2v(M[p][k])) p = r;
-                if (p != k)
-                    for (int c = 0; c <= n; c++) {
-                        T t = M[k][c];
-                        M[k][c] = M[p][c];
-                        M[p][c] = t;
-                    }
-                for (int r = k + 1; r < n; r++) {
-                    const T f = M[r][k] / M[k][k];
-                    for (int c = k; c <= n; c++) M[r][c] = M[r][c] - f * M[k][c];
-                }
-            }
-            for (int k = n - 1; k >= 0; k--) {
-                T v = M[k][n];
-                for (int c = k + 1; c < n; c++) v = v - M[k][c] * X[o + cell[c]];
-                X[o + cell[k]] = v / M[k][k];
-            }
+        for (int q = threadIdx.x; q < ni * nj; q += TAIL_NT) {
+            const int c = o + (1 + q / nj) * ld + 1 + q % nj;
+            X[c] = (ACT[c] && !(P[c] == T(0))) ? B[c] / P[c] : T(0);
         }
         __syncthreads();
     }

@@ -20,7 +20,7 @@
  *   smoother   red-black point Gauss-Seidel, nu1 = nu2 = 2 sweeps;
  *   transfers  restriction = w-weighted sum of the children's residuals,
  *              prolongation = piecewise constant;
- *   coarsest   dense LU (<= 8 unknowns);
+ *   coarsest   a single cell (dense LU of one unknown);
  *   outer      BiCGStab right-preconditioned by one V-cycle per application,
  *              from x = 0 (solverFuncs.hpp:106-109), stopping when the true
  *              residual ||b-Ax||_2/||b||_2 <= tol or after maxITR iterations.
@@ -35,7 +35,7 @@
 #include "oracle.h"
 
 #define MG_MAX_LEVELS 24
-static int MG_COARSE_MAX = 8;
+static int MG_COARSE_MAX = 1;
 
 typedef struct {
     int ni, nj, ld;
