@@ -18,30 +18,44 @@ static inline dim3 grid2(int ni, int nj) { return dim3(cdiv(nj, BX), cdiv(ni, BY
 
 // ------------------------------------------------------------------ level-0 setup
 // Zero couplings to inactive neighbours (HYPRE's off-grid convention), make
-// inactive cells identity rows, store the row excess s = max(aP - sum a, 0).
+// inactive cells identity rows, store the row excess s = max(aP - sum a, 0);
+// block partials of max|aP| (slot 0) and |b|^2 (slot 1) over active cells.
 __global__ void k_l0_prepare(int ni, int nj, int ld, const unsigned char* __restrict__ act,
                              double* aP, double* aE, double* aW, double* aN, double* aS,
-                             double* __restrict__ s_out)
+                             const double* __restrict__ bsol, double* __restrict__ s_out,
+                             double* __restrict__ part, int nb)
 {
+    __shared__ double lds[8];
     CELL_IJ;
-    if (i > ni || j > nj) return;
-    const size_t k = (size_t)i * ld + j;
-    if (!act[k]) {
-        aP[k] = 1.0;
-        aE[k] = aW[k] = aN[k] = aS[k] = 0.0;
-        s_out[k] = INFINITY;
-        return;
+    double amax = 0.0, bb = 0.0;
+    if (i <= ni && j <= nj) {
+        const size_t k = (size_t)i * ld + j;
+        if (!act[k]) {
+            aP[k] = 1.0;
+            aE[k] = aW[k] = aN[k] = aS[k] = 0.0;
+            s_out[k] = INFINITY;
+        } else {
+            double e = act[k + ld] ? aE[k] : 0.0;
+            double w = act[k - ld] ? aW[k] : 0.0;
+            double n = act[k + 1] ? aN[k] : 0.0;
+            double s = act[k - 1] ? aS[k] : 0.0;
+            aE[k] = e;
+            aW[k] = w;
+            aN[k] = n;
+            aS[k] = s;
+            double ex = aP[k] - (e + w + n + s);
+            s_out[k] = ex > 0.0 ? ex : 0.0;
+            amax = fabs(aP[k]);
+            bb = bsol[k] * bsol[k];
+        }
     }
-    double e = act[k + ld] ? aE[k] : 0.0;
-    double w = act[k - ld] ? aW[k] : 0.0;
-    double n = act[k + 1] ? aN[k] : 0.0;
-    double s = act[k - 1] ? aS[k] : 0.0;
-    aE[k] = e;
-    aW[k] = w;
-    aN[k] = n;
-    aS[k] = s;
-    double ex = aP[k] - (e + w + n + s);
-    s_out[k] = ex > 0.0 ? ex : 0.0;
+    const double t1 = block_max<BX * BY>(amax, lds);
+    const double t2 = block_sum<BX * BY>(bb, lds);
+    if (lin_tid() == 0) {
+        const int bid = blockIdx.y * gridDim.x + blockIdx.x;
+        part[bid] = t1;
+        part[nb + bid] = t2;
+    }
 }
 
 // Radial weights from column sums; one block.
@@ -77,14 +91,27 @@ __global__ void k_fill(double* p, int n, double v)
     if (k < n) p[k] = v;
 }
 
-// mass part m = w_j * min(s) over the 5-point neighbourhood
+// mass part m = w_j * min(s) over the 5-point neighbourhood; in mixed mode
+// also the FP32 copy (scaled by s32) of the level-0 coefficients
 __global__ void k_l0_mass(int ni, int nj, int ld, const unsigned char* __restrict__ act,
                           const double* __restrict__ s, const double* __restrict__ w,
-                          double* __restrict__ m)
+                          double* __restrict__ m, const double* __restrict__ aP, const double* __restrict__ aE,
+                          const double* __restrict__ aW, const double* __restrict__ aN,
+                          const double* __restrict__ aS, float* __restrict__ fP, float* __restrict__ fE,
+                          float* __restrict__ fW, float* __restrict__ fN, float* __restrict__ fS,
+                          const double* __restrict__ sc, int mixed)
 {
     CELL_IJ;
     if (i > ni || j > nj) return;
     const size_t k = (size_t)i * ld + j;
+    if (mixed) {
+        const double s32 = sc[3];
+        fP[k] = (float)(aP[k] * s32);
+        fE[k] = (float)(aE[k] * s32);
+        fW[k] = (float)(aW[k] * s32);
+        fN[k] = (float)(aN[k] * s32);
+        fS[k] = (float)(aS[k] * s32);
+    }
     if (!act[k]) {
         m[k] = 0.0;
         return;
@@ -97,66 +124,114 @@ __global__ void k_l0_mass(int ni, int nj, int ld, const unsigned char* __restric
     m[k] = w[j] * v;
 }
 
-// coarse operator of level F -> C (thread per coarse cell)
-__global__ void k_coarsen(int fni, int fnj, int fld, int fi, int fj,
-                          const double* __restrict__ faP, const double* __restrict__ faE,
-                          const double* __restrict__ faW, const double* __restrict__ faN,
-                          const double* __restrict__ faS, const double* __restrict__ fm,
-                          const unsigned char* __restrict__ fact, const double* __restrict__ w,
-                          int cni, int cnj, int cld, double* caP, double* caE, double* caW,
-                          double* caN, double* caS, double* cb, double* cx, double* cm,
-                          unsigned char* cact, double hx, double hy, double hb)
+// setup scalars: |b|^2, the mixed input scale and the FP32 matrix scale
+__global__ void k_setup_finish(const double* __restrict__ part, int nb, double* __restrict__ sc, double nact)
 {
-    const int J = 1 + blockIdx.x * BX + threadIdx.x;
-    const int I = 1 + blockIdx.y * BY + threadIdx.y;
-    if (I > cni || J > cnj) return;
-    const int i0 = fi * (I - 1) + 1, i1 = min(fi * I, fni);
-    const int j0 = fj * (J - 1) + 1, j1 = min(fj * J, fnj);
+    __shared__ double lds[8];
+    double m = 0.0, b = 0.0;
+    for (int k = threadIdx.x; k < nb; k += blockDim.x) {
+        m = fmax(m, part[k]);
+        b += part[nb + k];
+    }
+    m = block_max<256>(m, lds);
+    b = block_sum<256>(b, lds);
+    if (threadIdx.x == 0) {
+        sc[0] = b;                                  // SC_BB
+        sc[2] = b > 0.0 ? sqrt(nact / b) : 1.0;     // SC_SCALE
+        int e = 0;
+        (void)frexp(m > 0.0 ? m : 1.0, &e);
+        sc[3] = ldexp(1.0, -e);                     // SC_S32: s32 * max|aP| in [0.5, 1)
+    }
+}
+
+// coarse operator of coarse cell (I, J) from level F
+struct CoarsenArgs {
+    int fni, fnj, fld, fi, fj;
+    const double *faP, *faE, *faW, *faN, *faS, *fm;
+    const unsigned char* fact;
+    const double* w;
+    int cni, cnj, cld;
+    double *caP, *caE, *caW, *caN, *caS, *cb, *cx, *cm;
+    unsigned char* cact;
+    float *gP, *gE, *gW, *gN, *gS;  // FP32 copy of the coarse level (mixed)
+    double hx, hy, hb;
+};
+
+__device__ inline void coarsen_cell(const CoarsenArgs& a, int I, int J, const double* sc, int mixed)
+{
+    const int i0 = a.fi * (I - 1) + 1, i1 = min(a.fi * I, a.fni);
+    const int j0 = a.fj * (J - 1) + 1, j1 = min(a.fj * J, a.fnj);
     double S = 0.0, M = 0.0, E = 0.0, W = 0.0, N = 0.0, So = 0.0;
     int any = 0;
     for (int i = i0; i <= i1; i++)
         for (int j = j0; j <= j1; j++) {
-            const size_t k = (size_t)i * fld + j;
-            if (!fact[k]) continue;
+            const size_t k = (size_t)i * a.fld + j;
+            if (!a.fact[k]) continue;
             any = 1;
-            const double wj = w ? w[j] : 1.0;
-            const double e = faE[k], ww = faW[k], n = faN[k], s = faS[k];
-            const double ex = faP[k] - (e + ww + n + s);
+            const double wj = a.w ? a.w[j] : 1.0;
+            const double e = a.faE[k], ww = a.faW[k], n = a.faN[k], s = a.faS[k];
+            const double ex = a.faP[k] - (e + ww + n + s);
             S += wj * (ex > 0.0 ? ex : 0.0);
-            M += fm[k];
+            M += a.fm[k];
             if (i == i1) E += wj * e;
             if (i == i0) W += wj * ww;
             if (j == j1) N += wj * n;
             if (j == j0) So += wj * s;
         }
-    const size_t kc = (size_t)I * cld + J;
-    cact[kc] = (unsigned char)any;
-    cb[kc] = 0.0;
-    cx[kc] = 0.0;
+    const size_t kc = (size_t)I * a.cld + J;
+    a.cact[kc] = (unsigned char)any;
+    a.cb[kc] = 0.0;
+    a.cx[kc] = 0.0;
+    double P;
     if (!any) {
-        caP[kc] = 1.0;
-        caE[kc] = caW[kc] = caN[kc] = caS[kc] = 0.0;
-        cm[kc] = 0.0;
-        return;
+        P = 1.0;
+        E = W = N = So = 0.0;
+        a.cm[kc] = 0.0;
+    } else {
+        E *= a.hx;
+        W *= a.hx;
+        N *= a.hy;
+        So *= a.hy;
+        a.cm[kc] = M;
+        P = (M + a.hb * (S - M)) + (E + W + N + So);
     }
-    E *= hx;
-    W *= hx;
-    N *= hy;
-    So *= hy;
-    caE[kc] = E;
-    caW[kc] = W;
-    caN[kc] = N;
-    caS[kc] = So;
-    cm[kc] = M;
-    caP[kc] = (M + hb * (S - M)) + (E + W + N + So);
+    a.caP[kc] = P;
+    a.caE[kc] = E;
+    a.caW[kc] = W;
+    a.caN[kc] = N;
+    a.caS[kc] = So;
+    if (mixed) {
+        const double s32 = sc[3];
+        a.gP[kc] = (float)(P * s32);
+        a.gE[kc] = (float)(E * s32);
+        a.gW[kc] = (float)(W * s32);
+        a.gN[kc] = (float)(N * s32);
+        a.gS[kc] = (float)(So * s32);
+    }
 }
 
-__global__ void k_level_to_f32(size_t n, const double* __restrict__ src, float* __restrict__ dst,
-                               const double* __restrict__ scal)
+__global__ void k_coarsen(CoarsenArgs a, const double* __restrict__ sc, int mixed)
 {
-    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    dst[k] = (float)(src[k] * scal[3]);
+    const int J = 1 + blockIdx.x * BX + threadIdx.x;
+    const int I = 1 + blockIdx.y * BY + threadIdx.y;
+    if (I > a.cni || J > a.cnj) return;
+    coarsen_cell(a, I, J, sc, mixed);
+}
+
+// all coarse levels of the tail, one workgroup, level by level
+constexpr int CHAIN_MAX = 12;  // keeps the kernel argument block under 4 KiB
+struct CoarsenChain {
+    int n;
+    CoarsenArgs L[CHAIN_MAX];
+};
+
+__global__ void k_coarsen_chain(CoarsenChain c, const double* __restrict__ sc, int mixed)
+{
+    for (int l = 0; l < c.n; l++) {
+        const CoarsenArgs& a = c.L[l];
+        for (int q = threadIdx.x; q < a.cni * a.cnj; q += blockDim.x) coarsen_cell(a, 1 + q / a.cnj, 1 + q % a.cnj, sc, mixed);
+        __syncthreads();
+    }
 }
 
 // ------------------------------------------------------------------ V-cycle kernels
@@ -419,34 +494,6 @@ __global__ void k_bicg_finish(const double* __restrict__ part, int nb, double* _
     }
 }
 
-// max |aP| of level 0 -> power-of-two scale scal[3] with scal[3] * max|aP| in [0.5, 1)
-__global__ void k_maxabs(int ni, int nj, int ld, const double* __restrict__ aP,
-                         const unsigned char* __restrict__ act, double* __restrict__ part)
-{
-    __shared__ double lds[8];
-    CELL_IJ;
-    double v = 0.0;
-    if (i <= ni && j <= nj) {
-        const size_t k = (size_t)i * ld + j;
-        if (act[k]) v = fabs(aP[k]);
-    }
-    double m = block_max<BX * BY>(v, lds);
-    if (lin_tid() == 0) part[blockIdx.y * gridDim.x + blockIdx.x] = m;
-}
-
-__global__ void k_finish_scale(const double* __restrict__ part, int nb, double* __restrict__ scal)
-{
-    __shared__ double lds[8];
-    double m = 0.0;
-    for (int k = threadIdx.x; k < nb; k += blockDim.x) m = fmax(m, part[k]);
-    m = block_max<256>(m, lds);
-    if (threadIdx.x == 0) {
-        int e = 0;
-        (void)frexp(m > 0.0 ? m : 1.0, &e);
-        scal[3] = ldexp(1.0, -e);
-    }
-}
-
 // ------------------------------------------------------------------ host side
 MGSolver::~MGSolver()
 {
@@ -546,54 +593,68 @@ void MGSolver::configure(int ni, int nj, hipStream_t s)
     if (!h_scal) CATHODE_HIP(hipHostMalloc(&h_scal, 16 * sizeof(double), hipHostMallocDefault));
 }
 
+CoarsenArgs MGSolver::coarsen_args(int l, const double* w)
+{
+    Level<double>& A = L64[l];
+    Level<double>& C = L64[l + 1];
+    Level<float>& G = L32[l + 1];
+    CoarsenArgs a;
+    a.fni = A.ni; a.fnj = A.nj; a.fld = A.ld; a.fi = A.cx ? 2 : 1; a.fj = A.cy ? 2 : 1;
+    a.faP = A.aP; a.faE = A.aE; a.faW = A.aW; a.faN = A.aN; a.faS = A.aS;
+    a.fm = mass.p + level_off[l];
+    a.fact = actall.p + level_off[l];
+    a.w = l == 0 ? w : nullptr;
+    a.cni = C.ni; a.cnj = C.nj; a.cld = C.ld;
+    a.caP = C.aP; a.caE = C.aE; a.caW = C.aW; a.caN = C.aN; a.caS = C.aS; a.cb = C.b; a.cx = C.x;
+    a.cm = mass.p + level_off[l + 1];
+    a.cact = actall.p + level_off[l + 1];
+    a.gP = G.aP; a.gE = G.aE; a.gW = G.aW; a.gN = G.aN; a.gS = G.aS;
+    a.hx = A.cx ? 0.5 : 1.0;
+    a.hy = A.cy ? 0.5 : 1.0;
+    a.hb = (A.cx && A.cy) ? 0.5 : ((A.cx || A.cy) ? 0.75 : 1.0);
+    return a;
+}
+
+// Builds the hierarchy for the level-0 system the caller filled in, plus the
+// setup scalars (|b|^2, FP32 scales): 4 + lt launches.
 void MGSolver::setup(int mode, hipStream_t s)
 {
     Level<double>& F = L64[0];
     const size_t n0 = level_off[1] - level_off[0];
+    const int mixed = mode == CATHODE_MODE_MIXED ? 1 : 0;
+    const int nb = cdiv(F.nj, BX) * cdiv(F.ni, BY);
     // level-0 activity lives in act0 (filled by the caller); mirror into actall
     CATHODE_HIP(hipMemcpyAsync(actall.p, act0.p, n0, hipMemcpyDeviceToDevice, s));
-    double* sbuf = mass.p;  // excess s of level 0 temporarily in the mass slot of level 1.. (reused)
-    // use the level-0 b array as scratch for s (it is overwritten by the outer loop)
-    sbuf = F.b;
-    hipLaunchKernelGGL(k_l0_prepare, grid2(F.ni, F.nj), dim3(BX, BY), 0, s, F.ni, F.nj, F.ld, act0.p,
-                       F.aP, F.aE, F.aW, F.aN, F.aS, sbuf);
+    double* sbuf = F.b;  // row excess of level 0 (F.b is rewritten by the outer loop)
+    hipLaunchKernelGGL(k_l0_prepare, grid2(F.ni, F.nj), dim3(BX, BY), 0, s, F.ni, F.nj, F.ld, act0.p, F.aP, F.aE,
+                       F.aW, F.aN, F.aS, bsol.p, sbuf, part.p, nb);
+    hipLaunchKernelGGL(k_setup_finish, dim3(1), dim3(256), 0, s, part.p, nb, scal.p, (double)F.ni * (double)F.nj);
     CATHODE_LAUNCH_CHECK();
     const double* w = user_w;
     if (!w) {
-        hipLaunchKernelGGL(k_l0_weights, dim3(1), dim3(256), (F.nj + 2) * sizeof(double), s, F.ni, F.nj,
-                           F.ld, F.aN, F.aS, w0.p);
+        hipLaunchKernelGGL(k_l0_weights, dim3(1), dim3(256), (F.nj + 2) * sizeof(double), s, F.ni, F.nj, F.ld, F.aN,
+                           F.aS, w0.p);
         CATHODE_LAUNCH_CHECK();
         w = w0.p;
     }
-    hipLaunchKernelGGL(k_l0_mass, grid2(F.ni, F.nj), dim3(BX, BY), 0, s, F.ni, F.nj, F.ld, act0.p, sbuf,
-                       w, mass.p + level_off[0]);
+    Level<float>& G = L32[0];
+    hipLaunchKernelGGL(k_l0_mass, grid2(F.ni, F.nj), dim3(BX, BY), 0, s, F.ni, F.nj, F.ld, act0.p, sbuf, w,
+                       mass.p + level_off[0], F.aP, F.aE, F.aW, F.aN, F.aS, G.aP, G.aE, G.aW, G.aN, G.aS,
+                       (const double*)scal.p, mixed);
     CATHODE_LAUNCH_CHECK();
-    for (int l = 0; l + 1 < nl; l++) {
-        Level<double>& A = L64[l];
+    // fine coarse levels: one launch each; the rest in one workgroup
+    for (int l = 0; l + 1 < nl && l < lt; l++) {
         Level<double>& C = L64[l + 1];
-        const int fi = A.cx ? 2 : 1, fj = A.cy ? 2 : 1;
-        const double hx = A.cx ? 0.5 : 1.0, hy = A.cy ? 0.5 : 1.0;
-        const double hb = (A.cx && A.cy) ? 0.5 : ((A.cx || A.cy) ? 0.75 : 1.0);
-        hipLaunchKernelGGL(k_coarsen, grid2(C.ni, C.nj), dim3(BX, BY), 0, s, A.ni, A.nj, A.ld, fi, fj, A.aP,
-                           A.aE, A.aW, A.aN, A.aS, mass.p + level_off[l], actall.p + level_off[l],
-                           l == 0 ? w : (const double*)nullptr, C.ni, C.nj, C.ld, C.aP, C.aE, C.aW, C.aN,
-                           C.aS, C.b, C.x, mass.p + level_off[l + 1], actall.p + level_off[l + 1], hx, hy,
-                           hb);
-        CATHODE_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_coarsen, grid2(C.ni, C.nj), dim3(BX, BY), 0, s, coarsen_args(l, w), (const double*)scal.p,
+                           mixed);
     }
-    if (mode == CATHODE_MODE_MIXED) {
-        const int nb = cdiv(F.nj, BX) * cdiv(F.ni, BY);
-        hipLaunchKernelGGL(k_maxabs, grid2(F.ni, F.nj), dim3(BX, BY), 0, s, F.ni, F.nj, F.ld, F.aP, act0.p,
-                           part.p);
-        hipLaunchKernelGGL(k_finish_scale, dim3(1), dim3(256), 0, s, part.p, nb, scal.p);
-        for (int l = 0; l < nl; l++) {
-            const size_t n5 = 5 * (level_off[l + 1] - level_off[l]);
-            hipLaunchKernelGGL(k_level_to_f32, dim3(cdiv((long)n5, 256)), dim3(256), 0, s, n5, L64[l].aP,
-                               L32[l].aP, (const double*)scal.p);
-        }
-        CATHODE_LAUNCH_CHECK();
+    for (int l0 = lt; l0 + 1 < nl; l0 += CHAIN_MAX) {
+        CoarsenChain ch;
+        ch.n = 0;
+        for (int l = l0; l + 1 < nl && ch.n < CHAIN_MAX; l++) ch.L[ch.n++] = coarsen_args(l, w);
+        hipLaunchKernelGGL(k_coarsen_chain, dim3(1), dim3(512), 0, s, ch, (const double*)scal.p, mixed);
     }
-    (void)n0;
+    CATHODE_LAUNCH_CHECK();
 }
 
 template <class T>
@@ -681,10 +742,6 @@ int MGSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_sta
     const int mixed = mode == CATHODE_MODE_MIXED ? 1 : 0;
     const dim3 g = grid2(F.ni, F.nj), blk(BX, BY);
     setup(mode, s);
-    hipLaunchKernelGGL(k_sumsq, g, blk, 0, s, F.ni, F.nj, F.ld, bsol.p, act0.p, part.p);
-    hipLaunchKernelGGL(k_bicg_finish, dim3(1), dim3(256), 0, s, part.p, nb, scal.p, 0,
-                       (double)F.ni * (double)F.nj);
-    CATHODE_LAUNCH_CHECK();
     CATHODE_HIP(hipMemsetAsync(xsol.p, 0, n0 * sizeof(double), s));
     double* r = kv.p;
     double* r0 = r + n0;

@@ -30,6 +30,7 @@
 namespace cathode {
 
 constexpr int MG_MAX_LEVELS = 24;
+struct CoarsenArgs;
 
 template <class T>
 struct Level {
@@ -89,6 +90,7 @@ public:
 
 private:
     void setup(int mode, hipStream_t s);
+    struct CoarsenArgs coarsen_args(int l, const double* w);
     template <class T>
     void vcycle(Level<T>* L, hipStream_t s);
     void prof_begin(hipStream_t s);
