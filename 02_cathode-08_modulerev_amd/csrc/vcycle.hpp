@@ -149,17 +149,17 @@ struct TailArgs {
     const double* w;                 // restriction weights when the tail starts at level 0
 };
 
+// two red-black sweeps (red first) of one tail level; c = this thread's cell
+// (-1: none), colour its parity
 template <class T>
-__device__ inline void tail_smooth(T* P, T* E, T* W, T* N, T* S, T* B, T* X, int ni, int nj, int color)
+__device__ inline void tail_sweeps(const T* P, const T* E, const T* W, const T* N, const T* S, const T* B, T* X,
+                                   int ld, int c, int colour)
 {
-    const int ld = nj + 2;
-    for (int q = threadIdx.x; q < ni * nj; q += TAIL_NT) {
-        const int i = 1 + q / nj, j = 1 + q % nj;
-        if (((i + j) & 1) != color) continue;
-        const int c = i * ld + j;
-        X[c] = (B[c] + E[c] * X[c + ld] + W[c] * X[c - ld] + N[c] * X[c + 1] + S[c] * X[c - 1]) / P[c];
+    for (int sw = 0; sw < 4; sw++) {
+        if (c >= 0 && colour == (sw & 1))
+            X[c] = (B[c] + E[c] * X[c + ld] + W[c] * X[c - ld] + N[c] * X[c + 1] + S[c] * X[c - 1]) / P[c];
+        __syncthreads();
     }
-    __syncthreads();
 }
 
 template <class T>
@@ -206,22 +206,28 @@ __global__ void __launch_bounds__(TAIL_NT) k_tail(TailArgs<T> A)
         }
     }
     __syncthreads();
+    // each thread owns cell threadIdx.x of every tail level (<= TAIL_NT cells)
     // down
     for (int l = 0; l + 1 < A.nt; l++) {
         const int ni = A.ni[l], nj = A.nj[l], ld = nj + 2, o = A.off[l];
-        for (int sw = 0; sw < 4; sw++) tail_smooth(P + o, E + o, W + o, N + o, S + o, B + o, X + o, ni, nj, sw & 1);
+        const int q = threadIdx.x;
+        const int i = 1 + q / nj, j = 1 + q % nj;
+        const int c = q < ni * nj ? o + i * ld + j : -1;
+        tail_sweeps(P, E, W, N, S, B, X, ld, c, (i + j) & 1);
+        // restriction: coarse cell q of level l+1
         const int fi = A.cx[l] ? 2 : 1, fj = A.cy[l] ? 2 : 1;
         const int cni = A.ni[l + 1], cnj = A.nj[l + 1], cld = cnj + 2, co = A.off[l + 1];
-        for (int q = threadIdx.x; q < cni * cnj; q += TAIL_NT) {
+        if (q < cni * cnj) {
             const int I = 1 + q / cnj, J = 1 + q % cnj;
             T sum = T(0);
             for (int a = 0; a < fi; a++)
                 for (int bb = 0; bb < fj; bb++) {
-                    const int i = fi * (I - 1) + 1 + a, j = fj * (J - 1) + 1 + bb;
-                    if (i > ni || j > nj) continue;
-                    const int c = o + i * ld + j;
-                    T r = B[c] - P[c] * X[c] + E[c] * X[c + ld] + W[c] * X[c - ld] + N[c] * X[c + 1] + S[c] * X[c - 1];
-                    if (l == 0 && A.w) r = r * (T)A.w[j];
+                    const int fi_ = fi * (I - 1) + 1 + a, fj_ = fj * (J - 1) + 1 + bb;
+                    if (fi_ > ni || fj_ > nj) continue;
+                    const int cc = o + fi_ * ld + fj_;
+                    T r = B[cc] - P[cc] * X[cc] + E[cc] * X[cc + ld] + W[cc] * X[cc - ld] + N[cc] * X[cc + 1] +
+                          S[cc] * X[cc - 1];
+                    if (l == 0 && A.w) r = r * (T)A.w[fj_];
                     sum += r;
                 }
             B[co + I * cld + J] = sum;
@@ -232,7 +238,8 @@ __global__ void __launch_bounds__(TAIL_NT) k_tail(TailArgs<T> A)
     {
         const int l = A.nt - 1;
         const int ni = A.ni[l], nj = A.nj[l], ld = nj + 2, o = A.off[l];
-        for (int q = threadIdx.x; q < ni * nj; q += TAIL_NT) {
+        const int q = threadIdx.x;
+        if (q < ni * nj) {
             const int c = o + (1 + q / nj) * ld + 1 + q % nj;
             X[c] = (ACT[c] && !(P[c] == T(0))) ? B[c] / P[c] : T(0);
         }
@@ -242,15 +249,15 @@ __global__ void __launch_bounds__(TAIL_NT) k_tail(TailArgs<T> A)
     for (int l = A.nt - 2; l >= 0; l--) {
         const int ni = A.ni[l], nj = A.nj[l], ld = nj + 2, o = A.off[l];
         const int cld = A.nj[l + 1] + 2, co = A.off[l + 1];
-        for (int q = threadIdx.x; q < ni * nj; q += TAIL_NT) {
-            const int i = 1 + q / nj, j = 1 + q % nj;
-            const int c = o + i * ld + j;
-            if (!ACT[c]) continue;
+        const int q = threadIdx.x;
+        const int i = 1 + q / nj, j = 1 + q % nj;
+        const int c = q < ni * nj ? o + i * ld + j : -1;
+        if (c >= 0 && ACT[c]) {
             const int I = A.cx[l] ? (i + 1) >> 1 : i, J = A.cy[l] ? (j + 1) >> 1 : j;
             X[c] = X[c] + X[co + I * cld + J];
         }
         __syncthreads();
-        for (int sw = 0; sw < 4; sw++) tail_smooth(P + o, E + o, W + o, N + o, S + o, B + o, X + o, ni, nj, sw & 1);
+        tail_sweeps(P, E, W, N, S, B, X, ld, c, (i + j) & 1);
     }
     // write the first tail level
     {
