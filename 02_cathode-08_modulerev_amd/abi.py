@@ -73,6 +73,14 @@ class NddReport(ctypes.Structure):
     ]
 
 
+class EthetaParams(ctypes.Structure):
+    _fields_ = [("ni", ctypes.c_int), ("nj", ctypes.c_int), ("dx", ctypes.c_double),
+                ("dr", ctypes.c_double), ("rmin", ctypes.c_double), ("omega", ctypes.c_double),
+                ("nu", ctypes.c_double), ("epsr", ctypes.c_double), ("masse", ctypes.c_double),
+                ("pec_z", ctypes.c_int), ("pec_rmax", ctypes.c_int), ("maxITR", ctypes.c_int),
+                ("tol", ctypes.c_double), ("mode", ctypes.c_int)]
+
+
 def dptr(a):
     """Pointer to a C-contiguous float64 numpy array (None -> NULL)."""
     if a is None:
@@ -110,6 +118,8 @@ SIGNATURES = [
                   ctypes.POINTER(Status)]),
     ("cathode_helm5_solve", ctypes.c_int,
      _SOLVE5 + [ctypes.c_int, ctypes.c_double, ctypes.c_int, _D, ctypes.POINTER(Status)]),
+    ("cathode_etheta_solve", ctypes.c_int,
+     [ctypes.POINTER(EthetaParams), _D, _D, _D, _D, _D, ctypes.POINTER(Status)]),
     ("cathode_ndd_create", ctypes.c_int, [ctypes.POINTER(NddParams), ctypes.POINTER(ctypes.c_void_p)]),
     ("cathode_ndd_set_state", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(NddFields)]),
     ("cathode_ndd_get_state", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(NddFields)]),

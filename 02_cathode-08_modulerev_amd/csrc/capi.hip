@@ -10,6 +10,7 @@
 #include <mutex>
 #include <vector>
 
+#include "helm.hpp"
 #include "mg.hpp"
 #include "sor.hpp"
 
@@ -18,6 +19,7 @@ namespace cathode {
 static std::string g_err;
 static cathode_status g_last{};
 static std::mutex g_mu;
+std::mutex& api_mutex() { return g_mu; }
 
 void set_error(const std::string& m) { g_err = m; }
 const char* get_error() { return g_err.c_str(); }
@@ -128,19 +130,6 @@ static int struct5_impl(const double* aP, const double* aE, const double* aW, co
 
 using namespace cathode;
 
-#define CATHODE_GUARD_BEGIN \
-    std::lock_guard<std::mutex> lock_(g_mu); \
-    try {
-#define CATHODE_GUARD_END                 \
-    }                                     \
-    catch (const hip_failure&) {          \
-        return CATHODE_ERR_HIP;           \
-    }                                     \
-    catch (const std::bad_alloc&) {       \
-        set_error("host allocation failed"); \
-        return CATHODE_ERR_NOMEM;         \
-    }
-
 extern "C" {
 
 int cathode_abi_version(void) { return CATHODE_ABI_VERSION; }
@@ -183,6 +172,7 @@ int cathode_finalize(void)
     g_stage.release();
     g_actbuf.release();
     sor_release();
+    helm_release();
     if (c.stream) (void)hipStreamDestroy(c.stream);
     c.stream = nullptr;
     c.ready = false;

@@ -6,6 +6,8 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <mutex>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -32,6 +34,22 @@ struct hip_failure {
     } while (0)
 
 #define CATHODE_LAUNCH_CHECK() CATHODE_HIP(hipGetLastError())
+
+// Every extern "C" entry point runs under one library mutex and turns HIP
+// failures / host allocation failures into error codes.
+std::mutex& api_mutex();
+#define CATHODE_GUARD_BEGIN                                       \
+    std::lock_guard<std::mutex> lock_(::cathode::api_mutex());    \
+    try {
+#define CATHODE_GUARD_END                                \
+    }                                                    \
+    catch (const ::cathode::hip_failure&) {              \
+        return CATHODE_ERR_HIP;                          \
+    }                                                    \
+    catch (const std::bad_alloc&) {                      \
+        ::cathode::set_error("host allocation failed"); \
+        return CATHODE_ERR_NOMEM;                        \
+    }
 
 // ------------------------------------------------------------------ context
 struct Context {

@@ -1,0 +1,9 @@
+// helm.hpp -- complex 5-point solver (helm.hip): library-level release hook.
+#pragma once
+
+namespace cathode {
+
+// Frees the complex solver and its staging buffers (cathode_finalize).
+void helm_release();
+
+}  // namespace cathode

@@ -823,6 +823,12 @@ int MGSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_sta
         // the recursion has drifted from the true residual: restart from it
         if (sqrt(h_scal[SC_RR] / bb) <= tol) restart = true;
     }
+    if (maxITR <= 0) {
+        // no iteration: x = 0, relative residual 1 (0 when b = 0)
+        CATHODE_HIP(hipMemcpyAsync(h_scal, scal.p, SC_COUNT * sizeof(double), hipMemcpyDeviceToHost, s));
+        CATHODE_HIP(hipStreamSynchronize(s));
+        rel = h_scal[SC_BB] > 0.0 ? 1.0 : 0.0;
+    }
     if (xa != xsol.p) CATHODE_HIP(hipMemcpyAsync(xsol.p, xa, n0 * sizeof(double), hipMemcpyDeviceToDevice, s));
     if (prof.enabled) {
         CATHODE_HIP(hipStreamSynchronize(s));

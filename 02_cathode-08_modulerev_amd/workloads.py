@@ -144,5 +144,33 @@ def config1(seed=1, ni=512, nj=256, h=5e-5, V_bias=30.0, mode=abi.CATHODE_MODE_F
     return p, consistent_state(p, n, n.copy(), Te)
 
 
+def config2(seed=2, ni=1024, nj=512, h=5e-5, mode=abi.CATHODE_MODE_FP64, maxITR=100, tol=1e-10):
+    """E_theta microwave solve: omega = 2.67e10 rad/s, nu = 1e9 1/s, PEC on
+    every wall (axis at r = 0), n_e = 1e14 + a seeded Gaussian of peak
+    5e17 m^-3 (centre z uniform in the middle half, r = 0, sigma uniform in
+    [2, 5] mm), excitation: a one-cell ring current at r = 3 mm, mid z,
+    carrying J dz dr = 1 A.  Returns (EthetaParams, ne, J) with J complex128."""
+    p = abi.EthetaParams()
+    p.ni, p.nj = ni, nj
+    p.dx = p.dr = h
+    p.rmin = 0.0
+    p.omega = 2.67e10
+    p.nu = 1e9
+    p.epsr = 1.0
+    p.masse = MASSE
+    p.pec_z = p.pec_rmax = 1
+    p.maxITR, p.tol, p.mode = maxITR, tol, mode
+    rng = np.random.Generator(np.random.PCG64(seed))
+    z = axial(ni, h)[:, None]
+    r = radii(nj, h)[None, :]
+    zc = rng.uniform(0.25, 0.75) * ni * h
+    sig = rng.uniform(2e-3, 5e-3)
+    ne = 1e14 + 5e17 * np.exp(-((z - zc) ** 2 + r ** 2) / (2.0 * sig * sig))
+    J = np.zeros((ni + 2, nj + 2), dtype=np.complex128)
+    jr = 1 + int(np.argmin(np.abs(radii(nj, h)[1:nj + 1] - 3e-3)))
+    J[1 + ni // 2, jr] = 1.0 / (h * h)
+    return p, np.ascontiguousarray(ne), J
+
+
 def copy_state(st):
     return {k: v.copy() for k, v in st.items()}

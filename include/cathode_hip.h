@@ -118,6 +118,32 @@ int cathode_helm5_solve(const double* aP, const double* aE, const double* aW,
                         int maxITR, double tol, int mode,
                         double* x, cathode_status* st);
 
+/* E_theta microwave field with B = 0, assembled, solved and post-processed
+ * on the device: EmfieldModule::solve_Microwave's E_theta rows
+ * (include/emfieldModule.hpp:2414-2454), PEC / first-order open folds
+ * (:2713-2773), ghost cells (:3213-3264) and J = sigma E, Pabs =
+ * 0.5 Re(J E*) (:7733-7743).  Arrays are (ni+2)*(nj+2) cells, ld = nj+2:
+ * ne real, J and E interleaved complex128.  Pabs (real, may be NULL) gets
+ * the interior values; *P_total (may be NULL) the volume integral
+ * sum Pabs 2 pi r dx dr [W].  Ghost corners (and j = 0 on the axis) of E are
+ * written as 0. */
+typedef struct cathode_etheta_params {
+    int ni, nj;
+    double dx, dr, rmin;
+    double omega;               /* angular frequency [rad/s] */
+    double nu;                  /* electron collision frequency [1/s] */
+    double epsr;                /* background relative permittivity */
+    double masse;               /* electron mass [kg] */
+    int pec_z, pec_rmax;        /* 1 = PEC at the z walls / r = rmax, 0 = open */
+    int maxITR;
+    double tol;
+    int mode;                   /* CATHODE_MODE_* */
+} cathode_etheta_params;
+
+int cathode_etheta_solve(const cathode_etheta_params* p, const double* ne,
+                         const double* J, double* E, double* Pabs,
+                         double* P_total, cathode_status* st);
+
 /* ------------------------------------------------------------------ */
 /* device-resident implicit NDD step                                  */
 /* (FluidModule::solve_phi_couple_wdTe_wSEE_PC, solve_rhoe_wdTe_wSEE_PC,

@@ -32,3 +32,30 @@ def sor(lib, aP, aE, aW, aN, aS, b, boxes, alpha, maxITR, tol, x):
                                bp, nb, ni2, ld, ld, alpha, maxITR, tol, abi.dptr(x), ctypes.byref(st))
     abi.check(rc, lib)
     return x, st
+
+
+def helm5(lib, aP, aE, aW, aN, aS, b, boxes, maxITR=100, tol=1e-10, mode=0, x=None):
+    """complex128 arrays (ni2, nj2)"""
+    ni2, ld = aP.shape
+    x = np.zeros_like(aP) if x is None else x
+    bk, bp, nb = _boxes(boxes)
+    st = abi.Status()
+    v = [np.ascontiguousarray(a, dtype=np.complex128).view(np.float64) for a in (aP, aE, aW, aN, aS, b)]
+    rc = lib.cathode_helm5_solve(*[abi.dptr(a) for a in v], bp, nb, ni2, ld, ld, maxITR, tol, mode,
+                                 abi.dptr(x.view(np.float64)), ctypes.byref(st))
+    abi.check(rc, lib)
+    return x, st
+
+
+def etheta(lib, p, ne, J):
+    """E (complex, with ghosts), Pabs, P_total, status"""
+    E = np.zeros(ne.shape, dtype=np.complex128)
+    Pabs = np.zeros(ne.shape)
+    Pt = ctypes.c_double()
+    st = abi.Status()
+    rc = lib.cathode_etheta_solve(ctypes.byref(p), abi.dptr(np.ascontiguousarray(ne)),
+                                  abi.dptr(np.ascontiguousarray(J).view(np.float64)),
+                                  abi.dptr(E.view(np.float64)), abi.dptr(Pabs), ctypes.byref(Pt),
+                                  ctypes.byref(st))
+    abi.check(rc, lib)
+    return E, Pabs, Pt.value, st

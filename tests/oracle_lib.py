@@ -167,3 +167,58 @@ def rel_l2(a, b):
     d = np.linalg.norm((a - b).ravel())
     n = np.linalg.norm(b.ravel())
     return d / n if n > 0 else d
+
+
+def helm_params(p):
+    """orc_helm_params from an abi.EthetaParams"""
+    h = HelmParams()
+    for n, _ in HelmParams._fields_:
+        setattr(h, n, getattr(p, n))
+    return h
+
+
+def helm_assemble(p, ne, J):
+    """Folded E_theta system (complex (ni+2, nj+2) arrays aP, aE, aW, aN, aS, b)."""
+    shp = ne.shape
+    out = [np.zeros(shp, dtype=np.complex128) for _ in range(6)]
+    hp = helm_params(p)
+    rc = lib().orc_helm_assemble(ctypes.byref(hp), P(np.ascontiguousarray(ne)),
+                                 P(np.ascontiguousarray(J).view(np.float64)),
+                                 *[P(a.view(np.float64)) for a in out])
+    assert rc == 0, rc
+    return out
+
+
+def helm_post(p, ne, E):
+    """Fills E's ghosts in place, returns (Pabs, P_total)."""
+    Pabs = np.zeros(ne.shape)
+    hp = helm_params(p)
+    Pt = lib().orc_helm_post(ctypes.byref(hp), P(np.ascontiguousarray(ne)), P(E.view(np.float64)), P(Pabs))
+    return Pabs, Pt
+
+
+def helm_solve_direct(p, ne, J):
+    """Oracle E_theta: assemble, exact banded LU, ghosts and Pabs."""
+    aP, aE, aW, aN, aS, b = helm_assemble(p, ne, J)
+    E = helm5_direct(aP, aE, aW, aN, aS, b, [1, p.ni, 1, p.nj])
+    Pabs, Pt = helm_post(p, ne, E)
+    return E, Pabs, Pt
+
+
+def apply5(aP, aE, aW, aN, aS, x, box):
+    """A x on one box (couplings to cells outside the box dropped); zeros elsewhere."""
+    i0, i1, j0, j1 = box
+    y = np.zeros_like(x)
+    m = np.zeros(x.shape, dtype=bool)
+    m[i0:i1 + 1, j0:j1 + 1] = True
+    xm = np.where(m, x, 0)
+    s = (slice(i0, i1 + 1), slice(j0, j1 + 1))
+    y[s] = (aP[s] * xm[s] - aE[s] * xm[i0 + 1:i1 + 2, j0:j1 + 1] - aW[s] * xm[i0 - 1:i1, j0:j1 + 1]
+            - aN[s] * xm[i0:i1 + 1, j0 + 1:j1 + 2] - aS[s] * xm[i0:i1 + 1, j0 - 1:j1])
+    return y
+
+
+def rel_residual_box(aP, aE, aW, aN, aS, b, box, x):
+    i0, i1, j0, j1 = box
+    r = (b - apply5(aP, aE, aW, aN, aS, x, box))[i0:i1 + 1, j0:j1 + 1]
+    return float(np.linalg.norm(r) / np.linalg.norm(b[i0:i1 + 1, j0:j1 + 1]))
