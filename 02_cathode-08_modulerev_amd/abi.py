@@ -13,6 +13,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "csrc", "libcathode_hip.so")
 
 CATHODE_OK = 0
+CATHODE_ERR_ARG = -1
+CATHODE_ERR_HIP = -2
+CATHODE_ERR_NOINIT = -3
+CATHODE_ERR_NOMEM = -4
+CATHODE_ERR_NUMERIC = -5
 CATHODE_MODE_FP64 = 0
 CATHODE_MODE_MIXED = 1
 SEE_MAX = 256

@@ -101,6 +101,7 @@ static int struct5_impl(const double* aP, const double* aE, const double* aW, co
         CATHODE_HIP(hipMemcpy2DAsync(dst[k], ld0 * sizeof(double), src[k], ld * sizeof(double),
                                      nj2 * sizeof(double), ni2, kind, s));
     S.set_weights(nullptr);
+    S.enable_nine(false);
     cathode_status local{};
     S.solve(maxITR, tol, mode, s, &local);
     if (device) {
@@ -212,6 +213,65 @@ int cathode_sor_solve(const double* aP, const double* aE, const double* aW, cons
     if (!require_init()) return CATHODE_ERR_NOINIT;
     return sor_solve_host(aP, aE, aW, aN, aS, b, boxes, n_bl, ni2, nj2, ld, alpha, maxITR, tol, x,
                           ctx().stream, st);
+    CATHODE_GUARD_END
+}
+
+int cathode_lu9_solve(const double* aP, const double* aE, const double* aW, const double* aN, const double* aS,
+                      const double* aEN, const double* aES, const double* aWN, const double* aWS, const double* b,
+                      const int* boxes, int n_bl, int ni2, int nj2, int ld, double* x, cathode_status* st)
+{
+    CATHODE_GUARD_BEGIN
+    if (!require_init()) return CATHODE_ERR_NOINIT;
+    auto t0 = std::chrono::steady_clock::now();
+    if (!aP || !aE || !aW || !aN || !aS || !aEN || !aES || !aWN || !aWS || !b || !x || ni2 < 3 || nj2 < 3 ||
+        ld < nj2) {
+        set_error("lu9: invalid arguments");
+        return CATHODE_ERR_ARG;
+    }
+    hipStream_t s = ctx().stream;
+    if (!g_solver) g_solver.reset(new MGSolver());
+    MGSolver& S = *g_solver;
+    S.configure(ni2 - 2, nj2 - 2, s);
+    const int ld0 = S.ld0();
+    std::vector<unsigned char> mask;
+    if (!build_mask(boxes, n_bl, ni2, nj2, ld0, mask)) {
+        set_error("lu9: boxes must lie inside the interior cells 1..ni, 1..nj");
+        return CATHODE_ERR_ARG;
+    }
+    CATHODE_HIP(hipMemcpyAsync(S.act(), mask.data(), mask.size(), hipMemcpyHostToDevice, s));
+    S.enable_nine(true);
+    const size_t n0 = S.n0();
+    const double* src[10] = {aP, b, aE, aW, aN, aS, aEN, aES, aWN, aWS};
+    double* dst[10] = {S.aP(), S.rhs()};
+    for (int q = 0; q < 8; q++) dst[2 + q] = S.op9() + q * n0;
+    for (int k = 0; k < 10; k++)
+        CATHODE_HIP(hipMemcpy2DAsync(dst[k], ld0 * sizeof(double), src[k], ld * sizeof(double), nj2 * sizeof(double),
+                                     ni2, hipMemcpyHostToDevice, s));
+    S.lump_nine(s);
+    S.set_weights(nullptr);
+    cathode_status local{};
+    try {
+        // solver_LU_9p factorizes exactly: iterate to round-off level
+        S.solve(200, 1e-13, CATHODE_MODE_FP64, s, &local);
+    } catch (...) {
+        S.enable_nine(false);
+        throw;
+    }
+    S.enable_nine(false);
+    g_hx.resize((size_t)ni2 * ld0);
+    CATHODE_HIP(hipMemcpyAsync(g_hx.data(), S.sol(), g_hx.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    CATHODE_HIP(hipStreamSynchronize(s));
+    for (int i = 0; i < ni2; i++)
+        for (int j = 0; j < nj2; j++)
+            if (mask[(size_t)i * ld0 + j]) x[(size_t)i * ld + j] = g_hx[(size_t)i * ld0 + j];
+    local.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    record_status(local);
+    if (st) *st = local;
+    if (!std::isfinite(local.rel_residual)) {
+        set_error("lu9: non-finite residual (NaN/Inf in the system)");
+        return CATHODE_ERR_NUMERIC;
+    }
+    return CATHODE_OK;
     CATHODE_GUARD_END
 }
 

@@ -288,10 +288,43 @@ __device__ inline double from_pc(T y, const double* sc, int mixed)
     return mixed ? (double)y / sc[SC_SCALE] : (double)y;
 }
 
+// Sum over the off-diagonal couplings of coupling * neighbour value: the
+// level-0 5-point arrays, or (nine != nullptr) the nine-point couplings
+// E W N S EN ES WN WS stored n0 apart.  xat(m) gives the neighbour value;
+// inactive and ghost cells hold 0 in every vector it is used on.
+template <class F>
+__device__ inline double offdiag(const double* __restrict__ aE, const double* __restrict__ aW,
+                                 const double* __restrict__ aN, const double* __restrict__ aS,
+                                 const double* __restrict__ nine, size_t n0, size_t k, int ld, F xat)
+{
+    if (!nine) return aE[k] * xat(k + ld) + aW[k] * xat(k - ld) + aN[k] * xat(k + 1) + aS[k] * xat(k - 1);
+    return nine[k] * xat(k + ld) + nine[n0 + k] * xat(k - ld) + nine[2 * n0 + k] * xat(k + 1) +
+           nine[3 * n0 + k] * xat(k - 1) + nine[4 * n0 + k] * xat(k + ld + 1) +
+           nine[5 * n0 + k] * xat(k + ld - 1) + nine[6 * n0 + k] * xat(k - ld + 1) +
+           nine[7 * n0 + k] * xat(k - ld - 1);
+}
+
+// level-0 5-point preconditioner couplings from the nine-point ones
+__global__ void k_lump9(int ni, int nj, int ld, const double* __restrict__ nine, size_t n0,
+                        const unsigned char* __restrict__ act, double* __restrict__ aE, double* __restrict__ aW,
+                        double* __restrict__ aN, double* __restrict__ aS)
+{
+    CELL_IJ;
+    if (i > ni || j > nj) return;
+    const size_t k = (size_t)i * ld + j;
+    const double en = act[k + ld + 1] ? nine[4 * n0 + k] : 0.0, es = act[k + ld - 1] ? nine[5 * n0 + k] : 0.0;
+    const double wn = act[k - ld + 1] ? nine[6 * n0 + k] : 0.0, ws = act[k - ld - 1] ? nine[7 * n0 + k] : 0.0;
+    aE[k] = nine[k] + 0.5 * (en + es);
+    aW[k] = nine[n0 + k] + 0.5 * (wn + ws);
+    aN[k] = nine[2 * n0 + k] + 0.5 * (en + wn);
+    aS[k] = nine[3 * n0 + k] + 0.5 * (es + ws);
+}
+
 // r = b - A x, r0 = r, p = v = 0 (restart); partials |r|^2 (slot 0), (r0,r) (slot 1)
 __global__ void k_bicg_restart(int ni, int nj, int ld, const double* __restrict__ aP,
                                const double* __restrict__ aE, const double* __restrict__ aW,
                                const double* __restrict__ aN, const double* __restrict__ aS,
+                               const double* __restrict__ nine, size_t n0,
                                const double* __restrict__ b, const double* __restrict__ x,
                                const unsigned char* __restrict__ act, double* __restrict__ r,
                                double* __restrict__ r0, double* __restrict__ p, double* __restrict__ v,
@@ -304,8 +337,7 @@ __global__ void k_bicg_restart(int ni, int nj, int ld, const double* __restrict_
         const size_t k = (size_t)i * ld + j;
         double q = 0.0;
         if (act[k]) {
-            q = b[k] - aP[k] * x[k] + aE[k] * x[k + ld] + aW[k] * x[k - ld] + aN[k] * x[k + 1] +
-                aS[k] * x[k - 1];
+            q = b[k] - aP[k] * x[k] + offdiag(aE, aW, aN, aS, nine, n0, k, ld, [&](size_t m) { return x[m]; });
             rr = q * q;
         }
         r[k] = q;
@@ -352,6 +384,7 @@ template <class T>
 __global__ void k_bicg_apply(int ni, int nj, int ld, const double* __restrict__ aP,
                              const double* __restrict__ aE, const double* __restrict__ aW,
                              const double* __restrict__ aN, const double* __restrict__ aS,
+                             const double* __restrict__ nine, size_t n0,
                              const unsigned char* __restrict__ act, const T* __restrict__ vx,
                              double* __restrict__ y, double* __restrict__ out,
                              const double* __restrict__ dotvec, const double* __restrict__ sc, int mixed,
@@ -367,8 +400,8 @@ __global__ void k_bicg_apply(int ni, int nj, int ld, const double* __restrict__ 
         if (act[k]) {
             // inactive neighbours have zero coupling and a zero V-cycle output
             yk = from_pc<T>(vx[k], sc, mixed);
-            o = aP[k] * yk - aE[k] * from_pc<T>(vx[k + ld], sc, mixed) - aW[k] * from_pc<T>(vx[k - ld], sc, mixed) -
-                aN[k] * from_pc<T>(vx[k + 1], sc, mixed) - aS[k] * from_pc<T>(vx[k - 1], sc, mixed);
+            o = aP[k] * yk -
+                offdiag(aE, aW, aN, aS, nine, n0, k, ld, [&](size_t m) { return from_pc<T>(vx[m], sc, mixed); });
             d1 = dotvec[k] * o;
             d2 = o * o;
         }
@@ -405,7 +438,8 @@ __global__ void k_bicg_s(int ni, int nj, int ld, const unsigned char* __restrict
 // |b - A x_new|^2 (slot 2), evaluating the neighbours' x_new from the old buffer
 __global__ void k_bicg_x(int ni, int nj, int ld, const double* __restrict__ aP, const double* __restrict__ aE,
                          const double* __restrict__ aW, const double* __restrict__ aN,
-                         const double* __restrict__ aS, const double* __restrict__ bv,
+                         const double* __restrict__ aS, const double* __restrict__ nine, size_t n0,
+                         const double* __restrict__ bv,
                          const unsigned char* __restrict__ act, const double* __restrict__ x,
                          double* __restrict__ xn, double* __restrict__ r, const double* __restrict__ r0,
                          const double* __restrict__ ph, const double* __restrict__ sh,
@@ -426,8 +460,7 @@ __global__ void k_bicg_x(int ni, int nj, int ld, const double* __restrict__ aP, 
             a = q * q;
             c = r0[k] * q;
             auto xnew = [&](size_t m) { return act[m] ? x[m] + al * ph[m] + om * sh[m] : 0.0; };
-            const double rt = bv[k] - aP[k] * xk + aE[k] * xnew(k + ld) + aW[k] * xnew(k - ld) +
-                              aN[k] * xnew(k + 1) + aS[k] * xnew(k - 1);
+            const double rt = bv[k] - aP[k] * xk + offdiag(aE, aW, aN, aS, nine, n0, k, ld, xnew);
             e = rt * rt;
         }
         xn[k] = xk;
@@ -733,6 +766,20 @@ double MGSolver::read_scalar(const double* d, hipStream_t s)
     return h_scal[0];
 }
 
+void MGSolver::enable_nine(bool on)
+{
+    use_nine = on;
+    if (on) nine.alloc(8 * n0());
+}
+
+void MGSolver::lump_nine(hipStream_t s)
+{
+    const Level<double>& F = L64[0];
+    hipLaunchKernelGGL(k_lump9, grid2(F.ni, F.nj), dim3(BX, BY), 0, s, F.ni, F.nj, F.ld, (const double*)nine.p, n0(),
+                       (const unsigned char*)act0.p, F.aE, F.aW, F.aN, F.aS);
+    CATHODE_LAUNCH_CHECK();
+}
+
 int MGSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_status* st)
 {
     auto t0 = std::chrono::steady_clock::now();
@@ -752,6 +799,7 @@ int MGSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_sta
     double* ph = t + n0;
     double* sh = ph + n0;
     aborted = false;
+    const double* n9 = use_nine ? nine.p : nullptr;
     double* xa = xsol.p;   // current iterate
     double* xb = xalt.p;   // next iterate (double buffer)
     int it = 0;
@@ -759,8 +807,8 @@ int MGSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_sta
     bool restart = true;
     while (it < maxITR) {
         if (restart) {
-            hipLaunchKernelGGL(k_bicg_restart, g, blk, 0, s, F.ni, F.nj, F.ld, F.aP, F.aE, F.aW, F.aN, F.aS,
-                               bsol.p, xa, act0.p, r, r0, p, v, part.p, nb);
+            hipLaunchKernelGGL(k_bicg_restart, g, blk, 0, s, F.ni, F.nj, F.ld, F.aP, F.aE, F.aW, F.aN, F.aS, n9,
+                               n0, bsol.p, xa, act0.p, r, r0, p, v, part.p, nb);
             hipLaunchKernelGGL(k_bicg_finish, dim3(1), dim3(256), 0, s, part.p, nb, scal.p, 1, 0.0);
         }
         it++;
@@ -770,13 +818,13 @@ int MGSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_sta
                                L32[0].x, scal.p, restart ? 1 : 0, 1);
             vcycle<float>(L32, s);
             hipLaunchKernelGGL(k_bicg_apply<float>, g, blk, 0, s, F.ni, F.nj, F.ld, F.aP, F.aE, F.aW, F.aN,
-                               F.aS, act0.p, L32[0].x, ph, v, r0, scal.p, 1, 0, part.p, nb);
+                               F.aS, n9, n0, act0.p, L32[0].x, ph, v, r0, scal.p, 1, 0, part.p, nb);
         } else {
             hipLaunchKernelGGL(k_bicg_p<double>, g, blk, 0, s, F.ni, F.nj, F.ld, act0.p, r, v, p, F.b, F.x,
                                scal.p, restart ? 1 : 0, 0);
             vcycle<double>(L64, s);
             hipLaunchKernelGGL(k_bicg_apply<double>, g, blk, 0, s, F.ni, F.nj, F.ld, F.aP, F.aE, F.aW, F.aN,
-                               F.aS, act0.p, F.x, ph, v, r0, scal.p, 0, 0, part.p, nb);
+                               F.aS, n9, n0, act0.p, F.x, ph, v, r0, scal.p, 0, 0, part.p, nb);
         }
         hipLaunchKernelGGL(k_bicg_finish, dim3(1), dim3(256), 0, s, part.p, nb, scal.p, 2, 0.0);
         restart = false;
@@ -786,17 +834,18 @@ int MGSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_sta
                                L32[0].x, scal.p, 1);
             vcycle<float>(L32, s);
             hipLaunchKernelGGL(k_bicg_apply<float>, g, blk, 0, s, F.ni, F.nj, F.ld, F.aP, F.aE, F.aW, F.aN,
-                               F.aS, act0.p, L32[0].x, sh, t, sv, scal.p, 1, 1, part.p, nb);
+                               F.aS, n9, n0, act0.p, L32[0].x, sh, t, sv, scal.p, 1, 1, part.p, nb);
         } else {
             hipLaunchKernelGGL(k_bicg_s<double>, g, blk, 0, s, F.ni, F.nj, F.ld, act0.p, r, v, sv, F.b, F.x,
                                scal.p, 0);
             vcycle<double>(L64, s);
             hipLaunchKernelGGL(k_bicg_apply<double>, g, blk, 0, s, F.ni, F.nj, F.ld, F.aP, F.aE, F.aW, F.aN,
-                               F.aS, act0.p, F.x, sh, t, sv, scal.p, 0, 1, part.p, nb);
+                               F.aS, n9, n0, act0.p, F.x, sh, t, sv, scal.p, 0, 1, part.p, nb);
         }
         hipLaunchKernelGGL(k_bicg_finish, dim3(1), dim3(256), 0, s, part.p, nb, scal.p, 3, 0.0);
         // x, recursive and true residuals
-        hipLaunchKernelGGL(k_bicg_x, g, blk, 0, s, F.ni, F.nj, F.ld, F.aP, F.aE, F.aW, F.aN, F.aS, bsol.p, act0.p,
+        hipLaunchKernelGGL(k_bicg_x, g, blk, 0, s, F.ni, F.nj, F.ld, F.aP, F.aE, F.aW, F.aN, F.aS, n9, n0, bsol.p,
+                           act0.p,
                            xa, xb, r, r0, ph, sh, sv, t, scal.p, part.p, nb);
         hipLaunchKernelGGL(k_bicg_finish, dim3(1), dim3(256), 0, s, part.p, nb, scal.p, 4, 0.0);
         CATHODE_LAUNCH_CHECK();

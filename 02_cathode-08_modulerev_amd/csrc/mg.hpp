@@ -73,6 +73,17 @@ public:
     // Radial weights w[0..nj+1] on the device; nullptr = infer from the matrix.
     void set_weights(const double* d_w) { user_w = d_w; }
 
+    // Nine-point outer operator (solver_LU_9p systems).  enable_nine(true)
+    // allocates op9(): eight arrays in the level-0 layout, n0() apart, holding
+    // the couplings E, W, N, S, EN, ES, WN, WS; the BiCGStab products then use
+    // them with aP(), while the V-cycle runs on the level-0 5-point arrays that
+    // lump_nine() fills from op9 (each diagonal coupling to an active cell
+    // split in halves onto its two adjacent faces, so row sums are kept).
+    void enable_nine(bool on);
+    double* op9() { return nine.p; }
+    size_t n0() const { return level_off[1] - level_off[0]; }
+    void lump_nine(hipStream_t s);
+
     // Solves A x = rhs from x = 0 into sol().  Returns iterations; fills st.
     int solve(int maxITR, double tol, int mode, hipStream_t s, cathode_status* st);
 
@@ -109,6 +120,8 @@ private:
     DevBuf<unsigned char> act0;
     DevBuf<double> bsol, xsol, xalt, w0;
     const double* user_w = nullptr;
+    DevBuf<double> nine;       // nine-point couplings (enable_nine)
+    bool use_nine = false;
     int lt = 0;                // first level handled by the one-workgroup tail kernel
     int tail_cells = 0;        // LDS cells of the tail levels
     DevBuf<double> part;       // reduction partials (2 slots)

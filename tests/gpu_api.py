@@ -59,3 +59,15 @@ def etheta(lib, p, ne, J):
                                   ctypes.byref(st))
     abi.check(rc, lib)
     return E, Pabs, Pt.value, st
+
+
+def lu9(lib, coefs, b, boxes, x=None):
+    """coefs = (aP, aE, aW, aN, aS, aEN, aES, aWN, aWS)"""
+    ni2, ld = b.shape
+    x = np.zeros_like(b) if x is None else x
+    bk, bp, nb = _boxes(boxes)
+    st = abi.Status()
+    rc = lib.cathode_lu9_solve(*[abi.dptr(np.ascontiguousarray(a)) for a in coefs], abi.dptr(np.ascontiguousarray(b)),
+                               bp, nb, ni2, ld, ld, abi.dptr(x), ctypes.byref(st))
+    abi.check(rc, lib)
+    return x, st

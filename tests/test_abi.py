@@ -5,6 +5,7 @@ import os
 import re
 import subprocess
 
+import numpy as np
 import pytest
 
 from oracle_lib import ROOT, abi
@@ -45,5 +46,36 @@ def test_library_loads_and_binds_without_gpu():
 
 
 def test_python_binding_fails_loudly_when_library_missing(tmp_path):
-    with pytest.raises(RuntimeError):
-        abi.load(str(tmp_path / "nope.so")) if abi._LIB is None else abi.load.__wrapped__  # noqa
+    saved = abi._LIB
+    abi._LIB = None
+    try:
+        with pytest.raises(RuntimeError):
+            abi.load(str(tmp_path / "nope.so"))
+    finally:
+        abi._LIB = saved
+
+
+def test_binding_covers_every_declared_function():
+    bound = {n for n, _, _ in abi.SIGNATURES}
+    assert declared_functions() <= bound, sorted(declared_functions() - bound)
+
+
+def test_entry_points_refuse_before_init():
+    """Without cathode_init every compute entry returns CATHODE_ERR_NOINIT
+    before touching the device (so this runs without a GPU)."""
+    lib = abi.bind(ctypes.CDLL(abi.LIB_PATH))
+    z = np.zeros((5, 5))
+    box = np.array([1, 3, 1, 3], dtype=np.int32)
+    bp = box.ctypes.data_as(abi.c_int_p)
+    d = abi.dptr(z)
+    st = abi.Status()
+    assert lib.cathode_struct5_solve(d, d, d, d, d, d, bp, 1, 5, 5, 5, 10, 1e-10, 0, d,
+                                     ctypes.byref(st)) == abi.CATHODE_ERR_NOINIT
+    assert b"cathode_init" in lib.cathode_last_error()
+    assert lib.cathode_sor_solve(d, d, d, d, d, d, bp, 1, 5, 5, 5, 1.0, 10, 1e-10, d,
+                                 ctypes.byref(st)) == abi.CATHODE_ERR_NOINIT
+    assert lib.cathode_lu9_solve(*([d] * 10), bp, 1, 5, 5, 5, d, ctypes.byref(st)) == abi.CATHODE_ERR_NOINIT
+    assert lib.cathode_helm5_solve(d, d, d, d, d, d, bp, 1, 5, 5, 5, 10, 1e-10, 0, d,
+                                   ctypes.byref(st)) == abi.CATHODE_ERR_NOINIT
+    p = abi.EthetaParams()
+    assert lib.cathode_etheta_solve(ctypes.byref(p), d, d, d, d, None, ctypes.byref(st)) == abi.CATHODE_ERR_NOINIT

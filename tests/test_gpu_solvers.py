@@ -112,3 +112,48 @@ def test_sor_box_union_bit_exact(hiplib):
     n, err = O.sor(aP, aE, aW, aN, aS, b, boxes, 1.5, 40, 1e-8, xo)
     assert st.iterations == n
     np.testing.assert_array_equal(xg, xo)
+
+
+def random_nine(rng, ni, nj):
+    shp = (ni + 2, nj + 2)
+    five = [rng.uniform(0.5, 2.0, shp) for _ in range(4)]
+    diag = [rng.uniform(-0.15, 0.3, shp) for _ in range(4)]
+    aP = sum(five) + sum(np.abs(d) for d in diag) + 0.01
+    aE, aW, aN, aS = five
+    aP[1, :] += aW[1, :]  # Dirichlet wall on the left
+    aW[1, :] = 0
+    return (aP, aE, aW, aN, aS, *diag)
+
+
+def residual9(coefs, b, x, mask):
+    aP, aE, aW, aN, aS, aEN, aES, aWN, aWS = coefs
+    xm = np.where(mask, x, 0.0)
+    ax = np.zeros_like(x)
+    s = (slice(1, -1), slice(1, -1))
+    ax[s] = (aP[s] * xm[s] - aE[s] * xm[2:, 1:-1] - aW[s] * xm[:-2, 1:-1] - aN[s] * xm[1:-1, 2:]
+             - aS[s] * xm[1:-1, :-2] - aEN[s] * xm[2:, 2:] - aES[s] * xm[2:, :-2] - aWN[s] * xm[:-2, 2:]
+             - aWS[s] * xm[:-2, :-2])
+    r = np.where(mask, b - ax, 0.0)
+    return np.linalg.norm(r) / np.linalg.norm(np.where(mask, b, 0.0))
+
+
+@pytest.mark.parametrize("union", [False, True])
+def test_lu9_matches_direct(hiplib, union):
+    """cathode_lu9_solve replaces solver_LU_9p (solverFuncs.hpp:223-378), an
+    exact factorization: the iterative answer must agree with the oracle's
+    banded LU to 1e-11 relative L2 (solved to relative residual 1e-13)."""
+    rng = np.random.default_rng(21)
+    ni, nj = 60, 30
+    coefs = random_nine(rng, ni, nj)
+    b = rng.standard_normal(coefs[0].shape)
+    boxes = cathode_blocks(ni, nj) if union else [1, ni, 1, nj]
+    x0 = np.full(b.shape, -5.0)
+    print(f"lu9 {ni}x{nj} union={union}", flush=True)
+    x, st = G.lu9(hiplib, coefs, b, boxes, x=x0.copy())
+    xd = O.lu9_direct(coefs, b, boxes, x=x0.copy())
+    mask = xd != -5.0
+    print(f"  its={st.iterations} rel={st.rel_residual:.3e}", flush=True)
+    assert st.converged == 1 and st.rel_residual <= 1e-13
+    np.testing.assert_array_equal(x == -5.0, ~mask)
+    assert residual9(coefs, b, x, mask) <= 2e-13
+    assert O.rel_l2(x[mask], xd[mask]) < 1e-11
