@@ -45,7 +45,10 @@ class NddParams(ctypes.Structure):
         ("maxITR_HES_rhoeps", ctypes.c_int), ("maxITR_HES_rhoi", ctypes.c_int),
         ("error_cnv_HES_phi", ctypes.c_double), ("error_cnv_HES_rhoe", ctypes.c_double),
         ("error_cnv_HES_rhoeps", ctypes.c_double), ("error_cnv_HES_rhoi", ctypes.c_double),
-        ("flag_error", ctypes.c_int), ("maxITR", ctypes.c_int), ("tol", ctypes.c_double),
+        ("flag_error", ctypes.c_int), ("maxITR_SOR_phi", ctypes.c_int), ("maxITR_SOR_rhoe", ctypes.c_int),
+        ("maxITR_SOR_rhoeps", ctypes.c_int), ("maxITR_SOR_rhoi", ctypes.c_int),
+        ("error_cnv_SOR_phi", ctypes.c_double), ("error_cnv_SOR_rhoe", ctypes.c_double),
+        ("error_cnv_SOR_rhoeps", ctypes.c_double), ("error_cnv_SOR_rhoi", ctypes.c_double),
         ("mode", ctypes.c_int), ("learned", ctypes.c_int),
         ("see_eb", ctypes.c_double * SEE_MAX), ("see_rd", ctypes.c_double * SEE_MAX),
         ("see_ts", ctypes.c_double * SEE_MAX),

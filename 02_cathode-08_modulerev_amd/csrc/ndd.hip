@@ -736,11 +736,11 @@ static void account(NddHandle& H, const cathode_status& st)
 // gate: the PC stopping threshold the previous iteration's calcRes is
 // tested against (-inf for the first iteration).  Returns false when the
 // loop should have stopped before this iteration.
-static bool solve(NddHandle& H, double gate)
+static bool solve(NddHandle& H, double gate, int maxITR, double tol)
 {
     cathode_status st{};
     H.mg.side_cnv = gate;
-    H.mg.solve(H.p.maxITR, H.p.tol, H.p.mode, H.s, &st);
+    H.mg.solve(maxITR, tol, H.p.mode, H.s, &st);
     H.mg.side_cnv = -INFINITY;
     if (H.mg.aborted) return false;
     account(H, st);
@@ -799,7 +799,7 @@ static void poisson_routine(NddHandle& H, bool report)
         hipLaunchKernelGGL(k_poisson_assemble, ggrid(V.ni, V.nj), gblk(), 0, H.s, V, A, H.mg.aP(), H.mg.aE(),
                            H.mg.aW(), H.mg.aN(), H.mg.aS(), H.mg.rhs());
         CATHODE_LAUNCH_CHECK();
-        if (!solve(H, ncount > 1 ? H.p.error_cnv_HES_phi : -INFINITY)) {
+        if (!solve(H, ncount > 1 ? H.p.error_cnv_HES_phi : -INFINITY, H.p.maxITR_SOR_phi, H.p.error_cnv_SOR_phi)) {
             // calcRes of the previous iteration met the PC criterion: undo the prediction
             d2d(H, F[F_NUEX], H.tmp_x);
             d2d(H, F[F_NUER], H.tmp_r);
@@ -837,18 +837,21 @@ static void sg_routine(NddHandle& H, int sp, bool report, double* err3)
     A.eloss = H.has_src[2] ? F[F_ELOSS] : nullptr;
     A.Fx_t = H.tmp_x;
     A.Fr_t = H.tmp_r;
-    int maxit;
-    double cnv;
+    int maxit, lin_maxit;
+    double cnv, lin_tol;
     double* n_old;
     if (sp == SP_ELEC) {
         A.n = F[F_RHOE]; A.Fx = F[F_RHOUEX]; A.Fr = F[F_RHOUER]; A.T = F[F_TE]; A.m = H.me; A.tfac = 1.0;
         maxit = H.p.maxITR_HES_rhoe; cnv = H.p.error_cnv_HES_rhoe; n_old = F[F_RHOE_OLD];
+        lin_maxit = H.p.maxITR_SOR_rhoe; lin_tol = H.p.error_cnv_SOR_rhoe;
     } else if (sp == SP_ION) {
         A.n = F[F_RHOI]; A.Fx = F[F_RHOUIX]; A.Fr = F[F_RHOUIR]; A.T = F[F_TE]; A.m = H.mi; A.tfac = 1.0;
         maxit = H.p.maxITR_HES_rhoi; cnv = H.p.error_cnv_HES_rhoi; n_old = H.rhoi_old;
+        lin_maxit = H.p.maxITR_SOR_rhoi; lin_tol = H.p.error_cnv_SOR_rhoi;
     } else {
         A.n = F[F_RHOEPS]; A.Fx = F[F_GX]; A.Fr = F[F_GR]; A.T = H.Te_old; A.m = H.me; A.tfac = 3.0 / 5.0;
         maxit = H.p.maxITR_HES_rhoeps; cnv = H.p.error_cnv_HES_rhoeps; n_old = H.s_old;
+        lin_maxit = H.p.maxITR_SOR_rhoeps; lin_tol = H.p.error_cnv_SOR_rhoeps;
     }
     A.n_old = n_old;
     d2d(H, n_old, A.n);
@@ -865,7 +868,7 @@ static void sg_routine(NddHandle& H, int sp, bool report, double* err3)
         hipLaunchKernelGGL(k_sg_assemble, ggrid(V.ni, V.nj), gblk(), 0, H.s, V, A, H.mg.aP(), H.mg.aE(), H.mg.aW(),
                            H.mg.aN(), H.mg.aS(), H.mg.rhs());
         CATHODE_LAUNCH_CHECK();
-        if (!solve(H, ncount > 1 ? cnv : -INFINITY)) {
+        if (!solve(H, ncount > 1 ? cnv : -INFINITY, lin_maxit, lin_tol)) {
             d2d(H, A.Fx, H.tmp_x);
             d2d(H, A.Fr, H.tmp_r);
             break;

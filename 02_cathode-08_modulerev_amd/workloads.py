@@ -43,8 +43,8 @@ def default_params(ni, nj, dx, dr, dt=1e-10, V_bias=30.0, energy_on=False,
     p.error_cnv_HES_phi = p.error_cnv_HES_rhoe = p.error_cnv_HES_rhoeps = 1e-21
     p.error_cnv_HES_rhoi = 1e-21
     p.flag_error = 3
-    p.maxITR = 30
-    p.tol = 1e-10
+    p.maxITR_SOR_phi = p.maxITR_SOR_rhoe = p.maxITR_SOR_rhoeps = p.maxITR_SOR_rhoi = 30
+    p.error_cnv_SOR_phi = p.error_cnv_SOR_rhoe = p.error_cnv_SOR_rhoeps = p.error_cnv_SOR_rhoi = 1e-10
     p.mode = mode
     p.learned = 0
     return p

@@ -134,7 +134,9 @@ typedef struct cathode_etheta_params {
     double nu;                  /* electron collision frequency [1/s] */
     double epsr;                /* background relative permittivity */
     double masse;               /* electron mass [kg] */
-    int pec_z, pec_rmax;        /* 1 = PEC at the z walls / r = rmax, 0 = open */
+    int pec_z;                  /* left wall (z = xL): 1 = PEC, 0 = first-order open;
+                                   the right wall is PEC (emfieldModule.hpp:2729-2737) */
+    int pec_rmax;               /* outer wall r = rmax: 1 = PEC, 0 = first-order open */
     int maxITR;
     double tol;
     int mode;                   /* CATHODE_MODE_* */
@@ -168,8 +170,10 @@ typedef struct cathode_ndd_params {
     int maxITR_HES_phi, maxITR_HES_rhoe, maxITR_HES_rhoeps, maxITR_HES_rhoi;
     double error_cnv_HES_phi, error_cnv_HES_rhoe, error_cnv_HES_rhoeps, error_cnv_HES_rhoi;
     int flag_error;             /* calcRes mode 0..3 (miniFuncs.hpp:292-375) */
-    int maxITR;                 /* maxITR_SOR_*: linear-solve iteration cap */
-    double tol;                 /* error_cnv_SOR_*: linear-solve relative residual */
+    /* linear solves, per equation (solver_SMG's maxITR / error_cnv; setup.yaml
+     * solver_setting maxITR_SOR_* / error_cnv_SOR_*) */
+    int maxITR_SOR_phi, maxITR_SOR_rhoe, maxITR_SOR_rhoeps, maxITR_SOR_rhoi;
+    double error_cnv_SOR_phi, error_cnv_SOR_rhoe, error_cnv_SOR_rhoeps, error_cnv_SOR_rhoi;
     int mode;                   /* CATHODE_MODE_* for every linear solve */
     int learned;                /* 1 = learned initial guess for the Poisson correction */
     double see_eb[CATHODE_SEE_TABLE_MAX];

@@ -142,9 +142,8 @@ static void ctx_free(ctx_t* C)
 }
 
 /* the linear solve handed to solver_SMG in the reference */
-static void solve5(ctx_t* C, double* x)
+static void solve5(ctx_t* C, double* x, int maxITR, double tol)
 {
-    const cathode_ndd_params* p = C->p;
     int box[4] = {1, C->ni, 1, C->nj};
     double rel = 0.0;
     int its = 0;
@@ -154,14 +153,14 @@ static void solve5(ctx_t* C, double* x)
         its = 1;
     } else {
         its = orc_mg_solve(C->aP, C->aE, C->aW, C->aN, C->aS, C->b, box, 1, C->ni + 2, C->nj + 2, C->ld,
-                           p->maxITR, p->tol, x, &rel);
+                           maxITR, tol, x, &rel);
     }
     if (C->rep) {
         C->rep->solves++;
         C->rep->total_iterations += its;
         if (its > C->rep->max_iterations) C->rep->max_iterations = its;
         if (rel > C->rep->max_rel_residual) C->rep->max_rel_residual = rel;
-        if (C->solver != 0 && !(rel <= p->tol)) C->rep->nonconverged++;
+        if (C->solver != 0 && !(rel <= tol)) C->rep->nonconverged++;
     }
 }
 
@@ -338,7 +337,7 @@ static void poisson_routine(ctx_t* C, cathode_ndd_fields* f, double* phi_old, do
         memcpy(nUex_t, f->nUex, C->n * sizeof(double));
         memcpy(nUer_t, f->nUer, C->n * sizeof(double));
         poisson_predict_assemble(C, f, nUex_t, nUer_t);
-        solve5(C, dphi);
+        solve5(C, dphi, p->maxITR_SOR_phi, p->error_cnv_SOR_phi);
         /* ghost correction and wall-face field (:2850-2943) */
         for (int j = 1; j <= nj; j++) {
             dphi[IX(0, j)] = -dphi[IX(1, j)];
@@ -418,8 +417,8 @@ typedef struct {
     double mu;
     double cfac;    /* flux factor: 1 or 5/3 */
     double tfac;    /* time-term factor: 1 or 3/5 */
-    int maxit;
-    double cnv;
+    int maxit, lin_maxit;   /* PC iterations; linear-solve cap (maxITR_SOR_*) */
+    double cnv, lin_tol;    /* PC threshold; linear-solve tolerance (error_cnv_SOR_*) */
 } sg_eq_t;
 
 static void sg_predict_assemble(ctx_t* C, cathode_ndd_fields* f, int sp, const sg_eq_t* q,
@@ -568,7 +567,7 @@ static void sg_routine(ctx_t* C, cathode_ndd_fields* f, int sp, const sg_eq_t* q
         memcpy(Fx_t, q->Fx, C->n * sizeof(double));
         memcpy(Fr_t, q->Fr, C->n * sizeof(double));
         sg_predict_assemble(C, f, sp, q, Fx_t, Fr_t, n_old);
-        solve5(C, dn);
+        solve5(C, dn, q->lin_maxit, q->lin_tol);
         /* flux correction (:3682-3743) */
         for (int i = 2; i <= ni; i++)
             for (int j = 1; j <= nj; j++) {
@@ -606,12 +605,15 @@ static void set_eq(const cathode_ndd_params* p, cathode_ndd_fields* f, int sp, c
     if (sp == SP_ELEC) {
         q->n = f->rhoe; q->Fx = f->rhoUex; q->Fr = f->rhoUer; q->T = f->Te; q->mu = p->mu_e;
         q->cfac = 1.0; q->tfac = 1.0; q->maxit = p->maxITR_HES_rhoe; q->cnv = p->error_cnv_HES_rhoe;
+        q->lin_maxit = p->maxITR_SOR_rhoe; q->lin_tol = p->error_cnv_SOR_rhoe;
     } else if (sp == SP_ION) {
         q->n = f->rhoi; q->Fx = f->rhoUix; q->Fr = f->rhoUir; q->T = f->Te; q->mu = p->mu_i;
         q->cfac = 1.0; q->tfac = 1.0; q->maxit = p->maxITR_HES_rhoi; q->cnv = p->error_cnv_HES_rhoi;
+        q->lin_maxit = p->maxITR_SOR_rhoi; q->lin_tol = p->error_cnv_SOR_rhoi;
     } else {
         q->n = f->rhoeps; q->Fx = f->Gx; q->Fr = f->Gr; q->T = Te_old; q->mu = p->mu_e;
         q->cfac = 5.0 / 3.0; q->tfac = 3.0 / 5.0; q->maxit = p->maxITR_HES_rhoeps; q->cnv = p->error_cnv_HES_rhoeps;
+        q->lin_maxit = p->maxITR_SOR_rhoeps; q->lin_tol = p->error_cnv_SOR_rhoeps;
     }
 }
 

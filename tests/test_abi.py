@@ -79,3 +79,23 @@ def test_entry_points_refuse_before_init():
                                    ctypes.byref(st)) == abi.CATHODE_ERR_NOINIT
     p = abi.EthetaParams()
     assert lib.cathode_etheta_solve(ctypes.byref(p), d, d, d, d, None, ctypes.byref(st)) == abi.CATHODE_ERR_NOINIT
+
+
+@pytest.mark.parametrize("cname,pyname", [
+    ("cathode_status", "Status"), ("cathode_ndd_params", "NddParams"), ("cathode_ndd_fields", "NddFields"),
+    ("cathode_ndd_report", "NddReport"), ("cathode_etheta_params", "EthetaParams")])
+def test_ctypes_struct_layout_matches_header(cname, pyname, tmp_path):
+    """Every ctypes mirror has the C struct's size and field offsets (gcc on the header)."""
+    py = getattr(abi, pyname)
+    fields = [f[0] for f in py._fields_]
+    src = tmp_path / "lay.c"
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{ROOT}/include/cathode_hip.h"',
+             'int main(void) {', f'printf("%zu\\n", sizeof({cname}));']
+    lines += [f'printf("%zu\\n", offsetof({cname}, {f}));' for f in fields]
+    lines += ['return 0; }']
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "lay"
+    subprocess.run(["gcc", str(src), "-o", str(exe)], check=True)
+    out = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    assert out[0] == ctypes.sizeof(py)
+    assert out[1:] == [getattr(py, f).offset for f in fields]

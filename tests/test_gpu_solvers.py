@@ -157,3 +157,16 @@ def test_lu9_matches_direct(hiplib, union):
     np.testing.assert_array_equal(x == -5.0, ~mask)
     assert residual9(coefs, b, x, mask) <= 2e-13
     assert O.rel_l2(x[mask], xd[mask]) < 1e-11
+
+
+def test_solverfuncs_dropin_header():
+    """solver_SMG / solver_SOR / solver_LU_9p called with the reference's C++
+    signatures (host/solverFuncs.hpp) on a two-box system: SMG and LU_9p
+    against the oracle's banded LU, SOR bit-exact with the oracle's sweep."""
+    import subprocess
+
+    import host_build
+    exe = host_build.build_host_check()
+    r = subprocess.run([exe, "gpu"], capture_output=True, text=True, timeout=120)
+    print(r.stdout, flush=True)
+    assert r.returncode == 0 and "ok dropin" in r.stdout, r.stdout + r.stderr
