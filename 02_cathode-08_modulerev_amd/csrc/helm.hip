@@ -33,7 +33,7 @@ __device__ inline float abs2v(float v) { return v * v; }
 namespace cathode {
 
 constexpr int HX = 64, HY = 4;
-constexpr int HT_TI = 8, HT_TJ = 32;  // complex tiles (register budget)
+constexpr int HT_CFG = 2;  // complex tiles: 8 x 32 cells, 256 threads (register budget)
 constexpr int HL_MAX = 24;
 
 static inline dim3 hgrid(int ni, int nj) { return dim3(cdiv(nj, HX), cdiv(ni, HY)); }
@@ -598,9 +598,8 @@ void HelmSolver::vcycle(Level<T>* L, hipStream_t s)
         Level<T>& A = L[l];
         Level<T>& C = L[l + 1];
         VLevel<T> V{A.ni, A.nj, A.ld, A.aP, A.aE, A.aW, A.aN, A.aS, actall.p + off[l], A.b, A.x};
-        hipLaunchKernelGGL((k_vtile<T, HT_TI, HT_TJ, 5, true>), dim3(cdiv(A.nj, HT_TJ), cdiv(A.ni, HT_TI)), dim3(VT_NT),
-                           0, s, V, A.cx ? 2 : 1, A.cy ? 2 : 1, l == 0 ? (const double*)w0.p : (const double*)nullptr,
-                           C.ni, C.nj, C.ld, C.b);
+        launch_vtile_cfg<T, true, HT_CFG>(l == 0, s, V, A.cx ? 2 : 1, A.cy ? 2 : 1, (const double*)w0.p, C.ni, C.nj,
+                                           C.ld, C.b);
     }
     TailArgs<T> ta{};
     ta.nt = nl - lt;
@@ -630,9 +629,8 @@ void HelmSolver::vcycle(Level<T>* L, hipStream_t s)
         Level<T>& A = L[l];
         Level<T>& C = L[l + 1];
         VLevel<T> V{A.ni, A.nj, A.ld, A.aP, A.aE, A.aW, A.aN, A.aS, actall.p + off[l], A.b, A.x};
-        hipLaunchKernelGGL((k_vtile<T, HT_TI, HT_TJ, 4, false>), dim3(cdiv(A.nj, HT_TJ), cdiv(A.ni, HT_TI)),
-                           dim3(VT_NT), 0, s, V, A.cx ? 2 : 1, A.cy ? 2 : 1, (const double*)nullptr, C.ni, C.nj, C.ld,
-                           C.x);
+        launch_vtile_cfg<T, false, HT_CFG>(l == 0, s, V, A.cx ? 2 : 1, A.cy ? 2 : 1, (const double*)nullptr, C.ni,
+                                            C.nj, C.ld, C.x);
     }
     CATHODE_LAUNCH_CHECK();
 }

@@ -1,5 +1,6 @@
 // mg.hip -- multigrid kernels and the solve loop (see mg.hpp for the algorithm).
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <chrono>
 
@@ -542,6 +543,10 @@ void MGSolver::configure(int ni, int nj, hipStream_t s)
     if (ni == cfg_ni && nj == cfg_nj) return;
     cfg_ni = ni;
     cfg_nj = nj;
+    for (int q = 0; q < 2; q++) {
+        tuned[q] = false;
+        for (int l = 0; l < MG_MAX_LEVELS; l++) tiles[q][l] = 0;
+    }
     // shapes
     int dims[MG_MAX_LEVELS][2];
     int n = 0;
@@ -699,11 +704,9 @@ void MGSolver::vcycle(Level<T>* L, hipStream_t s)
         Level<T>& A = L[l];
         Level<T>& C = L[l + 1];
         VLevel<T> V{A.ni, A.nj, A.ld, A.aP, A.aE, A.aW, A.aN, A.aS, actall.p + level_off[l], A.b, A.x};
-        const dim3 g(cdiv(A.nj, VT_TJ), cdiv(A.ni, VT_TI));
         const bool rec = prof.enabled && l == 0;
         if (rec) prof_begin(s);
-        hipLaunchKernelGGL((k_vtile<T, VT_TI, VT_TJ, 5, true>), g, dim3(VT_NT), 0, s, V, A.cx ? 2 : 1, A.cy ? 2 : 1,
-                           l == 0 ? w : (const double*)nullptr, C.ni, C.nj, C.ld, C.b);
+        launch_vtile<T, true>(tile_of<T>(l), l == 0, s, V, A.cx ? 2 : 1, A.cy ? 2 : 1, w, C.ni, C.nj, C.ld, C.b);
         if (rec) prof_end(s);
     }
     // coarse levels in one workgroup
@@ -737,12 +740,8 @@ void MGSolver::vcycle(Level<T>* L, hipStream_t s)
         Level<T>& A = L[l];
         Level<T>& C = L[l + 1];
         VLevel<T> V{A.ni, A.nj, A.ld, A.aP, A.aE, A.aW, A.aN, A.aS, actall.p + level_off[l], A.b, A.x};
-        const dim3 g(cdiv(A.nj, VT_TJ), cdiv(A.ni, VT_TI));
-        const bool rec = prof.enabled && l == 0;
-        if (rec) prof_begin(s);
-        hipLaunchKernelGGL((k_vtile<T, VT_TI, VT_TJ, 4, false>), g, dim3(VT_NT), 0, s, V, A.cx ? 2 : 1, A.cy ? 2 : 1,
-                           (const double*)nullptr, C.ni, C.nj, C.ld, C.x);
-        if (rec) prof_end(s);
+        launch_vtile<T, false>(tile_of<T>(l), l == 0, s, V, A.cx ? 2 : 1, A.cy ? 2 : 1, (const double*)nullptr, C.ni,
+                               C.nj, C.ld, C.x);
     }
     CATHODE_LAUNCH_CHECK();
 }
@@ -764,6 +763,57 @@ double MGSolver::read_scalar(const double* d, hipStream_t s)
     CATHODE_HIP(hipMemcpyAsync(h_scal, d, sizeof(double), hipMemcpyDeviceToHost, s));
     CATHODE_HIP(hipStreamSynchronize(s));
     return h_scal[0];
+}
+
+template <class T>
+float MGSolver::time_tile(Level<T>* L, int l, int cfg, hipStream_t s)
+{
+    Level<T>& A = L[l];
+    Level<T>& C = L[l + 1];
+    const double* w = user_w ? user_w : w0.p;
+    VLevel<T> V{A.ni, A.nj, A.ld, A.aP, A.aE, A.aW, A.aN, A.aS, actall.p + level_off[l], A.b, A.x};
+    hipEvent_t e0, e1;
+    CATHODE_HIP(hipEventCreate(&e0));
+    CATHODE_HIP(hipEventCreate(&e1));
+    // one untimed pair (code load), then 4 timed pairs
+    float ms = 0.f;
+    for (int rep = 0; rep < 5; rep++) {
+        if (rep == 1) CATHODE_HIP(hipEventRecord(e0, s));
+        launch_vtile<T, true>(cfg, l == 0, s, V, A.cx ? 2 : 1, A.cy ? 2 : 1, w, C.ni, C.nj, C.ld, C.b);
+        launch_vtile<T, false>(cfg, l == 0, s, V, A.cx ? 2 : 1, A.cy ? 2 : 1, (const double*)nullptr, C.ni, C.nj,
+                               C.ld, C.x);
+    }
+    CATHODE_HIP(hipEventRecord(e1, s));
+    CATHODE_LAUNCH_CHECK();
+    CATHODE_HIP(hipEventSynchronize(e1));
+    CATHODE_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return ms;
+}
+
+void MGSolver::autotune(int mixed, hipStream_t s)
+{
+    const int q = mixed ? 1 : 0;
+    for (int l = 0; l < lt; l++) {
+        int best = 0;
+        float tbest = 0.f;
+        for (int cfg = 0; cfg < VT_NCFG; cfg++) {
+            // the down and up kernels overwrite only V-cycle scratch (x of this
+            // level, b of the next), which every V-cycle rewrites before use
+            const float t = mixed ? time_tile<float>(L32, l, cfg, s) : time_tile<double>(L64, l, cfg, s);
+            if (cfg == 0 || t < tbest) {
+                best = cfg;
+                tbest = t;
+            }
+        }
+        tiles[q][l] = best;
+        if (std::getenv("CATHODE_VERBOSE"))
+            std::fprintf(stderr, "[cathode] autotune %s level %d (%dx%d): tile %dx%d/%d, %.2f us per down+up\n",
+                         mixed ? "fp32" : "fp64", l, L64[l].ni, L64[l].nj, VT_CFGS[best].ti, VT_CFGS[best].tj,
+                         VT_CFGS[best].nt, 1e3 * tbest / 4);
+    }
+    tuned[q] = true;
 }
 
 void MGSolver::enable_nine(bool on)
@@ -789,6 +839,17 @@ int MGSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_sta
     const int mixed = mode == CATHODE_MODE_MIXED ? 1 : 0;
     const dim3 g = grid2(F.ni, F.nj), blk(BX, BY);
     setup(mode, s);
+    {
+        const int q = mixed ? 1 : 0;
+        const char* force = std::getenv("CATHODE_TILE");
+        if (force && *force) {
+            const int cfg = std::atoi(force);
+            for (int l = 0; l < MG_MAX_LEVELS; l++) tiles[q][l] = (cfg >= 0 && cfg < VT_NCFG) ? cfg : 0;
+            tuned[q] = false;
+        } else if (autotune_on && !tuned[q]) {
+            autotune(mixed, s);
+        }
+    }
     CATHODE_HIP(hipMemsetAsync(xsol.p, 0, n0 * sizeof(double), s));
     double* r = kv.p;
     double* r0 = r + n0;
@@ -893,7 +954,9 @@ int MGSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_sta
         // kernel reads 5 coefficients + b and writes x and 1/4 coarse b (7.25
         // values), the up kernel also reads x and 1/4 coarse x (8.25 values)
         const double vb = mixed ? 4.0 : 8.0;
-        prof.bytes_per_launch = 0.5 * (7.25 + 8.25) * vb * (double)F.ni * (double)F.nj;
+        // finest-level down kernel: reads aP aE aW aN aS b (6 values) and the
+        // activity byte, writes x and a quarter value of coarse b per cell
+        prof.bytes_per_launch = (7.25 * vb + 1.0) * (double)F.ni * (double)F.nj;
     }
     auto t1 = std::chrono::steady_clock::now();
     if (st) {

@@ -99,11 +99,25 @@ public:
 
     int levels() const { return nl; }
 
+    // Tile configuration (index into VT_CFGS) of each fine level, per
+    // precision (0 = FP64, 1 = FP32).  Chosen by autotune() on the first
+    // solve of a shape, unless CATHODE_TILE=<index> forces one for all levels
+    // or set_autotune(false) keeps the defaults.
+    int tile(int prec, int level) const { return tiles[prec][level]; }
+    void set_autotune(bool on) { autotune_on = on; }
+    // Times every configuration's down + up kernels on each fine level
+    // (the current hierarchy of the given precision) and keeps the fastest.
+    void autotune(int mixed, hipStream_t s);
+
 private:
     void setup(int mode, hipStream_t s);
     struct CoarsenArgs coarsen_args(int l, const double* w);
     template <class T>
     void vcycle(Level<T>* L, hipStream_t s);
+    template <class T>
+    int tile_of(int l) const { return tiles[sizeof(T) == 4 ? 1 : 0][l]; }
+    template <class T>
+    float time_tile(Level<T>* L, int l, int cfg, hipStream_t s);
     void prof_begin(hipStream_t s);
     void prof_end(hipStream_t s);
     double read_scalar(const double* d, hipStream_t s);
@@ -122,6 +136,9 @@ private:
     const double* user_w = nullptr;
     DevBuf<double> nine;       // nine-point couplings (enable_nine)
     bool use_nine = false;
+    int tiles[2][MG_MAX_LEVELS] = {};
+    bool tuned[2] = {false, false};
+    bool autotune_on = true;
     int lt = 0;                // first level handled by the one-workgroup tail kernel
     int tail_cells = 0;        // LDS cells of the tail levels
     DevBuf<double> part;       // reduction partials (2 slots)

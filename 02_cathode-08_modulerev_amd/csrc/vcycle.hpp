@@ -17,8 +17,15 @@
 
 namespace cathode {
 
-constexpr int VT_TI = 16, VT_TJ = 32;   // tile of the fused fine-level kernels
-constexpr int VT_NT = 256;              // threads of a tile workgroup
+// Tile shapes of the fused fine-level kernels: TI x TJ cells per workgroup of
+// NT threads.  The multigrid picks one per level (autotuned on the first solve
+// of a shape); index 0 is the default.
+struct TileCfg {
+    int ti, tj, nt;
+};
+constexpr int VT_NCFG = 6;
+constexpr TileCfg VT_CFGS[VT_NCFG] = {{16, 32, 256}, {16, 32, 512}, {8, 32, 256},
+                                      {16, 64, 512}, {32, 32, 512}, {8, 64, 512}};
 constexpr int TAIL_MAX_CELLS = 512;     // levels with at most this many cells go to the tail
 constexpr int TAIL_NT = 512;            // threads of the tail workgroup
 constexpr int TAIL_MAX_LEVELS = 16;
@@ -33,13 +40,15 @@ struct VLevel {
 };
 
 // ------------------------------------------------------------------ fine-level tiles
-template <class T, int TI, int TJ, int H, bool DOWN>
-__global__ void __launch_bounds__(VT_NT)
+// TOP: the finest level (its own instantiation, so profiles separate it;
+// only there are the restriction weights w applied).
+template <class T, int TI, int TJ, int NT, int H, bool DOWN, bool TOP>
+__global__ void __launch_bounds__(NT)
     k_vtile(VLevel<T> F, int fi, int fj, const double* __restrict__ w, int cni, int cnj, int cld,
             T* __restrict__ cbx)
 {
     constexpr int RI = TI + 2 * H, RJ = TJ + 2 * H, R = RI * RJ;
-    constexpr int K = (R + VT_NT - 1) / VT_NT;
+    constexpr int K = (R + NT - 1) / NT;
     __shared__ T xs[R];
     __shared__ T rs[DOWN ? TI * TJ : 1];
     const int tid = threadIdx.x;
@@ -49,7 +58,7 @@ __global__ void __launch_bounds__(VT_NT)
     unsigned flags[K];  // bit0 active, bit1 colour, bit2 updatable (not on the region rim), bit3 tile cell
 #pragma unroll
     for (int k = 0; k < K; k++) {
-        const int c = tid + k * VT_NT;
+        const int c = tid + k * NT;
         flags[k] = 0;
         cP[k] = T(1);
         cE[k] = cW[k] = cN[k] = cS[k] = cB[k] = T(0);
@@ -84,7 +93,7 @@ __global__ void __launch_bounds__(VT_NT)
         const unsigned col = (unsigned)(s & 1) << 1;
 #pragma unroll
         for (int k = 0; k < K; k++) {
-            const int c = tid + k * VT_NT;
+            const int c = tid + k * NT;
             if ((flags[k] & 7u) == (5u | col)) {
                 T v = cB[k] + cE[k] * xs[c + RJ] + cW[k] * xs[c - RJ] + cN[k] * xs[c + 1] + cS[k] * xs[c - 1];
                 xs[c] = v / cP[k];
@@ -94,7 +103,7 @@ __global__ void __launch_bounds__(VT_NT)
     }
 #pragma unroll
     for (int k = 0; k < K; k++) {
-        const int c = tid + k * VT_NT;
+        const int c = tid + k * NT;
         if (!(flags[k] & 8u)) continue;
         const int ri = c / RJ, rj = c - ri * RJ;
         const int gi = gi0 + ri, gj = gj0 + rj;
@@ -104,7 +113,7 @@ __global__ void __launch_bounds__(VT_NT)
             if (flags[k] & 1u) {
                 r = cB[k] - cP[k] * xs[c] + cE[k] * xs[c + RJ] + cW[k] * xs[c - RJ] + cN[k] * xs[c + 1] +
                     cS[k] * xs[c - 1];
-                if (w) r = r * (T)w[gj];
+                if (TOP && w) r = r * (T)w[gj];
             }
             rs[(ri - H) * TJ + (rj - H)] = r;
         }
@@ -114,7 +123,7 @@ __global__ void __launch_bounds__(VT_NT)
         // coarse cells of this tile
         const int CI = TI / fi, CJ = TJ / fj;
         const int I0 = (int)blockIdx.y * CI, J0 = (int)blockIdx.x * CJ;
-        for (int q = tid; q < CI * CJ; q += VT_NT) {
+        for (int q = tid; q < CI * CJ; q += NT) {
             const int il = q / CJ, jl = q - il * CJ;
             const int I = I0 + il + 1, J = J0 + jl + 1;
             if (I > cni || J > cnj) continue;
@@ -127,6 +136,36 @@ __global__ void __launch_bounds__(VT_NT)
                 }
             cbx[(size_t)I * cld + J] = sum;
         }
+    }
+}
+
+// Launches the down (DOWN) or up kernel of one fine level with tile config cfg.
+template <class T, bool DOWN, int C>
+inline void launch_vtile_cfg(bool top, hipStream_t s, const VLevel<T>& V, int fi, int fj, const double* w, int cni,
+                             int cnj, int cld, T* cbx)
+{
+    constexpr TileCfg G = VT_CFGS[C];
+    constexpr int H = DOWN ? 5 : 4;
+    const dim3 grid(cdiv(V.nj, G.tj), cdiv(V.ni, G.ti));
+    if (top)
+        hipLaunchKernelGGL((k_vtile<T, G.ti, G.tj, G.nt, H, DOWN, true>), grid, dim3(G.nt), 0, s, V, fi, fj, w, cni,
+                           cnj, cld, cbx);
+    else
+        hipLaunchKernelGGL((k_vtile<T, G.ti, G.tj, G.nt, H, DOWN, false>), grid, dim3(G.nt), 0, s, V, fi, fj, w, cni,
+                           cnj, cld, cbx);
+}
+
+template <class T, bool DOWN>
+inline void launch_vtile(int cfg, bool top, hipStream_t s, const VLevel<T>& V, int fi, int fj, const double* w,
+                         int cni, int cnj, int cld, T* cbx)
+{
+    switch (cfg) {
+    case 1: launch_vtile_cfg<T, DOWN, 1>(top, s, V, fi, fj, w, cni, cnj, cld, cbx); break;
+    case 2: launch_vtile_cfg<T, DOWN, 2>(top, s, V, fi, fj, w, cni, cnj, cld, cbx); break;
+    case 3: launch_vtile_cfg<T, DOWN, 3>(top, s, V, fi, fj, w, cni, cnj, cld, cbx); break;
+    case 4: launch_vtile_cfg<T, DOWN, 4>(top, s, V, fi, fj, w, cni, cnj, cld, cbx); break;
+    case 5: launch_vtile_cfg<T, DOWN, 5>(top, s, V, fi, fj, w, cni, cnj, cld, cbx); break;
+    default: launch_vtile_cfg<T, DOWN, 0>(top, s, V, fi, fj, w, cni, cnj, cld, cbx); break;
     }
 }
 

@@ -170,3 +170,20 @@ def test_solverfuncs_dropin_header():
     r = subprocess.run([exe, "gpu"], capture_output=True, text=True, timeout=120)
     print(r.stdout, flush=True)
     assert r.returncode == 0 and "ok dropin" in r.stdout, r.stdout + r.stderr
+
+
+@pytest.mark.parametrize("cfg", range(6))
+def test_struct5_every_tile_config(hiplib, cfg, monkeypatch):
+    """Each fused-tile configuration the autotuner can pick gives a converged,
+    oracle-matching solve (odd grid, so partial tiles and odd coarsening occur)."""
+    monkeypatch.setenv("CATHODE_TILE", str(cfg))
+    rng = np.random.default_rng(100 + cfg)
+    ni, nj = 77, 45
+    aP, aE, aW, aN, aS = random_mmatrix(rng, ni, nj, mass=0.01)
+    b = rng.standard_normal(aP.shape)
+    box = [1, ni, 1, nj]
+    xd = O.struct5_direct(aP, aE, aW, aN, aS, b, box)
+    for mode in (0, 1):
+        x, st = G.struct5(hiplib, aP, aE, aW, aN, aS, b, box, mode=mode)
+        assert st.converged == 1, (cfg, mode, st.iterations)
+        assert O.rel_l2(x[1:-1, 1:-1], xd[1:-1, 1:-1]) < 1e-8, (cfg, mode)
