@@ -2,6 +2,7 @@
 // library stream, launch helpers and deterministic reductions (gfx950).
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>

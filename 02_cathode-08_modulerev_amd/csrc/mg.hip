@@ -621,7 +621,7 @@ void MGSolver::configure(int ni, int nj, hipStream_t s)
     // levels from lt on run inside the one-workgroup tail kernel
     lt = nl - 1;
     while (lt > 0 && (long)dims[lt - 1][0] * dims[lt - 1][1] <= TAIL_MAX_CELLS) lt--;
-    if (nl - lt > TAIL_MAX_LEVELS) lt = nl - TAIL_MAX_LEVELS;
+    if (nl - lt > TAIL_REG_LEVELS) lt = nl - TAIL_REG_LEVELS;
     tail_cells = 0;
     for (int l = lt; l < nl; l++) tail_cells += (dims[l][0] + 2) * (dims[l][1] + 2);
     part.alloc(3 * (cdiv(nj, BX) * cdiv(ni, BY)) + 64);
@@ -704,10 +704,10 @@ void MGSolver::vcycle(Level<T>* L, hipStream_t s)
         Level<T>& A = L[l];
         Level<T>& C = L[l + 1];
         VLevel<T> V{A.ni, A.nj, A.ld, A.aP, A.aE, A.aW, A.aN, A.aS, actall.p + level_off[l], A.b, A.x};
-        const bool rec = prof.enabled && l == 0;
-        if (rec) prof_begin(s);
-        launch_vtile<T, true>(tile_of<T>(l), l == 0, s, V, A.cx ? 2 : 1, A.cy ? 2 : 1, w, C.ni, C.nj, C.ld, C.b);
-        if (rec) prof_end(s);
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (prof.enabled && l == 0) prof_events(e0, e1);
+        launch_vtile<T, true>(tile_of<T>(l), l == 0, s, V, A.cx ? 2 : 1, A.cy ? 2 : 1, w, C.ni, C.nj, C.ld, C.b, e0,
+                              e1);
     }
     // coarse levels in one workgroup
     TailArgs<T> ta{};
@@ -734,7 +734,9 @@ void MGSolver::vcycle(Level<T>* L, hipStream_t s)
     ta.x_out = L[lt].x;
     ta.w = lt == 0 ? w : nullptr;
     const size_t lds = (size_t)off * (7 * sizeof(T) + 1) + 16;
-    hipLaunchKernelGGL(k_tail<T>, dim3(1), dim3(TAIL_NT), lds, s, ta);
+    (void)lds;
+    hipLaunchKernelGGL(k_tail_reg<T>, dim3(1), dim3(TAIL_NT), (size_t)off * 2 * sizeof(T) + TAIL_NT * sizeof(T) + 16,
+                       s, ta);
     // fine levels: fused up kernels
     for (int l = lt - 1; l >= 0; l--) {
         Level<T>& A = L[l];
@@ -746,17 +748,13 @@ void MGSolver::vcycle(Level<T>* L, hipStream_t s)
     CATHODE_LAUNCH_CHECK();
 }
 
-void MGSolver::prof_begin(hipStream_t s)
+void MGSolver::prof_events(hipEvent_t& a, hipEvent_t& b)
 {
-    hipEvent_t a, b;
     CATHODE_HIP(hipEventCreate(&a));
     CATHODE_HIP(hipEventCreate(&b));
     prof.ev.push_back(a);
     prof.ev.push_back(b);
-    CATHODE_HIP(hipEventRecord(a, s));
 }
-
-void MGSolver::prof_end(hipStream_t s) { CATHODE_HIP(hipEventRecord(prof.ev.back(), s)); }
 
 double MGSolver::read_scalar(const double* d, hipStream_t s)
 {

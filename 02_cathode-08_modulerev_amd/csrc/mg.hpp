@@ -87,7 +87,7 @@ public:
     // Solves A x = rhs from x = 0 into sol().  Returns iterations; fills st.
     int solve(int maxITR, double tol, int mode, hipStream_t s, cathode_status* st);
 
-    MGProfile prof;  // events around the level-0 fused V-cycle launches
+    MGProfile prof;  // dispatch events of the finest-level down kernel
 
     // A device double the caller writes before solve(); it travels with the
     // solve's first host round trip.  When it is below side_cnv the solve
@@ -118,8 +118,7 @@ private:
     int tile_of(int l) const { return tiles[sizeof(T) == 4 ? 1 : 0][l]; }
     template <class T>
     float time_tile(Level<T>* L, int l, int cfg, hipStream_t s);
-    void prof_begin(hipStream_t s);
-    void prof_end(hipStream_t s);
+    void prof_events(hipEvent_t& a, hipEvent_t& b);
     double read_scalar(const double* d, hipStream_t s);
 
     int nl = 0;

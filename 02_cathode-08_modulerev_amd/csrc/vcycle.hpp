@@ -31,6 +31,21 @@ constexpr int TAIL_NT = 512;            // threads of the tail workgroup
 constexpr int TAIL_MAX_LEVELS = 16;
 constexpr int COARSEST_MAX = 1;         // cells of the coarsest level (solved by one division)
 
+// XCD-aware tile order.  Workgroups are dealt to the 8 XCDs round-robin
+// (linear id b goes to XCD b % 8), each XCD with its own L2.  Mapping the
+// tiles of XCD x to one contiguous run of the raster order keeps the halo
+// rows neighbouring tiles share inside one L2.  Bijective for any count.
+constexpr int N_XCD = 8;
+__device__ inline void xcd_tile(int& ti, int& tj)
+{
+    const int n = (int)(gridDim.x * gridDim.y);
+    const int b = (int)(blockIdx.y * gridDim.x + blockIdx.x);
+    const int q = n / N_XCD, r = n % N_XCD, x = b % N_XCD;
+    const int t = x * q + min(x, r) + b / N_XCD;
+    ti = t / (int)gridDim.x;
+    tj = t - ti * (int)gridDim.x;
+}
+
 template <class T>
 struct VLevel {
     int ni, nj, ld;
@@ -52,8 +67,10 @@ __global__ void __launch_bounds__(NT)
     __shared__ T xs[R];
     __shared__ T rs[DOWN ? TI * TJ : 1];
     const int tid = threadIdx.x;
-    const int gi0 = (int)blockIdx.y * TI + 1 - H;  // global i of region row 0
-    const int gj0 = (int)blockIdx.x * TJ + 1 - H;
+    int bi, bj;  // this workgroup's tile
+    xcd_tile(bi, bj);
+    const int gi0 = bi * TI + 1 - H;  // global i of region row 0
+    const int gj0 = bj * TJ + 1 - H;
     T cP[K], cE[K], cW[K], cN[K], cS[K], cB[K];
     unsigned flags[K];  // bit0 active, bit1 colour, bit2 updatable (not on the region rim), bit3 tile cell
 #pragma unroll
@@ -122,7 +139,7 @@ __global__ void __launch_bounds__(NT)
         __syncthreads();
         // coarse cells of this tile
         const int CI = TI / fi, CJ = TJ / fj;
-        const int I0 = (int)blockIdx.y * CI, J0 = (int)blockIdx.x * CJ;
+        const int I0 = bi * CI, J0 = bj * CJ;
         for (int q = tid; q < CI * CJ; q += NT) {
             const int il = q / CJ, jl = q - il * CJ;
             const int I = I0 + il + 1, J = J0 + jl + 1;
@@ -131,7 +148,7 @@ __global__ void __launch_bounds__(NT)
             for (int a = 0; a < fi; a++)
                 for (int b = 0; b < fj; b++) {
                     const int ti = il * fi + a, tj = jl * fj + b;
-                    const int gi = (int)blockIdx.y * TI + ti + 1, gj = (int)blockIdx.x * TJ + tj + 1;
+                    const int gi = bi * TI + ti + 1, gj = bj * TJ + tj + 1;
                     if (gi <= F.ni && gj <= F.nj) sum += rs[ti * TJ + tj];
                 }
             cbx[(size_t)I * cld + J] = sum;
@@ -140,32 +157,43 @@ __global__ void __launch_bounds__(NT)
 }
 
 // Launches the down (DOWN) or up kernel of one fine level with tile config cfg.
+// With ev0/ev1 set the launch goes through hipExtLaunchKernelGGL, whose
+// events carry the dispatch's own start / end timestamps (kernel time only,
+// as a profiler reports it).
 template <class T, bool DOWN, int C>
 inline void launch_vtile_cfg(bool top, hipStream_t s, const VLevel<T>& V, int fi, int fj, const double* w, int cni,
-                             int cnj, int cld, T* cbx)
+                             int cnj, int cld, T* cbx, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr)
 {
     constexpr TileCfg G = VT_CFGS[C];
     constexpr int H = DOWN ? 5 : 4;
     const dim3 grid(cdiv(V.nj, G.tj), cdiv(V.ni, G.ti));
-    if (top)
+    if (ev0) {
+        if (top)
+            hipExtLaunchKernelGGL((k_vtile<T, G.ti, G.tj, G.nt, H, DOWN, true>), grid, dim3(G.nt), 0, s, ev0, ev1, 0,
+                                  V, fi, fj, w, cni, cnj, cld, cbx);
+        else
+            hipExtLaunchKernelGGL((k_vtile<T, G.ti, G.tj, G.nt, H, DOWN, false>), grid, dim3(G.nt), 0, s, ev0, ev1, 0,
+                                  V, fi, fj, w, cni, cnj, cld, cbx);
+    } else if (top) {
         hipLaunchKernelGGL((k_vtile<T, G.ti, G.tj, G.nt, H, DOWN, true>), grid, dim3(G.nt), 0, s, V, fi, fj, w, cni,
                            cnj, cld, cbx);
-    else
+    } else {
         hipLaunchKernelGGL((k_vtile<T, G.ti, G.tj, G.nt, H, DOWN, false>), grid, dim3(G.nt), 0, s, V, fi, fj, w, cni,
                            cnj, cld, cbx);
+    }
 }
 
 template <class T, bool DOWN>
 inline void launch_vtile(int cfg, bool top, hipStream_t s, const VLevel<T>& V, int fi, int fj, const double* w,
-                         int cni, int cnj, int cld, T* cbx)
+                         int cni, int cnj, int cld, T* cbx, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr)
 {
     switch (cfg) {
-    case 1: launch_vtile_cfg<T, DOWN, 1>(top, s, V, fi, fj, w, cni, cnj, cld, cbx); break;
-    case 2: launch_vtile_cfg<T, DOWN, 2>(top, s, V, fi, fj, w, cni, cnj, cld, cbx); break;
-    case 3: launch_vtile_cfg<T, DOWN, 3>(top, s, V, fi, fj, w, cni, cnj, cld, cbx); break;
-    case 4: launch_vtile_cfg<T, DOWN, 4>(top, s, V, fi, fj, w, cni, cnj, cld, cbx); break;
-    case 5: launch_vtile_cfg<T, DOWN, 5>(top, s, V, fi, fj, w, cni, cnj, cld, cbx); break;
-    default: launch_vtile_cfg<T, DOWN, 0>(top, s, V, fi, fj, w, cni, cnj, cld, cbx); break;
+    case 1: launch_vtile_cfg<T, DOWN, 1>(top, s, V, fi, fj, w, cni, cnj, cld, cbx, ev0, ev1); break;
+    case 2: launch_vtile_cfg<T, DOWN, 2>(top, s, V, fi, fj, w, cni, cnj, cld, cbx, ev0, ev1); break;
+    case 3: launch_vtile_cfg<T, DOWN, 3>(top, s, V, fi, fj, w, cni, cnj, cld, cbx, ev0, ev1); break;
+    case 4: launch_vtile_cfg<T, DOWN, 4>(top, s, V, fi, fj, w, cni, cnj, cld, cbx, ev0, ev1); break;
+    case 5: launch_vtile_cfg<T, DOWN, 5>(top, s, V, fi, fj, w, cni, cnj, cld, cbx, ev0, ev1); break;
+    default: launch_vtile_cfg<T, DOWN, 0>(top, s, V, fi, fj, w, cni, cnj, cld, cbx, ev0, ev1); break;
     }
 }
 
@@ -305,6 +333,143 @@ __global__ void __launch_bounds__(TAIL_NT) k_tail(TailArgs<T> A)
             const int i = 1 + q / nj, j = 1 + q % nj;
             A.x_out[(size_t)i * A.gld[0] + j] = X[i * ld + j];
         }
+    }
+}
+
+// ------------------------------------------------------------------ register-resident tail (real types)
+// Same levels, same sweeps and transfers as k_tail, organised for latency:
+// thread q owns cell q of every tail level and keeps that cell's
+// coefficients (and 1/aP) in registers, so LDS holds only x, b and the
+// residuals; phases whose cells all lie in the first wavefront (<= 64 cells)
+// synchronise that wavefront alone instead of the whole workgroup.
+constexpr int TAIL_REG_LEVELS = 10;  // tail levels of <= 512 cells: at most 10
+
+__device__ inline void tail_sync(int cells)
+{
+    if (cells > 64) {
+        __syncthreads();
+    } else {
+        // one wavefront: its LDS operations complete in issue order; keep
+        // the compiler from moving them across the phase boundary
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+template <class T>
+__global__ void __launch_bounds__(TAIL_NT) k_tail_reg(TailArgs<T> A)
+{
+    extern __shared__ unsigned char tail_lds[];
+    T* X = reinterpret_cast<T*>(tail_lds);
+    T* B = X + A.ncell;
+    T* R = B + A.ncell;  // residuals of one level, indexed by owner thread
+    const int q = threadIdx.x;
+    const int nt = A.nt;
+    for (int c = q; c < A.ncell; c += TAIL_NT) {
+        X[c] = T(0);
+        B[c] = T(0);
+    }
+    __syncthreads();
+    T cP[TAIL_REG_LEVELS], iP[TAIL_REG_LEVELS], cE[TAIL_REG_LEVELS], cW[TAIL_REG_LEVELS], cN[TAIL_REG_LEVELS],
+        cS[TAIL_REG_LEVELS];
+    int cc[TAIL_REG_LEVELS], par[TAIL_REG_LEVELS], ncl[TAIL_REG_LEVELS];
+    unsigned fl[TAIL_REG_LEVELS];  // bit0 active, bit1 colour
+    T wq = T(1);                   // restriction weight of this thread's first-level cell
+#pragma unroll
+    for (int l = 0; l < TAIL_REG_LEVELS; l++) {
+        cc[l] = -1;
+        par[l] = 0;
+        fl[l] = 0;
+        ncl[l] = 0;
+        cP[l] = iP[l] = T(1);
+        cE[l] = cW[l] = cN[l] = cS[l] = T(0);
+        if (l < nt) {
+            const int ni = A.ni[l], nj = A.nj[l], ld = nj + 2;
+            ncl[l] = ni * nj;
+            if (q < ni * nj) {
+                const int i = 1 + q / nj, j = 1 + q % nj;
+                cc[l] = A.off[l] + i * ld + j;
+                const size_t g = (size_t)i * A.gld[l] + j;
+                fl[l] = ((i + j) & 1) ? 2u : 0u;
+                if (A.act[l][g]) {
+                    fl[l] |= 1u;
+                    cP[l] = A.aP[l][g];
+                    iP[l] = T(1) / cP[l];
+                    cE[l] = A.aE[l][g];
+                    cW[l] = A.aW[l][g];
+                    cN[l] = A.aN[l][g];
+                    cS[l] = A.aS[l][g];
+                    if (l == 0) B[cc[l]] = A.b_in[g];
+                }
+                if (l + 1 < nt) {
+                    const int I = A.cx[l] ? (i + 1) >> 1 : i, J = A.cy[l] ? (j + 1) >> 1 : j;
+                    par[l] = A.off[l + 1] + I * (A.nj[l + 1] + 2) + J;
+                }
+                if (l == 0 && A.w) wq = (T)A.w[j];
+            }
+        }
+    }
+    __syncthreads();
+    // down: sweeps, residual, restriction
+#pragma unroll
+    for (int l = 0; l < TAIL_REG_LEVELS - 1; l++) {
+        if (l + 1 < nt) {
+            const int ld = A.nj[l] + 2, c = cc[l];
+            for (int s = 0; s < 4; s++) {
+                if (c >= 0 && (fl[l] & 3u) == (1u | ((unsigned)(s & 1) << 1)))
+                    X[c] = (B[c] + cE[l] * X[c + ld] + cW[l] * X[c - ld] + cN[l] * X[c + 1] + cS[l] * X[c - 1]) * iP[l];
+                tail_sync(ncl[l]);
+            }
+            if (c >= 0) {
+                T r = T(0);
+                if (fl[l] & 1u) {
+                    r = B[c] - cP[l] * X[c] + cE[l] * X[c + ld] + cW[l] * X[c - ld] + cN[l] * X[c + 1] +
+                        cS[l] * X[c - 1];
+                    if (l == 0) r = r * wq;
+                }
+                R[q] = r;
+            }
+            tail_sync(ncl[l]);
+            // coarse cell q of level l+1 sums its children's residuals
+            if (cc[l + 1] >= 0) {
+                const int nj = A.nj[l], fi = A.cx[l] ? 2 : 1, fj = A.cy[l] ? 2 : 1;
+                const int cnj = A.nj[l + 1];
+                const int I = 1 + q / cnj, J = 1 + q % cnj;
+                T sum = T(0);
+                for (int a = 0; a < fi; a++)
+                    for (int b = 0; b < fj; b++) {
+                        const int i = fi * (I - 1) + 1 + a, j = fj * (J - 1) + 1 + b;
+                        if (i <= A.ni[l] && j <= nj) sum += R[(i - 1) * nj + (j - 1)];
+                    }
+                B[cc[l + 1]] = sum;
+            }
+            tail_sync(ncl[l]);
+        }
+    }
+    // coarsest level: x = b / aP (inactive or zero diagonal: 0)
+#pragma unroll
+    for (int l = 0; l < TAIL_REG_LEVELS; l++)
+        if (l == nt - 1 && cc[l] >= 0) X[cc[l]] = ((fl[l] & 1u) && !(cP[l] == T(0))) ? B[cc[l]] / cP[l] : T(0);
+    // up: prolongation, sweeps
+#pragma unroll
+    for (int l = TAIL_REG_LEVELS - 2; l >= 0; l--) {
+        if (l + 1 < nt) {
+            const int ld = A.nj[l] + 2, c = cc[l];
+            tail_sync(ncl[l]);
+            if (c >= 0 && (fl[l] & 1u)) X[c] = X[c] + X[par[l]];
+            tail_sync(ncl[l]);
+            for (int s = 0; s < 4; s++) {
+                if (c >= 0 && (fl[l] & 3u) == (1u | ((unsigned)(s & 1) << 1)))
+                    X[c] = (B[c] + cE[l] * X[c + ld] + cW[l] * X[c - ld] + cN[l] * X[c + 1] + cS[l] * X[c - 1]) * iP[l];
+                if (s < 3) tail_sync(ncl[l]);
+            }
+        }
+    }
+    // first tail level out (each thread its own cell)
+    if (cc[0] >= 0) {
+        const int nj = A.nj[0];
+        const int i = 1 + q / nj, j = 1 + q % nj;
+        A.x_out[(size_t)i * A.gld[0] + j] = X[cc[0]];
     }
 }
 

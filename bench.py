@@ -49,7 +49,8 @@ def run_path(lib, mode, steps, warmup, seed):
     solves = rep.solves
     nonconv = rep.nonconverged
     maxres = rep.max_rel_residual
-    # roofline pass: HIP events around every fine-grid smoothing launch
+    # roofline pass: HIP events around every launch of the finest-level
+    # fused down kernel (k_vtile ..., DOWN, TOP), on the step's stream
     nprof = min(5, steps)
     abi.check(lib.cathode_ndd_profile(h, 1), lib)
     abi.check(lib.cathode_ndd_step(h, nprof, ctypes.byref(rep)), lib)
@@ -72,6 +73,34 @@ def run_path(lib, mode, steps, warmup, seed):
         "smooth_bytes": bpl.value,
         "smooth_GBps": bpl.value / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0,
     }
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_k_vtile_down_l0_512x256.json")
+
+
+def traffic_per_launch(path):
+    """HBM bytes per launch of the finest-level down kernel from the committed
+    rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE measurement (calibrated against a
+    copy kernel of known size), or None when it has not been measured."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+        return t["paths"][path]["hbm_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def calibrate_copy(mib=256, reps=5):
+    """A device-to-device copy of known size for the FETCH_SIZE / WRITE_SIZE
+    calibration: reps copies of `mib` MiB (read mib, write mib each)."""
+    import torch
+    a = torch.ones(mib * 1024 * 1024 // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": mib * 1024 * 1024, "copy_bytes_written": mib * 1024 * 1024,
+                      "reps": reps}))
 
 
 def cpu_baseline(seed, budget_s=15.0):
@@ -102,7 +131,12 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--paths", default="fp64,mixed")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run the copy kernel used to calibrate the HBM traffic counters")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
               "the single-instance implicit step does not shard across GPUs here", file=sys.stderr)
@@ -121,8 +155,11 @@ def main():
     best = "mixed" if "mixed" in res else paths[0]
     b = res[best]
     roof = {"bound": "hbm", "achieved": round(b["smooth_GBps"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(b["smooth_GBps"] / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": "k_smooth (finest level red-black half sweep)",
+            "frac": round(b["smooth_GBps"] / HBM_PEAK_GBS, 4), "traffic": traffic_per_launch(best),
+            "kernel": "k_vtile finest-level down kernel (2 red-black sweeps + residual + restriction), "
+                      + ("FP32" if best == "mixed" else "FP64"),
+            "bytes_model": "per cell: aP aE aW aN aS b read + x write + 1/4 coarse b write "
+                           "(7.25 values) + 1 activity byte",
             "bytes_per_launch": b["smooth_bytes"], "avg_us": round(b["smooth_avg_us"], 3)}
     out = {
         "metric": METRIC,
