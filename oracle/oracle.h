@@ -20,8 +20,13 @@
  *   orc_ndd_step (SG continuity)          exponential 1-D steady profile is reproduced
  *                                         exactly; M-matrix signs; global conservation
  *                                         identity (sum of volume divergence = wall flux)
- *   orc_helm_*                            manufactured solution; PEC cavity eigenvalue;
- *                                         Pabs closed form 0.5 Re(sigma)|E|^2
+ *   orc_helm_*                            manufactured solution (second-order convergence);
+ *                                         open-boundary reflection closed forms; ghost
+ *                                         wall conditions; Pabs closed form 0.5 Re(sigma)|E|^2
+ *   parity unpinned (no check beyond the restatement and the conservation identity):
+ *                                         energy-equation wall heat flux and SEE yields, ion
+ *                                         kinetic erf wall flux, J.E / Pabs / eloss source
+ *                                         terms, Te cap
  *   orc_abcd_*                            S -> ABCD -> S round trip is the identity
  * See tests/test_oracle_*.py for each check.
  */
