@@ -133,6 +133,8 @@ SIGNATURES = [
     ("cathode_ndd_get_state", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(NddFields)]),
     ("cathode_ndd_step", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(NddReport)]),
     ("cathode_ndd_destroy", ctypes.c_int, [ctypes.c_void_p]),
+    ("cathode_ndd_batch_step", ctypes.c_int,
+     [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(NddReport)]),
     ("cathode_ndd_profile", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("cathode_ndd_profile_read", ctypes.c_int,
      [ctypes.c_void_p, c_int_p, c_double_p, c_double_p]),

@@ -2,6 +2,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <chrono>
 
 #include "mg.hpp"
@@ -790,9 +791,27 @@ float MGSolver::time_tile(Level<T>* L, int l, int cfg, hipStream_t s)
     return ms;
 }
 
+// Tile choices per (shape, precision), shared by every solver of the process:
+// a shape is timed once, later solvers of the same shape reuse the result.
+struct TileCacheEntry {
+    int ni, nj, prec;
+    int tiles[MG_MAX_LEVELS];
+};
+static std::mutex g_tile_mu;
+static std::vector<TileCacheEntry> g_tile_cache;
+
 void MGSolver::autotune(int mixed, hipStream_t s)
 {
     const int q = mixed ? 1 : 0;
+    {
+        std::lock_guard<std::mutex> lk(g_tile_mu);
+        for (const auto& e : g_tile_cache)
+            if (e.ni == cfg_ni && e.nj == cfg_nj && e.prec == q) {
+                for (int l = 0; l < MG_MAX_LEVELS; l++) tiles[q][l] = e.tiles[l];
+                tuned[q] = true;
+                return;
+            }
+    }
     for (int l = 0; l < lt; l++) {
         int best = 0;
         float tbest = 0.f;
@@ -812,6 +831,10 @@ void MGSolver::autotune(int mixed, hipStream_t s)
                          VT_CFGS[best].nt, 1e3 * tbest / 4);
     }
     tuned[q] = true;
+    TileCacheEntry e{cfg_ni, cfg_nj, q, {}};
+    for (int l = 0; l < MG_MAX_LEVELS; l++) e.tiles[l] = tiles[q][l];
+    std::lock_guard<std::mutex> lk(g_tile_mu);
+    g_tile_cache.push_back(e);
 }
 
 void MGSolver::enable_nine(bool on)

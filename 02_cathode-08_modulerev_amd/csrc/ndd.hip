@@ -23,6 +23,7 @@
 // without multiply-add contraction, so they reproduce a sequential FP64
 // evaluation up to the last-ulp differences of exp/expm1/erf.
 #include <algorithm>
+#include <thread>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -942,7 +943,9 @@ int cathode_ndd_create(const cathode_ndd_params* p, void** handle)
         }
         std::unique_ptr<NddHandle> H(new NddHandle());
         H->p = *p;
-        H->s = ctx().stream;
+        // each instance has its own stream so independent instances can run
+        // side by side (cathode_ndd_batch_step)
+        CATHODE_HIP(hipStreamCreateWithFlags(&H->s, hipStreamNonBlocking));
         const int ni = p->ni, nj = p->nj;
         H->n = (size_t)(ni + 2) * (nj + 2);
         H->fields.alloc(19 * H->n);
@@ -998,8 +1001,11 @@ int cathode_ndd_destroy(void* handle)
 {
     NddHandle* H = (NddHandle*)handle;
     if (!H) return CATHODE_OK;
+    if (H->s) (void)hipStreamSynchronize(H->s);
     if (H->h_scal) (void)hipHostFree(H->h_scal);
+    hipStream_t s = H->s;
     delete H;
+    if (s) (void)hipStreamDestroy(s);
     return CATHODE_OK;
 }
 
@@ -1053,14 +1059,10 @@ int cathode_ndd_get_state(void* handle, cathode_ndd_fields* host)
     }
 }
 
-int cathode_ndd_step(void* handle, int nsteps, cathode_ndd_report* rep)
+// nsteps implicit steps of one instance on its own stream; fills rep (may be NULL)
+static int ndd_run(NddHandle* H, int nsteps, cathode_ndd_report* rep)
 {
-    try {
-        NddHandle* H = (NddHandle*)handle;
-        if (!H || nsteps < 0) {
-            set_error("ndd_step: invalid argument");
-            return CATHODE_ERR_ARG;
-        }
+    {
         auto t0 = std::chrono::steady_clock::now();
         std::memset(&H->rep, 0, sizeof(H->rep));
         for (int s = 0; s < nsteps; s++) ndd_one_step(*H, rep != nullptr && s == nsteps - 1);
@@ -1084,9 +1086,103 @@ int cathode_ndd_step(void* handle, int nsteps, cathode_ndd_report* rep)
             return CATHODE_ERR_NUMERIC;
         }
         return CATHODE_OK;
+    }
+}
+
+int cathode_ndd_step(void* handle, int nsteps, cathode_ndd_report* rep)
+{
+    try {
+        NddHandle* H = (NddHandle*)handle;
+        if (!H || nsteps < 0) {
+            set_error("ndd_step: invalid argument");
+            return CATHODE_ERR_ARG;
+        }
+        return ndd_run(H, nsteps, rep);
     } catch (const hip_failure&) {
         return CATHODE_ERR_HIP;
     }
+}
+
+int cathode_ndd_batch_step(void** handles, int n, int nsteps, int workers, cathode_ndd_report* reps)
+{
+    if (!handles || n < 0 || nsteps < 0) {
+        set_error("ndd_batch_step: invalid argument");
+        return CATHODE_ERR_ARG;
+    }
+    for (int k = 0; k < n; k++)
+        if (!handles[k]) {
+            set_error("ndd_batch_step: null handle");
+            return CATHODE_ERR_ARG;
+        }
+    if (n == 0 || nsteps == 0) return CATHODE_OK;
+    workers = std::max(1, std::min(workers, n));
+    // largest instances first, dealt to the least loaded worker (LPT)
+    std::vector<int> order(n);
+    for (int k = 0; k < n; k++) order[k] = k;
+    auto cells = [&](int k) { return (long)((NddHandle*)handles[k])->p.ni * ((NddHandle*)handles[k])->p.nj; };
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cells(a) > cells(b); });
+    std::vector<std::vector<int> > queue(workers);
+    std::vector<long> load(workers, 0);
+    for (int k : order) {
+        const int w = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        queue[w].push_back(k);
+        load[w] += cells(k);
+    }
+    // one instance per grid shape and precision runs its first step alone, so
+    // the tile autotuner times that shape on an otherwise idle GPU; the
+    // others reuse its choice
+    std::vector<int> done(n, 0);
+    std::vector<cathode_ndd_report> first(n);
+    std::vector<int> rc(n, CATHODE_OK);
+    {
+        std::vector<long> seen;
+        for (int k : order) {
+            NddHandle* H = (NddHandle*)handles[k];
+            const long key = ((long)H->p.ni << 32) ^ ((long)H->p.nj << 2) ^ H->p.mode;
+            if (std::find(seen.begin(), seen.end(), key) != seen.end()) continue;
+            seen.push_back(key);
+            try {
+                rc[k] = ndd_run(H, 1, &first[k]);
+            } catch (const hip_failure&) {
+                rc[k] = CATHODE_ERR_HIP;
+            }
+            done[k] = 1;
+            if (rc[k] != CATHODE_OK) return rc[k];
+        }
+    }
+    std::vector<std::thread> pool;
+    for (int w = 0; w < workers; w++)
+        pool.emplace_back([&, w] {
+            for (int k : queue[w]) {
+                NddHandle* H = (NddHandle*)handles[k];
+                try {
+                    cathode_ndd_report r{};
+                    if (nsteps - done[k] == 0) {
+                        if (reps) reps[k] = first[k];
+                        continue;
+                    }
+                    rc[k] = ndd_run(H, nsteps - done[k], &r);
+                    if (reps) {
+                        if (done[k]) {  // merge the counters of the first step
+                            r.steps += first[k].steps;
+                            r.solves += first[k].solves;
+                            r.total_iterations += first[k].total_iterations;
+                            r.max_iterations = std::max(r.max_iterations, first[k].max_iterations);
+                            r.nonconverged += first[k].nonconverged;
+                            r.max_rel_residual = std::max(r.max_rel_residual, first[k].max_rel_residual);
+                            r.seconds += first[k].seconds;
+                        }
+                        reps[k] = r;
+                    }
+                } catch (const hip_failure&) {
+                    rc[k] = CATHODE_ERR_HIP;
+                }
+            }
+        });
+    for (auto& t : pool) t.join();
+    for (int k = 0; k < n; k++)
+        if (rc[k] != CATHODE_OK) return rc[k];
+    return CATHODE_OK;
 }
 
 int cathode_ndd_profile(void* handle, int enable)

@@ -172,5 +172,24 @@ def config2(seed=2, ni=1024, nj=512, h=5e-5, mode=abi.CATHODE_MODE_FP64, maxITR=
     return p, np.ascontiguousarray(ne), J
 
 
+SWEEP_SIZES = [(64, 32), (128, 64), (256, 128), (512, 256)]
+
+
+def config3(seed=3, count=64, h=1e-4, mode=abi.CATHODE_MODE_MIXED, sizes=SWEEP_SIZES):
+    """Bias sweep: `count` independent instances, each grid drawn uniformly
+    from `sizes` (cells of h = 0.1 mm), bias uniform in [0, 60] V, and the
+    config-0 initial state (Gaussian ne = ni of 1e17 peak, sigma 1 mm, at mid-z
+    on the axis, 1e14 floor, +-1% noise, Te = 5 eV) with the instance's own
+    noise seed.  Returns a list of (params, state, bias)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for k in range(count):
+        ni, nj = sizes[int(rng.integers(0, len(sizes)))]
+        bias = float(rng.uniform(0.0, 60.0))
+        p, st = config0(seed=seed * 1000 + k, ni=ni, nj=nj, h=h, V_bias=bias, mode=mode)
+        out.append((p, st, bias))
+    return out
+
+
 def copy_state(st):
     return {k: v.copy() for k, v in st.items()}

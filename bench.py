@@ -103,6 +103,43 @@ def calibrate_copy(mib=256, reps=5):
                       "reps": reps}))
 
 
+def run_sweep(lib, steps, workers):
+    """Bias sweep (64 instances of 64x32..512x256 cells, bias 0-60 V, 50 steps
+    each by default): all instances stepped together by cathode_ndd_batch_step,
+    then one at a time, on fresh handles from the same state."""
+    inst = W.config3(mode=abi.CATHODE_MODE_MIXED)
+
+    def handles():
+        hs = []
+        for p, st, _ in inst:
+            h = ctypes.c_void_p()
+            abi.check(lib.cathode_ndd_create(ctypes.byref(p), ctypes.byref(h)), lib)
+            abi.check(lib.cathode_ndd_set_state(h, ctypes.byref(abi.fields_struct(st))), lib)
+            hs.append(h)
+        return hs
+
+    def run(w):
+        hs = handles()
+        arr = (ctypes.c_void_p * len(hs))(*[h.value for h in hs])
+        reps = (abi.NddReport * len(hs))()
+        t0 = time.perf_counter()
+        abi.check(lib.cathode_ndd_batch_step(arr, len(hs), steps, w, reps), lib)
+        dt = time.perf_counter() - t0
+        for h in hs:
+            lib.cathode_ndd_destroy(h)
+        return dt, [r.I_anode for r in reps], sum(r.nonconverged for r in reps)
+
+    t_batch, I_batch, nc = run(workers)
+    t_seq, I_seq, _ = run(1)
+    dev = max(abs(a - b) / max(abs(b), 1e-300) for a, b in zip(I_batch, I_seq))
+    n = len(inst)
+    return {"instances": n, "steps": steps, "workers": workers,
+            "sizes": [[p.ni, p.nj] for p, _, _ in inst], "bias_V": [round(b, 4) for _, _, b in inst],
+            "instance_steps_per_s": round(n * steps / t_batch, 2),
+            "instance_steps_per_s_one_at_a_time": round(n * steps / t_seq, 2),
+            "collector_current_A": I_batch, "max_rel_dev_vs_one_at_a_time": dev, "nonconverged": nc}
+
+
 def cpu_baseline(seed, budget_s=15.0):
     """The oracle's host copy of the same algorithm (FP64 multigrid-BiCGStab),
     one thread, on a bounded number of steps of the same workload."""
@@ -131,6 +168,10 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--paths", default="fp64,mixed")
+    ap.add_argument("--sweep", action="store_true",
+                    help="also run the 64-instance bias sweep (batched vs one at a time)")
+    ap.add_argument("--sweep-steps", type=int, default=50)
+    ap.add_argument("--sweep-workers", type=int, default=4)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run the copy kernel used to calibrate the HBM traffic counters")
     args = ap.parse_args()
@@ -183,6 +224,12 @@ def main():
     }
     if "fp64" in res and best != "fp64":
         out["speedup_vs_fp64"] = round(b["steps_per_s"] / res["fp64"]["steps_per_s"], 3)
+    if args.sweep:
+        out["sweep"] = run_sweep(lib, args.sweep_steps, args.sweep_workers)
+        s = out["sweep"]
+        print(f"# sweep: {s['instance_steps_per_s']} instance-steps/s batched ({s['workers']} workers), "
+              f"{s['instance_steps_per_s_one_at_a_time']} one at a time, max dev {s['max_rel_dev_vs_one_at_a_time']:.2e}",
+              file=sys.stderr, flush=True)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.seed)
     lib.cathode_finalize()

@@ -211,6 +211,19 @@ int cathode_ndd_get_state(void* handle, cathode_ndd_fields* host);
 int cathode_ndd_step(void* handle, int nsteps, cathode_ndd_report* rep);
 int cathode_ndd_destroy(void* handle);
 
+/* Independent instances (e.g. the points of a bias sweep, each its own
+ * handle and grid size) stepped side by side: nsteps implicit steps of every
+ * handle.  Each handle runs on its own stream; the handles are dealt to
+ * `workers` host threads largest-first (LPT on cell count), so up to
+ * `workers` instances are in flight on the GPU at once.  The first instance
+ * of each grid shape takes its first step alone so the tile autotuner sees
+ * an idle GPU; the other instances of that shape reuse its choice.  Results
+ * per instance are those of cathode_ndd_step on the same handle.  reps
+ * (may be NULL) gets one report per handle.  No reference counterpart: the
+ * reference runs one bias point per process. */
+int cathode_ndd_batch_step(void** handles, int n, int nsteps, int workers,
+                           cathode_ndd_report* reps);
+
 /* Kernel timing of the dominant stencil kernel inside cathode_ndd_step:
  * when enabled, HIP events bracket every launch of the fine-grid
  * red-black smoothing kernel on the step's stream.  Returns the launch

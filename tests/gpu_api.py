@@ -71,3 +71,23 @@ def lu9(lib, coefs, b, boxes, x=None):
                                bp, nb, ni2, ld, ld, abi.dptr(x), ctypes.byref(st))
     abi.check(rc, lib)
     return x, st
+
+
+def ndd_handle(lib, p, st):
+    h = ctypes.c_void_p()
+    abi.check(lib.cathode_ndd_create(ctypes.byref(p), ctypes.byref(h)), lib)
+    abi.check(lib.cathode_ndd_set_state(h, ctypes.byref(abi.fields_struct(st))), lib)
+    return h
+
+
+def ndd_state(lib, h, like):
+    out = {k: np.zeros_like(v) for k, v in like.items()}
+    abi.check(lib.cathode_ndd_get_state(h, ctypes.byref(abi.fields_struct(out))), lib)
+    return out
+
+
+def ndd_batch(lib, handles, nsteps, workers):
+    arr = (ctypes.c_void_p * len(handles))(*[h.value for h in handles])
+    reps = (abi.NddReport * len(handles))()
+    abi.check(lib.cathode_ndd_batch_step(arr, len(handles), nsteps, workers, reps), lib)
+    return list(reps)
