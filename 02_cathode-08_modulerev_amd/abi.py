@@ -114,6 +114,9 @@ SIGNATURES = [
     ("cathode_abi_version", ctypes.c_int, []),
     ("cathode_last_error", ctypes.c_char_p, []),
     ("cathode_last_status", None, [ctypes.POINTER(Status)]),
+    ("cathode_dev_alloc", ctypes.c_int, [ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p)]),
+    ("cathode_dev_free", ctypes.c_int, [ctypes.c_void_p]),
+    ("cathode_dev_copy", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]),
     ("cathode_struct5_solve", ctypes.c_int,
      _SOLVE5 + [ctypes.c_int, ctypes.c_double, ctypes.c_int, _D, ctypes.POINTER(Status)]),
     ("cathode_struct5_solve_dev", ctypes.c_int,
@@ -135,6 +138,21 @@ SIGNATURES = [
     ("cathode_ndd_destroy", ctypes.c_int, [ctypes.c_void_p]),
     ("cathode_ndd_batch_step", ctypes.c_int,
      [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(NddReport)]),
+    ("cathode_cnn_num_params", ctypes.c_int, []),
+    ("cathode_cnn_create", ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
+    ("cathode_cnn_destroy", ctypes.c_int, [ctypes.c_void_p]),
+    ("cathode_cnn_get_params", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]),
+    ("cathode_cnn_set_params", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]),
+    ("cathode_cnn_forward_dev", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                               ctypes.c_void_p]),
+    ("cathode_cnn_grad_dev", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                            ctypes.c_void_p, ctypes.POINTER(ctypes.c_double),
+                                            ctypes.POINTER(ctypes.c_float)]),
+    ("cathode_cnn_train_step_dev", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                  ctypes.c_void_p, ctypes.c_void_p, ctypes.c_float,
+                                                  ctypes.POINTER(ctypes.c_double)]),
+    ("cathode_ndd_set_learned", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    ("cathode_ndd_capture", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     ("cathode_ndd_profile", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("cathode_ndd_profile_read", ctypes.c_int,
      [ctypes.c_void_p, c_int_p, c_double_p, c_double_p]),
@@ -168,6 +186,45 @@ def load(path=LIB_PATH):
                 f"libcathode_hip.so not built ({path}); run __graft_entry__.build() first")
         _LIB = bind(ctypes.CDLL(path))
     return _LIB
+
+
+class DeviceArray:
+    """float64 device buffer owned through the library (cathode_dev_*), so
+    no second GPU runtime is needed in the process."""
+
+    def __init__(self, lib, shape):
+        self.lib = lib
+        self.shape = tuple(int(s) for s in shape)
+        self.size = int(np.prod(self.shape))
+        self.ptr = ctypes.c_void_p()
+        check(lib.cathode_dev_alloc(max(8, self.size * 8), ctypes.byref(self.ptr)), lib)
+
+    def at(self, k):
+        """pointer to the k-th sub-array along the first axis"""
+        step = self.size // self.shape[0]
+        return ctypes.c_void_p(self.ptr.value + 8 * step * k)
+
+    def upload(self, a, k=None):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        dst = self.ptr if k is None else self.at(k)
+        check(self.lib.cathode_dev_copy(dst, a.ctypes.data_as(ctypes.c_void_p), a.nbytes, 0), self.lib)
+
+    def download(self, k=None):
+        shape = self.shape if k is None else self.shape[1:]
+        out = np.zeros(shape)
+        src = self.ptr if k is None else self.at(k)
+        check(self.lib.cathode_dev_copy(out.ctypes.data_as(ctypes.c_void_p), src, out.nbytes, 1), self.lib)
+        return out
+
+    def copy_from(self, k, other, m):
+        """self[k] = other[m] (device to device)"""
+        nb = 8 * (self.size // self.shape[0])
+        check(self.lib.cathode_dev_copy(self.at(k), other.at(m), nb, 2), self.lib)
+
+    def free(self):
+        if self.ptr.value:
+            self.lib.cathode_dev_free(self.ptr)
+            self.ptr = ctypes.c_void_p()
 
 
 def check(rc, lib=None):

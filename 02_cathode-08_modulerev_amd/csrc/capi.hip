@@ -216,6 +216,53 @@ int cathode_sor_solve(const double* aP, const double* aE, const double* aW, cons
     CATHODE_GUARD_END
 }
 
+int cathode_dev_alloc(size_t bytes, void** ptr)
+{
+    CATHODE_GUARD_BEGIN
+    if (!require_init()) return CATHODE_ERR_NOINIT;
+    if (!ptr || bytes == 0) {
+        set_error("dev_alloc: invalid arguments");
+        return CATHODE_ERR_ARG;
+    }
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+        set_error("dev_alloc: out of device memory");
+        return CATHODE_ERR_NOMEM;
+    }
+    CATHODE_HIP(hipMemset(p, 0, bytes));
+    *ptr = p;
+    return CATHODE_OK;
+    CATHODE_GUARD_END
+}
+
+int cathode_dev_free(void* ptr)
+{
+    CATHODE_GUARD_BEGIN
+    if (ptr) CATHODE_HIP(hipFree(ptr));
+    return CATHODE_OK;
+    CATHODE_GUARD_END
+}
+
+int cathode_dev_copy(void* dst, const void* src, size_t bytes, int kind)
+{
+    CATHODE_GUARD_BEGIN
+    if (!require_init()) return CATHODE_ERR_NOINIT;
+    if ((!dst || !src) && bytes) {
+        set_error("dev_copy: null pointer");
+        return CATHODE_ERR_ARG;
+    }
+    const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : kind == 1 ? hipMemcpyDeviceToHost
+                                                                          : hipMemcpyDeviceToDevice;
+    if (kind < 0 || kind > 2) {
+        set_error("dev_copy: kind must be 0 (to device), 1 (to host) or 2 (device to device)");
+        return CATHODE_ERR_ARG;
+    }
+    CATHODE_HIP(hipMemcpyAsync(dst, src, bytes, k, ctx().stream));
+    CATHODE_HIP(hipStreamSynchronize(ctx().stream));
+    return CATHODE_OK;
+    CATHODE_GUARD_END
+}
+
 int cathode_lu9_solve(const double* aP, const double* aE, const double* aW, const double* aN, const double* aS,
                       const double* aEN, const double* aES, const double* aWN, const double* aWS, const double* b,
                       const int* boxes, int n_bl, int ni2, int nj2, int ld, double* x, cathode_status* st)

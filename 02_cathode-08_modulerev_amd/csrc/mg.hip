@@ -871,7 +871,8 @@ int MGSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_sta
             autotune(mixed, s);
         }
     }
-    CATHODE_HIP(hipMemsetAsync(xsol.p, 0, n0 * sizeof(double), s));
+    if (!keep_guess) CATHODE_HIP(hipMemsetAsync(xsol.p, 0, n0 * sizeof(double), s));
+    keep_guess = false;
     double* r = kv.p;
     double* r0 = r + n0;
     double* p = r0 + n0;

@@ -85,7 +85,10 @@ public:
     void lump_nine(hipStream_t s);
 
     // Solves A x = rhs from x = 0 into sol().  Returns iterations; fills st.
+    // With keep_guess set, the solve starts from the x the caller left in
+    // sol() (interior cells; ghosts 0) instead; the flag clears itself.
     int solve(int maxITR, double tol, int mode, hipStream_t s, cathode_status* st);
+    bool keep_guess = false;
 
     MGProfile prof;  // dispatch events of the finest-level down kernel
 

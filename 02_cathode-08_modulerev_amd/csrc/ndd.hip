@@ -30,6 +30,7 @@
 #include <memory>
 #include <vector>
 
+#include "learned.hpp"
 #include "mg.hpp"
 
 namespace cathode {
@@ -677,6 +678,131 @@ __global__ void k_fill_one(int n, unsigned char* a, int ni, int nj, int ld)
 }
 
 // ================================================================== host side
+// ------------------------------------------------------------------ learned initial guess
+// Sum of squares over interior cells of two planes (pitches la, lb; b may be
+// NULL): partials in part[bid], part[nb + bid]
+__global__ void k_learn_sumsq(NddView V, const double* __restrict__ a, int la, const double* __restrict__ b, int lb,
+                              double* __restrict__ part, int nb)
+{
+    __shared__ double lds[8];
+    THREAD_IJ(1, 1);
+    double sa = 0.0, sb = 0.0;
+    if (i <= V.ni && j <= V.nj) {
+        const double va = a[(size_t)i * la + j];
+        sa = va * va;
+        if (b) {
+            const double vb = b[(size_t)i * lb + j];
+            sb = vb * vb;
+        }
+    }
+    const double ta = block_sum<NB_X * NB_Y>(sa, lds);
+    const double tb = block_sum<NB_X * NB_Y>(sb, lds);
+    if (lin_tid() == 0) {
+        const int bid = blockIdx.y * gridDim.x + blockIdx.x;
+        part[bid] = ta;
+        part[nb + bid] = tb;
+    }
+}
+
+// out[0], out[1] = root mean square of the two planes
+__global__ void k_learn_rms(const double* __restrict__ part, int nb, double ncell, double* __restrict__ out)
+{
+    __shared__ double lds[8];
+    double a = 0.0, b = 0.0;
+    for (int k = threadIdx.x; k < nb; k += blockDim.x) {
+        a += part[k];
+        b += part[nb + k];
+    }
+    a = block_sum<256>(a, lds);
+    b = block_sum<256>(b, lds);
+    if (threadIdx.x == 0) {
+        out[0] = sqrt(a / ncell);
+        out[1] = sqrt(b / ncell);
+    }
+}
+
+// feature planes (pitch V.ld): b / rms(b), r / r_top, x_prev / rms(x_prev)
+__global__ void k_learn_features(NddView V, const double* __restrict__ b, const double* __restrict__ xprev,
+                                 const double* __restrict__ rms, double r_top, double* __restrict__ feat)
+{
+    THREAD_IJ(0, 0);
+    if (i > V.ni + 1 || j > V.nj + 1) return;
+    const size_t plane = (size_t)(V.ni + 2) * V.ld, p = IX(i, j);
+    const bool in = i >= 1 && i <= V.ni && j >= 1 && j <= V.nj;
+    const double sb = rms[0] > 0.0 ? 1.0 / rms[0] : 0.0, sx = rms[1] > 0.0 ? 1.0 / rms[1] : 0.0;
+    feat[p] = in ? b[IX0(i, j)] * sb : 0.0;
+    feat[plane + p] = in ? V.r[j] / r_top : 0.0;
+    feat[2 * plane + p] = (in && xprev) ? xprev[p] * sx : 0.0;
+}
+
+// A g (level-0 5-point arrays, pitch ld0) against b: partials of <Ag, b>, <Ag, Ag>
+__global__ void k_guess_dots(NddView V, const double* __restrict__ aP, const double* __restrict__ aE,
+                             const double* __restrict__ aW, const double* __restrict__ aN,
+                             const double* __restrict__ aS, const double* __restrict__ b,
+                             const double* __restrict__ g, double* __restrict__ part, int nb)
+{
+    __shared__ double lds[8];
+    THREAD_IJ(1, 1);
+    double d1 = 0.0, d2 = 0.0;
+    if (i <= V.ni && j <= V.nj) {
+        const size_t k = IX0(i, j);
+        const double ag = aP[k] * g[IX(i, j)] - aE[k] * g[IX(i + 1, j)] - aW[k] * g[IX(i - 1, j)] -
+                          aN[k] * g[IX(i, j + 1)] - aS[k] * g[IX(i, j - 1)];
+        d1 = ag * b[k];
+        d2 = ag * ag;
+    }
+    const double t1 = block_sum<NB_X * NB_Y>(d1, lds);
+    const double t2 = block_sum<NB_X * NB_Y>(d2, lds);
+    if (lin_tid() == 0) {
+        const int bid = blockIdx.y * gridDim.x + blockIdx.x;
+        part[bid] = t1;
+        part[nb + bid] = t2;
+    }
+}
+
+// alpha = <Ag, b> / <Ag, Ag> minimises |b - alpha A g|; 0 when undefined.
+// out[2] = alpha, out[3] = 1 when the guess is used
+__global__ void k_guess_alpha(const double* __restrict__ part, int nb, double* __restrict__ out)
+{
+    __shared__ double lds[8];
+    double a = 0.0, b = 0.0;
+    for (int k = threadIdx.x; k < nb; k += blockDim.x) {
+        a += part[k];
+        b += part[nb + k];
+    }
+    a = block_sum<256>(a, lds);
+    b = block_sum<256>(b, lds);
+    if (threadIdx.x == 0) {
+        const double al = b > 0.0 ? a / b : 0.0;
+        const bool ok = isfinite(al) && al != 0.0;
+        out[2] = ok ? al : 0.0;
+        out[3] = ok ? 1.0 : 0.0;
+    }
+}
+
+// solver initial guess x (pitch ld0) = alpha g on interior cells, 0 on ghosts
+__global__ void k_guess_set(NddView V, const double* __restrict__ g, const double* __restrict__ alpha,
+                            double* __restrict__ x)
+{
+    THREAD_IJ(0, 0);
+    if (i > V.ni + 1 || j > V.nj + 1) return;
+    const bool in = i >= 1 && i <= V.ni && j >= 1 && j <= V.nj;
+    x[IX0(i, j)] = in ? alpha[0] * g[IX(i, j)] : 0.0;
+}
+
+// previous-step solution (pitch V.ld) and, for training capture, the
+// target x / rms(x)
+__global__ void k_learn_store(NddView V, const double* __restrict__ x, const double* __restrict__ rms,
+                              double* __restrict__ xprev, double* __restrict__ target)
+{
+    THREAD_IJ(0, 0);
+    if (i > V.ni + 1 || j > V.nj + 1) return;
+    const bool in = i >= 1 && i <= V.ni && j >= 1 && j <= V.nj;
+    const double v = in ? x[IX0(i, j)] : 0.0;
+    xprev[IX(i, j)] = v;
+    if (target) target[IX(i, j)] = rms[0] > 0.0 ? v / rms[0] : 0.0;
+}
+
 struct NddHandle {
     cathode_ndd_params p{};
     NddView V{};
@@ -695,6 +821,13 @@ struct NddHandle {
     MGSolver mg;
     cathode_ndd_report rep{};
     int nb = 0;
+    // learned initial guess of the first Poisson correction solve of a step
+    void* cnn = nullptr;              // network (cathode_ndd_set_learned), not owned
+    DevBuf<double> lfeat, lguess, xprev, lpart, lscal;
+    bool have_xprev = false;
+    double* cap_feat = nullptr;       // capture for training (device, caller-owned)
+    double* cap_target = nullptr;
+    int guesses = 0, guesses_used = 0;
 };
 
 enum {
@@ -773,6 +906,60 @@ static double calcres3(NddHandle& H, const ResPairs& P, double* err_out)
     return H.h_scal[3];
 }
 
+// Features of the assembled first Poisson correction system, the network's
+// guess g, and the solver's initial guess alpha g with alpha minimising
+// |b - alpha A g| (never worse than x = 0; 0 when the guess does not help).
+// With capture set, the features are copied out for training.
+static void learned_guess(NddHandle& H)
+{
+    const NddView& V = H.V;
+    const size_t plane = (size_t)(V.ni + 2) * V.ld;
+    H.lfeat.alloc(3 * plane);
+    H.lguess.alloc(plane);
+    H.xprev.alloc(plane);
+    H.lpart.alloc(2 * H.nb + 64);
+    H.lscal.alloc(8);
+    const dim3 g1 = ggrid(V.ni, V.nj), g2 = ggrid(V.ni + 2, V.nj + 2);
+    hipLaunchKernelGGL(k_learn_sumsq, g1, gblk(), 0, H.s, V, (const double*)H.mg.rhs(), V.ld0,
+                       H.have_xprev ? (const double*)H.xprev.p : (const double*)nullptr, V.ld, H.lpart.p, H.nb);
+    hipLaunchKernelGGL(k_learn_rms, dim3(1), dim3(256), 0, H.s, (const double*)H.lpart.p, H.nb,
+                       (double)V.ni * V.nj, H.lscal.p);
+    const double r_top = H.p.rmin + V.nj * V.dr;
+    hipLaunchKernelGGL(k_learn_features, g2, gblk(), 0, H.s, V, (const double*)H.mg.rhs(),
+                       H.have_xprev ? (const double*)H.xprev.p : (const double*)nullptr, (const double*)H.lscal.p,
+                       r_top, H.lfeat.p);
+    CATHODE_LAUNCH_CHECK();
+    if (H.cap_feat)
+        CATHODE_HIP(hipMemcpyAsync(H.cap_feat, H.lfeat.p, 3 * plane * sizeof(double), hipMemcpyDeviceToDevice, H.s));
+    if (!(H.p.learned && H.cnn)) return;
+    cnn_infer(H.cnn, V.ni, V.nj, H.lfeat.p, H.lguess.p, H.s);
+    hipLaunchKernelGGL(k_guess_dots, g1, gblk(), 0, H.s, V, (const double*)H.mg.aP(), (const double*)H.mg.aE(),
+                       (const double*)H.mg.aW(), (const double*)H.mg.aN(), (const double*)H.mg.aS(),
+                       (const double*)H.mg.rhs(), (const double*)H.lguess.p, H.lpart.p, H.nb);
+    hipLaunchKernelGGL(k_guess_alpha, dim3(1), dim3(256), 0, H.s, (const double*)H.lpart.p, H.nb, H.lscal.p);
+    hipLaunchKernelGGL(k_guess_set, g2, gblk(), 0, H.s, V, (const double*)H.lguess.p, (const double*)H.lscal.p + 2,
+                       H.mg.sol());
+    CATHODE_LAUNCH_CHECK();
+    H.mg.keep_guess = true;
+    H.guesses++;
+}
+
+// after the first Poisson correction solve: keep its solution as the next
+// step's feature, and write the training target when capturing
+static void learned_store(NddHandle& H)
+{
+    const NddView& V = H.V;
+    const dim3 g1 = ggrid(V.ni, V.nj), g2 = ggrid(V.ni + 2, V.nj + 2);
+    hipLaunchKernelGGL(k_learn_sumsq, g1, gblk(), 0, H.s, V, (const double*)H.mg.sol(), V.ld0,
+                       (const double*)nullptr, V.ld, H.lpart.p, H.nb);
+    hipLaunchKernelGGL(k_learn_rms, dim3(1), dim3(256), 0, H.s, (const double*)H.lpart.p, H.nb,
+                       (double)V.ni * V.nj, H.lscal.p + 4);
+    hipLaunchKernelGGL(k_learn_store, g2, gblk(), 0, H.s, V, (const double*)H.mg.sol(), (const double*)H.lscal.p + 4,
+                       H.xprev.p, H.cap_target);
+    CATHODE_LAUNCH_CHECK();
+    H.have_xprev = true;
+}
+
 static void poisson_routine(NddHandle& H, bool report)
 {
     const NddView& V = H.V;
@@ -800,12 +987,15 @@ static void poisson_routine(NddHandle& H, bool report)
         hipLaunchKernelGGL(k_poisson_assemble, ggrid(V.ni, V.nj), gblk(), 0, H.s, V, A, H.mg.aP(), H.mg.aE(),
                            H.mg.aW(), H.mg.aN(), H.mg.aS(), H.mg.rhs());
         CATHODE_LAUNCH_CHECK();
+        const bool learn = ncount == 1 && ((H.p.learned && H.cnn) || H.cap_feat);
+        if (learn) learned_guess(H);
         if (!solve(H, ncount > 1 ? H.p.error_cnv_HES_phi : -INFINITY, H.p.maxITR_SOR_phi, H.p.error_cnv_SOR_phi)) {
             // calcRes of the previous iteration met the PC criterion: undo the prediction
             d2d(H, F[F_NUEX], H.tmp_x);
             d2d(H, F[F_NUER], H.tmp_r);
             break;
         }
+        if (learn) learned_store(H);
         hipLaunchKernelGGL(k_poisson_update_phi, ggrid(V.ni + 2, V.nj + 2), gblk(), 0, H.s, V, H.mg.sol(), H.dphi,
                            F[F_PHI], F[F_EX], F[F_ER]);
         hipLaunchKernelGGL(k_poisson_fields, ggrid(V.ni, V.nj), gblk(), 0, H.s, V, F[F_PHI], H.dphi, F[F_RHOE],
@@ -1009,6 +1199,29 @@ int cathode_ndd_destroy(void* handle)
     return CATHODE_OK;
 }
 
+int cathode_ndd_set_learned(void* handle, void* net)
+{
+    NddHandle* H = (NddHandle*)handle;
+    if (!H) {
+        set_error("ndd_set_learned: null handle");
+        return CATHODE_ERR_ARG;
+    }
+    H->cnn = net;
+    return CATHODE_OK;
+}
+
+int cathode_ndd_capture(void* handle, double* feat_dev, double* target_dev)
+{
+    NddHandle* H = (NddHandle*)handle;
+    if (!H || (!feat_dev) != (!target_dev)) {
+        set_error("ndd_capture: null handle or only one of the two buffers");
+        return CATHODE_ERR_ARG;
+    }
+    H->cap_feat = feat_dev;
+    H->cap_target = target_dev;
+    return CATHODE_OK;
+}
+
 int cathode_ndd_set_state(void* handle, const cathode_ndd_fields* host)
 {
     try {
@@ -1018,6 +1231,7 @@ int cathode_ndd_set_state(void* handle, const cathode_ndd_fields* host)
             return CATHODE_ERR_ARG;
         }
         cathode_ndd_fields f = *host;
+        H->have_xprev = false;  // a new state: no previous Poisson correction
         for (int k = 0; k < 19; k++) {
             const double* src = *field_slot(&f, k);
             if (!src) {

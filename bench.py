@@ -10,8 +10,10 @@ Gaussians, Te = 2..10 eV smooth seeded field, bias 30 V, dt = 1e-10 s.
 Synthetic state generated on the host (seed 1) and resident on the GPU before
 the timed region.
 
-Prints one JSON line.  `value` is the steps/s of the fastest GPU path
-(mixed precision); the plain-FP64 GPU path is reported beside it.
+Prints one JSON line.  `value` is the steps/s of the learned + mixed path
+(the learned initial guess of the first Poisson correction solve, FP32
+V-cycles with FP64 refinement); the plain-FP64 and mixed GPU paths are
+reported beside it.
 """
 import argparse
 import ctypes
@@ -26,15 +28,19 @@ sys.path.insert(0, ROOT)
 PKG = "02_cathode-08_modulerev_amd"
 abi = importlib.import_module(PKG + ".abi")
 W = importlib.import_module(PKG + ".workloads")
+L = importlib.import_module(PKG + ".learned")
 
 METRIC = "NDD steps/s at 512x256 (resid 1e-10): learned+mixed vs plain FP64, 1 GPU"
 HBM_PEAK_GBS = 8000.0
 
 
-def run_path(lib, mode, steps, warmup, seed):
+def run_path(lib, mode, steps, warmup, seed, net=None):
     p, st = W.config1(seed=seed, mode=mode)
+    p.learned = 1 if net is not None else 0
     h = ctypes.c_void_p()
     abi.check(lib.cathode_ndd_create(ctypes.byref(p), ctypes.byref(h)), lib)
+    if net is not None:
+        abi.check(lib.cathode_ndd_set_learned(h, net), lib)
     f = abi.fields_struct(st)
     abi.check(lib.cathode_ndd_set_state(h, ctypes.byref(f)), lib)
     rep = abi.NddReport()
@@ -167,7 +173,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--paths", default="fp64,mixed")
+    ap.add_argument("--paths", default="fp64,mixed,learned",
+                    help="fp64, mixed (FP32 V-cycles + FP64 refinement), learned (mixed + learned initial guess)")
     ap.add_argument("--sweep", action="store_true",
                     help="also run the 64-instance bias sweep (batched vs one at a time)")
     ap.add_argument("--sweep-steps", type=int, default=50)
@@ -186,19 +193,23 @@ def main():
     abi.check(lib.cathode_init(0), lib)
     paths = [s.strip() for s in args.paths.split(",") if s.strip()]
     res = {}
+    net = None
     for name in paths:
-        mode = abi.CATHODE_MODE_MIXED if name == "mixed" else abi.CATHODE_MODE_FP64
-        res[name] = run_path(lib, mode, args.steps, args.warmup, args.seed)
+        mode = abi.CATHODE_MODE_FP64 if name == "fp64" else abi.CATHODE_MODE_MIXED
+        if name == "learned" and net is None:
+            net = L.load_net(lib)  # weights trained by learned.py, committed with the package
+        res[name] = run_path(lib, mode, args.steps, args.warmup, args.seed, net if name == "learned" else None)
         print(f"# {name}: {res[name]['steps_per_s']:.2f} steps/s, "
               f"{res[name]['iterations_per_solve']:.2f} BiCGStab it/solve, "
               f"smooth {res[name]['smooth_avg_us']:.2f} us ({res[name]['smooth_GBps']:.0f} GB/s)",
               file=sys.stderr, flush=True)
-    best = "mixed" if "mixed" in res else paths[0]
+    # the metric is the learned + mixed path against plain FP64
+    best = "learned" if "learned" in res else ("mixed" if "mixed" in res else paths[0])
     b = res[best]
     roof = {"bound": "hbm", "achieved": round(b["smooth_GBps"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(b["smooth_GBps"] / HBM_PEAK_GBS, 4), "traffic": traffic_per_launch(best),
+            "frac": round(b["smooth_GBps"] / HBM_PEAK_GBS, 4), "traffic": traffic_per_launch("mixed" if best == "learned" else best),
             "kernel": "k_vtile finest-level down kernel (2 red-black sweeps + residual + restriction), "
-                      + ("FP32" if best == "mixed" else "FP64"),
+                      + ("FP64" if best == "fp64" else "FP32"),
             "bytes_model": "per cell: aP aE aW aN aS b read + x write + 1/4 coarse b write "
                            "(7.25 values) + 1 activity byte",
             "bytes_per_launch": b["smooth_bytes"], "avg_us": round(b["smooth_avg_us"], 3)}
@@ -213,11 +224,12 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64" if best == "mixed" else "f64",
+        "dtype": {"fp64": "f64", "mixed": "f32+f64", "learned": "f32+f64 (bf16 network)"}[best],
         "data": "synthetic",
         "config": {"workload": "implicit NDD step, 512x256 cells of 0.05 mm, ions+Poisson+electrons+energy, "
                                "each solve to relative residual 1e-10", "grid": [512, 256], "seed": args.seed,
-                   "path": best},
+                   "path": {"fp64": "plain FP64", "mixed": "FP32 V-cycles + FP64 refinement",
+                            "learned": "learned initial guess + FP32 V-cycles + FP64 refinement"}[best]},
         "roofline": roof,
         "paths": {k: {kk: (round(vv, 6) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                   for k, v in res.items()},
@@ -232,6 +244,8 @@ def main():
               file=sys.stderr, flush=True)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.seed)
+    if net is not None:
+        lib.cathode_cnn_destroy(net)
     lib.cathode_finalize()
     print(json.dumps(out))
 

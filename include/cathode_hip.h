@@ -22,6 +22,8 @@
 #ifndef CATHODE_HIP_H
 #define CATHODE_HIP_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -58,6 +60,13 @@ int cathode_finalize(void);       /* frees every cached device buffer */
 int cathode_abi_version(void);
 const char* cathode_last_error(void);
 void cathode_last_status(cathode_status* st);
+
+/* Device memory owned by the caller, for the *_dev entry points (bytes are
+ * zeroed at allocation).  kind: 0 host -> device, 1 device -> host,
+ * 2 device -> device; copies are complete when the call returns. */
+int cathode_dev_alloc(size_t bytes, void** ptr);
+int cathode_dev_free(void* ptr);
+int cathode_dev_copy(void* dst, const void* src, size_t bytes, int kind);
 
 /* ------------------------------------------------------------------ */
 /* structured 5-point solves (drop-in for include/solverFuncs.hpp)    */
@@ -175,7 +184,8 @@ typedef struct cathode_ndd_params {
     int maxITR_SOR_phi, maxITR_SOR_rhoe, maxITR_SOR_rhoeps, maxITR_SOR_rhoi;
     double error_cnv_SOR_phi, error_cnv_SOR_rhoe, error_cnv_SOR_rhoeps, error_cnv_SOR_rhoi;
     int mode;                   /* CATHODE_MODE_* for every linear solve */
-    int learned;                /* 1 = learned initial guess for the Poisson correction */
+    int learned;                /* 1 = learned initial guess for the first Poisson
+                                   correction solve (network from cathode_ndd_set_learned) */
     double see_eb[CATHODE_SEE_TABLE_MAX];
     double see_rd[CATHODE_SEE_TABLE_MAX];
     double see_ts[CATHODE_SEE_TABLE_MAX];
@@ -223,6 +233,41 @@ int cathode_ndd_destroy(void* handle);
  * reference runs one bias point per process. */
 int cathode_ndd_batch_step(void** handles, int n, int nsteps, int workers,
                            cathode_ndd_report* reps);
+
+/* ------------------------------------------------------------------ */
+/* learned initial guess (no reference counterpart)                   */
+/* ------------------------------------------------------------------ */
+/* A 6-layer 3x3 convolutional network, 32 hidden channels, BF16 weights and
+ * activations with FP32 accumulation on the matrix cores, that predicts the
+ * solution of the first Poisson correction system of a step from three
+ * feature planes: b / rms(b), r / r_top and the previous step's solution of
+ * that system / its rms.  The step uses alpha * guess as the initial guess,
+ * alpha minimising |b - alpha A guess| (never worse than the zero guess the
+ * reference uses; 0 when the guess does not help), and then solves to the
+ * same tolerance, so converged results are unchanged.
+ * Device arrays are (ni+2)*(nj+2) doubles, pitch nj+2, ghosts 0. */
+int cathode_cnn_num_params(void);
+int cathode_cnn_create(int seed, void** net);       /* He-initialised weights */
+int cathode_cnn_destroy(void* net);
+int cathode_cnn_get_params(void* net, float* host); /* FP32 master weights */
+int cathode_cnn_set_params(void* net, const float* host);
+/* out = network(feat); feat = 3 planes back to back */
+int cathode_cnn_forward_dev(void* net, int ni, int nj, const double* feat, double* out);
+/* loss = mean over interior cells of (out - target)^2 and its gradient
+ * (FP32, parameter layout of cathode_cnn_get_params), no update */
+int cathode_cnn_grad_dev(void* net, int ni, int nj, const double* feat,
+                         const double* target, double* loss, float* grad_host);
+/* one Adam step (lr, beta 0.9 / 0.999, eps 1e-8) on the mean gradient of a
+ * batch: feat = batch x 3 planes, target = batch planes; *loss = mean loss */
+int cathode_cnn_train_step_dev(void* net, int ni, int nj, int batch, const double* feat,
+                               const double* target, float lr, double* loss);
+/* Attach a network to an NDD handle (NULL detaches); used when
+ * params.learned = 1.  The network must outlive its use by the handle. */
+int cathode_ndd_set_learned(void* handle, void* net);
+/* Training capture: each following step writes the features of its first
+ * Poisson correction system (3 planes) and the target (solution / rms) into
+ * these device buffers; NULL, NULL stops. */
+int cathode_ndd_capture(void* handle, double* feat_dev, double* target_dev);
 
 /* Kernel timing of the dominant stencil kernel inside cathode_ndd_step:
  * when enabled, HIP events bracket every launch of the fine-grid

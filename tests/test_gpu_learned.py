@@ -1,0 +1,142 @@
+"""Learned initial guess (csrc/learned.hip): the device network's forward
+pass and gradients against the CPU reference (tests/cnn_ref.py, BF16
+rounding emulated), training by the device kernels lowering the loss, the
+training capture of the NDD step, and the ablation on held-out seeds: with
+the committed weights the NDD step's converged fields are unchanged and the
+BiCGStab iteration count is reported beside the plain run's."""
+import ctypes
+import importlib
+import os
+
+import numpy as np
+import pytest
+
+import cnn_ref as R
+from oracle_lib import abi, rel_l2, workloads as W
+
+pytestmark = pytest.mark.gpu
+L = importlib.import_module("02_cathode-08_modulerev_amd.learned")
+
+
+def _net(lib, seed):
+    net = ctypes.c_void_p()
+    abi.check(lib.cathode_cnn_create(seed, ctypes.byref(net)), lib)
+    return net
+
+
+def _params(lib, net):
+    return L.get_params(lib, net).astype(np.float64)
+
+
+def _case(lib, ni, nj, seed):
+    rng = np.random.default_rng(seed)
+    feat = np.zeros((3, ni + 2, nj + 2))
+    feat[:, 1:-1, 1:-1] = rng.standard_normal((3, ni, nj))
+    tgt = np.zeros((ni + 2, nj + 2))
+    tgt[1:-1, 1:-1] = rng.standard_normal((ni, nj))
+    fd = abi.DeviceArray(lib, feat.shape)
+    fd.upload(feat)
+    td = abi.DeviceArray(lib, tgt.shape)
+    td.upload(tgt)
+    return feat, tgt, fd, td
+
+
+def test_cnn_forward_matches_cpu(hiplib):
+    ni, nj = 40, 27  # partial 16-pixel groups
+    net = _net(hiplib, 3)
+    try:
+        feat, _, fd, _ = _case(hiplib, ni, nj, 1)
+        out = abi.DeviceArray(hiplib, (ni + 2, nj + 2))
+        abi.check(hiplib.cathode_cnn_forward_dev(net, ni, nj, fd.ptr, out.ptr), hiplib)
+        y_ref, _ = R.forward(_params(hiplib, net), feat, ni, nj)
+        y = out.download()
+        err = rel_l2(y[1:-1, 1:-1], y_ref[1:-1, 1:-1])
+        print(f"cnn forward rel L2 {err:.2e}", flush=True)
+        assert err < 5e-3
+        assert np.all(y[0, :] == 0) and np.all(y[:, -1] == 0)
+    finally:
+        hiplib.cathode_cnn_destroy(net)
+
+
+def test_cnn_gradients_match_cpu(hiplib):
+    ni, nj = 36, 21
+    net = _net(hiplib, 4)
+    try:
+        feat, tgt, fd, td = _case(hiplib, ni, nj, 2)
+        n = hiplib.cathode_cnn_num_params()
+        g = np.zeros(n, dtype=np.float32)
+        loss = ctypes.c_double()
+        abi.check(hiplib.cathode_cnn_grad_dev(net, ni, nj, fd.ptr, td.ptr, ctypes.byref(loss),
+                                              g.ctypes.data_as(ctypes.POINTER(ctypes.c_float))), hiplib)
+        l_ref, g_ref = R.loss_and_grad(_params(hiplib, net), feat, tgt, ni, nj)
+        assert abs(loss.value - l_ref) <= 1e-3 * l_ref
+        off, _ = R.layout()
+        for l, ((w, b), cin, cout) in enumerate(zip(off, R.CIN, R.COUT)):
+            ew = rel_l2(g[w:b].astype(np.float64), g_ref[w:b])
+            eb = rel_l2(g[b:b + cout].astype(np.float64), g_ref[b:b + cout])
+            print(f"layer {l}: weight grad rel L2 {ew:.2e}, bias {eb:.2e}", flush=True)
+            assert ew < 3e-2 and eb < 3e-2, (l, ew, eb)
+    finally:
+        hiplib.cathode_cnn_destroy(net)
+
+
+def test_cnn_training_lowers_loss(hiplib):
+    ni, nj = 32, 32
+    rng = np.random.default_rng(5)
+    S = 8
+    feat = np.zeros((S, 3, ni + 2, nj + 2))
+    feat[:, :, 1:-1, 1:-1] = rng.standard_normal((S, 3, ni, nj))
+    tgt = np.zeros((S, ni + 2, nj + 2))
+    # a local linear map of the features: learnable by the network
+    tgt[:, 1:-1, 1:-1] = 0.5 * feat[:, 0, 1:-1, 1:-1] - 0.25 * feat[:, 1, 2:, 1:-1] + 0.25 * feat[:, 2, 1:-1, :-2]
+    fd = abi.DeviceArray(hiplib, feat.shape)
+    fd.upload(feat)
+    td = abi.DeviceArray(hiplib, tgt.shape)
+    td.upload(tgt)
+    net = _net(hiplib, 6)
+    try:
+        losses = L.train(hiplib, net, fd, td, epochs=1, batch=S, lr=3e-3)
+        first = None
+        for it in range(60):
+            loss = ctypes.c_double()
+            abi.check(hiplib.cathode_cnn_train_step_dev(net, ni, nj, S, fd.ptr, td.ptr, ctypes.c_float(3e-3),
+                                                        ctypes.byref(loss)), hiplib)
+            first = first if first is not None else loss.value
+        print(f"training: loss {first:.4f} -> {loss.value:.4f}", flush=True)
+        assert loss.value < 0.5 * first and np.isfinite(losses[0])
+    finally:
+        hiplib.cathode_cnn_destroy(net)
+
+
+def test_ndd_capture_features(hiplib):
+    ni, nj = 64, 32
+    feat, target = L.generate_pairs(hiplib, [11], 2, ni=ni, nj=nj, h=2e-4)
+    f = feat.download()
+    t = target.download()
+    feat.free()
+    target.free()
+    r = W.radii(nj, 2e-4)
+    for k in range(2):
+        np.testing.assert_allclose(f[k, 1, 1:-1, 1:-1], np.broadcast_to(r[1:-1] / (nj * 2e-4), (ni, nj)), rtol=1e-15)
+        assert abs(np.sqrt((f[k, 0, 1:-1, 1:-1] ** 2).mean()) - 1) < 1e-12
+        assert abs(np.sqrt((t[k, 1:-1, 1:-1] ** 2).mean()) - 1) < 1e-12
+    assert not np.any(f[0, 2])                      # no previous solution at the first step
+    np.testing.assert_allclose(f[1, 2], t[0], rtol=1e-12, atol=1e-300)  # the next step sees it
+
+
+@pytest.mark.skipif(not os.path.exists(L.WEIGHTS), reason="trained weights not committed")
+def test_learned_guess_ablation(hiplib):
+    net = L.load_net(hiplib)
+    try:
+        ev = L.evaluate(hiplib, net, [901, 902], 3)
+    finally:
+        hiplib.cathode_cnn_destroy(net)
+    print(f"ablation: {ev}", flush=True)
+    # converged answers unchanged: both runs solve every system to 1e-10
+    assert ev["max_field_rel_l2"] <= 1e-9
+    # the guess never starts from a larger residual than x = 0 (optimal
+    # scaling), but BiCGStab's count is not monotone in the starting
+    # residual: allow 2% either way and report the gain
+    gain = 1.0 - ev["learned"]["iterations"] / ev["plain"]["iterations"]
+    print(f"iterations plain {ev['plain']['iterations']}, learned {ev['learned']['iterations']}: gain {gain:+.2%}")
+    assert gain >= -0.02
