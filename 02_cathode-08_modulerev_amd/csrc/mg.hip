@@ -621,8 +621,9 @@ void MGSolver::configure(int ni, int nj, hipStream_t s)
     w0.alloc(nj + 2);
     // levels from lt on run inside the one-workgroup tail kernel
     lt = nl - 1;
-    while (lt > 0 && (long)dims[lt - 1][0] * dims[lt - 1][1] <= TAIL_MAX_CELLS) lt--;
-    if (nl - lt > TAIL_REG_LEVELS) lt = nl - TAIL_REG_LEVELS;
+    // the tail's first level may hold TAIL_TOP_MAX cells, every further one TAIL_NT
+    auto cells = [&](int l) { return (long)dims[l][0] * dims[l][1]; };
+    while (lt > 0 && cells(lt - 1) <= TAIL_TOP_MAX && cells(lt) <= TAIL_NT && nl - (lt - 1) <= TAIL_REG_LEVELS) lt--;
     tail_cells = 0;
     for (int l = lt; l < nl; l++) tail_cells += (dims[l][0] + 2) * (dims[l][1] + 2);
     part.alloc(3 * (cdiv(nj, BX) * cdiv(ni, BY)) + 64);
@@ -734,10 +735,11 @@ void MGSolver::vcycle(Level<T>* L, hipStream_t s)
     ta.b_in = L[lt].b;
     ta.x_out = L[lt].x;
     ta.w = lt == 0 ? w : nullptr;
-    const size_t lds = (size_t)off * (7 * sizeof(T) + 1) + 16;
-    (void)lds;
-    hipLaunchKernelGGL(k_tail_reg<T>, dim3(1), dim3(TAIL_NT), (size_t)off * 2 * sizeof(T) + TAIL_NT * sizeof(T) + 16,
-                       s, ta);
+    const int top = L[lt].ni * L[lt].nj;
+    const size_t lds = ((size_t)off * 2 + std::max(top, TAIL_NT)) * sizeof(T) + 16;
+    if (top <= TAIL_NT) launch_tail<T, 1>(ta, lds, s);
+    else if (top <= 2 * TAIL_NT) launch_tail<T, 2>(ta, lds, s);
+    else launch_tail<T, 4>(ta, lds, s);
     // fine levels: fused up kernels
     for (int l = lt - 1; l >= 0; l--) {
         Level<T>& A = L[l];
@@ -762,6 +764,20 @@ double MGSolver::read_scalar(const double* d, hipStream_t s)
     CATHODE_HIP(hipMemcpyAsync(h_scal, d, sizeof(double), hipMemcpyDeviceToHost, s));
     CATHODE_HIP(hipStreamSynchronize(s));
     return h_scal[0];
+}
+
+// the tail kernel with C0 first-level cells per thread; LDS beyond the
+// default 64 KiB is enabled once per instantiation
+template <class T, int C0>
+static void launch_tail(const TailArgs<T>& ta, size_t lds, hipStream_t s)
+{
+    static bool big_lds = false;
+    if (lds > 65536 && !big_lds) {
+        CATHODE_HIP(hipFuncSetAttribute((const void*)k_tail_reg<T, C0>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        160 * 1024));
+        big_lds = true;
+    }
+    hipLaunchKernelGGL((k_tail_reg<T, C0>), dim3(1), dim3(TAIL_NT), lds, s, ta);
 }
 
 template <class T>

@@ -337,12 +337,16 @@ __global__ void __launch_bounds__(TAIL_NT) k_tail(TailArgs<T> A)
 }
 
 // ------------------------------------------------------------------ register-resident tail (real types)
-// Same levels, same sweeps and transfers as k_tail, organised for latency:
-// thread q owns cell q of every tail level and keeps that cell's
-// coefficients (and 1/aP) in registers, so LDS holds only x, b and the
-// residuals; phases whose cells all lie in the first wavefront (<= 64 cells)
+// Same levels, same sweeps and transfers as k_tail, organised for latency.
+// The first tail level may hold up to TAIL_TOP_MAX cells: thread q owns its
+// C0 consecutive cells q*C0 .. q*C0+C0-1 (consecutive cells alternate in
+// colour, so every half sweep keeps every thread busy); every further level
+// has at most TAIL_NT cells, one per thread.  Each owned cell's coefficients
+// and 1/aP sit in registers, LDS holds only x, b and the residuals, and
+// phases whose cells all lie in the first wavefront (<= 64 cells)
 // synchronise that wavefront alone instead of the whole workgroup.
-constexpr int TAIL_REG_LEVELS = 10;  // tail levels of <= 512 cells: at most 10
+constexpr int TAIL_REG_LEVELS = 10;   // tail levels below the first: at most 9 (<= 512 cells)
+constexpr int TAIL_TOP_MAX = 4 * TAIL_NT;
 
 __device__ inline void tail_sync(int cells)
 {
@@ -356,13 +360,118 @@ __device__ inline void tail_sync(int cells)
     }
 }
 
+// the cells of one tail level owned by one thread
+template <class T, int C>
+struct TailCells {
+    T P[C], iP[C], E[C], W[C], N[C], S[C], wq[C];
+    int c[C];     // LDS index, -1 = none
+    int idx[C];   // cell number in the level (row-major over the interior)
+    int par[C];   // LDS index of the parent in the next level
+    unsigned fl[C];  // bit0 active, bit1 colour
+};
+
+template <class T, int C>
+__device__ inline void tail_load(TailCells<T, C>& L, const TailArgs<T>& A, int l, int q, T* B)
+{
+    const int ni = A.ni[l], nj = A.nj[l], ld = nj + 2;
+#pragma unroll
+    for (int k = 0; k < C; k++) {
+        L.c[k] = -1;
+        L.idx[k] = 0;
+        L.par[k] = 0;
+        L.fl[k] = 0;
+        L.P[k] = L.iP[k] = L.wq[k] = T(1);
+        L.E[k] = L.W[k] = L.N[k] = L.S[k] = T(0);
+        const int ci = q * C + k;
+        if (ci >= ni * nj) continue;
+        const int i = 1 + ci / nj, j = 1 + ci % nj;
+        L.c[k] = A.off[l] + i * ld + j;
+        L.idx[k] = ci;
+        const size_t g = (size_t)i * A.gld[l] + j;
+        L.fl[k] = ((i + j) & 1) ? 2u : 0u;
+        if (A.act[l][g]) {
+            L.fl[k] |= 1u;
+            L.P[k] = A.aP[l][g];
+            L.iP[k] = T(1) / L.P[k];
+            L.E[k] = A.aE[l][g];
+            L.W[k] = A.aW[l][g];
+            L.N[k] = A.aN[l][g];
+            L.S[k] = A.aS[l][g];
+            if (l == 0) B[L.c[k]] = A.b_in[g];
+        }
+        if (l + 1 < A.nt) {
+            const int I = A.cx[l] ? (i + 1) >> 1 : i, J = A.cy[l] ? (j + 1) >> 1 : j;
+            L.par[k] = A.off[l + 1] + I * (A.nj[l + 1] + 2) + J;
+        }
+        if (l == 0 && A.w) L.wq[k] = (T)A.w[j];
+    }
+}
+
+// 4 half sweeps (red first); the last one is not followed by a barrier
+template <class T, int C>
+__device__ inline void tail_sweeps4(const TailCells<T, C>& L, T* X, const T* B, int ld, int cells)
+{
+    for (int s = 0; s < 4; s++) {
+#pragma unroll
+        for (int k = 0; k < C; k++) {
+            const int c = L.c[k];
+            if (c >= 0 && (L.fl[k] & 3u) == (1u | ((unsigned)(s & 1) << 1)))
+                X[c] = (B[c] + L.E[k] * X[c + ld] + L.W[k] * X[c - ld] + L.N[k] * X[c + 1] + L.S[k] * X[c - 1]) *
+                       L.iP[k];
+        }
+        if (s < 3) tail_sync(cells);
+    }
+}
+
+template <class T, int C>
+__device__ inline void tail_residual(const TailCells<T, C>& L, const T* X, const T* B, T* R, int ld, bool weighted)
+{
+#pragma unroll
+    for (int k = 0; k < C; k++) {
+        const int c = L.c[k];
+        if (c < 0) continue;
+        T r = T(0);
+        if (L.fl[k] & 1u) {
+            r = B[c] - L.P[k] * X[c] + L.E[k] * X[c + ld] + L.W[k] * X[c - ld] + L.N[k] * X[c + 1] +
+                L.S[k] * X[c - 1];
+            if (weighted) r = r * L.wq[k];
+        }
+        R[L.idx[k]] = r;
+    }
+}
+
+// coarse cell of level l+1 owned by this thread sums its children's residuals
 template <class T>
+__device__ inline void tail_restrict(const TailCells<T, 1>& Lc, const TailArgs<T>& A, int l, const T* R, T* B)
+{
+    if (Lc.c[0] < 0) return;
+    const int nj = A.nj[l], fi = A.cx[l] ? 2 : 1, fj = A.cy[l] ? 2 : 1;
+    const int cnj = A.nj[l + 1];
+    const int I = 1 + Lc.idx[0] / cnj, J = 1 + Lc.idx[0] % cnj;
+    T sum = T(0);
+    for (int a = 0; a < fi; a++)
+        for (int b = 0; b < fj; b++) {
+            const int i = fi * (I - 1) + 1 + a, j = fj * (J - 1) + 1 + b;
+            if (i <= A.ni[l] && j <= nj) sum += R[(i - 1) * nj + (j - 1)];
+        }
+    B[Lc.c[0]] = sum;
+}
+
+template <class T, int C>
+__device__ inline void tail_prolong(const TailCells<T, C>& L, T* X)
+{
+#pragma unroll
+    for (int k = 0; k < C; k++)
+        if (L.c[k] >= 0 && (L.fl[k] & 1u)) X[L.c[k]] = X[L.c[k]] + X[L.par[k]];
+}
+
+template <class T, int C0>
 __global__ void __launch_bounds__(TAIL_NT) k_tail_reg(TailArgs<T> A)
 {
     extern __shared__ unsigned char tail_lds[];
     T* X = reinterpret_cast<T*>(tail_lds);
     T* B = X + A.ncell;
-    T* R = B + A.ncell;  // residuals of one level, indexed by owner thread
+    T* R = B + A.ncell;  // residuals of one level, by cell number (>= max(cells0, TAIL_NT) entries)
     const int q = threadIdx.x;
     const int nt = A.nt;
     for (int c = q; c < A.ncell; c += TAIL_NT) {
@@ -370,106 +479,76 @@ __global__ void __launch_bounds__(TAIL_NT) k_tail_reg(TailArgs<T> A)
         B[c] = T(0);
     }
     __syncthreads();
-    T cP[TAIL_REG_LEVELS], iP[TAIL_REG_LEVELS], cE[TAIL_REG_LEVELS], cW[TAIL_REG_LEVELS], cN[TAIL_REG_LEVELS],
-        cS[TAIL_REG_LEVELS];
-    int cc[TAIL_REG_LEVELS], par[TAIL_REG_LEVELS], ncl[TAIL_REG_LEVELS];
-    unsigned fl[TAIL_REG_LEVELS];  // bit0 active, bit1 colour
-    T wq = T(1);                   // restriction weight of this thread's first-level cell
+    TailCells<T, C0> L0;
+    TailCells<T, 1> Lk[TAIL_REG_LEVELS - 1];  // tail levels 1 ..
+    tail_load(L0, A, 0, q, B);
+    int ncl[TAIL_REG_LEVELS];
+    ncl[0] = A.ni[0] * A.nj[0];
 #pragma unroll
-    for (int l = 0; l < TAIL_REG_LEVELS; l++) {
-        cc[l] = -1;
-        par[l] = 0;
-        fl[l] = 0;
-        ncl[l] = 0;
-        cP[l] = iP[l] = T(1);
-        cE[l] = cW[l] = cN[l] = cS[l] = T(0);
-        if (l < nt) {
-            const int ni = A.ni[l], nj = A.nj[l], ld = nj + 2;
-            ncl[l] = ni * nj;
-            if (q < ni * nj) {
-                const int i = 1 + q / nj, j = 1 + q % nj;
-                cc[l] = A.off[l] + i * ld + j;
-                const size_t g = (size_t)i * A.gld[l] + j;
-                fl[l] = ((i + j) & 1) ? 2u : 0u;
-                if (A.act[l][g]) {
-                    fl[l] |= 1u;
-                    cP[l] = A.aP[l][g];
-                    iP[l] = T(1) / cP[l];
-                    cE[l] = A.aE[l][g];
-                    cW[l] = A.aW[l][g];
-                    cN[l] = A.aN[l][g];
-                    cS[l] = A.aS[l][g];
-                    if (l == 0) B[cc[l]] = A.b_in[g];
-                }
-                if (l + 1 < nt) {
-                    const int I = A.cx[l] ? (i + 1) >> 1 : i, J = A.cy[l] ? (j + 1) >> 1 : j;
-                    par[l] = A.off[l + 1] + I * (A.nj[l + 1] + 2) + J;
-                }
-                if (l == 0 && A.w) wq = (T)A.w[j];
-            }
-        }
+    for (int l = 1; l < TAIL_REG_LEVELS; l++) {
+        ncl[l] = l < nt ? A.ni[l] * A.nj[l] : 0;
+        if (l < nt) tail_load(Lk[l - 1], A, l, q, B);
+        else Lk[l - 1].c[0] = -1;
     }
     __syncthreads();
-    // down: sweeps, residual, restriction
+    // ---- down
+    if (nt > 1) {
+        const int ld = A.nj[0] + 2;
+        tail_sweeps4(L0, X, B, ld, ncl[0]);
+        tail_sync(ncl[0]);
+        tail_residual(L0, X, B, R, ld, A.w != nullptr);
+        tail_sync(ncl[0]);
+        tail_restrict(Lk[0], A, 0, R, B);
+        tail_sync(ncl[0]);
+    }
 #pragma unroll
-    for (int l = 0; l < TAIL_REG_LEVELS - 1; l++) {
+    for (int l = 1; l < TAIL_REG_LEVELS - 1; l++) {
         if (l + 1 < nt) {
-            const int ld = A.nj[l] + 2, c = cc[l];
-            for (int s = 0; s < 4; s++) {
-                if (c >= 0 && (fl[l] & 3u) == (1u | ((unsigned)(s & 1) << 1)))
-                    X[c] = (B[c] + cE[l] * X[c + ld] + cW[l] * X[c - ld] + cN[l] * X[c + 1] + cS[l] * X[c - 1]) * iP[l];
-                tail_sync(ncl[l]);
-            }
-            if (c >= 0) {
-                T r = T(0);
-                if (fl[l] & 1u) {
-                    r = B[c] - cP[l] * X[c] + cE[l] * X[c + ld] + cW[l] * X[c - ld] + cN[l] * X[c + 1] +
-                        cS[l] * X[c - 1];
-                    if (l == 0) r = r * wq;
-                }
-                R[q] = r;
-            }
+            const int ld = A.nj[l] + 2;
+            tail_sweeps4(Lk[l - 1], X, B, ld, ncl[l]);
             tail_sync(ncl[l]);
-            // coarse cell q of level l+1 sums its children's residuals
-            if (cc[l + 1] >= 0) {
-                const int nj = A.nj[l], fi = A.cx[l] ? 2 : 1, fj = A.cy[l] ? 2 : 1;
-                const int cnj = A.nj[l + 1];
-                const int I = 1 + q / cnj, J = 1 + q % cnj;
-                T sum = T(0);
-                for (int a = 0; a < fi; a++)
-                    for (int b = 0; b < fj; b++) {
-                        const int i = fi * (I - 1) + 1 + a, j = fj * (J - 1) + 1 + b;
-                        if (i <= A.ni[l] && j <= nj) sum += R[(i - 1) * nj + (j - 1)];
-                    }
-                B[cc[l + 1]] = sum;
-            }
+            tail_residual(Lk[l - 1], X, B, R, ld, false);
+            tail_sync(ncl[l]);
+            tail_restrict(Lk[l], A, l, R, B);
             tail_sync(ncl[l]);
         }
     }
-    // coarsest level: x = b / aP (inactive or zero diagonal: 0)
+    // ---- coarsest level: x = b / aP (inactive or zero diagonal: 0)
+    if (nt == 1) {
 #pragma unroll
-    for (int l = 0; l < TAIL_REG_LEVELS; l++)
-        if (l == nt - 1 && cc[l] >= 0) X[cc[l]] = ((fl[l] & 1u) && !(cP[l] == T(0))) ? B[cc[l]] / cP[l] : T(0);
-    // up: prolongation, sweeps
+        for (int k = 0; k < C0; k++)
+            if (L0.c[k] >= 0)
+                X[L0.c[k]] = ((L0.fl[k] & 1u) && !(L0.P[k] == T(0))) ? B[L0.c[k]] / L0.P[k] : T(0);
+    }
 #pragma unroll
-    for (int l = TAIL_REG_LEVELS - 2; l >= 0; l--) {
+    for (int l = 1; l < TAIL_REG_LEVELS; l++) {
+        const TailCells<T, 1>& L = Lk[l - 1];
+        if (l == nt - 1 && L.c[0] >= 0)
+            X[L.c[0]] = ((L.fl[0] & 1u) && !(L.P[0] == T(0))) ? B[L.c[0]] / L.P[0] : T(0);
+    }
+    // ---- up
+#pragma unroll
+    for (int l = TAIL_REG_LEVELS - 2; l >= 1; l--) {
         if (l + 1 < nt) {
-            const int ld = A.nj[l] + 2, c = cc[l];
             tail_sync(ncl[l]);
-            if (c >= 0 && (fl[l] & 1u)) X[c] = X[c] + X[par[l]];
+            tail_prolong(Lk[l - 1], X);
             tail_sync(ncl[l]);
-            for (int s = 0; s < 4; s++) {
-                if (c >= 0 && (fl[l] & 3u) == (1u | ((unsigned)(s & 1) << 1)))
-                    X[c] = (B[c] + cE[l] * X[c + ld] + cW[l] * X[c - ld] + cN[l] * X[c + 1] + cS[l] * X[c - 1]) * iP[l];
-                if (s < 3) tail_sync(ncl[l]);
-            }
+            tail_sweeps4(Lk[l - 1], X, B, A.nj[l] + 2, ncl[l]);
         }
     }
-    // first tail level out (each thread its own cell)
-    if (cc[0] >= 0) {
+    if (nt > 1) {
+        tail_sync(ncl[0]);
+        tail_prolong(L0, X);
+        tail_sync(ncl[0]);
+        tail_sweeps4(L0, X, B, A.nj[0] + 2, ncl[0]);
+    }
+    // ---- first tail level out (each thread its own cells)
+#pragma unroll
+    for (int k = 0; k < C0; k++) {
+        if (L0.c[k] < 0) continue;
         const int nj = A.nj[0];
-        const int i = 1 + q / nj, j = 1 + q % nj;
-        A.x_out[(size_t)i * A.gld[0] + j] = X[cc[0]];
+        const int i = 1 + L0.idx[k] / nj, j = 1 + L0.idx[k] % nj;
+        A.x_out[(size_t)i * A.gld[0] + j] = X[L0.c[k]];
     }
 }
 
