@@ -626,9 +626,13 @@ void MGSolver::configure(int ni, int nj, hipStream_t s)
     while (lt > 0 && cells(lt - 1) <= TAIL_TOP_MAX && cells(lt) <= TAIL_NT && nl - (lt - 1) <= TAIL_REG_LEVELS) lt--;
     tail_cells = 0;
     for (int l = lt; l < nl; l++) tail_cells += (dims[l][0] + 2) * (dims[l][1] + 2);
-    part.alloc(3 * (cdiv(nj, BX) * cdiv(ni, BY)) + 64);
+    {
+        int tiles_max = 0;
+        for (int cfg = 0; cfg < VT_NCFG; cfg++) tiles_max = std::max(tiles_max, vtile_count(cfg, ni, nj));
+        part.alloc(std::max(3 * (cdiv(nj, BX) * cdiv(ni, BY)), 2 * tiles_max) + 64);
+    }
     scal.alloc(16);
-    kv.alloc(8 * n0);
+    kv.alloc(9 * n0);
     CATHODE_HIP(hipMemset(kv.p, 0, 8 * n0 * sizeof(double)));
     if (!h_scal) CATHODE_HIP(hipHostMalloc(&h_scal, 16 * sizeof(double), hipHostMallocDefault));
 }
@@ -698,7 +702,7 @@ void MGSolver::setup(int mode, hipStream_t s)
 }
 
 template <class T>
-void MGSolver::vcycle(Level<T>* L, hipStream_t s)
+void MGSolver::vcycle(Level<T>* L, hipStream_t s, const VFuse& fdown, const VFuse& fup)
 {
     const double* w = user_w ? user_w : w0.p;
     // fine levels: fused down kernels
@@ -708,8 +712,8 @@ void MGSolver::vcycle(Level<T>* L, hipStream_t s)
         VLevel<T> V{A.ni, A.nj, A.ld, A.aP, A.aE, A.aW, A.aN, A.aS, actall.p + level_off[l], A.b, A.x};
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (prof.enabled && l == 0) prof_events(e0, e1);
-        launch_vtile<T, true>(tile_of<T>(l), l == 0, s, V, A.cx ? 2 : 1, A.cy ? 2 : 1, w, C.ni, C.nj, C.ld, C.b, e0,
-                              e1);
+        launch_vtile<T, true>(tile_of<T>(l), l == 0, s, V, A.cx ? 2 : 1, A.cy ? 2 : 1, w, C.ni, C.nj, C.ld, C.b,
+                              l == 0 ? fdown : VFuse(), e0, e1);
     }
     // coarse levels in one workgroup
     TailArgs<T> ta{};
@@ -746,7 +750,7 @@ void MGSolver::vcycle(Level<T>* L, hipStream_t s)
         Level<T>& C = L[l + 1];
         VLevel<T> V{A.ni, A.nj, A.ld, A.aP, A.aE, A.aW, A.aN, A.aS, actall.p + level_off[l], A.b, A.x};
         launch_vtile<T, false>(tile_of<T>(l), l == 0, s, V, A.cx ? 2 : 1, A.cy ? 2 : 1, (const double*)nullptr, C.ni,
-                               C.nj, C.ld, C.x);
+                               C.nj, C.ld, C.x, l == 0 ? fup : VFuse());
     }
     CATHODE_LAUNCH_CHECK();
 }
@@ -897,6 +901,29 @@ int MGSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_sta
     double* t = sv + n0;
     double* ph = t + n0;
     double* sh = ph + n0;
+    double* pa = p;         // current search direction
+    double* pb = sh + n0;   // the next one (fused path: never the buffer being read)
+    // BiCGStab work fused into the finest V-cycle kernels: needs a finest
+    // level handled by the tile kernels and the 5-point operator
+    const bool fused = !use_nine && lt > 0;
+    const int ntiles = vtile_count(mixed ? tiles[1][0] : tiles[0][0], F.ni, F.nj);
+    VFuse fbase;
+    fbase.mixed = mixed;
+    fbase.s_s32 = SC_S32;
+    fbase.s_scale = SC_SCALE;
+    fbase.s_rho1 = SC_RHO1;
+    fbase.s_rho = SC_RHO;
+    fbase.s_al = SC_AL;
+    fbase.s_om = SC_OM;
+    fbase.r = r;
+    fbase.v = v;
+    fbase.sc = scal.p;
+    fbase.aP = F.aP;
+    fbase.aE = F.aE;
+    fbase.aW = F.aW;
+    fbase.aN = F.aN;
+    fbase.aS = F.aS;
+    fbase.part = part.p;
     aborted = false;
     const double* n9 = use_nine ? nine.p : nullptr;
     double* xa = xsol.p;   // current iterate
@@ -907,19 +934,50 @@ int MGSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_sta
     while (it < maxITR) {
         if (restart) {
             hipLaunchKernelGGL(k_bicg_restart, g, blk, 0, s, F.ni, F.nj, F.ld, F.aP, F.aE, F.aW, F.aN, F.aS, n9,
-                               n0, bsol.p, xa, act0.p, r, r0, p, v, part.p, nb);
+                               n0, bsol.p, xa, act0.p, r, r0, pa, v, part.p, nb);
             hipLaunchKernelGGL(k_bicg_finish, dim3(1), dim3(256), 0, s, part.p, nb, scal.p, 1, 0.0);
         }
         it++;
+        if (fused) {
+            // p and the V-cycle input in the finest down kernel; ph, v = A ph
+            // and (r0, v) in the finest up kernel; al
+            VFuse fd = fbase, fu = fbase;
+            fd.pro = 1;
+            fd.first = restart ? 1 : 0;
+            fd.pin = pa;
+            fd.ps = pb;
+            fu.epi = 1;
+            fu.y = ph;
+            fu.out = v;
+            fu.dotvec = r0;
+            fu.want_tt = 0;
+            if (mixed) vcycle<float>(L32, s, fd, fu);
+            else vcycle<double>(L64, s, fd, fu);
+            hipLaunchKernelGGL(k_bicg_finish, dim3(1), dim3(256), 0, s, part.p, ntiles, scal.p, 2, 0.0);
+            std::swap(pa, pb);
+            restart = false;
+            // s and the V-cycle input; sh, t = A sh, (t, s) and (t, t); om
+            fd.pro = 2;
+            fd.first = 0;
+            fd.pin = nullptr;
+            fd.ps = sv;
+            fu.y = sh;
+            fu.out = t;
+            fu.dotvec = sv;
+            fu.want_tt = 1;
+            if (mixed) vcycle<float>(L32, s, fd, fu);
+            else vcycle<double>(L64, s, fd, fu);
+            hipLaunchKernelGGL(k_bicg_finish, dim3(1), dim3(256), 0, s, part.p, ntiles, scal.p, 3, 0.0);
+        } else {
         // p, ph = M^-1 p, v = A ph, al
         if (mixed) {
-            hipLaunchKernelGGL(k_bicg_p<float>, g, blk, 0, s, F.ni, F.nj, F.ld, act0.p, r, v, p, L32[0].b,
+            hipLaunchKernelGGL(k_bicg_p<float>, g, blk, 0, s, F.ni, F.nj, F.ld, act0.p, r, v, pa, L32[0].b,
                                L32[0].x, scal.p, restart ? 1 : 0, 1);
             vcycle<float>(L32, s);
             hipLaunchKernelGGL(k_bicg_apply<float>, g, blk, 0, s, F.ni, F.nj, F.ld, F.aP, F.aE, F.aW, F.aN,
                                F.aS, n9, n0, act0.p, L32[0].x, ph, v, r0, scal.p, 1, 0, part.p, nb);
         } else {
-            hipLaunchKernelGGL(k_bicg_p<double>, g, blk, 0, s, F.ni, F.nj, F.ld, act0.p, r, v, p, F.b, F.x,
+            hipLaunchKernelGGL(k_bicg_p<double>, g, blk, 0, s, F.ni, F.nj, F.ld, act0.p, r, v, pa, F.b, F.x,
                                scal.p, restart ? 1 : 0, 0);
             vcycle<double>(L64, s);
             hipLaunchKernelGGL(k_bicg_apply<double>, g, blk, 0, s, F.ni, F.nj, F.ld, F.aP, F.aE, F.aW, F.aN,
@@ -942,6 +1000,7 @@ int MGSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_sta
                                F.aS, n9, n0, act0.p, F.x, sh, t, sv, scal.p, 0, 1, part.p, nb);
         }
         hipLaunchKernelGGL(k_bicg_finish, dim3(1), dim3(256), 0, s, part.p, nb, scal.p, 3, 0.0);
+        }
         // x, recursive and true residuals
         hipLaunchKernelGGL(k_bicg_x, g, blk, 0, s, F.ni, F.nj, F.ld, F.aP, F.aE, F.aW, F.aN, F.aS, n9, n0, bsol.p,
                            act0.p,
@@ -992,9 +1051,12 @@ int MGSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_sta
         // kernel reads 5 coefficients + b and writes x and 1/4 coarse b (7.25
         // values), the up kernel also reads x and 1/4 coarse x (8.25 values)
         const double vb = mixed ? 4.0 : 8.0;
-        // finest-level down kernel: reads aP aE aW aN aS b (6 values) and the
-        // activity byte, writes x and a quarter value of coarse b per cell
-        prof.bytes_per_launch = (7.25 * vb + 1.0) * (double)F.ni * (double)F.nj;
+        // finest-level down kernel, per cell: aP aE aW aN aS and the activity
+        // byte read, x and a quarter value of coarse b written; unfused, b
+        // is read; fused, r and v (and the old p every other launch) are read
+        // in FP64, the new p or s (FP64) and the level's b are written
+        const double per_cell = fused ? 7.25 * vb + 1.0 + 8.0 * (2.5 + 1.0) : 7.25 * vb + 1.0;
+        prof.bytes_per_launch = per_cell * (double)F.ni * (double)F.nj;
     }
     auto t1 = std::chrono::steady_clock::now();
     if (st) {

@@ -26,6 +26,7 @@
 #pragma once
 
 #include "common.hpp"
+#include "vfuse.hpp"
 
 namespace cathode {
 
@@ -116,7 +117,7 @@ private:
     void setup(int mode, hipStream_t s);
     struct CoarsenArgs coarsen_args(int l, const double* w);
     template <class T>
-    void vcycle(Level<T>* L, hipStream_t s);
+    void vcycle(Level<T>* L, hipStream_t s, const VFuse& fdown = VFuse(), const VFuse& fup = VFuse());
     template <class T>
     int tile_of(int l) const { return tiles[sizeof(T) == 4 ? 1 : 0][l]; }
     template <class T>
@@ -144,7 +145,7 @@ private:
     int lt = 0;                // first level handled by the one-workgroup tail kernel
     int tail_cells = 0;        // LDS cells of the tail levels
     DevBuf<double> part;       // reduction partials (2 slots)
-    DevBuf<double> kv;         // BiCGStab vectors r, r0, p, v, s, t, ph, sh
+    DevBuf<double> kv;         // BiCGStab vectors r, r0, p, v, s, t, ph, sh, p (second buffer)
     DevBuf<double> scal;       // device scalars: [0] |b|^2, [1] |r|^2, [2] scale, [3] f32 matrix scale
     double* h_scal = nullptr;  // pinned mirror
 };

@@ -15,6 +15,10 @@
 // LDS (k_tail), down to a single coarsest cell solved by one division.
 #pragma once
 
+#include <type_traits>
+
+#include "vfuse.hpp"
+
 namespace cathode {
 
 // Tile shapes of the fused fine-level kernels: TI x TJ cells per workgroup of
@@ -60,8 +64,9 @@ struct VLevel {
 template <class T, int TI, int TJ, int NT, int H, bool DOWN, bool TOP>
 __global__ void __launch_bounds__(NT)
     k_vtile(VLevel<T> F, int fi, int fj, const double* __restrict__ w, int cni, int cnj, int cld,
-            T* __restrict__ cbx)
+            T* __restrict__ cbx, VFuse fu)
 {
+    constexpr bool REAL = std::is_floating_point<T>::value;
     constexpr int RI = TI + 2 * H, RJ = TJ + 2 * H, R = RI * RJ;
     constexpr int K = (R + NT - 1) / NT;
     __shared__ T xs[R];
@@ -85,6 +90,7 @@ __global__ void __launch_bounds__(NT)
         T xv = T(0);
         if (gi >= 1 && gi <= F.ni && gj >= 1 && gj <= F.nj) {
             const size_t g = (size_t)gi * F.ld + gj;
+            const bool owner = ri >= H && ri < H + TI && rj >= H && rj < H + TJ;
             if (F.act[g]) {
                 flags[k] |= 1u;
                 cP[k] = F.aP[g];
@@ -92,7 +98,30 @@ __global__ void __launch_bounds__(NT)
                 cW[k] = F.aW[g];
                 cN[k] = F.aN[g];
                 cS[k] = F.aS[g];
-                cB[k] = F.b[g];
+                if constexpr (REAL && DOWN && TOP) {
+                    if (fu.pro) {
+                        double q;
+                        if (fu.pro == 2) {
+                            q = fu.r[g] - fu.sc[fu.s_al] * fu.v[g];
+                        } else if (fu.first) {
+                            q = fu.r[g];
+                        } else {
+                            const double beta =
+                                (fu.sc[fu.s_rho1] / fu.sc[fu.s_rho]) * (fu.sc[fu.s_al] / fu.sc[fu.s_om]);
+                            q = fu.r[g] + beta * (fu.pin[g] - fu.sc[fu.s_om] * fu.v[g]);
+                        }
+                        const double f = fu.mixed ? fu.sc[fu.s_s32] * fu.sc[fu.s_scale] : 1.0;
+                        cB[k] = (T)(q * f);
+                        if (owner) {  // the old p is read from pin, so no tile reads what another writes
+                            fu.ps[g] = q;
+                            F.b[g] = cB[k];
+                        }
+                    } else {
+                        cB[k] = F.b[g];
+                    }
+                } else {
+                    cB[k] = F.b[g];
+                }
                 if (!DOWN) {
                     const int I = fi == 2 ? (gi + 1) >> 1 : gi, J = fj == 2 ? (gj + 1) >> 1 : gj;
                     xv = F.x[g] + cbx[(size_t)I * cld + J];
@@ -154,47 +183,93 @@ __global__ void __launch_bounds__(NT)
             cbx[(size_t)I * cld + J] = sum;
         }
     }
+    if constexpr (REAL && !DOWN && TOP) {
+        if (fu.epi) {
+            // y = V-cycle output in FP64, out = A y (FP64 finest operator),
+            // partial dot products of this tile
+            __shared__ double red[NT / 64 > 8 ? NT / 64 : 8];
+            const double oscale = fu.mixed ? 1.0 / fu.sc[fu.s_scale] : 1.0;
+            double d1 = 0.0, d2 = 0.0;
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const int c = tid + k * NT;
+                if (!(flags[k] & 8u)) continue;
+                const int ri = c / RJ, rj = c - ri * RJ;
+                const size_t g = (size_t)(gi0 + ri) * F.ld + (gj0 + rj);
+                double yk = 0.0, o = 0.0;
+                if (flags[k] & 1u) {
+                    yk = (double)xs[c] * oscale;
+                    o = fu.aP[g] * yk - fu.aE[g] * ((double)xs[c + RJ] * oscale) -
+                        fu.aW[g] * ((double)xs[c - RJ] * oscale) - fu.aN[g] * ((double)xs[c + 1] * oscale) -
+                        fu.aS[g] * ((double)xs[c - 1] * oscale);
+                    d1 = d1 + fu.dotvec[g] * o;
+                    d2 = d2 + o * o;
+                }
+                fu.y[g] = yk;
+                fu.out[g] = o;
+            }
+            const double t1 = block_sum<NT>(d1, red);
+            const double t2 = fu.want_tt ? block_sum<NT>(d2, red) : 0.0;
+            if (tid == 0) {
+                const int ntiles = (int)(gridDim.x * gridDim.y);
+                const int tile = bi * (int)gridDim.x + bj;
+                fu.part[tile] = t1;
+                if (fu.want_tt) fu.part[ntiles + tile] = t2;
+            }
+        }
+    }
 }
 
 // Launches the down (DOWN) or up kernel of one fine level with tile config cfg.
 // With ev0/ev1 set the launch goes through hipExtLaunchKernelGGL, whose
 // events carry the dispatch's own start / end timestamps (kernel time only,
 // as a profiler reports it).
+// The finest level's up kernel uses a halo of 5 (the fused epilogue needs
+// the ring next to the tile); the other up kernels 4, the down kernels 5.
 template <class T, bool DOWN, int C>
 inline void launch_vtile_cfg(bool top, hipStream_t s, const VLevel<T>& V, int fi, int fj, const double* w, int cni,
-                             int cnj, int cld, T* cbx, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr)
+                             int cnj, int cld, T* cbx, const VFuse& fu = VFuse(), hipEvent_t ev0 = nullptr,
+                             hipEvent_t ev1 = nullptr)
 {
     constexpr TileCfg G = VT_CFGS[C];
-    constexpr int H = DOWN ? 5 : 4;
+    constexpr int HN = DOWN ? 5 : 4;  // not the finest level
     const dim3 grid(cdiv(V.nj, G.tj), cdiv(V.ni, G.ti));
     if (ev0) {
         if (top)
-            hipExtLaunchKernelGGL((k_vtile<T, G.ti, G.tj, G.nt, H, DOWN, true>), grid, dim3(G.nt), 0, s, ev0, ev1, 0,
-                                  V, fi, fj, w, cni, cnj, cld, cbx);
+            hipExtLaunchKernelGGL((k_vtile<T, G.ti, G.tj, G.nt, 5, DOWN, true>), grid, dim3(G.nt), 0, s, ev0, ev1, 0,
+                                  V, fi, fj, w, cni, cnj, cld, cbx, fu);
         else
-            hipExtLaunchKernelGGL((k_vtile<T, G.ti, G.tj, G.nt, H, DOWN, false>), grid, dim3(G.nt), 0, s, ev0, ev1, 0,
-                                  V, fi, fj, w, cni, cnj, cld, cbx);
+            hipExtLaunchKernelGGL((k_vtile<T, G.ti, G.tj, G.nt, HN, DOWN, false>), grid, dim3(G.nt), 0, s, ev0, ev1,
+                                  0, V, fi, fj, w, cni, cnj, cld, cbx, fu);
     } else if (top) {
-        hipLaunchKernelGGL((k_vtile<T, G.ti, G.tj, G.nt, H, DOWN, true>), grid, dim3(G.nt), 0, s, V, fi, fj, w, cni,
-                           cnj, cld, cbx);
+        hipLaunchKernelGGL((k_vtile<T, G.ti, G.tj, G.nt, 5, DOWN, true>), grid, dim3(G.nt), 0, s, V, fi, fj, w, cni,
+                           cnj, cld, cbx, fu);
     } else {
-        hipLaunchKernelGGL((k_vtile<T, G.ti, G.tj, G.nt, H, DOWN, false>), grid, dim3(G.nt), 0, s, V, fi, fj, w, cni,
-                           cnj, cld, cbx);
+        hipLaunchKernelGGL((k_vtile<T, G.ti, G.tj, G.nt, HN, DOWN, false>), grid, dim3(G.nt), 0, s, V, fi, fj, w, cni,
+                           cnj, cld, cbx, fu);
     }
 }
 
 template <class T, bool DOWN>
 inline void launch_vtile(int cfg, bool top, hipStream_t s, const VLevel<T>& V, int fi, int fj, const double* w,
-                         int cni, int cnj, int cld, T* cbx, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr)
+                         int cni, int cnj, int cld, T* cbx, const VFuse& fu = VFuse(), hipEvent_t ev0 = nullptr,
+                         hipEvent_t ev1 = nullptr)
 {
     switch (cfg) {
-    case 1: launch_vtile_cfg<T, DOWN, 1>(top, s, V, fi, fj, w, cni, cnj, cld, cbx, ev0, ev1); break;
-    case 2: launch_vtile_cfg<T, DOWN, 2>(top, s, V, fi, fj, w, cni, cnj, cld, cbx, ev0, ev1); break;
-    case 3: launch_vtile_cfg<T, DOWN, 3>(top, s, V, fi, fj, w, cni, cnj, cld, cbx, ev0, ev1); break;
-    case 4: launch_vtile_cfg<T, DOWN, 4>(top, s, V, fi, fj, w, cni, cnj, cld, cbx, ev0, ev1); break;
-    case 5: launch_vtile_cfg<T, DOWN, 5>(top, s, V, fi, fj, w, cni, cnj, cld, cbx, ev0, ev1); break;
-    default: launch_vtile_cfg<T, DOWN, 0>(top, s, V, fi, fj, w, cni, cnj, cld, cbx, ev0, ev1); break;
+    case 1: launch_vtile_cfg<T, DOWN, 1>(top, s, V, fi, fj, w, cni, cnj, cld, cbx, fu, ev0, ev1); break;
+    case 2: launch_vtile_cfg<T, DOWN, 2>(top, s, V, fi, fj, w, cni, cnj, cld, cbx, fu, ev0, ev1); break;
+    case 3: launch_vtile_cfg<T, DOWN, 3>(top, s, V, fi, fj, w, cni, cnj, cld, cbx, fu, ev0, ev1); break;
+    case 4: launch_vtile_cfg<T, DOWN, 4>(top, s, V, fi, fj, w, cni, cnj, cld, cbx, fu, ev0, ev1); break;
+    case 5: launch_vtile_cfg<T, DOWN, 5>(top, s, V, fi, fj, w, cni, cnj, cld, cbx, fu, ev0, ev1); break;
+    default: launch_vtile_cfg<T, DOWN, 0>(top, s, V, fi, fj, w, cni, cnj, cld, cbx, fu, ev0, ev1); break;
     }
+}
+
+// tiles of the finest level with configuration cfg (partials of the fused epilogue)
+inline int vtile_count(int cfg, int ni, int nj)
+{
+    const TileCfg G = VT_CFGS[cfg < 0 || cfg >= VT_NCFG ? 0 : cfg];
+    return cdiv(nj, G.tj) * cdiv(ni, G.ti);
 }
 
 // ------------------------------------------------------------------ coarse levels in one workgroup

@@ -1,0 +1,31 @@
+// vfuse.hpp -- arguments of the BiCGStab work fused into the finest-level
+// V-cycle kernels (vcycle.hpp), shared with the solver (mg.hpp).
+#pragma once
+
+namespace cathode {
+
+// BiCGStab work fused into the finest-level kernels (real types; all other
+// levels and the complex path pass pro = epi = 0):
+//   down kernel, pro = 1: the V-cycle input is p = r (first) or
+//       r + beta (p_old - om v) with p_old read from pin; pro = 2:
+//       s = r - al v.  The new p / s is written to ps (never the buffer read)
+//       and the (scaled, for FP32 levels) input to b of the level.
+//   up kernel, epi = 1: y = V-cycle output (FP64), out = A y with the FP64
+//       finest operator, block partials of <dotvec, out> (and <out, out>
+//       with want_tt) into part[tile] and part[ntiles + tile].  Needs a halo
+//       of 5 so the ring next to the tile is still valid after 4 half sweeps.
+// Scalars come from sc at the slot indices given (beta = (rho1/rho)(al/om);
+// the FP32 input scale is s32 * scale, the output scale 1 / scale).
+struct VFuse {
+    int pro = 0, epi = 0, first = 0, mixed = 0, want_tt = 0;
+    int s_s32 = 0, s_scale = 0, s_rho1 = 0, s_rho = 0, s_al = 0, s_om = 0;
+    const double *r = nullptr, *v = nullptr, *pin = nullptr;
+    double* ps = nullptr;
+    const double* sc = nullptr;
+    const double *aP = nullptr, *aE = nullptr, *aW = nullptr, *aN = nullptr, *aS = nullptr;
+    double *y = nullptr, *out = nullptr;
+    const double* dotvec = nullptr;
+    double* part = nullptr;
+};
+
+}  // namespace cathode

@@ -208,10 +208,12 @@ def main():
     b = res[best]
     roof = {"bound": "hbm", "achieved": round(b["smooth_GBps"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(b["smooth_GBps"] / HBM_PEAK_GBS, 4), "traffic": traffic_per_launch("mixed" if best == "learned" else best),
-            "kernel": "k_vtile finest-level down kernel (2 red-black sweeps + residual + restriction), "
+            "kernel": "k_vtile finest-level down kernel (BiCGStab p/s update + 2 red-black sweeps + residual + "
+                      "restriction), "
                       + ("FP64" if best == "fp64" else "FP32"),
-            "bytes_model": "per cell: aP aE aW aN aS b read + x write + 1/4 coarse b write "
-                           "(7.25 values) + 1 activity byte",
+            "bytes_model": "per cell: aP aE aW aN aS + activity byte read, x + b + 1/4 coarse b written at "
+                           "level precision; r, v (+ old p on every other launch) read and new p or s written "
+                           "in FP64 (fused BiCGStab update): 58 B (FP32 levels), 87 B (FP64)",
             "bytes_per_launch": b["smooth_bytes"], "avg_us": round(b["smooth_avg_us"], 3)}
     out = {
         "metric": METRIC,
