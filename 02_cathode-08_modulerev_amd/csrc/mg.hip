@@ -330,7 +330,7 @@ __global__ void k_bicg_restart(int ni, int nj, int ld, const double* __restrict_
                                const double* __restrict__ b, const double* __restrict__ x,
                                const unsigned char* __restrict__ act, double* __restrict__ r,
                                double* __restrict__ r0, double* __restrict__ p, double* __restrict__ v,
-                               double* __restrict__ part, int nb)
+                               double* __restrict__ part, int nb, double* __restrict__ xzero)
 {
     __shared__ double lds[8];
     CELL_IJ;
@@ -338,7 +338,14 @@ __global__ void k_bicg_restart(int ni, int nj, int ld, const double* __restrict_
     if (i <= ni && j <= nj) {
         const size_t k = (size_t)i * ld + j;
         double q = 0.0;
-        if (act[k]) {
+        if (xzero) {
+            // zero initial guess: x = 0 on the interior, r = b
+            xzero[k] = 0.0;
+            if (act[k]) {
+                q = b[k];
+                rr = q * q;
+            }
+        } else if (act[k]) {
             q = b[k] - aP[k] * x[k] + offdiag(aE, aW, aN, aS, nine, n0, k, ld, [&](size_t m) { return x[m]; });
             rr = q * q;
         }
@@ -668,7 +675,10 @@ void MGSolver::setup(int mode, hipStream_t s)
     const int mixed = mode == CATHODE_MODE_MIXED ? 1 : 0;
     const int nb = cdiv(F.nj, BX) * cdiv(F.ni, BY);
     // level-0 activity lives in act0 (filled by the caller); mirror into actall
-    CATHODE_HIP(hipMemcpyAsync(actall.p, act0.p, n0, hipMemcpyDeviceToDevice, s));
+    if (act_dirty) {
+        CATHODE_HIP(hipMemcpyAsync(actall.p, act0.p, n0, hipMemcpyDeviceToDevice, s));
+        act_dirty = false;
+    }
     double* sbuf = F.b;  // row excess of level 0 (F.b is rewritten by the outer loop)
     hipLaunchKernelGGL(k_l0_prepare, grid2(F.ni, F.nj), dim3(BX, BY), 0, s, F.ni, F.nj, F.ld, act0.p, F.aP, F.aE,
                        F.aW, F.aN, F.aS, bsol.p, sbuf, part.p, nb);
@@ -891,7 +901,8 @@ int MGSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_sta
             autotune(mixed, s);
         }
     }
-    if (!keep_guess) CATHODE_HIP(hipMemsetAsync(xsol.p, 0, n0 * sizeof(double), s));
+    // without a kept guess the first restart zeroes x (interior; ghosts stay 0)
+    bool zero_guess = !keep_guess;
     keep_guess = false;
     double* r = kv.p;
     double* r0 = r + n0;
@@ -934,7 +945,8 @@ int MGSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_sta
     while (it < maxITR) {
         if (restart) {
             hipLaunchKernelGGL(k_bicg_restart, g, blk, 0, s, F.ni, F.nj, F.ld, F.aP, F.aE, F.aW, F.aN, F.aS, n9,
-                               n0, bsol.p, xa, act0.p, r, r0, pa, v, part.p, nb);
+                               n0, bsol.p, xa, act0.p, r, r0, pa, v, part.p, nb, zero_guess ? xa : nullptr);
+            zero_guess = false;
             hipLaunchKernelGGL(k_bicg_finish, dim3(1), dim3(256), 0, s, part.p, nb, scal.p, 1, 0.0);
         }
         it++;
@@ -1032,11 +1044,14 @@ int MGSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_sta
     }
     if (maxITR <= 0) {
         // no iteration: x = 0, relative residual 1 (0 when b = 0)
+        if (zero_guess) CATHODE_HIP(hipMemsetAsync(xsol.p, 0, n0 * sizeof(double), s));
         CATHODE_HIP(hipMemcpyAsync(h_scal, scal.p, SC_COUNT * sizeof(double), hipMemcpyDeviceToHost, s));
         CATHODE_HIP(hipStreamSynchronize(s));
         rel = h_scal[SC_BB] > 0.0 ? 1.0 : 0.0;
     }
-    if (xa != xsol.p) CATHODE_HIP(hipMemcpyAsync(xsol.p, xa, n0 * sizeof(double), hipMemcpyDeviceToDevice, s));
+    // the iterate ends in either buffer: it becomes the solution buffer (both
+    // hold zero ghosts and padding, every solve rewrites the interior)
+    if (xa != xsol.p) std::swap(xsol.p, xalt.p);
     if (prof.enabled) {
         CATHODE_HIP(hipStreamSynchronize(s));
         for (size_t k = 0; k + 1 < prof.ev.size(); k += 2) {

@@ -70,7 +70,13 @@ public:
     double* aS() { return L64[0].aS; }
     double* rhs() { return bsol.p; }
     double* sol() { return xsol.p; }
-    unsigned char* act() { return act0.p; }
+    // level-0 activity mask; taking the pointer marks the mask as written,
+    // so the next setup mirrors it into the level hierarchy
+    unsigned char* act()
+    {
+        act_dirty = true;
+        return act0.p;
+    }
     // Radial weights w[0..nj+1] on the device; nullptr = infer from the matrix.
     void set_weights(const double* d_w) { user_w = d_w; }
 
@@ -90,6 +96,7 @@ public:
     // sol() (interior cells; ghosts 0) instead; the flag clears itself.
     int solve(int maxITR, double tol, int mode, hipStream_t s, cathode_status* st);
     bool keep_guess = false;
+    bool act_dirty = true;
 
     MGProfile prof;  // dispatch events of the finest-level down kernel
 

@@ -27,6 +27,8 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <initializer_list>
+#include <utility>
 #include <memory>
 #include <vector>
 
@@ -846,9 +848,34 @@ static double** field_slot(cathode_ndd_fields* f, int k)
 static inline dim3 gblk() { return dim3(NB_X, NB_Y); }
 static inline dim3 ggrid(int ni, int nj) { return dim3(cdiv(nj, NB_X), cdiv(ni, NB_Y)); }
 
-static void d2d(NddHandle& H, double* dst, const double* src)
+// up to 8 whole-plane copies in one launch (blockIdx.y picks the pair)
+struct CopyList {
+    double* dst[8];
+    const double* src[8];
+};
+
+__global__ void k_copy_planes(CopyList L, size_t n)
 {
-    CATHODE_HIP(hipMemcpyAsync(dst, src, H.n * sizeof(double), hipMemcpyDeviceToDevice, H.s));
+    double* __restrict__ d = L.dst[blockIdx.y];
+    const double* __restrict__ a = L.src[blockIdx.y];
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x)
+        d[k] = a[k];
+}
+
+// dst_q = src_q for each listed pair, in one launch
+static void d2d(NddHandle& H, std::initializer_list<std::pair<double*, const double*>> pairs)
+{
+    CopyList L{};
+    int q = 0;
+    for (const auto& pr : pairs) {
+        L.dst[q] = pr.first;
+        L.src[q] = pr.second;
+        q++;
+    }
+    if (q == 0) return;
+    const int gx = std::min(cdiv((long)H.n, 256), 512);
+    hipLaunchKernelGGL(k_copy_planes, dim3(gx, q), dim3(256), 0, H.s, L, H.n);
+    CATHODE_LAUNCH_CHECK();
 }
 
 static double read_dev(NddHandle& H, const double* d, int count = 1)
@@ -964,11 +991,8 @@ static void poisson_routine(NddHandle& H, bool report)
 {
     const NddView& V = H.V;
     double** F = H.F;
-    d2d(H, H.phi_old, F[F_PHI]);
-    d2d(H, H.Ex_old, F[F_EX]);
-    d2d(H, H.Er_old, F[F_ER]);
-    d2d(H, H.sx_old, F[F_NUEX]);
-    d2d(H, H.sr_old, F[F_NUER]);
+    d2d(H, {{H.phi_old, F[F_PHI]}, {H.Ex_old, F[F_EX]}, {H.Er_old, F[F_ER]}, {H.sx_old, F[F_NUEX]},
+            {H.sr_old, F[F_NUER]}});
     hipLaunchKernelGGL(k_poisson_ghost, dim3(cdiv(std::max(V.ni, V.nj), 256)), dim3(256), 0, H.s, V, F[F_PHI]);
     PoissonArgs A{};
     A.phi = F[F_PHI]; A.rhoe = F[F_RHOE]; A.rhoi = F[F_RHOI]; A.Te = F[F_TE]; A.rhoe_old = F[F_RHOE_OLD];
@@ -979,9 +1003,7 @@ static void poisson_routine(NddHandle& H, bool report)
     int ncount = 0;
     for (;;) {
         ncount++;
-        d2d(H, H.tmp_c, F[F_PHI]);
-        d2d(H, H.tmp_x, F[F_NUEX]);
-        d2d(H, H.tmp_r, F[F_NUER]);
+        d2d(H, {{H.tmp_c, F[F_PHI]}, {H.tmp_x, F[F_NUEX]}, {H.tmp_r, F[F_NUER]}});
         hipLaunchKernelGGL(k_poisson_predict_x, ggrid(V.ni + 1, V.nj), gblk(), 0, H.s, V, A);
         hipLaunchKernelGGL(k_poisson_predict_r, ggrid(V.ni, V.nj), gblk(), 0, H.s, V, A);
         hipLaunchKernelGGL(k_poisson_assemble, ggrid(V.ni, V.nj), gblk(), 0, H.s, V, A, H.mg.aP(), H.mg.aE(),
@@ -991,8 +1013,7 @@ static void poisson_routine(NddHandle& H, bool report)
         if (learn) learned_guess(H);
         if (!solve(H, ncount > 1 ? H.p.error_cnv_HES_phi : -INFINITY, H.p.maxITR_SOR_phi, H.p.error_cnv_SOR_phi)) {
             // calcRes of the previous iteration met the PC criterion: undo the prediction
-            d2d(H, F[F_NUEX], H.tmp_x);
-            d2d(H, F[F_NUER], H.tmp_r);
+            d2d(H, {{F[F_NUEX], H.tmp_x}, {F[F_NUER], H.tmp_r}});
             break;
         }
         if (learn) learned_store(H);
@@ -1045,23 +1066,27 @@ static void sg_routine(NddHandle& H, int sp, bool report, double* err3)
         lin_maxit = H.p.maxITR_SOR_rhoeps; lin_tol = H.p.error_cnv_SOR_rhoeps;
     }
     A.n_old = n_old;
-    d2d(H, n_old, A.n);
-    d2d(H, H.sx_old, A.Fx);
-    d2d(H, H.sr_old, A.Fr);
     int ncount = 0;
     for (;;) {
         ncount++;
-        d2d(H, H.tmp_c, A.n);
-        d2d(H, H.tmp_x, A.Fx);
-        d2d(H, H.tmp_r, A.Fr);
+        if (ncount == 1) {
+            // the step's old values and this iteration's, one launch
+            if (sp == SP_ENERGY)
+                d2d(H, {{n_old, A.n}, {H.sx_old, A.Fx}, {H.sr_old, A.Fr}, {H.tmp_c, A.n}, {H.tmp_x, A.Fx},
+                        {H.tmp_r, A.Fr}, {H.Te_old, F[F_TE]}});
+            else
+                d2d(H, {{n_old, A.n}, {H.sx_old, A.Fx}, {H.sr_old, A.Fr}, {H.tmp_c, A.n}, {H.tmp_x, A.Fx},
+                        {H.tmp_r, A.Fr}});
+        } else {
+            d2d(H, {{H.tmp_c, A.n}, {H.tmp_x, A.Fx}, {H.tmp_r, A.Fr}});
+        }
         hipLaunchKernelGGL(k_sg_predict_x, ggrid(V.ni + 1, V.nj), gblk(), 0, H.s, V, A);
         hipLaunchKernelGGL(k_sg_predict_r, ggrid(V.ni, V.nj), gblk(), 0, H.s, V, A);
         hipLaunchKernelGGL(k_sg_assemble, ggrid(V.ni, V.nj), gblk(), 0, H.s, V, A, H.mg.aP(), H.mg.aE(), H.mg.aW(),
                            H.mg.aN(), H.mg.aS(), H.mg.rhs());
         CATHODE_LAUNCH_CHECK();
         if (!solve(H, ncount > 1 ? cnv : -INFINITY, lin_maxit, lin_tol)) {
-            d2d(H, A.Fx, H.tmp_x);
-            d2d(H, A.Fr, H.tmp_r);
+            d2d(H, {{A.Fx, H.tmp_x}, {A.Fr, H.tmp_r}});
             break;
         }
         hipLaunchKernelGGL(k_sg_correct, ggrid(V.ni, V.nj), gblk(), 0, H.s, V, A, H.mg.sol());
@@ -1096,7 +1121,7 @@ static void ndd_one_step(NddHandle& H, bool report)
     }
     // 4. electron energy
     if (H.p.energy_on) {
-        d2d(H, H.Te_old, F[F_TE]);
+        // Te_old = Te is taken with the routine's first copies
         sg_routine(H, SP_ENERGY, report, e3);
         if (report) {
             H.rep.error_rhoeps = e3[0];
