@@ -127,6 +127,22 @@ __device__ inline double block_sum(double v, double* lds)
     return s;
 }
 
+// Block-level sum returned to every thread (same fixed order in each, so
+// every block of a launch that reduces the same data gets the same value).
+// lds holds NT / 64 doubles.
+template <int NT>
+__device__ inline double block_sum_bcast(double v, double* lds)
+{
+    v = wave_sum(v);
+    const int t = lin_tid();
+    if ((t & 63) == 0) lds[t >> 6] = v;
+    __syncthreads();
+    double s = 0.0;
+    for (int k = 0; k < NT / 64; k++) s += lds[k];
+    __syncthreads();
+    return s;
+}
+
 template <int NT>
 __device__ inline double block_max(double v, double* lds)
 {

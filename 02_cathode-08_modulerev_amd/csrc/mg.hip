@@ -356,9 +356,11 @@ __global__ void k_bicg_restart(int ni, int nj, int ld, const double* __restrict_
     }
     double t = block_sum<BX * BY>(rr, lds);
     if (lin_tid() == 0) {
+        // the k_bicg_x layout: (r, r), (r0, r) = (r, r), |b - A x|^2 = (r, r)
         const int bid = blockIdx.y * gridDim.x + blockIdx.x;
         part[bid] = t;
         part[nb + bid] = t;
+        part[2 * nb + bid] = t;
     }
 }
 
@@ -453,14 +455,35 @@ __global__ void k_bicg_x(int ni, int nj, int ld, const double* __restrict__ aP, 
                          double* __restrict__ xn, double* __restrict__ r, const double* __restrict__ r0,
                          const double* __restrict__ ph, const double* __restrict__ sh,
                          const double* __restrict__ sv, const double* __restrict__ t,
-                         const double* __restrict__ sc, double* __restrict__ part, int nb)
+                         double* __restrict__ sc, double* __restrict__ part, int nb,
+                         const double* __restrict__ p2, int n2)
 {
     __shared__ double lds[8];
     CELL_IJ;
+    double om;
+    if (p2) {
+        // om = (t, s) / (t, t) from the up kernel's tile partials, summed by
+        // every workgroup in the same order; workgroup 0 stores it
+        double ts = 0.0, tt = 0.0;
+        for (int q = lin_tid(); q < n2; q += BX * BY) {
+            ts += p2[q];
+            tt += p2[n2 + q];
+        }
+        ts = block_sum_bcast<BX * BY>(ts, lds);
+        tt = block_sum_bcast<BX * BY>(tt, lds);
+        om = (tt > 0.0 && isfinite(ts / tt)) ? ts / tt : 0.0;
+        if (lin_tid() == 0 && blockIdx.x == 0 && blockIdx.y == 0) {
+            sc[SC_TS] = ts;
+            sc[SC_TT] = tt;
+            sc[SC_OM] = om;
+        }
+    } else {
+        om = sc[SC_OM];
+    }
     double a = 0.0, c = 0.0, e = 0.0;
     if (i <= ni && j <= nj) {
         const size_t k = (size_t)i * ld + j;
-        const double al = sc[SC_AL], om = sc[SC_OM];
+        const double al = sc[SC_AL];
         double xk = 0.0;
         if (act[k]) {
             xk = x[k] + al * ph[k] + om * sh[k];
@@ -638,10 +661,18 @@ void MGSolver::configure(int ni, int nj, hipStream_t s)
         for (int cfg = 0; cfg < VT_NCFG; cfg++) tiles_max = std::max(tiles_max, vtile_count(cfg, ni, nj));
         part.alloc(std::max(3 * (cdiv(nj, BX) * cdiv(ni, BY)), 2 * tiles_max) + 64);
     }
-    scal.alloc(16);
+    // the scalar slots, then three partial vectors of the 2-D block grid
+    const size_t nsc = SC_COUNT + 3 * (size_t)(cdiv(nj, BX) * cdiv(ni, BY)) + 64;
+    scal.alloc(nsc);
+    CATHODE_HIP(hipMemset(scal.p, 0, nsc * sizeof(double)));
     kv.alloc(9 * n0);
     CATHODE_HIP(hipMemset(kv.p, 0, 8 * n0 * sizeof(double)));
-    if (!h_scal) CATHODE_HIP(hipHostMalloc(&h_scal, 16 * sizeof(double), hipHostMallocDefault));
+    if (h_scal_n < nsc) {
+        if (h_scal) CATHODE_HIP(hipHostFree(h_scal));
+        h_scal = nullptr;
+        CATHODE_HIP(hipHostMalloc(&h_scal, nsc * sizeof(double), hipHostMallocDefault));
+        h_scal_n = nsc;
+    }
 }
 
 CoarsenArgs MGSolver::coarsen_args(int l, const double* w)
@@ -942,22 +973,33 @@ int MGSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_sta
     int it = 0;
     double rel = 0.0, bb = -1.0;
     bool restart = true;
+    // partials of the restart and x-update kernels: right after the scalar
+    // slots, so one copy brings both to the host
+    double* part3 = scal.p + SC_COUNT;
     while (it < maxITR) {
         if (restart) {
             hipLaunchKernelGGL(k_bicg_restart, g, blk, 0, s, F.ni, F.nj, F.ld, F.aP, F.aE, F.aW, F.aN, F.aS, n9,
-                               n0, bsol.p, xa, act0.p, r, r0, pa, v, part.p, nb, zero_guess ? xa : nullptr);
+                               n0, bsol.p, xa, act0.p, r, r0, pa, v, part3, nb, zero_guess ? xa : nullptr);
             zero_guess = false;
-            hipLaunchKernelGGL(k_bicg_finish, dim3(1), dim3(256), 0, s, part.p, nb, scal.p, 1, 0.0);
+            // the fused path sums these partials in its first down kernel
+            if (!fused) hipLaunchKernelGGL(k_bicg_finish, dim3(1), dim3(256), 0, s, part3, nb, scal.p, 1, 0.0);
         }
         it++;
         if (fused) {
-            // p and the V-cycle input in the finest down kernel; ph, v = A ph
-            // and (r0, v) in the finest up kernel; al
+            // p and the V-cycle input in the finest down kernel (rho1 from the
+            // x-update / restart partials); ph, v = A ph and (r0, v) partials in
+            // the finest up kernel.  rho1 alternates between two slots, so the
+            // down kernel reads the previous one while workgroup 0 stores the new.
+            const int s_new = (it & 1) ? SC_RHO1 : SC_RHO, s_prev = (it & 1) ? SC_RHO : SC_RHO1;
             VFuse fd = fbase, fu = fbase;
             fd.pro = 1;
             fd.first = restart ? 1 : 0;
             fd.pin = pa;
             fd.ps = pb;
+            fd.pred = part3 + nb;
+            fd.npred = nb;
+            fd.s_rho1 = s_new;
+            fd.s_rho = s_prev;
             fu.epi = 1;
             fu.y = ph;
             fu.out = v;
@@ -965,21 +1007,23 @@ int MGSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_sta
             fu.want_tt = 0;
             if (mixed) vcycle<float>(L32, s, fd, fu);
             else vcycle<double>(L64, s, fd, fu);
-            hipLaunchKernelGGL(k_bicg_finish, dim3(1), dim3(256), 0, s, part.p, ntiles, scal.p, 2, 0.0);
             std::swap(pa, pb);
             restart = false;
-            // s and the V-cycle input; sh, t = A sh, (t, s) and (t, t); om
+            // s = r - al v (al from the (r0, v) partials) and the V-cycle
+            // input; sh, t = A sh, (t, s) and (t, t) partials
             fd.pro = 2;
             fd.first = 0;
             fd.pin = nullptr;
             fd.ps = sv;
+            fd.pred = part.p;
+            fd.npred = ntiles;
+            fd.s_r0v = SC_R0V;
             fu.y = sh;
             fu.out = t;
             fu.dotvec = sv;
             fu.want_tt = 1;
             if (mixed) vcycle<float>(L32, s, fd, fu);
             else vcycle<double>(L64, s, fd, fu);
-            hipLaunchKernelGGL(k_bicg_finish, dim3(1), dim3(256), 0, s, part.p, ntiles, scal.p, 3, 0.0);
         } else {
         // p, ph = M^-1 p, v = A ph, al
         if (mixed) {
@@ -1013,16 +1057,28 @@ int MGSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_sta
         }
         hipLaunchKernelGGL(k_bicg_finish, dim3(1), dim3(256), 0, s, part.p, nb, scal.p, 3, 0.0);
         }
-        // x, recursive and true residuals
+        // x, recursive and true residuals (fused: om from the up kernel's partials)
         hipLaunchKernelGGL(k_bicg_x, g, blk, 0, s, F.ni, F.nj, F.ld, F.aP, F.aE, F.aW, F.aN, F.aS, n9, n0, bsol.p,
-                           act0.p,
-                           xa, xb, r, r0, ph, sh, sv, t, scal.p, part.p, nb);
-        hipLaunchKernelGGL(k_bicg_finish, dim3(1), dim3(256), 0, s, part.p, nb, scal.p, 4, 0.0);
+                           act0.p, xa, xb, r, r0, ph, sh, sv, t, scal.p, part3, nb,
+                           fused ? (const double*)part.p : (const double*)nullptr, ntiles);
+        if (!fused) hipLaunchKernelGGL(k_bicg_finish, dim3(1), dim3(256), 0, s, part3, nb, scal.p, 4, 0.0);
         CATHODE_LAUNCH_CHECK();
         std::swap(xa, xb);
-        // one host round trip per iteration: |b|^2, |r|^2, |b - A x|^2
-        CATHODE_HIP(hipMemcpyAsync(h_scal, scal.p, SC_COUNT * sizeof(double), hipMemcpyDeviceToHost, s));
+        // one host round trip per iteration: the scalars and the x-update
+        // partials; |r|^2 and |b - A x|^2 summed here in a fixed order
+        CATHODE_HIP(hipMemcpyAsync(h_scal, scal.p, (SC_COUNT + 3 * (size_t)nb) * sizeof(double),
+                                   hipMemcpyDeviceToHost, s));
         CATHODE_HIP(hipStreamSynchronize(s));
+        {
+            double rr = 0.0, rt2 = 0.0;
+            const double* hp = h_scal + SC_COUNT;
+            for (int q = 0; q < nb; q++) {
+                rr += hp[q];
+                rt2 += hp[2 * nb + q];
+            }
+            h_scal[SC_RR] = rr;
+            h_scal[SC_RT] = rt2;
+        }
         bb = h_scal[SC_BB];
         if (it == 1 && h_scal[SC_SIDE] < side_cnv) {
             // the caller's stopping test (read with this round trip) says the

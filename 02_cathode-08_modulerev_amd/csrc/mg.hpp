@@ -154,7 +154,8 @@ private:
     DevBuf<double> part;       // reduction partials (2 slots)
     DevBuf<double> kv;         // BiCGStab vectors r, r0, p, v, s, t, ph, sh, p (second buffer)
     DevBuf<double> scal;       // device scalars: [0] |b|^2, [1] |r|^2, [2] scale, [3] f32 matrix scale
-    double* h_scal = nullptr;  // pinned mirror
+    double* h_scal = nullptr;  // pinned mirror of scal
+    size_t h_scal_n = 0;
 };
 
 // FP64 residual-norm helper used by tests of the library itself

@@ -78,6 +78,37 @@ __global__ void __launch_bounds__(NT)
     const int gj0 = bj * TJ + 1 - H;
     T cP[K], cE[K], cW[K], cN[K], cS[K], cB[K];
     unsigned flags[K];  // bit0 active, bit1 colour, bit2 updatable (not on the region rim), bit3 tile cell
+    // fused BiCGStab scalars of the down prologue: q = r - al v (pro 2) or
+    // r + beta (p_old - om v) (pro 1); f scales the FP32 input
+    double f_al = 0.0, f_beta = 0.0, f_om = 0.0, f_in = 1.0;
+    if constexpr (REAL && DOWN && TOP) {
+        if (fu.pro) {
+            const bool lead = tid == 0 && blockIdx.x == 0 && blockIdx.y == 0;
+            if (fu.pred) {
+                __shared__ double pr[NT / 64];
+                double a = 0.0;
+                for (int k = tid; k < fu.npred; k += NT) a += fu.pred[k];
+                a = block_sum_bcast<NT>(a, pr);
+                if (fu.pro == 2) {
+                    const double al = fu.sc[fu.s_rho1] / a;
+                    f_al = (a != 0.0 && isfinite(al)) ? al : 0.0;  // breakdown: no step
+                    if (lead) {
+                        fu.sc[fu.s_r0v] = a;
+                        fu.sc[fu.s_al] = f_al;
+                    }
+                } else {
+                    if (!fu.first) f_beta = (a / fu.sc[fu.s_rho]) * (fu.sc[fu.s_al] / fu.sc[fu.s_om]);
+                    if (lead) fu.sc[fu.s_rho1] = a;
+                }
+            } else if (fu.pro == 2) {
+                f_al = fu.sc[fu.s_al];
+            } else if (!fu.first) {
+                f_beta = (fu.sc[fu.s_rho1] / fu.sc[fu.s_rho]) * (fu.sc[fu.s_al] / fu.sc[fu.s_om]);
+            }
+            f_om = fu.sc[fu.s_om];
+            f_in = fu.mixed ? fu.sc[fu.s_s32] * fu.sc[fu.s_scale] : 1.0;
+        }
+    }
 #pragma unroll
     for (int k = 0; k < K; k++) {
         const int c = tid + k * NT;
@@ -102,16 +133,13 @@ __global__ void __launch_bounds__(NT)
                     if (fu.pro) {
                         double q;
                         if (fu.pro == 2) {
-                            q = fu.r[g] - fu.sc[fu.s_al] * fu.v[g];
+                            q = fu.r[g] - f_al * fu.v[g];
                         } else if (fu.first) {
                             q = fu.r[g];
                         } else {
-                            const double beta =
-                                (fu.sc[fu.s_rho1] / fu.sc[fu.s_rho]) * (fu.sc[fu.s_al] / fu.sc[fu.s_om]);
-                            q = fu.r[g] + beta * (fu.pin[g] - fu.sc[fu.s_om] * fu.v[g]);
+                            q = fu.r[g] + f_beta * (fu.pin[g] - f_om * fu.v[g]);
                         }
-                        const double f = fu.mixed ? fu.sc[fu.s_s32] * fu.sc[fu.s_scale] : 1.0;
-                        cB[k] = (T)(q * f);
+                        cB[k] = (T)(q * f_in);
                         if (owner) {  // the old p is read from pin, so no tile reads what another writes
                             fu.ps[g] = q;
                             F.b[g] = cB[k];

@@ -16,12 +16,19 @@ namespace cathode {
 //       of 5 so the ring next to the tile is still valid after 4 half sweeps.
 // Scalars come from sc at the slot indices given (beta = (rho1/rho)(al/om);
 // the FP32 input scale is s32 * scale, the output scale 1 / scale).
+// With pred set, the down kernel first sums the npred block partials at
+// pred itself (every workgroup, same order) instead of reading a finished
+// scalar: pro = 1 takes rho1 = (r0, r) from them (stored to sc[s_rho1] by
+// workgroup 0; s_rho then names the previous iteration's slot), pro = 2
+// takes (r0, v) and al = rho1 / (r0, v) (stored to sc[s_r0v], sc[s_al]).
 struct VFuse {
     int pro = 0, epi = 0, first = 0, mixed = 0, want_tt = 0;
-    int s_s32 = 0, s_scale = 0, s_rho1 = 0, s_rho = 0, s_al = 0, s_om = 0;
+    int s_s32 = 0, s_scale = 0, s_rho1 = 0, s_rho = 0, s_al = 0, s_om = 0, s_r0v = 0;
     const double *r = nullptr, *v = nullptr, *pin = nullptr;
     double* ps = nullptr;
-    const double* sc = nullptr;
+    double* sc = nullptr;
+    const double* pred = nullptr;
+    int npred = 0;
     const double *aP = nullptr, *aE = nullptr, *aW = nullptr, *aN = nullptr, *aS = nullptr;
     double *y = nullptr, *out = nullptr;
     const double* dotvec = nullptr;
