@@ -230,6 +230,7 @@ int cathode_dev_alloc(size_t bytes, void** ptr)
         return CATHODE_ERR_NOMEM;
     }
     CATHODE_HIP(hipMemset(p, 0, bytes));
+    CATHODE_HIP(hipDeviceSynchronize());  // null-stream memset before any stream use
     *ptr = p;
     return CATHODE_OK;
     CATHODE_GUARD_END

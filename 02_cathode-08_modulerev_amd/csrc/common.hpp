@@ -144,6 +144,19 @@ __device__ inline double block_sum_bcast(double v, double* lds)
 }
 
 template <int NT>
+__device__ inline double block_max_bcast(double v, double* lds)
+{
+    v = wave_max(v);
+    const int t = lin_tid();
+    if ((t & 63) == 0) lds[t >> 6] = v;
+    __syncthreads();
+    double s = -INFINITY;
+    for (int k = 0; k < NT / 64; k++) s = fmax(s, lds[k]);
+    __syncthreads();
+    return s;
+}
+
+template <int NT>
 __device__ inline double block_max(double v, double* lds)
 {
     v = wave_max(v);

@@ -547,6 +547,9 @@ void HelmSolver::configure(int ni, int nj)
     part.alloc(4 * (cdiv(nj, HX) * cdiv(ni, HY)) + 64);
     sc.alloc(HS_COUNT);
     if (!h_sc) CATHODE_HIP(hipHostMalloc(&h_sc, HS_COUNT * sizeof(double), hipHostMallocDefault));
+    // the plain hipMemset calls above run on the null stream, which the
+    // library's non-blocking streams do not wait for
+    CATHODE_HIP(hipDeviceSynchronize());
 }
 
 HCoarsen HelmSolver::cargs(int l)

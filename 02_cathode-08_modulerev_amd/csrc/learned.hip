@@ -341,6 +341,7 @@ static void cnn_shape(CnnNet& N, int ni, int nj)
     CATHODE_HIP(hipMemset(N.g6.p, 0, px * 16 * sizeof(bf16_t)));
     N.y.alloc(px);
     CATHODE_HIP(hipMemset(N.y.p, 0, px * sizeof(float)));
+    CATHODE_HIP(hipDeviceSynchronize());  // null-stream memsets before any stream use
     // weight-gradient partials: 2-row bands, the largest layer's outputs
     N.part.alloc((size_t)cdiv(ni, 2) * (9 * CNN_MAXC * CNN_MAXC + CNN_MAXC));
     N.dpart.alloc(256);
@@ -461,6 +462,7 @@ int cathode_cnn_create(int seed, void** net)
     CATHODE_HIP(hipMemset(N->g.p, 0, n * sizeof(float)));
     CATHODE_HIP(hipMemset(N->m.p, 0, n * sizeof(float)));
     CATHODE_HIP(hipMemset(N->v.p, 0, n * sizeof(float)));
+    CATHODE_HIP(hipDeviceSynchronize());  // null-stream memsets before any stream use
     for (int l = 0; l < CNN_L; l++) {
         N->wf[l].alloc(frag_elems(CNN_CIN[l], CNN_COUT[l]));
         N->wb[l].alloc(frag_elems(CNN_COUT[l], CNN_CIN[l]));

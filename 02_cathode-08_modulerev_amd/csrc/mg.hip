@@ -93,59 +93,6 @@ __global__ void k_fill(double* p, int n, double v)
     if (k < n) p[k] = v;
 }
 
-// mass part m = w_j * min(s) over the 5-point neighbourhood; in mixed mode
-// also the FP32 copy (scaled by s32) of the level-0 coefficients
-__global__ void k_l0_mass(int ni, int nj, int ld, const unsigned char* __restrict__ act,
-                          const double* __restrict__ s, const double* __restrict__ w,
-                          double* __restrict__ m, const double* __restrict__ aP, const double* __restrict__ aE,
-                          const double* __restrict__ aW, const double* __restrict__ aN,
-                          const double* __restrict__ aS, float* __restrict__ fP, float* __restrict__ fE,
-                          float* __restrict__ fW, float* __restrict__ fN, float* __restrict__ fS,
-                          const double* __restrict__ sc, int mixed)
-{
-    CELL_IJ;
-    if (i > ni || j > nj) return;
-    const size_t k = (size_t)i * ld + j;
-    if (mixed) {
-        const double s32 = sc[3];
-        fP[k] = (float)(aP[k] * s32);
-        fE[k] = (float)(aE[k] * s32);
-        fW[k] = (float)(aW[k] * s32);
-        fN[k] = (float)(aN[k] * s32);
-        fS[k] = (float)(aS[k] * s32);
-    }
-    if (!act[k]) {
-        m[k] = 0.0;
-        return;
-    }
-    double v = s[k];
-    if (act[k + ld]) v = fmin(v, s[k + ld]);
-    if (act[k - ld]) v = fmin(v, s[k - ld]);
-    if (act[k + 1]) v = fmin(v, s[k + 1]);
-    if (act[k - 1]) v = fmin(v, s[k - 1]);
-    m[k] = w[j] * v;
-}
-
-// setup scalars: |b|^2, the mixed input scale and the FP32 matrix scale
-__global__ void k_setup_finish(const double* __restrict__ part, int nb, double* __restrict__ sc, double nact)
-{
-    __shared__ double lds[8];
-    double m = 0.0, b = 0.0;
-    for (int k = threadIdx.x; k < nb; k += blockDim.x) {
-        m = fmax(m, part[k]);
-        b += part[nb + k];
-    }
-    m = block_max<256>(m, lds);
-    b = block_sum<256>(b, lds);
-    if (threadIdx.x == 0) {
-        sc[0] = b;                                  // SC_BB
-        sc[2] = b > 0.0 ? sqrt(nact / b) : 1.0;     // SC_SCALE
-        int e = 0;
-        (void)frexp(m > 0.0 ? m : 1.0, &e);
-        sc[3] = ldexp(1.0, -e);                     // SC_S32: s32 * max|aP| in [0.5, 1)
-    }
-}
-
 // coarse operator of coarse cell (I, J) from level F
 struct CoarsenArgs {
     int fni, fnj, fld, fi, fj;
@@ -159,7 +106,7 @@ struct CoarsenArgs {
     double hx, hy, hb;
 };
 
-__device__ inline void coarsen_cell(const CoarsenArgs& a, int I, int J, const double* sc, int mixed)
+__device__ inline void coarsen_cell(const CoarsenArgs& a, int I, int J, double s32, int mixed)
 {
     const int i0 = a.fi * (I - 1) + 1, i1 = min(a.fi * I, a.fni);
     const int j0 = a.fj * (J - 1) + 1, j1 = min(a.fj * J, a.fnj);
@@ -203,7 +150,6 @@ __device__ inline void coarsen_cell(const CoarsenArgs& a, int I, int J, const do
     a.caN[kc] = N;
     a.caS[kc] = So;
     if (mixed) {
-        const double s32 = sc[3];
         a.gP[kc] = (float)(P * s32);
         a.gE[kc] = (float)(E * s32);
         a.gW[kc] = (float)(W * s32);
@@ -217,7 +163,7 @@ __global__ void k_coarsen(CoarsenArgs a, const double* __restrict__ sc, int mixe
     const int J = 1 + blockIdx.x * BX + threadIdx.x;
     const int I = 1 + blockIdx.y * BY + threadIdx.y;
     if (I > a.cni || J > a.cnj) return;
-    coarsen_cell(a, I, J, sc, mixed);
+    coarsen_cell(a, I, J, mixed ? sc[3] : 1.0, mixed);
 }
 
 // all coarse levels of the tail, one workgroup, level by level
@@ -229,10 +175,94 @@ struct CoarsenChain {
 
 __global__ void k_coarsen_chain(CoarsenChain c, const double* __restrict__ sc, int mixed)
 {
+    const double s32 = mixed ? sc[3] : 1.0;
     for (int l = 0; l < c.n; l++) {
         const CoarsenArgs& a = c.L[l];
-        for (int q = threadIdx.x; q < a.cni * a.cnj; q += blockDim.x) coarsen_cell(a, 1 + q / a.cnj, 1 + q % a.cnj, sc, mixed);
+        for (int q = threadIdx.x; q < a.cni * a.cnj; q += blockDim.x)
+            coarsen_cell(a, 1 + q / a.cnj, 1 + q % a.cnj, s32, mixed);
         __syncthreads();
+    }
+}
+
+// The setup after k_l0_prepare in one launch: the setup scalars (every
+// workgroup sums the prepare partials in the same order, workgroup 0 stores
+// them), the level-0 mass and FP32 copy, and the first nf coarsenings.
+// Aggregates nest, so a workgroup owning an ri x rj block of level nf owns
+// the cells of every finer level that block aggregates and coarsens them
+// level by level with only its own barriers.
+constexpr int SETUP_FUSE_MAX = 3;
+constexpr int SETUP_NT = 256;
+struct SetupLevels {
+    int ni, nj, ld;
+    const unsigned char* act;
+    const double *s, *w;
+    double* m;
+    const double *aP, *aE, *aW, *aN, *aS;
+    float *fP, *fE, *fW, *fN, *fS;
+    const double* part;
+    int nb;
+    double nact;
+    int nf, ri, rj;
+    int pi[SETUP_FUSE_MAX + 1], pj[SETUP_FUSE_MAX + 1];  // level-l block = (ri pi[l]) x (rj pj[l]) cells
+    CoarsenArgs L[SETUP_FUSE_MAX];
+};
+
+__global__ void __launch_bounds__(SETUP_NT) k_setup_levels(SetupLevels a, double* __restrict__ sc, int mixed)
+{
+    __shared__ double lds[SETUP_NT / 64];
+    const int tid = threadIdx.x;
+    double mx = 0.0, bb = 0.0;
+    for (int k = tid; k < a.nb; k += SETUP_NT) {
+        mx = fmax(mx, a.part[k]);
+        bb += a.part[a.nb + k];
+    }
+    mx = block_max_bcast<SETUP_NT>(mx, lds);
+    bb = block_sum_bcast<SETUP_NT>(bb, lds);
+    int e = 0;
+    (void)frexp(mx > 0.0 ? mx : 1.0, &e);
+    const double s32 = ldexp(1.0, -e);  // s32 * max|aP| in [0.5, 1)
+    if (tid == 0 && blockIdx.x == 0 && blockIdx.y == 0) {
+        sc[0] = bb;                                    // |b|^2
+        sc[2] = bb > 0.0 ? sqrt(a.nact / bb) : 1.0;   // mixed input scale
+        sc[3] = s32;
+    }
+    // level 0: mass part m = w_j min(s) over the 5-point neighbourhood, FP32 copy
+    {
+        const int rows = a.ri * a.pi[0], cols = a.rj * a.pj[0];
+        const int i0 = 1 + (int)blockIdx.y * rows, j0 = 1 + (int)blockIdx.x * cols;
+        for (int q = tid; q < rows * cols; q += SETUP_NT) {
+            const int i = i0 + q / cols, j = j0 + q % cols;
+            if (i > a.ni || j > a.nj) continue;
+            const size_t k = (size_t)i * a.ld + j;
+            if (mixed) {
+                a.fP[k] = (float)(a.aP[k] * s32);
+                a.fE[k] = (float)(a.aE[k] * s32);
+                a.fW[k] = (float)(a.aW[k] * s32);
+                a.fN[k] = (float)(a.aN[k] * s32);
+                a.fS[k] = (float)(a.aS[k] * s32);
+            }
+            if (!a.act[k]) {
+                a.m[k] = 0.0;
+                continue;
+            }
+            double v = a.s[k];
+            if (a.act[k + a.ld]) v = fmin(v, a.s[k + a.ld]);
+            if (a.act[k - a.ld]) v = fmin(v, a.s[k - a.ld]);
+            if (a.act[k + 1]) v = fmin(v, a.s[k + 1]);
+            if (a.act[k - 1]) v = fmin(v, a.s[k - 1]);
+            a.m[k] = a.w[j] * v;
+        }
+    }
+    for (int l = 0; l < a.nf; l++) {
+        __syncthreads();  // level l of this block written (global, same workgroup)
+        const CoarsenArgs& c = a.L[l];
+        const int rows = a.ri * a.pi[l + 1], cols = a.rj * a.pj[l + 1];
+        const int I0 = 1 + (int)blockIdx.y * rows, J0 = 1 + (int)blockIdx.x * cols;
+        for (int q = tid; q < rows * cols; q += SETUP_NT) {
+            const int I = I0 + q / cols, J = J0 + q % cols;
+            if (I > c.cni || J > c.cnj) continue;
+            coarsen_cell(c, I, J, s32, mixed);
+        }
     }
 }
 
@@ -673,6 +703,9 @@ void MGSolver::configure(int ni, int nj, hipStream_t s)
         CATHODE_HIP(hipHostMalloc(&h_scal, nsc * sizeof(double), hipHostMallocDefault));
         h_scal_n = nsc;
     }
+    // the plain hipMemset calls above run on the null stream, which the
+    // library's non-blocking streams do not wait for
+    CATHODE_HIP(hipDeviceSynchronize());
 }
 
 CoarsenArgs MGSolver::coarsen_args(int l, const double* w)
@@ -698,7 +731,7 @@ CoarsenArgs MGSolver::coarsen_args(int l, const double* w)
 }
 
 // Builds the hierarchy for the level-0 system the caller filled in, plus the
-// setup scalars (|b|^2, FP32 scales): 4 + lt launches.
+// setup scalars (|b|^2, FP32 scales): 3 launches while lt <= 3.
 void MGSolver::setup(int mode, hipStream_t s)
 {
     Level<double>& F = L64[0];
@@ -713,7 +746,6 @@ void MGSolver::setup(int mode, hipStream_t s)
     double* sbuf = F.b;  // row excess of level 0 (F.b is rewritten by the outer loop)
     hipLaunchKernelGGL(k_l0_prepare, grid2(F.ni, F.nj), dim3(BX, BY), 0, s, F.ni, F.nj, F.ld, act0.p, F.aP, F.aE,
                        F.aW, F.aN, F.aS, bsol.p, sbuf, part.p, nb);
-    hipLaunchKernelGGL(k_setup_finish, dim3(1), dim3(256), 0, s, part.p, nb, scal.p, (double)F.ni * (double)F.nj);
     CATHODE_LAUNCH_CHECK();
     const double* w = user_w;
     if (!w) {
@@ -722,13 +754,33 @@ void MGSolver::setup(int mode, hipStream_t s)
         CATHODE_LAUNCH_CHECK();
         w = w0.p;
     }
-    Level<float>& G = L32[0];
-    hipLaunchKernelGGL(k_l0_mass, grid2(F.ni, F.nj), dim3(BX, BY), 0, s, F.ni, F.nj, F.ld, act0.p, sbuf, w,
-                       mass.p + level_off[0], F.aP, F.aE, F.aW, F.aN, F.aS, G.aP, G.aE, G.aW, G.aN, G.aS,
-                       (const double*)scal.p, mixed);
-    CATHODE_LAUNCH_CHECK();
-    // fine coarse levels: one launch each; the rest in one workgroup
-    for (int l = 0; l + 1 < nl && l < lt; l++) {
+    // scalars, level-0 mass and the first coarsenings: one launch; a
+    // workgroup owns a 16 x 32 block of level 0 (fewer rows / columns where a
+    // direction stops coarsening)
+    const int nf = std::min(std::min(lt, nl - 1), SETUP_FUSE_MAX);
+    {
+        Level<float>& G = L32[0];
+        SetupLevels a{};
+        a.ni = F.ni; a.nj = F.nj; a.ld = F.ld; a.act = act0.p; a.s = sbuf; a.w = w; a.m = mass.p + level_off[0];
+        a.aP = F.aP; a.aE = F.aE; a.aW = F.aW; a.aN = F.aN; a.aS = F.aS;
+        a.fP = G.aP; a.fE = G.aE; a.fW = G.aW; a.fN = G.aN; a.fS = G.aS;
+        a.part = part.p; a.nb = nb; a.nact = (double)F.ni * (double)F.nj;
+        a.nf = nf;
+        a.pi[nf] = a.pj[nf] = 1;
+        for (int l = nf - 1; l >= 0; l--) {
+            a.pi[l] = a.pi[l + 1] * (L64[l].cx ? 2 : 1);
+            a.pj[l] = a.pj[l + 1] * (L64[l].cy ? 2 : 1);
+            a.L[l] = coarsen_args(l, w);
+        }
+        a.ri = std::max(1, 16 / a.pi[0]);
+        a.rj = std::max(1, 32 / a.pj[0]);
+        const int cni = L64[nf].ni, cnj = L64[nf].nj;
+        hipLaunchKernelGGL(k_setup_levels, dim3(cdiv(cnj, a.rj), cdiv(cni, a.ri)), dim3(SETUP_NT), 0, s, a, scal.p,
+                           mixed);
+        CATHODE_LAUNCH_CHECK();
+    }
+    // further fine levels: one launch each; the rest in one workgroup
+    for (int l = nf; l + 1 < nl && l < lt; l++) {
         Level<double>& C = L64[l + 1];
         hipLaunchKernelGGL(k_coarsen, grid2(C.ni, C.nj), dim3(BX, BY), 0, s, coarsen_args(l, w), (const double*)scal.p,
                            mixed);

@@ -1168,6 +1168,7 @@ int cathode_ndd_create(const cathode_ndd_params* p, void** handle)
         for (int k = 0; k < 19; k++) H->F[k] = H->fields.p + k * H->n;
         H->work.alloc(12 * H->n);
         CATHODE_HIP(hipMemset(H->work.p, 0, 12 * H->n * sizeof(double)));
+        CATHODE_HIP(hipDeviceSynchronize());  // null-stream memsets before the handle's stream runs
         double** w[12] = {&H->phi_old, &H->Ex_old, &H->Er_old, &H->tmp_c, &H->tmp_x, &H->tmp_r,
                           &H->dphi, &H->s_old, &H->sx_old, &H->sr_old, &H->Te_old, &H->rhoi_old};
         for (int k = 0; k < 12; k++) *w[k] = H->work.p + k * H->n;
