@@ -180,4 +180,10 @@ __global__ void k_reduce_partials(const double* part, int nblocks, int nvec, con
 void reduce_partials(const double* part, int nblocks, int nvec, const int* d_is_max, double* out,
                      hipStream_t s);
 
+// Small device-to-host reads: a kernel on s stores n doubles from device
+// memory d straight into host (pinned, hipHostMalloc) memory h; the caller
+// synchronises s before reading h.  Used for the per-iteration scalars
+// instead of the runtime's copy path.
+void fetch_host(double* h, const double* d, size_t n, hipStream_t s);
+
 }  // namespace cathode

@@ -508,12 +508,12 @@ void HelmSolver::configure(int ni, int nj)
         tot += (size_t)(d[l][0] + 2) * hpad(d[l][1]);
     }
     off[nl] = tot;
-    st64.alloc(7 * tot);
-    st32.alloc(7 * tot);
+    st64.alloc(8 * tot);
+    st32.alloc(8 * tot);
     mass.alloc(tot);
     actall.alloc(tot);
-    CATHODE_HIP(hipMemset(st64.p, 0, 7 * tot * sizeof(cxd)));
-    CATHODE_HIP(hipMemset(st32.p, 0, 7 * tot * sizeof(cxf)));
+    CATHODE_HIP(hipMemset(st64.p, 0, 8 * tot * sizeof(cxd)));
+    CATHODE_HIP(hipMemset(st32.p, 0, 8 * tot * sizeof(cxf)));
     CATHODE_HIP(hipMemset(mass.p, 0, tot * sizeof(cxd)));
     CATHODE_HIP(hipMemset(actall.p, 0, tot));
     for (int l = 0; l < nl; l++) {
@@ -528,9 +528,9 @@ void HelmSolver::configure(int ni, int nj)
         cxd* p = st64.p;
         cxf* q = st32.p;
         A.aP = p + 5 * o; A.aE = A.aP + sz; A.aW = A.aE + sz; A.aN = A.aW + sz; A.aS = A.aN + sz;
-        A.b = p + 5 * tot + o; A.x = p + 6 * tot + o;
+        A.b = p + 5 * tot + o; A.x = p + 6 * tot + o; A.xu = p + 7 * tot + o;
         B.aP = q + 5 * o; B.aE = B.aP + sz; B.aW = B.aE + sz; B.aN = B.aW + sz; B.aS = B.aN + sz;
-        B.b = q + 5 * tot + o; B.x = q + 6 * tot + o;
+        B.b = q + 5 * tot + o; B.x = q + 6 * tot + o; B.xu = q + 7 * tot + o;
     }
     const size_t n0 = off[1];
     act0.alloc(n0);
@@ -600,7 +600,7 @@ void HelmSolver::vcycle(Level<T>* L, hipStream_t s)
     for (int l = 0; l < lt; l++) {
         Level<T>& A = L[l];
         Level<T>& C = L[l + 1];
-        VLevel<T> V{A.ni, A.nj, A.ld, A.aP, A.aE, A.aW, A.aN, A.aS, actall.p + off[l], A.b, A.x};
+        VLevel<T> V{A.ni, A.nj, A.ld, A.aP, A.aE, A.aW, A.aN, A.aS, actall.p + off[l], A.b, A.x, nullptr};
         launch_vtile_cfg<T, true, HT_CFG>(l == 0, s, V, A.cx ? 2 : 1, A.cy ? 2 : 1, (const double*)w0.p, C.ni, C.nj,
                                            C.ld, C.b);
     }
@@ -631,9 +631,9 @@ void HelmSolver::vcycle(Level<T>* L, hipStream_t s)
     for (int l = lt - 1; l >= 0; l--) {
         Level<T>& A = L[l];
         Level<T>& C = L[l + 1];
-        VLevel<T> V{A.ni, A.nj, A.ld, A.aP, A.aE, A.aW, A.aN, A.aS, actall.p + off[l], A.b, A.x};
+        VLevel<T> V{A.ni, A.nj, A.ld, A.aP, A.aE, A.aW, A.aN, A.aS, actall.p + off[l], A.b, A.x, A.xu};
         launch_vtile_cfg<T, false, HT_CFG>(l == 0, s, V, A.cx ? 2 : 1, A.cy ? 2 : 1, (const double*)nullptr, C.ni,
-                                            C.nj, C.ld, C.x);
+                                            C.nj, C.ld, l + 1 < lt ? C.xu : C.x);
     }
     CATHODE_LAUNCH_CHECK();
 }
@@ -651,6 +651,9 @@ void HelmSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_
     HOp A{F.ni, F.nj, F.ld, F.aP, F.aE, F.aW, F.aN, F.aS, act0.p};
     cxd *r = kv.p, *r0 = r + n0, *p = r0 + n0, *v = p + n0, *sv = v + n0, *t = sv + n0, *ph = t + n0, *sh = ph + n0;
     cxd *xa = xsol.p, *xb = xalt.p;
+    // V-cycle output on level 0 (x when the whole hierarchy runs in the tail)
+    const cxf* vout32 = lt > 0 ? L32[0].xu : L32[0].x;
+    const cxd* vout64 = lt > 0 ? F.xu : F.x;
     int it = 0;
     double rel = 0.0;
     bool restart = true;
@@ -663,29 +666,29 @@ void HelmSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_
         if (mixed) {
             hipLaunchKernelGGL(hk_p<cxf>, g, blk, 0, s, A, r, v, p, L32[0].b, L32[0].x, sc.p, restart ? 1 : 0, 1);
             vcycle<cxf>(L32, s);
-            hipLaunchKernelGGL(hk_apply<cxf>, g, blk, 0, s, A, L32[0].x, ph, v, r0, sc.p, 1, 0, part.p, nb);
+            hipLaunchKernelGGL(hk_apply<cxf>, g, blk, 0, s, A, (const cxf*)vout32, ph, v, r0, sc.p, 1, 0, part.p, nb);
         } else {
             hipLaunchKernelGGL(hk_p<cxd>, g, blk, 0, s, A, r, v, p, F.b, F.x, sc.p, restart ? 1 : 0, 0);
             vcycle<cxd>(L64, s);
-            hipLaunchKernelGGL(hk_apply<cxd>, g, blk, 0, s, A, F.x, ph, v, r0, sc.p, 0, 0, part.p, nb);
+            hipLaunchKernelGGL(hk_apply<cxd>, g, blk, 0, s, A, (const cxd*)vout64, ph, v, r0, sc.p, 0, 0, part.p, nb);
         }
         hipLaunchKernelGGL(hk_finish, dim3(1), dim3(256), 0, s, part.p, nb, sc.p, 2);
         restart = false;
         if (mixed) {
             hipLaunchKernelGGL(hk_s<cxf>, g, blk, 0, s, A, r, v, sv, L32[0].b, L32[0].x, sc.p, 1);
             vcycle<cxf>(L32, s);
-            hipLaunchKernelGGL(hk_apply<cxf>, g, blk, 0, s, A, L32[0].x, sh, t, sv, sc.p, 1, 1, part.p, nb);
+            hipLaunchKernelGGL(hk_apply<cxf>, g, blk, 0, s, A, (const cxf*)vout32, sh, t, sv, sc.p, 1, 1, part.p, nb);
         } else {
             hipLaunchKernelGGL(hk_s<cxd>, g, blk, 0, s, A, r, v, sv, F.b, F.x, sc.p, 0);
             vcycle<cxd>(L64, s);
-            hipLaunchKernelGGL(hk_apply<cxd>, g, blk, 0, s, A, F.x, sh, t, sv, sc.p, 0, 1, part.p, nb);
+            hipLaunchKernelGGL(hk_apply<cxd>, g, blk, 0, s, A, (const cxd*)vout64, sh, t, sv, sc.p, 0, 1, part.p, nb);
         }
         hipLaunchKernelGGL(hk_finish, dim3(1), dim3(256), 0, s, part.p, nb, sc.p, 3);
         hipLaunchKernelGGL(hk_x, g, blk, 0, s, A, bsol.p, xa, xb, r, r0, ph, sh, sv, t, sc.p, part.p, nb);
         hipLaunchKernelGGL(hk_finish, dim3(1), dim3(256), 0, s, part.p, nb, sc.p, 4);
         CATHODE_LAUNCH_CHECK();
         std::swap(xa, xb);
-        CATHODE_HIP(hipMemcpyAsync(h_sc, sc.p, HS_COUNT * sizeof(double), hipMemcpyDeviceToHost, s));
+        fetch_host(h_sc, sc.p, HS_COUNT, s);
         CATHODE_HIP(hipStreamSynchronize(s));
         const double bb = h_sc[HS_BB];
         if (!(bb > 0.0)) {
@@ -699,7 +702,7 @@ void HelmSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_
     }
     if (maxITR <= 0) {
         // no iteration: x = 0, relative residual 1 (0 when b = 0)
-        CATHODE_HIP(hipMemcpyAsync(h_sc, sc.p, HS_COUNT * sizeof(double), hipMemcpyDeviceToHost, s));
+        fetch_host(h_sc, sc.p, HS_COUNT, s);
         CATHODE_HIP(hipStreamSynchronize(s));
         rel = h_sc[HS_BB] > 0.0 ? 1.0 : 0.0;
     }

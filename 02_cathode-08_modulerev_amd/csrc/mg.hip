@@ -592,6 +592,7 @@ __global__ void k_bicg_finish(const double* __restrict__ part, int nb, double* _
 // ------------------------------------------------------------------ host side
 MGSolver::~MGSolver()
 {
+    drop_graphs();
     for (auto& e : prof.ev) (void)hipEventDestroy(e);
     if (h_scal) (void)hipHostFree(h_scal);
 }
@@ -602,6 +603,7 @@ void MGSolver::configure(int ni, int nj, hipStream_t s)
 {
     (void)s;
     if (ni == cfg_ni && nj == cfg_nj) return;
+    drop_graphs();  // they hold the old buffers
     cfg_ni = ni;
     cfg_nj = nj;
     for (int q = 0; q < 2; q++) {
@@ -628,12 +630,12 @@ void MGSolver::configure(int ni, int nj, hipStream_t s)
         tot += (size_t)(dims[l][0] + 2) * pad_ld(dims[l][1]);
     }
     level_off[nl] = tot;
-    store64.alloc(7 * tot);
-    store32.alloc(7 * tot);
+    store64.alloc(8 * tot);
+    store32.alloc(8 * tot);
     mass.alloc(tot);
     actall.alloc(tot);
-    CATHODE_HIP(hipMemset(store64.p, 0, 7 * tot * sizeof(double)));
-    CATHODE_HIP(hipMemset(store32.p, 0, 7 * tot * sizeof(float)));
+    CATHODE_HIP(hipMemset(store64.p, 0, 8 * tot * sizeof(double)));
+    CATHODE_HIP(hipMemset(store32.p, 0, 8 * tot * sizeof(float)));
     CATHODE_HIP(hipMemset(mass.p, 0, tot * sizeof(double)));
     CATHODE_HIP(hipMemset(actall.p, 0, tot));
     for (int l = 0; l < nl; l++) {
@@ -647,11 +649,13 @@ void MGSolver::configure(int ni, int nj, hipStream_t s)
         A.cy = B.cy = (l + 1 < nl) && dims[l][1] > 1;
         double* p = store64.p;
         float* q = store32.p;
-        // b and x of level l; the five coefficient arrays are placed below
+        // b, x and xu of level l; the five coefficient arrays are placed below
         A.b = p + 5 * tot + o;
         A.x = p + 6 * tot + o;
+        A.xu = p + 7 * tot + o;
         B.b = q + 5 * tot + o;
         B.x = q + 6 * tot + o;
+        B.xu = q + 7 * tot + o;
         (void)sz;
     }
     // the coefficient block of level l starts at 5*level_off[l], so that the
@@ -802,7 +806,7 @@ void MGSolver::vcycle(Level<T>* L, hipStream_t s, const VFuse& fdown, const VFus
     for (int l = 0; l < lt; l++) {
         Level<T>& A = L[l];
         Level<T>& C = L[l + 1];
-        VLevel<T> V{A.ni, A.nj, A.ld, A.aP, A.aE, A.aW, A.aN, A.aS, actall.p + level_off[l], A.b, A.x};
+        VLevel<T> V{A.ni, A.nj, A.ld, A.aP, A.aE, A.aW, A.aN, A.aS, actall.p + level_off[l], A.b, A.x, nullptr};
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (prof.enabled && l == 0) prof_events(e0, e1);
         launch_vtile<T, true>(tile_of<T>(l), l == 0, s, V, A.cx ? 2 : 1, A.cy ? 2 : 1, w, C.ni, C.nj, C.ld, C.b,
@@ -841,9 +845,10 @@ void MGSolver::vcycle(Level<T>* L, hipStream_t s, const VFuse& fdown, const VFus
     for (int l = lt - 1; l >= 0; l--) {
         Level<T>& A = L[l];
         Level<T>& C = L[l + 1];
-        VLevel<T> V{A.ni, A.nj, A.ld, A.aP, A.aE, A.aW, A.aN, A.aS, actall.p + level_off[l], A.b, A.x};
+        // the coarse correction: the next level's final x (the tail writes x)
+        VLevel<T> V{A.ni, A.nj, A.ld, A.aP, A.aE, A.aW, A.aN, A.aS, actall.p + level_off[l], A.b, A.x, A.xu};
         launch_vtile<T, false>(tile_of<T>(l), l == 0, s, V, A.cx ? 2 : 1, A.cy ? 2 : 1, (const double*)nullptr, C.ni,
-                               C.nj, C.ld, C.x, l == 0 ? fup : VFuse());
+                               C.nj, C.ld, l + 1 < lt ? C.xu : C.x, l == 0 ? fup : VFuse());
     }
     CATHODE_LAUNCH_CHECK();
 }
@@ -858,7 +863,7 @@ void MGSolver::prof_events(hipEvent_t& a, hipEvent_t& b)
 
 double MGSolver::read_scalar(const double* d, hipStream_t s)
 {
-    CATHODE_HIP(hipMemcpyAsync(h_scal, d, sizeof(double), hipMemcpyDeviceToHost, s));
+    fetch_host(h_scal, d, 1, s);
     CATHODE_HIP(hipStreamSynchronize(s));
     return h_scal[0];
 }
@@ -883,7 +888,7 @@ float MGSolver::time_tile(Level<T>* L, int l, int cfg, hipStream_t s)
     Level<T>& A = L[l];
     Level<T>& C = L[l + 1];
     const double* w = user_w ? user_w : w0.p;
-    VLevel<T> V{A.ni, A.nj, A.ld, A.aP, A.aE, A.aW, A.aN, A.aS, actall.p + level_off[l], A.b, A.x};
+    VLevel<T> V{A.ni, A.nj, A.ld, A.aP, A.aE, A.aW, A.aN, A.aS, actall.p + level_off[l], A.b, A.x, A.xu};
     hipEvent_t e0, e1;
     CATHODE_HIP(hipEventCreate(&e0));
     CATHODE_HIP(hipEventCreate(&e1));
@@ -964,6 +969,72 @@ void MGSolver::lump_nine(hipStream_t s)
     CATHODE_LAUNCH_CHECK();
 }
 
+void MGSolver::drop_graphs()
+{
+    for (auto& gr : graphs)
+        if (gr.exec) (void)hipGraphExecDestroy(gr.exec);
+    graphs.clear();
+}
+
+const MGSolver::IterGraph* MGSolver::find_graph(int mixed, int parity, const double* xa, const double* pa) const
+{
+    const double* w = user_w ? user_w : w0.p;
+    for (const auto& gr : graphs) {
+        if (gr.mixed != mixed || gr.parity != parity || gr.xa != xa || gr.pa != pa || gr.w != w) continue;
+        if (std::memcmp(gr.tiles, tiles[mixed], sizeof(gr.tiles)) != 0) continue;
+        return &gr;
+    }
+    return nullptr;
+}
+
+// Captures what enqueue launches on s into an executable graph.  A failed
+// capture or instantiation leaves the stream usable and returns nullptr (the
+// caller then launches eagerly); a failing launch inside is rethrown after
+// the capture has been ended.
+template <class Fn>
+const MGSolver::IterGraph* MGSolver::capture_graph(int mixed, int parity, const double* xa, const double* pa,
+                                                   hipStream_t s, Fn&& enqueue)
+{
+    if (graphs.size() >= 8) drop_graphs();
+    if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    bool failed = false;
+    hip_failure err{hipSuccess};
+    try {
+        enqueue();
+    } catch (const hip_failure& f) {
+        failed = true;
+        err = f;
+    }
+    hipGraph_t gr = nullptr;
+    const hipError_t e = hipStreamEndCapture(s, &gr);
+    if (failed || e != hipSuccess || !gr) {
+        if (gr) (void)hipGraphDestroy(gr);
+        (void)hipGetLastError();
+        if (failed) throw err;
+        return nullptr;
+    }
+    hipGraphExec_t ex = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    if (ei != hipSuccess || !ex) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    IterGraph ig;
+    ig.exec = ex;
+    ig.mixed = mixed;
+    ig.parity = parity;
+    ig.xa = xa;
+    ig.pa = pa;
+    ig.w = user_w ? user_w : w0.p;
+    std::memcpy(ig.tiles, tiles[mixed], sizeof(ig.tiles));
+    graphs.push_back(ig);
+    return &graphs.back();
+}
+
 int MGSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_status* st)
 {
     auto t0 = std::chrono::steady_clock::now();
@@ -1028,6 +1099,58 @@ int MGSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_sta
     // partials of the restart and x-update kernels: right after the scalar
     // slots, so one copy brings both to the host
     double* part3 = scal.p + SC_COUNT;
+    // One fused iteration: p and the V-cycle input in the finest down kernel
+    // (rho1 from the x-update / restart partials); ph, v = A ph and (r0, v)
+    // partials in the finest up kernel; s = r - al v (al from those
+    // partials) and the second V-cycle; sh, t = A sh, (t, s), (t, t); then
+    // the x update, which takes om from the last partials.  rho1 alternates
+    // between two slots, so a down kernel reads the previous one while
+    // workgroup 0 stores the new.  Everything it launches depends only on
+    // (mode, iteration parity, x / p buffers), so it can be captured.
+    auto fused_iter = [&](int first, double* xa_, double* xb_, double* pa_, double* pb_) {
+        const int s_new = (it & 1) ? SC_RHO1 : SC_RHO, s_prev = (it & 1) ? SC_RHO : SC_RHO1;
+        VFuse fd = fbase, fu = fbase;
+        fd.pro = 1;
+        fd.first = first;
+        fd.pin = pa_;
+        fd.ps = pb_;
+        fd.pred = part3 + nb;
+        fd.npred = nb;
+        fd.s_rho1 = s_new;
+        fd.s_rho = s_prev;
+        fu.epi = 1;
+        fu.y = ph;
+        fu.out = v;
+        fu.dotvec = r0;
+        fu.want_tt = 0;
+        if (mixed) vcycle<float>(L32, s, fd, fu);
+        else vcycle<double>(L64, s, fd, fu);
+        fd.pro = 2;
+        fd.first = 0;
+        fd.pin = nullptr;
+        fd.ps = sv;
+        fd.pred = part.p;
+        fd.npred = ntiles;
+        fd.s_r0v = SC_R0V;
+        fu.y = sh;
+        fu.out = t;
+        fu.dotvec = sv;
+        fu.want_tt = 1;
+        if (mixed) vcycle<float>(L32, s, fd, fu);
+        else vcycle<double>(L64, s, fd, fu);
+        hipLaunchKernelGGL(k_bicg_x, g, blk, 0, s, F.ni, F.nj, F.ld, F.aP, F.aE, F.aW, F.aN, F.aS, n9, n0, bsol.p,
+                           act0.p, xa_, xb_, r, r0, ph, sh, sv, t, scal.p, part3, nb, (const double*)part.p, ntiles);
+        CATHODE_LAUNCH_CHECK();
+    };
+    // V-cycle output on level 0: the finest up kernel's xu (x when the
+    // whole hierarchy runs in the tail)
+    const float* vout32 = lt > 0 ? L32[0].xu : L32[0].x;
+    const double* vout64 = lt > 0 ? F.xu : F.x;
+    // graph replay of the later iterations: CATHODE_GRAPHS=1 (off by
+    // default: on ROCm 7 one hipGraphLaunch of the ~14 nodes costs more than
+    // launching them one by one)
+    const char* genv = std::getenv("CATHODE_GRAPHS");
+    const bool graphs_on = fused && !prof.enabled && genv && genv[0] == '1';
     while (it < maxITR) {
         if (restart) {
             hipLaunchKernelGGL(k_bicg_restart, g, blk, 0, s, F.ni, F.nj, F.ld, F.aP, F.aE, F.aW, F.aN, F.aS, n9,
@@ -1038,44 +1161,18 @@ int MGSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_sta
         }
         it++;
         if (fused) {
-            // p and the V-cycle input in the finest down kernel (rho1 from the
-            // x-update / restart partials); ph, v = A ph and (r0, v) partials in
-            // the finest up kernel.  rho1 alternates between two slots, so the
-            // down kernel reads the previous one while workgroup 0 stores the new.
-            const int s_new = (it & 1) ? SC_RHO1 : SC_RHO, s_prev = (it & 1) ? SC_RHO : SC_RHO1;
-            VFuse fd = fbase, fu = fbase;
-            fd.pro = 1;
-            fd.first = restart ? 1 : 0;
-            fd.pin = pa;
-            fd.ps = pb;
-            fd.pred = part3 + nb;
-            fd.npred = nb;
-            fd.s_rho1 = s_new;
-            fd.s_rho = s_prev;
-            fu.epi = 1;
-            fu.y = ph;
-            fu.out = v;
-            fu.dotvec = r0;
-            fu.want_tt = 0;
-            if (mixed) vcycle<float>(L32, s, fd, fu);
-            else vcycle<double>(L64, s, fd, fu);
+            const int first = restart ? 1 : 0;
+            if (!first && graphs_on) {
+                // later iterations replay a captured copy of the same launches
+                const IterGraph* gr = find_graph(mixed, it & 1, xa, pa);
+                if (!gr) gr = capture_graph(mixed, it & 1, xa, pa, s, [&] { fused_iter(first, xa, xb, pa, pb); });
+                if (gr) CATHODE_HIP(hipGraphLaunch(gr->exec, s));
+                else fused_iter(first, xa, xb, pa, pb);
+            } else {
+                fused_iter(first, xa, xb, pa, pb);
+            }
             std::swap(pa, pb);
             restart = false;
-            // s = r - al v (al from the (r0, v) partials) and the V-cycle
-            // input; sh, t = A sh, (t, s) and (t, t) partials
-            fd.pro = 2;
-            fd.first = 0;
-            fd.pin = nullptr;
-            fd.ps = sv;
-            fd.pred = part.p;
-            fd.npred = ntiles;
-            fd.s_r0v = SC_R0V;
-            fu.y = sh;
-            fu.out = t;
-            fu.dotvec = sv;
-            fu.want_tt = 1;
-            if (mixed) vcycle<float>(L32, s, fd, fu);
-            else vcycle<double>(L64, s, fd, fu);
         } else {
         // p, ph = M^-1 p, v = A ph, al
         if (mixed) {
@@ -1083,13 +1180,13 @@ int MGSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_sta
                                L32[0].x, scal.p, restart ? 1 : 0, 1);
             vcycle<float>(L32, s);
             hipLaunchKernelGGL(k_bicg_apply<float>, g, blk, 0, s, F.ni, F.nj, F.ld, F.aP, F.aE, F.aW, F.aN,
-                               F.aS, n9, n0, act0.p, L32[0].x, ph, v, r0, scal.p, 1, 0, part.p, nb);
+                               F.aS, n9, n0, act0.p, vout32, ph, v, r0, scal.p, 1, 0, part.p, nb);
         } else {
             hipLaunchKernelGGL(k_bicg_p<double>, g, blk, 0, s, F.ni, F.nj, F.ld, act0.p, r, v, pa, F.b, F.x,
                                scal.p, restart ? 1 : 0, 0);
             vcycle<double>(L64, s);
             hipLaunchKernelGGL(k_bicg_apply<double>, g, blk, 0, s, F.ni, F.nj, F.ld, F.aP, F.aE, F.aW, F.aN,
-                               F.aS, n9, n0, act0.p, F.x, ph, v, r0, scal.p, 0, 0, part.p, nb);
+                               F.aS, n9, n0, act0.p, vout64, ph, v, r0, scal.p, 0, 0, part.p, nb);
         }
         hipLaunchKernelGGL(k_bicg_finish, dim3(1), dim3(256), 0, s, part.p, nb, scal.p, 2, 0.0);
         restart = false;
@@ -1099,27 +1196,25 @@ int MGSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_sta
                                L32[0].x, scal.p, 1);
             vcycle<float>(L32, s);
             hipLaunchKernelGGL(k_bicg_apply<float>, g, blk, 0, s, F.ni, F.nj, F.ld, F.aP, F.aE, F.aW, F.aN,
-                               F.aS, n9, n0, act0.p, L32[0].x, sh, t, sv, scal.p, 1, 1, part.p, nb);
+                               F.aS, n9, n0, act0.p, vout32, sh, t, sv, scal.p, 1, 1, part.p, nb);
         } else {
             hipLaunchKernelGGL(k_bicg_s<double>, g, blk, 0, s, F.ni, F.nj, F.ld, act0.p, r, v, sv, F.b, F.x,
                                scal.p, 0);
             vcycle<double>(L64, s);
             hipLaunchKernelGGL(k_bicg_apply<double>, g, blk, 0, s, F.ni, F.nj, F.ld, F.aP, F.aE, F.aW, F.aN,
-                               F.aS, n9, n0, act0.p, F.x, sh, t, sv, scal.p, 0, 1, part.p, nb);
+                               F.aS, n9, n0, act0.p, vout64, sh, t, sv, scal.p, 0, 1, part.p, nb);
         }
         hipLaunchKernelGGL(k_bicg_finish, dim3(1), dim3(256), 0, s, part.p, nb, scal.p, 3, 0.0);
-        }
-        // x, recursive and true residuals (fused: om from the up kernel's partials)
+        // x, recursive and true residuals
         hipLaunchKernelGGL(k_bicg_x, g, blk, 0, s, F.ni, F.nj, F.ld, F.aP, F.aE, F.aW, F.aN, F.aS, n9, n0, bsol.p,
-                           act0.p, xa, xb, r, r0, ph, sh, sv, t, scal.p, part3, nb,
-                           fused ? (const double*)part.p : (const double*)nullptr, ntiles);
-        if (!fused) hipLaunchKernelGGL(k_bicg_finish, dim3(1), dim3(256), 0, s, part3, nb, scal.p, 4, 0.0);
+                           act0.p, xa, xb, r, r0, ph, sh, sv, t, scal.p, part3, nb, (const double*)nullptr, 0);
+        hipLaunchKernelGGL(k_bicg_finish, dim3(1), dim3(256), 0, s, part3, nb, scal.p, 4, 0.0);
+        }
         CATHODE_LAUNCH_CHECK();
         std::swap(xa, xb);
         // one host round trip per iteration: the scalars and the x-update
         // partials; |r|^2 and |b - A x|^2 summed here in a fixed order
-        CATHODE_HIP(hipMemcpyAsync(h_scal, scal.p, (SC_COUNT + 3 * (size_t)nb) * sizeof(double),
-                                   hipMemcpyDeviceToHost, s));
+        fetch_host(h_scal, scal.p, SC_COUNT + 3 * (size_t)nb, s);
         CATHODE_HIP(hipStreamSynchronize(s));
         {
             double rr = 0.0, rt2 = 0.0;
@@ -1153,7 +1248,7 @@ int MGSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_sta
     if (maxITR <= 0) {
         // no iteration: x = 0, relative residual 1 (0 when b = 0)
         if (zero_guess) CATHODE_HIP(hipMemsetAsync(xsol.p, 0, n0 * sizeof(double), s));
-        CATHODE_HIP(hipMemcpyAsync(h_scal, scal.p, SC_COUNT * sizeof(double), hipMemcpyDeviceToHost, s));
+        fetch_host(h_scal, scal.p, SC_COUNT, s);
         CATHODE_HIP(hipStreamSynchronize(s));
         rel = h_scal[SC_BB] > 0.0 ? 1.0 : 0.0;
     }
@@ -1206,6 +1301,18 @@ __global__ void k_reduce_partials(const double* part, int nblocks, int nvec, con
         double t = mx ? block_max<256>(s, lds) : block_sum<256>(s, lds);
         if (threadIdx.x == 0) out[v] = t;
     }
+}
+
+__global__ void k_fetch_host(const double* __restrict__ d, size_t n, double* __restrict__ h)
+{
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x) h[k] = d[k];
+}
+
+void fetch_host(double* h, const double* d, size_t n, hipStream_t s)
+{
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_fetch_host, dim3(std::min(cdiv((long)n, 256), 16)), dim3(256), 0, s, d, n, h);
+    CATHODE_LAUNCH_CHECK();
 }
 
 void reduce_partials(const double* part, int nblocks, int nvec, const int* d_is_max, double* out,

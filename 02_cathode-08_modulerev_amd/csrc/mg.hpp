@@ -39,6 +39,7 @@ struct Level {
     int cx = 0, cy = 0;  // coarsen to the next level in i / j
     T *aP = nullptr, *aE = nullptr, *aW = nullptr, *aN = nullptr, *aS = nullptr;
     T *b = nullptr, *x = nullptr;
+    T* xu = nullptr;  // up kernel output (the level's corrected, post-smoothed x)
 };
 
 struct MGProfile {
@@ -147,6 +148,20 @@ private:
     DevBuf<double> nine;       // nine-point couplings (enable_nine)
     bool use_nine = false;
     int tiles[2][MG_MAX_LEVELS] = {};
+    // captured fused iterations (graph replays of the same launches), keyed
+    // by everything their arguments depend on
+    struct IterGraph {
+        hipGraphExec_t exec = nullptr;
+        int mixed = 0, parity = 0;
+        const double *xa = nullptr, *pa = nullptr, *w = nullptr;
+        int tiles[MG_MAX_LEVELS] = {};
+    };
+    std::vector<IterGraph> graphs;
+    const IterGraph* find_graph(int mixed, int parity, const double* xa, const double* pa) const;
+    template <class Fn>
+    const IterGraph* capture_graph(int mixed, int parity, const double* xa, const double* pa, hipStream_t s,
+                                   Fn&& enqueue);
+    void drop_graphs();
     bool tuned[2] = {false, false};
     bool autotune_on = true;
     int lt = 0;                // first level handled by the one-workgroup tail kernel

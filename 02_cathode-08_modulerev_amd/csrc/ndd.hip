@@ -880,7 +880,7 @@ static void d2d(NddHandle& H, std::initializer_list<std::pair<double*, const dou
 
 static double read_dev(NddHandle& H, const double* d, int count = 1)
 {
-    CATHODE_HIP(hipMemcpyAsync(H.h_scal, d, count * sizeof(double), hipMemcpyDeviceToHost, H.s));
+    fetch_host(H.h_scal, d, count, H.s);
     CATHODE_HIP(hipStreamSynchronize(H.s));
     return H.h_scal[0];
 }
@@ -926,7 +926,7 @@ static double calcres3(NddHandle& H, const ResPairs& P, double* err_out)
     hipLaunchKernelGGL(k_calcres, ggrid(V.ni, V.nj), gblk(), 0, H.s, V, P, H.p.flag_error, 1e-100, H.part.p, H.nb);
     hipLaunchKernelGGL(k_calcres_finish, dim3(1), dim3(256), 0, H.s, H.part.p, H.nb, H.p.flag_error, H.scal.p);
     CATHODE_LAUNCH_CHECK();
-    CATHODE_HIP(hipMemcpyAsync(H.h_scal, H.scal.p, 4 * sizeof(double), hipMemcpyDeviceToHost, H.s));
+    fetch_host(H.h_scal, H.scal.p, 4, H.s);
     CATHODE_HIP(hipStreamSynchronize(H.s));
     if (err_out)
         for (int q = 0; q < 3; q++) err_out[q] = H.h_scal[q];
@@ -1309,7 +1309,7 @@ static int ndd_run(NddHandle* H, int nsteps, cathode_ndd_report* rep)
         if (rep && nsteps > 0) {
             hipLaunchKernelGGL(k_anode, dim3(1), dim3(256), 0, H->s, H->V, H->F[F_RHOUIX], H->F[F_RHOUEX], H->scal.p + 12);
             CATHODE_LAUNCH_CHECK();
-            CATHODE_HIP(hipMemcpyAsync(H->h_scal, H->scal.p + 12, 2 * sizeof(double), hipMemcpyDeviceToHost, H->s));
+            fetch_host(H->h_scal, H->scal.p + 12, 2, H->s);
         }
         CATHODE_HIP(hipStreamSynchronize(H->s));
         if (rep) {

@@ -50,12 +50,18 @@ __device__ inline void xcd_tile(int& ti, int& tj)
     tj = t - ti * (int)gridDim.x;
 }
 
+// b, x: the level's right-hand side and the down kernel's output (the
+// pre-smoothed x).  xo: where the up kernel writes the corrected,
+// post-smoothed x.  The up kernel reads x over its whole halo region, so it
+// must never write x itself: a neighbouring tile that starts later would
+// read post-smoothed halo values instead of the pre-smoothed ones the tile
+// decomposition is exact for.
 template <class T>
 struct VLevel {
     int ni, nj, ld;
     const T *aP, *aE, *aW, *aN, *aS;
     const unsigned char* act;
-    T *b, *x;
+    T *b, *x, *xo;
 };
 
 // ------------------------------------------------------------------ fine-level tiles
@@ -181,7 +187,7 @@ __global__ void __launch_bounds__(NT)
         if (!(flags[k] & 8u)) continue;
         const int ri = c / RJ, rj = c - ri * RJ;
         const int gi = gi0 + ri, gj = gj0 + rj;
-        F.x[(size_t)gi * F.ld + gj] = xs[c];
+        (DOWN ? F.x : F.xo)[(size_t)gi * F.ld + gj] = xs[c];
         if (DOWN) {
             T r = T(0);
             if (flags[k] & 1u) {

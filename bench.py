@@ -178,7 +178,7 @@ def main():
     ap.add_argument("--sweep", action="store_true",
                     help="also run the 64-instance bias sweep (batched vs one at a time)")
     ap.add_argument("--sweep-steps", type=int, default=50)
-    ap.add_argument("--sweep-workers", type=int, default=4)
+    ap.add_argument("--sweep-workers", type=int, default=8)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run the copy kernel used to calibrate the HBM traffic counters")
     args = ap.parse_args()
