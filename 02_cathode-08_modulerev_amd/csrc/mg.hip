@@ -173,7 +173,110 @@ struct CoarsenChain {
     CoarsenArgs L[CHAIN_MAX];
 };
 
-__global__ void k_coarsen_chain(CoarsenChain c, const double* __restrict__ sc, int mixed)
+// the coarsest level, for its dense inverse
+struct CoarseInv {
+    int ni, nj, ld;
+    const double *aP, *aE, *aW, *aN, *aS;
+    const unsigned char* act;
+    double* inv64;  // n x n, row-major
+    float* inv32;   // the same divided by s32 (the FP32 levels hold s32 A)
+};
+
+__device__ inline double readlane_f64(double v, int lane)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Dense inverse of the coarsest level (n = ni nj <= MG_COARSEST_MAX) by
+// Gauss-Jordan elimination without pivoting on the system padded to NM =
+// MG_COARSEST_MAX unknowns (rows and columns n.. are the identity, so the
+// leading n x n block of the result is A^-1), run by the 8 wavefronts of a
+// 512-thread workgroup.  Lane t holds column t of [A | I] (t < NM: A, t >=
+// NM: I); wavefront w holds rows 4w .. 4w+3 of it in registers.  Step k: the
+// wavefront owning row k normalises it and publishes it in LDS (two buffers,
+// so one barrier per step); every row i != k subtracts A[i][k] times it,
+// A[i][k] coming from lane k of the same wavefront by readlane.  With the
+// fixed size every loop unrolls without branches.  The coarse operators are
+// diagonally dominant with non-positive couplings (M-matrices), for which
+// elimination without pivoting is stable.  A pivot at or below 1e-13
+// max|aP| (a singular operator, no Dirichlet part) zeroes its row, so the
+// coarse solve returns a particular solution.
+constexpr int CINV_NT = 512;
+__device__ __forceinline__ void coarse_inverse_block(const CoarseInv& c, double s32, double* lds_rows,
+                                                     double* lds_red)
+{
+    constexpr int NM = MG_COARSEST_MAX;
+    constexpr int RPW = NM / (CINV_NT / 64);  // rows per wavefront
+    const int t = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
+    const int n = c.ni * c.nj, nj = c.nj;
+    double m[RPW];
+    double amax = 0.0;
+    // rows 4w+r of column t: row q's diagonal, or its coupling to t when t is
+    // q's neighbour (E of q is q + nj, W q - nj, N q + 1, S q - 1)
+#pragma unroll
+    for (int r = 0; r < RPW; r++) {
+        const int q = w * RPW + r;
+        double v = 0.0;
+        if (t < NM) {
+            if (q < n && t < n) {
+                const int iq = q / nj, jq = q % nj;
+                const size_t g = (size_t)(1 + iq) * c.ld + 1 + jq;
+                const bool a = c.act[g] != 0;
+                if (q == t) {
+                    v = a ? c.aP[g] : 1.0;
+                    amax = fabs(v);
+                } else if (a) {
+                    if (t == q + nj && iq + 1 < c.ni) v = -c.aE[g];
+                    else if (t == q - nj && iq > 0) v = -c.aW[g];
+                    else if (t == q + 1 && jq + 1 < nj) v = -c.aN[g];
+                    else if (t == q - 1 && jq > 0) v = -c.aS[g];
+                }
+            } else if (q == t) {
+                v = 1.0;  // padding
+            }
+        } else {
+            v = (t - NM == q) ? 1.0 : 0.0;
+        }
+        m[r] = v;
+    }
+    const double tiny = 1e-13 * block_max_bcast<CINV_NT>(amax, lds_red);
+#pragma unroll
+    for (int k = 0; k < NM; k++) {
+        const int wk = k / RPW, rk = k % RPW;
+        double* row = lds_rows + (k & 1) * 64;
+        if (w == wk) {
+            const double pv = readlane_f64(m[rk], k);
+            const double rcp = (fabs(pv) > tiny && isfinite(pv)) ? 1.0 / pv : 0.0;
+            m[rk] = m[rk] * rcp;
+            row[t] = m[rk];
+        }
+        __syncthreads();
+        const double rj = row[t];
+#pragma unroll
+        for (int r = 0; r < RPW; r++) {
+            const double a = readlane_f64(m[r], k);  // A[4w+r][k], before this step's update
+            const double f = (w == wk && r == rk) ? 0.0 : a;
+            m[r] = fma(-f, rj, m[r]);
+        }
+    }
+    if (t >= NM && t - NM < n) {
+        const int j = t - NM;
+#pragma unroll
+        for (int r = 0; r < RPW; r++) {
+            const int q = w * RPW + r;
+            if (q < n) {
+                c.inv64[q * n + j] = m[r];
+                c.inv32[q * n + j] = (float)(m[r] / s32);
+            }
+        }
+    }
+}
+
+__global__ void __launch_bounds__(512) k_coarsen_chain(CoarsenChain c, const double* __restrict__ sc, int mixed,
+                                                      CoarseInv ci)
 {
     const double s32 = mixed ? sc[3] : 1.0;
     for (int l = 0; l < c.n; l++) {
@@ -182,6 +285,20 @@ __global__ void k_coarsen_chain(CoarsenChain c, const double* __restrict__ sc, i
             coarsen_cell(a, 1 + q / a.cnj, 1 + q % a.cnj, s32, mixed);
         __syncthreads();
     }
+    // the chain that built the coarsest level also inverts it (all 512 threads)
+    if (ci.inv64) {
+        __shared__ double rows[2 * 64];
+        __shared__ double red[CINV_NT / 64];
+        coarse_inverse_block(ci, sc[3], rows, red);
+    }
+}
+
+// the coarsest level's inverse when no chain builds it (it is a fine level)
+__global__ void __launch_bounds__(CINV_NT) k_coarse_inverse(CoarseInv ci, const double* __restrict__ sc)
+{
+    __shared__ double rows[2 * 64];
+    __shared__ double red[CINV_NT / 64];
+    coarse_inverse_block(ci, sc[3], rows, red);
 }
 
 // The setup after k_l0_prepare in one launch: the setup scalars (every
@@ -618,7 +735,7 @@ void MGSolver::configure(int ni, int nj, hipStream_t s)
     n = 1;
     while (n < MG_MAX_LEVELS) {
         int a = dims[n - 1][0], b = dims[n - 1][1];
-        if ((long)a * b <= COARSEST_MAX || (a == 1 && b == 1)) break;
+        if ((long)a * b <= MG_COARSEST_MAX || (a == 1 && b == 1)) break;
         dims[n][0] = a > 1 ? (a + 1) / 2 : a;
         dims[n][1] = b > 1 ? (b + 1) / 2 : b;
         n++;
@@ -701,6 +818,8 @@ void MGSolver::configure(int ni, int nj, hipStream_t s)
     CATHODE_HIP(hipMemset(scal.p, 0, nsc * sizeof(double)));
     kv.alloc(9 * n0);
     CATHODE_HIP(hipMemset(kv.p, 0, 8 * n0 * sizeof(double)));
+    cinv64.alloc(MG_COARSEST_MAX * MG_COARSEST_MAX);
+    cinv32.alloc(MG_COARSEST_MAX * MG_COARSEST_MAX);
     if (h_scal_n < nsc) {
         if (h_scal) CATHODE_HIP(hipHostFree(h_scal));
         h_scal = nullptr;
@@ -789,12 +908,27 @@ void MGSolver::setup(int mode, hipStream_t s)
         hipLaunchKernelGGL(k_coarsen, grid2(C.ni, C.nj), dim3(BX, BY), 0, s, coarsen_args(l, w), (const double*)scal.p,
                            mixed);
     }
+    CoarseInv ci{};
+    {
+        const Level<double>& Z = L64[nl - 1];
+        ci.ni = Z.ni; ci.nj = Z.nj; ci.ld = Z.ld;
+        ci.aP = Z.aP; ci.aE = Z.aE; ci.aW = Z.aW; ci.aN = Z.aN; ci.aS = Z.aS;
+        ci.act = actall.p + level_off[nl - 1];
+        ci.inv64 = cinv64.p;
+        ci.inv32 = cinv32.p;
+    }
+    bool inverted = false;
     for (int l0 = lt; l0 + 1 < nl; l0 += CHAIN_MAX) {
         CoarsenChain ch;
         ch.n = 0;
         for (int l = l0; l + 1 < nl && ch.n < CHAIN_MAX; l++) ch.L[ch.n++] = coarsen_args(l, w);
-        hipLaunchKernelGGL(k_coarsen_chain, dim3(1), dim3(512), 0, s, ch, (const double*)scal.p, mixed);
+        const bool last = l0 + ch.n + 1 >= nl;
+        CoarseInv cc = ci;
+        if (!last) cc.inv64 = nullptr;
+        hipLaunchKernelGGL(k_coarsen_chain, dim3(1), dim3(512), 0, s, ch, (const double*)scal.p, mixed, cc);
+        inverted = inverted || last;
     }
+    if (!inverted) hipLaunchKernelGGL(k_coarse_inverse, dim3(1), dim3(CINV_NT), 0, s, ci, (const double*)scal.p);
     CATHODE_LAUNCH_CHECK();
 }
 
@@ -836,8 +970,12 @@ void MGSolver::vcycle(Level<T>* L, hipStream_t s, const VFuse& fdown, const VFus
     ta.b_in = L[lt].b;
     ta.x_out = L[lt].x;
     ta.w = lt == 0 ? w : nullptr;
+    ta.cn = L[nl - 1].ni * L[nl - 1].nj;
+    if constexpr (sizeof(T) == 4) ta.cinv = (const T*)cinv32.p;
+    else ta.cinv = (const T*)cinv64.p;
     const int top = L[lt].ni * L[lt].nj;
-    const size_t lds = ((size_t)off * 2 + std::max(top, TAIL_NT)) * sizeof(T) + 16;
+    const size_t lds =
+        ((size_t)off * 2 + std::max(top, TAIL_NT) + (size_t)CINV_LD * MG_COARSEST_MAX) * sizeof(T) + 16;
     if (top <= TAIL_NT) launch_tail<T, 1>(ta, lds, s);
     else if (top <= 2 * TAIL_NT) launch_tail<T, 2>(ta, lds, s);
     else launch_tail<T, 4>(ta, lds, s);

@@ -13,7 +13,8 @@
 //              part s - m (halved); 5-point on every level;
 //   smoother   red-black point Gauss-Seidel, V(2,2);
 //   transfers  w-weighted sum restriction, piecewise-constant prolongation;
-//   coarsest   a single cell (x = b / aP);
+//   coarsest   the first level of <= 32 cells, solved with its dense
+//              inverse (built at setup by Gauss-Jordan elimination);
 //   kernels    one fused down kernel (2 sweeps + residual + restriction) and
 //              one fused up kernel (prolongation + 2 sweeps) per fine level,
 //              and one workgroup for all levels of <= 512 cells (vcycle.hpp).
@@ -29,6 +30,11 @@
 #include "vfuse.hpp"
 
 namespace cathode {
+
+// real path: the coarsest level has at most this many cells and is solved
+// with its dense inverse (LDS row pitch CINV_LD, no bank conflicts)
+constexpr int MG_COARSEST_MAX = 32;
+constexpr int CINV_LD = MG_COARSEST_MAX + 1;
 
 constexpr int MG_MAX_LEVELS = 24;
 struct CoarsenArgs;
@@ -167,6 +173,8 @@ private:
     int lt = 0;                // first level handled by the one-workgroup tail kernel
     int tail_cells = 0;        // LDS cells of the tail levels
     DevBuf<double> part;       // reduction partials (2 slots)
+    DevBuf<double> cinv64;     // dense inverse of the coarsest level (FP64)
+    DevBuf<float> cinv32;      // the same for the FP32 levels (divided by the FP32 scale)
     DevBuf<double> kv;         // BiCGStab vectors r, r0, p, v, s, t, ph, sh, p (second buffer)
     DevBuf<double> scal;       // device scalars: [0] |b|^2, [1] |r|^2, [2] scale, [3] f32 matrix scale
     double* h_scal = nullptr;  // pinned mirror of scal

@@ -33,7 +33,7 @@ constexpr TileCfg VT_CFGS[VT_NCFG] = {{16, 32, 256}, {16, 32, 512}, {8, 32, 256}
 constexpr int TAIL_MAX_CELLS = 512;     // levels with at most this many cells go to the tail
 constexpr int TAIL_NT = 512;            // threads of the tail workgroup
 constexpr int TAIL_MAX_LEVELS = 16;
-constexpr int COARSEST_MAX = 1;         // cells of the coarsest level (solved by one division)
+constexpr int COARSEST_MAX = 1;         // complex path: coarsest level of one cell (one division)
 
 // XCD-aware tile order.  Workgroups are dealt to the 8 XCDs round-robin
 // (linear id b goes to XCD b % 8), each XCD with its own L2.  Mapping the
@@ -323,6 +323,8 @@ struct TailArgs {
     const T* b_in;                   // right-hand side of the first tail level (global)
     T* x_out;                        // its solution (global)
     const double* w;                 // restriction weights when the tail starts at level 0
+    const T* cinv;                   // k_tail_reg: dense inverse of the coarsest level (cn x cn, row-major)
+    int cn;                          // its cells
 };
 
 // two red-black sweeps (red first) of one tail level; c = this thread's cell
@@ -583,10 +585,13 @@ __global__ void __launch_bounds__(TAIL_NT) k_tail_reg(TailArgs<T> A)
     T* R = B + A.ncell;  // residuals of one level, by cell number (>= max(cells0, TAIL_NT) entries)
     const int q = threadIdx.x;
     const int nt = A.nt;
+    const int rn = max(A.ni[0] * A.nj[0], TAIL_NT);
+    T* CI = R + rn;      // the coarsest level's inverse, row pitch CINV_LD
     for (int c = q; c < A.ncell; c += TAIL_NT) {
         X[c] = T(0);
         B[c] = T(0);
     }
+    for (int e = q; e < A.cn * A.cn; e += TAIL_NT) CI[(e / A.cn) * CINV_LD + e % A.cn] = A.cinv[e];
     __syncthreads();
     TailCells<T, C0> L0;
     TailCells<T, 1> Lk[TAIL_REG_LEVELS - 1];  // tail levels 1 ..
@@ -622,18 +627,17 @@ __global__ void __launch_bounds__(TAIL_NT) k_tail_reg(TailArgs<T> A)
             tail_sync(ncl[l]);
         }
     }
-    // ---- coarsest level: x = b / aP (inactive or zero diagonal: 0)
-    if (nt == 1) {
-#pragma unroll
-        for (int k = 0; k < C0; k++)
-            if (L0.c[k] >= 0)
-                X[L0.c[k]] = ((L0.fl[k] & 1u) && !(L0.P[k] == T(0))) ? B[L0.c[k]] / L0.P[k] : T(0);
-    }
-#pragma unroll
-    for (int l = 1; l < TAIL_REG_LEVELS; l++) {
-        const TailCells<T, 1>& L = Lk[l - 1];
-        if (l == nt - 1 && L.c[0] >= 0)
-            X[L.c[0]] = ((L.fl[0] & 1u) && !(L.P[0] == T(0))) ? B[L.c[0]] / L.P[0] : T(0);
+    // ---- coarsest level (<= MG_COARSEST_MAX cells): x = A^-1 b with the
+    // inverse built at setup; thread q < cn forms row q
+    {
+        const int l = nt - 1;
+        const int nj = A.nj[l], ld = nj + 2, o = A.off[l], cn = A.cn;
+        if (q < cn) {
+            T s = T(0);
+            for (int t = 0; t < cn; t++) s += CI[q * CINV_LD + t] * B[o + (1 + t / nj) * ld + 1 + t % nj];
+            X[o + (1 + q / nj) * ld + 1 + q % nj] = s;
+        }
+        if (nt == 1) __syncthreads();  // the output below reads cells other threads solved
     }
     // ---- up
 #pragma unroll

@@ -20,7 +20,9 @@
  *   smoother   red-black point Gauss-Seidel, nu1 = nu2 = 2 sweeps;
  *   transfers  restriction = w-weighted sum of the children's residuals,
  *              prolongation = piecewise constant;
- *   coarsest   a single cell (dense LU of one unknown);
+ *   coarsest   the first level of <= 32 cells, solved by dense LU with
+ *              partial pivoting (the device inverts it by Gauss-Jordan
+ *              elimination without pivoting: the same solution to rounding);
  *   outer      BiCGStab right-preconditioned by one V-cycle per application,
  *              from x = 0 (solverFuncs.hpp:106-109), stopping when the true
  *              residual ||b-Ax||_2/||b||_2 <= tol or after maxITR iterations.
@@ -35,7 +37,7 @@
 #include "oracle.h"
 
 #define MG_MAX_LEVELS 24
-static int MG_COARSE_MAX = 1;
+static int MG_COARSE_MAX = 32;
 
 typedef struct {
     int ni, nj, ld;
