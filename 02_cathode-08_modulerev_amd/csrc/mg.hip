@@ -802,9 +802,9 @@ void MGSolver::configure(int ni, int nj, hipStream_t s)
     w0.alloc(nj + 2);
     // levels from lt on run inside the one-workgroup tail kernel
     lt = nl - 1;
-    // the tail's first level may hold TAIL_TOP_MAX cells, every further one TAIL_NT
+    // the tail's first level may hold TAIL_TOP_MAX cells, every further one TAILR_NT
     auto cells = [&](int l) { return (long)dims[l][0] * dims[l][1]; };
-    while (lt > 0 && cells(lt - 1) <= TAIL_TOP_MAX && cells(lt) <= TAIL_NT && nl - (lt - 1) <= TAIL_REG_LEVELS) lt--;
+    while (lt > 0 && cells(lt - 1) <= TAIL_TOP_MAX && cells(lt) <= TAILR_NT && nl - (lt - 1) <= TAIL_REG_LEVELS) lt--;
     tail_cells = 0;
     for (int l = lt; l < nl; l++) tail_cells += (dims[l][0] + 2) * (dims[l][1] + 2);
     {
@@ -975,10 +975,9 @@ void MGSolver::vcycle(Level<T>* L, hipStream_t s, const VFuse& fdown, const VFus
     else ta.cinv = (const T*)cinv64.p;
     const int top = L[lt].ni * L[lt].nj;
     const size_t lds =
-        ((size_t)off * 2 + std::max(top, TAIL_NT) + (size_t)CINV_LD * MG_COARSEST_MAX) * sizeof(T) + 16;
-    if (top <= TAIL_NT) launch_tail<T, 1>(ta, lds, s);
-    else if (top <= 2 * TAIL_NT) launch_tail<T, 2>(ta, lds, s);
-    else launch_tail<T, 4>(ta, lds, s);
+        ((size_t)off * 2 + std::max(top, TAILR_NT) + (size_t)CINV_LD * MG_COARSEST_MAX) * sizeof(T) + 16;
+    if (top <= TAILR_NT) launch_tail<T, 1>(ta, lds, s);
+    else launch_tail<T, 2>(ta, lds, s);
     // fine levels: fused up kernels
     for (int l = lt - 1; l >= 0; l--) {
         Level<T>& A = L[l];
@@ -1017,7 +1016,7 @@ static void launch_tail(const TailArgs<T>& ta, size_t lds, hipStream_t s)
                                         160 * 1024));
         big_lds = true;
     }
-    hipLaunchKernelGGL((k_tail_reg<T, C0>), dim3(1), dim3(TAIL_NT), lds, s, ta);
+    hipLaunchKernelGGL((k_tail_reg<T, C0>), dim3(1), dim3(TAILR_NT), lds, s, ta);
 }
 
 template <class T>

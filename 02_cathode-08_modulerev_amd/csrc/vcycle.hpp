@@ -451,15 +451,17 @@ __global__ void __launch_bounds__(TAIL_NT) k_tail(TailArgs<T> A)
 
 // ------------------------------------------------------------------ register-resident tail (real types)
 // Same levels, same sweeps and transfers as k_tail, organised for latency.
-// The first tail level may hold up to TAIL_TOP_MAX cells: thread q owns its
-// C0 consecutive cells q*C0 .. q*C0+C0-1 (consecutive cells alternate in
-// colour, so every half sweep keeps every thread busy); every further level
-// has at most TAIL_NT cells, one per thread.  Each owned cell's coefficients
-// and 1/aP sit in registers, LDS holds only x, b and the residuals, and
-// phases whose cells all lie in the first wavefront (<= 64 cells)
-// synchronise that wavefront alone instead of the whole workgroup.
-constexpr int TAIL_REG_LEVELS = 10;   // tail levels below the first: at most 9 (<= 512 cells)
-constexpr int TAIL_TOP_MAX = 4 * TAIL_NT;
+// The workgroup has TAILR_NT threads.  The first tail level may hold up to
+// TAIL_TOP_MAX cells: thread q owns its C0 consecutive cells q*C0 ..
+// q*C0+C0-1 (consecutive cells alternate in colour, so every half sweep
+// keeps every thread busy); every further level has at most TAILR_NT cells,
+// one per thread.  Each owned cell's coefficients and 1/aP sit in registers,
+// LDS holds only x, b and the residuals, and phases whose cells all lie in
+// the first wavefront (<= 64 cells) synchronise that wavefront alone instead
+// of the whole workgroup.
+constexpr int TAILR_NT = 1024;
+constexpr int TAIL_REG_LEVELS = 7;    // the first tail level and at most 6 below it
+constexpr int TAIL_TOP_MAX = 2 * TAILR_NT;
 
 __device__ inline void tail_sync(int cells)
 {
@@ -579,21 +581,21 @@ __device__ inline void tail_prolong(const TailCells<T, C>& L, T* X)
 }
 
 template <class T, int C0>
-__global__ void __launch_bounds__(TAIL_NT) k_tail_reg(TailArgs<T> A)
+__global__ void __launch_bounds__(TAILR_NT) k_tail_reg(TailArgs<T> A)
 {
     extern __shared__ unsigned char tail_lds[];
     T* X = reinterpret_cast<T*>(tail_lds);
     T* B = X + A.ncell;
-    T* R = B + A.ncell;  // residuals of one level, by cell number (>= max(cells0, TAIL_NT) entries)
+    T* R = B + A.ncell;  // residuals of one level, by cell number (>= max(cells0, TAILR_NT) entries)
     const int q = threadIdx.x;
     const int nt = A.nt;
-    const int rn = max(A.ni[0] * A.nj[0], TAIL_NT);
+    const int rn = max(A.ni[0] * A.nj[0], TAILR_NT);
     T* CI = R + rn;      // the coarsest level's inverse, row pitch CINV_LD
-    for (int c = q; c < A.ncell; c += TAIL_NT) {
+    for (int c = q; c < A.ncell; c += TAILR_NT) {
         X[c] = T(0);
         B[c] = T(0);
     }
-    for (int e = q; e < A.cn * A.cn; e += TAIL_NT) CI[(e / A.cn) * CINV_LD + e % A.cn] = A.cinv[e];
+    for (int e = q; e < A.cn * A.cn; e += TAILR_NT) CI[(e / A.cn) * CINV_LD + e % A.cn] = A.cinv[e];
     __syncthreads();
     TailCells<T, C0> L0;
     TailCells<T, 1> Lk[TAIL_REG_LEVELS - 1];  // tail levels 1 ..
