@@ -603,7 +603,7 @@ __global__ void k_bicg_x(int ni, int nj, int ld, const double* __restrict__ aP, 
                          const double* __restrict__ ph, const double* __restrict__ sh,
                          const double* __restrict__ sv, const double* __restrict__ t,
                          double* __restrict__ sc, double* __restrict__ part, int nb,
-                         const double* __restrict__ p2, int n2)
+                         const double* __restrict__ p2, int n2, double* __restrict__ host)
 {
     __shared__ double lds[8];
     CELL_IJ;
@@ -652,6 +652,18 @@ __global__ void k_bicg_x(int ni, int nj, int ld, const double* __restrict__ aP, 
         part[bid] = t1;
         part[nb + bid] = t2;
         part[2 * nb + bid] = t3;
+        if (host) {
+            // the host's copy for the convergence test (pinned memory, read
+            // after the stream is synchronised): the partials after the
+            // scalar slots, and the two scalars it reads
+            host[SC_COUNT + bid] = t1;
+            host[SC_COUNT + nb + bid] = t2;
+            host[SC_COUNT + 2 * nb + bid] = t3;
+            if (bid == 0) {
+                host[SC_BB] = sc[SC_BB];
+                host[SC_SIDE] = sc[SC_SIDE];
+            }
+        }
     }
 }
 
@@ -1276,7 +1288,8 @@ int MGSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_sta
         if (mixed) vcycle<float>(L32, s, fd, fu);
         else vcycle<double>(L64, s, fd, fu);
         hipLaunchKernelGGL(k_bicg_x, g, blk, 0, s, F.ni, F.nj, F.ld, F.aP, F.aE, F.aW, F.aN, F.aS, n9, n0, bsol.p,
-                           act0.p, xa_, xb_, r, r0, ph, sh, sv, t, scal.p, part3, nb, (const double*)part.p, ntiles);
+                           act0.p, xa_, xb_, r, r0, ph, sh, sv, t, scal.p, part3, nb, (const double*)part.p, ntiles,
+                           h_scal);
         CATHODE_LAUNCH_CHECK();
     };
     // V-cycle output on level 0: the finest up kernel's xu (x when the
@@ -1344,14 +1357,14 @@ int MGSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_sta
         hipLaunchKernelGGL(k_bicg_finish, dim3(1), dim3(256), 0, s, part.p, nb, scal.p, 3, 0.0);
         // x, recursive and true residuals
         hipLaunchKernelGGL(k_bicg_x, g, blk, 0, s, F.ni, F.nj, F.ld, F.aP, F.aE, F.aW, F.aN, F.aS, n9, n0, bsol.p,
-                           act0.p, xa, xb, r, r0, ph, sh, sv, t, scal.p, part3, nb, (const double*)nullptr, 0);
+                           act0.p, xa, xb, r, r0, ph, sh, sv, t, scal.p, part3, nb, (const double*)nullptr, 0, h_scal);
         hipLaunchKernelGGL(k_bicg_finish, dim3(1), dim3(256), 0, s, part3, nb, scal.p, 4, 0.0);
         }
         CATHODE_LAUNCH_CHECK();
         std::swap(xa, xb);
-        // one host round trip per iteration: the scalars and the x-update
-        // partials; |r|^2 and |b - A x|^2 summed here in a fixed order
-        fetch_host(h_scal, scal.p, SC_COUNT + 3 * (size_t)nb, s);
+        // one host round trip per iteration: the x update stored its partials
+        // and |b|^2 and the side slot in h_scal; |r|^2 and |b - A x|^2 are
+        // summed here in a fixed order
         CATHODE_HIP(hipStreamSynchronize(s));
         {
             double rr = 0.0, rt2 = 0.0;
