@@ -156,6 +156,7 @@ SIGNATURES = [
     ("cathode_ndd_profile", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("cathode_ndd_profile_read", ctypes.c_int,
      [ctypes.c_void_p, c_int_p, c_double_p, c_double_p]),
+    ("cathode_ndd_tile", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, c_int_p, c_int_p, c_int_p]),
 ]
 
 

@@ -477,17 +477,41 @@ __global__ void k_bicg_restart(int ni, int nj, int ld, const double* __restrict_
                                const double* __restrict__ b, const double* __restrict__ x,
                                const unsigned char* __restrict__ act, double* __restrict__ r,
                                double* __restrict__ r0, double* __restrict__ p, double* __restrict__ v,
-                               double* __restrict__ part, int nb, double* __restrict__ xzero)
+                               double* __restrict__ part, int nb, double* __restrict__ xout,
+                               const double* __restrict__ wy, const double* __restrict__ wpart, int wnb)
 {
     __shared__ double lds[8];
     CELL_IJ;
+    // warm start x = alpha y, alpha = <A y, b> / <A y, A y> from the caller's
+    // partials (every workgroup sums them in the same order)
+    double alpha = 0.0;
+    if (wy) {
+        double d1 = 0.0, d2 = 0.0;
+        for (int q = lin_tid(); q < wnb; q += BX * BY) {
+            d1 += wpart[q];
+            d2 += wpart[wnb + q];
+        }
+        d1 = block_sum_bcast<BX * BY>(d1, lds);
+        d2 = block_sum_bcast<BX * BY>(d2, lds);
+        const double al = d2 > 0.0 ? d1 / d2 : 0.0;
+        alpha = isfinite(al) ? al : 0.0;
+    }
     double rr = 0.0;
     if (i <= ni && j <= nj) {
         const size_t k = (size_t)i * ld + j;
         double q = 0.0;
-        if (xzero) {
+        if (wy) {
+            // x = alpha y on the interior (y has zero ghosts), r = b - alpha A y
+            xout[k] = alpha * wy[k];
+            if (act[k]) {
+                const double ay =
+                    aP[k] * wy[k] - offdiag(aE, aW, aN, aS, nine, n0, k, ld, [&](size_t m) { return wy[m]; });
+                q = b[k] - alpha * ay;
+                rr = q * q;
+            }
+        } else if (xout) {
             // zero initial guess: x = 0 on the interior, r = b
-            xzero[k] = 0.0;
+            xout[k] = 0.0;
             if (act[k]) {
                 q = b[k];
                 rr = q * q;
@@ -1118,6 +1142,20 @@ void MGSolver::lump_nine(hipStream_t s)
     CATHODE_LAUNCH_CHECK();
 }
 
+void MGSolver::tile_shape(int mixed, int l, int* ti, int* tj, int* nt) const
+{
+    int a = 0, b = 0, c = 0;
+    if (l >= 0 && l < lt) {
+        const TileCfg& G = VT_CFGS[tiles[mixed ? 1 : 0][l]];
+        a = G.ti;
+        b = G.tj;
+        c = G.nt;
+    }
+    if (ti) *ti = a;
+    if (tj) *tj = b;
+    if (nt) *nt = c;
+}
+
 void MGSolver::drop_graphs()
 {
     for (auto& gr : graphs)
@@ -1204,9 +1242,12 @@ int MGSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_sta
             autotune(mixed, s);
         }
     }
-    // without a kept guess the first restart zeroes x (interior; ghosts stay 0)
-    bool zero_guess = !keep_guess;
+    // without a kept guess the first restart zeroes x (interior; ghosts stay
+    // 0), or sets the caller's warm start alpha y
+    const double* wy = keep_guess ? nullptr : warm_y;
+    bool zero_guess = !keep_guess && !wy;
     keep_guess = false;
+    warm_y = nullptr;
     double* r = kv.p;
     double* r0 = r + n0;
     double* p = r0 + n0;
@@ -1304,8 +1345,10 @@ int MGSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_sta
     while (it < maxITR) {
         if (restart) {
             hipLaunchKernelGGL(k_bicg_restart, g, blk, 0, s, F.ni, F.nj, F.ld, F.aP, F.aE, F.aW, F.aN, F.aS, n9,
-                               n0, bsol.p, xa, act0.p, r, r0, pa, v, part3, nb, zero_guess ? xa : nullptr);
+                               n0, bsol.p, xa, act0.p, r, r0, pa, v, part3, nb,
+                               (zero_guess || wy) ? xa : nullptr, wy, warm_part, warm_np);
             zero_guess = false;
+            wy = nullptr;
             // the fused path sums these partials in its first down kernel
             if (!fused) hipLaunchKernelGGL(k_bicg_finish, dim3(1), dim3(256), 0, s, part3, nb, scal.p, 1, 0.0);
         }
@@ -1397,7 +1440,7 @@ int MGSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_sta
     }
     if (maxITR <= 0) {
         // no iteration: x = 0, relative residual 1 (0 when b = 0)
-        if (zero_guess) CATHODE_HIP(hipMemsetAsync(xsol.p, 0, n0 * sizeof(double), s));
+        if (zero_guess || wy) CATHODE_HIP(hipMemsetAsync(xsol.p, 0, n0 * sizeof(double), s));
         fetch_host(h_scal, scal.p, SC_COUNT, s);
         CATHODE_HIP(hipStreamSynchronize(s));
         rel = h_scal[SC_BB] > 0.0 ? 1.0 : 0.0;

@@ -77,6 +77,8 @@ public:
     double* aS() { return L64[0].aS; }
     double* rhs() { return bsol.p; }
     double* sol() { return xsol.p; }
+    // tile shape in use on fine level l (0 x 0 from the tail on)
+    void tile_shape(int mixed, int l, int* ti, int* tj, int* nt) const;
     // level-0 activity mask; taking the pointer marks the mask as written,
     // so the next setup mirrors it into the level hierarchy
     unsigned char* act()
@@ -103,7 +105,29 @@ public:
     // sol() (interior cells; ghosts 0) instead; the flag clears itself.
     int solve(int maxITR, double tol, int mode, hipStream_t s, cathode_status* st);
     bool keep_guess = false;
+    // Warm start for the next solve: x0 = alpha y with alpha = <A y, b> /
+    // <A y, A y>, from partials part[0..np) of <A y, b> and part[np..2np) of
+    // <A y, A y> that the caller has computed on the same stream (y: level-0
+    // pitch, zero ghosts).  Ignored when keep_guess is set.
+    void set_warm_start(const double* y, const double* part, int np)
+    {
+        warm_y = y;
+        warm_part = part;
+        warm_np = np;
+    }
+    // Hands the solution buffer to the caller in exchange for buf (same
+    // size, its own allocation, zero ghosts): no copy.  Kernels already
+    // launched with the old sol() pointer still read the solution.
+    double* exchange_solution(double* buf)
+    {
+        double* old = xsol.p;
+        xsol.p = buf;
+        return old;
+    }
     bool act_dirty = true;
+    const double* warm_y = nullptr;
+    const double* warm_part = nullptr;
+    int warm_np = 0;
 
     MGProfile prof;  // dispatch events of the finest-level down kernel
 

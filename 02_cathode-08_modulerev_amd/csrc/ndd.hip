@@ -737,21 +737,102 @@ __global__ void k_learn_features(NddView V, const double* __restrict__ b, const 
     feat[2 * plane + p] = (in && xprev) ? xprev[p] * sx : 0.0;
 }
 
-// A g (level-0 5-point arrays, pitch ld0) against b: partials of <Ag, b>, <Ag, Ag>
+// Initial guess of the first Poisson correction solve from the network's
+// output g (pitch V.ld) and the previous step's solution y of that solve
+// (pitch ld0, nullptr before the first step): x0 = a1 y + a2 g with (a1, a2)
+// minimising |b - A (a1 y + a2 g)| over their span (never a larger starting
+// residual than either term alone or x = 0).  Partials, in this order:
+// <Ag, b>, <Ag, Ag>, <Ay, b>, <Ay, Ay>, <Ay, Ag>.
 __global__ void k_guess_dots(NddView V, const double* __restrict__ aP, const double* __restrict__ aE,
                              const double* __restrict__ aW, const double* __restrict__ aN,
                              const double* __restrict__ aS, const double* __restrict__ b,
-                             const double* __restrict__ g, double* __restrict__ part, int nb)
+                             const double* __restrict__ g, const double* __restrict__ y, double* __restrict__ part,
+                             int nb)
+{
+    __shared__ double lds[8];
+    THREAD_IJ(1, 1);
+    double d[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if (i <= V.ni && j <= V.nj) {
+        const size_t k = IX0(i, j);
+        const double ag = aP[k] * g[IX(i, j)] - aE[k] * g[IX(i + 1, j)] - aW[k] * g[IX(i - 1, j)] -
+                          aN[k] * g[IX(i, j + 1)] - aS[k] * g[IX(i, j - 1)];
+        d[0] = ag * b[k];
+        d[1] = ag * ag;
+        if (y) {
+            const double ay = aP[k] * y[k] - aE[k] * y[IX0(i + 1, j)] - aW[k] * y[IX0(i - 1, j)] -
+                              aN[k] * y[IX0(i, j + 1)] - aS[k] * y[IX0(i, j - 1)];
+            d[2] = ay * b[k];
+            d[3] = ay * ay;
+            d[4] = ay * ag;
+        }
+    }
+    const int bid = blockIdx.y * gridDim.x + blockIdx.x;
+#pragma unroll
+    for (int q = 0; q < 5; q++) {
+        const double t = block_sum<NB_X * NB_Y>(d[q], lds);
+        if (lin_tid() == 0) part[q * nb + bid] = t;
+    }
+}
+
+// out[2] = a2 (network), out[3] = 1 when the guess is used, out[5] = a1
+// (previous solution)
+__global__ void k_guess_alpha(const double* __restrict__ part, int nb, double* __restrict__ out)
+{
+    __shared__ double lds[8];
+    double t[5];
+#pragma unroll
+    for (int q = 0; q < 5; q++) {
+        double a = 0.0;
+        for (int k = threadIdx.x; k < nb; k += blockDim.x) a += part[q * nb + k];
+        t[q] = block_sum<256>(a, lds);
+    }
+    if (threadIdx.x == 0) {
+        const double c2 = t[0], a22 = t[1], c1 = t[2], a11 = t[3], a12 = t[4];
+        double a1 = 0.0, a2 = 0.0;
+        const double det = a11 * a22 - a12 * a12;
+        if (a11 > 0.0 && a22 > 0.0 && det > 1e-12 * a11 * a22) {
+            a1 = (c1 * a22 - c2 * a12) / det;
+            a2 = (a11 * c2 - a12 * c1) / det;
+        } else if (a11 > 0.0 && (a22 <= 0.0 || c1 * c1 / a11 >= c2 * c2 / a22)) {
+            a1 = c1 / a11;  // one term only: the one that lowers the residual more
+        } else if (a22 > 0.0) {
+            a2 = c2 / a22;
+        }
+        if (!isfinite(a1) || !isfinite(a2)) a1 = a2 = 0.0;
+        out[2] = a2;
+        out[3] = (a1 != 0.0 || a2 != 0.0) ? 1.0 : 0.0;
+        out[5] = a1;
+    }
+}
+
+// solver initial guess x (pitch ld0) = a1 y + a2 g on interior cells, 0 on ghosts
+__global__ void k_guess_set(NddView V, const double* __restrict__ g, const double* __restrict__ y,
+                            const double* __restrict__ alpha, double* __restrict__ x)
+{
+    THREAD_IJ(0, 0);
+    if (i > V.ni + 1 || j > V.nj + 1) return;
+    const bool in = i >= 1 && i <= V.ni && j >= 1 && j <= V.nj;
+    const double a2 = alpha[0], a1 = alpha[3];
+    x[IX0(i, j)] = in ? a2 * g[IX(i, j)] + (y ? a1 * y[IX0(i, j)] : 0.0) : 0.0;
+}
+
+// Warm start from the previous step's solution y of the same solve (pitch
+// ld0): partials of <A y, b> and <A y, A y>, then x = alpha y with alpha =
+// <A y, b> / <A y, A y> minimising |b - alpha A y| (0 when y does not help).
+__global__ void k_warm_dots(NddView V, const double* __restrict__ aP, const double* __restrict__ aE,
+                            const double* __restrict__ aW, const double* __restrict__ aN,
+                            const double* __restrict__ aS, const double* __restrict__ b,
+                            const double* __restrict__ y, double* __restrict__ part, int nb)
 {
     __shared__ double lds[8];
     THREAD_IJ(1, 1);
     double d1 = 0.0, d2 = 0.0;
     if (i <= V.ni && j <= V.nj) {
         const size_t k = IX0(i, j);
-        const double ag = aP[k] * g[IX(i, j)] - aE[k] * g[IX(i + 1, j)] - aW[k] * g[IX(i - 1, j)] -
-                          aN[k] * g[IX(i, j + 1)] - aS[k] * g[IX(i, j - 1)];
-        d1 = ag * b[k];
-        d2 = ag * ag;
+        const double ay = aP[k] * y[k] - aE[k] * y[IX0(i + 1, j)] - aW[k] * y[IX0(i - 1, j)] -
+                          aN[k] * y[IX0(i, j + 1)] - aS[k] * y[IX0(i, j - 1)];
+        d1 = ay * b[k];
+        d2 = ay * ay;
     }
     const double t1 = block_sum<NB_X * NB_Y>(d1, lds);
     const double t2 = block_sum<NB_X * NB_Y>(d2, lds);
@@ -760,36 +841,6 @@ __global__ void k_guess_dots(NddView V, const double* __restrict__ aP, const dou
         part[bid] = t1;
         part[nb + bid] = t2;
     }
-}
-
-// alpha = <Ag, b> / <Ag, Ag> minimises |b - alpha A g|; 0 when undefined.
-// out[2] = alpha, out[3] = 1 when the guess is used
-__global__ void k_guess_alpha(const double* __restrict__ part, int nb, double* __restrict__ out)
-{
-    __shared__ double lds[8];
-    double a = 0.0, b = 0.0;
-    for (int k = threadIdx.x; k < nb; k += blockDim.x) {
-        a += part[k];
-        b += part[nb + k];
-    }
-    a = block_sum<256>(a, lds);
-    b = block_sum<256>(b, lds);
-    if (threadIdx.x == 0) {
-        const double al = b > 0.0 ? a / b : 0.0;
-        const bool ok = isfinite(al) && al != 0.0;
-        out[2] = ok ? al : 0.0;
-        out[3] = ok ? 1.0 : 0.0;
-    }
-}
-
-// solver initial guess x (pitch ld0) = alpha g on interior cells, 0 on ghosts
-__global__ void k_guess_set(NddView V, const double* __restrict__ g, const double* __restrict__ alpha,
-                            double* __restrict__ x)
-{
-    THREAD_IJ(0, 0);
-    if (i > V.ni + 1 || j > V.nj + 1) return;
-    const bool in = i >= 1 && i <= V.ni && j >= 1 && j <= V.nj;
-    x[IX0(i, j)] = in ? alpha[0] * g[IX(i, j)] : 0.0;
 }
 
 // previous-step solution (pitch V.ld) and, for training capture, the
@@ -827,6 +878,11 @@ struct NddHandle {
     void* cnn = nullptr;              // network (cathode_ndd_set_learned), not owned
     DevBuf<double> lfeat, lguess, xprev, lpart, lscal;
     bool have_xprev = false;
+    // learned mode: every solve starts from its own previous-step solution
+    // (slot = equation x PC iteration), scaled by the least-squares alpha
+    DevBuf<double> wbuf[8];  // one allocation each: they trade places with the solver's solution buffer
+    DevBuf<double> wpart, wscal;
+    bool wvalid[8] = {false, false, false, false, false, false, false, false};
     double* cap_feat = nullptr;       // capture for training (device, caller-owned)
     double* cap_target = nullptr;
     int guesses = 0, guesses_used = 0;
@@ -937,6 +993,9 @@ static double calcres3(NddHandle& H, const ResPairs& P, double* err_out)
 // guess g, and the solver's initial guess alpha g with alpha minimising
 // |b - alpha A g| (never worse than x = 0; 0 when the guess does not help).
 // With capture set, the features are copied out for training.
+enum { EQ_ELEC = 0, EQ_ION = 1, EQ_ENERGY = 2, EQ_POISSON = 3 };
+static int warm_slot(int eq, int ncount) { return eq * 2 + (ncount >= 2 ? 1 : 0); }
+
 static void learned_guess(NddHandle& H)
 {
     const NddView& V = H.V;
@@ -944,7 +1003,7 @@ static void learned_guess(NddHandle& H)
     H.lfeat.alloc(3 * plane);
     H.lguess.alloc(plane);
     H.xprev.alloc(plane);
-    H.lpart.alloc(2 * H.nb + 64);
+    H.lpart.alloc(5 * H.nb + 64);
     H.lscal.alloc(8);
     const dim3 g1 = ggrid(V.ni, V.nj), g2 = ggrid(V.ni + 2, V.nj + 2);
     hipLaunchKernelGGL(k_learn_sumsq, g1, gblk(), 0, H.s, V, (const double*)H.mg.rhs(), V.ld0,
@@ -960,11 +1019,13 @@ static void learned_guess(NddHandle& H)
         CATHODE_HIP(hipMemcpyAsync(H.cap_feat, H.lfeat.p, 3 * plane * sizeof(double), hipMemcpyDeviceToDevice, H.s));
     if (!(H.p.learned && H.cnn)) return;
     cnn_infer(H.cnn, V.ni, V.nj, H.lfeat.p, H.lguess.p, H.s);
+    const int slot = warm_slot(EQ_POISSON, 1);
+    const double* y = (H.p.learned && H.wvalid[slot]) ? (const double*)H.wbuf[slot].p : (const double*)nullptr;
     hipLaunchKernelGGL(k_guess_dots, g1, gblk(), 0, H.s, V, (const double*)H.mg.aP(), (const double*)H.mg.aE(),
                        (const double*)H.mg.aW(), (const double*)H.mg.aN(), (const double*)H.mg.aS(),
-                       (const double*)H.mg.rhs(), (const double*)H.lguess.p, H.lpart.p, H.nb);
+                       (const double*)H.mg.rhs(), (const double*)H.lguess.p, y, H.lpart.p, H.nb);
     hipLaunchKernelGGL(k_guess_alpha, dim3(1), dim3(256), 0, H.s, (const double*)H.lpart.p, H.nb, H.lscal.p);
-    hipLaunchKernelGGL(k_guess_set, g2, gblk(), 0, H.s, V, (const double*)H.lguess.p, (const double*)H.lscal.p + 2,
+    hipLaunchKernelGGL(k_guess_set, g2, gblk(), 0, H.s, V, (const double*)H.lguess.p, y, (const double*)H.lscal.p + 2,
                        H.mg.sol());
     CATHODE_LAUNCH_CHECK();
     H.mg.keep_guess = true;
@@ -985,6 +1046,30 @@ static void learned_store(NddHandle& H)
                        H.xprev.p, H.cap_target);
     CATHODE_LAUNCH_CHECK();
     H.have_xprev = true;
+}
+
+
+// learned mode: x0 = alpha y, y the previous step's solution of this solve;
+// the dot products here, alpha and x0 in the solver's first restart kernel
+static void warm_guess(NddHandle& H, int slot)
+{
+    if (!H.p.learned || !H.wvalid[slot]) return;
+    const NddView& V = H.V;
+    const double* y = H.wbuf[slot].p;
+    hipLaunchKernelGGL(k_warm_dots, ggrid(V.ni, V.nj), gblk(), 0, H.s, V, (const double*)H.mg.aP(),
+                       (const double*)H.mg.aE(), (const double*)H.mg.aW(), (const double*)H.mg.aN(),
+                       (const double*)H.mg.aS(), (const double*)H.mg.rhs(), y, H.wpart.p, H.nb);
+    CATHODE_LAUNCH_CHECK();
+    H.mg.set_warm_start(y, H.wpart.p, H.nb);
+}
+
+// after the kernels that read the solution have been launched: the slot
+// takes the solution buffer, the solver the slot's old one (no copy)
+static void warm_store(NddHandle& H, int slot)
+{
+    if (!H.p.learned) return;
+    H.wbuf[slot].p = H.mg.exchange_solution(H.wbuf[slot].p);
+    H.wvalid[slot] = true;
 }
 
 static void poisson_routine(NddHandle& H, bool report)
@@ -1011,6 +1096,7 @@ static void poisson_routine(NddHandle& H, bool report)
         CATHODE_LAUNCH_CHECK();
         const bool learn = ncount == 1 && ((H.p.learned && H.cnn) || H.cap_feat);
         if (learn) learned_guess(H);
+        else warm_guess(H, warm_slot(EQ_POISSON, ncount));
         if (!solve(H, ncount > 1 ? H.p.error_cnv_HES_phi : -INFINITY, H.p.maxITR_SOR_phi, H.p.error_cnv_SOR_phi)) {
             // calcRes of the previous iteration met the PC criterion: undo the prediction
             d2d(H, {{F[F_NUEX], H.tmp_x}, {F[F_NUER], H.tmp_r}});
@@ -1019,6 +1105,7 @@ static void poisson_routine(NddHandle& H, bool report)
         if (learn) learned_store(H);
         hipLaunchKernelGGL(k_poisson_update_phi, ggrid(V.ni + 2, V.nj + 2), gblk(), 0, H.s, V, H.mg.sol(), H.dphi,
                            F[F_PHI], F[F_EX], F[F_ER]);
+        warm_store(H, warm_slot(EQ_POISSON, ncount));
         hipLaunchKernelGGL(k_poisson_fields, ggrid(V.ni, V.nj), gblk(), 0, H.s, V, F[F_PHI], H.dphi, F[F_RHOE],
                            F[F_EX], F[F_ER], F[F_NUEX], F[F_NUER], H.me);
         CATHODE_LAUNCH_CHECK();
@@ -1085,12 +1172,15 @@ static void sg_routine(NddHandle& H, int sp, bool report, double* err3)
         hipLaunchKernelGGL(k_sg_assemble, ggrid(V.ni, V.nj), gblk(), 0, H.s, V, A, H.mg.aP(), H.mg.aE(), H.mg.aW(),
                            H.mg.aN(), H.mg.aS(), H.mg.rhs());
         CATHODE_LAUNCH_CHECK();
+        const int eq = sp == SP_ELEC ? EQ_ELEC : (sp == SP_ION ? EQ_ION : EQ_ENERGY);
+        warm_guess(H, warm_slot(eq, ncount));
         if (!solve(H, ncount > 1 ? cnv : -INFINITY, lin_maxit, lin_tol)) {
             d2d(H, {{A.Fx, H.tmp_x}, {A.Fr, H.tmp_r}});
             break;
         }
         hipLaunchKernelGGL(k_sg_correct, ggrid(V.ni, V.nj), gblk(), 0, H.s, V, A, H.mg.sol());
         CATHODE_LAUNCH_CHECK();
+        warm_store(H, warm_slot(eq, ncount));
         if (ncount >= maxit) break;
         ResPairs P{A.n, H.tmp_c, A.Fx, H.tmp_x, A.Fr, H.tmp_r};
         calcres3_async(H, P);
@@ -1201,6 +1291,15 @@ int cathode_ndd_create(const cathode_ndd_params* p, void** handle)
         H->nb = cdiv(nj, NB_X) * cdiv(ni, NB_Y);
         H->part.alloc(6 * H->nb + 64);
         H->scal.alloc(16);
+        if (p->learned) {
+            for (auto& wb : H->wbuf) {
+                wb.alloc((size_t)n0);
+                CATHODE_HIP(hipMemset(wb.p, 0, (size_t)n0 * sizeof(double)));  // zero ghosts and padding
+            }
+            H->wpart.alloc(6 * H->nb + 64);
+            H->wscal.alloc(8);
+            CATHODE_HIP(hipDeviceSynchronize());  // the memsets run on the null stream
+        }
         CATHODE_HIP(hipHostMalloc(&H->h_scal, 16 * sizeof(double), hipHostMallocDefault));
         CATHODE_HIP(hipStreamSynchronize(H->s));
         *handle = H.release();
@@ -1258,6 +1357,7 @@ int cathode_ndd_set_state(void* handle, const cathode_ndd_fields* host)
         }
         cathode_ndd_fields f = *host;
         H->have_xprev = false;  // a new state: no previous Poisson correction
+        for (bool& v : H->wvalid) v = false;
         for (int k = 0; k < 19; k++) {
             const double* src = *field_slot(&f, k);
             if (!src) {
@@ -1432,6 +1532,14 @@ int cathode_ndd_profile(void* handle, int enable)
     H->mg.prof.enabled = enable != 0;
     H->mg.prof.launches = 0;
     H->mg.prof.total_ms = 0.0;
+    return CATHODE_OK;
+}
+
+int cathode_ndd_tile(void* handle, int mixed, int level, int* ti, int* tj, int* nt)
+{
+    NddHandle* H = (NddHandle*)handle;
+    if (!H || level < 0) return CATHODE_ERR_ARG;
+    H->mg.tile_shape(mixed, level, ti, tj, nt);
     return CATHODE_OK;
 }
 

@@ -65,6 +65,10 @@ def run_path(lib, mode, steps, warmup, seed, net=None):
     bpl = ctypes.c_double()
     abi.check(lib.cathode_ndd_profile_read(h, ctypes.byref(launches), ctypes.byref(tot_ms), ctypes.byref(bpl)), lib)
     abi.check(lib.cathode_ndd_profile(h, 0), lib)
+    # the finest level's tile shape the autotuner picked for this run
+    ti, tj, nt = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    abi.check(lib.cathode_ndd_tile(h, 0 if mode == abi.CATHODE_MODE_FP64 else 1, 0, ctypes.byref(ti),
+                                   ctypes.byref(tj), ctypes.byref(nt)), lib)
     lib.cathode_ndd_destroy(h)
     avg_ms = tot_ms.value / max(launches.value, 1)
     return {
@@ -78,20 +82,22 @@ def run_path(lib, mode, steps, warmup, seed, net=None):
         "smooth_avg_us": 1e3 * avg_ms,
         "smooth_bytes": bpl.value,
         "smooth_GBps": bpl.value / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0,
+        "smooth_tile": f"{ti.value}x{tj.value}/{nt.value}",
     }
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_k_vtile_down_l0_512x256.json")
 
 
-def traffic_per_launch(path):
-    """HBM bytes per launch of the finest-level down kernel from the committed
-    rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE measurement (calibrated against a
-    copy kernel of known size), or None when it has not been measured."""
+def traffic_per_launch(path, tile):
+    """HBM bytes per launch of the finest-level down kernel with this tile
+    shape, from the committed rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+    measurement (calibrated against a copy kernel of known size), or None
+    when that path and shape have not been measured."""
     try:
         with open(TRAFFIC_FILE) as f:
             t = json.load(f)
-        return t["paths"][path]["hbm_bytes_per_launch"]
+        return t["paths"][path][tile]["hbm_bytes_per_launch"]
     except (OSError, KeyError, ValueError):
         return None
 
@@ -207,10 +213,11 @@ def main():
     best = "learned" if "learned" in res else ("mixed" if "mixed" in res else paths[0])
     b = res[best]
     roof = {"bound": "hbm", "achieved": round(b["smooth_GBps"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(b["smooth_GBps"] / HBM_PEAK_GBS, 4), "traffic": traffic_per_launch("mixed" if best == "learned" else best),
+            "frac": round(b["smooth_GBps"] / HBM_PEAK_GBS, 4),
+            "traffic": traffic_per_launch("mixed" if best == "learned" else best, b["smooth_tile"]),
             "kernel": "k_vtile finest-level down kernel (BiCGStab p/s update + 2 red-black sweeps + residual + "
                       "restriction), "
-                      + ("FP64" if best == "fp64" else "FP32"),
+                      + ("FP64" if best == "fp64" else "FP32") + ", tile " + b["smooth_tile"],
             "bytes_model": "per cell: aP aE aW aN aS + activity byte read, x + b + 1/4 coarse b written at "
                            "level precision; r, v (+ old p on every other launch) read and new p or s written "
                            "in FP64 (fused BiCGStab update): 58 B (FP32 levels), 87 B (FP64)",

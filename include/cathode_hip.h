@@ -277,6 +277,12 @@ int cathode_ndd_capture(void* handle, double* feat_dev, double* target_dev);
 int cathode_ndd_profile(void* handle, int enable);
 int cathode_ndd_profile_read(void* handle, int* launches, double* total_ms,
                              double* bytes_per_launch);
+/* The tile shape the multigrid uses on fine level `level` for the given
+ * precision (mixed = 0: FP64 levels, 1: FP32 levels), after the autotuner
+ * has run: rows ti x columns tj per workgroup of nt threads.  Levels from
+ * the tail on report 0 x 0.  No counterpart in the reference (diagnostics
+ * of this implementation's kernels). */
+int cathode_ndd_tile(void* handle, int mixed, int level, int* ti, int* tj, int* nt);
 
 #ifdef __cplusplus
 }
