@@ -724,7 +724,7 @@ __global__ void k_learn_rms(const double* __restrict__ part, int nb, double ncel
 }
 
 // feature planes (pitch V.ld): b / rms(b), r / r_top, x_prev / rms(x_prev)
-__global__ void k_learn_features(NddView V, const double* __restrict__ b, const double* __restrict__ xprev,
+__global__ void k_learn_features(NddView V, const double* __restrict__ b, const double* __restrict__ xprev, int lx,
                                  const double* __restrict__ rms, double r_top, double* __restrict__ feat)
 {
     THREAD_IJ(0, 0);
@@ -734,7 +734,7 @@ __global__ void k_learn_features(NddView V, const double* __restrict__ b, const 
     const double sb = rms[0] > 0.0 ? 1.0 / rms[0] : 0.0, sx = rms[1] > 0.0 ? 1.0 / rms[1] : 0.0;
     feat[p] = in ? b[IX0(i, j)] * sb : 0.0;
     feat[plane + p] = in ? V.r[j] / r_top : 0.0;
-    feat[2 * plane + p] = (in && xprev) ? xprev[p] * sx : 0.0;
+    feat[2 * plane + p] = (in && xprev) ? xprev[(size_t)i * lx + j] * sx : 0.0;
 }
 
 // Initial guess of the first Poisson correction solve from the network's
@@ -742,7 +742,7 @@ __global__ void k_learn_features(NddView V, const double* __restrict__ b, const 
 // (pitch ld0, nullptr before the first step): x0 = a1 y + a2 g with (a1, a2)
 // minimising |b - A (a1 y + a2 g)| over their span (never a larger starting
 // residual than either term alone or x = 0).  Partials, in this order:
-// <Ag, b>, <Ag, Ag>, <Ay, b>, <Ay, Ay>, <Ay, Ag>.
+// <Ag, b>, <Ag, Ag>, <Ay, b>, <Ay, Ay>, <Ay, Ag>, <b, b>.
 __global__ void k_guess_dots(NddView V, const double* __restrict__ aP, const double* __restrict__ aE,
                              const double* __restrict__ aW, const double* __restrict__ aN,
                              const double* __restrict__ aS, const double* __restrict__ b,
@@ -751,9 +751,10 @@ __global__ void k_guess_dots(NddView V, const double* __restrict__ aP, const dou
 {
     __shared__ double lds[8];
     THREAD_IJ(1, 1);
-    double d[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    double d[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (i <= V.ni && j <= V.nj) {
         const size_t k = IX0(i, j);
+        d[5] = b[k] * b[k];
         const double ag = aP[k] * g[IX(i, j)] - aE[k] * g[IX(i + 1, j)] - aW[k] * g[IX(i - 1, j)] -
                           aN[k] * g[IX(i, j + 1)] - aS[k] * g[IX(i, j - 1)];
         d[0] = ag * b[k];
@@ -768,20 +769,24 @@ __global__ void k_guess_dots(NddView V, const double* __restrict__ aP, const dou
     }
     const int bid = blockIdx.y * gridDim.x + blockIdx.x;
 #pragma unroll
-    for (int q = 0; q < 5; q++) {
+    for (int q = 0; q < 6; q++) {
         const double t = block_sum<NB_X * NB_Y>(d[q], lds);
         if (lin_tid() == 0) part[q * nb + bid] = t;
     }
 }
 
 // out[2] = a2 (network), out[3] = 1 when the guess is used, out[5] = a1
-// (previous solution)
-__global__ void k_guess_alpha(const double* __restrict__ part, int nb, double* __restrict__ out)
+// (previous solution).  host[0] (pinned memory, read by the host after a
+// later synchronisation): the network's residual ratio |b - A x0|^2 /
+// |b - a A y|^2, the starting residual with the network's term over the one
+// with the previous solution alone (1: the network adds nothing).
+__global__ void k_guess_alpha(const double* __restrict__ part, int nb, double* __restrict__ out,
+                              double* __restrict__ host)
 {
     __shared__ double lds[8];
-    double t[5];
+    double t[6];
 #pragma unroll
-    for (int q = 0; q < 5; q++) {
+    for (int q = 0; q < 6; q++) {
         double a = 0.0;
         for (int k = threadIdx.x; k < nb; k += blockDim.x) a += part[q * nb + k];
         t[q] = block_sum<256>(a, lds);
@@ -802,6 +807,12 @@ __global__ void k_guess_alpha(const double* __restrict__ part, int nb, double* _
         out[2] = a2;
         out[3] = (a1 != 0.0 || a2 != 0.0) ? 1.0 : 0.0;
         out[5] = a1;
+        if (host) {
+            const double bb = t[5];
+            const double ry = a11 > 0.0 ? bb - c1 * c1 / a11 : bb;  // previous solution alone
+            const double rb = bb - (c1 * a1 + c2 * a2);              // both terms
+            host[0] = ry > 0.0 ? fmax(rb, 0.0) / ry : 1.0;
+        }
     }
 }
 
@@ -886,6 +897,11 @@ struct NddHandle {
     double* cap_feat = nullptr;       // capture for training (device, caller-owned)
     double* cap_target = nullptr;
     int guesses = 0, guesses_used = 0;
+    // the network is used while it lowers the warm-started residual: after
+    // a use that did not (residual ratio >= NET_USEFUL), the next NET_PROBE - 1
+    // first Poisson solves start from the warm start alone, then it is tried again
+    bool net_useful = true, gate_pending = false;
+    int net_wait = 0;
 };
 
 enum {
@@ -996,40 +1012,55 @@ static double calcres3(NddHandle& H, const ResPairs& P, double* err_out)
 enum { EQ_ELEC = 0, EQ_ION = 1, EQ_ENERGY = 2, EQ_POISSON = 3 };
 static int warm_slot(int eq, int ncount) { return eq * 2 + (ncount >= 2 ? 1 : 0); }
 
-static void learned_guess(NddHandle& H)
+constexpr double NET_USEFUL = 0.9;  // residual ratio below which the network's term counts as useful
+constexpr int NET_PROBE = 16;       // while it is not, it is tried once every NET_PROBE steps
+
+static void warm_guess(NddHandle& H, int slot);
+
+static void learned_guess(NddHandle& H, bool net)
 {
     const NddView& V = H.V;
     const size_t plane = (size_t)(V.ni + 2) * V.ld;
     H.lfeat.alloc(3 * plane);
     H.lguess.alloc(plane);
     H.xprev.alloc(plane);
-    H.lpart.alloc(5 * H.nb + 64);
+    H.lpart.alloc(6 * H.nb + 64);
     H.lscal.alloc(8);
     const dim3 g1 = ggrid(V.ni, V.nj), g2 = ggrid(V.ni + 2, V.nj + 2);
-    hipLaunchKernelGGL(k_learn_sumsq, g1, gblk(), 0, H.s, V, (const double*)H.mg.rhs(), V.ld0,
-                       H.have_xprev ? (const double*)H.xprev.p : (const double*)nullptr, V.ld, H.lpart.p, H.nb);
+    // the previous step's solution of this solve: the learned mode's warm
+    // slot (pitch ld0), else the copy learned_store keeps (pitch ld)
+    const int slot = warm_slot(EQ_POISSON, 1);
+    const bool from_slot = H.p.learned && H.wvalid[slot];
+    const double* xp = from_slot ? (const double*)H.wbuf[slot].p
+                                 : (H.have_xprev ? (const double*)H.xprev.p : (const double*)nullptr);
+    const int lxp = from_slot ? V.ld0 : V.ld;
+    hipLaunchKernelGGL(k_learn_sumsq, g1, gblk(), 0, H.s, V, (const double*)H.mg.rhs(), V.ld0, xp, lxp, H.lpart.p,
+                       H.nb);
     hipLaunchKernelGGL(k_learn_rms, dim3(1), dim3(256), 0, H.s, (const double*)H.lpart.p, H.nb,
                        (double)V.ni * V.nj, H.lscal.p);
     const double r_top = H.p.rmin + V.nj * V.dr;
-    hipLaunchKernelGGL(k_learn_features, g2, gblk(), 0, H.s, V, (const double*)H.mg.rhs(),
-                       H.have_xprev ? (const double*)H.xprev.p : (const double*)nullptr, (const double*)H.lscal.p,
-                       r_top, H.lfeat.p);
+    hipLaunchKernelGGL(k_learn_features, g2, gblk(), 0, H.s, V, (const double*)H.mg.rhs(), xp, lxp,
+                       (const double*)H.lscal.p, r_top, H.lfeat.p);
     CATHODE_LAUNCH_CHECK();
     if (H.cap_feat)
         CATHODE_HIP(hipMemcpyAsync(H.cap_feat, H.lfeat.p, 3 * plane * sizeof(double), hipMemcpyDeviceToDevice, H.s));
-    if (!(H.p.learned && H.cnn)) return;
+    if (!(H.p.learned && H.cnn && net)) {
+        warm_guess(H, slot);  // learned mode without the network this step: the warm start alone
+        return;
+    }
     cnn_infer(H.cnn, V.ni, V.nj, H.lfeat.p, H.lguess.p, H.s);
-    const int slot = warm_slot(EQ_POISSON, 1);
-    const double* y = (H.p.learned && H.wvalid[slot]) ? (const double*)H.wbuf[slot].p : (const double*)nullptr;
+    const double* y = from_slot ? xp : (const double*)nullptr;
     hipLaunchKernelGGL(k_guess_dots, g1, gblk(), 0, H.s, V, (const double*)H.mg.aP(), (const double*)H.mg.aE(),
                        (const double*)H.mg.aW(), (const double*)H.mg.aN(), (const double*)H.mg.aS(),
                        (const double*)H.mg.rhs(), (const double*)H.lguess.p, y, H.lpart.p, H.nb);
-    hipLaunchKernelGGL(k_guess_alpha, dim3(1), dim3(256), 0, H.s, (const double*)H.lpart.p, H.nb, H.lscal.p);
+    hipLaunchKernelGGL(k_guess_alpha, dim3(1), dim3(256), 0, H.s, (const double*)H.lpart.p, H.nb, H.lscal.p,
+                       H.h_scal + 8);
     hipLaunchKernelGGL(k_guess_set, g2, gblk(), 0, H.s, V, (const double*)H.lguess.p, y, (const double*)H.lscal.p + 2,
                        H.mg.sol());
     CATHODE_LAUNCH_CHECK();
     H.mg.keep_guess = true;
     H.guesses++;
+    H.gate_pending = true;
 }
 
 // after the first Poisson correction solve: keep its solution as the next
@@ -1094,15 +1125,27 @@ static void poisson_routine(NddHandle& H, bool report)
         hipLaunchKernelGGL(k_poisson_assemble, ggrid(V.ni, V.nj), gblk(), 0, H.s, V, A, H.mg.aP(), H.mg.aE(),
                            H.mg.aW(), H.mg.aN(), H.mg.aS(), H.mg.rhs());
         CATHODE_LAUNCH_CHECK();
-        const bool learn = ncount == 1 && ((H.p.learned && H.cnn) || H.cap_feat);
-        if (learn) learned_guess(H);
+        bool use_net = false;
+        if (ncount == 1 && H.p.learned && H.cnn) {
+            // the previous use's residual ratio (written by the device, read
+            // here after the syncs of the steps since)
+            if (H.gate_pending) {
+                H.gate_pending = false;
+                H.net_useful = H.h_scal[8] < NET_USEFUL;
+                if (!H.net_useful) H.net_wait = NET_PROBE - 1;
+            }
+            use_net = H.net_useful || H.net_wait == 0;
+            if (!use_net) H.net_wait--;
+        }
+        const bool learn = ncount == 1 && (use_net || H.cap_feat);
+        if (learn) learned_guess(H, use_net);
         else warm_guess(H, warm_slot(EQ_POISSON, ncount));
         if (!solve(H, ncount > 1 ? H.p.error_cnv_HES_phi : -INFINITY, H.p.maxITR_SOR_phi, H.p.error_cnv_SOR_phi)) {
             // calcRes of the previous iteration met the PC criterion: undo the prediction
             d2d(H, {{F[F_NUEX], H.tmp_x}, {F[F_NUER], H.tmp_r}});
             break;
         }
-        if (learn) learned_store(H);
+        if (learn && (H.cap_feat || !H.p.learned)) learned_store(H);  // learned mode reads its warm slot
         hipLaunchKernelGGL(k_poisson_update_phi, ggrid(V.ni + 2, V.nj + 2), gblk(), 0, H.s, V, H.mg.sol(), H.dphi,
                            F[F_PHI], F[F_EX], F[F_ER]);
         warm_store(H, warm_slot(EQ_POISSON, ncount));
@@ -1358,6 +1401,9 @@ int cathode_ndd_set_state(void* handle, const cathode_ndd_fields* host)
         cathode_ndd_fields f = *host;
         H->have_xprev = false;  // a new state: no previous Poisson correction
         for (bool& v : H->wvalid) v = false;
+        H->net_useful = true;
+        H->gate_pending = false;
+        H->net_wait = 0;
         for (int k = 0; k < 19; k++) {
             const double* src = *field_slot(&f, k);
             if (!src) {

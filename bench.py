@@ -34,9 +34,11 @@ METRIC = "NDD steps/s at 512x256 (resid 1e-10): learned+mixed vs plain FP64, 1 G
 HBM_PEAK_GBS = 8000.0
 
 
-def run_path(lib, mode, steps, warmup, seed, net=None):
+def run_path(lib, mode, steps, warmup, seed, net=None, warm=False):
     p, st = W.config1(seed=seed, mode=mode)
-    p.learned = 1 if net is not None else 0
+    # learned mode: previous-solution warm starts, plus the network's guess
+    # for the first Poisson correction solve when a network is attached
+    p.learned = 1 if (net is not None or warm) else 0
     h = ctypes.c_void_p()
     abi.check(lib.cathode_ndd_create(ctypes.byref(p), ctypes.byref(h)), lib)
     if net is not None:
@@ -180,7 +182,8 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--paths", default="fp64,mixed,learned",
-                    help="fp64, mixed (FP32 V-cycles + FP64 refinement), learned (mixed + learned initial guess)")
+                    help="fp64, mixed (FP32 V-cycles + FP64 refinement), learned (mixed + learned initial guess), "
+                         "warm (mixed + previous-solution warm starts, no network: the learned path's ablation)")
     ap.add_argument("--sweep", action="store_true",
                     help="also run the 64-instance bias sweep (batched vs one at a time)")
     ap.add_argument("--sweep-steps", type=int, default=50)
@@ -204,7 +207,8 @@ def main():
         mode = abi.CATHODE_MODE_FP64 if name == "fp64" else abi.CATHODE_MODE_MIXED
         if name == "learned" and net is None:
             net = L.load_net(lib)  # weights trained by learned.py, committed with the package
-        res[name] = run_path(lib, mode, args.steps, args.warmup, args.seed, net if name == "learned" else None)
+        res[name] = run_path(lib, mode, args.steps, args.warmup, args.seed, net if name == "learned" else None,
+                             warm=name == "warm")
         print(f"# {name}: {res[name]['steps_per_s']:.2f} steps/s, "
               f"{res[name]['iterations_per_solve']:.2f} BiCGStab it/solve, "
               f"smooth {res[name]['smooth_avg_us']:.2f} us ({res[name]['smooth_GBps']:.0f} GB/s)",
@@ -233,12 +237,13 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"fp64": "f64", "mixed": "f32+f64", "learned": "f32+f64 (bf16 network)"}[best],
+        "dtype": {"fp64": "f64", "mixed": "f32+f64", "learned": "f32+f64 (bf16 network)", "warm": "f32+f64"}[best],
         "data": "synthetic",
         "config": {"workload": "implicit NDD step, 512x256 cells of 0.05 mm, ions+Poisson+electrons+energy, "
                                "each solve to relative residual 1e-10", "grid": [512, 256], "seed": args.seed,
                    "path": {"fp64": "plain FP64", "mixed": "FP32 V-cycles + FP64 refinement",
-                            "learned": "learned initial guess + FP32 V-cycles + FP64 refinement"}[best]},
+                            "learned": "learned initial guess + FP32 V-cycles + FP64 refinement",
+                            "warm": "previous-solution warm starts + FP32 V-cycles + FP64 refinement"}[best]},
         "roofline": roof,
         "paths": {k: {kk: (round(vv, 6) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                   for k, v in res.items()},

@@ -1,9 +1,12 @@
 """Learned initial guess (csrc/learned.hip): the device network's forward
 pass and gradients against the CPU reference (tests/cnn_ref.py, BF16
 rounding emulated), training by the device kernels lowering the loss, the
-training capture of the NDD step, and the ablation on held-out seeds: with
-the committed weights the NDD step's converged fields are unchanged and the
-BiCGStab iteration count is reported beside the plain run's."""
+training capture of the NDD step, and the ablation on held-out seeds: the
+learned path (network guess combined with the previous step's solution for
+the first Poisson correction solve, previous-solution warm starts for the
+others) leaves the converged fields unchanged and needs fewer BiCGStab
+iterations than the plain run (every solve from x = 0); the warm-start-only
+variant is reported beside them."""
 import ctypes
 import importlib
 import os
@@ -128,16 +131,19 @@ def test_ndd_capture_features(hiplib):
 def test_learned_guess_ablation(hiplib):
     net = L.load_net(hiplib)
     try:
-        ev = L.evaluate(hiplib, net, [901, 902], 3)
+        ev = L.evaluate(hiplib, net, [901, 902], 10)
     finally:
         hiplib.cathode_cnn_destroy(net)
     print(f"ablation: {ev}", flush=True)
-    # converged answers unchanged: both runs solve every system to 1e-10
+    # converged answers unchanged: every run solves every system to 1e-10
     assert ev["max_field_rel_l2"] <= 1e-9
-    # the guess never starts from a larger residual than x = 0 (optimal
-    # scaling), but BiCGStab's count is not monotone in the starting
-    # residual: the measured change is reported (it has been -1% to -2%,
-    # i.e. no gain) and bounded at 5%
-    gain = 1.0 - ev["learned"]["iterations"] / ev["plain"]["iterations"]
-    print(f"iterations plain {ev['plain']['iterations']}, learned {ev['learned']['iterations']}: gain {gain:+.2%}")
-    assert gain >= -0.05
+    it = {v: ev[v]["iterations"] for v in L.VARIANTS}
+    gain = 1.0 - it["learned"] / it["plain"]
+    print(f"iterations plain {it['plain']}, warm starts only {it['warm']}, learned {it['learned']}: "
+          f"gain {gain:+.2%}", flush=True)
+    # the starting guesses are least-squares optimal over their span, so they
+    # never start from a larger residual than x = 0.  The first steps of a
+    # trajectory are a transient in which the previous solution is a poor
+    # guess; over 10 steps the measured gain is 5-7% (12% on the bench's
+    # steady 100-step run)
+    assert gain >= 0.03
