@@ -143,6 +143,26 @@ __device__ inline double block_sum_bcast(double v, double* lds)
     return s;
 }
 
+// M sums at once, returned to every thread; lds holds M * NT / 64 doubles
+template <int NT, int M>
+__device__ inline void block_sum_vec_bcast(double (&v)[M], double* lds)
+{
+#pragma unroll
+    for (int m = 0; m < M; m++) v[m] = wave_sum(v[m]);
+    const int t = lin_tid();
+    if ((t & 63) == 0)
+#pragma unroll
+        for (int m = 0; m < M; m++) lds[(t >> 6) * M + m] = v[m];
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < M; m++) {
+        double s = 0.0;
+        for (int w = 0; w < NT / 64; w++) s += lds[w * M + m];
+        v[m] = s;
+    }
+    __syncthreads();
+}
+
 template <int NT>
 __device__ inline double block_max_bcast(double v, double* lds)
 {

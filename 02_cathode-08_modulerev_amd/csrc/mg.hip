@@ -470,43 +470,158 @@ __global__ void k_lump9(int ni, int nj, int ld, const double* __restrict__ nine,
 }
 
 // r = b - A x, r0 = r, p = v = 0 (restart); partials |r|^2 (slot 0), (r0,r) (slot 1)
-__global__ void k_bicg_restart(int ni, int nj, int ld, const double* __restrict__ aP,
+// A y_q at cell (i, j) of level 0 (5-point operator, couplings to inactive
+// cells already zero); y_q has its own row pitch
+__device__ inline double warm_ay(const WarmSet& ws, int q, int i, int j, size_t k, const double* aP,
+                                 const double* aE, const double* aW, const double* aN, const double* aS)
+{
+    const double* y = ws.y[q];
+    const int L = ws.ld[q];
+    const size_t c = (size_t)i * L + j;
+    return aP[k] * y[c] - (aE[k] * y[c + L] + aW[k] * y[c - L] + aN[k] * y[c + 1] + aS[k] * y[c - 1]);
+}
+
+// partials of the warm start's dot products (layout of WARM_NP, each nb long)
+__global__ void __launch_bounds__(BX * BY) k_warm_dots(int ni, int nj, int ld, const double* __restrict__ aP, const double* __restrict__ aE,
+                            const double* __restrict__ aW, const double* __restrict__ aN,
+                            const double* __restrict__ aS, const double* __restrict__ b,
+                            const unsigned char* __restrict__ act, WarmSet ws, double* __restrict__ part, int nb)
+{
+    __shared__ double lds[WARM_NP * (BX * BY / 64)];
+    CELL_IJ;
+    double d[WARM_NP];
+#pragma unroll
+    for (int m = 0; m < WARM_NP; m++) d[m] = 0.0;
+    if (i <= ni && j <= nj) {
+        const size_t k = (size_t)i * ld + j;
+        if (act[k]) {
+            double ay[WARM_MAX];
+#pragma unroll
+            for (int q = 0; q < WARM_MAX; q++) ay[q] = q < ws.k ? warm_ay(ws, q, i, j, k, aP, aE, aW, aN, aS) : 0.0;
+#pragma unroll
+            for (int q = 0; q < WARM_MAX; q++) {
+                d[q] = ay[q] * b[k];
+#pragma unroll
+                for (int p = 0; p <= q; p++) d[WARM_MAX + warm_gidx(p, q)] = ay[p] * ay[q];
+            }
+            d[WARM_NP - 1] = b[k] * b[k];
+        }
+    }
+    block_sum_vec_bcast<BX * BY, WARM_NP>(d, lds);
+    if (lin_tid() == 0) {
+        const int bid = blockIdx.y * gridDim.x + blockIdx.x;
+#pragma unroll
+        for (int m = 0; m < WARM_NP; m++) part[m * nb + bid] = d[m];
+    }
+}
+
+// the least-squares coefficients of the first n vectors from the summed
+// dot products t: Gaussian elimination on the (symmetric positive
+// semidefinite) Gram matrix without pivoting; a pivot at or below 1e-12
+// times its diagonal entry (a vector in the span of the earlier ones) drops
+// that vector.  Returns |b - A x0|^2 = <b, b> - sum a_q <Ay_q, b>.
+__device__ inline double warm_coeffs(const double* t, int n, double* a)
+{
+    double G[WARM_MAX][WARM_MAX], r[WARM_MAX], d0[WARM_MAX];
+    bool on[WARM_MAX];
+#pragma unroll
+    for (int p = 0; p < WARM_MAX; p++) {
+        r[p] = p < n ? t[p] : 0.0;
+#pragma unroll
+        for (int q = 0; q < WARM_MAX; q++)
+            G[p][q] = (p < n && q < n) ? t[WARM_MAX + (p <= q ? warm_gidx(p, q) : warm_gidx(q, p))]
+                                       : (p == q ? 1.0 : 0.0);
+        d0[p] = G[p][p];
+        on[p] = p < n && d0[p] > 0.0;
+    }
+#pragma unroll
+    for (int p = 0; p < WARM_MAX; p++) {
+        if (on[p] && !(G[p][p] > 1e-12 * d0[p])) on[p] = false;
+        if (!on[p]) {
+#pragma unroll
+            for (int q = 0; q < WARM_MAX; q++) G[p][q] = G[q][p] = 0.0;
+            G[p][p] = 1.0;
+            r[p] = 0.0;
+            continue;
+        }
+#pragma unroll
+        for (int q = p + 1; q < WARM_MAX; q++) {
+            const double f = G[q][p] / G[p][p];
+#pragma unroll
+            for (int c = p; c < WARM_MAX; c++) G[q][c] -= f * G[p][c];
+            r[q] -= f * r[p];
+        }
+    }
+#pragma unroll
+    for (int p = WARM_MAX - 1; p >= 0; p--) {
+        double s = r[p];
+#pragma unroll
+        for (int q = p + 1; q < WARM_MAX; q++) s -= G[p][q] * a[q];
+        a[p] = on[p] ? s / G[p][p] : 0.0;
+    }
+    double red = 0.0;
+    bool ok = true;
+#pragma unroll
+    for (int p = 0; p < WARM_MAX; p++) {
+        ok = ok && isfinite(a[p]);
+        red += p < n ? a[p] * t[p] : 0.0;
+    }
+    if (!ok) {
+#pragma unroll
+        for (int p = 0; p < WARM_MAX; p++) a[p] = 0.0;
+        red = 0.0;
+    }
+    return t[WARM_NP - 1] - red;
+}
+
+__global__ void __launch_bounds__(BX * BY) k_bicg_restart(int ni, int nj, int ld, const double* __restrict__ aP,
                                const double* __restrict__ aE, const double* __restrict__ aW,
                                const double* __restrict__ aN, const double* __restrict__ aS,
                                const double* __restrict__ nine, size_t n0,
                                const double* __restrict__ b, const double* __restrict__ x,
                                const unsigned char* __restrict__ act, double* __restrict__ r,
                                double* __restrict__ r0, double* __restrict__ p, double* __restrict__ v,
-                               double* __restrict__ part, int nb, double* __restrict__ xout,
-                               const double* __restrict__ wy, const double* __restrict__ wpart, int wnb)
+                               double* __restrict__ part, int nb, double* __restrict__ xout, WarmSet ws,
+                               const double* __restrict__ wpart, int wnb)
 {
     __shared__ double lds[8];
+    __shared__ double wl[WARM_NP * (BX * BY / 64)];
     CELL_IJ;
-    // warm start x = alpha y, alpha = <A y, b> / <A y, A y> from the caller's
-    // partials (every workgroup sums them in the same order)
-    double alpha = 0.0;
-    if (wy) {
-        double d1 = 0.0, d2 = 0.0;
-        for (int q = lin_tid(); q < wnb; q += BX * BY) {
-            d1 += wpart[q];
-            d2 += wpart[wnb + q];
+    // warm start x = sum a_q y_q from the caller's partials (every workgroup
+    // sums them in the same order and solves the same small system)
+    double a[WARM_MAX] = {0.0, 0.0, 0.0, 0.0};
+    if (ws.k > 0) {
+        double t[WARM_NP];
+#pragma unroll
+        for (int m = 0; m < WARM_NP; m++) {
+            double s = 0.0;
+            for (int q = lin_tid(); q < wnb; q += BX * BY) s += wpart[m * wnb + q];
+            t[m] = s;
         }
-        d1 = block_sum_bcast<BX * BY>(d1, lds);
-        d2 = block_sum_bcast<BX * BY>(d2, lds);
-        const double al = d2 > 0.0 ? d1 / d2 : 0.0;
-        alpha = isfinite(al) ? al : 0.0;
+        block_sum_vec_bcast<BX * BY, WARM_NP>(t, wl);
+        const double res = warm_coeffs(t, ws.k, a);
+        if (ws.report && lin_tid() == 0 && blockIdx.x == 0 && blockIdx.y == 0) {
+            double a2[WARM_MAX];
+            const double res_less = warm_coeffs(t, ws.k - 1, a2);
+            ws.report[0] = res_less > 0.0 ? fmax(res, 0.0) / res_less : 1.0;
+        }
     }
     double rr = 0.0;
     if (i <= ni && j <= nj) {
         const size_t k = (size_t)i * ld + j;
         double q = 0.0;
-        if (wy) {
-            // x = alpha y on the interior (y has zero ghosts), r = b - alpha A y
-            xout[k] = alpha * wy[k];
+        if (ws.k > 0) {
+            // x = sum a_q y_q on the interior (the y_q have zero ghosts), r = b - A x
+            double x0 = 0.0, ax = 0.0;
+#pragma unroll
+            for (int m = 0; m < WARM_MAX; m++)
+                if (m < ws.k) {
+                    x0 += a[m] * ws.y[m][(size_t)i * ws.ld[m] + j];
+                    if (act[k]) ax += a[m] * warm_ay(ws, m, i, j, k, aP, aE, aW, aN, aS);
+                }
+            xout[k] = x0;
             if (act[k]) {
-                const double ay =
-                    aP[k] * wy[k] - offdiag(aE, aW, aN, aS, nine, n0, k, ld, [&](size_t m) { return wy[m]; });
-                q = b[k] - alpha * ay;
+                q = b[k] - ax;
                 rr = q * q;
             }
         } else if (xout) {
@@ -1243,11 +1358,17 @@ int MGSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_sta
         }
     }
     // without a kept guess the first restart zeroes x (interior; ghosts stay
-    // 0), or sets the caller's warm start alpha y
-    const double* wy = keep_guess ? nullptr : warm_y;
-    bool zero_guess = !keep_guess && !wy;
+    // 0), or sets the caller's warm start (dot products here, after setup)
+    WarmSet ws = (keep_guess || use_nine) ? WarmSet() : warm;
+    warm = WarmSet();
+    bool zero_guess = !keep_guess && ws.k == 0;
     keep_guess = false;
-    warm_y = nullptr;
+    if (ws.k > 0) {
+        wpart.alloc((size_t)WARM_NP * nb + 64);
+        hipLaunchKernelGGL(k_warm_dots, g, blk, 0, s, F.ni, F.nj, F.ld, F.aP, F.aE, F.aW, F.aN, F.aS, bsol.p, act0.p,
+                           ws, wpart.p, nb);
+        CATHODE_LAUNCH_CHECK();
+    }
     double* r = kv.p;
     double* r0 = r + n0;
     double* p = r0 + n0;
@@ -1346,9 +1467,9 @@ int MGSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_sta
         if (restart) {
             hipLaunchKernelGGL(k_bicg_restart, g, blk, 0, s, F.ni, F.nj, F.ld, F.aP, F.aE, F.aW, F.aN, F.aS, n9,
                                n0, bsol.p, xa, act0.p, r, r0, pa, v, part3, nb,
-                               (zero_guess || wy) ? xa : nullptr, wy, warm_part, warm_np);
+                               (zero_guess || ws.k > 0) ? xa : nullptr, ws, (const double*)wpart.p, nb);
             zero_guess = false;
-            wy = nullptr;
+            ws = WarmSet();
             // the fused path sums these partials in its first down kernel
             if (!fused) hipLaunchKernelGGL(k_bicg_finish, dim3(1), dim3(256), 0, s, part3, nb, scal.p, 1, 0.0);
         }
@@ -1440,7 +1561,7 @@ int MGSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_sta
     }
     if (maxITR <= 0) {
         // no iteration: x = 0, relative residual 1 (0 when b = 0)
-        if (zero_guess || wy) CATHODE_HIP(hipMemsetAsync(xsol.p, 0, n0 * sizeof(double), s));
+        if (zero_guess || ws.k > 0) CATHODE_HIP(hipMemsetAsync(xsol.p, 0, n0 * sizeof(double), s));
         fetch_host(h_scal, scal.p, SC_COUNT, s);
         CATHODE_HIP(hipStreamSynchronize(s));
         rel = h_scal[SC_BB] > 0.0 ? 1.0 : 0.0;

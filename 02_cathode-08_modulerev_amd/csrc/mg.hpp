@@ -105,16 +105,10 @@ public:
     // sol() (interior cells; ghosts 0) instead; the flag clears itself.
     int solve(int maxITR, double tol, int mode, hipStream_t s, cathode_status* st);
     bool keep_guess = false;
-    // Warm start for the next solve: x0 = alpha y with alpha = <A y, b> /
-    // <A y, A y>, from partials part[0..np) of <A y, b> and part[np..2np) of
-    // <A y, A y> that the caller has computed on the same stream (y: level-0
-    // pitch, zero ghosts).  Ignored when keep_guess is set.
-    void set_warm_start(const double* y, const double* part, int np)
-    {
-        warm_y = y;
-        warm_part = part;
-        warm_np = np;
-    }
+    // Warm start for the next solve (ignored when keep_guess is set or on
+    // the nine-point path): the least-squares combination of the given
+    // vectors, formed after the setup of that solve.
+    void set_warm_start(const WarmSet& ws) { warm = ws; }
     // Hands the solution buffer to the caller in exchange for buf (same
     // size, its own allocation, zero ghosts): no copy.  Kernels already
     // launched with the old sol() pointer still read the solution.
@@ -125,9 +119,8 @@ public:
         return old;
     }
     bool act_dirty = true;
-    const double* warm_y = nullptr;
-    const double* warm_part = nullptr;
-    int warm_np = 0;
+    WarmSet warm;
+    DevBuf<double> wpart;  // warm-start partials
 
     MGProfile prof;  // dispatch events of the finest-level down kernel
 

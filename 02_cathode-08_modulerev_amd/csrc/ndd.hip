@@ -737,123 +737,6 @@ __global__ void k_learn_features(NddView V, const double* __restrict__ b, const 
     feat[2 * plane + p] = (in && xprev) ? xprev[(size_t)i * lx + j] * sx : 0.0;
 }
 
-// Initial guess of the first Poisson correction solve from the network's
-// output g (pitch V.ld) and the previous step's solution y of that solve
-// (pitch ld0, nullptr before the first step): x0 = a1 y + a2 g with (a1, a2)
-// minimising |b - A (a1 y + a2 g)| over their span (never a larger starting
-// residual than either term alone or x = 0).  Partials, in this order:
-// <Ag, b>, <Ag, Ag>, <Ay, b>, <Ay, Ay>, <Ay, Ag>, <b, b>.
-__global__ void k_guess_dots(NddView V, const double* __restrict__ aP, const double* __restrict__ aE,
-                             const double* __restrict__ aW, const double* __restrict__ aN,
-                             const double* __restrict__ aS, const double* __restrict__ b,
-                             const double* __restrict__ g, const double* __restrict__ y, double* __restrict__ part,
-                             int nb)
-{
-    __shared__ double lds[8];
-    THREAD_IJ(1, 1);
-    double d[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    if (i <= V.ni && j <= V.nj) {
-        const size_t k = IX0(i, j);
-        d[5] = b[k] * b[k];
-        const double ag = aP[k] * g[IX(i, j)] - aE[k] * g[IX(i + 1, j)] - aW[k] * g[IX(i - 1, j)] -
-                          aN[k] * g[IX(i, j + 1)] - aS[k] * g[IX(i, j - 1)];
-        d[0] = ag * b[k];
-        d[1] = ag * ag;
-        if (y) {
-            const double ay = aP[k] * y[k] - aE[k] * y[IX0(i + 1, j)] - aW[k] * y[IX0(i - 1, j)] -
-                              aN[k] * y[IX0(i, j + 1)] - aS[k] * y[IX0(i, j - 1)];
-            d[2] = ay * b[k];
-            d[3] = ay * ay;
-            d[4] = ay * ag;
-        }
-    }
-    const int bid = blockIdx.y * gridDim.x + blockIdx.x;
-#pragma unroll
-    for (int q = 0; q < 6; q++) {
-        const double t = block_sum<NB_X * NB_Y>(d[q], lds);
-        if (lin_tid() == 0) part[q * nb + bid] = t;
-    }
-}
-
-// out[2] = a2 (network), out[3] = 1 when the guess is used, out[5] = a1
-// (previous solution).  host[0] (pinned memory, read by the host after a
-// later synchronisation): the network's residual ratio |b - A x0|^2 /
-// |b - a A y|^2, the starting residual with the network's term over the one
-// with the previous solution alone (1: the network adds nothing).
-__global__ void k_guess_alpha(const double* __restrict__ part, int nb, double* __restrict__ out,
-                              double* __restrict__ host)
-{
-    __shared__ double lds[8];
-    double t[6];
-#pragma unroll
-    for (int q = 0; q < 6; q++) {
-        double a = 0.0;
-        for (int k = threadIdx.x; k < nb; k += blockDim.x) a += part[q * nb + k];
-        t[q] = block_sum<256>(a, lds);
-    }
-    if (threadIdx.x == 0) {
-        const double c2 = t[0], a22 = t[1], c1 = t[2], a11 = t[3], a12 = t[4];
-        double a1 = 0.0, a2 = 0.0;
-        const double det = a11 * a22 - a12 * a12;
-        if (a11 > 0.0 && a22 > 0.0 && det > 1e-12 * a11 * a22) {
-            a1 = (c1 * a22 - c2 * a12) / det;
-            a2 = (a11 * c2 - a12 * c1) / det;
-        } else if (a11 > 0.0 && (a22 <= 0.0 || c1 * c1 / a11 >= c2 * c2 / a22)) {
-            a1 = c1 / a11;  // one term only: the one that lowers the residual more
-        } else if (a22 > 0.0) {
-            a2 = c2 / a22;
-        }
-        if (!isfinite(a1) || !isfinite(a2)) a1 = a2 = 0.0;
-        out[2] = a2;
-        out[3] = (a1 != 0.0 || a2 != 0.0) ? 1.0 : 0.0;
-        out[5] = a1;
-        if (host) {
-            const double bb = t[5];
-            const double ry = a11 > 0.0 ? bb - c1 * c1 / a11 : bb;  // previous solution alone
-            const double rb = bb - (c1 * a1 + c2 * a2);              // both terms
-            host[0] = ry > 0.0 ? fmax(rb, 0.0) / ry : 1.0;
-        }
-    }
-}
-
-// solver initial guess x (pitch ld0) = a1 y + a2 g on interior cells, 0 on ghosts
-__global__ void k_guess_set(NddView V, const double* __restrict__ g, const double* __restrict__ y,
-                            const double* __restrict__ alpha, double* __restrict__ x)
-{
-    THREAD_IJ(0, 0);
-    if (i > V.ni + 1 || j > V.nj + 1) return;
-    const bool in = i >= 1 && i <= V.ni && j >= 1 && j <= V.nj;
-    const double a2 = alpha[0], a1 = alpha[3];
-    x[IX0(i, j)] = in ? a2 * g[IX(i, j)] + (y ? a1 * y[IX0(i, j)] : 0.0) : 0.0;
-}
-
-// Warm start from the previous step's solution y of the same solve (pitch
-// ld0): partials of <A y, b> and <A y, A y>, then x = alpha y with alpha =
-// <A y, b> / <A y, A y> minimising |b - alpha A y| (0 when y does not help).
-__global__ void k_warm_dots(NddView V, const double* __restrict__ aP, const double* __restrict__ aE,
-                            const double* __restrict__ aW, const double* __restrict__ aN,
-                            const double* __restrict__ aS, const double* __restrict__ b,
-                            const double* __restrict__ y, double* __restrict__ part, int nb)
-{
-    __shared__ double lds[8];
-    THREAD_IJ(1, 1);
-    double d1 = 0.0, d2 = 0.0;
-    if (i <= V.ni && j <= V.nj) {
-        const size_t k = IX0(i, j);
-        const double ay = aP[k] * y[k] - aE[k] * y[IX0(i + 1, j)] - aW[k] * y[IX0(i - 1, j)] -
-                          aN[k] * y[IX0(i, j + 1)] - aS[k] * y[IX0(i, j - 1)];
-        d1 = ay * b[k];
-        d2 = ay * ay;
-    }
-    const double t1 = block_sum<NB_X * NB_Y>(d1, lds);
-    const double t2 = block_sum<NB_X * NB_Y>(d2, lds);
-    if (lin_tid() == 0) {
-        const int bid = blockIdx.y * gridDim.x + blockIdx.x;
-        part[bid] = t1;
-        part[nb + bid] = t2;
-    }
-}
-
 // previous-step solution (pitch V.ld) and, for training capture, the
 // target x / rms(x)
 __global__ void k_learn_store(NddView V, const double* __restrict__ x, const double* __restrict__ rms,
@@ -891,9 +774,11 @@ struct NddHandle {
     bool have_xprev = false;
     // learned mode: every solve starts from its own previous-step solution
     // (slot = equation x PC iteration), scaled by the least-squares alpha
-    DevBuf<double> wbuf[8];  // one allocation each: they trade places with the solver's solution buffer
-    DevBuf<double> wpart, wscal;
-    bool wvalid[8] = {false, false, false, false, false, false, false, false};
+    // the last WARM_HIST solutions of each solve, newest first; one
+    // allocation each: they trade places with the solver's solution buffer
+    static constexpr int WARM_HIST = 3;
+    DevBuf<double> wbuf[8][WARM_HIST];
+    int nhist[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     double* cap_feat = nullptr;       // capture for training (device, caller-owned)
     double* cap_target = nullptr;
     int guesses = 0, guesses_used = 0;
@@ -1015,7 +900,7 @@ static int warm_slot(int eq, int ncount) { return eq * 2 + (ncount >= 2 ? 1 : 0)
 constexpr double NET_USEFUL = 0.9;  // residual ratio below which the network's term counts as useful
 constexpr int NET_PROBE = 16;       // while it is not, it is tried once every NET_PROBE steps
 
-static void warm_guess(NddHandle& H, int slot);
+static void warm_guess(NddHandle& H, int slot, const double* g = nullptr);
 
 static void learned_guess(NddHandle& H, bool net)
 {
@@ -1030,8 +915,8 @@ static void learned_guess(NddHandle& H, bool net)
     // the previous step's solution of this solve: the learned mode's warm
     // slot (pitch ld0), else the copy learned_store keeps (pitch ld)
     const int slot = warm_slot(EQ_POISSON, 1);
-    const bool from_slot = H.p.learned && H.wvalid[slot];
-    const double* xp = from_slot ? (const double*)H.wbuf[slot].p
+    const bool from_slot = H.p.learned && H.nhist[slot] > 0;
+    const double* xp = from_slot ? (const double*)H.wbuf[slot][0].p
                                  : (H.have_xprev ? (const double*)H.xprev.p : (const double*)nullptr);
     const int lxp = from_slot ? V.ld0 : V.ld;
     hipLaunchKernelGGL(k_learn_sumsq, g1, gblk(), 0, H.s, V, (const double*)H.mg.rhs(), V.ld0, xp, lxp, H.lpart.p,
@@ -1049,16 +934,9 @@ static void learned_guess(NddHandle& H, bool net)
         return;
     }
     cnn_infer(H.cnn, V.ni, V.nj, H.lfeat.p, H.lguess.p, H.s);
-    const double* y = from_slot ? xp : (const double*)nullptr;
-    hipLaunchKernelGGL(k_guess_dots, g1, gblk(), 0, H.s, V, (const double*)H.mg.aP(), (const double*)H.mg.aE(),
-                       (const double*)H.mg.aW(), (const double*)H.mg.aN(), (const double*)H.mg.aS(),
-                       (const double*)H.mg.rhs(), (const double*)H.lguess.p, y, H.lpart.p, H.nb);
-    hipLaunchKernelGGL(k_guess_alpha, dim3(1), dim3(256), 0, H.s, (const double*)H.lpart.p, H.nb, H.lscal.p,
-                       H.h_scal + 8);
-    hipLaunchKernelGGL(k_guess_set, g2, gblk(), 0, H.s, V, (const double*)H.lguess.p, y, (const double*)H.lscal.p + 2,
-                       H.mg.sol());
-    CATHODE_LAUNCH_CHECK();
-    H.mg.keep_guess = true;
+    // the network's output joins the solve's history in the least-squares
+    // combination; the device reports what it adds (h_scal[8])
+    warm_guess(H, slot, H.lguess.p);
     H.guesses++;
     H.gate_pending = true;
 }
@@ -1080,27 +958,38 @@ static void learned_store(NddHandle& H)
 }
 
 
-// learned mode: x0 = alpha y, y the previous step's solution of this solve;
-// the dot products here, alpha and x0 in the solver's first restart kernel
-static void warm_guess(NddHandle& H, int slot)
+// learned mode: x0 = the least-squares combination of this solve's last
+// solutions (and of the network's guess g, pitch V.ld, when given; then its
+// contribution is reported to h_scal[8])
+static void warm_guess(NddHandle& H, int slot, const double* g)
 {
-    if (!H.p.learned || !H.wvalid[slot]) return;
-    const NddView& V = H.V;
-    const double* y = H.wbuf[slot].p;
-    hipLaunchKernelGGL(k_warm_dots, ggrid(V.ni, V.nj), gblk(), 0, H.s, V, (const double*)H.mg.aP(),
-                       (const double*)H.mg.aE(), (const double*)H.mg.aW(), (const double*)H.mg.aN(),
-                       (const double*)H.mg.aS(), (const double*)H.mg.rhs(), y, H.wpart.p, H.nb);
-    CATHODE_LAUNCH_CHECK();
-    H.mg.set_warm_start(y, H.wpart.p, H.nb);
+    if (!H.p.learned) return;
+    WarmSet ws;
+    for (int q = 0; q < H.nhist[slot]; q++) {
+        ws.y[ws.k] = H.wbuf[slot][q].p;
+        ws.ld[ws.k] = H.V.ld0;
+        ws.k++;
+    }
+    if (g) {
+        ws.y[ws.k] = g;
+        ws.ld[ws.k] = H.V.ld;
+        ws.k++;
+        ws.report = H.h_scal + 8;
+    }
+    if (ws.k > 0) H.mg.set_warm_start(ws);
 }
 
 // after the kernels that read the solution have been launched: the slot
-// takes the solution buffer, the solver the slot's old one (no copy)
+// takes the solution buffer as its newest entry, the solver its oldest
+// buffer (no copy)
 static void warm_store(NddHandle& H, int slot)
 {
     if (!H.p.learned) return;
-    H.wbuf[slot].p = H.mg.exchange_solution(H.wbuf[slot].p);
-    H.wvalid[slot] = true;
+    auto& h = H.wbuf[slot];
+    double* oldest = h[NddHandle::WARM_HIST - 1].p;
+    for (int q = NddHandle::WARM_HIST - 1; q > 0; q--) h[q].p = h[q - 1].p;
+    h[0].p = H.mg.exchange_solution(oldest);
+    H.nhist[slot] = std::min(H.nhist[slot] + 1, NddHandle::WARM_HIST);
 }
 
 static void poisson_routine(NddHandle& H, bool report)
@@ -1335,12 +1224,11 @@ int cathode_ndd_create(const cathode_ndd_params* p, void** handle)
         H->part.alloc(6 * H->nb + 64);
         H->scal.alloc(16);
         if (p->learned) {
-            for (auto& wb : H->wbuf) {
-                wb.alloc((size_t)n0);
-                CATHODE_HIP(hipMemset(wb.p, 0, (size_t)n0 * sizeof(double)));  // zero ghosts and padding
-            }
-            H->wpart.alloc(6 * H->nb + 64);
-            H->wscal.alloc(8);
+            for (auto& slot : H->wbuf)
+                for (auto& wb : slot) {
+                    wb.alloc((size_t)n0);
+                    CATHODE_HIP(hipMemset(wb.p, 0, (size_t)n0 * sizeof(double)));  // zero ghosts and padding
+                }
             CATHODE_HIP(hipDeviceSynchronize());  // the memsets run on the null stream
         }
         CATHODE_HIP(hipHostMalloc(&H->h_scal, 16 * sizeof(double), hipHostMallocDefault));
@@ -1400,7 +1288,7 @@ int cathode_ndd_set_state(void* handle, const cathode_ndd_fields* host)
         }
         cathode_ndd_fields f = *host;
         H->have_xprev = false;  // a new state: no previous Poisson correction
-        for (bool& v : H->wvalid) v = false;
+        for (int& v : H->nhist) v = 0;
         H->net_useful = true;
         H->gate_pending = false;
         H->net_wait = 0;

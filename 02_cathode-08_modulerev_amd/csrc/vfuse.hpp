@@ -21,6 +21,22 @@ namespace cathode {
 // scalar: pro = 1 takes rho1 = (r0, r) from them (stored to sc[s_rho1] by
 // workgroup 0; s_rho then names the previous iteration's slot), pro = 2
 // takes (r0, v) and al = rho1 / (r0, v) (stored to sc[s_r0v], sc[s_al]).
+// Warm start of a solve: x0 = sum_q a_q y_q over up to WARM_MAX vectors
+// (each with its own row pitch, zero ghosts), (a_q) minimising |b - A x0|
+// over their span.  report (pinned host memory, optional): the residual
+// ratio |b - A x0|^2 / |b - A x0'|^2, x0' the optimum without the last
+// vector, i.e. what the last vector adds.
+constexpr int WARM_MAX = 4;
+// dot products: <Ay_q, b> at q, <Ay_p, Ay_q> (p <= q) at WARM_MAX + warm_gidx(p, q), <b, b> last
+constexpr int WARM_NP = WARM_MAX + WARM_MAX * (WARM_MAX + 1) / 2 + 1;
+__host__ __device__ constexpr int warm_gidx(int p, int q) { return p * WARM_MAX - p * (p - 1) / 2 + (q - p); }
+struct WarmSet {
+    int k = 0;
+    const double* y[WARM_MAX] = {nullptr, nullptr, nullptr, nullptr};
+    int ld[WARM_MAX] = {0, 0, 0, 0};
+    double* report = nullptr;
+};
+
 struct VFuse {
     int pro = 0, epi = 0, first = 0, mixed = 0, want_tt = 0;
     int s_s32 = 0, s_scale = 0, s_rho1 = 0, s_rho = 0, s_al = 0, s_om = 0, s_r0v = 0;
