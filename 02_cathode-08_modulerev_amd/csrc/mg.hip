@@ -301,6 +301,17 @@ __global__ void __launch_bounds__(CINV_NT) k_coarse_inverse(CoarseInv ci, const 
     coarse_inverse_block(ci, sc[3], rows, red);
 }
 
+// A y_q at cell (i, j) of level 0 (5-point operator, couplings to inactive
+// cells already zero); y_q has its own row pitch
+__device__ inline double warm_ay(const WarmSet& ws, int q, int i, int j, size_t k, const double* aP,
+                                 const double* aE, const double* aW, const double* aN, const double* aS)
+{
+    const double* y = ws.y[q];
+    const int L = ws.ld[q];
+    const size_t c = (size_t)i * L + j;
+    return aP[k] * y[c] - (aE[k] * y[c + L] + aW[k] * y[c - L] + aN[k] * y[c + 1] + aS[k] * y[c - 1]);
+}
+
 // The setup after k_l0_prepare in one launch: the setup scalars (every
 // workgroup sums the prepare partials in the same order, workgroup 0 stores
 // them), the level-0 mass and FP32 copy, and the first nf coarsenings.
@@ -322,6 +333,11 @@ struct SetupLevels {
     int nf, ri, rj;
     int pi[SETUP_FUSE_MAX + 1], pj[SETUP_FUSE_MAX + 1];  // level-l block = (ri pi[l]) x (rj pj[l]) cells
     CoarsenArgs L[SETUP_FUSE_MAX];
+    // warm start of the coming solve: the dot products of its vectors, one
+    // partial per workgroup (layout of WARM_NP, gridDim.x * gridDim.y long)
+    WarmSet ws;
+    const double* b;
+    double* wpart;
 };
 
 __global__ void __launch_bounds__(SETUP_NT) k_setup_levels(SetupLevels a, double* __restrict__ sc, int mixed)
@@ -343,7 +359,11 @@ __global__ void __launch_bounds__(SETUP_NT) k_setup_levels(SetupLevels a, double
         sc[2] = bb > 0.0 ? sqrt(a.nact / bb) : 1.0;   // mixed input scale
         sc[3] = s32;
     }
-    // level 0: mass part m = w_j min(s) over the 5-point neighbourhood, FP32 copy
+    // level 0: mass part m = w_j min(s) over the 5-point neighbourhood, FP32
+    // copy, and the warm start's dot products
+    double wd[WARM_NP];
+#pragma unroll
+    for (int q = 0; q < WARM_NP; q++) wd[q] = 0.0;
     {
         const int rows = a.ri * a.pi[0], cols = a.rj * a.pj[0];
         const int i0 = 1 + (int)blockIdx.y * rows, j0 = 1 + (int)blockIdx.x * cols;
@@ -362,12 +382,35 @@ __global__ void __launch_bounds__(SETUP_NT) k_setup_levels(SetupLevels a, double
                 a.m[k] = 0.0;
                 continue;
             }
+            if (a.ws.k > 0) {
+                double ay[WARM_MAX];
+#pragma unroll
+                for (int q = 0; q < WARM_MAX; q++)
+                    ay[q] = q < a.ws.k ? warm_ay(a.ws, q, i, j, k, a.aP, a.aE, a.aW, a.aN, a.aS) : 0.0;
+                const double bk = a.b[k];
+#pragma unroll
+                for (int q = 0; q < WARM_MAX; q++) {
+                    wd[q] += ay[q] * bk;
+#pragma unroll
+                    for (int p = 0; p <= q; p++) wd[WARM_MAX + warm_gidx(p, q)] += ay[p] * ay[q];
+                }
+                wd[WARM_NP - 1] += bk * bk;
+            }
             double v = a.s[k];
             if (a.act[k + a.ld]) v = fmin(v, a.s[k + a.ld]);
             if (a.act[k - a.ld]) v = fmin(v, a.s[k - a.ld]);
             if (a.act[k + 1]) v = fmin(v, a.s[k + 1]);
             if (a.act[k - 1]) v = fmin(v, a.s[k - 1]);
             a.m[k] = a.w[j] * v;
+        }
+    }
+    if (a.ws.k > 0) {
+        __shared__ double wl[WARM_NP * (SETUP_NT / 64)];
+        block_sum_vec_bcast<SETUP_NT, WARM_NP>(wd, wl);
+        if (tid == 0) {
+            const int nsb = (int)(gridDim.x * gridDim.y), bid = (int)(blockIdx.y * gridDim.x + blockIdx.x);
+#pragma unroll
+            for (int q = 0; q < WARM_NP; q++) a.wpart[q * nsb + bid] = wd[q];
         }
     }
     for (int l = 0; l < a.nf; l++) {
@@ -470,50 +513,6 @@ __global__ void k_lump9(int ni, int nj, int ld, const double* __restrict__ nine,
 }
 
 // r = b - A x, r0 = r, p = v = 0 (restart); partials |r|^2 (slot 0), (r0,r) (slot 1)
-// A y_q at cell (i, j) of level 0 (5-point operator, couplings to inactive
-// cells already zero); y_q has its own row pitch
-__device__ inline double warm_ay(const WarmSet& ws, int q, int i, int j, size_t k, const double* aP,
-                                 const double* aE, const double* aW, const double* aN, const double* aS)
-{
-    const double* y = ws.y[q];
-    const int L = ws.ld[q];
-    const size_t c = (size_t)i * L + j;
-    return aP[k] * y[c] - (aE[k] * y[c + L] + aW[k] * y[c - L] + aN[k] * y[c + 1] + aS[k] * y[c - 1]);
-}
-
-// partials of the warm start's dot products (layout of WARM_NP, each nb long)
-__global__ void __launch_bounds__(BX * BY) k_warm_dots(int ni, int nj, int ld, const double* __restrict__ aP, const double* __restrict__ aE,
-                            const double* __restrict__ aW, const double* __restrict__ aN,
-                            const double* __restrict__ aS, const double* __restrict__ b,
-                            const unsigned char* __restrict__ act, WarmSet ws, double* __restrict__ part, int nb)
-{
-    __shared__ double lds[WARM_NP * (BX * BY / 64)];
-    CELL_IJ;
-    double d[WARM_NP];
-#pragma unroll
-    for (int m = 0; m < WARM_NP; m++) d[m] = 0.0;
-    if (i <= ni && j <= nj) {
-        const size_t k = (size_t)i * ld + j;
-        if (act[k]) {
-            double ay[WARM_MAX];
-#pragma unroll
-            for (int q = 0; q < WARM_MAX; q++) ay[q] = q < ws.k ? warm_ay(ws, q, i, j, k, aP, aE, aW, aN, aS) : 0.0;
-#pragma unroll
-            for (int q = 0; q < WARM_MAX; q++) {
-                d[q] = ay[q] * b[k];
-#pragma unroll
-                for (int p = 0; p <= q; p++) d[WARM_MAX + warm_gidx(p, q)] = ay[p] * ay[q];
-            }
-            d[WARM_NP - 1] = b[k] * b[k];
-        }
-    }
-    block_sum_vec_bcast<BX * BY, WARM_NP>(d, lds);
-    if (lin_tid() == 0) {
-        const int bid = blockIdx.y * gridDim.x + blockIdx.x;
-#pragma unroll
-        for (int m = 0; m < WARM_NP; m++) part[m * nb + bid] = d[m];
-    }
-}
 
 // the least-squares coefficients of the first n vectors from the summed
 // dot products t: Gaussian elimination on the (symmetric positive
@@ -1006,7 +1005,7 @@ CoarsenArgs MGSolver::coarsen_args(int l, const double* w)
 
 // Builds the hierarchy for the level-0 system the caller filled in, plus the
 // setup scalars (|b|^2, FP32 scales): 3 launches while lt <= 3.
-void MGSolver::setup(int mode, hipStream_t s)
+void MGSolver::setup(int mode, hipStream_t s, const WarmSet& ws)
 {
     Level<double>& F = L64[0];
     const size_t n0 = level_off[1] - level_off[0];
@@ -1049,8 +1048,15 @@ void MGSolver::setup(int mode, hipStream_t s)
         a.ri = std::max(1, 16 / a.pi[0]);
         a.rj = std::max(1, 32 / a.pj[0]);
         const int cni = L64[nf].ni, cnj = L64[nf].nj;
-        hipLaunchKernelGGL(k_setup_levels, dim3(cdiv(cnj, a.rj), cdiv(cni, a.ri)), dim3(SETUP_NT), 0, s, a, scal.p,
-                           mixed);
+        const dim3 sg(cdiv(cnj, a.rj), cdiv(cni, a.ri));
+        a.ws = ws;
+        a.b = bsol.p;
+        if (ws.k > 0) {
+            warm_nb = (int)(sg.x * sg.y);
+            wpart.alloc((size_t)WARM_NP * warm_nb + 64);
+            a.wpart = wpart.p;
+        }
+        hipLaunchKernelGGL(k_setup_levels, sg, dim3(SETUP_NT), 0, s, a, scal.p, mixed);
         CATHODE_LAUNCH_CHECK();
     }
     // further fine levels: one launch each; the rest in one workgroup
@@ -1345,7 +1351,11 @@ int MGSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_sta
     const int nb = cdiv(F.nj, BX) * cdiv(F.ni, BY);
     const int mixed = mode == CATHODE_MODE_MIXED ? 1 : 0;
     const dim3 g = grid2(F.ni, F.nj), blk(BX, BY);
-    setup(mode, s);
+    // the caller's warm start (not with a kept guess or the nine-point
+    // operator): its dot products are formed by the setup launch
+    WarmSet ws = (keep_guess || use_nine) ? WarmSet() : warm;
+    warm = WarmSet();
+    setup(mode, s, ws);
     {
         const int q = mixed ? 1 : 0;
         const char* force = std::getenv("CATHODE_TILE");
@@ -1358,17 +1368,9 @@ int MGSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_sta
         }
     }
     // without a kept guess the first restart zeroes x (interior; ghosts stay
-    // 0), or sets the caller's warm start (dot products here, after setup)
-    WarmSet ws = (keep_guess || use_nine) ? WarmSet() : warm;
-    warm = WarmSet();
+    // 0), or sets the caller's warm start
     bool zero_guess = !keep_guess && ws.k == 0;
     keep_guess = false;
-    if (ws.k > 0) {
-        wpart.alloc((size_t)WARM_NP * nb + 64);
-        hipLaunchKernelGGL(k_warm_dots, g, blk, 0, s, F.ni, F.nj, F.ld, F.aP, F.aE, F.aW, F.aN, F.aS, bsol.p, act0.p,
-                           ws, wpart.p, nb);
-        CATHODE_LAUNCH_CHECK();
-    }
     double* r = kv.p;
     double* r0 = r + n0;
     double* p = r0 + n0;
@@ -1467,7 +1469,7 @@ int MGSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_sta
         if (restart) {
             hipLaunchKernelGGL(k_bicg_restart, g, blk, 0, s, F.ni, F.nj, F.ld, F.aP, F.aE, F.aW, F.aN, F.aS, n9,
                                n0, bsol.p, xa, act0.p, r, r0, pa, v, part3, nb,
-                               (zero_guess || ws.k > 0) ? xa : nullptr, ws, (const double*)wpart.p, nb);
+                               (zero_guess || ws.k > 0) ? xa : nullptr, ws, (const double*)wpart.p, warm_nb);
             zero_guess = false;
             ws = WarmSet();
             // the fused path sums these partials in its first down kernel

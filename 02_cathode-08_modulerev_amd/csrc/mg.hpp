@@ -120,7 +120,8 @@ public:
     }
     bool act_dirty = true;
     WarmSet warm;
-    DevBuf<double> wpart;  // warm-start partials
+    DevBuf<double> wpart;  // warm-start partials, warm_nb per dot product
+    int warm_nb = 0;
 
     MGProfile prof;  // dispatch events of the finest-level down kernel
 
@@ -145,7 +146,7 @@ public:
     void autotune(int mixed, hipStream_t s);
 
 private:
-    void setup(int mode, hipStream_t s);
+    void setup(int mode, hipStream_t s, const WarmSet& ws);
     struct CoarsenArgs coarsen_args(int l, const double* w);
     template <class T>
     void vcycle(Level<T>* L, hipStream_t s, const VFuse& fdown = VFuse(), const VFuse& fup = VFuse());
