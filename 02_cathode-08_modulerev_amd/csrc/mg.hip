@@ -1463,6 +1463,9 @@ int MGSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_sta
     // graph replay of the later iterations: CATHODE_GRAPHS=1 (off by
     // default: on ROCm 7 one hipGraphLaunch of the ~14 nodes costs more than
     // launching them one by one)
+    const int unchecked = std::min(expected_its - 1, maxITR - 1);
+    expected_its = 0;
+    bool checked = false;
     const char* genv = std::getenv("CATHODE_GRAPHS");
     const bool graphs_on = fused && !prof.enabled && genv && genv[0] == '1';
     while (it < maxITR) {
@@ -1528,6 +1531,9 @@ int MGSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_sta
         }
         CATHODE_LAUNCH_CHECK();
         std::swap(xa, xb);
+        // the caller's expected iteration count: the iterations before the
+        // last expected one run without a check (no round trip)
+        if (it < unchecked) continue;
         // one host round trip per iteration: the x update stored its partials
         // and |b|^2 and the side slot in h_scal; |r|^2 and |b - A x|^2 are
         // summed here in a fixed order
@@ -1543,7 +1549,9 @@ int MGSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_sta
             h_scal[SC_RT] = rt2;
         }
         bb = h_scal[SC_BB];
-        if (it == 1 && h_scal[SC_SIDE] < side_cnv) {
+        const bool first_check = !checked;
+        checked = true;
+        if (first_check && h_scal[SC_SIDE] < side_cnv) {
             // the caller's stopping test (read with this round trip) says the
             // solve is not wanted: x is left as it is, the caller rolls back
             aborted = true;

@@ -109,6 +109,13 @@ public:
     // the nine-point path): the least-squares combination of the given
     // vectors, formed after the setup of that solve.
     void set_warm_start(const WarmSet& ws) { warm = ws; }
+    // The number of iterations the next solve is expected to take (e.g. the
+    // count of the same solve one time step earlier; 0 = unknown): the
+    // iterations before the last expected one run back to back, without a
+    // convergence check.  A solve that converges earlier than that keeps
+    // iterating up to the first check (a more accurate answer, never a
+    // less accurate one); one that needs more continues with checks.
+    int expected_its = 0;
     // Hands the solution buffer to the caller in exchange for buf (same
     // size, its own allocation, zero ghosts): no copy.  Kernels already
     // launched with the old sol() pointer still read the solution.

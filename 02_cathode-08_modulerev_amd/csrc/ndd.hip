@@ -779,6 +779,7 @@ struct NddHandle {
     static constexpr int WARM_HIST = 3;
     DevBuf<double> wbuf[8][WARM_HIST];
     int nhist[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int last_its[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // each solve's iteration count one step earlier
     double* cap_feat = nullptr;       // capture for training (device, caller-owned)
     double* cap_target = nullptr;
     int guesses = 0, guesses_used = 0;
@@ -854,13 +855,17 @@ static void account(NddHandle& H, const cathode_status& st)
 // gate: the PC stopping threshold the previous iteration's calcRes is
 // tested against (-inf for the first iteration).  Returns false when the
 // loop should have stopped before this iteration.
-static bool solve(NddHandle& H, double gate, int maxITR, double tol)
+// slot: the solve's warm-start slot; in learned mode its iteration count of
+// the previous step is the solver's expected count (fewer host round trips)
+static bool solve(NddHandle& H, double gate, int maxITR, double tol, int slot)
 {
     cathode_status st{};
     H.mg.side_cnv = gate;
+    if (H.p.learned) H.mg.expected_its = H.last_its[slot];
     H.mg.solve(maxITR, tol, H.p.mode, H.s, &st);
     H.mg.side_cnv = -INFINITY;
     if (H.mg.aborted) return false;
+    H.last_its[slot] = st.iterations;
     account(H, st);
     return true;
 }
@@ -1029,7 +1034,8 @@ static void poisson_routine(NddHandle& H, bool report)
         const bool learn = ncount == 1 && (use_net || H.cap_feat);
         if (learn) learned_guess(H, use_net);
         else warm_guess(H, warm_slot(EQ_POISSON, ncount));
-        if (!solve(H, ncount > 1 ? H.p.error_cnv_HES_phi : -INFINITY, H.p.maxITR_SOR_phi, H.p.error_cnv_SOR_phi)) {
+        if (!solve(H, ncount > 1 ? H.p.error_cnv_HES_phi : -INFINITY, H.p.maxITR_SOR_phi, H.p.error_cnv_SOR_phi,
+                   warm_slot(EQ_POISSON, ncount))) {
             // calcRes of the previous iteration met the PC criterion: undo the prediction
             d2d(H, {{F[F_NUEX], H.tmp_x}, {F[F_NUER], H.tmp_r}});
             break;
@@ -1106,7 +1112,7 @@ static void sg_routine(NddHandle& H, int sp, bool report, double* err3)
         CATHODE_LAUNCH_CHECK();
         const int eq = sp == SP_ELEC ? EQ_ELEC : (sp == SP_ION ? EQ_ION : EQ_ENERGY);
         warm_guess(H, warm_slot(eq, ncount));
-        if (!solve(H, ncount > 1 ? cnv : -INFINITY, lin_maxit, lin_tol)) {
+        if (!solve(H, ncount > 1 ? cnv : -INFINITY, lin_maxit, lin_tol, warm_slot(eq, ncount))) {
             d2d(H, {{A.Fx, H.tmp_x}, {A.Fr, H.tmp_r}});
             break;
         }
@@ -1289,6 +1295,7 @@ int cathode_ndd_set_state(void* handle, const cathode_ndd_fields* host)
         cathode_ndd_fields f = *host;
         H->have_xprev = false;  // a new state: no previous Poisson correction
         for (int& v : H->nhist) v = 0;
+        for (int& v : H->last_its) v = 0;
         H->net_useful = true;
         H->gate_pending = false;
         H->net_wait = 0;
