@@ -94,18 +94,6 @@ __global__ void k_fill(double* p, int n, double v)
 }
 
 // coarse operator of coarse cell (I, J) from level F
-struct CoarsenArgs {
-    int fni, fnj, fld, fi, fj;
-    const double *faP, *faE, *faW, *faN, *faS, *fm;
-    const unsigned char* fact;
-    const double* w;
-    int cni, cnj, cld;
-    double *caP, *caE, *caW, *caN, *caS, *cb, *cx, *cm;
-    unsigned char* cact;
-    float *gP, *gE, *gW, *gN, *gS;  // FP32 copy of the coarse level (mixed)
-    double hx, hy, hb;
-};
-
 __device__ inline void coarsen_cell(const CoarsenArgs& a, int I, int J, double s32, int mixed)
 {
     const int i0 = a.fi * (I - 1) + 1, i1 = min(a.fi * I, a.fni);
@@ -166,21 +154,6 @@ __global__ void k_coarsen(CoarsenArgs a, const double* __restrict__ sc, int mixe
     coarsen_cell(a, I, J, mixed ? sc[3] : 1.0, mixed);
 }
 
-// all coarse levels of the tail, one workgroup, level by level
-constexpr int CHAIN_MAX = 12;  // keeps the kernel argument block under 4 KiB
-struct CoarsenChain {
-    int n;
-    CoarsenArgs L[CHAIN_MAX];
-};
-
-// the coarsest level, for its dense inverse
-struct CoarseInv {
-    int ni, nj, ld;
-    const double *aP, *aE, *aW, *aN, *aS;
-    const unsigned char* act;
-    double* inv64;  // n x n, row-major
-    float* inv32;   // the same divided by s32 (the FP32 levels hold s32 A)
-};
 
 __device__ inline double readlane_f64(double v, int lane)
 {
@@ -205,12 +178,14 @@ __device__ inline double readlane_f64(double v, int lane)
 // max|aP| (a singular operator, no Dirichlet part) zeroes its row, so the
 // coarse solve returns a particular solution.
 constexpr int CINV_NT = 512;
+template <int NT>
 __device__ __forceinline__ void coarse_inverse_block(const CoarseInv& c, double s32, double* lds_rows,
                                                      double* lds_red)
 {
     constexpr int NM = MG_COARSEST_MAX;
-    constexpr int RPW = NM / (CINV_NT / 64);  // rows per wavefront
-    const int t = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
+    constexpr int RPW = NM / (NT / 64);  // rows per wavefront
+    static_assert(NM % (NT / 64) == 0, "rows split evenly over the wavefronts");
+    const int t = lin_tid() & 63, w = lin_tid() >> 6;
     const int n = c.ni * c.nj, nj = c.nj;
     double m[RPW];
     double amax = 0.0;
@@ -242,7 +217,7 @@ __device__ __forceinline__ void coarse_inverse_block(const CoarseInv& c, double 
         }
         m[r] = v;
     }
-    const double tiny = 1e-13 * block_max_bcast<CINV_NT>(amax, lds_red);
+    const double tiny = 1e-13 * block_max_bcast<NT>(amax, lds_red);
 #pragma unroll
     for (int k = 0; k < NM; k++) {
         const int wk = k / RPW, rk = k % RPW;
@@ -275,22 +250,29 @@ __device__ __forceinline__ void coarse_inverse_block(const CoarseInv& c, double 
     }
 }
 
-__global__ void __launch_bounds__(512) k_coarsen_chain(CoarsenChain c, const double* __restrict__ sc, int mixed,
-                                                      CoarseInv ci)
+// the chain's levels one after another, then (when ci.inv64 is set) the
+// coarsest inverse; NT threads
+template <int NT>
+__device__ __forceinline__ void chain_block(const CoarsenChain& c, const double* __restrict__ sc, int mixed,
+                                            const CoarseInv& ci)
 {
     const double s32 = mixed ? sc[3] : 1.0;
     for (int l = 0; l < c.n; l++) {
         const CoarsenArgs& a = c.L[l];
-        for (int q = threadIdx.x; q < a.cni * a.cnj; q += blockDim.x)
-            coarsen_cell(a, 1 + q / a.cnj, 1 + q % a.cnj, s32, mixed);
+        for (int q = lin_tid(); q < a.cni * a.cnj; q += NT) coarsen_cell(a, 1 + q / a.cnj, 1 + q % a.cnj, s32, mixed);
         __syncthreads();
     }
-    // the chain that built the coarsest level also inverts it (all 512 threads)
     if (ci.inv64) {
         __shared__ double rows[2 * 64];
-        __shared__ double red[CINV_NT / 64];
-        coarse_inverse_block(ci, sc[3], rows, red);
+        __shared__ double red[NT / 64];
+        coarse_inverse_block<NT>(ci, sc[3], rows, red);
     }
+}
+
+__global__ void __launch_bounds__(512) k_coarsen_chain(CoarsenChain c, const double* __restrict__ sc, int mixed,
+                                                      CoarseInv ci)
+{
+    chain_block<512>(c, sc, mixed, ci);
 }
 
 // the coarsest level's inverse when no chain builds it (it is a fine level)
@@ -298,7 +280,7 @@ __global__ void __launch_bounds__(CINV_NT) k_coarse_inverse(CoarseInv ci, const 
 {
     __shared__ double rows[2 * 64];
     __shared__ double red[CINV_NT / 64];
-    coarse_inverse_block(ci, sc[3], rows, red);
+    coarse_inverse_block<CINV_NT>(ci, sc[3], rows, red);
 }
 
 // A y_q at cell (i, j) of level 0 (5-point operator, couplings to inactive
@@ -573,19 +555,39 @@ __device__ inline double warm_coeffs(const double* t, int n, double* a)
     return t[WARM_NP - 1] - red;
 }
 
-__global__ void __launch_bounds__(BX * BY) k_bicg_restart(int ni, int nj, int ld, const double* __restrict__ aP,
-                               const double* __restrict__ aE, const double* __restrict__ aW,
-                               const double* __restrict__ aN, const double* __restrict__ aS,
-                               const double* __restrict__ nine, size_t n0,
-                               const double* __restrict__ b, const double* __restrict__ x,
-                               const unsigned char* __restrict__ act, double* __restrict__ r,
-                               double* __restrict__ r0, double* __restrict__ p, double* __restrict__ v,
-                               double* __restrict__ part, int nb, double* __restrict__ xout, WarmSet ws,
-                               const double* __restrict__ wpart, int wnb)
+struct RestartArgs {
+    int ni, nj, ld;
+    const double *aP, *aE, *aW, *aN, *aS, *nine;
+    size_t n0;
+    const double *b, *x;
+    const unsigned char* act;
+    double *r, *r0, *p, *v, *part;
+    int nb;
+    double* xout;
+    WarmSet ws;
+    const double* wpart;
+    int wnb;
+};
+
+// NT / (BX BY) blocks of the restart in one workgroup of BX x (NT / BX)
+// threads: blocks blk0, blk0 + 1, ... of the 2-D block grid in row-major
+// order, one per BX x BY threads.  Every block's partial sums its four
+// wavefronts in the same order whatever NT is.
+template <int NT>
+__device__ __forceinline__ void restart_block(const RestartArgs& A, int blk0)
 {
-    __shared__ double lds[8];
-    __shared__ double wl[WARM_NP * (BX * BY / 64)];
-    CELL_IJ;
+    __shared__ double lds[NT / 64];
+    __shared__ double wl[WARM_NP * (NT / 64)];
+    const int ni = A.ni, nj = A.nj, ld = A.ld;
+    const double *aP = A.aP, *aE = A.aE, *aW = A.aW, *aN = A.aN, *aS = A.aS, *nine = A.nine, *b = A.b, *x = A.x;
+    const size_t n0 = A.n0;
+    const unsigned char* act = A.act;
+    const WarmSet& ws = A.ws;
+    const int nbx = (nj + BX - 1) / BX, nby = (ni + BY - 1) / BY;
+    const int bid = blk0 + lin_tid() / (BX * BY);
+    const bool live = bid < nbx * nby;
+    const int j = 1 + (bid % nbx) * BX + (int)threadIdx.x;
+    const int i = 1 + (bid / nbx) * BY + (int)threadIdx.y % BY;
     // warm start x = sum a_q y_q from the caller's partials (every workgroup
     // sums them in the same order and solves the same small system)
     double a[WARM_MAX] = {0.0, 0.0, 0.0, 0.0};
@@ -594,19 +596,19 @@ __global__ void __launch_bounds__(BX * BY) k_bicg_restart(int ni, int nj, int ld
 #pragma unroll
         for (int m = 0; m < WARM_NP; m++) {
             double s = 0.0;
-            for (int q = lin_tid(); q < wnb; q += BX * BY) s += wpart[m * wnb + q];
+            for (int q = lin_tid(); q < A.wnb; q += NT) s += A.wpart[m * A.wnb + q];
             t[m] = s;
         }
-        block_sum_vec_bcast<BX * BY, WARM_NP>(t, wl);
+        block_sum_vec_bcast<NT, WARM_NP>(t, wl);
         const double res = warm_coeffs(t, ws.k, a);
-        if (ws.report && lin_tid() == 0 && blockIdx.x == 0 && blockIdx.y == 0) {
+        if (ws.report && lin_tid() == 0 && blk0 == 0) {
             double a2[WARM_MAX];
             const double res_less = warm_coeffs(t, ws.k - 1, a2);
             ws.report[0] = res_less > 0.0 ? fmax(res, 0.0) / res_less : 1.0;
         }
     }
     double rr = 0.0;
-    if (i <= ni && j <= nj) {
+    if (live && i <= ni && j <= nj) {
         const size_t k = (size_t)i * ld + j;
         double q = 0.0;
         if (ws.k > 0) {
@@ -618,14 +620,14 @@ __global__ void __launch_bounds__(BX * BY) k_bicg_restart(int ni, int nj, int ld
                     x0 += a[m] * ws.y[m][(size_t)i * ws.ld[m] + j];
                     if (act[k]) ax += a[m] * warm_ay(ws, m, i, j, k, aP, aE, aW, aN, aS);
                 }
-            xout[k] = x0;
+            A.xout[k] = x0;
             if (act[k]) {
                 q = b[k] - ax;
                 rr = q * q;
             }
-        } else if (xout) {
+        } else if (A.xout) {
             // zero initial guess: x = 0 on the interior, r = b
-            xout[k] = 0.0;
+            A.xout[k] = 0.0;
             if (act[k]) {
                 q = b[k];
                 rr = q * q;
@@ -634,19 +636,43 @@ __global__ void __launch_bounds__(BX * BY) k_bicg_restart(int ni, int nj, int ld
             q = b[k] - aP[k] * x[k] + offdiag(aE, aW, aN, aS, nine, n0, k, ld, [&](size_t m) { return x[m]; });
             rr = q * q;
         }
-        r[k] = q;
-        r0[k] = q;
-        p[k] = 0.0;
-        v[k] = 0.0;
+        A.r[k] = q;
+        A.r0[k] = q;
+        A.p[k] = 0.0;
+        A.v[k] = 0.0;
     }
-    double t = block_sum<BX * BY>(rr, lds);
-    if (lin_tid() == 0) {
-        // the k_bicg_x layout: (r, r), (r0, r) = (r, r), |b - A x|^2 = (r, r)
-        const int bid = blockIdx.y * gridDim.x + blockIdx.x;
-        part[bid] = t;
-        part[nb + bid] = t;
-        part[2 * nb + bid] = t;
+    // the k_bicg_x layout: (r, r), (r0, r) = (r, r), |b - A x|^2 = (r, r)
+    rr = wave_sum(rr);
+    const int w = lin_tid() >> 6;
+    if ((lin_tid() & 63) == 0) lds[w] = rr;
+    __syncthreads();
+    if (live && lin_tid() % (BX * BY) == 0) {
+        double t = 0.0;
+        for (int q = 0; q < BX * BY / 64; q++) t += lds[w + q];
+        A.part[bid] = t;
+        A.part[A.nb + bid] = t;
+        A.part[2 * A.nb + bid] = t;
     }
+}
+
+__global__ void __launch_bounds__(BX * BY) k_bicg_restart(RestartArgs A)
+{
+    restart_block<BX * BY>(A, (int)(blockIdx.y * gridDim.x + blockIdx.x));
+}
+
+// The first restart of a solve together with the coarse chain of its setup:
+// both read only what the setup launch wrote and write disjoint buffers.
+// Workgroup 0 (dispatched first) builds the chain's levels and the coarsest
+// inverse; workgroups 1.. take two restart blocks each, in row-major order.
+constexpr int RC_NT = 2 * BX * BY;
+__global__ void __launch_bounds__(RC_NT) k_restart_chain(RestartArgs A, CoarsenChain c, const double* __restrict__ sc,
+                                                        int mixed, CoarseInv ci)
+{
+    if (blockIdx.x == 0) {
+        chain_block<RC_NT>(c, sc, mixed, ci);
+        return;
+    }
+    restart_block<RC_NT>(A, 2 * ((int)blockIdx.x - 1));
 }
 
 // p = r + beta (p - om v), beta = (rho1/rho)(al/om) (0 after a restart);
@@ -1082,7 +1108,14 @@ void MGSolver::setup(int mode, hipStream_t s, const WarmSet& ws)
         const bool last = l0 + ch.n + 1 >= nl;
         CoarseInv cc = ci;
         if (!last) cc.inv64 = nullptr;
-        hipLaunchKernelGGL(k_coarsen_chain, dim3(1), dim3(512), 0, s, ch, (const double*)scal.p, mixed, cc);
+        if (last) {
+            chain_pending = true;
+            pend_mixed = mixed;
+            pend_ch = ch;
+            pend_ci = cc;
+        } else {
+            hipLaunchKernelGGL(k_coarsen_chain, dim3(1), dim3(512), 0, s, ch, (const double*)scal.p, mixed, cc);
+        }
         inverted = inverted || last;
     }
     if (!inverted) hipLaunchKernelGGL(k_coarse_inverse, dim3(1), dim3(CINV_NT), 0, s, ci, (const double*)scal.p);
@@ -1144,6 +1177,14 @@ void MGSolver::vcycle(Level<T>* L, hipStream_t s, const VFuse& fdown, const VFus
         launch_vtile<T, false>(tile_of<T>(l), l == 0, s, V, A.cx ? 2 : 1, A.cy ? 2 : 1, (const double*)nullptr, C.ni,
                                C.nj, C.ld, l + 1 < lt ? C.xu : C.x, l == 0 ? fup : VFuse());
     }
+    CATHODE_LAUNCH_CHECK();
+}
+
+void MGSolver::flush_chain(hipStream_t s)
+{
+    if (!chain_pending) return;
+    chain_pending = false;
+    hipLaunchKernelGGL(k_coarsen_chain, dim3(1), dim3(512), 0, s, pend_ch, (const double*)scal.p, pend_mixed, pend_ci);
     CATHODE_LAUNCH_CHECK();
 }
 
@@ -1470,9 +1511,20 @@ int MGSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_sta
     const bool graphs_on = fused && !prof.enabled && genv && genv[0] == '1';
     while (it < maxITR) {
         if (restart) {
-            hipLaunchKernelGGL(k_bicg_restart, g, blk, 0, s, F.ni, F.nj, F.ld, F.aP, F.aE, F.aW, F.aN, F.aS, n9,
-                               n0, bsol.p, xa, act0.p, r, r0, pa, v, part3, nb,
-                               (zero_guess || ws.k > 0) ? xa : nullptr, ws, (const double*)wpart.p, warm_nb);
+            RestartArgs ra{};
+            ra.ni = F.ni; ra.nj = F.nj; ra.ld = F.ld;
+            ra.aP = F.aP; ra.aE = F.aE; ra.aW = F.aW; ra.aN = F.aN; ra.aS = F.aS; ra.nine = n9; ra.n0 = n0;
+            ra.b = bsol.p; ra.x = xa; ra.act = act0.p;
+            ra.r = r; ra.r0 = r0; ra.p = pa; ra.v = v; ra.part = part3; ra.nb = nb;
+            ra.xout = (zero_guess || ws.k > 0) ? xa : nullptr;
+            ra.ws = ws; ra.wpart = wpart.p; ra.wnb = warm_nb;
+            if (chain_pending) {
+                chain_pending = false;
+                hipLaunchKernelGGL(k_restart_chain, dim3(1 + cdiv(g.x * g.y, 2)), dim3(BX, RC_NT / BX), 0, s, ra,
+                                   pend_ch, (const double*)scal.p, pend_mixed, pend_ci);
+            } else {
+                hipLaunchKernelGGL(k_bicg_restart, g, blk, 0, s, ra);
+            }
             zero_guess = false;
             ws = WarmSet();
             // the fused path sums these partials in its first down kernel
@@ -1600,6 +1652,7 @@ int MGSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_sta
         const double per_cell = fused ? 7.25 * vb + 1.0 + 8.0 * (2.5 + 1.0) : 7.25 * vb + 1.0;
         prof.bytes_per_launch = per_cell * (double)F.ni * (double)F.nj;
     }
+    flush_chain(s);  // a solve that ran no restart still leaves a whole hierarchy
     auto t1 = std::chrono::steady_clock::now();
     if (st) {
         st->iterations = it;

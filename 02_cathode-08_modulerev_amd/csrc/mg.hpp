@@ -37,7 +37,6 @@ constexpr int MG_COARSEST_MAX = 32;
 constexpr int CINV_LD = MG_COARSEST_MAX + 1;
 
 constexpr int MG_MAX_LEVELS = 24;
-struct CoarsenArgs;
 
 template <class T>
 struct Level {
@@ -54,6 +53,35 @@ struct MGProfile {
     double total_ms = 0.0;
     double bytes_per_launch = 0.0;
     std::vector<hipEvent_t> ev;  // pairs of events, read after the solve
+};
+
+// one coarsening step: fine level f -> coarse level c (and its FP32 copy)
+struct CoarsenArgs {
+    int fni, fnj, fld, fi, fj;
+    const double *faP, *faE, *faW, *faN, *faS, *fm;
+    const unsigned char* fact;
+    const double* w;
+    int cni, cnj, cld;
+    double *caP, *caE, *caW, *caN, *caS, *cb, *cx, *cm;
+    unsigned char* cact;
+    float *gP, *gE, *gW, *gN, *gS;  // FP32 copy of the coarse level (mixed)
+    double hx, hy, hb;
+};
+
+// all coarse levels of the tail, one workgroup, level by level
+constexpr int CHAIN_MAX = 12;  // keeps the kernel argument block under 4 KiB
+struct CoarsenChain {
+    int n;
+    CoarsenArgs L[CHAIN_MAX];
+};
+
+// the coarsest level, for its dense inverse
+struct CoarseInv {
+    int ni, nj, ld;
+    const double *aP, *aE, *aW, *aN, *aS;
+    const unsigned char* act;
+    double* inv64;  // n x n, row-major
+    float* inv32;   // the same divided by s32 (the FP32 levels hold s32 A)
 };
 
 class MGSolver {
@@ -163,6 +191,15 @@ private:
     float time_tile(Level<T>* L, int l, int cfg, hipStream_t s);
     void prof_events(hipEvent_t& a, hipEvent_t& b);
     double read_scalar(const double* d, hipStream_t s);
+    // The setup's last chain launch (coarse levels of the tail and the
+    // coarsest inverse) is left pending: the solve's first restart runs it
+    // in the same launch.  flush_chain() launches it alone (before anything
+    // else reads the coarse levels).
+    bool chain_pending = false;
+    int pend_mixed = 0;
+    CoarsenChain pend_ch{};
+    CoarseInv pend_ci{};
+    void flush_chain(hipStream_t s);
 
     int nl = 0;
     int cfg_ni = -1, cfg_nj = -1;

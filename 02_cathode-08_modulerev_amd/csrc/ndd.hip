@@ -168,10 +168,9 @@ struct PoissonArgs {
 };
 
 // predicted x flux on faces i = 1..ni+1 (fluidModule.hpp:2266-2462)
-__global__ void k_poisson_predict_x(NddView V, PoissonArgs A)
+__device__ __forceinline__ void poisson_predict_x(const NddView& V, const PoissonArgs& A, int i, int j)
 {
 #pragma clang fp contract(off)
-    THREAD_IJ(1, 1);
     if (i > V.ni + 1 || j > V.nj) return;
     const double dx = V.dx;
     const double* r = V.r;
@@ -204,10 +203,9 @@ __global__ void k_poisson_predict_x(NddView V, PoissonArgs A)
 }
 
 // predicted r flux on faces j = 2..nj+1 (fluidModule.hpp:2466-2621)
-__global__ void k_poisson_predict_r(NddView V, PoissonArgs A)
+__device__ __forceinline__ void poisson_predict_r(const NddView& V, const PoissonArgs& A, int i, int j)
 {
 #pragma clang fp contract(off)
-    THREAD_IJ(1, 2);
     if (i > V.ni || j > V.nj + 1) return;
     const double dr = V.dr;
     const double* r = V.r;
@@ -227,6 +225,19 @@ __global__ void k_poisson_predict_r(NddView V, PoissonArgs A)
     } else {
         const double uir = V.mu_i * A.Er[IX(i, V.nj + 1)] + K_EPS_VD;
         A.nUer[IX(i, V.nj + 1)] = A.rhoi[IX(i, V.nj)] * uir;
+    }
+}
+
+// both predictors in one launch: x faces on blockIdx.z == 0, r faces on 1;
+// they read only the copies of the previous fluxes, so they are independent
+__global__ void k_poisson_predict(NddView V, PoissonArgs A)
+{
+    if (blockIdx.z == 0) {
+        THREAD_IJ(1, 1);
+        poisson_predict_x(V, A, i, j);
+    } else {
+        THREAD_IJ(1, 2);
+        poisson_predict_r(V, A, i, j);
     }
 }
 
@@ -362,10 +373,9 @@ struct SgArgs {
     double *n, *Fx, *Fr;
 };
 
-__global__ void k_sg_predict_x(NddView V, SgArgs A)
+__device__ __forceinline__ void sg_predict_x(const NddView& V, const SgArgs& A, int i, int j)
 {
 #pragma clang fp contract(off)
-    THREAD_IJ(1, 1);
     if (i > V.ni + 1 || j > V.nj) return;
     const double dx = V.dx;
     const double* r = V.r;
@@ -417,10 +427,9 @@ __global__ void k_sg_predict_x(NddView V, SgArgs A)
     }
 }
 
-__global__ void k_sg_predict_r(NddView V, SgArgs A)
+__device__ __forceinline__ void sg_predict_r(const NddView& V, const SgArgs& A, int i, int j)
 {
 #pragma clang fp contract(off)
-    THREAD_IJ(1, 2);
     if (i > V.ni || j > V.nj + 1) return;
     const double dr = V.dr;
     const double* r = V.r;
@@ -441,6 +450,19 @@ __global__ void k_sg_predict_r(NddView V, SgArgs A)
     } else {
         const double uir = V.mu_i * A.Er[IX(i, V.nj + 1)] + K_EPS_VD;
         A.Fr[IX(i, V.nj + 1)] = A.rhoi[IX(i, V.nj)] * uir;
+    }
+}
+
+// both predictors in one launch: x faces on blockIdx.z == 0, r faces on 1;
+// they read only the copies of the previous fluxes, so they are independent
+__global__ void k_sg_predict(NddView V, SgArgs A)
+{
+    if (blockIdx.z == 0) {
+        THREAD_IJ(1, 1);
+        sg_predict_x(V, A, i, j);
+    } else {
+        THREAD_IJ(1, 2);
+        sg_predict_r(V, A, i, j);
     }
 }
 
@@ -805,6 +827,8 @@ static double** field_slot(cathode_ndd_fields* f, int k)
 
 static inline dim3 gblk() { return dim3(NB_X, NB_Y); }
 static inline dim3 ggrid(int ni, int nj) { return dim3(cdiv(nj, NB_X), cdiv(ni, NB_Y)); }
+// predictor grid: x faces i = 1..ni+1, j = 1..nj (z = 0) and r faces i = 1..ni, j = 2..nj+1 (z = 1)
+static inline dim3 pgrid(const NddView& V) { return dim3(cdiv(V.nj, NB_X), cdiv(V.ni + 1, NB_Y), 2); }
 
 // up to 8 whole-plane copies in one launch (blockIdx.y picks the pair)
 struct CopyList {
@@ -1014,8 +1038,7 @@ static void poisson_routine(NddHandle& H, bool report)
     for (;;) {
         ncount++;
         d2d(H, {{H.tmp_c, F[F_PHI]}, {H.tmp_x, F[F_NUEX]}, {H.tmp_r, F[F_NUER]}});
-        hipLaunchKernelGGL(k_poisson_predict_x, ggrid(V.ni + 1, V.nj), gblk(), 0, H.s, V, A);
-        hipLaunchKernelGGL(k_poisson_predict_r, ggrid(V.ni, V.nj), gblk(), 0, H.s, V, A);
+        hipLaunchKernelGGL(k_poisson_predict, pgrid(V), gblk(), 0, H.s, V, A);
         hipLaunchKernelGGL(k_poisson_assemble, ggrid(V.ni, V.nj), gblk(), 0, H.s, V, A, H.mg.aP(), H.mg.aE(),
                            H.mg.aW(), H.mg.aN(), H.mg.aS(), H.mg.rhs());
         CATHODE_LAUNCH_CHECK();
@@ -1105,8 +1128,7 @@ static void sg_routine(NddHandle& H, int sp, bool report, double* err3)
         } else {
             d2d(H, {{H.tmp_c, A.n}, {H.tmp_x, A.Fx}, {H.tmp_r, A.Fr}});
         }
-        hipLaunchKernelGGL(k_sg_predict_x, ggrid(V.ni + 1, V.nj), gblk(), 0, H.s, V, A);
-        hipLaunchKernelGGL(k_sg_predict_r, ggrid(V.ni, V.nj), gblk(), 0, H.s, V, A);
+        hipLaunchKernelGGL(k_sg_predict, pgrid(V), gblk(), 0, H.s, V, A);
         hipLaunchKernelGGL(k_sg_assemble, ggrid(V.ni, V.nj), gblk(), 0, H.s, V, A, H.mg.aP(), H.mg.aE(), H.mg.aW(),
                            H.mg.aN(), H.mg.aS(), H.mg.rhs());
         CATHODE_LAUNCH_CHECK();

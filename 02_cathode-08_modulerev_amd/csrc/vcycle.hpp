@@ -485,40 +485,41 @@ struct TailCells {
     unsigned fl[C];  // bit0 active, bit1 colour
 };
 
+// The loads do not depend on each other or on the activity byte (a thread
+// without a cell reads the level's first cell and discards it), so every
+// level's loads are in flight together.
 template <class T, int C>
 __device__ inline void tail_load(TailCells<T, C>& L, const TailArgs<T>& A, int l, int q, T* B)
 {
     const int ni = A.ni[l], nj = A.nj[l], ld = nj + 2;
 #pragma unroll
     for (int k = 0; k < C; k++) {
-        L.c[k] = -1;
-        L.idx[k] = 0;
-        L.par[k] = 0;
-        L.fl[k] = 0;
-        L.P[k] = L.iP[k] = L.wq[k] = T(1);
-        L.E[k] = L.W[k] = L.N[k] = L.S[k] = T(0);
         const int ci = q * C + k;
-        if (ci >= ni * nj) continue;
-        const int i = 1 + ci / nj, j = 1 + ci % nj;
-        L.c[k] = A.off[l] + i * ld + j;
-        L.idx[k] = ci;
+        const bool here = ci < ni * nj;
+        const int cc = here ? ci : 0;
+        const int i = 1 + cc / nj, j = 1 + cc % nj;
         const size_t g = (size_t)i * A.gld[l] + j;
-        L.fl[k] = ((i + j) & 1) ? 2u : 0u;
-        if (A.act[l][g]) {
-            L.fl[k] |= 1u;
-            L.P[k] = A.aP[l][g];
-            L.iP[k] = T(1) / L.P[k];
-            L.E[k] = A.aE[l][g];
-            L.W[k] = A.aW[l][g];
-            L.N[k] = A.aN[l][g];
-            L.S[k] = A.aS[l][g];
-            if (l == 0) B[L.c[k]] = A.b_in[g];
+        const bool on = here && A.act[l][g] != 0;
+        const T P = A.aP[l][g], E = A.aE[l][g], W = A.aW[l][g], N = A.aN[l][g], S = A.aS[l][g];
+        L.c[k] = here ? A.off[l] + i * ld + j : -1;
+        L.idx[k] = here ? ci : 0;
+        L.fl[k] = (here && ((i + j) & 1) ? 2u : 0u) | (on ? 1u : 0u);
+        L.P[k] = on ? P : T(1);
+        L.iP[k] = on ? T(1) / P : T(1);
+        L.E[k] = on ? E : T(0);
+        L.W[k] = on ? W : T(0);
+        L.N[k] = on ? N : T(0);
+        L.S[k] = on ? S : T(0);
+        if (l == 0) {
+            const T b = A.b_in[g];
+            if (on) B[L.c[k]] = b;
         }
-        if (l + 1 < A.nt) {
+        L.par[k] = 0;
+        if (here && l + 1 < A.nt) {
             const int I = A.cx[l] ? (i + 1) >> 1 : i, J = A.cy[l] ? (j + 1) >> 1 : j;
             L.par[k] = A.off[l + 1] + I * (A.nj[l + 1] + 2) + J;
         }
-        if (l == 0 && A.w) L.wq[k] = (T)A.w[j];
+        L.wq[k] = (l == 0 && A.w) ? (T)A.w[j] : T(1);
     }
 }
 
