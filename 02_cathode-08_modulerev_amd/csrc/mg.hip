@@ -100,16 +100,23 @@ __device__ inline void coarsen_cell(const CoarsenArgs& a, int I, int J, double s
     const int j0 = a.fj * (J - 1) + 1, j1 = min(a.fj * J, a.fnj);
     double S = 0.0, M = 0.0, E = 0.0, W = 0.0, N = 0.0, So = 0.0;
     int any = 0;
-    for (int i = i0; i <= i1; i++)
-        for (int j = j0; j <= j1; j++) {
-            const size_t k = (size_t)i * a.fld + j;
-            if (!a.fact[k]) continue;
+    // the (up to) 2 x 2 children in row-major order; every load is issued
+    // before any is tested (an absent child reads child (i0, j0) and drops out)
+#pragma unroll
+    for (int di = 0; di < 2; di++)
+#pragma unroll
+        for (int dj = 0; dj < 2; dj++) {
+            const int i = i0 + di, j = j0 + dj;
+            const bool here = i <= i1 && j <= j1;
+            const size_t k = here ? (size_t)i * a.fld + j : (size_t)i0 * a.fld + j0;
+            const bool on = here && a.fact[k] != 0;
+            const double wj = a.w ? a.w[here ? j : j0] : 1.0;
+            const double p = a.faP[k], e = a.faE[k], ww = a.faW[k], n = a.faN[k], s = a.faS[k], m = a.fm[k];
+            if (!on) continue;
             any = 1;
-            const double wj = a.w ? a.w[j] : 1.0;
-            const double e = a.faE[k], ww = a.faW[k], n = a.faN[k], s = a.faS[k];
-            const double ex = a.faP[k] - (e + ww + n + s);
+            const double ex = p - (e + ww + n + s);
             S += wj * (ex > 0.0 ? ex : 0.0);
-            M += a.fm[k];
+            M += m;
             if (i == i1) E += wj * e;
             if (i == i0) W += wj * ww;
             if (j == j1) N += wj * n;
@@ -164,19 +171,22 @@ __device__ inline double readlane_f64(double v, int lane)
 }
 
 // Dense inverse of the coarsest level (n = ni nj <= MG_COARSEST_MAX) by
-// Gauss-Jordan elimination without pivoting on the system padded to NM =
-// MG_COARSEST_MAX unknowns (rows and columns n.. are the identity, so the
-// leading n x n block of the result is A^-1), run by the 8 wavefronts of a
-// 512-thread workgroup.  Lane t holds column t of [A | I] (t < NM: A, t >=
-// NM: I); wavefront w holds rows 4w .. 4w+3 of it in registers.  Step k: the
-// wavefront owning row k normalises it and publishes it in LDS (two buffers,
-// so one barrier per step); every row i != k subtracts A[i][k] times it,
-// A[i][k] coming from lane k of the same wavefront by readlane.  With the
-// fixed size every loop unrolls without branches.  The coarse operators are
-// diagonally dominant with non-positive couplings (M-matrices), for which
-// elimination without pivoting is stable.  A pivot at or below 1e-13
-// max|aP| (a singular operator, no Dirichlet part) zeroes its row, so the
-// coarse solve returns a particular solution.
+// block Gauss-Jordan elimination without pivoting on the system padded to
+// NM = MG_COARSEST_MAX unknowns (rows and columns n.. are the identity, so
+// the leading n x n block of the result is A^-1), run by the NT / 64
+// wavefronts of the workgroup.  Lane t holds column t of [A | I] (t < NM:
+// A, t >= NM: I); wavefront w holds rows RPW w .. RPW w + RPW-1 of it in
+// registers.  Block step K eliminates the RPW pivot columns of wavefront K's
+// rows at once: that wavefront first reduces its own rows to the identity
+// in those columns (Gauss-Jordan inside the wavefront, factors by
+// readlane), publishes them in LDS (two buffers, so one barrier per block
+// step), and every other row subtracts its entries in the pivot columns
+// (read before the step, by readlane) times those rows.  With the fixed size
+// every loop unrolls without branches.  The coarse operators are diagonally
+// dominant with non-positive couplings (M-matrices), for which elimination
+// without pivoting is stable.  A pivot at or below 1e-13 max|aP| (a singular
+// operator, no Dirichlet part) zeroes its row, so the coarse solve returns a
+// particular solution.
 constexpr int CINV_NT = 512;
 template <int NT>
 __device__ __forceinline__ void coarse_inverse_block(const CoarseInv& c, double s32, double* lds_rows,
@@ -189,8 +199,8 @@ __device__ __forceinline__ void coarse_inverse_block(const CoarseInv& c, double 
     const int n = c.ni * c.nj, nj = c.nj;
     double m[RPW];
     double amax = 0.0;
-    // rows 4w+r of column t: row q's diagonal, or its coupling to t when t is
-    // q's neighbour (E of q is q + nj, W q - nj, N q + 1, S q - 1)
+    // rows RPW w + r of column t: row q's diagonal, or its coupling to t when
+    // t is q's neighbour (E of q is q + nj, W q - nj, N q + 1, S q - 1)
 #pragma unroll
     for (int r = 0; r < RPW; r++) {
         const int q = w * RPW + r;
@@ -219,22 +229,39 @@ __device__ __forceinline__ void coarse_inverse_block(const CoarseInv& c, double 
     }
     const double tiny = 1e-13 * block_max_bcast<NT>(amax, lds_red);
 #pragma unroll
-    for (int k = 0; k < NM; k++) {
-        const int wk = k / RPW, rk = k % RPW;
-        double* row = lds_rows + (k & 1) * 64;
-        if (w == wk) {
-            const double pv = readlane_f64(m[rk], k);
-            const double rcp = (fabs(pv) > tiny && isfinite(pv)) ? 1.0 / pv : 0.0;
-            m[rk] = m[rk] * rcp;
-            row[t] = m[rk];
+    for (int K = 0; K < NT / 64; K++) {
+        double* rows = lds_rows + (K & 1) * (RPW * 64);
+        if (w == K) {
+            // pivot columns RPW K .. RPW K + RPW-1 inside the wavefront
+#pragma unroll
+            for (int kk = 0; kk < RPW; kk++) {
+                const int k = K * RPW + kk;
+                const double pv = readlane_f64(m[kk], k);
+                const double rcp = (fabs(pv) > tiny && isfinite(pv)) ? 1.0 / pv : 0.0;
+                m[kk] = m[kk] * rcp;
+#pragma unroll
+                for (int r = 0; r < RPW; r++) {
+                    if (r == kk) continue;
+                    const double a = readlane_f64(m[r], k);
+                    m[r] = fma(-a, m[kk], m[r]);
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < RPW; r++) rows[r * 64 + t] = m[r];
         }
         __syncthreads();
-        const double rj = row[t];
+        if (w != K) {
+            double f[RPW][RPW];
 #pragma unroll
-        for (int r = 0; r < RPW; r++) {
-            const double a = readlane_f64(m[r], k);  // A[4w+r][k], before this step's update
-            const double f = (w == wk && r == rk) ? 0.0 : a;
-            m[r] = fma(-f, rj, m[r]);
+            for (int r = 0; r < RPW; r++)
+#pragma unroll
+                for (int kk = 0; kk < RPW; kk++) f[r][kk] = readlane_f64(m[r], K * RPW + kk);
+#pragma unroll
+            for (int kk = 0; kk < RPW; kk++) {
+                const double p = rows[kk * 64 + t];
+#pragma unroll
+                for (int r = 0; r < RPW; r++) m[r] = fma(-f[r][kk], p, m[r]);
+            }
         }
     }
     if (t >= NM && t - NM < n) {
@@ -263,7 +290,7 @@ __device__ __forceinline__ void chain_block(const CoarsenChain& c, const double*
         __syncthreads();
     }
     if (ci.inv64) {
-        __shared__ double rows[2 * 64];
+        __shared__ double rows[2 * (MG_COARSEST_MAX / (NT / 64)) * 64];
         __shared__ double red[NT / 64];
         coarse_inverse_block<NT>(ci, sc[3], rows, red);
     }
@@ -278,7 +305,7 @@ __global__ void __launch_bounds__(512) k_coarsen_chain(CoarsenChain c, const dou
 // the coarsest level's inverse when no chain builds it (it is a fine level)
 __global__ void __launch_bounds__(CINV_NT) k_coarse_inverse(CoarseInv ci, const double* __restrict__ sc)
 {
-    __shared__ double rows[2 * 64];
+    __shared__ double rows[2 * (MG_COARSEST_MAX / (CINV_NT / 64)) * 64];
     __shared__ double red[CINV_NT / 64];
     coarse_inverse_block<CINV_NT>(ci, sc[3], rows, red);
 }
