@@ -796,19 +796,42 @@ __global__ void k_bicg_x(int ni, int nj, int ld, const double* __restrict__ aP, 
                          double* __restrict__ sc, double* __restrict__ part, int nb,
                          const double* __restrict__ p2, int n2, double* __restrict__ host)
 {
-    __shared__ double lds[8];
+    __shared__ double lds[3 * (BX * BY / 64)];
     CELL_IJ;
+    const bool in = i <= ni && j <= nj;
+    const size_t k = in ? (size_t)i * ld + j : (size_t)ld + 1;  // threads past the level read cell (1, 1)
+    // this cell's values and, on the 5-point operator, the neighbours' x,
+    // ph, sh and activity: loaded before om is known (om only scales them)
+    const bool on = in && act[k] != 0;
+    const double xk0 = x[k], phk = ph[k], shk = sh[k], svk = sv[k], tk = t[k], r0k = r0[k], bk = bv[k], pk = aP[k];
+    const size_t nbr[4] = {k + ld, k - ld, k + 1, k - 1};
+    double cpl[4] = {0.0, 0.0, 0.0, 0.0}, nx[4] = {0.0, 0.0, 0.0, 0.0}, nph[4] = {0.0, 0.0, 0.0, 0.0},
+           nsh[4] = {0.0, 0.0, 0.0, 0.0};
+    bool nact[4] = {false, false, false, false};
+    if (!nine) {
+        cpl[0] = aE[k];
+        cpl[1] = aW[k];
+        cpl[2] = aN[k];
+        cpl[3] = aS[k];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            nact[q] = act[nbr[q]] != 0;
+            nx[q] = x[nbr[q]];
+            nph[q] = ph[nbr[q]];
+            nsh[q] = sh[nbr[q]];
+        }
+    }
     double om;
     if (p2) {
         // om = (t, s) / (t, t) from the up kernel's tile partials, summed by
         // every workgroup in the same order; workgroup 0 stores it
-        double ts = 0.0, tt = 0.0;
+        double tv[2] = {0.0, 0.0};
         for (int q = lin_tid(); q < n2; q += BX * BY) {
-            ts += p2[q];
-            tt += p2[n2 + q];
+            tv[0] += p2[q];
+            tv[1] += p2[n2 + q];
         }
-        ts = block_sum_bcast<BX * BY>(ts, lds);
-        tt = block_sum_bcast<BX * BY>(tt, lds);
+        block_sum_vec_bcast<BX * BY, 2>(tv, lds);
+        const double ts = tv[0], tt = tv[1];
         om = (tt > 0.0 && isfinite(ts / tt)) ? ts / tt : 0.0;
         if (lin_tid() == 0 && blockIdx.x == 0 && blockIdx.y == 0) {
             sc[SC_TS] = ts;
@@ -818,27 +841,32 @@ __global__ void k_bicg_x(int ni, int nj, int ld, const double* __restrict__ aP, 
     } else {
         om = sc[SC_OM];
     }
-    double a = 0.0, c = 0.0, e = 0.0;
-    if (i <= ni && j <= nj) {
-        const size_t k = (size_t)i * ld + j;
+    double sums[3] = {0.0, 0.0, 0.0};  // (r, r), (r0, r), |b - A x|^2
+    if (in) {
         const double al = sc[SC_AL];
         double xk = 0.0;
-        if (act[k]) {
-            xk = x[k] + al * ph[k] + om * sh[k];
-            const double q = sv[k] - om * t[k];
+        if (on) {
+            xk = xk0 + al * phk + om * shk;
+            const double q = svk - om * tk;
             r[k] = q;
-            a = q * q;
-            c = r0[k] * q;
-            auto xnew = [&](size_t m) { return act[m] ? x[m] + al * ph[m] + om * sh[m] : 0.0; };
-            const double rt = bv[k] - aP[k] * xk + offdiag(aE, aW, aN, aS, nine, n0, k, ld, xnew);
-            e = rt * rt;
+            sums[0] = q * q;
+            sums[1] = r0k * q;
+            double off;
+            if (!nine) {
+                auto xnb = [&](int m) { return nact[m] ? nx[m] + al * nph[m] + om * nsh[m] : 0.0; };
+                off = cpl[0] * xnb(0) + cpl[1] * xnb(1) + cpl[2] * xnb(2) + cpl[3] * xnb(3);
+            } else {
+                auto xnew = [&](size_t m) { return act[m] ? x[m] + al * ph[m] + om * sh[m] : 0.0; };
+                off = offdiag(aE, aW, aN, aS, nine, n0, k, ld, xnew);
+            }
+            const double rt = bk - pk * xk + off;
+            sums[2] = rt * rt;
         }
         xn[k] = xk;
     }
-    double t1 = block_sum<BX * BY>(a, lds);
-    double t2 = block_sum<BX * BY>(c, lds);
-    double t3 = block_sum<BX * BY>(e, lds);
+    block_sum_vec_bcast<BX * BY, 3>(sums, lds);
     if (lin_tid() == 0) {
+        const double t1 = sums[0], t2 = sums[1], t3 = sums[2];
         const int bid = blockIdx.y * gridDim.x + blockIdx.x;
         part[bid] = t1;
         part[nb + bid] = t2;

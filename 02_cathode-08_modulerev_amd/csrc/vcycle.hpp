@@ -84,11 +84,55 @@ __global__ void __launch_bounds__(NT)
     const int gj0 = bj * TJ + 1 - H;
     T cP[K], cE[K], cW[K], cN[K], cS[K], cB[K];
     unsigned flags[K];  // bit0 active, bit1 colour, bit2 updatable (not on the region rim), bit3 tile cell
+    // fused BiCGStab inputs of the down prologue (FP64 r, v and the old p),
+    // loaded before the scalars that combine them are known
+    double fr[K], fv[K], fp[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int c = tid + k * NT;
+        flags[k] = 0;
+        cP[k] = T(1);
+        cE[k] = cW[k] = cN[k] = cS[k] = cB[k] = T(0);
+        if (c >= R) continue;
+        const int ri = c / RJ, rj = c - ri * RJ;
+        const int gi = gi0 + ri, gj = gj0 + rj;
+        T xv = T(0);
+        if (gi >= 1 && gi <= F.ni && gj >= 1 && gj <= F.nj) {
+            const size_t g = (size_t)gi * F.ld + gj;
+            if (F.act[g]) {
+                flags[k] |= 1u;
+                cP[k] = F.aP[g];
+                cE[k] = F.aE[g];
+                cW[k] = F.aW[g];
+                cN[k] = F.aN[g];
+                cS[k] = F.aS[g];
+                if constexpr (REAL && DOWN && TOP) {
+                    if (fu.pro) {
+                        fr[k] = fu.r[g];
+                        fv[k] = (fu.pro == 2 || !fu.first) ? fu.v[g] : 0.0;
+                        fp[k] = (fu.pro == 1 && !fu.first) ? fu.pin[g] : 0.0;
+                    } else {
+                        cB[k] = F.b[g];
+                    }
+                } else {
+                    cB[k] = F.b[g];
+                }
+                if (!DOWN) {
+                    const int I = fi == 2 ? (gi + 1) >> 1 : gi, J = fj == 2 ? (gj + 1) >> 1 : gj;
+                    xv = F.x[g] + cbx[(size_t)I * cld + J];
+                }
+            }
+        }
+        flags[k] |= ((gi + gj) & 1) ? 2u : 0u;
+        if (ri >= 1 && ri <= RI - 2 && rj >= 1 && rj <= RJ - 2) flags[k] |= 4u;
+        if (ri >= H && ri < H + TI && rj >= H && rj < H + TJ && gi <= F.ni && gj <= F.nj) flags[k] |= 8u;
+        xs[c] = xv;
+    }
     // fused BiCGStab scalars of the down prologue: q = r - al v (pro 2) or
     // r + beta (p_old - om v) (pro 1); f scales the FP32 input
-    double f_al = 0.0, f_beta = 0.0, f_om = 0.0, f_in = 1.0;
     if constexpr (REAL && DOWN && TOP) {
         if (fu.pro) {
+            double f_al = 0.0, f_beta = 0.0, f_om = 0.0, f_in = 1.0;
             const bool lead = tid == 0 && blockIdx.x == 0 && blockIdx.y == 0;
             if (fu.pred) {
                 __shared__ double pr[NT / 64];
@@ -113,59 +157,27 @@ __global__ void __launch_bounds__(NT)
             }
             f_om = fu.sc[fu.s_om];
             f_in = fu.mixed ? fu.sc[fu.s_s32] * fu.sc[fu.s_scale] : 1.0;
-        }
-    }
+            // q = r - al v (pro 2), r (first), r + beta (p_old - om v) (pro 1)
 #pragma unroll
-    for (int k = 0; k < K; k++) {
-        const int c = tid + k * NT;
-        flags[k] = 0;
-        cP[k] = T(1);
-        cE[k] = cW[k] = cN[k] = cS[k] = cB[k] = T(0);
-        if (c >= R) continue;
-        const int ri = c / RJ, rj = c - ri * RJ;
-        const int gi = gi0 + ri, gj = gj0 + rj;
-        T xv = T(0);
-        if (gi >= 1 && gi <= F.ni && gj >= 1 && gj <= F.nj) {
-            const size_t g = (size_t)gi * F.ld + gj;
-            const bool owner = ri >= H && ri < H + TI && rj >= H && rj < H + TJ;
-            if (F.act[g]) {
-                flags[k] |= 1u;
-                cP[k] = F.aP[g];
-                cE[k] = F.aE[g];
-                cW[k] = F.aW[g];
-                cN[k] = F.aN[g];
-                cS[k] = F.aS[g];
-                if constexpr (REAL && DOWN && TOP) {
-                    if (fu.pro) {
-                        double q;
-                        if (fu.pro == 2) {
-                            q = fu.r[g] - f_al * fu.v[g];
-                        } else if (fu.first) {
-                            q = fu.r[g];
-                        } else {
-                            q = fu.r[g] + f_beta * (fu.pin[g] - f_om * fu.v[g]);
-                        }
-                        cB[k] = (T)(q * f_in);
-                        if (owner) {  // the old p is read from pin, so no tile reads what another writes
-                            fu.ps[g] = q;
-                            F.b[g] = cB[k];
-                        }
-                    } else {
-                        cB[k] = F.b[g];
-                    }
+            for (int k = 0; k < K; k++) {
+                if (!(flags[k] & 1u)) continue;
+                double q;
+                if (fu.pro == 2) {
+                    q = fr[k] - f_al * fv[k];
+                } else if (fu.first) {
+                    q = fr[k];
                 } else {
-                    cB[k] = F.b[g];
+                    q = fr[k] + f_beta * (fp[k] - f_om * fv[k]);
                 }
-                if (!DOWN) {
-                    const int I = fi == 2 ? (gi + 1) >> 1 : gi, J = fj == 2 ? (gj + 1) >> 1 : gj;
-                    xv = F.x[g] + cbx[(size_t)I * cld + J];
+                cB[k] = (T)(q * f_in);
+                if (flags[k] & 8u) {  // the old p is read from pin, so no tile reads what another writes
+                    const int c = tid + k * NT, ri = c / RJ, rj = c - ri * RJ;
+                    const size_t g = (size_t)(gi0 + ri) * F.ld + (gj0 + rj);
+                    fu.ps[g] = q;
+                    F.b[g] = cB[k];
                 }
             }
         }
-        flags[k] |= ((gi + gj) & 1) ? 2u : 0u;
-        if (ri >= 1 && ri <= RI - 2 && rj >= 1 && rj <= RJ - 2) flags[k] |= 4u;
-        if (ri >= H && ri < H + TI && rj >= H && rj < H + TJ && gi <= F.ni && gj <= F.nj) flags[k] |= 8u;
-        xs[c] = xv;
     }
     __syncthreads();
 #pragma unroll
