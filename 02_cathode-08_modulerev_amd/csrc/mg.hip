@@ -1031,6 +1031,13 @@ void MGSolver::configure(int ni, int nj, hipStream_t s)
     CATHODE_HIP(hipMemset(bsol.p, 0, n0 * sizeof(double)));
     CATHODE_HIP(hipMemset(xsol.p, 0, n0 * sizeof(double)));
     w0.alloc(nj + 2);
+    // pairs of fine levels in one launch each way (CATHODE_PAIR=0 turns this off)
+    {
+        const char* e = std::getenv("CATHODE_PAIR");
+        pair_on = !(e && e[0] == '0');
+        const char* c = std::getenv("CATHODE_PAIR_TILE");
+        pair_cfg = (c && *c) ? std::atoi(c) : 0;
+    }
     // levels from lt on run inside the one-workgroup tail kernel
     lt = nl - 1;
     // the tail's first level may hold TAIL_TOP_MAX cells, every further one TAILR_NT
@@ -1181,15 +1188,32 @@ template <class T>
 void MGSolver::vcycle(Level<T>* L, hipStream_t s, const VFuse& fdown, const VFuse& fup)
 {
     const double* w = user_w ? user_w : w0.p;
-    // fine levels: fused down kernels
-    for (int l = 0; l < lt; l++) {
+    auto vlevel = [&](int l) {
         Level<T>& A = L[l];
+        return VLevel<T>{A.ni, A.nj, A.ld, A.aP, A.aE, A.aW, A.aN, A.aS, actall.p + level_off[l], A.b, A.x, A.xu};
+    };
+    // fine levels: fused down kernels; below the finest, a level that
+    // coarsens 2x2 into a next fine level runs with it (pair_start[l])
+    bool pair_start[MG_MAX_LEVELS] = {};
+    for (int l = 0; l < lt;) {
+        Level<T>& A = L[l];
+        if (pair_on && l >= 1 && l + 1 < lt && A.cx && A.cy) {
+            Level<T>& C = L[l + 1];
+            Level<T>& D = L[l + 2];
+            pair_start[l] = true;
+            launch_vpair<T>(pair_cfg, true, s, vlevel(l), vlevel(l + 1), C.cx ? 2 : 1, C.cy ? 2 : 1, D.ni, D.nj, D.ld,
+                            D.b);
+            l += 2;
+            continue;
+        }
         Level<T>& C = L[l + 1];
-        VLevel<T> V{A.ni, A.nj, A.ld, A.aP, A.aE, A.aW, A.aN, A.aS, actall.p + level_off[l], A.b, A.x, nullptr};
+        VLevel<T> V = vlevel(l);
+        V.xo = nullptr;
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (prof.enabled && l == 0) prof_events(e0, e1);
         launch_vtile<T, true>(tile_of<T>(l), l == 0, s, V, A.cx ? 2 : 1, A.cy ? 2 : 1, w, C.ni, C.nj, C.ld, C.b,
                               l == 0 ? fdown : VFuse(), e0, e1);
+        l++;
     }
     // coarse levels in one workgroup
     TailArgs<T> ta{};
@@ -1223,14 +1247,23 @@ void MGSolver::vcycle(Level<T>* L, hipStream_t s, const VFuse& fdown, const VFus
         ((size_t)off * 2 + std::max(top, TAILR_NT) + (size_t)CINV_LD * MG_COARSEST_MAX) * sizeof(T) + 16;
     if (top <= TAILR_NT) launch_tail<T, 1>(ta, lds, s);
     else launch_tail<T, 2>(ta, lds, s);
-    // fine levels: fused up kernels
-    for (int l = lt - 1; l >= 0; l--) {
+    // fine levels: fused up kernels; the coarse correction is the next
+    // level's final x (xu of a fine level or of a pair's first level, x of
+    // the tail's first level)
+    for (int l = lt - 1; l >= 0;) {
+        if (l >= 1 && pair_start[l - 1]) {
+            Level<T>& C = L[l];
+            Level<T>& D = L[l + 1];
+            launch_vpair<T>(pair_cfg, false, s, vlevel(l - 1), vlevel(l), C.cx ? 2 : 1, C.cy ? 2 : 1, D.ni, D.nj, D.ld,
+                            l + 1 < lt ? D.xu : D.x);
+            l -= 2;
+            continue;
+        }
         Level<T>& A = L[l];
         Level<T>& C = L[l + 1];
-        // the coarse correction: the next level's final x (the tail writes x)
-        VLevel<T> V{A.ni, A.nj, A.ld, A.aP, A.aE, A.aW, A.aN, A.aS, actall.p + level_off[l], A.b, A.x, A.xu};
-        launch_vtile<T, false>(tile_of<T>(l), l == 0, s, V, A.cx ? 2 : 1, A.cy ? 2 : 1, (const double*)nullptr, C.ni,
-                               C.nj, C.ld, l + 1 < lt ? C.xu : C.x, l == 0 ? fup : VFuse());
+        launch_vtile<T, false>(tile_of<T>(l), l == 0, s, vlevel(l), A.cx ? 2 : 1, A.cy ? 2 : 1, (const double*)nullptr,
+                               C.ni, C.nj, C.ld, l + 1 < lt ? C.xu : C.x, l == 0 ? fup : VFuse());
+        l--;
     }
     CATHODE_LAUNCH_CHECK();
 }

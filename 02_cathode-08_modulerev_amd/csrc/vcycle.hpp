@@ -320,6 +320,293 @@ inline int vtile_count(int cfg, int ni, int nj)
     return cdiv(nj, G.tj) * cdiv(ni, G.ti);
 }
 
+// ------------------------------------------------------------------ two fine levels per launch
+// A fine level F that coarsens 2x2 into the next level C runs, with C, in one
+// down and one up launch instead of two each.  A workgroup owns a TI x TJ
+// tile of C and the 2TI x 2TJ tile of F it aggregates.  Every value it
+// writes is the one the single-level kernels write (same operations in the
+// same order), so the results are bitwise those of the single-level pair.
+//
+// Down (halo 5 per level): the C region is the C tile plus 5 rings; its b is
+// the restriction of F's residuals on the children of the C region, which
+// F's 2 sweeps leave exact on F's region shrunk by 5 rings, so F's region is
+// those children plus 5 rings.  Writes F's x and C's b and x on the owned
+// tiles and the restriction of C's residual into the level below C (cb).
+//
+// Up (halo 4 per level): F's region is its tile plus 4 rings; the C cells
+// that are parents of it (C tile plus 2 rings) must hold their
+// post-smoothed x, so C's region is those plus 4 rings.  C starts from its
+// down x plus the correction from the level below (cx3); F from its down x
+// plus C's post-smoothed x.  Writes F's xo on the owned tile (C's
+// post-smoothed x has no reader outside the launch).
+
+// one level's region in LDS: coefficients of the NT-strided cells in registers
+template <class T, int RI, int RJ, int NT>
+struct PairRegion {
+    static constexpr int R = RI * RJ, K = (R + NT - 1) / NT;
+    T cP[K], cE[K], cW[K], cN[K], cS[K], cB[K];
+    unsigned flags[K];  // bit0 active, bit1 colour, bit2 updatable (not on the region rim)
+};
+
+// the coefficients of region cells (gi0, gj0: level index of region row /
+// column 0); cells outside the level are inactive
+template <class T, int RI, int RJ, int NT>
+__device__ inline void pair_load(PairRegion<T, RI, RJ, NT>& G, const VLevel<T>& L, int gi0, int gj0)
+{
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < G.K; k++) {
+        const int c = tid + k * NT;
+        G.flags[k] = 0;
+        G.cP[k] = T(1);
+        G.cE[k] = G.cW[k] = G.cN[k] = G.cS[k] = G.cB[k] = T(0);
+        if (c >= G.R) continue;
+        const int ri = c / RJ, rj = c - ri * RJ;
+        const int gi = gi0 + ri, gj = gj0 + rj;
+        if (gi >= 1 && gi <= L.ni && gj >= 1 && gj <= L.nj) {
+            const size_t g = (size_t)gi * L.ld + gj;
+            if (L.act[g]) {
+                G.flags[k] |= 1u;
+                G.cP[k] = L.aP[g];
+                G.cE[k] = L.aE[g];
+                G.cW[k] = L.aW[g];
+                G.cN[k] = L.aN[g];
+                G.cS[k] = L.aS[g];
+            }
+        }
+        G.flags[k] |= ((gi + gj) & 1) ? 2u : 0u;
+        if (ri >= 1 && ri <= RI - 2 && rj >= 1 && rj <= RJ - 2) G.flags[k] |= 4u;
+    }
+}
+
+// 4 half sweeps (red first) of the region in xs; ends with a barrier
+template <class T, int RI, int RJ, int NT>
+__device__ inline void pair_sweeps4(const PairRegion<T, RI, RJ, NT>& G, T* xs)
+{
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+        const unsigned col = (unsigned)(s & 1) << 1;
+#pragma unroll
+        for (int k = 0; k < G.K; k++) {
+            const int c = tid + k * NT;
+            if ((G.flags[k] & 7u) == (5u | col)) {
+                T v = G.cB[k] + G.cE[k] * xs[c + RJ] + G.cW[k] * xs[c - RJ] + G.cN[k] * xs[c + 1] +
+                      G.cS[k] * xs[c - 1];
+                xs[c] = v / G.cP[k];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <class T>
+__device__ inline T pair_residual(T b, T P, T E, T W, T N, T S, const T* xs, int c, int RJ)
+{
+    return b - P * xs[c] + E * xs[c + RJ] + W * xs[c - RJ] + N * xs[c + 1] + S * xs[c - 1];
+}
+
+template <class T, int TI, int TJ, int NT>
+__global__ void __launch_bounds__(NT)
+    k_vpair_down(VLevel<T> F, VLevel<T> C, int fi3, int fj3, int c3ni, int c3nj, int c3ld, T* __restrict__ cb3)
+{
+    constexpr int H = 5;
+    constexpr int CI = TI + 2 * H, CJ = TJ + 2 * H;      // C region
+    constexpr int QI = 2 * CI, QJ = 2 * CJ;              // F cells restricted into it
+    constexpr int FI = QI + 2 * H, FJ = QJ + 2 * H;      // F region
+    __shared__ T xf[FI * FJ];
+    __shared__ T rq[QI * QJ];
+    __shared__ T xc[CI * CJ];
+    __shared__ T rc[TI * TJ];
+    const int tid = threadIdx.x;
+    int bi, bj;  // this workgroup's C tile
+    xcd_tile(bi, bj);
+    const int ci0 = bi * TI + 1 - H, cj0 = bj * TJ + 1 - H;  // C index of C region row / column 0
+    const int fi0 = 2 * ci0 - 1 - H, fj0 = 2 * cj0 - 1 - H;  // F index of F region row / column 0
+    PairRegion<T, FI, FJ, NT> Gf;
+    PairRegion<T, CI, CJ, NT> Gc;
+    pair_load(Gf, F, fi0, fj0);
+    pair_load(Gc, C, ci0, cj0);
+#pragma unroll
+    for (int k = 0; k < Gf.K; k++) {
+        const int c = tid + k * NT;
+        if (c >= Gf.R) continue;
+        if (Gf.flags[k] & 1u) {
+            const int ri = c / FJ, rj = c - ri * FJ;
+            Gf.cB[k] = F.b[(size_t)(fi0 + ri) * F.ld + (fj0 + rj)];
+        }
+        xf[c] = T(0);
+    }
+    for (int c = tid; c < CI * CJ; c += NT) xc[c] = T(0);
+    __syncthreads();
+    pair_sweeps4(Gf, xf);
+    // F's x on its tile, F's residuals on the children of the C region
+#pragma unroll
+    for (int k = 0; k < Gf.K; k++) {
+        const int c = tid + k * NT;
+        if (c >= Gf.R) continue;
+        const int ri = c / FJ, rj = c - ri * FJ;
+        const int gi = fi0 + ri, gj = fj0 + rj;
+        if (ri >= 3 * H && ri < 3 * H + 2 * TI && rj >= 3 * H && rj < 3 * H + 2 * TJ && gi <= F.ni && gj <= F.nj)
+            F.x[(size_t)gi * F.ld + gj] = xf[c];
+        if (ri >= H && ri < H + QI && rj >= H && rj < H + QJ) {
+            T r = T(0);
+            if (Gf.flags[k] & 1u) r = pair_residual(Gf.cB[k], Gf.cP[k], Gf.cE[k], Gf.cW[k], Gf.cN[k], Gf.cS[k], xf, c, FJ);
+            rq[(ri - H) * QJ + (rj - H)] = r;
+        }
+    }
+    __syncthreads();
+    // C's b: the sum of its children's residuals (children inside F)
+#pragma unroll
+    for (int k = 0; k < Gc.K; k++) {
+        const int c = tid + k * NT;
+        if (c >= Gc.R) continue;
+        const int ri = c / CJ, rj = c - ri * CJ;
+        const int gi = ci0 + ri, gj = cj0 + rj;
+        if (gi < 1 || gi > C.ni || gj < 1 || gj > C.nj) continue;
+        T sum = T(0);
+        for (int a = 0; a < 2; a++)
+            for (int b = 0; b < 2; b++) {
+                const int fi = 2 * gi - 1 + a, fj = 2 * gj - 1 + b;
+                if (fi <= F.ni && fj <= F.nj) sum += rq[(2 * ri + a) * QJ + (2 * rj + b)];
+            }
+        if (Gc.flags[k] & 1u) Gc.cB[k] = sum;
+        if (ri >= H && ri < H + TI && rj >= H && rj < H + TJ) C.b[(size_t)gi * C.ld + gj] = sum;
+    }
+    pair_sweeps4(Gc, xc);
+#pragma unroll
+    for (int k = 0; k < Gc.K; k++) {
+        const int c = tid + k * NT;
+        if (c >= Gc.R) continue;
+        const int ri = c / CJ, rj = c - ri * CJ;
+        const int gi = ci0 + ri, gj = cj0 + rj;
+        if (!(ri >= H && ri < H + TI && rj >= H && rj < H + TJ && gi <= C.ni && gj <= C.nj)) continue;
+        C.x[(size_t)gi * C.ld + gj] = xc[c];
+        T r = T(0);
+        if (Gc.flags[k] & 1u) r = pair_residual(Gc.cB[k], Gc.cP[k], Gc.cE[k], Gc.cW[k], Gc.cN[k], Gc.cS[k], xc, c, CJ);
+        rc[(ri - H) * TJ + (rj - H)] = r;
+    }
+    __syncthreads();
+    // the level below C: coarse cells of this tile
+    const int KI = TI / fi3, KJ = TJ / fj3;
+    const int I0 = bi * KI, J0 = bj * KJ;
+    for (int q = tid; q < KI * KJ; q += NT) {
+        const int il = q / KJ, jl = q - il * KJ;
+        const int I = I0 + il + 1, J = J0 + jl + 1;
+        if (I > c3ni || J > c3nj) continue;
+        T sum = T(0);
+        for (int a = 0; a < fi3; a++)
+            for (int b = 0; b < fj3; b++) {
+                const int ti = il * fi3 + a, tj = jl * fj3 + b;
+                const int gi = bi * TI + ti + 1, gj = bj * TJ + tj + 1;
+                if (gi <= C.ni && gj <= C.nj) sum += rc[ti * TJ + tj];
+            }
+        cb3[(size_t)I * c3ld + J] = sum;
+    }
+}
+
+template <class T, int TI, int TJ, int NT>
+__global__ void __launch_bounds__(NT)
+    k_vpair_up(VLevel<T> F, VLevel<T> C, int fi3, int fj3, int c3ld, const T* __restrict__ cx3)
+{
+    constexpr int H = 4;
+    constexpr int FI = 2 * TI + 2 * H, FJ = 2 * TJ + 2 * H;  // F region
+    constexpr int CI = TI + 2 + 2 * H + 2, CJ = TJ + 2 + 2 * H + 2;  // C region: parents of F's region + 4 rings
+    __shared__ T xf[FI * FJ];
+    __shared__ T xc[CI * CJ];
+    const int tid = threadIdx.x;
+    int bi, bj;
+    xcd_tile(bi, bj);
+    const int fi0 = 2 * bi * TI + 1 - H, fj0 = 2 * bj * TJ + 1 - H;  // F index of F region row / column 0
+    const int ci0 = bi * TI + 1 - 2 - H, cj0 = bj * TJ + 1 - 2 - H;  // C index of C region row / column 0
+    PairRegion<T, FI, FJ, NT> Gf;
+    PairRegion<T, CI, CJ, NT> Gc;
+    pair_load(Gf, F, fi0, fj0);
+    pair_load(Gc, C, ci0, cj0);
+    T xdf[Gf.K];  // F's down x, corrected once C is post-smoothed
+#pragma unroll
+    for (int k = 0; k < Gf.K; k++) {
+        xdf[k] = T(0);
+        const int c = tid + k * NT;
+        if (c >= Gf.R || !(Gf.flags[k] & 1u)) continue;
+        const int ri = c / FJ, rj = c - ri * FJ;
+        const size_t g = (size_t)(fi0 + ri) * F.ld + (fj0 + rj);
+        Gf.cB[k] = F.b[g];
+        xdf[k] = F.x[g];
+    }
+#pragma unroll
+    for (int k = 0; k < Gc.K; k++) {
+        const int c = tid + k * NT;
+        if (c >= Gc.R) continue;
+        T xv = T(0);
+        if (Gc.flags[k] & 1u) {
+            const int ri = c / CJ, rj = c - ri * CJ;
+            const int gi = ci0 + ri, gj = cj0 + rj;
+            const size_t g = (size_t)gi * C.ld + gj;
+            Gc.cB[k] = C.b[g];
+            const int I = fi3 == 2 ? (gi + 1) >> 1 : gi, J = fj3 == 2 ? (gj + 1) >> 1 : gj;
+            xv = C.x[g] + cx3[(size_t)I * c3ld + J];
+        }
+        xc[c] = xv;
+    }
+    __syncthreads();
+    pair_sweeps4(Gc, xc);
+    // F: down x plus the parent's post-smoothed x
+#pragma unroll
+    for (int k = 0; k < Gf.K; k++) {
+        const int c = tid + k * NT;
+        if (c >= Gf.R) continue;
+        T xv = T(0);
+        if (Gf.flags[k] & 1u) {
+            const int ri = c / FJ, rj = c - ri * FJ;
+            const int gi = fi0 + ri, gj = fj0 + rj;
+            const int pi = ((gi + 1) >> 1) - ci0, pj = ((gj + 1) >> 1) - cj0;
+            xv = xdf[k] + xc[pi * CJ + pj];
+        }
+        xf[c] = xv;
+    }
+    __syncthreads();
+    pair_sweeps4(Gf, xf);
+#pragma unroll
+    for (int k = 0; k < Gf.K; k++) {
+        const int c = tid + k * NT;
+        if (c >= Gf.R) continue;
+        const int ri = c / FJ, rj = c - ri * FJ;
+        const int gi = fi0 + ri, gj = fj0 + rj;
+        if (ri >= H && ri < H + 2 * TI && rj >= H && rj < H + 2 * TJ && gi <= F.ni && gj <= F.nj)
+            F.xo[(size_t)gi * F.ld + gj] = xf[c];
+    }
+}
+
+// the pair kernels' tiles of the second level and workgroup sizes
+constexpr int VP_NCFG = 3;
+constexpr TileCfg VP_CFGS[VP_NCFG] = {{16, 16, 1024}, {8, 16, 512}, {8, 16, 1024}};
+
+template <class T, int CFG>
+inline void launch_vpair_cfg(bool down, hipStream_t s, const VLevel<T>& F, const VLevel<T>& C, int fi3, int fj3,
+                             int c3ni, int c3nj, int c3ld, T* c3)
+{
+    constexpr TileCfg G = VP_CFGS[CFG];
+    const dim3 grid(cdiv(C.nj, G.tj), cdiv(C.ni, G.ti));
+    if (down)
+        hipLaunchKernelGGL((k_vpair_down<T, G.ti, G.tj, G.nt>), grid, dim3(G.nt), 0, s, F, C, fi3, fj3, c3ni, c3nj,
+                           c3ld, c3);
+    else
+        hipLaunchKernelGGL((k_vpair_up<T, G.ti, G.tj, G.nt>), grid, dim3(G.nt), 0, s, F, C, fi3, fj3, c3ld,
+                           (const T*)c3);
+}
+
+template <class T>
+inline void launch_vpair(int cfg, bool down, hipStream_t s, const VLevel<T>& F, const VLevel<T>& C, int fi3, int fj3,
+                         int c3ni, int c3nj, int c3ld, T* c3)
+{
+    switch (cfg) {
+    case 1: launch_vpair_cfg<T, 1>(down, s, F, C, fi3, fj3, c3ni, c3nj, c3ld, c3); break;
+    case 2: launch_vpair_cfg<T, 2>(down, s, F, C, fi3, fj3, c3ni, c3nj, c3ld, c3); break;
+    default: launch_vpair_cfg<T, 0>(down, s, F, C, fi3, fj3, c3ni, c3nj, c3ld, c3); break;
+    }
+}
+
 // ------------------------------------------------------------------ coarse levels in one workgroup
 template <class T>
 struct TailArgs {
