@@ -93,34 +93,55 @@ __global__ void k_fill(double* p, int n, double v)
     if (k < n) p[k] = v;
 }
 
-// coarse operator of coarse cell (I, J) from level F
-__device__ inline void coarsen_cell(const CoarsenArgs& a, int I, int J, double s32, int mixed)
+// one child of a coarse cell as the coarsening reads it
+struct ChildVals {
+    bool on;                     // present and active
+    double p, e, w, n, s, m, wj;  // aP aE aW aN aS, mass, radial weight
+};
+
+// a level's block in LDS (k_setup_levels): cell (i, j) at (i - i0) * cols + (j - j0)
+struct LevelStage {
+    double *P, *E, *W, *N, *S, *M;
+    unsigned char* act;
+    int i0, j0, cols;
+};
+
+// coarse operator of coarse cell (I, J) from its (up to) 2 x 2 children in
+// row-major order, fetch(i, j, here, ci0, cj0) giving each child's values
+// (an absent child, here = false, reads child (ci0, cj0) and drops out, so
+// every load can be issued before any is tested); writes the coarse level
+// and, with out set, the values a further coarsening reads into out
+template <class Fetch>
+__device__ inline void coarsen_cell_f(const CoarsenArgs& a, int I, int J, double s32, int mixed, Fetch fetch,
+                                      const LevelStage* out)
 {
     const int i0 = a.fi * (I - 1) + 1, i1 = min(a.fi * I, a.fni);
     const int j0 = a.fj * (J - 1) + 1, j1 = min(a.fj * J, a.fnj);
     double S = 0.0, M = 0.0, E = 0.0, W = 0.0, N = 0.0, So = 0.0;
     int any = 0;
-    // the (up to) 2 x 2 children in row-major order; every load is issued
-    // before any is tested (an absent child reads child (i0, j0) and drops out)
+    ChildVals ch[4];
 #pragma unroll
     for (int di = 0; di < 2; di++)
 #pragma unroll
         for (int dj = 0; dj < 2; dj++) {
             const int i = i0 + di, j = j0 + dj;
-            const bool here = i <= i1 && j <= j1;
-            const size_t k = here ? (size_t)i * a.fld + j : (size_t)i0 * a.fld + j0;
-            const bool on = here && a.fact[k] != 0;
-            const double wj = a.w ? a.w[here ? j : j0] : 1.0;
-            const double p = a.faP[k], e = a.faE[k], ww = a.faW[k], n = a.faN[k], s = a.faS[k], m = a.fm[k];
-            if (!on) continue;
+            ch[2 * di + dj] = fetch(i, j, i <= i1 && j <= j1, i0, j0);
+        }
+#pragma unroll
+    for (int di = 0; di < 2; di++)
+#pragma unroll
+        for (int dj = 0; dj < 2; dj++) {
+            const int i = i0 + di, j = j0 + dj;
+            const ChildVals& c = ch[2 * di + dj];
+            if (!c.on) continue;
             any = 1;
-            const double ex = p - (e + ww + n + s);
-            S += wj * (ex > 0.0 ? ex : 0.0);
-            M += m;
-            if (i == i1) E += wj * e;
-            if (i == i0) W += wj * ww;
-            if (j == j1) N += wj * n;
-            if (j == j0) So += wj * s;
+            const double ex = c.p - (c.e + c.w + c.n + c.s);
+            S += c.wj * (ex > 0.0 ? ex : 0.0);
+            M += c.m;
+            if (i == i1) E += c.wj * c.e;
+            if (i == i0) W += c.wj * c.w;
+            if (j == j1) N += c.wj * c.n;
+            if (j == j0) So += c.wj * c.s;
         }
     const size_t kc = (size_t)I * a.cld + J;
     a.cact[kc] = (unsigned char)any;
@@ -130,15 +151,15 @@ __device__ inline void coarsen_cell(const CoarsenArgs& a, int I, int J, double s
     if (!any) {
         P = 1.0;
         E = W = N = So = 0.0;
-        a.cm[kc] = 0.0;
+        M = 0.0;
     } else {
         E *= a.hx;
         W *= a.hx;
         N *= a.hy;
         So *= a.hy;
-        a.cm[kc] = M;
         P = (M + a.hb * (S - M)) + (E + W + N + So);
     }
+    a.cm[kc] = M;
     a.caP[kc] = P;
     a.caE[kc] = E;
     a.caW[kc] = W;
@@ -151,6 +172,37 @@ __device__ inline void coarsen_cell(const CoarsenArgs& a, int I, int J, double s
         a.gN[kc] = (float)(N * s32);
         a.gS[kc] = (float)(So * s32);
     }
+    if (out) {
+        const int q = (I - out->i0) * out->cols + (J - out->j0);
+        out->act[q] = (unsigned char)any;
+        out->P[q] = P;
+        out->E[q] = E;
+        out->W[q] = W;
+        out->N[q] = N;
+        out->S[q] = So;
+        out->M[q] = M;
+    }
+}
+
+// the same from the fine level in global memory
+__device__ inline void coarsen_cell(const CoarsenArgs& a, int I, int J, double s32, int mixed)
+{
+    coarsen_cell_f(
+        a, I, J, s32, mixed,
+        [&](int i, int j, bool here, int ci0, int cj0) {
+            const size_t k = here ? (size_t)i * a.fld + j : (size_t)ci0 * a.fld + cj0;
+            ChildVals c;
+            c.on = here && a.fact[k] != 0;
+            c.wj = a.w ? a.w[here ? j : cj0] : 1.0;
+            c.p = a.faP[k];
+            c.e = a.faE[k];
+            c.w = a.faW[k];
+            c.n = a.faN[k];
+            c.s = a.faS[k];
+            c.m = a.fm[k];
+            return c;
+        },
+        (const LevelStage*)nullptr);
 }
 
 __global__ void k_coarsen(CoarsenArgs a, const double* __restrict__ sc, int mixed)
@@ -329,6 +381,9 @@ __device__ inline double warm_ay(const WarmSet& ws, int q, int i, int j, size_t 
 // level by level with only its own barriers.
 constexpr int SETUP_FUSE_MAX = 3;
 constexpr int SETUP_NT = 256;
+constexpr int SETUP_STAGE0 = 16 * 32;           // a workgroup's level-0 block
+constexpr int SETUP_STAGE1 = SETUP_STAGE0 / 2;  // the next level's (at least one direction coarsens)
+constexpr int SETUP_STAGE_COLS = 32;            // level-0 block columns (radial weights)
 struct SetupLevels {
     int ni, nj, ld;
     const unsigned char* act;
@@ -368,6 +423,22 @@ __global__ void __launch_bounds__(SETUP_NT) k_setup_levels(SetupLevels a, double
         sc[2] = bb > 0.0 ? sqrt(a.nact / bb) : 1.0;   // mixed input scale
         sc[3] = s32;
     }
+    // Each level of this workgroup's blocks is also staged in LDS (act, aP..aS,
+    // mass; level l in buffer set l & 1), so every coarsening reads its
+    // children there instead of waiting for the global stores of the level
+    // before.  Staged only when the blocks fit the buffers (always, for the
+    // 16 x 32 level-0 block of every fused depth).
+    __shared__ double st0[6 * SETUP_STAGE0], st1[6 * SETUP_STAGE1];
+    __shared__ unsigned char sa0[SETUP_STAGE0], sa1[SETUP_STAGE1];
+    __shared__ double wst[SETUP_STAGE_COLS];
+    auto stage = [&](int l, int i0, int j0, int cols) {
+        double* b = (l & 1) ? st1 : st0;
+        const int n = (l & 1) ? SETUP_STAGE1 : SETUP_STAGE0;
+        return LevelStage{b, b + n, b + 2 * n, b + 3 * n, b + 4 * n, b + 5 * n, (l & 1) ? sa1 : sa0, i0, j0, cols};
+    };
+    bool staged = a.rj * a.pj[0] <= SETUP_STAGE_COLS;
+    for (int l = 0; l <= a.nf; l++)
+        staged = staged && a.ri * a.pi[l] * a.rj * a.pj[l] <= ((l & 1) ? SETUP_STAGE1 : SETUP_STAGE0);
     // level 0: mass part m = w_j min(s) over the 5-point neighbourhood, FP32
     // copy, and the warm start's dot products
     double wd[WARM_NP];
@@ -376,8 +447,11 @@ __global__ void __launch_bounds__(SETUP_NT) k_setup_levels(SetupLevels a, double
     {
         const int rows = a.ri * a.pi[0], cols = a.rj * a.pj[0];
         const int i0 = 1 + (int)blockIdx.y * rows, j0 = 1 + (int)blockIdx.x * cols;
+        const LevelStage S0 = stage(0, i0, j0, cols);
+        if (staged && tid < cols) wst[tid] = j0 + tid <= a.nj + 1 ? a.w[j0 + tid] : 1.0;
         for (int q = tid; q < rows * cols; q += SETUP_NT) {
             const int i = i0 + q / cols, j = j0 + q % cols;
+            if (staged) S0.act[q] = 0;
             if (i > a.ni || j > a.nj) continue;
             const size_t k = (size_t)i * a.ld + j;
             if (mixed) {
@@ -410,7 +484,17 @@ __global__ void __launch_bounds__(SETUP_NT) k_setup_levels(SetupLevels a, double
             if (a.act[k - a.ld]) v = fmin(v, a.s[k - a.ld]);
             if (a.act[k + 1]) v = fmin(v, a.s[k + 1]);
             if (a.act[k - 1]) v = fmin(v, a.s[k - 1]);
-            a.m[k] = a.w[j] * v;
+            const double mk = a.w[j] * v;
+            a.m[k] = mk;
+            if (staged) {
+                S0.act[q] = 1;
+                S0.P[q] = a.aP[k];
+                S0.E[q] = a.aE[k];
+                S0.W[q] = a.aW[k];
+                S0.N[q] = a.aN[k];
+                S0.S[q] = a.aS[k];
+                S0.M[q] = mk;
+            }
         }
     }
     if (a.ws.k > 0) {
@@ -423,14 +507,38 @@ __global__ void __launch_bounds__(SETUP_NT) k_setup_levels(SetupLevels a, double
         }
     }
     for (int l = 0; l < a.nf; l++) {
-        __syncthreads();  // level l of this block written (global, same workgroup)
+        __syncthreads();  // level l of this block staged (LDS) and written (global, same workgroup)
         const CoarsenArgs& c = a.L[l];
+        const int fr = a.ri * a.pi[l], fc = a.rj * a.pj[l];
         const int rows = a.ri * a.pi[l + 1], cols = a.rj * a.pj[l + 1];
         const int I0 = 1 + (int)blockIdx.y * rows, J0 = 1 + (int)blockIdx.x * cols;
+        const LevelStage src = stage(l, 1 + (int)blockIdx.y * fr, 1 + (int)blockIdx.x * fc, fc);
+        const LevelStage dst = stage(l + 1, I0, J0, cols);
         for (int q = tid; q < rows * cols; q += SETUP_NT) {
             const int I = I0 + q / cols, J = J0 + q % cols;
+            if (staged) dst.act[q] = 0;
             if (I > c.cni || J > c.cnj) continue;
-            coarsen_cell(c, I, J, s32, mixed);
+            if (!staged) {
+                coarsen_cell(c, I, J, s32, mixed);
+                continue;
+            }
+            coarsen_cell_f(
+                c, I, J, s32, mixed,
+                [&](int i, int j, bool here, int ci0, int cj0) {
+                    const int ii = here ? i : ci0, jj = here ? j : cj0;
+                    const int lq = (ii - src.i0) * src.cols + (jj - src.j0);
+                    ChildVals v;
+                    v.on = here && src.act[lq] != 0;
+                    v.wj = c.w ? wst[jj - src.j0] : 1.0;  // the weights apply to level 0 only
+                    v.p = src.P[lq];
+                    v.e = src.E[lq];
+                    v.w = src.W[lq];
+                    v.n = src.N[lq];
+                    v.s = src.S[lq];
+                    v.m = src.M[lq];
+                    return v;
+                },
+                &dst);
         }
     }
 }
