@@ -167,11 +167,10 @@ struct PoissonArgs {
     Mob m;
 };
 
-// predicted x flux on faces i = 1..ni+1 (fluidModule.hpp:2266-2462)
-__device__ __forceinline__ void poisson_predict_x(const NddView& V, const PoissonArgs& A, int i, int j)
+// predicted x flux on face (i, j), i = 1..ni+1 (fluidModule.hpp:2266-2462)
+__device__ __forceinline__ double poisson_flux_x(const NddView& V, const PoissonArgs& A, int i, int j)
 {
 #pragma clang fp contract(off)
-    if (i > V.ni + 1 || j > V.nj) return;
     const double dx = V.dx;
     const double* r = V.r;
     if (i >= 2 && i <= V.ni) {
@@ -184,29 +183,28 @@ __device__ __forceinline__ void poisson_predict_x(const NddView& V, const Poisso
         const double tr = (rR * A.nUer_t[IX(i, j + 1)] * bRr + rL * A.nUer_t[IX(i, j)] * bLr +
                            rR * A.nUer_t[IX(i - 1, j + 1)] * bRrD + rL * A.nUer_t[IX(i - 1, j)] * bLrD) /
                           (4.0 * r[j]);
-        A.nUex[IX(i, j)] = A.nUex_t[IX(i, j)] - (A.m.inv_par * A.nUex_t[IX(i, j)] + A.m.inv_cross * tr) * A.m.mu_star +
+        return A.nUex_t[IX(i, j)] - (A.m.inv_par * A.nUex_t[IX(i, j)] + A.m.inv_cross * tr) * A.m.mu_star +
                            A.m.mu_star * (n_f * dphidx - dpedx / K_E0);
     } else if (i == 1) {
         const double Te = A.Te[IX(1, j)];
         const double dphi = fmax(-A.Ex_old[IX(1, j)] * dx / 2.0, 0.0);
         const double gpn = wall_gpn(V, Te, dphi, -1.0);
         const double see = -V.coefIISEE_ts * A.rhoUix[IX(1, j)] - eisee_sum(V, Te) * A.rhoe_old[IX(1, j)] * gpn;
-        A.nUex[IX(1, j)] = A.rhoe[IX(1, j)] * gpn + see;
+        return A.rhoe[IX(1, j)] * gpn + see;
     } else {
         const int c = V.ni;
         const double Te = A.Te[IX(c, j)];
         const double dphi = fmax(A.Ex_old[IX(c + 1, j)] * dx / 2.0, 0.0);
         const double gpn = wall_gpn(V, Te, dphi, 1.0);
         const double see = -V.coefIISEE_ts * A.rhoUix[IX(c + 1, j)] - eisee_sum(V, Te) * A.rhoe_old[IX(c, j)] * gpn;
-        A.nUex[IX(c + 1, j)] = A.rhoe[IX(c, j)] * gpn + see;
+        return A.rhoe[IX(c, j)] * gpn + see;
     }
 }
 
-// predicted r flux on faces j = 2..nj+1 (fluidModule.hpp:2466-2621)
-__device__ __forceinline__ void poisson_predict_r(const NddView& V, const PoissonArgs& A, int i, int j)
+// predicted r flux on face (i, j), j = 2..nj+1 (fluidModule.hpp:2466-2621)
+__device__ __forceinline__ double poisson_flux_r(const NddView& V, const PoissonArgs& A, int i, int j)
 {
 #pragma clang fp contract(off)
-    if (i > V.ni || j > V.nj + 1) return;
     const double dr = V.dr;
     const double* r = V.r;
     if (j <= V.nj) {
@@ -220,26 +218,14 @@ __device__ __forceinline__ void poisson_predict_r(const NddView& V, const Poisso
         const double tx = (r[j] * A.nUex_t[IX(i + 1, j)] * bRx + r[j] * A.nUex_t[IX(i, j)] * bLx +
                            r[j - 1] * A.nUex_t[IX(i + 1, j - 1)] * bRxD + r[j - 1] * A.nUex_t[IX(i, j - 1)] * bLxD) /
                           (4.0 * r_t);
-        A.nUer[IX(i, j)] = A.nUer_t[IX(i, j)] - (A.m.inv_cross * tx + A.m.inv_par * A.nUer_t[IX(i, j)]) * A.m.mu_star +
+        return A.nUer_t[IX(i, j)] - (A.m.inv_cross * tx + A.m.inv_par * A.nUer_t[IX(i, j)]) * A.m.mu_star +
                            A.m.mu_star * (n_f * dphidr - dpedr / K_E0);
     } else {
         const double uir = V.mu_i * A.Er[IX(i, V.nj + 1)] + K_EPS_VD;
-        A.nUer[IX(i, V.nj + 1)] = A.rhoi[IX(i, V.nj)] * uir;
+        return A.rhoi[IX(i, V.nj)] * uir;
     }
 }
 
-// both predictors in one launch: x faces on blockIdx.z == 0, r faces on 1;
-// they read only the copies of the previous fluxes, so they are independent
-__global__ void k_poisson_predict(NddView V, PoissonArgs A)
-{
-    if (blockIdx.z == 0) {
-        THREAD_IJ(1, 1);
-        poisson_predict_x(V, A, i, j);
-    } else {
-        THREAD_IJ(1, 2);
-        poisson_predict_r(V, A, i, j);
-    }
-}
 
 // 5-point Poisson correction system and its BC folds (fluidModule.hpp:2627-2824)
 __global__ void k_poisson_assemble(NddView V, PoissonArgs A, double* aP, double* aE, double* aW, double* aN,
@@ -268,7 +254,15 @@ __global__ void k_poisson_assemble(NddView V, PoissonArgs A, double* aP, double*
     double vW = eLx * dr * dr;
     double vN = qR * eRr * dx * dx;
     const double vS = qL * eLr * dx * dx;
-    const double div = (A.nUex[IX(i + 1, j)] - A.nUex[IX(i, j)]) / dx + (qR * A.nUer[IX(i, j + 1)] - qL * A.nUer[IX(i, j)]) / dr;
+    // predicted fluxes of the four faces, stored as in the SG assembly
+    const double fxL = poisson_flux_x(V, A, i, j), fxR = poisson_flux_x(V, A, i + 1, j);
+    const double frL = j >= 2 ? poisson_flux_r(V, A, i, j) : A.nUer[IX(i, 1)];
+    const double frR = poisson_flux_r(V, A, i, j + 1);
+    A.nUex[IX(i, j)] = fxL;
+    if (i == V.ni) A.nUex[IX(i + 1, j)] = fxR;
+    if (j >= 2) A.nUer[IX(i, j)] = frL;
+    if (j == V.nj) A.nUer[IX(i, j + 1)] = frR;
+    const double div = (fxR - fxL) / dx + (qR * frR - qL * frL) / dr;
     const double lap = -(A.Ex[IX(i + 1, j)] - A.Ex[IX(i, j)]) / dx - (qR * A.Er[IX(i, j + 1)] - qL * A.Er[IX(i, j)]) / dr;
     const double S = A.S ? A.S[IX(i, j)] : 0.0;
     const double vb = (K_E0 * A.rhoi[IX(i, j)] - K_E0 * A.rhoe[IX(i, j)] - K_E0 * dt * S + K_EPS0 * lap + K_E0 * dt * div) *
@@ -373,10 +367,13 @@ struct SgArgs {
     double *n, *Fx, *Fr;
 };
 
-__device__ __forceinline__ void sg_predict_x(const NddView& V, const SgArgs& A, int i, int j)
+// predicted SG flux on x face (i, j), i = 1..ni+1: the Scharfetter-Gummel
+// flux plus the mobility-tensor correction of the previous fluxes inside,
+// the wall fluxes on faces 1 and ni+1 (fluidModule.hpp:3239-3367, energy
+// :4161-4316)
+__device__ __forceinline__ double sg_flux_x(const NddView& V, const SgArgs& A, int i, int j)
 {
 #pragma clang fp contract(off)
-    if (i > V.ni + 1 || j > V.nj) return;
     const double dx = V.dx;
     const double* r = V.r;
     const double* T = A.T;
@@ -389,8 +386,7 @@ __device__ __forceinline__ void sg_predict_x(const NddView& V, const SgArgs& A, 
                           (4.0 * r[j]);
         double sg = bern_bp(F.vd, F.z, F.D / dx) * A.n[IX(i - 1, j)] - bern_bm(F.vd, F.z, F.D / dx) * A.n[IX(i, j)];
         if (A.sp == SP_ENERGY) sg = sg * 5.0 / 3.0;
-        A.Fx[IX(i, j)] = sg + A.Fx_t[IX(i, j)] - A.m.mu_star * (A.m.inv_par * A.Fx_t[IX(i, j)] + A.m.inv_cross * tr);
-        return;
+        return sg + A.Fx_t[IX(i, j)] - A.m.mu_star * (A.m.inv_par * A.Fx_t[IX(i, j)] + A.m.inv_cross * tr);
     }
     const int side = i == 1 ? 0 : 1;
     const int c = side ? V.ni : 1;  // wall cell
@@ -398,13 +394,13 @@ __device__ __forceinline__ void sg_predict_x(const NddView& V, const SgArgs& A, 
     const double sgn = side ? 1.0 : -1.0;
     if (A.sp == SP_ION) {
         const double U = V.mu_i * A.Ex[IX(fi, j)];
-        A.Fx[IX(fi, j)] = A.n[IX(c, j)] * ion_wall_gpn(V, U, sgn);
+        return A.n[IX(c, j)] * ion_wall_gpn(V, U, sgn);
     } else if (A.sp == SP_ELEC) {
         const double Te = A.Te[IX(c, j)];
         const double dphi = fmax(-sgn * A.Ex[IX(fi, j)] * dx / 2.0, 0.0);
         const double gpn = wall_gpn(V, Te, dphi, sgn);
         const double see = -V.coefIISEE_ts * A.rhoUix[IX(fi, j)] - eisee_sum(V, Te) * A.rhoe_old[IX(c, j)] * gpn;
-        A.Fx[IX(fi, j)] = A.n[IX(c, j)] * gpn + see;
+        return A.n[IX(c, j)] * gpn + see;
     } else {
         const double Teo = T[IX(c, j)];
         const double dphi = fmax(-sgn * A.Ex[IX(fi, j)] * dx / 2.0, 0.0);
@@ -423,14 +419,15 @@ __device__ __forceinline__ void sg_predict_x(const NddView& V, const SgArgs& A, 
         const double see = -eps_emit * V.coefIISEE_ts * A.rhoUix[IX(fi, j)] -
                            (eb + V.ratioEngy_EISEE_rd * rd) * A.n_old[IX(c, j)] * gpne -
                            eps_emit * ts * A.rhoe_old[IX(c, j)] * gpn;
-        A.Fx[IX(fi, j)] = A.n[IX(c, j)] * gpne + see;
+        return A.n[IX(c, j)] * gpne + see;
     }
 }
 
-__device__ __forceinline__ void sg_predict_r(const NddView& V, const SgArgs& A, int i, int j)
+// predicted SG flux on r face (i, j), j = 2..nj+1; face nj+1 is the open
+// upper boundary (fluidModule.hpp:3443-3542, energy :4403-4506)
+__device__ __forceinline__ double sg_flux_r(const NddView& V, const SgArgs& A, int i, int j)
 {
 #pragma clang fp contract(off)
-    if (i > V.ni || j > V.nj + 1) return;
     const double dr = V.dr;
     const double* r = V.r;
     const double* T = A.T;
@@ -444,27 +441,15 @@ __device__ __forceinline__ void sg_predict_r(const NddView& V, const SgArgs& A, 
                           (4.0 * r_t);
         double sg = bern_bp(F.vd, F.z, F.D / dr) * A.n[IX(i, j - 1)] - bern_bm(F.vd, F.z, F.D / dr) * A.n[IX(i, j)];
         if (A.sp == SP_ENERGY) sg = sg * 5.0 / 3.0;
-        A.Fr[IX(i, j)] = sg + A.Fr_t[IX(i, j)] - A.m.mu_star * (A.m.inv_cross * tx + A.m.inv_par * A.Fr_t[IX(i, j)]);
+        return sg + A.Fr_t[IX(i, j)] - A.m.mu_star * (A.m.inv_cross * tx + A.m.inv_par * A.Fr_t[IX(i, j)]);
     } else if (A.sp == SP_ENERGY) {
-        A.Fr[IX(i, V.nj + 1)] = 5.0 / 2.0 * K_BOLTZ * T[IX(i, V.nj)] * fmax(A.rhoUer[IX(i, V.nj + 1)], 0.0);
+        return 5.0 / 2.0 * K_BOLTZ * T[IX(i, V.nj)] * fmax(A.rhoUer[IX(i, V.nj + 1)], 0.0);
     } else {
         const double uir = V.mu_i * A.Er[IX(i, V.nj + 1)] + K_EPS_VD;
-        A.Fr[IX(i, V.nj + 1)] = A.rhoi[IX(i, V.nj)] * uir;
+        return A.rhoi[IX(i, V.nj)] * uir;
     }
 }
 
-// both predictors in one launch: x faces on blockIdx.z == 0, r faces on 1;
-// they read only the copies of the previous fluxes, so they are independent
-__global__ void k_sg_predict(NddView V, SgArgs A)
-{
-    if (blockIdx.z == 0) {
-        THREAD_IJ(1, 1);
-        sg_predict_x(V, A, i, j);
-    } else {
-        THREAD_IJ(1, 2);
-        sg_predict_r(V, A, i, j);
-    }
-}
 
 __global__ void k_sg_assemble(NddView V, SgArgs A, double* aP, double* aE, double* aW, double* aN, double* aS,
                               double* b)
@@ -493,7 +478,17 @@ __global__ void k_sg_assemble(NddView V, SgArgs A, double* aP, double* aE, doubl
     aW[k] = bern_bp(Lx.vd, Lx.z, Lx.D / dx) * bLx * dr;
     aN[k] = qR * bern_bm(Rr.vd, Rr.z, Rr.D / dr) * bRr * dx;
     aS[k] = qL * bern_bp(Lr.vd, Lr.z, Lr.D / dr) * bLr * dx;
-    const double div = (A.Fx[IX(i + 1, j)] - A.Fx[IX(i, j)]) / dx + (qR * A.Fr[IX(i, j + 1)] - qL * A.Fr[IX(i, j)]) / dr;
+    // the predicted fluxes of the cell's four faces; the cell stores its west
+    // and south faces (east and north too on the last row and column), the
+    // axis face j = 1 is not predicted
+    const double fxL = sg_flux_x(V, A, i, j), fxR = sg_flux_x(V, A, i + 1, j);
+    const double frL = j >= 2 ? sg_flux_r(V, A, i, j) : A.Fr[IX(i, 1)];
+    const double frR = sg_flux_r(V, A, i, j + 1);
+    A.Fx[IX(i, j)] = fxL;
+    if (i == V.ni) A.Fx[IX(i + 1, j)] = fxR;
+    if (j >= 2) A.Fr[IX(i, j)] = frL;
+    if (j == V.nj) A.Fr[IX(i, j + 1)] = frR;
+    const double div = (fxR - fxL) / dx + (qR * frR - qL * frL) / dr;
     if (A.sp == SP_ENERGY) {
         const double bLxw = (double)(i != 1), bRxw = (double)(i != V.ni);
         const double JEx = -K_E0 * (A.rhoUex[IX(i + 1, j)] * A.Ex[IX(i + 1, j)] * bRxw + A.rhoUex[IX(i, j)] * A.Ex[IX(i, j)] * bLxw) / 2.0;
@@ -586,8 +581,9 @@ __global__ void k_calcres(NddView V, ResPairs P, int mode, double eps, double* _
     }
 }
 
-// errors e[0..2] and their max e[3]
-__global__ void k_calcres_finish(const double* __restrict__ part, int nb, int mode, double* __restrict__ e)
+// errors e[0..2] and their max e[3] (also into side when set)
+__global__ void k_calcres_finish(const double* __restrict__ part, int nb, int mode, double* __restrict__ e,
+                                 double* __restrict__ side)
 {
     __shared__ double lds[8];
     const bool mx_a = mode == 0 || mode == 2, mx_c = mode == 2;
@@ -615,6 +611,7 @@ __global__ void k_calcres_finish(const double* __restrict__ part, int nb, int mo
         e[1] = err[1];
         e[2] = err[2];
         e[3] = fmax(fmax(err[0], err[1]), err[2]);
+        if (side) side[0] = e[3];
     }
 }
 
@@ -827,8 +824,6 @@ static double** field_slot(cathode_ndd_fields* f, int k)
 
 static inline dim3 gblk() { return dim3(NB_X, NB_Y); }
 static inline dim3 ggrid(int ni, int nj) { return dim3(cdiv(nj, NB_X), cdiv(ni, NB_Y)); }
-// predictor grid: x faces i = 1..ni+1, j = 1..nj (z = 0) and r faces i = 1..ni, j = 2..nj+1 (z = 1)
-static inline dim3 pgrid(const NddView& V) { return dim3(cdiv(V.nj, NB_X), cdiv(V.ni + 1, NB_Y), 2); }
 
 // up to 8 whole-plane copies in one launch (blockIdx.y picks the pair)
 struct CopyList {
@@ -900,8 +895,8 @@ static void calcres3_async(NddHandle& H, const ResPairs& P)
 {
     const NddView& V = H.V;
     hipLaunchKernelGGL(k_calcres, ggrid(V.ni, V.nj), gblk(), 0, H.s, V, P, H.p.flag_error, 1e-100, H.part.p, H.nb);
-    hipLaunchKernelGGL(k_calcres_finish, dim3(1), dim3(256), 0, H.s, H.part.p, H.nb, H.p.flag_error, H.scal.p);
-    CATHODE_HIP(hipMemcpyAsync(H.mg.side_slot(), H.scal.p + 3, sizeof(double), hipMemcpyDeviceToDevice, H.s));
+    hipLaunchKernelGGL(k_calcres_finish, dim3(1), dim3(256), 0, H.s, H.part.p, H.nb, H.p.flag_error, H.scal.p,
+                       H.mg.side_slot());
     CATHODE_LAUNCH_CHECK();
 }
 
@@ -910,7 +905,8 @@ static double calcres3(NddHandle& H, const ResPairs& P, double* err_out)
 {
     const NddView& V = H.V;
     hipLaunchKernelGGL(k_calcres, ggrid(V.ni, V.nj), gblk(), 0, H.s, V, P, H.p.flag_error, 1e-100, H.part.p, H.nb);
-    hipLaunchKernelGGL(k_calcres_finish, dim3(1), dim3(256), 0, H.s, H.part.p, H.nb, H.p.flag_error, H.scal.p);
+    hipLaunchKernelGGL(k_calcres_finish, dim3(1), dim3(256), 0, H.s, H.part.p, H.nb, H.p.flag_error, H.scal.p,
+                       (double*)nullptr);
     CATHODE_LAUNCH_CHECK();
     fetch_host(H.h_scal, H.scal.p, 4, H.s);
     CATHODE_HIP(hipStreamSynchronize(H.s));
@@ -1038,7 +1034,6 @@ static void poisson_routine(NddHandle& H, bool report)
     for (;;) {
         ncount++;
         d2d(H, {{H.tmp_c, F[F_PHI]}, {H.tmp_x, F[F_NUEX]}, {H.tmp_r, F[F_NUER]}});
-        hipLaunchKernelGGL(k_poisson_predict, pgrid(V), gblk(), 0, H.s, V, A);
         hipLaunchKernelGGL(k_poisson_assemble, ggrid(V.ni, V.nj), gblk(), 0, H.s, V, A, H.mg.aP(), H.mg.aE(),
                            H.mg.aW(), H.mg.aN(), H.mg.aS(), H.mg.rhs());
         CATHODE_LAUNCH_CHECK();
@@ -1128,7 +1123,6 @@ static void sg_routine(NddHandle& H, int sp, bool report, double* err3)
         } else {
             d2d(H, {{H.tmp_c, A.n}, {H.tmp_x, A.Fx}, {H.tmp_r, A.Fr}});
         }
-        hipLaunchKernelGGL(k_sg_predict, pgrid(V), gblk(), 0, H.s, V, A);
         hipLaunchKernelGGL(k_sg_assemble, ggrid(V.ni, V.nj), gblk(), 0, H.s, V, A, H.mg.aP(), H.mg.aE(), H.mg.aW(),
                            H.mg.aN(), H.mg.aS(), H.mg.rhs());
         CATHODE_LAUNCH_CHECK();
