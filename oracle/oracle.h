@@ -23,10 +23,15 @@
  *   orc_helm_*                            manufactured solution (second-order convergence);
  *                                         open-boundary reflection closed forms; ghost
  *                                         wall conditions; Pabs closed form 0.5 Re(sigma)|E|^2
+ *   orc_ndd_assemble (wall faces)         ion, electron and electron-energy wall fluxes are
+ *                                         the Maxwellian moments computed by quadrature
+ *                                         (drifting ions; electrons over the sheath barrier,
+ *                                         2 kT carried per particle); SEE off
+ *   orc_ndd_assemble (energy RHS)         J.E source = 3/5 (-e Gamma E) dx dr for uniform
+ *                                         fields and flux
  *   parity unpinned (no check beyond the restatement and the conservation identity):
- *                                         energy-equation wall heat flux and SEE yields, ion
- *                                         kinetic erf wall flux, J.E / Pabs / eloss source
- *                                         terms, Te cap
+ *                                         SEE yields (IISEE/EISEE; off in the configurations),
+ *                                         Pabs / eloss source terms, Te cap
  *   orc_abcd_*                            S -> ABCD -> S round trip is the identity
  * See tests/test_oracle_*.py for each check.
  */
