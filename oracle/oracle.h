@@ -28,10 +28,11 @@
  *                                         (drifting ions; electrons over the sheath barrier,
  *                                         2 kT carried per particle); SEE off
  *   orc_ndd_assemble (energy RHS)         J.E source = 3/5 (-e Gamma E) dx dr for uniform
- *                                         fields and flux
+ *                                         fields and flux; 3/5 (Pabs - n_e eloss) dx dr
+ *                                         without them
  *   parity unpinned (no check beyond the restatement and the conservation identity):
  *                                         SEE yields (IISEE/EISEE; off in the configurations),
- *                                         Pabs / eloss source terms, Te cap
+ *                                         Te cap
  *   orc_abcd_*                            S -> ABCD -> S round trip is the identity
  * See tests/test_oracle_*.py for each check.
  */

@@ -111,3 +111,22 @@ def test_energy_source_is_minus_e_flux_dot_field():
     expect = 0.6 * (-E0 * G * Ez) * p.dx * p.dr
     inner = b[3:ni - 1, 1:nj + 1]
     np.testing.assert_allclose(inner, expect, rtol=1e-6)
+
+
+def test_energy_source_absorbed_power_minus_loss():
+    """Uniform state, no field and no flux: the interior energy right-hand side
+    is 3/5 (Pabs - n_e eloss) dx dr (fluidModule.hpp:4619-4637)."""
+    ni, nj, h = 10, 6, 1e-4
+    p = W.default_params(ni, nj, h, h, energy_on=True)
+    n = np.full((ni + 2, nj + 2), 2e16)
+    Te = np.full((ni + 2, nj + 2), 4.0 * E0 / KB)
+    st = W.consistent_state(p, n, n.copy(), Te)
+    st["Ex"][...] = 0.0
+    st["Er"][...] = 0.0
+    rng = np.random.default_rng(9)
+    st["Pabs"] = 1e8 * (1.0 + rng.uniform(size=n.shape))
+    st["rate_eloss"] = 2e-9 * (1.0 + rng.uniform(size=n.shape))
+    b = O.ndd_assemble(p, st, 3)[5]
+    sl = (slice(3, ni - 1), slice(1, nj + 1))
+    expect = 0.6 * (st["Pabs"][sl] - n[sl] * st["rate_eloss"][sl]) * p.dx * p.dr
+    np.testing.assert_allclose(b[sl], expect, rtol=1e-7)
