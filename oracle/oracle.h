@@ -30,9 +30,11 @@
  *   orc_ndd_assemble (energy RHS)         J.E source = 3/5 (-e Gamma E) dx dr for uniform
  *                                         fields and flux; 3/5 (Pabs - n_e eloss) dx dr
  *                                         without them
+ *   orc_ndd_step (Te update)              Te0 / 1.001 (the 1e-3 max n_e floor) and the
+ *                                         1000 eV cap after a step that barely moves a
+ *                                         uniform state
  *   parity unpinned (no check beyond the restatement and the conservation identity):
- *                                         SEE yields (IISEE/EISEE; off in the configurations),
- *                                         Te cap
+ *                                         SEE yields (IISEE/EISEE; off in the configurations)
  *   orc_abcd_*                            S -> ABCD -> S round trip is the identity
  * See tests/test_oracle_*.py for each check.
  */

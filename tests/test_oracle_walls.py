@@ -130,3 +130,21 @@ def test_energy_source_absorbed_power_minus_loss():
     sl = (slice(3, ni - 1), slice(1, nj + 1))
     expect = 0.6 * (st["Pabs"][sl] - n[sl] * st["rate_eloss"][sl]) * p.dx * p.dr
     np.testing.assert_allclose(b[sl], expect, rtol=1e-7)
+
+
+@pytest.mark.parametrize("Te_eV", [300.0, 2000.0])
+def test_te_update_floor_and_cap(Te_eV):
+    """After a step that barely moves a uniform state (dt = 1e-14 s, no bias):
+    Te = n_eps / (3/2 k (n_e + 1e-3 max n_e)) = Te0 / 1.001, capped at 1000 eV
+    (fluidModule.hpp:4897-4905)."""
+    ni, nj, h = 12, 6, 1e-4
+    p = W.default_params(ni, nj, h, h, dt=1e-14, V_bias=0.0, energy_on=True)
+    n = np.full((ni + 2, nj + 2), 1e16)
+    Te = np.full((ni + 2, nj + 2), Te_eV * E0 / KB)
+    st = W.consistent_state(p, n, n.copy(), Te)
+    O.ndd_step(p, st, 1, solver=0)
+    sl = (slice(4, ni - 2), slice(1, nj + 1))
+    if Te_eV < 1000.0:
+        np.testing.assert_allclose(st["Te"][sl] / Te[sl], 1.0 / 1.001, rtol=1e-9)
+    else:
+        assert np.all(st["Te"][1:-1, 1:-1] == 1000.0 * E0 / KB)
