@@ -1213,9 +1213,12 @@ void MGSolver::setup(int mode, hipStream_t s, const WarmSet& ws)
         act_dirty = false;
     }
     double* sbuf = F.b;  // row excess of level 0 (F.b is rewritten by the outer loop)
-    hipLaunchKernelGGL(k_l0_prepare, grid2(F.ni, F.nj), dim3(BX, BY), 0, s, F.ni, F.nj, F.ld, act0.p, F.aP, F.aE,
-                       F.aW, F.aN, F.aS, bsol.p, sbuf, part.p, nb);
-    CATHODE_LAUNCH_CHECK();
+    if (!prepared) {
+        hipLaunchKernelGGL(k_l0_prepare, grid2(F.ni, F.nj), dim3(BX, BY), 0, s, F.ni, F.nj, F.ld, act0.p, F.aP, F.aE,
+                           F.aW, F.aN, F.aS, bsol.p, sbuf, part.p, nb);
+        CATHODE_LAUNCH_CHECK();
+    }
+    prepared = false;
     const double* w = user_w;
     if (!w) {
         hipLaunchKernelGGL(k_l0_weights, dim3(1), dim3(256), (F.nj + 2) * sizeof(double), s, F.ni, F.nj, F.ld, F.aN,

@@ -154,6 +154,13 @@ public:
         return old;
     }
     bool act_dirty = true;
+    // The caller's kernel already did the level-0 preparation for the next
+    // solve (couplings to inactive cells zeroed, row excess into prep_s(),
+    // max|aP| and |b|^2 block partials into prep_part() in the 64 x 4 block
+    // layout); valid for that one solve, the flag clears itself.
+    bool prepared = false;
+    double* prep_s() { return L64[0].b; }
+    double* prep_part() { return part.p; }
     WarmSet warm;
     DevBuf<double> wpart;  // warm-start partials, warm_nb per dot product
     int warm_nb = 0;

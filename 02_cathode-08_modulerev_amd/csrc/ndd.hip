@@ -227,13 +227,46 @@ __device__ __forceinline__ double poisson_flux_r(const NddView& V, const Poisson
 }
 
 
+// The multigrid's level-0 preparation, done by the assembly kernels for the
+// NDD mask (every interior cell active, the ghost ring inactive): couplings
+// to ghost cells zeroed (HYPRE's off-grid convention), the row excess
+// s = max(aP - sum a, 0), and the block partials of max|aP| (slot 0) and
+// |b|^2 (slot 1) in the multigrid's 2-D block layout (this grid's).
+__device__ inline void prep_cell(const NddView& V, int i, int j, double* aP, double* aE, double* aW, double* aN,
+                                 double* aS, const double* b, double* s_out, double& amax, double& bb)
+{
+    const size_t k = IX0(i, j);
+    const double e = i < V.ni ? aE[k] : 0.0;
+    const double w = i > 1 ? aW[k] : 0.0;
+    const double n = j < V.nj ? aN[k] : 0.0;
+    const double s = j > 1 ? aS[k] : 0.0;
+    aE[k] = e;
+    aW[k] = w;
+    aN[k] = n;
+    aS[k] = s;
+    const double ex = aP[k] - (e + w + n + s);
+    s_out[k] = ex > 0.0 ? ex : 0.0;
+    amax = fabs(aP[k]);
+    bb = b[k] * b[k];
+}
+
+__device__ inline void prep_partials(double amax, double bb, double* part)
+{
+    __shared__ double lds[8];
+    const double t1 = block_max<NB_X * NB_Y>(amax, lds);
+    const double t2 = block_sum<NB_X * NB_Y>(bb, lds);
+    if (lin_tid() == 0) {
+        const int nb = (int)(gridDim.x * gridDim.y), bid = (int)(blockIdx.y * gridDim.x + blockIdx.x);
+        part[bid] = t1;
+        part[nb + bid] = t2;
+    }
+}
+
 // 5-point Poisson correction system and its BC folds (fluidModule.hpp:2627-2824)
-__global__ void k_poisson_assemble(NddView V, PoissonArgs A, double* aP, double* aE, double* aW, double* aN,
-                                   double* aS, double* b)
+__device__ __forceinline__ void poisson_assemble_cell(const NddView& V, const PoissonArgs& A, int i, int j, double* aP,
+                                                      double* aE, double* aW, double* aN, double* aS, double* b)
 {
 #pragma clang fp contract(off)
-    THREAD_IJ(1, 1);
-    if (i > V.ni || j > V.nj) return;
     const double dx = V.dx, dr = V.dr, dt = V.dt;
     const double* r = V.r;
     const double qR = (r[j] + r[j + 1]) / 2.0 / r[j];
@@ -286,6 +319,19 @@ __global__ void k_poisson_assemble(NddView V, PoissonArgs A, double* aP, double*
     aN[k] = vN;
     aS[k] = vS;
     b[k] = vb;
+}
+
+// with s_out set, also the multigrid's level-0 preparation (prep_cell)
+__global__ void k_poisson_assemble(NddView V, PoissonArgs A, double* aP, double* aE, double* aW, double* aN,
+                                   double* aS, double* b, double* s_out, double* part)
+{
+    THREAD_IJ(1, 1);
+    double amax = 0.0, bb = 0.0;
+    if (i <= V.ni && j <= V.nj) {
+        poisson_assemble_cell(V, A, i, j, aP, aE, aW, aN, aS, b);
+        if (s_out) prep_cell(V, i, j, aP, aE, aW, aN, aS, b, s_out, amax, bb);
+    }
+    if (s_out) prep_partials(amax, bb, part);
 }
 
 // phi ghosts before the PC loop (fluidModule.hpp:2160-2253): thread per edge cell
@@ -451,12 +497,10 @@ __device__ __forceinline__ double sg_flux_r(const NddView& V, const SgArgs& A, i
 }
 
 
-__global__ void k_sg_assemble(NddView V, SgArgs A, double* aP, double* aE, double* aW, double* aN, double* aS,
-                              double* b)
+__device__ __forceinline__ void sg_assemble_cell(const NddView& V, const SgArgs& A, int i, int j, double* aP,
+                                                 double* aE, double* aW, double* aN, double* aS, double* b)
 {
 #pragma clang fp contract(off)
-    THREAD_IJ(1, 1);
-    if (i > V.ni || j > V.nj) return;
     const double dx = V.dx, dr = V.dr, dt = V.dt;
     const double* r = V.r;
     const double* T = A.T;
@@ -505,6 +549,19 @@ __global__ void k_sg_assemble(NddView V, SgArgs A, double* aP, double* aE, doubl
         const double S = A.S ? A.S[IX(i, j)] : 0.0;
         b[k] = (-div + S - (A.n[IX(i, j)] - A.n_old[IX(i, j)]) / dt) * dx * dr;
     }
+}
+
+// with s_out set, also the multigrid's level-0 preparation (prep_cell)
+__global__ void k_sg_assemble(NddView V, SgArgs A, double* aP, double* aE, double* aW, double* aN, double* aS,
+                              double* b, double* s_out, double* part)
+{
+    THREAD_IJ(1, 1);
+    double amax = 0.0, bb = 0.0;
+    if (i <= V.ni && j <= V.nj) {
+        sg_assemble_cell(V, A, i, j, aP, aE, aW, aN, aS, b);
+        if (s_out) prep_cell(V, i, j, aP, aE, aW, aN, aS, b, s_out, amax, bb);
+    }
+    if (s_out) prep_partials(amax, bb, part);
 }
 
 // flux correction on interior faces and the positivity-guarded density
@@ -1035,7 +1092,8 @@ static void poisson_routine(NddHandle& H, bool report)
         ncount++;
         d2d(H, {{H.tmp_c, F[F_PHI]}, {H.tmp_x, F[F_NUEX]}, {H.tmp_r, F[F_NUER]}});
         hipLaunchKernelGGL(k_poisson_assemble, ggrid(V.ni, V.nj), gblk(), 0, H.s, V, A, H.mg.aP(), H.mg.aE(),
-                           H.mg.aW(), H.mg.aN(), H.mg.aS(), H.mg.rhs());
+                           H.mg.aW(), H.mg.aN(), H.mg.aS(), H.mg.rhs(), H.mg.prep_s(), H.mg.prep_part());
+        H.mg.prepared = true;
         CATHODE_LAUNCH_CHECK();
         bool use_net = false;
         if (ncount == 1 && H.p.learned && H.cnn) {
@@ -1124,7 +1182,8 @@ static void sg_routine(NddHandle& H, int sp, bool report, double* err3)
             d2d(H, {{H.tmp_c, A.n}, {H.tmp_x, A.Fx}, {H.tmp_r, A.Fr}});
         }
         hipLaunchKernelGGL(k_sg_assemble, ggrid(V.ni, V.nj), gblk(), 0, H.s, V, A, H.mg.aP(), H.mg.aE(), H.mg.aW(),
-                           H.mg.aN(), H.mg.aS(), H.mg.rhs());
+                           H.mg.aN(), H.mg.aS(), H.mg.rhs(), H.mg.prep_s(), H.mg.prep_part());
+        H.mg.prepared = true;
         CATHODE_LAUNCH_CHECK();
         const int eq = sp == SP_ELEC ? EQ_ELEC : (sp == SP_ION ? EQ_ION : EQ_ENERGY);
         warm_guess(H, warm_slot(eq, ncount));
