@@ -110,10 +110,10 @@ struct LevelStage {
 // row-major order, fetch(i, j, here, ci0, cj0) giving each child's values
 // (an absent child, here = false, reads child (ci0, cj0) and drops out, so
 // every load can be issued before any is tested); writes the coarse level
-// and, with out set, the values a further coarsening reads into out
+// and, with stage_out, the values a further coarsening reads into out
 template <class Fetch>
 __device__ inline void coarsen_cell_f(const CoarsenArgs& a, int I, int J, double s32, int mixed, Fetch fetch,
-                                      const LevelStage* out)
+                                      const LevelStage& out, bool stage_out)
 {
     const int i0 = a.fi * (I - 1) + 1, i1 = min(a.fi * I, a.fni);
     const int j0 = a.fj * (J - 1) + 1, j1 = min(a.fj * J, a.fnj);
@@ -172,37 +172,41 @@ __device__ inline void coarsen_cell_f(const CoarsenArgs& a, int I, int J, double
         a.gN[kc] = (float)(N * s32);
         a.gS[kc] = (float)(So * s32);
     }
-    if (out) {
-        const int q = (I - out->i0) * out->cols + (J - out->j0);
-        out->act[q] = (unsigned char)any;
-        out->P[q] = P;
-        out->E[q] = E;
-        out->W[q] = W;
-        out->N[q] = N;
-        out->S[q] = So;
-        out->M[q] = M;
+    if (stage_out) {
+        const int q = (I - out.i0) * out.cols + (J - out.j0);
+        out.act[q] = (unsigned char)any;
+        out.P[q] = P;
+        out.E[q] = E;
+        out.W[q] = W;
+        out.N[q] = N;
+        out.S[q] = So;
+        out.M[q] = M;
     }
 }
+
+// a child read from the fine level in global memory
+struct GlobalChild {
+    const CoarsenArgs& a;
+    __device__ ChildVals operator()(int i, int j, bool here, int ci0, int cj0) const
+    {
+        const size_t k = here ? (size_t)i * a.fld + j : (size_t)ci0 * a.fld + cj0;
+        ChildVals c;
+        c.on = here && a.fact[k] != 0;
+        c.wj = a.w ? a.w[here ? j : cj0] : 1.0;
+        c.p = a.faP[k];
+        c.e = a.faE[k];
+        c.w = a.faW[k];
+        c.n = a.faN[k];
+        c.s = a.faS[k];
+        c.m = a.fm[k];
+        return c;
+    }
+};
 
 // the same from the fine level in global memory
 __device__ inline void coarsen_cell(const CoarsenArgs& a, int I, int J, double s32, int mixed)
 {
-    coarsen_cell_f(
-        a, I, J, s32, mixed,
-        [&](int i, int j, bool here, int ci0, int cj0) {
-            const size_t k = here ? (size_t)i * a.fld + j : (size_t)ci0 * a.fld + cj0;
-            ChildVals c;
-            c.on = here && a.fact[k] != 0;
-            c.wj = a.w ? a.w[here ? j : cj0] : 1.0;
-            c.p = a.faP[k];
-            c.e = a.faE[k];
-            c.w = a.faW[k];
-            c.n = a.faN[k];
-            c.s = a.faS[k];
-            c.m = a.fm[k];
-            return c;
-        },
-        (const LevelStage*)nullptr);
+    coarsen_cell_f(a, I, J, s32, mixed, GlobalChild{a}, LevelStage{}, false);
 }
 
 __global__ void k_coarsen(CoarsenArgs a, const double* __restrict__ sc, int mixed)
@@ -329,6 +333,8 @@ __device__ __forceinline__ void coarse_inverse_block(const CoarseInv& c, double 
     }
 }
 
+// the chain's levels one after another, then (when ci.inv64 is set) the
+// coarsest inverse; NT threads
 // the chain's levels one after another, then (when ci.inv64 is set) the
 // coarsest inverse; NT threads
 template <int NT>
@@ -538,7 +544,7 @@ __global__ void __launch_bounds__(SETUP_NT) k_setup_levels(SetupLevels a, double
                     v.m = src.M[lq];
                     return v;
                 },
-                &dst);
+                dst, true);
         }
     }
 }
