@@ -349,27 +349,38 @@ __global__ void k_poisson_ghost(NddView V, double* phi)
     }
 }
 
-// dphi with its ghost reflections, phi += dphi on every cell, wall-face
-// field update (fluidModule.hpp:2850-2958)
-__global__ void k_poisson_update_phi(NddView V, const double* __restrict__ sol, double* __restrict__ dphi,
-                                     double* __restrict__ phi, double* __restrict__ Ex, double* __restrict__ Er)
+// the correction at (i, j) of the extended grid: the solution inside, its
+// odd mirror on the z-wall ghosts (Dirichlet) and its copy on the open top
+__device__ inline double dphi_at(const NddView& V, const double* __restrict__ sol, int i, int j)
+{
+    const int ni = V.ni, nj = V.nj;
+    const bool in_i = i >= 1 && i <= ni, in_j = j >= 1 && j <= nj;
+    if (in_i && in_j) return sol[IX0(i, j)];
+    if (i == 0 && in_j) return -sol[IX0(1, j)];
+    if (i == ni + 1 && in_j) return -sol[IX0(ni, j)];
+    if (j == nj + 1 && in_i) return sol[IX0(i, nj)];
+    return 0.0;
+}
+
+// phi += dphi on the extended grid with the ghost reflections of dphi, the
+// wall-face field update (fluidModule.hpp:2850-2958), then the interior face
+// fields E = -grad phi and the flux correction mu* n_f grad dphi
+// (fluidModule.hpp:2959-3033).  phi0 is phi before the update (the PC pass's
+// copy), so every thread can form the updated phi of its neighbours.
+__global__ void k_poisson_update(NddView V, const double* __restrict__ sol, const double* __restrict__ phi0,
+                                 double* __restrict__ dphi, double* __restrict__ phi, double* __restrict__ Ex,
+                                 double* __restrict__ Er, const double* __restrict__ rhoe, double* __restrict__ nUex,
+                                 double* __restrict__ nUer, Mob m)
 {
 #pragma clang fp contract(off)
     THREAD_IJ(0, 0);
     if (i > V.ni + 1 || j > V.nj + 1) return;
     const int ni = V.ni, nj = V.nj;
-    double d = 0.0;
     const bool in_i = i >= 1 && i <= ni, in_j = j >= 1 && j <= nj;
-    if (in_i && in_j)
-        d = sol[IX0(i, j)];
-    else if (i == 0 && in_j)
-        d = -sol[IX0(1, j)];
-    else if (i == ni + 1 && in_j)
-        d = -sol[IX0(ni, j)];
-    else if (j == nj + 1 && in_i)
-        d = sol[IX0(i, nj)];
+    const double d = dphi_at(V, sol, i, j);
+    const double p = phi0[IX(i, j)] + d;
     dphi[IX(i, j)] = d;
-    phi[IX(i, j)] = phi[IX(i, j)] + d;
+    phi[IX(i, j)] = p;
     if (in_j && i == 1) {
         const double d1 = sol[IX0(1, j)];
         Ex[IX(1, j)] = Ex[IX(1, j)] - (d1 - (-d1)) / V.dx;
@@ -379,25 +390,20 @@ __global__ void k_poisson_update_phi(NddView V, const double* __restrict__ sol, 
         Ex[IX(ni + 1, j)] = Ex[IX(ni + 1, j)] - ((-dn) - dn) / V.dx;
     }
     if (in_i && j == nj) Er[IX(i, nj + 1)] = 0.0;
-}
-
-// interior fields from phi and the flux correction (fluidModule.hpp:2959-3033)
-__global__ void k_poisson_fields(NddView V, const double* __restrict__ phi, const double* __restrict__ dphi,
-                                 const double* __restrict__ rhoe, double* __restrict__ Ex, double* __restrict__ Er,
-                                 double* __restrict__ nUex, double* __restrict__ nUer, Mob m)
-{
-#pragma clang fp contract(off)
-    THREAD_IJ(1, 1);
-    if (i > V.ni || j > V.nj) return;
+    if (!(in_i && in_j)) return;
     if (i >= 2) {
-        Ex[IX(i, j)] = -(phi[IX(i, j)] - phi[IX(i - 1, j)]) / V.dx;
+        const double dl = dphi_at(V, sol, i - 1, j);
+        const double pl = phi0[IX(i - 1, j)] + dl;
+        Ex[IX(i, j)] = -(p - pl) / V.dx;
         const double n_f = (rhoe[IX(i, j)] + rhoe[IX(i - 1, j)]) / 2.0;
-        nUex[IX(i, j)] = nUex[IX(i, j)] + m.mu_star * n_f * ((dphi[IX(i, j)] - dphi[IX(i - 1, j)]) / V.dx);
+        nUex[IX(i, j)] = nUex[IX(i, j)] + m.mu_star * n_f * ((d - dl) / V.dx);
     }
     if (j >= 2) {
-        Er[IX(i, j)] = -(phi[IX(i, j)] - phi[IX(i, j - 1)]) / V.dr;
+        const double ds = dphi_at(V, sol, i, j - 1);
+        const double ps = phi0[IX(i, j - 1)] + ds;
+        Er[IX(i, j)] = -(p - ps) / V.dr;
         const double n_f = (rhoe[IX(i, j)] + rhoe[IX(i, j - 1)]) / 2.0;
-        nUer[IX(i, j)] = nUer[IX(i, j)] + m.mu_star * n_f * ((dphi[IX(i, j)] - dphi[IX(i, j - 1)]) / V.dr);
+        nUer[IX(i, j)] = nUer[IX(i, j)] + m.mu_star * n_f * ((d - ds) / V.dr);
     }
 }
 
@@ -687,39 +693,26 @@ __global__ void k_energy_guard_max(NddView V, double* __restrict__ rhoeps, const
     if (lin_tid() == 0) part[blockIdx.y * gridDim.x + blockIdx.x] = m;
 }
 
-__global__ void k_max_finish(const double* __restrict__ part, int nb, double* __restrict__ out)
-{
-    __shared__ double lds[8];
-    double m = 0.0;
-    for (int k = threadIdx.x; k < nb; k += blockDim.x) m = fmax(m, part[k]);
-    m = block_max<256>(m, lds);
-    if (threadIdx.x == 0) *out = m;
-}
-
 __global__ void k_te_update(NddView V, const double* __restrict__ rhoeps, const double* __restrict__ rhoe,
-                            const double* __restrict__ nmax, double* __restrict__ Te)
+                            const double* __restrict__ part, int nb, double* __restrict__ Te)
 {
 #pragma clang fp contract(off)
+    __shared__ double lds[NB_X * NB_Y / 64];
     THREAD_IJ(1, 1);
+    // max n_e from the guard kernel's block partials (every workgroup gets the
+    // same value: a maximum does not depend on the order)
+    double mx = 0.0;
+    for (int k = lin_tid(); k < nb; k += NB_X * NB_Y) mx = fmax(mx, part[k]);
+    const double nmax = block_max_bcast<NB_X * NB_Y>(mx, lds);
     if (i > V.ni || j > V.nj) return;
     const double Te_max = 1000.0 * K_E0 / K_BOLTZ, ratio = 0.001;
-    Te[IX(i, j)] = fmin(rhoeps[IX(i, j)] / (3.0 / 2.0 * (rhoe[IX(i, j)] + *nmax * ratio) * K_BOLTZ + 1e-100), Te_max);
-}
-
-// zero-gradient ghosts on the four sides (corners untouched)
-__global__ void k_ghost_copy4(NddView V, double* __restrict__ a)
-{
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < V.nj) {
-        const int j = t + 1;
-        a[IX(0, j)] = a[IX(1, j)];
-        a[IX(V.ni + 1, j)] = a[IX(V.ni, j)];
-    }
-    if (t < V.ni) {
-        const int i = t + 1;
-        a[IX(i, 0)] = a[IX(i, 1)];
-        a[IX(i, V.nj + 1)] = a[IX(i, V.nj)];
-    }
+    const double t = fmin(rhoeps[IX(i, j)] / (3.0 / 2.0 * (rhoe[IX(i, j)] + nmax * ratio) * K_BOLTZ + 1e-100), Te_max);
+    Te[IX(i, j)] = t;
+    // the ghost ring copies its neighbour (no corners)
+    if (i == 1) Te[IX(0, j)] = t;
+    if (i == V.ni) Te[IX(V.ni + 1, j)] = t;
+    if (j == 1) Te[IX(i, 0)] = t;
+    if (j == V.nj) Te[IX(i, V.nj + 1)] = t;
 }
 
 __global__ void k_top_copy(NddView V, double* __restrict__ a)
@@ -1117,11 +1110,11 @@ static void poisson_routine(NddHandle& H, bool report)
             break;
         }
         if (learn && (H.cap_feat || !H.p.learned)) learned_store(H);  // learned mode reads its warm slot
-        hipLaunchKernelGGL(k_poisson_update_phi, ggrid(V.ni + 2, V.nj + 2), gblk(), 0, H.s, V, H.mg.sol(), H.dphi,
-                           F[F_PHI], F[F_EX], F[F_ER]);
+        // the pass's copy tmp_c is phi before the update
+        hipLaunchKernelGGL(k_poisson_update, ggrid(V.ni + 2, V.nj + 2), gblk(), 0, H.s, V, H.mg.sol(),
+                           (const double*)H.tmp_c, H.dphi, F[F_PHI], F[F_EX], F[F_ER], (const double*)F[F_RHOE],
+                           F[F_NUEX], F[F_NUER], H.me);
         warm_store(H, warm_slot(EQ_POISSON, ncount));
-        hipLaunchKernelGGL(k_poisson_fields, ggrid(V.ni, V.nj), gblk(), 0, H.s, V, F[F_PHI], H.dphi, F[F_RHOE],
-                           F[F_EX], F[F_ER], F[F_NUEX], F[F_NUER], H.me);
         CATHODE_LAUNCH_CHECK();
         if (ncount >= H.p.maxITR_HES_phi) break;
         ResPairs P{F[F_PHI], H.tmp_c, F[F_NUEX], H.tmp_x, F[F_NUER], H.tmp_r};
@@ -1233,10 +1226,8 @@ static void ndd_one_step(NddHandle& H, bool report)
         }
         hipLaunchKernelGGL(k_energy_guard_max, ggrid(V.ni, V.nj), gblk(), 0, H.s, V, F[F_RHOEPS], H.s_old, F[F_RHOE],
                            H.part.p);
-        hipLaunchKernelGGL(k_max_finish, dim3(1), dim3(256), 0, H.s, H.part.p, H.nb, H.scal.p + 8);
         hipLaunchKernelGGL(k_te_update, ggrid(V.ni, V.nj), gblk(), 0, H.s, V, F[F_RHOEPS], F[F_RHOE],
-                           (const double*)(H.scal.p + 8), F[F_TE]);
-        hipLaunchKernelGGL(k_ghost_copy4, dim3(cdiv(std::max(V.ni, V.nj), 256)), dim3(256), 0, H.s, V, F[F_TE]);
+                           (const double*)H.part.p, H.nb, F[F_TE]);
         CATHODE_LAUNCH_CHECK();
     }
     H.rep.steps++;
