@@ -192,6 +192,30 @@ __device__ inline double block_max(double v, double* lds)
     return s;
 }
 
+// M reductions at once, each a sum or (is_max[m]) a maximum, combined in the
+// same order as block_sum / block_max, with one barrier; results valid in the
+// thread with lin_tid() == 0.  lds holds M * NT / 64 doubles and is not
+// released by a trailing barrier (one use per kernel).
+template <int NT, int M>
+__device__ inline void block_reduce_vec(double (&v)[M], const bool (&is_max)[M], double* lds)
+{
+#pragma unroll
+    for (int m = 0; m < M; m++) v[m] = is_max[m] ? wave_max(v[m]) : wave_sum(v[m]);
+    const int t = lin_tid();
+    if ((t & 63) == 0)
+#pragma unroll
+        for (int m = 0; m < M; m++) lds[(t >> 6) * M + m] = v[m];
+    __syncthreads();
+    if (t == 0) {
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            double s = is_max[m] ? -INFINITY : 0.0;
+            for (int w = 0; w < NT / 64; w++) s = is_max[m] ? fmax(s, lds[w * M + m]) : s + lds[w * M + m];
+            v[m] = s;
+        }
+    }
+}
+
 // Sums `nvec` interleaved partial vectors part[b*nvec + k] over blocks b in
 // fixed order; out[k] = total.  One block of 256 threads.
 __global__ void k_reduce_partials(const double* part, int nblocks, int nvec, const int* is_max,

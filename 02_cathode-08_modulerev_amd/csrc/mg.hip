@@ -51,8 +51,10 @@ __global__ void k_l0_prepare(int ni, int nj, int ld, const unsigned char* __rest
             bb = bsol[k] * bsol[k];
         }
     }
-    const double t1 = block_max<BX * BY>(amax, lds);
-    const double t2 = block_sum<BX * BY>(bb, lds);
+    double v[2] = {amax, bb};
+    const bool mx[2] = {true, false};
+    block_reduce_vec<BX * BY, 2>(v, mx, lds);
+    const double t1 = v[0], t2 = v[1];
     if (lin_tid() == 0) {
         const int bid = blockIdx.y * gridDim.x + blockIdx.x;
         part[bid] = t1;

@@ -232,29 +232,42 @@ __device__ __forceinline__ double poisson_flux_r(const NddView& V, const Poisson
 // to ghost cells zeroed (HYPRE's off-grid convention), the row excess
 // s = max(aP - sum a, 0), and the block partials of max|aP| (slot 0) and
 // |b|^2 (slot 1) in the multigrid's 2-D block layout (this grid's).
-__device__ inline void prep_cell(const NddView& V, int i, int j, double* aP, double* aE, double* aW, double* aN,
-                                 double* aS, const double* b, double* s_out, double& amax, double& bb)
+// one assembled row: the five coefficients and the right-hand side
+struct Row5 {
+    double P, E, W, N, S, b;
+};
+
+__device__ inline void prep_row(const NddView& V, int i, int j, Row5& R, double* s_out, double& amax, double& bb)
+{
+    R.E = i < V.ni ? R.E : 0.0;
+    R.W = i > 1 ? R.W : 0.0;
+    R.N = j < V.nj ? R.N : 0.0;
+    R.S = j > 1 ? R.S : 0.0;
+    const double ex = R.P - (R.E + R.W + R.N + R.S);
+    s_out[IX0(i, j)] = ex > 0.0 ? ex : 0.0;
+    amax = fabs(R.P);
+    bb = R.b * R.b;
+}
+
+__device__ inline void store_row(const NddView& V, int i, int j, const Row5& R, double* aP, double* aE, double* aW,
+                                 double* aN, double* aS, double* b)
 {
     const size_t k = IX0(i, j);
-    const double e = i < V.ni ? aE[k] : 0.0;
-    const double w = i > 1 ? aW[k] : 0.0;
-    const double n = j < V.nj ? aN[k] : 0.0;
-    const double s = j > 1 ? aS[k] : 0.0;
-    aE[k] = e;
-    aW[k] = w;
-    aN[k] = n;
-    aS[k] = s;
-    const double ex = aP[k] - (e + w + n + s);
-    s_out[k] = ex > 0.0 ? ex : 0.0;
-    amax = fabs(aP[k]);
-    bb = b[k] * b[k];
+    aP[k] = R.P;
+    aE[k] = R.E;
+    aW[k] = R.W;
+    aN[k] = R.N;
+    aS[k] = R.S;
+    b[k] = R.b;
 }
 
 __device__ inline void prep_partials(double amax, double bb, double* part)
 {
     __shared__ double lds[8];
-    const double t1 = block_max<NB_X * NB_Y>(amax, lds);
-    const double t2 = block_sum<NB_X * NB_Y>(bb, lds);
+    double v[2] = {amax, bb};
+    const bool mx[2] = {true, false};
+    block_reduce_vec<NB_X * NB_Y, 2>(v, mx, lds);
+    const double t1 = v[0], t2 = v[1];
     if (lin_tid() == 0) {
         const int nb = (int)(gridDim.x * gridDim.y), bid = (int)(blockIdx.y * gridDim.x + blockIdx.x);
         part[bid] = t1;
@@ -263,8 +276,9 @@ __device__ inline void prep_partials(double amax, double bb, double* part)
 }
 
 // 5-point Poisson correction system and its BC folds (fluidModule.hpp:2627-2824)
-__device__ __forceinline__ void poisson_assemble_cell(const NddView& V, const PoissonArgs& A, int i, int j, double* aP,
-                                                      double* aE, double* aW, double* aN, double* aS, double* b)
+// (the predicted face fluxes are stored after every load of the cell, so the
+// loads need not wait for the stores)
+__device__ __forceinline__ Row5 poisson_assemble_cell(const NddView& V, const PoissonArgs& A, int i, int j)
 {
 #pragma clang fp contract(off)
     const double dx = V.dx, dr = V.dr, dt = V.dt;
@@ -291,10 +305,6 @@ __device__ __forceinline__ void poisson_assemble_cell(const NddView& V, const Po
     const double fxL = poisson_flux_x(V, A, i, j), fxR = poisson_flux_x(V, A, i + 1, j);
     const double frL = j >= 2 ? poisson_flux_r(V, A, i, j) : A.nUer[IX(i, 1)];
     const double frR = poisson_flux_r(V, A, i, j + 1);
-    A.nUex[IX(i, j)] = fxL;
-    if (i == V.ni) A.nUex[IX(i + 1, j)] = fxR;
-    if (j >= 2) A.nUer[IX(i, j)] = frL;
-    if (j == V.nj) A.nUer[IX(i, j + 1)] = frR;
     const double div = (fxR - fxL) / dx + (qR * frR - qL * frL) / dr;
     const double lap = -(A.Ex[IX(i + 1, j)] - A.Ex[IX(i, j)]) / dx - (qR * A.Er[IX(i, j + 1)] - qL * A.Er[IX(i, j)]) / dr;
     const double S = A.S ? A.S[IX(i, j)] : 0.0;
@@ -312,24 +322,23 @@ __device__ __forceinline__ void poisson_assemble_cell(const NddView& V, const Po
         vP = vP - vN;
         vN = 0.0;
     }
-    const size_t k = IX0(i, j);
-    aP[k] = vP;
-    aE[k] = vE;
-    aW[k] = vW;
-    aN[k] = vN;
-    aS[k] = vS;
-    b[k] = vb;
+    A.nUex[IX(i, j)] = fxL;
+    if (i == V.ni) A.nUex[IX(i + 1, j)] = fxR;
+    if (j >= 2) A.nUer[IX(i, j)] = frL;
+    if (j == V.nj) A.nUer[IX(i, j + 1)] = frR;
+    return Row5{vP, vE, vW, vN, vS, vb};
 }
 
-// with s_out set, also the multigrid's level-0 preparation (prep_cell)
+// with s_out set, also the multigrid's level-0 preparation (prep_row)
 __global__ void k_poisson_assemble(NddView V, PoissonArgs A, double* aP, double* aE, double* aW, double* aN,
                                    double* aS, double* b, double* s_out, double* part)
 {
     THREAD_IJ(1, 1);
     double amax = 0.0, bb = 0.0;
     if (i <= V.ni && j <= V.nj) {
-        poisson_assemble_cell(V, A, i, j, aP, aE, aW, aN, aS, b);
-        if (s_out) prep_cell(V, i, j, aP, aE, aW, aN, aS, b, s_out, amax, bb);
+        Row5 R = poisson_assemble_cell(V, A, i, j);
+        if (s_out) prep_row(V, i, j, R, s_out, amax, bb);
+        store_row(V, i, j, R, aP, aE, aW, aN, aS, b);
     }
     if (s_out) prep_partials(amax, bb, part);
 }
@@ -422,8 +431,8 @@ struct SgArgs {
 // predicted SG flux on x face (i, j), i = 1..ni+1: the Scharfetter-Gummel
 // flux plus the mobility-tensor correction of the previous fluxes inside,
 // the wall fluxes on faces 1 and ni+1 (fluidModule.hpp:3239-3367, energy
-// :4161-4316)
-__device__ __forceinline__ double sg_flux_x(const NddView& V, const SgArgs& A, int i, int j)
+// :4161-4316).  bp, bm: the face's weights bern_bp / bern_bm (inner faces).
+__device__ __forceinline__ double sg_flux_x(const NddView& V, const SgArgs& A, int i, int j, double bp, double bm)
 {
 #pragma clang fp contract(off)
     const double dx = V.dx;
@@ -431,12 +440,11 @@ __device__ __forceinline__ double sg_flux_x(const NddView& V, const SgArgs& A, i
     const double* T = A.T;
     if (i >= 2 && i <= V.ni) {
         const double rL = (r[j] + r[j - 1]) / 2.0, rR = (r[j] + r[j + 1]) / 2.0;
-        const Face F = sg_face(V, A.sp, A.m.mu_star, A.Ex[IX(i, j)], T[IX(i - 1, j)], T[IX(i, j)], dx);
         const double bLr = (double)(j != 1), bLrD = (double)(j != 1), bRr = 1.0, bRrD = 1.0;
         const double tr = (rR * A.Fr_t[IX(i, j + 1)] * bRr + rL * A.Fr_t[IX(i, j)] * bLr +
                            rR * A.Fr_t[IX(i - 1, j + 1)] * bRrD + rL * A.Fr_t[IX(i - 1, j)] * bLrD) /
                           (4.0 * r[j]);
-        double sg = bern_bp(F.vd, F.z, F.D / dx) * A.n[IX(i - 1, j)] - bern_bm(F.vd, F.z, F.D / dx) * A.n[IX(i, j)];
+        double sg = bp * A.n[IX(i - 1, j)] - bm * A.n[IX(i, j)];
         if (A.sp == SP_ENERGY) sg = sg * 5.0 / 3.0;
         return sg + A.Fx_t[IX(i, j)] - A.m.mu_star * (A.m.inv_par * A.Fx_t[IX(i, j)] + A.m.inv_cross * tr);
     }
@@ -476,8 +484,9 @@ __device__ __forceinline__ double sg_flux_x(const NddView& V, const SgArgs& A, i
 }
 
 // predicted SG flux on r face (i, j), j = 2..nj+1; face nj+1 is the open
-// upper boundary (fluidModule.hpp:3443-3542, energy :4403-4506)
-__device__ __forceinline__ double sg_flux_r(const NddView& V, const SgArgs& A, int i, int j)
+// upper boundary (fluidModule.hpp:3443-3542, energy :4403-4506).  bp, bm as
+// for sg_flux_x.
+__device__ __forceinline__ double sg_flux_r(const NddView& V, const SgArgs& A, int i, int j, double bp, double bm)
 {
 #pragma clang fp contract(off)
     const double dr = V.dr;
@@ -485,13 +494,12 @@ __device__ __forceinline__ double sg_flux_r(const NddView& V, const SgArgs& A, i
     const double* T = A.T;
     if (j <= V.nj) {
         const double r_t = (r[j] + r[j - 1]) / 2.0;
-        const Face F = sg_face(V, A.sp, A.m.mu_star, A.Er[IX(i, j)], T[IX(i, j - 1)], T[IX(i, j)], dr);
         const double bLx = (double)(i != 1), bLxD = (double)(i != 1);
         const double bRx = (double)(i != V.ni), bRxD = (double)(i != V.ni);
         const double tx = (r[j] * A.Fx_t[IX(i + 1, j)] * bRx + r[j] * A.Fx_t[IX(i, j)] * bLx +
                            r[j - 1] * A.Fx_t[IX(i + 1, j - 1)] * bRxD + r[j - 1] * A.Fx_t[IX(i, j - 1)] * bLxD) /
                           (4.0 * r_t);
-        double sg = bern_bp(F.vd, F.z, F.D / dr) * A.n[IX(i, j - 1)] - bern_bm(F.vd, F.z, F.D / dr) * A.n[IX(i, j)];
+        double sg = bp * A.n[IX(i, j - 1)] - bm * A.n[IX(i, j)];
         if (A.sp == SP_ENERGY) sg = sg * 5.0 / 3.0;
         return sg + A.Fr_t[IX(i, j)] - A.m.mu_star * (A.m.inv_cross * tx + A.m.inv_par * A.Fr_t[IX(i, j)]);
     } else if (A.sp == SP_ENERGY) {
@@ -503,8 +511,9 @@ __device__ __forceinline__ double sg_flux_r(const NddView& V, const SgArgs& A, i
 }
 
 
-__device__ __forceinline__ void sg_assemble_cell(const NddView& V, const SgArgs& A, int i, int j, double* aP,
-                                                 double* aE, double* aW, double* aN, double* aS, double* b)
+// the weights of each face are formed once, for the coefficients and the
+// predicted flux; the fluxes are stored after every load of the cell
+__device__ __forceinline__ Row5 sg_assemble_cell(const NddView& V, const SgArgs& A, int i, int j)
 {
 #pragma clang fp contract(off)
     const double dx = V.dx, dr = V.dr, dt = V.dt;
@@ -520,24 +529,23 @@ __device__ __forceinline__ void sg_assemble_cell(const NddView& V, const SgArgs&
     const Face Rr = sg_face(V, A.sp, mu, A.Er[IX(i, j + 1)], T[IX(i, j)], T[IX(i, j + 1)], dr);
     const double bLx = (double)(i != 1), bRx = (double)(i != V.ni);
     const double bLr = (double)(j != 1), bRr = (double)(j != V.nj);
-    const double vP = bern_bp(Rx.vd, Rx.z, Rx.D / dx) * bRx * dr + bern_bm(Lx.vd, Lx.z, Lx.D / dx) * bLx * dr +
-                      qR * bern_bp(Rr.vd, Rr.z, Rr.D / dr) * bRr * dx + qL * bern_bm(Lr.vd, Lr.z, Lr.D / dr) * bLr * dx;
-    const size_t k = IX0(i, j);
-    aP[k] = vP + A.tfac * dx * dr / dt;
-    aE[k] = bern_bm(Rx.vd, Rx.z, Rx.D / dx) * bRx * dr;
-    aW[k] = bern_bp(Lx.vd, Lx.z, Lx.D / dx) * bLx * dr;
-    aN[k] = qR * bern_bm(Rr.vd, Rr.z, Rr.D / dr) * bRr * dx;
-    aS[k] = qL * bern_bp(Lr.vd, Lr.z, Lr.D / dr) * bLr * dx;
+    const double bpLx = bern_bp(Lx.vd, Lx.z, Lx.D / dx), bmLx = bern_bm(Lx.vd, Lx.z, Lx.D / dx);
+    const double bpRx = bern_bp(Rx.vd, Rx.z, Rx.D / dx), bmRx = bern_bm(Rx.vd, Rx.z, Rx.D / dx);
+    const double bpLr = bern_bp(Lr.vd, Lr.z, Lr.D / dr), bmLr = bern_bm(Lr.vd, Lr.z, Lr.D / dr);
+    const double bpRr = bern_bp(Rr.vd, Rr.z, Rr.D / dr), bmRr = bern_bm(Rr.vd, Rr.z, Rr.D / dr);
+    const double vP = bpRx * bRx * dr + bmLx * bLx * dr + qR * bpRr * bRr * dx + qL * bmLr * bLr * dx;
+    Row5 R;
+    R.P = vP + A.tfac * dx * dr / dt;
+    R.E = bmRx * bRx * dr;
+    R.W = bpLx * bLx * dr;
+    R.N = qR * bmRr * bRr * dx;
+    R.S = qL * bpLr * bLr * dx;
     // the predicted fluxes of the cell's four faces; the cell stores its west
     // and south faces (east and north too on the last row and column), the
     // axis face j = 1 is not predicted
-    const double fxL = sg_flux_x(V, A, i, j), fxR = sg_flux_x(V, A, i + 1, j);
-    const double frL = j >= 2 ? sg_flux_r(V, A, i, j) : A.Fr[IX(i, 1)];
-    const double frR = sg_flux_r(V, A, i, j + 1);
-    A.Fx[IX(i, j)] = fxL;
-    if (i == V.ni) A.Fx[IX(i + 1, j)] = fxR;
-    if (j >= 2) A.Fr[IX(i, j)] = frL;
-    if (j == V.nj) A.Fr[IX(i, j + 1)] = frR;
+    const double fxL = sg_flux_x(V, A, i, j, bpLx, bmLx), fxR = sg_flux_x(V, A, i + 1, j, bpRx, bmRx);
+    const double frL = j >= 2 ? sg_flux_r(V, A, i, j, bpLr, bmLr) : A.Fr[IX(i, 1)];
+    const double frR = sg_flux_r(V, A, i, j + 1, bpRr, bmRr);
     const double div = (fxR - fxL) / dx + (qR * frR - qL * frL) / dr;
     if (A.sp == SP_ENERGY) {
         const double bLxw = (double)(i != 1), bRxw = (double)(i != V.ni);
@@ -550,22 +558,28 @@ __device__ __forceinline__ void sg_assemble_cell(const NddView& V, const SgArgs&
         const double eloss = A.eloss ? A.eloss[IX(i, j)] : 0.0;
         const double R3 = Pabs - A.rhoe[IX(i, j)] * eloss;
         const double R4 = -div;
-        b[k] = 3.0 / 5.0 * (R1 + R2 + R3 + R4) * dx * dr;
+        R.b = 3.0 / 5.0 * (R1 + R2 + R3 + R4) * dx * dr;
     } else {
         const double S = A.S ? A.S[IX(i, j)] : 0.0;
-        b[k] = (-div + S - (A.n[IX(i, j)] - A.n_old[IX(i, j)]) / dt) * dx * dr;
+        R.b = (-div + S - (A.n[IX(i, j)] - A.n_old[IX(i, j)]) / dt) * dx * dr;
     }
+    A.Fx[IX(i, j)] = fxL;
+    if (i == V.ni) A.Fx[IX(i + 1, j)] = fxR;
+    if (j >= 2) A.Fr[IX(i, j)] = frL;
+    if (j == V.nj) A.Fr[IX(i, j + 1)] = frR;
+    return R;
 }
 
-// with s_out set, also the multigrid's level-0 preparation (prep_cell)
+// with s_out set, also the multigrid's level-0 preparation (prep_row)
 __global__ void k_sg_assemble(NddView V, SgArgs A, double* aP, double* aE, double* aW, double* aN, double* aS,
                               double* b, double* s_out, double* part)
 {
     THREAD_IJ(1, 1);
     double amax = 0.0, bb = 0.0;
     if (i <= V.ni && j <= V.nj) {
-        sg_assemble_cell(V, A, i, j, aP, aE, aW, aN, aS, b);
-        if (s_out) prep_cell(V, i, j, aP, aE, aW, aN, aS, b, s_out, amax, bb);
+        Row5 R = sg_assemble_cell(V, A, i, j);
+        if (s_out) prep_row(V, i, j, R, s_out, amax, bb);
+        store_row(V, i, j, R, aP, aE, aW, aN, aS, b);
     }
     if (s_out) prep_partials(amax, bb, part);
 }
@@ -607,16 +621,19 @@ struct ResPairs {
 
 __global__ void k_calcres(NddView V, ResPairs P, int mode, double eps, double* __restrict__ part, int nb)
 {
-    __shared__ double lds[8];
+    __shared__ double lds[6 * NB_X * NB_Y / 64];
     THREAD_IJ(1, 1);
     double a[3] = {0.0, 0.0, 0.0}, c[3] = {0.0, 0.0, 0.0};
     if (i <= V.ni && j <= V.nj) {
-        const double* nw[3] = {P.n0, P.n1, P.n2};
-        const double* od[3] = {P.o0, P.o1, P.o2};
+        // all six loads first (the face boxes start one row or column later;
+        // the first face's value is loaded and not used)
+        const double vnq[3] = {P.n0[IX(i, j)], P.n1[IX(i, j)], P.n2[IX(i, j)]};
+        const double voq[3] = {P.o0[IX(i, j)], P.o1[IX(i, j)], P.o2[IX(i, j)]};
         const bool in[3] = {true, i >= 2, j >= 2};
+#pragma unroll
         for (int q = 0; q < 3; q++) {
             if (!in[q]) continue;
-            const double vn = nw[q][IX(i, j)], vo = od[q][IX(i, j)];
+            const double vn = vnq[q], vo = voq[q];
             const double d = vn - vo;
             if (mode == 0) {
                 a[q] = fabs(d);
@@ -634,32 +651,43 @@ __global__ void k_calcres(NddView V, ResPairs P, int mode, double eps, double* _
     }
     const bool mx_a = mode == 0 || mode == 2, mx_c = mode == 2;
     const int bid = blockIdx.y * gridDim.x + blockIdx.x;
-    for (int q = 0; q < 3; q++) {
-        const double ta = mx_a ? block_max<NB_X * NB_Y>(a[q], lds) : block_sum<NB_X * NB_Y>(a[q], lds);
-        const double tc = mx_c ? block_max<NB_X * NB_Y>(c[q], lds) : block_sum<NB_X * NB_Y>(c[q], lds);
-        if (lin_tid() == 0) {
-            part[(2 * q) * nb + bid] = ta;
-            part[(2 * q + 1) * nb + bid] = tc;
-        }
-    }
+    double v[6] = {a[0], c[0], a[1], c[1], a[2], c[2]};
+    const bool mx[6] = {mx_a, mx_c, mx_a, mx_c, mx_a, mx_c};
+    block_reduce_vec<NB_X * NB_Y, 6>(v, mx, lds);
+    if (lin_tid() == 0)
+        for (int k = 0; k < 6; k++) part[k * nb + bid] = v[k];
 }
 
 // errors e[0..2] and their max e[3] (also into side when set)
 __global__ void k_calcres_finish(const double* __restrict__ part, int nb, int mode, double* __restrict__ e,
                                  double* __restrict__ side)
 {
-    __shared__ double lds[8];
+    __shared__ double lds[6 * 256 / 64];
     const bool mx_a = mode == 0 || mode == 2, mx_c = mode == 2;
+    double v[6];
+    const bool mx[6] = {mx_a, mx_c, mx_a, mx_c, mx_a, mx_c};
+#pragma unroll
+    for (int m = 0; m < 6; m++) v[m] = 0.0;
+    // each thread's partials in the order k = t, t + 256, ...; the loads of
+    // two rounds are issued before they are combined
+    for (int k = threadIdx.x; k < nb; k += 2 * 256) {
+        const bool two = k + 256 < nb;
+        double p0[6], p1[6];
+#pragma unroll
+        for (int m = 0; m < 6; m++) {
+            p0[m] = part[m * nb + k];
+            p1[m] = two ? part[m * nb + k + 256] : 0.0;
+        }
+#pragma unroll
+        for (int m = 0; m < 6; m++) {
+            v[m] = mx[m] ? fmax(v[m], p0[m]) : v[m] + p0[m];
+            if (two) v[m] = mx[m] ? fmax(v[m], p1[m]) : v[m] + p1[m];
+        }
+    }
+    block_reduce_vec<256, 6>(v, mx, lds);
     double err[3];
     for (int q = 0; q < 3; q++) {
-        double sa = mx_a ? 0.0 : 0.0, sc = 0.0;
-        for (int k = threadIdx.x; k < nb; k += blockDim.x) {
-            const double va = part[(2 * q) * nb + k], vc = part[(2 * q + 1) * nb + k];
-            sa = mx_a ? fmax(sa, va) : sa + va;
-            sc = mx_c ? fmax(sc, vc) : sc + vc;
-        }
-        sa = mx_a ? block_max<256>(sa, lds) : block_sum<256>(sa, lds);
-        sc = mx_c ? block_max<256>(sc, lds) : block_sum<256>(sc, lds);
+        const double sa = v[2 * q], sc = v[2 * q + 1];
         if (mode == 0)
             err[q] = sa;
         else if (mode == 1)
