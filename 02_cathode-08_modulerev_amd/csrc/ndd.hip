@@ -426,6 +426,11 @@ struct SgArgs {
     const double *S, *Pabs, *eloss;
     const double *Fx_t, *Fr_t;
     double *n, *Fx, *Fr;
+    // the SG weights bp, bm of each cell's x face (i, j) and r face (i, j),
+    // planes of wn doubles in the multigrid's row pitch: written by the
+    // assembly, read by the correction of the same PC pass
+    double* wgt;
+    size_t wn;
 };
 
 // predicted SG flux on x face (i, j), i = 1..ni+1: the Scharfetter-Gummel
@@ -567,6 +572,11 @@ __device__ __forceinline__ Row5 sg_assemble_cell(const NddView& V, const SgArgs&
     if (i == V.ni) A.Fx[IX(i + 1, j)] = fxR;
     if (j >= 2) A.Fr[IX(i, j)] = frL;
     if (j == V.nj) A.Fr[IX(i, j + 1)] = frR;
+    const size_t k = IX0(i, j);
+    A.wgt[k] = bpLx;
+    A.wgt[A.wn + k] = bmLx;
+    A.wgt[2 * A.wn + k] = bpLr;
+    A.wgt[3 * A.wn + k] = bmLr;
     return R;
 }
 
@@ -586,24 +596,21 @@ __global__ void k_sg_assemble(NddView V, SgArgs A, double* aP, double* aE, doubl
 
 // flux correction on interior faces and the positivity-guarded density
 // update (fluidModule.hpp:3682-3757); thread (i, j) owns x-face (i, j),
-// r-face (i, j) and cell (i, j)
+// r-face (i, j) and cell (i, j).  The faces' SG weights are the ones the
+// assembly of this PC pass formed (same fields, same temperatures).
 __global__ void k_sg_correct(NddView V, SgArgs A, const double* __restrict__ dn)
 {
 #pragma clang fp contract(off)
     THREAD_IJ(1, 1);
     if (i > V.ni || j > V.nj) return;
-    const double dx = V.dx, dr = V.dr;
-    const double* T = A.T;
-    const double mu = A.m.mu_star;
+    const size_t k = IX0(i, j);
     if (i >= 2) {
-        const Face F = sg_face(V, A.sp, mu, A.Ex[IX(i, j)], T[IX(i - 1, j)], T[IX(i, j)], dx);
-        double d = bern_bp(F.vd, F.z, F.D / dx) * dn[IX0(i - 1, j)] - bern_bm(F.vd, F.z, F.D / dx) * dn[IX0(i, j)];
+        double d = A.wgt[k] * dn[IX0(i - 1, j)] - A.wgt[A.wn + k] * dn[k];
         if (A.sp == SP_ENERGY) d = d * 5.0 / 3.0;
         A.Fx[IX(i, j)] = A.Fx[IX(i, j)] + d;
     }
     if (j >= 2) {
-        const Face F = sg_face(V, A.sp, mu, A.Er[IX(i, j)], T[IX(i, j - 1)], T[IX(i, j)], dr);
-        double d = bern_bp(F.vd, F.z, F.D / dr) * dn[IX0(i, j - 1)] - bern_bm(F.vd, F.z, F.D / dr) * dn[IX0(i, j)];
+        double d = A.wgt[2 * A.wn + k] * dn[IX0(i, j - 1)] - A.wgt[3 * A.wn + k] * dn[k];
         if (A.sp == SP_ENERGY) d = d * 5.0 / 3.0;
         A.Fr[IX(i, j)] = A.Fr[IX(i, j)] + d;
     }
@@ -857,6 +864,7 @@ struct NddHandle {
     DevBuf<double> work;    // scratch arrays
     DevBuf<double> radii, see;
     DevBuf<double> part, scal;
+    DevBuf<double> sgw;     // SG face weights of one PC pass (SgArgs::wgt)
     double* h_scal = nullptr;
     double* F[19] = {nullptr};
     bool has_src[3] = {false, false, false};
@@ -1171,6 +1179,8 @@ static void sg_routine(NddHandle& H, int sp, bool report, double* err3)
     A.eloss = H.has_src[2] ? F[F_ELOSS] : nullptr;
     A.Fx_t = H.tmp_x;
     A.Fr_t = H.tmp_r;
+    A.wgt = H.sgw.p;
+    A.wn = (size_t)(V.ni + 2) * V.ld0;
     int maxit, lin_maxit;
     double cnv, lin_tol;
     double* n_old;
@@ -1322,6 +1332,7 @@ int cathode_ndd_create(const cathode_ndd_params* p, void** handle)
         H->mg.set_weights(H->radii.p);  // conservative rows: w_j = r_j
         H->nb = cdiv(nj, NB_X) * cdiv(ni, NB_Y);
         H->part.alloc(6 * H->nb + 64);
+        H->sgw.alloc(4 * (size_t)n0);
         H->scal.alloc(16);
         if (p->learned) {
             for (auto& slot : H->wbuf)
