@@ -116,6 +116,16 @@ __device__ inline Face sg_face(const NddView& V, int sp, double mu_star, double 
     return F;
 }
 
+// the face's Scharfetter-Gummel weights bp (upstream) and bm (downstream)
+__device__ inline void sg_weights(const NddView& V, int sp, double mu_star, double E, double Ta, double Tc, double h,
+                                  double& bp, double& bm)
+{
+#pragma clang fp contract(off)
+    const Face F = sg_face(V, sp, mu_star, E, Ta, Tc, h);
+    bp = bern_bp(F.vd, F.z, F.D / h);
+    bm = bern_bm(F.vd, F.z, F.D / h);
+}
+
 // func_Te, miniFuncs.hpp:35-46
 __device__ inline double func_Te(const double* v, int n, double Te, double dTe)
 {
@@ -431,6 +441,7 @@ struct SgArgs {
     // assembly, read by the correction of the same PC pass
     double* wgt;
     size_t wn;
+    int reuse;  // the weights are already in wgt (a later PC pass of the routine)
 };
 
 // predicted SG flux on x face (i, j), i = 1..ni+1: the Scharfetter-Gummel
@@ -528,16 +539,35 @@ __device__ __forceinline__ Row5 sg_assemble_cell(const NddView& V, const SgArgs&
     const double qR = (r[j] + r[j + 1]) / 2.0 / r[j];
     const double qL = (r[j] + r[j - 1]) / 2.0 / r[j];
     const double mu = A.m.mu_star;
-    const Face Lx = sg_face(V, A.sp, mu, A.Ex[IX(i, j)], T[IX(i - 1, j)], T[IX(i, j)], dx);
-    const Face Rx = sg_face(V, A.sp, mu, A.Ex[IX(i + 1, j)], T[IX(i, j)], T[IX(i + 1, j)], dx);
-    const Face Lr = sg_face(V, A.sp, mu, A.Er[IX(i, j)], T[IX(i, j - 1)], T[IX(i, j)], dr);
-    const Face Rr = sg_face(V, A.sp, mu, A.Er[IX(i, j + 1)], T[IX(i, j)], T[IX(i, j + 1)], dr);
     const double bLx = (double)(i != 1), bRx = (double)(i != V.ni);
     const double bLr = (double)(j != 1), bRr = (double)(j != V.nj);
-    const double bpLx = bern_bp(Lx.vd, Lx.z, Lx.D / dx), bmLx = bern_bm(Lx.vd, Lx.z, Lx.D / dx);
-    const double bpRx = bern_bp(Rx.vd, Rx.z, Rx.D / dx), bmRx = bern_bm(Rx.vd, Rx.z, Rx.D / dx);
-    const double bpLr = bern_bp(Lr.vd, Lr.z, Lr.D / dr), bmLr = bern_bm(Lr.vd, Lr.z, Lr.D / dr);
-    const double bpRr = bern_bp(Rr.vd, Rr.z, Rr.D / dr), bmRr = bern_bm(Rr.vd, Rr.z, Rr.D / dr);
+    // the weights of the four faces: formed here on the routine's first PC
+    // pass, read back on later passes (the fields and temperatures they
+    // depend on do not change within the routine); the east face of the last
+    // row and the north face of the last column belong to no cell's store
+    const size_t k = IX0(i, j), wn = A.wn;
+    double bpLx, bmLx, bpRx, bmRx, bpLr, bmLr, bpRr, bmRr;
+    if (A.reuse) {
+        bpLx = A.wgt[k];
+        bmLx = A.wgt[wn + k];
+        bpLr = A.wgt[2 * wn + k];
+        bmLr = A.wgt[3 * wn + k];
+    } else {
+        sg_weights(V, A.sp, mu, A.Ex[IX(i, j)], T[IX(i - 1, j)], T[IX(i, j)], dx, bpLx, bmLx);
+        sg_weights(V, A.sp, mu, A.Er[IX(i, j)], T[IX(i, j - 1)], T[IX(i, j)], dr, bpLr, bmLr);
+    }
+    if (A.reuse && i < V.ni) {
+        bpRx = A.wgt[k + V.ld0];
+        bmRx = A.wgt[wn + k + V.ld0];
+    } else {
+        sg_weights(V, A.sp, mu, A.Ex[IX(i + 1, j)], T[IX(i, j)], T[IX(i + 1, j)], dx, bpRx, bmRx);
+    }
+    if (A.reuse && j < V.nj) {
+        bpRr = A.wgt[2 * wn + k + 1];
+        bmRr = A.wgt[3 * wn + k + 1];
+    } else {
+        sg_weights(V, A.sp, mu, A.Er[IX(i, j + 1)], T[IX(i, j)], T[IX(i, j + 1)], dr, bpRr, bmRr);
+    }
     const double vP = bpRx * bRx * dr + bmLx * bLx * dr + qR * bpRr * bRr * dx + qL * bmLr * bLr * dx;
     Row5 R;
     R.P = vP + A.tfac * dx * dr / dt;
@@ -572,11 +602,12 @@ __device__ __forceinline__ Row5 sg_assemble_cell(const NddView& V, const SgArgs&
     if (i == V.ni) A.Fx[IX(i + 1, j)] = fxR;
     if (j >= 2) A.Fr[IX(i, j)] = frL;
     if (j == V.nj) A.Fr[IX(i, j + 1)] = frR;
-    const size_t k = IX0(i, j);
-    A.wgt[k] = bpLx;
-    A.wgt[A.wn + k] = bmLx;
-    A.wgt[2 * A.wn + k] = bpLr;
-    A.wgt[3 * A.wn + k] = bmLr;
+    if (!A.reuse) {
+        A.wgt[k] = bpLx;
+        A.wgt[wn + k] = bmLx;
+        A.wgt[2 * wn + k] = bpLr;
+        A.wgt[3 * wn + k] = bmLr;
+    }
     return R;
 }
 
@@ -1212,6 +1243,7 @@ static void sg_routine(NddHandle& H, int sp, bool report, double* err3)
         } else {
             d2d(H, {{H.tmp_c, A.n}, {H.tmp_x, A.Fx}, {H.tmp_r, A.Fr}});
         }
+        A.reuse = ncount > 1;
         hipLaunchKernelGGL(k_sg_assemble, ggrid(V.ni, V.nj), gblk(), 0, H.s, V, A, H.mg.aP(), H.mg.aE(), H.mg.aW(),
                            H.mg.aN(), H.mg.aS(), H.mg.rhs(), H.mg.prep_s(), H.mg.prep_part());
         H.mg.prepared = true;
