@@ -28,6 +28,7 @@
 #include <cmath>
 #include <cstring>
 #include <initializer_list>
+#include <stdexcept>
 #include <utility>
 #include <memory>
 #include <vector>
@@ -628,8 +629,12 @@ __global__ void k_sg_assemble(NddView V, SgArgs A, double* aP, double* aE, doubl
 // flux correction on interior faces and the positivity-guarded density
 // update (fluidModule.hpp:3682-3757); thread (i, j) owns x-face (i, j),
 // r-face (i, j) and cell (i, j).  The faces' SG weights are the ones the
-// assembly of this PC pass formed (same fields, same temperatures).
-__global__ void k_sg_correct(NddView V, SgArgs A, const double* __restrict__ dn)
+// assembly of this PC pass formed (same fields, same temperatures).  The
+// corrected density and every face flux the assembly predicted also go to
+// the next pass's start-of-pass copies nc, nx, nr (their other entries were
+// copied at the routine's first pass and do not change).
+__global__ void k_sg_correct(NddView V, SgArgs A, const double* __restrict__ dn, double* __restrict__ nc,
+                             double* __restrict__ nx, double* __restrict__ nr)
 {
 #pragma clang fp contract(off)
     THREAD_IJ(1, 1);
@@ -638,15 +643,27 @@ __global__ void k_sg_correct(NddView V, SgArgs A, const double* __restrict__ dn)
     if (i >= 2) {
         double d = A.wgt[k] * dn[IX0(i - 1, j)] - A.wgt[A.wn + k] * dn[k];
         if (A.sp == SP_ENERGY) d = d * 5.0 / 3.0;
-        A.Fx[IX(i, j)] = A.Fx[IX(i, j)] + d;
+        const double f = A.Fx[IX(i, j)] + d;
+        A.Fx[IX(i, j)] = f;
+        nx[IX(i, j)] = f;
+    } else {
+        nx[IX(1, j)] = A.Fx[IX(1, j)];
     }
+    if (i == V.ni) nx[IX(i + 1, j)] = A.Fx[IX(i + 1, j)];
     if (j >= 2) {
         double d = A.wgt[2 * A.wn + k] * dn[IX0(i, j - 1)] - A.wgt[3 * A.wn + k] * dn[k];
         if (A.sp == SP_ENERGY) d = d * 5.0 / 3.0;
-        A.Fr[IX(i, j)] = A.Fr[IX(i, j)] + d;
+        const double f = A.Fr[IX(i, j)] + d;
+        A.Fr[IX(i, j)] = f;
+        nr[IX(i, j)] = f;
+    } else {
+        nr[IX(i, 1)] = A.Fr[IX(i, 1)];
     }
-    const double nn = A.n[IX(i, j)] + dn[IX0(i, j)];
+    if (j == V.nj) nr[IX(i, j + 1)] = A.Fr[IX(i, j + 1)];
+    const double n0 = A.n[IX(i, j)];
+    const double nn = n0 + dn[IX0(i, j)];
     if (nn > 0.0) A.n[IX(i, j)] = nn;
+    nc[IX(i, j)] = nn > 0.0 ? nn : n0;
 }
 
 // ================================================================== reductions
@@ -901,6 +918,7 @@ struct NddHandle {
     bool has_src[3] = {false, false, false};
     // work arrays
     double *phi_old, *Ex_old, *Er_old, *tmp_c, *tmp_x, *tmp_r, *dphi, *s_old, *sx_old, *sr_old, *Te_old, *rhoi_old;
+    double *tmp2_c, *tmp2_x, *tmp2_r;  // the other set of start-of-pass copies (SG passes alternate)
     MGSolver mg;
     cathode_ndd_report rep{};
     int nb = 0;
@@ -943,9 +961,10 @@ static inline dim3 gblk() { return dim3(NB_X, NB_Y); }
 static inline dim3 ggrid(int ni, int nj) { return dim3(cdiv(nj, NB_X), cdiv(ni, NB_Y)); }
 
 // up to 8 whole-plane copies in one launch (blockIdx.y picks the pair)
+constexpr int COPY_MAX = 12;
 struct CopyList {
-    double* dst[8];
-    const double* src[8];
+    double* dst[COPY_MAX];
+    const double* src[COPY_MAX];
 };
 
 __global__ void k_copy_planes(CopyList L, size_t n)
@@ -962,6 +981,7 @@ static void d2d(NddHandle& H, std::initializer_list<std::pair<double*, const dou
     CopyList L{};
     int q = 0;
     for (const auto& pr : pairs) {
+        if (q == COPY_MAX) throw std::logic_error("d2d: more than COPY_MAX planes");
         L.dst[q] = pr.first;
         L.src[q] = pr.second;
         q++;
@@ -1208,8 +1228,6 @@ static void sg_routine(NddHandle& H, int sp, bool report, double* err3)
     A.S = H.has_src[0] ? F[F_S] : nullptr;
     A.Pabs = H.has_src[1] ? F[F_PABS] : nullptr;
     A.eloss = H.has_src[2] ? F[F_ELOSS] : nullptr;
-    A.Fx_t = H.tmp_x;
-    A.Fr_t = H.tmp_r;
     A.wgt = H.sgw.p;
     A.wn = (size_t)(V.ni + 2) * V.ld0;
     int maxit, lin_maxit;
@@ -1229,20 +1247,26 @@ static void sg_routine(NddHandle& H, int sp, bool report, double* err3)
         lin_maxit = H.p.maxITR_SOR_rhoeps; lin_tol = H.p.error_cnv_SOR_rhoeps;
     }
     A.n_old = n_old;
+    // start-of-pass copies of n, Fx, Fr: pass p reads set (p - 1) & 1 and its
+    // correction writes set p & 1 for the next pass
+    double* const snap[2][3] = {{H.tmp_c, H.tmp_x, H.tmp_r}, {H.tmp2_c, H.tmp2_x, H.tmp2_r}};
     int ncount = 0;
     for (;;) {
         ncount++;
+        double* const* S = snap[(ncount - 1) & 1];
+        double* const* Sn = snap[ncount & 1];
         if (ncount == 1) {
-            // the step's old values and this iteration's, one launch
+            // the step's old values and this pass's, one launch; both flux
+            // sets get the entries no correction rewrites
             if (sp == SP_ENERGY)
-                d2d(H, {{n_old, A.n}, {H.sx_old, A.Fx}, {H.sr_old, A.Fr}, {H.tmp_c, A.n}, {H.tmp_x, A.Fx},
-                        {H.tmp_r, A.Fr}, {H.Te_old, F[F_TE]}});
+                d2d(H, {{n_old, A.n}, {H.sx_old, A.Fx}, {H.sr_old, A.Fr}, {S[0], A.n}, {S[1], A.Fx}, {S[2], A.Fr},
+                        {Sn[1], A.Fx}, {Sn[2], A.Fr}, {H.Te_old, F[F_TE]}});
             else
-                d2d(H, {{n_old, A.n}, {H.sx_old, A.Fx}, {H.sr_old, A.Fr}, {H.tmp_c, A.n}, {H.tmp_x, A.Fx},
-                        {H.tmp_r, A.Fr}});
-        } else {
-            d2d(H, {{H.tmp_c, A.n}, {H.tmp_x, A.Fx}, {H.tmp_r, A.Fr}});
+                d2d(H, {{n_old, A.n}, {H.sx_old, A.Fx}, {H.sr_old, A.Fr}, {S[0], A.n}, {S[1], A.Fx}, {S[2], A.Fr},
+                        {Sn[1], A.Fx}, {Sn[2], A.Fr}});
         }
+        A.Fx_t = S[1];
+        A.Fr_t = S[2];
         A.reuse = ncount > 1;
         hipLaunchKernelGGL(k_sg_assemble, ggrid(V.ni, V.nj), gblk(), 0, H.s, V, A, H.mg.aP(), H.mg.aE(), H.mg.aW(),
                            H.mg.aN(), H.mg.aS(), H.mg.rhs(), H.mg.prep_s(), H.mg.prep_part());
@@ -1251,14 +1275,14 @@ static void sg_routine(NddHandle& H, int sp, bool report, double* err3)
         const int eq = sp == SP_ELEC ? EQ_ELEC : (sp == SP_ION ? EQ_ION : EQ_ENERGY);
         warm_guess(H, warm_slot(eq, ncount));
         if (!solve(H, ncount > 1 ? cnv : -INFINITY, lin_maxit, lin_tol, warm_slot(eq, ncount))) {
-            d2d(H, {{A.Fx, H.tmp_x}, {A.Fr, H.tmp_r}});
+            d2d(H, {{A.Fx, S[1]}, {A.Fr, S[2]}});
             break;
         }
-        hipLaunchKernelGGL(k_sg_correct, ggrid(V.ni, V.nj), gblk(), 0, H.s, V, A, H.mg.sol());
+        hipLaunchKernelGGL(k_sg_correct, ggrid(V.ni, V.nj), gblk(), 0, H.s, V, A, H.mg.sol(), Sn[0], Sn[1], Sn[2]);
         CATHODE_LAUNCH_CHECK();
         warm_store(H, warm_slot(eq, ncount));
         if (ncount >= maxit) break;
-        ResPairs P{A.n, H.tmp_c, A.Fx, H.tmp_x, A.Fr, H.tmp_r};
+        ResPairs P{A.n, S[0], A.Fx, S[1], A.Fr, S[2]};
         calcres3_async(H, P);
     }
     if (report) {
@@ -1330,12 +1354,13 @@ int cathode_ndd_create(const cathode_ndd_params* p, void** handle)
         H->fields.alloc(19 * H->n);
         CATHODE_HIP(hipMemset(H->fields.p, 0, 19 * H->n * sizeof(double)));
         for (int k = 0; k < 19; k++) H->F[k] = H->fields.p + k * H->n;
-        H->work.alloc(12 * H->n);
-        CATHODE_HIP(hipMemset(H->work.p, 0, 12 * H->n * sizeof(double)));
+        constexpr int NW = 15;
+        H->work.alloc(NW * H->n);
+        CATHODE_HIP(hipMemset(H->work.p, 0, NW * H->n * sizeof(double)));
         CATHODE_HIP(hipDeviceSynchronize());  // null-stream memsets before the handle's stream runs
-        double** w[12] = {&H->phi_old, &H->Ex_old, &H->Er_old, &H->tmp_c, &H->tmp_x, &H->tmp_r,
-                          &H->dphi, &H->s_old, &H->sx_old, &H->sr_old, &H->Te_old, &H->rhoi_old};
-        for (int k = 0; k < 12; k++) *w[k] = H->work.p + k * H->n;
+        double** w[NW] = {&H->phi_old, &H->Ex_old, &H->Er_old, &H->tmp_c,  &H->tmp_x,  &H->tmp_r,  &H->dphi,    &H->s_old,
+                          &H->sx_old,  &H->sr_old, &H->Te_old, &H->rhoi_old, &H->tmp2_c, &H->tmp2_x, &H->tmp2_r};
+        for (int k = 0; k < NW; k++) *w[k] = H->work.p + k * H->n;
         std::vector<double> r(nj + 2);
         for (int j = 0; j < nj + 2; j++) r[j] = p->rmin + (j - 0.5) * p->dr;  // initialModule.hpp:86-88
         H->radii.alloc(nj + 2);
