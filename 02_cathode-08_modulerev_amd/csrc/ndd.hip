@@ -639,31 +639,33 @@ __global__ void k_sg_correct(NddView V, SgArgs A, const double* __restrict__ dn,
 #pragma clang fp contract(off)
     THREAD_IJ(1, 1);
     if (i > V.ni || j > V.nj) return;
-    const size_t k = IX0(i, j);
+    const size_t k = IX0(i, j), wn = A.wn;
+    // every load before the first store
+    const double fx = A.Fx[IX(i, j)], fr = A.Fr[IX(i, j)], n0 = A.n[IX(i, j)];
+    const double fxE = i == V.ni ? A.Fx[IX(i + 1, j)] : 0.0;
+    const double frN = j == V.nj ? A.Fr[IX(i, j + 1)] : 0.0;
+    const double dc = dn[k], dW = dn[IX0(i - 1, j)], dS = dn[IX0(i, j - 1)];
+    const double bpx = A.wgt[k], bmx = A.wgt[wn + k], bpr = A.wgt[2 * wn + k], bmr = A.wgt[3 * wn + k];
+    double fx1 = fx, fr1 = fr;
     if (i >= 2) {
-        double d = A.wgt[k] * dn[IX0(i - 1, j)] - A.wgt[A.wn + k] * dn[k];
+        double d = bpx * dW - bmx * dc;
         if (A.sp == SP_ENERGY) d = d * 5.0 / 3.0;
-        const double f = A.Fx[IX(i, j)] + d;
-        A.Fx[IX(i, j)] = f;
-        nx[IX(i, j)] = f;
-    } else {
-        nx[IX(1, j)] = A.Fx[IX(1, j)];
+        fx1 = fx + d;
+        A.Fx[IX(i, j)] = fx1;
     }
-    if (i == V.ni) nx[IX(i + 1, j)] = A.Fx[IX(i + 1, j)];
     if (j >= 2) {
-        double d = A.wgt[2 * A.wn + k] * dn[IX0(i, j - 1)] - A.wgt[3 * A.wn + k] * dn[k];
+        double d = bpr * dS - bmr * dc;
         if (A.sp == SP_ENERGY) d = d * 5.0 / 3.0;
-        const double f = A.Fr[IX(i, j)] + d;
-        A.Fr[IX(i, j)] = f;
-        nr[IX(i, j)] = f;
-    } else {
-        nr[IX(i, 1)] = A.Fr[IX(i, 1)];
+        fr1 = fr + d;
+        A.Fr[IX(i, j)] = fr1;
     }
-    if (j == V.nj) nr[IX(i, j + 1)] = A.Fr[IX(i, j + 1)];
-    const double n0 = A.n[IX(i, j)];
-    const double nn = n0 + dn[IX0(i, j)];
+    const double nn = n0 + dc;
     if (nn > 0.0) A.n[IX(i, j)] = nn;
+    nx[IX(i, j)] = fx1;
+    nr[IX(i, j)] = fr1;
     nc[IX(i, j)] = nn > 0.0 ? nn : n0;
+    if (i == V.ni) nx[IX(i + 1, j)] = fxE;
+    if (j == V.nj) nr[IX(i, j + 1)] = frN;
 }
 
 // ================================================================== reductions
