@@ -355,17 +355,23 @@ __global__ void k_poisson_assemble(NddView V, PoissonArgs A, double* aP, double*
 }
 
 // phi ghosts before the PC loop (fluidModule.hpp:2160-2253): thread per edge cell
-__global__ void k_poisson_ghost(NddView V, double* phi)
+// (the same values also into cp, the first pass's copy of phi)
+__global__ void k_poisson_ghost(NddView V, double* __restrict__ phi, double* __restrict__ cp)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < V.nj) {
         const int j = t + 1;
-        phi[IX(0, j)] = -phi[IX(1, j)];
-        phi[IX(V.ni + 1, j)] = 2.0 * V.V_bias - phi[IX(V.ni, j)];
+        const double w = -phi[IX(1, j)], e = 2.0 * V.V_bias - phi[IX(V.ni, j)];
+        phi[IX(0, j)] = w;
+        phi[IX(V.ni + 1, j)] = e;
+        cp[IX(0, j)] = w;
+        cp[IX(V.ni + 1, j)] = e;
     }
     if (t < V.ni) {
         const int i = t + 1;
-        phi[IX(i, V.nj + 1)] = phi[IX(i, V.nj)];
+        const double n = phi[IX(i, V.nj)];
+        phi[IX(i, V.nj + 1)] = n;
+        cp[IX(i, V.nj + 1)] = n;
     }
 }
 
@@ -385,46 +391,76 @@ __device__ inline double dphi_at(const NddView& V, const double* __restrict__ so
 // phi += dphi on the extended grid with the ghost reflections of dphi, the
 // wall-face field update (fluidModule.hpp:2850-2958), then the interior face
 // fields E = -grad phi and the flux correction mu* n_f grad dphi
-// (fluidModule.hpp:2959-3033).  phi0 is phi before the update (the PC pass's
-// copy), so every thread can form the updated phi of its neighbours.
+// (fluidModule.hpp:2959-3033).  phi0 is phi at the start of the PC pass, so
+// every thread can form the updated phi of its neighbours.  The new phi and
+// every face flux the assembly predicted also go to the next pass's
+// start-of-pass copies nc, nx, nr (their other entries were copied at the
+// routine's start and do not change).  Every load is issued before the first
+// store.
 __global__ void k_poisson_update(NddView V, const double* __restrict__ sol, const double* __restrict__ phi0,
                                  double* __restrict__ dphi, double* __restrict__ phi, double* __restrict__ Ex,
                                  double* __restrict__ Er, const double* __restrict__ rhoe, double* __restrict__ nUex,
-                                 double* __restrict__ nUer, Mob m)
+                                 double* __restrict__ nUer, Mob m, double* __restrict__ nc, double* __restrict__ nx,
+                                 double* __restrict__ nr)
 {
 #pragma clang fp contract(off)
     THREAD_IJ(0, 0);
     if (i > V.ni + 1 || j > V.nj + 1) return;
     const int ni = V.ni, nj = V.nj;
-    const bool in_i = i >= 1 && i <= ni, in_j = j >= 1 && j <= nj;
+    const bool cell = i >= 1 && i <= ni && j >= 1 && j <= nj;
     const double d = dphi_at(V, sol, i, j);
     const double p = phi0[IX(i, j)] + d;
+    double dl = 0.0, pl = 0.0, ds = 0.0, ps = 0.0, rc = 0.0, rl = 0.0, rs = 0.0;
+    double ux = 0.0, ur = 0.0, uxE = 0.0, urN = 0.0, ex1 = 0.0, exN = 0.0;
+    if (cell) {
+        ux = nUex[IX(i, j)];
+        ur = nUer[IX(i, j)];
+        rc = rhoe[IX(i, j)];
+        if (i >= 2) {
+            dl = dphi_at(V, sol, i - 1, j);
+            pl = phi0[IX(i - 1, j)] + dl;
+            rl = rhoe[IX(i - 1, j)];
+        }
+        if (j >= 2) {
+            ds = dphi_at(V, sol, i, j - 1);
+            ps = phi0[IX(i, j - 1)] + ds;
+            rs = rhoe[IX(i, j - 1)];
+        }
+        if (i == 1) ex1 = Ex[IX(1, j)];
+        if (i == ni) {
+            exN = Ex[IX(ni + 1, j)];
+            uxE = nUex[IX(ni + 1, j)];
+        }
+        if (j == nj) urN = nUer[IX(i, nj + 1)];
+    }
     dphi[IX(i, j)] = d;
     phi[IX(i, j)] = p;
-    if (in_j && i == 1) {
-        const double d1 = sol[IX0(1, j)];
-        Ex[IX(1, j)] = Ex[IX(1, j)] - (d1 - (-d1)) / V.dx;
-    }
-    if (in_j && i == ni) {
-        const double dn = sol[IX0(ni, j)];
-        Ex[IX(ni + 1, j)] = Ex[IX(ni + 1, j)] - ((-dn) - dn) / V.dx;
-    }
-    if (in_i && j == nj) Er[IX(i, nj + 1)] = 0.0;
-    if (!(in_i && in_j)) return;
+    nc[IX(i, j)] = p;
+    if (!cell) return;
+    // wall faces: d is the solution at (1, j) or (ni, j) there
+    if (i == 1) Ex[IX(1, j)] = ex1 - (d - (-d)) / V.dx;
+    if (i == ni) Ex[IX(ni + 1, j)] = exN - ((-d) - d) / V.dx;
+    if (j == nj) Er[IX(i, nj + 1)] = 0.0;
     if (i >= 2) {
-        const double dl = dphi_at(V, sol, i - 1, j);
-        const double pl = phi0[IX(i - 1, j)] + dl;
         Ex[IX(i, j)] = -(p - pl) / V.dx;
-        const double n_f = (rhoe[IX(i, j)] + rhoe[IX(i - 1, j)]) / 2.0;
-        nUex[IX(i, j)] = nUex[IX(i, j)] + m.mu_star * n_f * ((d - dl) / V.dx);
+        const double n_f = (rc + rl) / 2.0;
+        const double u = ux + m.mu_star * n_f * ((d - dl) / V.dx);
+        nUex[IX(i, j)] = u;
+        nx[IX(i, j)] = u;
+    } else {
+        nx[IX(1, j)] = ux;
     }
+    if (i == ni) nx[IX(ni + 1, j)] = uxE;
     if (j >= 2) {
-        const double ds = dphi_at(V, sol, i, j - 1);
-        const double ps = phi0[IX(i, j - 1)] + ds;
         Er[IX(i, j)] = -(p - ps) / V.dr;
-        const double n_f = (rhoe[IX(i, j)] + rhoe[IX(i, j - 1)]) / 2.0;
-        nUer[IX(i, j)] = nUer[IX(i, j)] + m.mu_star * n_f * ((d - ds) / V.dr);
+        const double n_f = (rc + rs) / 2.0;
+        const double u = ur + m.mu_star * n_f * ((d - ds) / V.dr);
+        nUer[IX(i, j)] = u;
+        nr[IX(i, j)] = u;
+    } else {
+        nr[IX(i, 1)] = ur;
     }
+    if (j == nj) nr[IX(i, nj + 1)] = urN;
 }
 
 // ================================================================== SG equations
@@ -1160,19 +1196,30 @@ static void poisson_routine(NddHandle& H, bool report)
 {
     const NddView& V = H.V;
     double** F = H.F;
+    // start-of-pass copies of phi, nUex, nUer: pass p reads set (p - 1) & 1
+    // and its update writes set p & 1 for the next pass
+    double* const snap[2][3] = {{H.tmp_c, H.tmp_x, H.tmp_r}, {H.tmp2_c, H.tmp2_x, H.tmp2_r}};
+    // the step's old values and the first pass's copies (both flux sets get
+    // the entries no update rewrites), one launch; the ghost kernel then sets
+    // the ghosts of phi and of its copy
     d2d(H, {{H.phi_old, F[F_PHI]}, {H.Ex_old, F[F_EX]}, {H.Er_old, F[F_ER]}, {H.sx_old, F[F_NUEX]},
-            {H.sr_old, F[F_NUER]}});
-    hipLaunchKernelGGL(k_poisson_ghost, dim3(cdiv(std::max(V.ni, V.nj), 256)), dim3(256), 0, H.s, V, F[F_PHI]);
+            {H.sr_old, F[F_NUER]}, {H.tmp_c, F[F_PHI]}, {H.tmp_x, F[F_NUEX]}, {H.tmp_r, F[F_NUER]},
+            {H.tmp2_x, F[F_NUEX]}, {H.tmp2_r, F[F_NUER]}});
+    hipLaunchKernelGGL(k_poisson_ghost, dim3(cdiv(std::max(V.ni, V.nj), 256)), dim3(256), 0, H.s, V, F[F_PHI],
+                       H.tmp_c);
     PoissonArgs A{};
     A.phi = F[F_PHI]; A.rhoe = F[F_RHOE]; A.rhoi = F[F_RHOI]; A.Te = F[F_TE]; A.rhoe_old = F[F_RHOE_OLD];
     A.Ex = F[F_EX]; A.Er = F[F_ER]; A.Ex_old = H.Ex_old; A.rhoUix = F[F_RHOUIX];
     A.S = H.has_src[0] ? F[F_S] : nullptr;
-    A.nUex_t = H.tmp_x; A.nUer_t = H.tmp_r; A.nUex = F[F_NUEX]; A.nUer = F[F_NUER];
+    A.nUex = F[F_NUEX]; A.nUer = F[F_NUER];
     A.m = H.me;
     int ncount = 0;
     for (;;) {
         ncount++;
-        d2d(H, {{H.tmp_c, F[F_PHI]}, {H.tmp_x, F[F_NUEX]}, {H.tmp_r, F[F_NUER]}});
+        double* const* S = snap[(ncount - 1) & 1];
+        double* const* Sn = snap[ncount & 1];
+        A.nUex_t = S[1];
+        A.nUer_t = S[2];
         hipLaunchKernelGGL(k_poisson_assemble, ggrid(V.ni, V.nj), gblk(), 0, H.s, V, A, H.mg.aP(), H.mg.aE(),
                            H.mg.aW(), H.mg.aN(), H.mg.aS(), H.mg.rhs(), H.mg.prep_s(), H.mg.prep_part());
         H.mg.prepared = true;
@@ -1195,18 +1242,17 @@ static void poisson_routine(NddHandle& H, bool report)
         if (!solve(H, ncount > 1 ? H.p.error_cnv_HES_phi : -INFINITY, H.p.maxITR_SOR_phi, H.p.error_cnv_SOR_phi,
                    warm_slot(EQ_POISSON, ncount))) {
             // calcRes of the previous iteration met the PC criterion: undo the prediction
-            d2d(H, {{F[F_NUEX], H.tmp_x}, {F[F_NUER], H.tmp_r}});
+            d2d(H, {{F[F_NUEX], S[1]}, {F[F_NUER], S[2]}});
             break;
         }
         if (learn && (H.cap_feat || !H.p.learned)) learned_store(H);  // learned mode reads its warm slot
-        // the pass's copy tmp_c is phi before the update
         hipLaunchKernelGGL(k_poisson_update, ggrid(V.ni + 2, V.nj + 2), gblk(), 0, H.s, V, H.mg.sol(),
-                           (const double*)H.tmp_c, H.dphi, F[F_PHI], F[F_EX], F[F_ER], (const double*)F[F_RHOE],
-                           F[F_NUEX], F[F_NUER], H.me);
+                           (const double*)S[0], H.dphi, F[F_PHI], F[F_EX], F[F_ER], (const double*)F[F_RHOE],
+                           F[F_NUEX], F[F_NUER], H.me, Sn[0], Sn[1], Sn[2]);
         warm_store(H, warm_slot(EQ_POISSON, ncount));
         CATHODE_LAUNCH_CHECK();
         if (ncount >= H.p.maxITR_HES_phi) break;
-        ResPairs P{F[F_PHI], H.tmp_c, F[F_NUEX], H.tmp_x, F[F_NUER], H.tmp_r};
+        ResPairs P{F[F_PHI], S[0], F[F_NUEX], S[1], F[F_NUER], S[2]};
         calcres3_async(H, P);
     }
     if (report) {
