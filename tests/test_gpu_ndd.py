@@ -103,3 +103,21 @@ def test_config1_full_size(hiplib, mode):
     rep = gpu_steps(hiplib, p, st, 2)
     assert rep.nonconverged == 0 and rr.nonconverged == 0
     compare(st, ref, FIELDS + ["rhoeps", "Te", "Gx", "Gr"])
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("cnv", [1e-21, 1e-9, 1e-5])
+def test_pc_loop_depth_and_early_stop(hiplib, mode, cnv):
+    """Four PC passes per routine with a stopping threshold: the never-met
+    threshold runs every pass (the start-of-pass copy sets alternate three
+    times); the others stop the loop after a pass whose calcRes meets it
+    (the speculative solve aborts and the predicted fluxes are restored)."""
+    p, st = W.config0(mode=mode)
+    p.maxITR_HES_phi = p.maxITR_HES_rhoe = p.maxITR_HES_rhoeps = p.maxITR_HES_rhoi = 4
+    p.error_cnv_HES_phi = p.error_cnv_HES_rhoe = p.error_cnv_HES_rhoeps = p.error_cnv_HES_rhoi = cnv
+    print(f"ndd config0 96x64 pc depth 4 cnv={cnv} mode={mode}", flush=True)
+    ref = W.copy_state(st)
+    O.ndd_step(p, ref, 2, solver=0)
+    rep = gpu_steps(hiplib, p, st, 2)
+    assert rep.nonconverged == 0
+    compare(st, ref, FIELDS)
