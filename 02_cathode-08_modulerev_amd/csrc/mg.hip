@@ -737,10 +737,15 @@ __device__ __forceinline__ void restart_block(const RestartArgs& A, int blk0)
     if (ws.k > 0) {
         double t[WARM_NP];
 #pragma unroll
-        for (int m = 0; m < WARM_NP; m++) {
-            double s = 0.0;
-            for (int q = lin_tid(); q < A.wnb; q += NT) s += A.wpart[m * A.wnb + q];
-            t[m] = s;
+        for (int m = 0; m < WARM_NP; m++) t[m] = 0.0;
+        // all WARM_NP loads of a partial index issued together (each sum
+        // still runs over q = t, t + NT, ... in order)
+        for (int q = lin_tid(); q < A.wnb; q += NT) {
+            double v[WARM_NP];
+#pragma unroll
+            for (int m = 0; m < WARM_NP; m++) v[m] = A.wpart[m * A.wnb + q];
+#pragma unroll
+            for (int m = 0; m < WARM_NP; m++) t[m] += v[m];
         }
         block_sum_vec_bcast<NT, WARM_NP>(t, wl);
         const double res = warm_coeffs(t, ws.k, a);
