@@ -417,9 +417,18 @@ __global__ void __launch_bounds__(SETUP_NT) k_setup_levels(SetupLevels a, double
     __shared__ double lds[SETUP_NT / 64];
     const int tid = threadIdx.x;
     double mx = 0.0, bb = 0.0;
-    for (int k = tid; k < a.nb; k += SETUP_NT) {
-        mx = fmax(mx, a.part[k]);
-        bb += a.part[a.nb + k];
+    // two rounds of partial loads issued before they are combined (each sum
+    // still runs over k = tid, tid + SETUP_NT, ... in order)
+    for (int k = tid; k < a.nb; k += 2 * SETUP_NT) {
+        const bool two = k + SETUP_NT < a.nb;
+        const double m0 = a.part[k], b0 = a.part[a.nb + k];
+        const double m1 = two ? a.part[k + SETUP_NT] : 0.0, b1 = two ? a.part[a.nb + k + SETUP_NT] : 0.0;
+        mx = fmax(mx, m0);
+        bb += b0;
+        if (two) {
+            mx = fmax(mx, m1);
+            bb += b1;
+        }
     }
     mx = block_max_bcast<SETUP_NT>(mx, lds);
     bb = block_sum_bcast<SETUP_NT>(bb, lds);
