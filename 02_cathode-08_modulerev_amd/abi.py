@@ -122,6 +122,9 @@ SIGNATURES = [
     ("cathode_struct5_solve_dev", ctypes.c_int,
      _SOLVE5 + [ctypes.c_int, ctypes.c_double, ctypes.c_int, _D, ctypes.c_void_p,
                 ctypes.POINTER(Status)]),
+    ("cathode_struct5_solve_dev_noalloc", ctypes.c_int,
+     _SOLVE5 + [ctypes.c_int, ctypes.c_double, ctypes.c_int, _D, ctypes.c_void_p,
+                ctypes.POINTER(Status)]),
     ("cathode_sor_solve", ctypes.c_int,
      _SOLVE5 + [ctypes.c_double, ctypes.c_int, ctypes.c_double, _D, ctypes.POINTER(Status)]),
     ("cathode_lu9_solve", ctypes.c_int,

@@ -72,6 +72,44 @@ __global__ void k_scatter_solution(int ni2, int nj2, int ld_src, const double* _
     if (act[(size_t)i * ld_src + j]) x[(size_t)i * ld_dst + j] = xs[(size_t)i * ld_src + j];
 }
 
+// ------------------------------------------------------------------ no-allocation solve
+// cathode_struct5_solve_dev_noalloc runs the multigrid of one solve inside
+// `scratch`, a static device array each GPU holds for the life of the
+// process (sized for grids like the reference's shipped 194 x 70 and up to
+// about 256 x 256 cells; the bytes a shape needs are counted before use).
+// In host code `scratch` names a host-side stand-in, not the device copy:
+// cathode_init looks up each device's address once (g_scratch_dev) and only
+// that address is used.  Every call shares the array and the host block, so
+// calls that overlap on one GPU would race; the library's API mutex and the
+// synchronisation at the end of each call keep one process's calls apart.
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)16 << 20;
+constexpr size_t NOALLOC_HOST_DOUBLES = 16384;
+constexpr int NOALLOC_MAX_BOXES = 64;
+constexpr int MAX_DEVICES = 64;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+static unsigned char* g_scratch_dev[MAX_DEVICES];   // device address of scratch, per device
+static double g_noalloc_host[NOALLOC_HOST_DOUBLES];  // what the solve's kernels leave for the host
+static MGSolver g_noalloc_solver;                    // its buffers are pieces of scratch
+
+struct BoxList {
+    int n;
+    int b[4 * NOALLOC_MAX_BOXES];  // ilo, ihi, jlo, jhi per box
+};
+
+// act[i][j] = 1 on the union of the boxes, 0 elsewhere (ghosts and padding too)
+__global__ void k_mask_from_boxes(BoxList L, int ni2, int ld0, unsigned char* __restrict__ act)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y;
+    if (i >= ni2 || j >= ld0) return;
+    unsigned char on = 0;
+    for (int q = 0; q < L.n; q++) {
+        const int* bx = L.b + 4 * q;
+        if (i >= bx[0] && i <= bx[1] && j >= bx[2] && j <= bx[3]) on = 1;
+    }
+    act[(size_t)i * ld0 + j] = on;
+}
+
 // common body of the host and device struct5 entry points
 static int struct5_impl(const double* aP, const double* aE, const double* aW, const double* aN,
                         const double* aS, const double* b, const int* boxes, int n_bl, int ni2,
@@ -157,6 +195,13 @@ int cathode_init(int device)
         return CATHODE_ERR_ARG;
     }
     CATHODE_HIP(hipSetDevice(device));
+    if (device < MAX_DEVICES && !g_scratch_dev[device]) {
+        // this device's copy of the no-allocation scratch (the name itself is
+        // a host-side stand-in that kernels must never receive)
+        void* d = nullptr;
+        CATHODE_HIP(hipGetSymbolAddress(&d, HIP_SYMBOL(scratch)));
+        g_scratch_dev[device] = (unsigned char*)d;
+    }
     if (!c.stream) CATHODE_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
     c.device = device;
     c.ready = true;
@@ -202,6 +247,77 @@ int cathode_struct5_solve_dev(const double* aP, const double* aE, const double* 
     if (!require_init()) return CATHODE_ERR_NOINIT;
     hipStream_t s = stream ? (hipStream_t)stream : ctx().stream;
     return struct5_impl(aP, aE, aW, aN, aS, b, boxes, n_bl, ni2, nj2, ld, maxITR, tol, mode, x, true, s, st);
+    CATHODE_GUARD_END
+}
+
+int cathode_struct5_solve_dev_noalloc(const double* aP, const double* aE, const double* aW, const double* aN,
+                                      const double* aS, const double* b, const int* boxes, int n_bl, int ni2,
+                                      int nj2, int ld, int maxITR, double tol, int mode, double* x, void* stream,
+                                      cathode_status* st)
+{
+    CATHODE_GUARD_BEGIN
+    if (!require_init()) return CATHODE_ERR_NOINIT;
+    auto t0 = std::chrono::steady_clock::now();
+    hipStream_t s = stream ? (hipStream_t)stream : ctx().stream;
+    if (!aP || !aE || !aW || !aN || !aS || !b || !x || !boxes || ni2 < 3 || nj2 < 3 || ld < nj2 ||
+        (mode != CATHODE_MODE_FP64 && mode != CATHODE_MODE_MIXED)) {
+        set_error("struct5_noalloc: invalid arguments");
+        return CATHODE_ERR_ARG;
+    }
+    if (n_bl < 1 || n_bl > NOALLOC_MAX_BOXES) {
+        set_error("struct5_noalloc: n_bl must be 1.." + std::to_string(NOALLOC_MAX_BOXES));
+        return CATHODE_ERR_ARG;
+    }
+    BoxList bl;
+    bl.n = n_bl;
+    for (int q = 0; q < n_bl; q++) {
+        const int* bx = boxes + 4 * q;
+        if (bx[0] < 1 || bx[1] > ni2 - 2 || bx[2] < 1 || bx[3] > nj2 - 2) {
+            set_error("struct5_noalloc: boxes must lie inside the interior cells 1..ni, 1..nj");
+            return CATHODE_ERR_ARG;
+        }
+        for (int k = 0; k < 4; k++) bl.b[4 * q + k] = bx[k];
+    }
+    const int dev = ctx().device;
+    unsigned char* base = dev < MAX_DEVICES ? g_scratch_dev[dev] : nullptr;
+    if (!base) {
+        set_error("struct5_noalloc: no scratch address for this device");
+        return CATHODE_ERR_HIP;
+    }
+    MGSolver& S = g_noalloc_solver;  // another device's address differs, so its layout is rebuilt
+    const size_t need = S.configure_in(ni2 - 2, nj2 - 2, base, NOALLOC_SCRATCH_BYTES, g_noalloc_host,
+                                       NOALLOC_HOST_DOUBLES, s);
+    if (need > NOALLOC_SCRATCH_BYTES) {
+        set_error("struct5_noalloc: a " + std::to_string(ni2 - 2) + " x " + std::to_string(nj2 - 2) +
+                  " grid needs " + (need == SIZE_MAX ? std::string("a larger host block") :
+                                    std::to_string(need) + " bytes of scratch") +
+                  "; the static scratch holds " + std::to_string(NOALLOC_SCRATCH_BYTES) + " bytes");
+        return CATHODE_ERR_ARG;
+    }
+    const int ld0 = S.ld0();
+    hipLaunchKernelGGL(k_mask_from_boxes, dim3(cdiv(ld0, 64), ni2), dim3(64), 0, s, bl, ni2, ld0, S.act());
+    CATHODE_LAUNCH_CHECK();
+    const double* src[6] = {aP, aE, aW, aN, aS, b};
+    double* dst[6] = {S.aP(), S.aE(), S.aW(), S.aN(), S.aS(), S.rhs()};
+    for (int k = 0; k < 6; k++)
+        CATHODE_HIP(hipMemcpy2DAsync(dst[k], ld0 * sizeof(double), src[k], ld * sizeof(double),
+                                     nj2 * sizeof(double), ni2, hipMemcpyDeviceToDevice, s));
+    S.set_weights(nullptr);
+    S.enable_nine(false);
+    cathode_status local{};
+    S.solve(maxITR, tol, mode, s, &local);
+    hipLaunchKernelGGL(k_scatter_solution, dim3(cdiv(nj2, 64), ni2), dim3(64), 0, s, ni2, nj2, ld0, S.sol(), S.act(),
+                       ld, x);
+    CATHODE_LAUNCH_CHECK();
+    CATHODE_HIP(hipStreamSynchronize(s));
+    local.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    record_status(local);
+    if (st) *st = local;
+    if (!std::isfinite(local.rel_residual)) {
+        set_error("struct5_noalloc: non-finite residual (NaN/Inf in the system)");
+        return CATHODE_ERR_NUMERIC;
+    }
+    return CATHODE_OK;
     CATHODE_GUARD_END
 }
 

@@ -63,29 +63,54 @@ bool require_init();
 void record_status(const cathode_status& s);
 
 // ------------------------------------------------------------------ device buffers
-// Owned device allocation (freed in the destructor).
+// Device allocation, owned (hipMalloc, freed in the destructor) or borrowed
+// from memory someone else holds (never freed here).
 template <class T>
 struct DevBuf {
     T* p = nullptr;
     size_t n = 0;
+    bool owned = true;
     DevBuf() = default;
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
     ~DevBuf() { release(); }
     void release()
     {
-        if (p) (void)hipFree(p);
+        if (p && owned) (void)hipFree(p);
         p = nullptr;
         n = 0;
+        owned = true;
     }
     void alloc(size_t count)
     {
-        if (count <= n && p) return;
+        if (count <= n && p && owned) return;
         release();
         CATHODE_HIP(hipMalloc(&p, count * sizeof(T) + 256));
         n = count;
     }
+    void borrow(T* q, size_t count)
+    {
+        release();
+        p = q;
+        n = count;
+        owned = false;
+    }
     void zero(hipStream_t s) { CATHODE_HIP(hipMemsetAsync(p, 0, n * sizeof(T), s)); }
+};
+
+// A fixed region of device memory handed out in 256-byte aligned pieces, each
+// followed by 256 bytes of slack (as DevBuf::alloc adds); nothing is
+// allocated.  With base == nullptr it only counts the bytes the pieces need.
+struct Arena {
+    unsigned char* base = nullptr;
+    size_t used = 0;
+    template <class T>
+    T* take(size_t count)
+    {
+        const size_t off = (used + 255) & ~(size_t)255;
+        used = off + count * sizeof(T) + 256;
+        return base ? reinterpret_cast<T*>(base + off) : nullptr;
+    }
 };
 
 inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }

@@ -1072,7 +1072,7 @@ MGSolver::~MGSolver()
 {
     drop_graphs();
     for (auto& e : prof.ev) (void)hipEventDestroy(e);
-    if (h_scal) (void)hipHostFree(h_scal);
+    if (h_scal && h_owned) (void)hipHostFree(h_scal);
 }
 
 static int pad_ld(int nj) { return ((nj + 2 + 7) / 8) * 8; }
@@ -1080,7 +1080,78 @@ static int pad_ld(int nj) { return ((nj + 2 + 7) / 8) * 8; }
 void MGSolver::configure(int ni, int nj, hipStream_t s)
 {
     (void)s;
-    if (ni == cfg_ni && nj == cfg_nj) return;
+    if (ni == cfg_ni && nj == cfg_nj && !arena) return;
+    arena = nullptr;
+    arena_base = nullptr;
+    build(ni, nj);
+    const size_t nsc = scal.n;
+    if (h_scal_n < nsc || !h_owned) {
+        if (h_scal && h_owned) CATHODE_HIP(hipHostFree(h_scal));
+        h_scal = nullptr;
+        CATHODE_HIP(hipHostMalloc(&h_scal, nsc * sizeof(double), hipHostMallocDefault));
+        h_scal_n = nsc;
+        h_owned = true;
+    }
+    // plain fills on the null stream, which the library's non-blocking
+    // streams do not wait for
+    zero_buffers(nullptr);
+    CATHODE_HIP(hipDeviceSynchronize());
+}
+
+size_t MGSolver::configure_in(int ni, int nj, unsigned char* base, size_t cap, double* hbuf, size_t hn,
+                              hipStream_t s)
+{
+    size_t need, nsc;
+    {
+        // the bytes of this shape, counted by a solver that touches no memory
+        MGSolver probe;
+        Arena count;
+        probe.arena = &count;
+        probe.build(ni, nj);
+        need = count.used;
+        nsc = probe.scal.n;
+    }
+    if (need > cap) return need;
+    if (nsc > hn) return SIZE_MAX;
+    if (arena && arena_base == base && ni == cfg_ni && nj == cfg_nj) return need;  // unchanged
+    if (h_scal && h_owned) CATHODE_HIP(hipHostFree(h_scal));
+    region = Arena{base, 0};
+    arena = &region;
+    arena_base = base;
+    h_scal = hbuf;
+    h_scal_n = hn;
+    h_owned = false;
+    autotune_on = false;
+    build(ni, nj);
+    zero_buffers(s);
+    return need;
+}
+
+void MGSolver::pull_host(size_t n, hipStream_t s)
+{
+    if (!arena) return;
+    CATHODE_HIP(hipMemcpyAsync(h_scal, dmirror.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
+    CATHODE_HIP(hipStreamSynchronize(s));
+}
+
+void MGSolver::zero_buffers(hipStream_t s)
+{
+    const size_t tot = level_off[nl], n0 = level_off[1] - level_off[0];
+    CATHODE_HIP(hipMemsetAsync(store64.p, 0, 8 * tot * sizeof(double), s));
+    CATHODE_HIP(hipMemsetAsync(store32.p, 0, 8 * tot * sizeof(float), s));
+    CATHODE_HIP(hipMemsetAsync(mass.p, 0, tot * sizeof(double), s));
+    CATHODE_HIP(hipMemsetAsync(actall.p, 0, tot, s));
+    CATHODE_HIP(hipMemsetAsync(act0.p, 0, n0, s));
+    CATHODE_HIP(hipMemsetAsync(xalt.p, 0, n0 * sizeof(double), s));
+    CATHODE_HIP(hipMemsetAsync(bsol.p, 0, n0 * sizeof(double), s));
+    CATHODE_HIP(hipMemsetAsync(xsol.p, 0, n0 * sizeof(double), s));
+    CATHODE_HIP(hipMemsetAsync(scal.p, 0, scal.n * sizeof(double), s));
+    CATHODE_HIP(hipMemsetAsync(kv.p, 0, 8 * n0 * sizeof(double), s));
+    if (arena) CATHODE_HIP(hipMemsetAsync(dmirror.p, 0, dmirror.n * sizeof(double), s));
+}
+
+void MGSolver::build(int ni, int nj)
+{
     drop_graphs();  // they hold the old buffers
     cfg_ni = ni;
     cfg_nj = nj;
@@ -1108,14 +1179,10 @@ void MGSolver::configure(int ni, int nj, hipStream_t s)
         tot += (size_t)(dims[l][0] + 2) * pad_ld(dims[l][1]);
     }
     level_off[nl] = tot;
-    store64.alloc(8 * tot);
-    store32.alloc(8 * tot);
-    mass.alloc(tot);
-    actall.alloc(tot);
-    CATHODE_HIP(hipMemset(store64.p, 0, 8 * tot * sizeof(double)));
-    CATHODE_HIP(hipMemset(store32.p, 0, 8 * tot * sizeof(float)));
-    CATHODE_HIP(hipMemset(mass.p, 0, tot * sizeof(double)));
-    CATHODE_HIP(hipMemset(actall.p, 0, tot));
+    grab(store64, 8 * tot);
+    grab(store32, 8 * tot);
+    grab(mass, tot);
+    grab(actall, tot);
     for (int l = 0; l < nl; l++) {
         const size_t o = level_off[l], sz = level_off[l + 1] - level_off[l];
         Level<double>& A = L64[l];
@@ -1152,15 +1219,11 @@ void MGSolver::configure(int ni, int nj, hipStream_t s)
         L32[l].aS = store32.p + o + 4 * sz;
     }
     const size_t n0 = level_off[1] - level_off[0];
-    act0.alloc(n0);
-    CATHODE_HIP(hipMemset(act0.p, 0, n0));
-    bsol.alloc(n0);
-    xsol.alloc(n0);
-    xalt.alloc(n0);
-    CATHODE_HIP(hipMemset(xalt.p, 0, n0 * sizeof(double)));
-    CATHODE_HIP(hipMemset(bsol.p, 0, n0 * sizeof(double)));
-    CATHODE_HIP(hipMemset(xsol.p, 0, n0 * sizeof(double)));
-    w0.alloc(nj + 2);
+    grab(act0, n0);
+    grab(bsol, n0);
+    grab(xsol, n0);
+    grab(xalt, n0);
+    grab(w0, nj + 2);
     // pairs of fine levels in one launch each way (CATHODE_PAIR=0 turns this off)
     {
         const char* e = std::getenv("CATHODE_PAIR");
@@ -1178,25 +1241,15 @@ void MGSolver::configure(int ni, int nj, hipStream_t s)
     {
         int tiles_max = 0;
         for (int cfg = 0; cfg < VT_NCFG; cfg++) tiles_max = std::max(tiles_max, vtile_count(cfg, ni, nj));
-        part.alloc(std::max(3 * (cdiv(nj, BX) * cdiv(ni, BY)), 2 * tiles_max) + 64);
+        grab(part, std::max(3 * (cdiv(nj, BX) * cdiv(ni, BY)), 2 * tiles_max) + 64);
     }
     // the scalar slots, then three partial vectors of the 2-D block grid
     const size_t nsc = SC_COUNT + 3 * (size_t)(cdiv(nj, BX) * cdiv(ni, BY)) + 64;
-    scal.alloc(nsc);
-    CATHODE_HIP(hipMemset(scal.p, 0, nsc * sizeof(double)));
-    kv.alloc(9 * n0);
-    CATHODE_HIP(hipMemset(kv.p, 0, 8 * n0 * sizeof(double)));
-    cinv64.alloc(MG_COARSEST_MAX * MG_COARSEST_MAX);
-    cinv32.alloc(MG_COARSEST_MAX * MG_COARSEST_MAX);
-    if (h_scal_n < nsc) {
-        if (h_scal) CATHODE_HIP(hipHostFree(h_scal));
-        h_scal = nullptr;
-        CATHODE_HIP(hipHostMalloc(&h_scal, nsc * sizeof(double), hipHostMallocDefault));
-        h_scal_n = nsc;
-    }
-    // the plain hipMemset calls above run on the null stream, which the
-    // library's non-blocking streams do not wait for
-    CATHODE_HIP(hipDeviceSynchronize());
+    grab(scal, nsc);
+    grab(kv, 9 * n0);
+    grab(cinv64, MG_COARSEST_MAX * MG_COARSEST_MAX);
+    grab(cinv32, MG_COARSEST_MAX * MG_COARSEST_MAX);
+    if (arena) grab(dmirror, nsc);
 }
 
 CoarsenArgs MGSolver::coarsen_args(int l, const double* w)
@@ -1419,7 +1472,8 @@ void MGSolver::prof_events(hipEvent_t& a, hipEvent_t& b)
 
 double MGSolver::read_scalar(const double* d, hipStream_t s)
 {
-    fetch_host(h_scal, d, 1, s);
+    if (arena) CATHODE_HIP(hipMemcpyAsync(h_scal, d, sizeof(double), hipMemcpyDeviceToHost, s));
+    else fetch_host(h_scal, d, 1, s);
     CATHODE_HIP(hipStreamSynchronize(s));
     return h_scal[0];
 }
@@ -1715,7 +1769,7 @@ int MGSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_sta
         else vcycle<double>(L64, s, fd, fu);
         hipLaunchKernelGGL(k_bicg_x, g, blk, 0, s, F.ni, F.nj, F.ld, F.aP, F.aE, F.aW, F.aN, F.aS, n9, n0, bsol.p,
                            act0.p, xa_, xb_, r, r0, ph, sh, sv, t, scal.p, part3, nb, (const double*)part.p, ntiles,
-                           h_scal);
+                           host_dst());
         CATHODE_LAUNCH_CHECK();
     };
     // V-cycle output on level 0: the finest up kernel's xu (x when the
@@ -1729,7 +1783,7 @@ int MGSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_sta
     expected_its = 0;
     bool checked = false;
     const char* genv = std::getenv("CATHODE_GRAPHS");
-    const bool graphs_on = fused && !prof.enabled && genv && genv[0] == '1';
+    const bool graphs_on = fused && !prof.enabled && !arena && genv && genv[0] == '1';
     while (it < maxITR) {
         if (restart) {
             RestartArgs ra{};
@@ -1799,7 +1853,8 @@ int MGSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_sta
         hipLaunchKernelGGL(k_bicg_finish, dim3(1), dim3(256), 0, s, part.p, nb, scal.p, 3, 0.0);
         // x, recursive and true residuals
         hipLaunchKernelGGL(k_bicg_x, g, blk, 0, s, F.ni, F.nj, F.ld, F.aP, F.aE, F.aW, F.aN, F.aS, n9, n0, bsol.p,
-                           act0.p, xa, xb, r, r0, ph, sh, sv, t, scal.p, part3, nb, (const double*)nullptr, 0, h_scal);
+                           act0.p, xa, xb, r, r0, ph, sh, sv, t, scal.p, part3, nb, (const double*)nullptr, 0,
+                           host_dst());
         hipLaunchKernelGGL(k_bicg_finish, dim3(1), dim3(256), 0, s, part3, nb, scal.p, 4, 0.0);
         }
         CATHODE_LAUNCH_CHECK();
@@ -1811,6 +1866,7 @@ int MGSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_sta
         // and |b|^2 and the side slot in h_scal; |r|^2 and |b - A x|^2 are
         // summed here in a fixed order
         CATHODE_HIP(hipStreamSynchronize(s));
+        pull_host(SC_COUNT + 3 * (size_t)nb, s);
         {
             double rr = 0.0, rt2 = 0.0;
             const double* hp = h_scal + SC_COUNT;
@@ -1845,7 +1901,8 @@ int MGSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_sta
     if (maxITR <= 0) {
         // no iteration: x = 0, relative residual 1 (0 when b = 0)
         if (zero_guess || ws.k > 0) CATHODE_HIP(hipMemsetAsync(xsol.p, 0, n0 * sizeof(double), s));
-        fetch_host(h_scal, scal.p, SC_COUNT, s);
+        if (arena) CATHODE_HIP(hipMemcpyAsync(h_scal, scal.p, SC_COUNT * sizeof(double), hipMemcpyDeviceToHost, s));
+        else fetch_host(h_scal, scal.p, SC_COUNT, s);
         CATHODE_HIP(hipStreamSynchronize(s));
         rel = h_scal[SC_BB] > 0.0 ? 1.0 : 0.0;
     }

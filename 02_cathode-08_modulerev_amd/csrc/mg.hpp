@@ -94,6 +94,16 @@ public:
     // Shapes the hierarchy for ni x nj interior cells (no-op if unchanged).
     void configure(int ni, int nj, hipStream_t s);
 
+    // The same inside caller-held device memory [base, base + cap), with
+    // no allocation: every device buffer is a piece of that region and the
+    // scalars the host reads travel through the host block hbuf of hn
+    // doubles (a device mirror in the region is copied into it after each
+    // synchronisation).  Buffers are zeroed with asynchronous fills on s.
+    // Returns the bytes the shape needs; when that exceeds cap (or the host
+    // block is too small: returns SIZE_MAX) nothing is configured.  The
+    // graph replay and the tile autotuner (which allocate events) are off.
+    size_t configure_in(int ni, int nj, unsigned char* base, size_t cap, double* hbuf, size_t hn, hipStream_t s);
+
     // Level-0 operator, right-hand side, activity mask and solution, in the
     // solver's (ni+2) x ld0 layout.  Callers fill aP..aS, rhs and act before
     // solve(); inactive cells must have act = 0.
@@ -248,8 +258,25 @@ private:
     DevBuf<float> cinv32;      // the same for the FP32 levels (divided by the FP32 scale)
     DevBuf<double> kv;         // BiCGStab vectors r, r0, p, v, s, t, ph, sh, p (second buffer)
     DevBuf<double> scal;       // device scalars: [0] |b|^2, [1] |r|^2, [2] scale, [3] f32 matrix scale
-    double* h_scal = nullptr;  // pinned mirror of scal
+    double* h_scal = nullptr;  // pinned mirror of scal (arena mode: the caller's host block)
     size_t h_scal_n = 0;
+    bool h_owned = true;       // h_scal came from hipHostMalloc
+    // arena mode (configure_in): buffers are pieces of the caller's region
+    Arena* arena = nullptr;    // &region in arena mode; a counting arena while sizing a shape
+    Arena region;
+    unsigned char* arena_base = nullptr;
+    DevBuf<double> dmirror;    // arena mode: what kernels store for the host, copied into h_scal
+    template <class T>
+    void grab(DevBuf<T>& b, size_t count)
+    {
+        if (arena) b.borrow(arena->take<T>(count), count);
+        else b.alloc(count);
+    }
+    void build(int ni, int nj);      // shapes, level pointers and device buffers (no fills)
+    void zero_buffers(hipStream_t s);
+    // where kernels store values for the host, and bringing them there
+    double* host_dst() { return arena ? dmirror.p : h_scal; }
+    void pull_host(size_t n, hipStream_t s);
 };
 
 // FP64 residual-norm helper used by tests of the library itself

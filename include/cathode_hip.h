@@ -91,6 +91,26 @@ int cathode_struct5_solve_dev(const double* aP, const double* aE, const double* 
                               int maxITR, double tol, int mode,
                               double* x, void* stream, cathode_status* st);
 
+/* No counterpart in the reference: cathode_struct5_solve_dev for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments, semantics and result;
+ * it calls no hipMalloc / hipFree / hipHostMalloc.  The multigrid levels,
+ * Krylov vectors and reduction slots live in a static 16 MiB device array
+ * that each GPU keeps until the process ends (its address is taken once per
+ * device by cathode_init).  Limits: at most 64 boxes; grids whose hierarchy
+ * needs more than 16 MiB (about 68 000 padded cells, e.g. 256 x 256
+ * interior cells) are rejected with CATHODE_ERR_ARG and a message giving
+ * the bytes needed -- the reference's 194 x 70 grid needs 3.3 MiB.  The
+ * tile shapes are the defaults (no autotuning, which creates events).
+ * Every call on one GPU shares the array and keeps the last call's
+ * contents; calls that overlap on one GPU race (the library serialises its
+ * entry points and each call synchronises its stream before returning). */
+int cathode_struct5_solve_dev_noalloc(const double* aP, const double* aE, const double* aW,
+                                      const double* aN, const double* aS, const double* b,
+                                      const int* boxes, int n_bl, int ni2, int nj2, int ld,
+                                      int maxITR, double tol, int mode,
+                                      double* x, void* stream, cathode_status* st);
+
 /* Replaces solver_SOR (include/solverFuncs.hpp:169-216): lexicographic
  * point SOR (blocks in order, i then j), x_new = (1-w) x + w (sum a_nb x_nb
  * + b)/(aP + 1e-100); stops after a sweep whose max relative change
