@@ -17,12 +17,23 @@
 namespace cathode {
 
 static std::string g_err;
+static std::mutex g_err_mu;  // batch worker threads report errors concurrently
 static cathode_status g_last{};
 static std::mutex g_mu;
 std::mutex& api_mutex() { return g_mu; }
 
-void set_error(const std::string& m) { g_err = m; }
-const char* get_error() { return g_err.c_str(); }
+void set_error(const std::string& m)
+{
+    std::lock_guard<std::mutex> lk(g_err_mu);
+    g_err = m;
+}
+const char* get_error()
+{
+    static thread_local std::string copy;
+    std::lock_guard<std::mutex> lk(g_err_mu);
+    copy = g_err;
+    return copy.c_str();
+}
 
 Context& ctx()
 {

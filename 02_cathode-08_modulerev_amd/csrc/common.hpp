@@ -7,6 +7,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <exception>
 #include <mutex>
 #include <new>
 #include <string>
@@ -17,6 +18,8 @@
 namespace cathode {
 
 // ------------------------------------------------------------------ errors
+// The last error message, shared by every thread (set under a lock; the
+// reader gets a copy that stays valid until its own next call).
 void set_error(const std::string& msg);
 const char* get_error();
 
@@ -42,14 +45,23 @@ std::mutex& api_mutex();
 #define CATHODE_GUARD_BEGIN                                       \
     std::lock_guard<std::mutex> lock_(::cathode::api_mutex());    \
     try {
-#define CATHODE_GUARD_END                                \
-    }                                                    \
-    catch (const ::cathode::hip_failure&) {              \
-        return CATHODE_ERR_HIP;                          \
-    }                                                    \
-    catch (const std::bad_alloc&) {                      \
-        ::cathode::set_error("host allocation failed"); \
-        return CATHODE_ERR_NOMEM;                        \
+#define CATHODE_GUARD_END CATHODE_CATCH_END
+
+// The catch clauses alone, for entry points that do not take the library
+// mutex (the NDD handles, which independent threads step side by side): no
+// exception crosses the C boundary.
+#define CATHODE_CATCH_END                                   \
+    }                                                       \
+    catch (const ::cathode::hip_failure&) {                 \
+        return CATHODE_ERR_HIP;                             \
+    }                                                       \
+    catch (const std::bad_alloc&) {                         \
+        ::cathode::set_error("host allocation failed");    \
+        return CATHODE_ERR_NOMEM;                           \
+    }                                                       \
+    catch (const std::exception& ex_) {                     \
+        ::cathode::set_error(std::string("internal error: ") + ex_.what()); \
+        return CATHODE_ERR_HIP;                             \
     }
 
 // ------------------------------------------------------------------ context
@@ -114,6 +126,15 @@ struct Arena {
 };
 
 inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
+
+// BiCGStab direction weight beta = (rho1 / rho) (al / om).  After a
+// breakdown (om = 0: no step was taken, or rho = 0) it is not finite; the
+// direction then restarts from the residual (beta = 0, p = r).
+__host__ __device__ inline double bicg_beta(double rho1, double rho, double al, double om)
+{
+    const double b = (rho1 / rho) * (al / om);
+    return isfinite(b) ? b : 0.0;
+}
 
 // ------------------------------------------------------------------ reductions
 // Block-level sum of a double held by every thread of a 256-thread block;

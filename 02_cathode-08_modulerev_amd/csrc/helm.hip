@@ -307,7 +307,8 @@ __global__ void hk_p(HOp A, const cxd* __restrict__ r, const cxd* __restrict__ v
         if (first) {
             pk = r[k];
         } else {
-            const cxd beta = (ldc(sc, HS_RHO1) / ldc(sc, HS_RHO)) * (ldc(sc, HS_AL) / ldc(sc, HS_OM));
+            cxd beta = (ldc(sc, HS_RHO1) / ldc(sc, HS_RHO)) * (ldc(sc, HS_AL) / ldc(sc, HS_OM));
+            if (!(isfinite(beta.re) && isfinite(beta.im))) beta = cxd(0.0);  // breakdown: restart from r
             pk = r[k] + beta * (p[k] - ldc(sc, HS_OM) * v[k]);
         }
     }

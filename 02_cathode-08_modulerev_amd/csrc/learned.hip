@@ -307,7 +307,6 @@ struct CnnNet {
     DevBuf<double> dpart, scal;
     int ni = -1, nj = -1;
     int adam_t = 0;
-    bool packed = false;
 };
 
 static size_t frag_elems(int cin, int cout) { return (size_t)9 * (cin / 16) * (cout / 16) * 64 * 4; }
@@ -322,7 +321,6 @@ static void cnn_pack(CnnNet& N, hipStream_t s)
                                1, N.wb[l].p);
     }
     CATHODE_LAUNCH_CHECK();
-    N.packed = true;
 }
 
 static void cnn_shape(CnnNet& N, int ni, int nj)
@@ -351,21 +349,30 @@ static void cnn_shape(CnnNet& N, int ni, int nj)
 
 static dim3 conv_grid(int ni, int nj) { return dim3(cdiv(nj, 16), cdiv(ni, 4)); }
 
-static void cnn_forward(CnnNet& N, int ni, int nj, const double* feat, hipStream_t s)
+// forward pass into the activation planes act[0..CNN_L-1] and the output y
+// (zero ghosts); the weight fragments are packed whenever the weights change
+static void cnn_forward(const CnnNet& N, int ni, int nj, const double* feat, bf16_t* const* act, float* y,
+                        hipStream_t s)
 {
-    if (!N.packed) cnn_pack(N, s);
     const dim3 gp(cdiv(nj + 2, 256), ni + 2), gc = conv_grid(ni, nj), blk(64, 4);
-    hipLaunchKernelGGL(k_cnn_features, gp, dim3(256), 0, s, ni, nj, feat, N.act[0].p);
+    hipLaunchKernelGGL(k_cnn_features, gp, dim3(256), 0, s, ni, nj, feat, act[0]);
     auto W = [&](int l) { return reinterpret_cast<const bf16x4*>(N.wf[l].p); };
     auto B = [&](int l) { return (const float*)(N.w.p + g_lay.b_off[l]); };
-    hipLaunchKernelGGL((k_cnn_conv<16, 32, 0>), gc, blk, 0, s, ni, nj, N.act[0].p, W(0), B(0), N.act[1].p,
-                       (float*)nullptr, (const bf16_t*)nullptr);
+    hipLaunchKernelGGL((k_cnn_conv<16, 32, 0>), gc, blk, 0, s, ni, nj, act[0], W(0), B(0), act[1], (float*)nullptr,
+                       (const bf16_t*)nullptr);
     for (int l = 1; l < CNN_L - 1; l++)
-        hipLaunchKernelGGL((k_cnn_conv<32, 32, 0>), gc, blk, 0, s, ni, nj, N.act[l].p, W(l), B(l), N.act[l + 1].p,
+        hipLaunchKernelGGL((k_cnn_conv<32, 32, 0>), gc, blk, 0, s, ni, nj, act[l], W(l), B(l), act[l + 1],
                            (float*)nullptr, (const bf16_t*)nullptr);
-    hipLaunchKernelGGL((k_cnn_conv<32, 16, 1>), gc, blk, 0, s, ni, nj, N.act[CNN_L - 1].p, W(CNN_L - 1),
-                       B(CNN_L - 1), (bf16_t*)nullptr, N.y.p, (const bf16_t*)nullptr);
+    hipLaunchKernelGGL((k_cnn_conv<32, 16, 1>), gc, blk, 0, s, ni, nj, act[CNN_L - 1], W(CNN_L - 1), B(CNN_L - 1),
+                       (bf16_t*)nullptr, y, (const bf16_t*)nullptr);
     CATHODE_LAUNCH_CHECK();
+}
+
+static void cnn_forward(CnnNet& N, int ni, int nj, const double* feat, hipStream_t s)
+{
+    bf16_t* act[CNN_L];
+    for (int l = 0; l < CNN_L; l++) act[l] = N.act[l].p;
+    cnn_forward(N, ni, nj, feat, act, N.y.p, s);
 }
 
 template <int CIN, int COUT>
@@ -416,13 +423,35 @@ using namespace cathode;
 // ------------------------------------------------------------------ internal API for the NDD step
 namespace cathode {
 
-void cnn_infer(void* net, int ni, int nj, const double* feat, double* out, hipStream_t s)
+struct CnnScratch {
+    DevBuf<bf16_t> act[CNN_L];
+    DevBuf<float> y;
+    int ni = -1, nj = -1;
+};
+
+CnnScratch* cnn_scratch_new() { return new CnnScratch(); }
+void cnn_scratch_free(CnnScratch* sc) { delete sc; }
+
+void cnn_infer(void* net, int ni, int nj, const double* feat, double* out, CnnScratch* sc, hipStream_t s)
 {
-    CnnNet& N = *(CnnNet*)net;
-    cnn_shape(N, ni, nj);
-    cnn_forward(N, ni, nj, feat, s);
+    const CnnNet& N = *(const CnnNet*)net;
+    if (sc->ni != ni || sc->nj != nj) {
+        // zero ghosts and padding, ordered on the caller's stream
+        const size_t px = (size_t)(ni + 2) * (nj + 2);
+        for (int l = 0; l < CNN_L; l++) {
+            sc->act[l].alloc(px * CNN_CIN[l]);
+            CATHODE_HIP(hipMemsetAsync(sc->act[l].p, 0, px * CNN_CIN[l] * sizeof(bf16_t), s));
+        }
+        sc->y.alloc(px);
+        CATHODE_HIP(hipMemsetAsync(sc->y.p, 0, px * sizeof(float), s));
+        sc->ni = ni;
+        sc->nj = nj;
+    }
+    bf16_t* act[CNN_L];
+    for (int l = 0; l < CNN_L; l++) act[l] = sc->act[l].p;
+    cnn_forward(N, ni, nj, feat, act, sc->y.p, s);
     hipLaunchKernelGGL(k_cnn_out_plane, dim3(cdiv(nj + 2, 256), ni + 2), dim3(256), 0, s, ni, nj,
-                       (const float*)N.y.p, out);
+                       (const float*)sc->y.p, out);
     CATHODE_LAUNCH_CHECK();
 }
 
@@ -468,6 +497,8 @@ int cathode_cnn_create(int seed, void** net)
         N->wb[l].alloc(frag_elems(CNN_COUT[l], CNN_CIN[l]));
     }
     N->scal.alloc(4);
+    cnn_pack(*N, ctx().stream);
+    CATHODE_HIP(hipStreamSynchronize(ctx().stream));
     *net = N.release();
     return CATHODE_OK;
     CATHODE_GUARD_END
@@ -500,7 +531,10 @@ int cathode_cnn_set_params(void* net, const float* host)
     }
     CnnNet& N = *(CnnNet*)net;
     CATHODE_HIP(hipMemcpy(N.w.p, host, g_lay.total * sizeof(float), hipMemcpyHostToDevice));
-    N.packed = false;
+    // the fragments the passes read are packed here, so inference (which may
+    // run for several handles at once) only reads the network
+    cnn_pack(N, ctx().stream);
+    CATHODE_HIP(hipStreamSynchronize(ctx().stream));
     return CATHODE_OK;
     CATHODE_GUARD_END
 }
@@ -514,7 +548,12 @@ int cathode_cnn_forward_dev(void* net, int ni, int nj, const double* feat, doubl
         return CATHODE_ERR_ARG;
     }
     hipStream_t s = ctx().stream;
-    cnn_infer(net, ni, nj, feat, out, s);
+    CnnNet& N = *(CnnNet*)net;
+    cnn_shape(N, ni, nj);
+    cnn_forward(N, ni, nj, feat, s);
+    hipLaunchKernelGGL(k_cnn_out_plane, dim3(cdiv(nj + 2, 256), ni + 2), dim3(256), 0, s, ni, nj,
+                       (const float*)N.y.p, out);
+    CATHODE_LAUNCH_CHECK();
     CATHODE_HIP(hipStreamSynchronize(s));
     return CATHODE_OK;
     CATHODE_GUARD_END

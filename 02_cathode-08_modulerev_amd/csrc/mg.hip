@@ -848,7 +848,7 @@ __global__ void k_bicg_p(int ni, int nj, int ld, const unsigned char* __restrict
         if (first) {
             pk = r[k];
         } else {
-            const double beta = (sc[SC_RHO1] / sc[SC_RHO]) * (sc[SC_AL] / sc[SC_OM]);
+            const double beta = bicg_beta(sc[SC_RHO1], sc[SC_RHO], sc[SC_AL], sc[SC_OM]);
             pk = r[k] + beta * (p[k] - sc[SC_OM] * v[k]);
         }
     }

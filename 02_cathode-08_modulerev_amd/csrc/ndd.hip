@@ -962,6 +962,7 @@ struct NddHandle {
     int nb = 0;
     // learned initial guess of the first Poisson correction solve of a step
     void* cnn = nullptr;              // network (cathode_ndd_set_learned), not owned
+    std::unique_ptr<CnnScratch, void (*)(CnnScratch*)> cnn_sc{nullptr, cnn_scratch_free};  // this handle's activations
     DevBuf<double> lfeat, lguess, xprev, lpart, lscal;
     bool have_xprev = false;
     // learned mode: every solve starts from its own previous-step solution
@@ -1133,7 +1134,8 @@ static void learned_guess(NddHandle& H, bool net)
         warm_guess(H, slot);  // learned mode without the network this step: the warm start alone
         return;
     }
-    cnn_infer(H.cnn, V.ni, V.nj, H.lfeat.p, H.lguess.p, H.s);
+    if (!H.cnn_sc) H.cnn_sc.reset(cnn_scratch_new());
+    cnn_infer(H.cnn, V.ni, V.nj, H.lfeat.p, H.lguess.p, H.cnn_sc.get(), H.s);
     // the network's output joins the solve's history in the least-squares
     // combination; the device reports what it adds (h_scal[8])
     warm_guess(H, slot, H.lguess.p);
@@ -1451,12 +1453,7 @@ int cathode_ndd_create(const cathode_ndd_params* p, void** handle)
         CATHODE_HIP(hipStreamSynchronize(H->s));
         *handle = H.release();
         return CATHODE_OK;
-    } catch (const hip_failure&) {
-        return CATHODE_ERR_HIP;
-    } catch (const std::bad_alloc&) {
-        set_error("ndd_create: host allocation failed");
-        return CATHODE_ERR_NOMEM;
-    }
+    CATHODE_CATCH_END
 }
 
 int cathode_ndd_destroy(void* handle)
@@ -1525,9 +1522,7 @@ int cathode_ndd_set_state(void* handle, const cathode_ndd_fields* host)
         }
         CATHODE_HIP(hipStreamSynchronize(H->s));
         return CATHODE_OK;
-    } catch (const hip_failure&) {
-        return CATHODE_ERR_HIP;
-    }
+    CATHODE_CATCH_END
 }
 
 int cathode_ndd_get_state(void* handle, cathode_ndd_fields* host)
@@ -1545,9 +1540,7 @@ int cathode_ndd_get_state(void* handle, cathode_ndd_fields* host)
         }
         CATHODE_HIP(hipStreamSynchronize(H->s));
         return CATHODE_OK;
-    } catch (const hip_failure&) {
-        return CATHODE_ERR_HIP;
-    }
+    CATHODE_CATCH_END
 }
 
 // nsteps implicit steps of one instance on its own stream; fills rep (may be NULL)
@@ -1580,6 +1573,14 @@ static int ndd_run(NddHandle* H, int nsteps, cathode_ndd_report* rep)
     }
 }
 
+// ndd_run with every exception turned into an error code (worker threads)
+static int ndd_run_guarded(NddHandle* H, int nsteps, cathode_ndd_report* rep)
+{
+    try {
+        return ndd_run(H, nsteps, rep);
+    CATHODE_CATCH_END
+}
+
 int cathode_ndd_step(void* handle, int nsteps, cathode_ndd_report* rep)
 {
     try {
@@ -1589,9 +1590,7 @@ int cathode_ndd_step(void* handle, int nsteps, cathode_ndd_report* rep)
             return CATHODE_ERR_ARG;
         }
         return ndd_run(H, nsteps, rep);
-    } catch (const hip_failure&) {
-        return CATHODE_ERR_HIP;
-    }
+    CATHODE_CATCH_END
 }
 
 int cathode_ndd_batch_step(void** handles, int n, int nsteps, int workers, cathode_ndd_report* reps)
@@ -1632,11 +1631,7 @@ int cathode_ndd_batch_step(void** handles, int n, int nsteps, int workers, catho
             const long key = ((long)H->p.ni << 32) ^ ((long)H->p.nj << 2) ^ H->p.mode;
             if (std::find(seen.begin(), seen.end(), key) != seen.end()) continue;
             seen.push_back(key);
-            try {
-                rc[k] = ndd_run(H, 1, &first[k]);
-            } catch (const hip_failure&) {
-                rc[k] = CATHODE_ERR_HIP;
-            }
+            rc[k] = ndd_run_guarded(H, 1, &first[k]);
             done[k] = 1;
             if (rc[k] != CATHODE_OK) return rc[k];
         }
@@ -1646,13 +1641,13 @@ int cathode_ndd_batch_step(void** handles, int n, int nsteps, int workers, catho
         pool.emplace_back([&, w] {
             for (int k : queue[w]) {
                 NddHandle* H = (NddHandle*)handles[k];
-                try {
+                {
                     cathode_ndd_report r{};
                     if (nsteps - done[k] == 0) {
                         if (reps) reps[k] = first[k];
                         continue;
                     }
-                    rc[k] = ndd_run(H, nsteps - done[k], &r);
+                    rc[k] = ndd_run_guarded(H, nsteps - done[k], &r);
                     if (reps) {
                         if (done[k]) {  // merge the counters of the first step
                             r.steps += first[k].steps;
@@ -1665,8 +1660,6 @@ int cathode_ndd_batch_step(void** handles, int n, int nsteps, int workers, catho
                         }
                         reps[k] = r;
                     }
-                } catch (const hip_failure&) {
-                    rc[k] = CATHODE_ERR_HIP;
                 }
             }
         });

@@ -147,13 +147,13 @@ __global__ void __launch_bounds__(NT)
                         fu.sc[fu.s_al] = f_al;
                     }
                 } else {
-                    if (!fu.first) f_beta = (a / fu.sc[fu.s_rho]) * (fu.sc[fu.s_al] / fu.sc[fu.s_om]);
+                    if (!fu.first) f_beta = bicg_beta(a, fu.sc[fu.s_rho], fu.sc[fu.s_al], fu.sc[fu.s_om]);
                     if (lead) fu.sc[fu.s_rho1] = a;
                 }
             } else if (fu.pro == 2) {
                 f_al = fu.sc[fu.s_al];
             } else if (!fu.first) {
-                f_beta = (fu.sc[fu.s_rho1] / fu.sc[fu.s_rho]) * (fu.sc[fu.s_al] / fu.sc[fu.s_om]);
+                f_beta = bicg_beta(fu.sc[fu.s_rho1], fu.sc[fu.s_rho], fu.sc[fu.s_al], fu.sc[fu.s_om]);
             }
             f_om = fu.sc[fu.s_om];
             f_in = fu.mixed ? fu.sc[fu.s_s32] * fu.sc[fu.s_scale] : 1.0;

@@ -1,12 +1,13 @@
 // host_check.cpp -- checks of the C++ host headers (02_cathode-08_modulerev_amd/host).
 //
-//   host_check cpu <tmpdir>   CmpMat2 / S<->ABCD, setup.yaml reader, restart CSV
+//   host_check cpu <tmpdir> <setup.yaml>   CmpMat2 / S<->ABCD, setup.yaml reader, restart CSV
 //   host_check gpu            solver_SMG / solver_SOR / solver_LU_9p through the
 //                             drop-in header against the oracle (liboracle.so)
 // Prints one line per check and exits non-zero on the first failure.
 #include <cmath>
 #include <complex>
 #include <cstdio>
+#include <fstream>
 #include <cstring>
 #include <random>
 #include <sstream>
@@ -89,76 +90,71 @@ static void check_abcd()
     std::printf("ok abcd\n");
 }
 
-static const char* kYaml =
-    "domain: #for cathode\n"
-    "# --------------------------------\n"
-    "  ni: 96                        # (int) cells in x\n"
-    "  nj: 64\n"
-    "  xL: 0.00\n"
-    "  xR: 0.0096\n"
-    "  rmin: 0.00\n"
-    "  rmax: 0.0064\n"
-    "operating_condition:\n"
-    "  V_bias: 30.0\n"
-    "physical_parameter:\n"
-    "  masse: 9.10938356e-31\n"
-    "  massi: 2.18017156e-25\n"
-    "  Ti: 300.0\n"
-    "secondary_electron_emission:\n"
-    "  coefIISEE_ts: 0.08\n"
-    "  coefMISEE_ts: 0.08\n"
-    "  Te_emitSEE_eV: 2.5\n"
-    "  ratioEngy_EISEE_rd: 0.5\n"
-    "solver_setting:\n"
-    "  error_cnv_SOR_rhoi: 1.0e-10\n"
-    "  maxITR_SOR_rhoi: 30\n"
-    "  error_cnv_SOR_phi: 1.0e-10\n"
-    "  maxITR_SOR_phi: 25\n"
-    "  error_cnv_HES_phi: 1.0e-21\n"
-    "  maxITR_HES_phi: 2\n"
-    "  error_cnv_SOR_rhoe: 2.0e-10\n"
-    "  maxITR_SOR_rhoe: 30\n"
-    "  error_cnv_HES_rhoe: 1.0e-21\n"
-    "  maxITR_HES_rhoe: 2\n"
-    "  error_cnv_SOR_rhoeps: 1.0e-10\n"
-    "  maxITR_SOR_rhoeps: 30\n"
-    "  error_cnv_HES_rhoeps: 1.0e-21\n"
-    "  maxITR_HES_rhoeps: 3\n"
-    "  maxITR_SOR_phi: 99            # repeated key: first value kept\n"
-    "simulation_control:\n"
-    "  dt_ini: 1.0e-10\n"
-    "  flag_error: 3\n";
-
-static void check_yaml()
+// every key of the reference's reader, from a file in the shipped layout
+static void check_yaml(const std::string& path)
 {
-    std::istringstream in(kYaml);
-    SetupYaml y = parse_setup_yaml(in);
+    std::ifstream f(path);
+    CHECK((bool)f, "cannot open %s", path.c_str());
+    std::stringstream buf;
+    buf << f.rdbuf();
+    const std::string text = buf.str();
+    std::istringstream in(text);
+    const SetupYaml y = parse_setup_yaml(in);
+    CHECK(setup_keys().size() == 85, "the reference reads 85 keys, table has %zu", setup_keys().size());
+    int cats = 0;
+    for (const auto& c : y.cat) cats += c.second.empty() ? 0 : 1;
+    CHECK(cats == 14, "14 categories, found %d", cats);
+    const SetupParams sp = setup_params_from_yaml(y);
+    CHECK(sp.ni == 96 && sp.nj == 64 && sp.xR == 0.0096 && sp.rmax == 0.0064 && sp.width_neutIn == 0.002, "domain");
+    CHECK(sp.V_bias == 30.0 && sp.Pmw == 10.0 && sp.fmw == 4.25e9 && sp.flag_mwRef == 1, "operating_condition");
+    CHECK(sp.S11_mag == 0.25 && sp.S11_arg_deg == -550.0 && sp.Z0 == 50.0 && sp.phase_pre_deg == 4600.0,
+          "transmission_line: first of a repeated key kept");
+    CHECK(sp.flag_Bohm == 1 && sp.flag_Sagdeev == 0 && sp.alpha_Bohm == 16.0 && sp.nu_eff == 2.0e9, "models");
+    CHECK(std::fabs(sp.Te_ini - 5.0 * 1.60217662e-19 / 1.380649e-23) < 1e-9, "Te_ini eV -> K");
+    CHECK(sp.flag_PC == 1 && sp.Te_rep_eV == 0.1 && sp.beta_rhoUe == 1.0 && sp.maxITR_HES_rhoeps == 2, "solver");
+    CHECK(sp.flag_adp_dt == 0 && sp.dt_ini == 1e-10 && sp.CFL == 1.0 && sp.ntime == 200 && sp.ndiv_MW == 1000 &&
+          sp.flag_error == 3 && sp.ndiv_fout == 2000 && sp.flag_impTest == 0, "control");
     cathode_ndd_params p{};
     ndd_params_from_setup(y, &p);
     CHECK(p.ni == 96 && p.nj == 64 && std::fabs(p.dx - 1e-4) < 1e-18 && std::fabs(p.dr - 1e-4) < 1e-18, "grid");
-    CHECK(p.V_bias == 30.0 && p.maxITR_SOR_phi == 25 && p.error_cnv_SOR_rhoe == 2e-10 && p.maxITR_HES_rhoeps == 3,
+    CHECK(p.V_bias == 30.0 && p.maxITR_SOR_phi == 30 && p.error_cnv_SOR_rhoe == 1e-10 && p.maxITR_HES_rhoeps == 2,
           "solver keys");
     CHECK(p.dt == 1e-10 && p.flag_error == 3 && p.coefIISEE_ts == 0.08, "control keys");
-    // a missing key and a wrong type are reported as the reference does
-    std::string bad(kYaml);
-    bad.replace(bad.find("  V_bias: 30.0\n"), 15, "");
-    std::istringstream in2(bad);
+    // every key missing at once is reported, as the reference does
+    auto drop = [&](const std::string& t, const std::string& line) {
+        std::string b(t);
+        const size_t at = b.find(line);
+        if (at != std::string::npos) b.erase(at, b.find('\n', at) - at + 1);
+        return b;
+    };
     bool threw = false;
     try {
+        std::istringstream in2(drop(drop(text, "  V_bias:"), "  flag_impTest:"));
         cathode_ndd_params q{};
         ndd_params_from_setup(parse_setup_yaml(in2), &q);
     } catch (const std::runtime_error& e) {
-        threw = std::string(e.what()).find("operating_condition.V_bias") != std::string::npos &&
-                std::string(e.what()).find("Missing YAML parameters") != std::string::npos;
+        const std::string m = e.what();
+        threw = m.find("Missing YAML parameters") != std::string::npos &&
+                m.find("operating_condition.V_bias") != std::string::npos &&
+                m.find("test_mode.flag_impTest") != std::string::npos;
     }
-    CHECK(threw, "missing key");
-    std::string bad2(kYaml);
-    bad2.replace(bad2.find("  nj: 64\n"), 9, "  nj: 6.4\n");
-    std::istringstream in3(bad2);
+    CHECK(threw, "missing keys");
+    // a key the step does not use is still required
+    threw = false;
+    try {
+        std::istringstream in3(drop(text, "  nu_eff:"));
+        (void)setup_params_from_yaml(parse_setup_yaml(in3));
+    } catch (const std::runtime_error& e) {
+        threw = std::string(e.what()).find("collisionless_heating_model.nu_eff") != std::string::npos;
+    }
+    CHECK(threw, "unused key still required");
+    std::string bad2(text);
+    bad2.replace(bad2.find("  nj: 64"), 8, "  nj: 6.4");
+    std::istringstream in4(bad2);
     threw = false;
     try {
         cathode_ndd_params q{};
-        ndd_params_from_setup(parse_setup_yaml(in3), &q);
+        ndd_params_from_setup(parse_setup_yaml(in4), &q);
     } catch (const std::runtime_error& e) {
         threw = std::string(e.what()) == "Type conversion error: domain.nj should be int";
     }
@@ -310,7 +306,7 @@ int main(int argc, char** argv)
     const std::string mode = argc > 1 ? argv[1] : "cpu";
     if (mode == "cpu") {
         check_abcd();
-        check_yaml();
+        check_yaml(argc > 3 ? argv[3] : "tests/golden/setup_reference_layout.yaml");
         check_restart(argc > 2 ? argv[2] : ".");
     } else if (mode == "gpu") {
         check_dropin_gpu();

@@ -17,7 +17,8 @@ def exe():
 
 
 def test_host_headers_cpu(exe, tmp_path):
-    r = subprocess.run([exe, "cpu", str(tmp_path)], capture_output=True, text=True, timeout=60)
+    yaml = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "setup_reference_layout.yaml")
+    r = subprocess.run([exe, "cpu", str(tmp_path), yaml], capture_output=True, text=True, timeout=60)
     assert r.returncode == 0, r.stdout + r.stderr
     for tag in ("ok abcd", "ok yaml", "ok restart"):
         assert tag in r.stdout
