@@ -44,11 +44,15 @@ def compare(gpu, ref, fields, tol=TOL):
     return worst
 
 
-@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("mode,learned", [(0, 0), (1, 0), (1, 1)])
 @pytest.mark.parametrize("nsteps", [1, 3])
-def test_config0_steps(hiplib, mode, nsteps):
+def test_config0_steps(hiplib, mode, learned, nsteps):
+    """learned = 1 without a network: every solve starts from the least-squares
+    combination of its last solutions and runs its expected iterations
+    without checks; the converged fields must not change."""
     p, st = W.config0(mode=mode)
-    print(f"ndd config0 96x64 mode={mode} steps={nsteps}", flush=True)
+    p.learned = learned
+    print(f"ndd config0 96x64 mode={mode} learned={learned} steps={nsteps}", flush=True)
     ref = W.copy_state(st)
     rr = O.ndd_step(p, ref, nsteps, solver=0)
     rep = gpu_steps(hiplib, p, st, nsteps)
@@ -58,13 +62,15 @@ def test_config0_steps(hiplib, mode, nsteps):
     assert abs(rep.I_anode - rr.I_anode) <= 1e-9 * abs(rr.I_anode)
 
 
-@pytest.mark.parametrize("mode", [0, 1])
-def test_config1_small_energy(hiplib, mode):
+@pytest.mark.parametrize("mode,learned", [(0, 0), (1, 0), (1, 1)])
+def test_config1_small_energy(hiplib, mode, learned):
     p, st = W.config1(ni=128, nj=64, h=2e-4, mode=mode)
-    print(f"ndd config1 128x64 energy mode={mode}", flush=True)
+    p.learned = learned
+    nsteps = 4 if learned else 2  # warm starts need a history
+    print(f"ndd config1 128x64 energy mode={mode} learned={learned} steps={nsteps}", flush=True)
     ref = W.copy_state(st)
-    O.ndd_step(p, ref, 2, solver=0)
-    rep = gpu_steps(hiplib, p, st, 2)
+    O.ndd_step(p, ref, nsteps, solver=0)
+    rep = gpu_steps(hiplib, p, st, nsteps)
     assert rep.nonconverged == 0
     compare(st, ref, FIELDS + ["rhoeps", "Te", "Gx", "Gr"])
 
@@ -105,19 +111,23 @@ def test_config1_full_size(hiplib, mode):
     compare(st, ref, FIELDS + ["rhoeps", "Te", "Gx", "Gr"])
 
 
-@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("mode,learned", [(0, 0), (1, 0), (1, 1)])
 @pytest.mark.parametrize("cnv", [1e-21, 1e-9, 1e-5])
-def test_pc_loop_depth_and_early_stop(hiplib, mode, cnv):
+def test_pc_loop_depth_and_early_stop(hiplib, mode, learned, cnv):
     """Four PC passes per routine with a stopping threshold: the never-met
     threshold runs every pass (the start-of-pass copy sets alternate three
     times); the others stop the loop after a pass whose calcRes meets it
-    (the speculative solve aborts and the predicted fluxes are restored)."""
+    (the speculative solve aborts and the predicted fluxes are restored).
+    learned = 1 (warm starts, expected iterations unchecked) over 4 steps, so
+    the aborted solves also start from a history."""
     p, st = W.config0(mode=mode)
+    p.learned = learned
     p.maxITR_HES_phi = p.maxITR_HES_rhoe = p.maxITR_HES_rhoeps = p.maxITR_HES_rhoi = 4
     p.error_cnv_HES_phi = p.error_cnv_HES_rhoe = p.error_cnv_HES_rhoeps = p.error_cnv_HES_rhoi = cnv
-    print(f"ndd config0 96x64 pc depth 4 cnv={cnv} mode={mode}", flush=True)
+    nsteps = 4 if learned else 2
+    print(f"ndd config0 96x64 pc depth 4 cnv={cnv} mode={mode} learned={learned} steps={nsteps}", flush=True)
     ref = W.copy_state(st)
-    O.ndd_step(p, ref, 2, solver=0)
-    rep = gpu_steps(hiplib, p, st, 2)
+    O.ndd_step(p, ref, nsteps, solver=0)
+    rep = gpu_steps(hiplib, p, st, nsteps)
     assert rep.nonconverged == 0
     compare(st, ref, FIELDS)
