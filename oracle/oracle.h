@@ -16,7 +16,11 @@
  *   orc_mg_solve                          agrees with orc_struct5_direct to tolerance
  *   orc_bern_*                            B(0)=1, B'(0)=-1/2, B(-z)-B(z)=z, large-|z| limits
  *   orc_ndd_step (Poisson)                exactly linear potential for n_e=n_i, no flux
- *                                         (closed form for any r-weighting)
+ *                                         (closed form for any r-weighting); the discrete
+ *                                         Gauss law eps0 div E - e dt div Gamma = e(n_i - n_e)
+ *                                         - e dt S after the corrected pass (pins the
+ *                                         permittivity's dt, mobility, face density and the
+ *                                         source term); uniform plasma: Gamma = -mu_e n E
  *   orc_ndd_step (SG continuity)          exponential 1-D steady profile is reproduced
  *                                         exactly; M-matrix signs; global conservation
  *                                         identity (sum of volume divergence = wall flux)
@@ -33,8 +37,11 @@
  *   orc_ndd_step (Te update)              Te0 / 1.001 (the 1e-3 max n_e floor) and the
  *                                         1000 eV cap after a step that barely moves a
  *                                         uniform state
- *   parity unpinned (no check beyond the restatement and the conservation identity):
- *                                         SEE yields (IISEE/EISEE; off in the configurations)
+ *   orc_ndd_assemble (SEE)                constant yields: emitted particle / energy wall
+ *                                         flux = yield x primary (ion flux, thermal flux of
+ *                                         n_e at the step start, thermal energy flux)
+ *   orc_plant                             plants one named mistake; each check above that
+ *                                         covers it then fails (tests/test_oracle_pins.py)
  *   orc_abcd_*                            S -> ABCD -> S round trip is the identity
  * See tests/test_oracle_*.py for each check.
  */
@@ -63,6 +70,22 @@ extern "C" {
 double orc_bern_bp(double vd, double z, double D_over_h, int literal);
 double orc_bern_bm(double vd, double z, double D_over_h, int literal);
 void orc_set_literal_bernoulli(int on);
+
+/* Test hook: plant one named mistake in the restatement (0 restores it), so
+ * a test can show that the independent check covering that formula fails. */
+enum {
+    ORC_PLANT_NONE = 0,
+    ORC_PLANT_POISSON_DT = 1,         /* permittivity e dt n mu with 2 dt */
+    ORC_PLANT_POISSON_MU = 2,         /* permittivity with the ion mobility */
+    ORC_PLANT_POISSON_DENSITY = 3,    /* permittivity face density from n_i */
+    ORC_PLANT_POISSON_SOURCE = 4,     /* right-hand side without -e dt S */
+    ORC_PLANT_POISSON_FLUX_MU = 5,    /* flux correction with the ion mobility */
+    ORC_PLANT_SEE_DENSITY = 6,        /* EISEE from n_e instead of n_e at the step start */
+    ORC_PLANT_SEE_ION_SIGN = 7,       /* IISEE with the wrong sign */
+    ORC_PLANT_SEE_RD_RATIO = 8,       /* energy EISEE without the rediffused energy ratio */
+    ORC_PLANT_SEE_EMIT_ENERGY = 9     /* emitted energy e T_emit instead of 2 e T_emit */
+};
+void orc_plant(int which);
 
 /* ---- 5-point solves with the HYPRE box semantics (solverFuncs.hpp:26-162) ---- */
 /* Exact direct solve (banded LU, no pivoting: the SG/Poisson matrices are

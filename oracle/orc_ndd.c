@@ -33,6 +33,11 @@
 static int g_literal = 0;
 void orc_set_literal_bernoulli(int on) { g_literal = on; }
 
+/* Planted mistakes for the tests that show each independent check catches
+ * the error it guards against (ORC_PLANT_*, oracle.h); 0 = the restatement. */
+static int g_plant = 0;
+void orc_plant(int which) { g_plant = which; }
+
 double orc_bern_bp(double vd, double z, double D_over_h, int literal)
 {
     if (literal) return vd / (1.0 - exp(fmin(-z, ORC_LIM_EXP)));
@@ -271,16 +276,21 @@ static void poisson_predict_assemble(ctx_t* C, cathode_ndd_fields* f, const doub
             double qR = (r[j] + r[j + 1]) / 2.0 / r[j];
             double qL = (r[j] + r[j - 1]) / 2.0 / r[j];
             double mu = m.mu_star; /* (mu_perp[i][j] + mu_perp[nb]) / 2 * c_mu */
-            double nLx = (f->rhoe[IX(i, j)] + f->rhoe[IX(i - 1, j)]) / 2.0;
-            double nRx = (f->rhoe[IX(i, j)] + f->rhoe[IX(i + 1, j)]) / 2.0;
-            double nLr = (f->rhoe[IX(i, j)] + f->rhoe[IX(i, j - 1)]) / 2.0;
-            double nRr = (f->rhoe[IX(i, j)] + f->rhoe[IX(i, j + 1)]) / 2.0;
+            const double* ne = f->rhoe;
+            double dte = dt;
+            if (g_plant == ORC_PLANT_POISSON_DT) dte = 2.0 * dt;
+            if (g_plant == ORC_PLANT_POISSON_MU) mu = mobility(p->mu_i).mu_star;
+            if (g_plant == ORC_PLANT_POISSON_DENSITY) ne = f->rhoi;
+            double nLx = (ne[IX(i, j)] + ne[IX(i - 1, j)]) / 2.0;
+            double nRx = (ne[IX(i, j)] + ne[IX(i + 1, j)]) / 2.0;
+            double nLr = (ne[IX(i, j)] + ne[IX(i, j - 1)]) / 2.0;
+            double nRr = (ne[IX(i, j)] + ne[IX(i, j + 1)]) / 2.0;
             double bLx = (double)(i != 1), bRx = (double)(i != ni);
             double bLr = (double)(j != 1), bRr = (double)(j != nj);
-            double eRx = ORC_EPS0 + ORC_E0 * dt * nRx * mu * bRx;
-            double eLx = ORC_EPS0 + ORC_E0 * dt * nLx * mu * bLx;
-            double eRr = ORC_EPS0 + ORC_E0 * dt * nRr * mu * bRr;
-            double eLr = ORC_EPS0 + ORC_E0 * dt * nLr * mu * bLr;
+            double eRx = ORC_EPS0 + ORC_E0 * dte * nRx * mu * bRx;
+            double eLx = ORC_EPS0 + ORC_E0 * dte * nLx * mu * bLx;
+            double eRr = ORC_EPS0 + ORC_E0 * dte * nRr * mu * bRr;
+            double eLr = ORC_EPS0 + ORC_E0 * dte * nLr * mu * bLr;
             C->aP[IX(i, j)] = (eRx + eLx) * dr * dr + (qR * eRr + qL * eLr) * dx * dx;
             C->aE[IX(i, j)] = eRx * dr * dr;
             C->aW[IX(i, j)] = eLx * dr * dr;
@@ -289,7 +299,8 @@ static void poisson_predict_assemble(ctx_t* C, cathode_ndd_fields* f, const doub
             double div = (f->nUex[IX(i + 1, j)] - f->nUex[IX(i, j)]) / dx +
                          (qR * f->nUer[IX(i, j + 1)] - qL * f->nUer[IX(i, j)]) / dr;
             double lap = -(f->Ex[IX(i + 1, j)] - f->Ex[IX(i, j)]) / dx - (qR * f->Er[IX(i, j + 1)] - qL * f->Er[IX(i, j)]) / dr;
-            C->b[IX(i, j)] = (ORC_E0 * f->rhoi[IX(i, j)] - ORC_E0 * f->rhoe[IX(i, j)] - ORC_E0 * dt * S[IX(i, j)] +
+            const double s_on = g_plant == ORC_PLANT_POISSON_SOURCE ? 0.0 : 1.0;
+            C->b[IX(i, j)] = (ORC_E0 * f->rhoi[IX(i, j)] - ORC_E0 * f->rhoe[IX(i, j)] - s_on * ORC_E0 * dt * S[IX(i, j)] +
                               ORC_EPS0 * lap + ORC_E0 * dt * div) *
                              dx * dx * dr * dr;
         }
@@ -357,6 +368,7 @@ static void poisson_routine(ctx_t* C, cathode_ndd_fields* f, double* phi_old, do
         for (int i = 1; i <= ni; i++)
             for (int j = 2; j <= nj; j++) f->Er[IX(i, j)] = -(f->phi[IX(i, j)] - f->phi[IX(i, j - 1)]) / dr;
         /* flux correction (:2980-3033) */
+        if (g_plant == ORC_PLANT_POISSON_FLUX_MU) m = mobility(p->mu_i);
         for (int i = 2; i <= ni; i++)
             for (int j = 1; j <= nj; j++) {
                 double n_f = (f->rhoe[IX(i, j)] + f->rhoe[IX(i - 1, j)]) / 2.0;
@@ -472,7 +484,9 @@ static void sg_predict_assemble(ctx_t* C, cathode_ndd_fields* f, int sp, const s
                 double Te = f->Te[IX(i, j)];
                 double dphi = fmax(-sgn * f->Ex[IX(fi, j)] * dx / 2.0, 0.0);
                 double gpn = wall_gpn(p, Te, dphi, sgn);
-                double see = -p->coefIISEE_ts * f->rhoUix[IX(fi, j)] - eisee_sum(p, Te) * f->rhoe_old[IX(i, j)] * gpn;
+                const double* nold = g_plant == ORC_PLANT_SEE_DENSITY ? n : f->rhoe_old;
+                const double gi = g_plant == ORC_PLANT_SEE_ION_SIGN ? -p->coefIISEE_ts : p->coefIISEE_ts;
+                double see = -gi * f->rhoUix[IX(fi, j)] - eisee_sum(p, Te) * nold[IX(i, j)] * gpn;
                 q->Fx[IX(fi, j)] = n[IX(i, j)] * gpn + see;
             } else {
                 double Teo = T[IX(i, j)];
@@ -480,7 +494,7 @@ static void sg_predict_assemble(ctx_t* C, cathode_ndd_fields* f, int sp, const s
                 double vth = sqrt(8.0 * ORC_BOLTZ * Teo / (M_PI * p->masse));
                 double ex = exp(fmin(-ORC_E0 * dphi / (ORC_BOLTZ * Teo + 1e-100), ORC_LIM_EXP));
                 double gpne = sgn * vth * ex / 3.0 * 1.0;
-                double eps_emit = 2.0 * ORC_E0 * p->Te_emitSEE_eV;
+                double eps_emit = (g_plant == ORC_PLANT_SEE_EMIT_ENERGY ? 1.0 : 2.0) * ORC_E0 * p->Te_emitSEE_eV;
                 double gpn = sgn * 0.25 * 1.0 * vth * ex;
                 double Te = f->Te[IX(i, j)];
                 double eb = 0.0, rd = 0.0, ts = 0.0;
@@ -490,7 +504,8 @@ static void sg_predict_assemble(ctx_t* C, cathode_ndd_fields* f, int sp, const s
                     ts = orc_func_Te(p->see_ts, p->n_see, Te, p->dTe_SEE);
                 }
                 double see = -eps_emit * p->coefIISEE_ts * f->rhoUix[IX(fi, j)] -
-                             (eb + p->ratioEngy_EISEE_rd * rd) * n_old[IX(i, j)] * gpne -
+                             (eb + (g_plant == ORC_PLANT_SEE_RD_RATIO ? 1.0 : p->ratioEngy_EISEE_rd) * rd) *
+                                 n_old[IX(i, j)] * gpne -
                              eps_emit * ts * f->rhoe_old[IX(i, j)] * gpn;
                 q->Fx[IX(fi, j)] = n[IX(i, j)] * gpne + see;
             }

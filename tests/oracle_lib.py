@@ -37,6 +37,7 @@ _SIGS = [
     ("orc_bern_bp", ctypes.c_double, [ctypes.c_double] * 3 + [ctypes.c_int]),
     ("orc_bern_bm", ctypes.c_double, [ctypes.c_double] * 3 + [ctypes.c_int]),
     ("orc_set_literal_bernoulli", None, [ctypes.c_int]),
+    ("orc_plant", None, [ctypes.c_int]),
     ("orc_struct5_direct", ctypes.c_int, [_D] * 6 + [_I] + [ctypes.c_int] * 4 + [_D]),
     ("orc_helm5_direct", ctypes.c_int, [_D] * 6 + [_I] + [ctypes.c_int] * 4 + [_D]),
     ("orc_lu9_direct", ctypes.c_int, [_D] * 10 + [_I] + [ctypes.c_int] * 4 + [_D]),
@@ -222,3 +223,16 @@ def rel_residual_box(aP, aE, aW, aN, aS, b, box, x):
     i0, i1, j0, j1 = box
     r = (b - apply5(aP, aE, aW, aN, aS, x, box))[i0:i1 + 1, j0:j1 + 1]
     return float(np.linalg.norm(r) / np.linalg.norm(b[i0:i1 + 1, j0:j1 + 1]))
+
+
+# planted mistakes (oracle.h ORC_PLANT_*), for the tests that show each
+# independent check catches the error it guards against
+ORC_PLANT_POISSON_DT = 1
+ORC_PLANT_POISSON_MU = 2
+ORC_PLANT_POISSON_DENSITY = 3
+ORC_PLANT_POISSON_SOURCE = 4
+ORC_PLANT_POISSON_FLUX_MU = 5
+ORC_PLANT_SEE_DENSITY = 6
+ORC_PLANT_SEE_ION_SIGN = 7
+ORC_PLANT_SEE_RD_RATIO = 8
+ORC_PLANT_SEE_EMIT_ENERGY = 9
