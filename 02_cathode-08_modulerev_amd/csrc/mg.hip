@@ -247,7 +247,7 @@ __device__ inline double readlane_f64(double v, int lane)
 // particular solution.
 constexpr int CINV_NT = 512;
 template <int NT>
-__device__ __forceinline__ void coarse_inverse_block(const CoarseInv& c, double s32, double* lds_rows,
+__device__ __forceinline__ void coarse_inverse_block(const CoarseInv& c, double* lds_rows,
                                                      double* lds_red)
 {
     constexpr int NM = MG_COARSEST_MAX;
@@ -327,10 +327,7 @@ __device__ __forceinline__ void coarse_inverse_block(const CoarseInv& c, double 
 #pragma unroll
         for (int r = 0; r < RPW; r++) {
             const int q = w * RPW + r;
-            if (q < n) {
-                c.inv64[q * n + j] = m[r];
-                c.inv32[q * n + j] = (float)(m[r] / s32);
-            }
+            if (q < n) c.inv64[q * n + j] = m[r];
         }
     }
 }
@@ -352,7 +349,7 @@ __device__ __forceinline__ void chain_block(const CoarsenChain& c, const double*
     if (ci.inv64) {
         __shared__ double rows[2 * (MG_COARSEST_MAX / (NT / 64)) * 64];
         __shared__ double red[NT / 64];
-        coarse_inverse_block<NT>(ci, sc[3], rows, red);
+        coarse_inverse_block<NT>(ci, rows, red);
     }
 }
 
@@ -367,7 +364,8 @@ __global__ void __launch_bounds__(CINV_NT) k_coarse_inverse(CoarseInv ci, const 
 {
     __shared__ double rows[2 * (MG_COARSEST_MAX / (CINV_NT / 64)) * 64];
     __shared__ double red[CINV_NT / 64];
-    coarse_inverse_block<CINV_NT>(ci, sc[3], rows, red);
+    (void)sc;
+    coarse_inverse_block<CINV_NT>(ci, rows, red);
 }
 
 // A y_q at cell (i, j) of level 0 (5-point operator, couplings to inactive
@@ -1248,7 +1246,6 @@ void MGSolver::build(int ni, int nj)
     grab(scal, nsc);
     grab(kv, 9 * n0);
     grab(cinv64, MG_COARSEST_MAX * MG_COARSEST_MAX);
-    grab(cinv32, MG_COARSEST_MAX * MG_COARSEST_MAX);
     if (arena) grab(dmirror, nsc);
 }
 
@@ -1346,7 +1343,6 @@ void MGSolver::setup(int mode, hipStream_t s, const WarmSet& ws)
         ci.aP = Z.aP; ci.aE = Z.aE; ci.aW = Z.aW; ci.aN = Z.aN; ci.aS = Z.aS;
         ci.act = actall.p + level_off[nl - 1];
         ci.inv64 = cinv64.p;
-        ci.inv32 = cinv32.p;
     }
     bool inverted = false;
     for (int l0 = lt; l0 + 1 < nl; l0 += CHAIN_MAX) {
@@ -1426,11 +1422,12 @@ void MGSolver::vcycle(Level<T>* L, hipStream_t s, const VFuse& fdown, const VFus
     ta.x_out = L[lt].x;
     ta.w = lt == 0 ? w : nullptr;
     ta.cn = L[nl - 1].ni * L[nl - 1].nj;
-    if constexpr (sizeof(T) == 4) ta.cinv = (const T*)cinv32.p;
-    else ta.cinv = (const T*)cinv64.p;
+    // the coarsest level is solved in FP64 in both precisions: the FP32
+    // levels hold s32 A, so their solution is (A^-1 b) / s32
+    ta.cinv = cinv64.p;
+    ta.s32 = sizeof(T) == 4 ? scal.p + SC_S32 : nullptr;
     const int top = L[lt].ni * L[lt].nj;
-    const size_t lds =
-        ((size_t)off * 2 + std::max(top, TAILR_NT) + (size_t)CINV_LD * MG_COARSEST_MAX) * sizeof(T) + 16;
+    const size_t lds = tail_lds_bytes<T>(off, top);
     if (top <= TAILR_NT) launch_tail<T, 1>(ta, lds, s);
     else launch_tail<T, 2>(ta, lds, s);
     // fine levels: fused up kernels; the coarse correction is the next

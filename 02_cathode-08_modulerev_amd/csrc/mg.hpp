@@ -80,8 +80,7 @@ struct CoarseInv {
     int ni, nj, ld;
     const double *aP, *aE, *aW, *aN, *aS;
     const unsigned char* act;
-    double* inv64;  // n x n, row-major
-    float* inv32;   // the same divided by s32 (the FP32 levels hold s32 A)
+    double* inv64;  // n x n, row-major (applied in FP64 by both precisions' tails)
 };
 
 class MGSolver {
@@ -254,8 +253,7 @@ private:
     int pair_cfg = 0;          // their tile configuration (VP_CFGS; CATHODE_PAIR_TILE picks another)
     int tail_cells = 0;        // LDS cells of the tail levels
     DevBuf<double> part;       // reduction partials (2 slots)
-    DevBuf<double> cinv64;     // dense inverse of the coarsest level (FP64)
-    DevBuf<float> cinv32;      // the same for the FP32 levels (divided by the FP32 scale)
+    DevBuf<double> cinv64;     // dense inverse of the coarsest level (FP64, used by both precisions)
     DevBuf<double> kv;         // BiCGStab vectors r, r0, p, v, s, t, ph, sh, p (second buffer)
     DevBuf<double> scal;       // device scalars: [0] |b|^2, [1] |r|^2, [2] scale, [3] f32 matrix scale
     double* h_scal = nullptr;  // pinned mirror of scal (arena mode: the caller's host block)

@@ -624,7 +624,8 @@ struct TailArgs {
     const T* b_in;                   // right-hand side of the first tail level (global)
     T* x_out;                        // its solution (global)
     const double* w;                 // restriction weights when the tail starts at level 0
-    const T* cinv;                   // k_tail_reg: dense inverse of the coarsest level (cn x cn, row-major)
+    const double* cinv;              // k_tail_reg: dense FP64 inverse of the coarsest level (cn x cn, row-major)
+    const double* s32;               // FP32 levels: their scale (they hold s32 A); nullptr for FP64 levels
     int cn;                          // its cells
 };
 
@@ -880,6 +881,20 @@ __device__ inline void tail_prolong(const TailCells<T, C>& L, T* X)
         if (L.c[k] >= 0 && (L.fl[k] & 1u)) X[L.c[k]] = X[L.c[k]] + X[L.par[k]];
 }
 
+// LDS of k_tail_reg: X and B (ncell each), R (rn), then, 8-byte aligned, the
+// FP64 coarsest inverse (CINV_LD x MG_COARSEST_MAX) and its gathered b
+template <class T>
+__host__ __device__ inline size_t tail_ci_offset(int ncell, int rn)
+{
+    return (((size_t)ncell * 2 + rn) * sizeof(T) + 7) & ~(size_t)7;
+}
+template <class T>
+inline size_t tail_lds_bytes(int ncell, int top)
+{
+    return tail_ci_offset<T>(ncell, std::max(top, TAILR_NT)) +
+           ((size_t)CINV_LD * MG_COARSEST_MAX + MG_COARSEST_MAX) * sizeof(double) + 16;
+}
+
 template <class T, int C0>
 __global__ void __launch_bounds__(TAILR_NT) k_tail_reg(TailArgs<T> A)
 {
@@ -890,12 +905,16 @@ __global__ void __launch_bounds__(TAILR_NT) k_tail_reg(TailArgs<T> A)
     const int q = threadIdx.x;
     const int nt = A.nt;
     const int rn = max(A.ni[0] * A.nj[0], TAILR_NT);
-    T* CI = R + rn;      // the coarsest level's inverse, row pitch CINV_LD
+    // the coarsest level's FP64 inverse (row pitch CINV_LD) and its gathered
+    // right-hand side, 8-byte aligned after the residuals
+    double* CI = reinterpret_cast<double*>(tail_lds + tail_ci_offset<T>(A.ncell, rn));
+    double* CB = CI + CINV_LD * MG_COARSEST_MAX;
     for (int c = q; c < A.ncell; c += TAILR_NT) {
         X[c] = T(0);
         B[c] = T(0);
     }
     for (int e = q; e < A.cn * A.cn; e += TAILR_NT) CI[(e / A.cn) * CINV_LD + e % A.cn] = A.cinv[e];
+    const double inv_s32 = A.s32 ? 1.0 / *A.s32 : 1.0;  // a power of two: exact
     __syncthreads();
     TailCells<T, C0> L0;
     TailCells<T, 1> Lk[TAIL_REG_LEVELS - 1];  // tail levels 1 ..
@@ -931,15 +950,20 @@ __global__ void __launch_bounds__(TAILR_NT) k_tail_reg(TailArgs<T> A)
             tail_sync(ncl[l]);
         }
     }
-    // ---- coarsest level (<= MG_COARSEST_MAX cells): x = A^-1 b with the
-    // inverse built at setup; thread q < cn forms row q
+    // ---- coarsest level (<= MG_COARSEST_MAX cells, all in wavefront 0):
+    // x = A^-1 b in FP64 with the inverse built at setup; thread q < cn
+    // gathers b_q, then forms row q
     {
         const int l = nt - 1;
         const int nj = A.nj[l], ld = nj + 2, o = A.off[l], cn = A.cn;
+        const int cell = q < cn ? o + (1 + q / nj) * ld + 1 + q % nj : 0;
+        if (q < cn) CB[q] = (double)B[cell];
+        tail_sync(cn);
         if (q < cn) {
-            T s = T(0);
-            for (int t = 0; t < cn; t++) s += CI[q * CINV_LD + t] * B[o + (1 + t / nj) * ld + 1 + t % nj];
-            X[o + (1 + q / nj) * ld + 1 + q % nj] = s;
+            double s = 0.0;
+#pragma unroll 8
+            for (int t = 0; t < cn; t++) s += CI[q * CINV_LD + t] * CB[t];
+            X[cell] = (T)(s * inv_s32);
         }
         if (nt == 1) __syncthreads();  // the output below reads cells other threads solved
     }
