@@ -760,7 +760,7 @@ __global__ void __launch_bounds__(TAIL_NT) k_tail(TailArgs<T> A)
 // the first wavefront (<= 64 cells) synchronise that wavefront alone instead
 // of the whole workgroup.
 constexpr int TAILR_NT = 1024;
-constexpr int TAIL_REG_LEVELS = 7;    // the first tail level and at most 6 below it
+constexpr int TAIL_REG_LEVELS = 5;    // the first tail level and at most 4 below it (register budget of the FP64 tail)
 constexpr int TAIL_TOP_MAX = 2 * TAILR_NT;
 
 __device__ inline void tail_sync(int cells)
@@ -785,11 +785,15 @@ struct TailCells {
     unsigned fl[C];  // bit0 active, bit1 colour
 };
 
-// The loads do not depend on each other or on the activity byte (a thread
-// without a cell reads the level's first cell and discards it), so every
-// level's loads are in flight together.
+// Two phases, so that every global load of the launch is in flight at once
+// (a use of any loaded value waits for all loads issued before it):
+// tail_fetch issues one level's loads -- coefficients, activity byte and, on
+// the first level, b -- without using them (a thread without a cell reads
+// the level's first cell and discards it); tail_finish applies the activity
+// mask, forms 1/aP and puts the first level's b into LDS.
 template <class T, int C>
-__device__ inline void tail_load(TailCells<T, C>& L, const TailArgs<T>& A, int l, int q, T* B)
+__device__ inline void tail_fetch(TailCells<T, C>& L, const TailArgs<T>& A, int l, int q, T (&b)[C],
+                                  unsigned (&act)[C])
 {
     const int ni = A.ni[l], nj = A.nj[l], ld = nj + 2;
 #pragma unroll
@@ -799,27 +803,39 @@ __device__ inline void tail_load(TailCells<T, C>& L, const TailArgs<T>& A, int l
         const int cc = here ? ci : 0;
         const int i = 1 + cc / nj, j = 1 + cc % nj;
         const size_t g = (size_t)i * A.gld[l] + j;
-        const bool on = here && A.act[l][g] != 0;
-        const T P = A.aP[l][g], E = A.aE[l][g], W = A.aW[l][g], N = A.aN[l][g], S = A.aS[l][g];
+        act[k] = A.act[l][g];
+        L.P[k] = A.aP[l][g];
+        L.E[k] = A.aE[l][g];
+        L.W[k] = A.aW[l][g];
+        L.N[k] = A.aN[l][g];
+        L.S[k] = A.aS[l][g];
+        b[k] = l == 0 ? A.b_in[g] : T(0);
+        L.wq[k] = (l == 0 && A.w) ? (T)A.w[j] : T(1);
         L.c[k] = here ? A.off[l] + i * ld + j : -1;
         L.idx[k] = here ? ci : 0;
-        L.fl[k] = (here && ((i + j) & 1) ? 2u : 0u) | (on ? 1u : 0u);
-        L.P[k] = on ? P : T(1);
-        L.iP[k] = on ? T(1) / P : T(1);
-        L.E[k] = on ? E : T(0);
-        L.W[k] = on ? W : T(0);
-        L.N[k] = on ? N : T(0);
-        L.S[k] = on ? S : T(0);
-        if (l == 0) {
-            const T b = A.b_in[g];
-            if (on) B[L.c[k]] = b;
-        }
+        L.fl[k] = (here && ((i + j) & 1)) ? 2u : 0u;
         L.par[k] = 0;
         if (here && l + 1 < A.nt) {
             const int I = A.cx[l] ? (i + 1) >> 1 : i, J = A.cy[l] ? (j + 1) >> 1 : j;
             L.par[k] = A.off[l + 1] + I * (A.nj[l + 1] + 2) + J;
         }
-        L.wq[k] = (l == 0 && A.w) ? (T)A.w[j] : T(1);
+    }
+}
+
+template <class T, int C>
+__device__ inline void tail_finish(TailCells<T, C>& L, int l, const T (&b)[C], const unsigned (&act)[C], T* B)
+{
+#pragma unroll
+    for (int k = 0; k < C; k++) {
+        const bool on = L.c[k] >= 0 && act[k] != 0;
+        L.fl[k] |= on ? 1u : 0u;
+        L.iP[k] = on ? T(1) / L.P[k] : T(1);
+        L.P[k] = on ? L.P[k] : T(1);
+        L.E[k] = on ? L.E[k] : T(0);
+        L.W[k] = on ? L.W[k] : T(0);
+        L.N[k] = on ? L.N[k] : T(0);
+        L.S[k] = on ? L.S[k] : T(0);
+        if (l == 0 && on) B[L.c[k]] = b[k];
     }
 }
 
@@ -909,24 +925,41 @@ __global__ void __launch_bounds__(TAILR_NT) k_tail_reg(TailArgs<T> A)
     // right-hand side, 8-byte aligned after the residuals
     double* CI = reinterpret_cast<double*>(tail_lds + tail_ci_offset<T>(A.ncell, rn));
     double* CB = CI + CINV_LD * MG_COARSEST_MAX;
+    static_assert(MG_COARSEST_MAX * MG_COARSEST_MAX <= TAILR_NT, "one inverse entry per thread");
     for (int c = q; c < A.ncell; c += TAILR_NT) {
         X[c] = T(0);
         B[c] = T(0);
     }
-    for (int e = q; e < A.cn * A.cn; e += TAILR_NT) CI[(e / A.cn) * CINV_LD + e % A.cn] = A.cinv[e];
-    const double inv_s32 = A.s32 ? 1.0 / *A.s32 : 1.0;  // a power of two: exact
-    __syncthreads();
+    // every global load in flight together: the inverse, the FP32 scale,
+    // each level's coefficients and activity, the first level's b
+    const int cn2 = A.cn * A.cn;
+    const double cinv_q = q < cn2 ? A.cinv[q] : 0.0;
+    const double s32v = A.s32 ? *A.s32 : 1.0;
     TailCells<T, C0> L0;
+    T b0[C0];
+    unsigned a0[C0];
+    tail_fetch(L0, A, 0, q, b0, a0);
     TailCells<T, 1> Lk[TAIL_REG_LEVELS - 1];  // tail levels 1 ..
-    tail_load(L0, A, 0, q, B);
+    T bk[TAIL_REG_LEVELS - 1][1];
+    unsigned ak[TAIL_REG_LEVELS - 1][1];
     int ncl[TAIL_REG_LEVELS];
     ncl[0] = A.ni[0] * A.nj[0];
 #pragma unroll
     for (int l = 1; l < TAIL_REG_LEVELS; l++) {
         ncl[l] = l < nt ? A.ni[l] * A.nj[l] : 0;
-        if (l < nt) tail_load(Lk[l - 1], A, l, q, B);
-        else Lk[l - 1].c[0] = -1;
+        if (l < nt) {
+            tail_fetch(Lk[l - 1], A, l, q, bk[l - 1], ak[l - 1]);
+        } else {
+            Lk[l - 1].c[0] = -1;
+            ak[l - 1][0] = 0;
+        }
     }
+    __syncthreads();  // the zeroing is complete before b lands in B
+    if (q < cn2) CI[(q / A.cn) * CINV_LD + q % A.cn] = cinv_q;
+    const double inv_s32 = 1.0 / s32v;  // a power of two: exact
+    tail_finish(L0, 0, b0, a0, B);
+#pragma unroll
+    for (int l = 1; l < TAIL_REG_LEVELS; l++) tail_finish(Lk[l - 1], l, bk[l - 1], ak[l - 1], B);
     __syncthreads();
     // ---- down
     if (nt > 1) {
