@@ -52,6 +52,7 @@ class NddParams(ctypes.Structure):
         ("mode", ctypes.c_int), ("learned", ctypes.c_int),
         ("see_eb", ctypes.c_double * SEE_MAX), ("see_rd", ctypes.c_double * SEE_MAX),
         ("see_ts", ctypes.c_double * SEE_MAX),
+        ("flag_adp_dt", ctypes.c_int), ("CFL", ctypes.c_double),
     ]
 
 
@@ -78,6 +79,8 @@ class NddReport(ctypes.Structure):
         ("error_Gx", ctypes.c_double), ("error_Gr", ctypes.c_double),
         ("I_anode", ctypes.c_double), ("Ii_anode", ctypes.c_double), ("Ie_anode", ctypes.c_double),
         ("seconds", ctypes.c_double),
+        ("dt", ctypes.c_double), ("CFL", ctypes.c_double), ("dt_min", ctypes.c_double),
+        ("rhoe_max", ctypes.c_double), ("Te_max", ctypes.c_double), ("ue_max", ctypes.c_double),
     ]
 
 

@@ -8,6 +8,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <exception>
+#include <stdexcept>
 #include <mutex>
 #include <new>
 #include <string>
@@ -58,6 +59,10 @@ std::mutex& api_mutex();
     catch (const std::bad_alloc&) {                         \
         ::cathode::set_error("host allocation failed");    \
         return CATHODE_ERR_NOMEM;                           \
+    }                                                       \
+    catch (const std::domain_error& ex_) {                  \
+        ::cathode::set_error(ex_.what());                   \
+        return CATHODE_ERR_NUMERIC;                         \
     }                                                       \
     catch (const std::exception& ex_) {                     \
         ::cathode::set_error(std::string("internal error: ") + ex_.what()); \

@@ -861,6 +861,68 @@ __global__ void k_anode(NddView V, const double* __restrict__ rhoUix, const doub
     }
 }
 
+// check_CFL (fluidModule.hpp:57-115), first stage: per block of the
+// extended grid the maxima of n_e and Te over every cell (ghosts included)
+// and of the electron speed |Gamma_e| / n_e over the interior cells
+// (x flux averaged over the cell's two faces, r flux r-weighted)
+__global__ void k_cfl_partials(NddView V, const double* __restrict__ rhoe, const double* __restrict__ Te,
+                               const double* __restrict__ Fx, const double* __restrict__ Fr, double* __restrict__ part)
+{
+#pragma clang fp contract(off)
+    __shared__ double lds[NB_X * NB_Y / 64];
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y * blockDim.y + threadIdx.y;
+    double n = 0.0, t = 0.0, u = 0.0;
+    if (i <= V.ni + 1 && j <= V.nj + 1) {
+        n = rhoe[IX(i, j)];
+        t = Te[IX(i, j)];
+        if (i >= 1 && i <= V.ni && j >= 1 && j <= V.nj) {
+            const double rL = (V.r[j] + V.r[j - 1]) / 2.0, rR = (V.r[j] + V.r[j + 1]) / 2.0;
+            const double ux = (Fx[IX(i + 1, j)] + Fx[IX(i, j)]) / 2.0 / (n + 1e-100);
+            const double ur = (rR * Fr[IX(i, j + 1)] + rL * Fr[IX(i, j)]) / (2.0 * V.r[j]) / (n + 1e-100);
+            u = sqrt(ux * ux + ur * ur);
+        }
+    }
+    // the maxima start at 0, as the reference's loops do
+    n = block_max<NB_X * NB_Y>(fmax(n, 0.0), lds);
+    t = block_max<NB_X * NB_Y>(fmax(t, 0.0), lds);
+    u = block_max<NB_X * NB_Y>(u, lds);
+    if (lin_tid() == 0) {
+        const int b = blockIdx.y * gridDim.x + blockIdx.x;
+        part[3 * b] = n;
+        part[3 * b + 1] = t;
+        part[3 * b + 2] = u;
+    }
+}
+
+// second stage: out = {dt, CFL, dt_min, max n_e, max Te, max u_e + vth(max Te)};
+// adaptive: dt = CFL dt_min, else CFL = dt / dt_min (dt_min = eps0 / (e n mu))
+__global__ void k_cfl_finish(const double* __restrict__ part, int nb, double mu_max, double masse, double dt,
+                             double CFL, int adaptive, double* __restrict__ out)
+{
+#pragma clang fp contract(off)
+    __shared__ double lds[4];
+    double n = 0.0, t = 0.0, u = 0.0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+        n = fmax(n, part[3 * b]);
+        t = fmax(t, part[3 * b + 1]);
+        u = fmax(u, part[3 * b + 2]);
+    }
+    n = block_max<256>(n, lds);
+    t = block_max<256>(t, lds);
+    u = block_max<256>(u, lds);
+    if (threadIdx.x == 0) {
+        const double vth = sqrt(8.0 * K_BOLTZ * t / (M_PI * masse));
+        const double dt_min = K_EPS0 / (K_E0 * n * mu_max);
+        out[0] = adaptive ? dt_min * CFL : dt;
+        out[1] = adaptive ? CFL : dt / dt_min;
+        out[2] = dt_min;
+        out[3] = n;
+        out[4] = t;
+        out[5] = u + vth;
+    }
+}
+
 __global__ void k_fill_one(int n, unsigned char* a, int ni, int nj, int ld)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -950,6 +1012,7 @@ struct NddHandle {
     DevBuf<double> work;    // scratch arrays
     DevBuf<double> radii, see;
     DevBuf<double> part, scal;
+    DevBuf<double> cfl_part;  // check_CFL block maxima (3 per block of the extended grid)
     DevBuf<double> sgw;     // SG face weights of one PC pass (SgArgs::wgt)
     double* h_scal = nullptr;
     double* F[19] = {nullptr};
@@ -1341,8 +1404,29 @@ static void sg_routine(NddHandle& H, int sp, bool report, double* err3)
     }
 }
 
+// check_CFL before a step (main.cpp:174) into H.scal[16..21]; with
+// flag_adp_dt = 1 the step's dt is read back and set (one host round trip)
+static void check_cfl(NddHandle& H)
+{
+    NddView& V = H.V;
+    const dim3 g = ggrid(V.ni + 2, V.nj + 2);
+    const int nb = (int)(g.x * g.y);
+    hipLaunchKernelGGL(k_cfl_partials, g, gblk(), 0, H.s, V, (const double*)H.F[F_RHOE], (const double*)H.F[F_TE],
+                       (const double*)H.F[F_RHOUEX], (const double*)H.F[F_RHOUER], H.cfl_part.p);
+    hipLaunchKernelGGL(k_cfl_finish, dim3(1), dim3(256), 0, H.s, (const double*)H.cfl_part.p, nb,
+                       std::max(0.0, H.p.mu_e), H.p.masse, V.dt, H.p.CFL, H.p.flag_adp_dt == 1 ? 1 : 0,
+                       H.scal.p + 16);
+    CATHODE_LAUNCH_CHECK();
+    if (H.p.flag_adp_dt == 1) {
+        V.dt = read_dev(H, H.scal.p + 16);
+        if (!(V.dt > 0.0) || !std::isfinite(V.dt)) throw std::domain_error("check_CFL: non-finite adaptive dt");
+    }
+}
+
 static void ndd_one_step(NddHandle& H, bool report)
 {
+    // time step: every step when dt adapts, else for the report of the last step
+    if (H.p.flag_adp_dt == 1 || report) check_cfl(H);
     const NddView& V = H.V;
     double** F = H.F;
     double e3[3] = {0, 0, 0};
@@ -1390,6 +1474,7 @@ int cathode_ndd_create(const cathode_ndd_params* p, void** handle)
         if (!p || !handle || p->ni < 2 || p->nj < 2 || !(p->dx > 0) || !(p->dr > 0) || !(p->dt > 0) ||
             p->n_see < 0 || p->n_see > CATHODE_SEE_TABLE_MAX || (p->mode != CATHODE_MODE_FP64 && p->mode != CATHODE_MODE_MIXED) ||
             p->maxITR_HES_phi < 1 || p->maxITR_HES_rhoe < 1 || p->maxITR_HES_rhoi < 1 ||
+            (p->flag_adp_dt == 1 && !(p->CFL > 0.0)) ||
             (p->energy_on && p->maxITR_HES_rhoeps < 1) || p->flag_error < 0 || p->flag_error > 3) {
             set_error("ndd_create: invalid parameters");
             return CATHODE_ERR_ARG;
@@ -1440,7 +1525,11 @@ int cathode_ndd_create(const cathode_ndd_params* p, void** handle)
         H->nb = cdiv(nj, NB_X) * cdiv(ni, NB_Y);
         H->part.alloc(6 * H->nb + 64);
         H->sgw.alloc(4 * (size_t)n0);
-        H->scal.alloc(16);
+        H->scal.alloc(32);
+        {
+            const dim3 gc = ggrid(ni + 2, nj + 2);
+            H->cfl_part.alloc(3 * (size_t)gc.x * gc.y + 64);
+        }
         if (p->learned) {
             for (auto& slot : H->wbuf)
                 for (auto& wb : slot) {
@@ -1449,7 +1538,7 @@ int cathode_ndd_create(const cathode_ndd_params* p, void** handle)
                 }
             CATHODE_HIP(hipDeviceSynchronize());  // the memsets run on the null stream
         }
-        CATHODE_HIP(hipHostMalloc(&H->h_scal, 16 * sizeof(double), hipHostMallocDefault));
+        CATHODE_HIP(hipHostMalloc(&H->h_scal, 32 * sizeof(double), hipHostMallocDefault));
         CATHODE_HIP(hipStreamSynchronize(H->s));
         *handle = H.release();
         return CATHODE_OK;
@@ -1554,6 +1643,7 @@ static int ndd_run(NddHandle* H, int nsteps, cathode_ndd_report* rep)
             hipLaunchKernelGGL(k_anode, dim3(1), dim3(256), 0, H->s, H->V, H->F[F_RHOUIX], H->F[F_RHOUEX], H->scal.p + 12);
             CATHODE_LAUNCH_CHECK();
             fetch_host(H->h_scal, H->scal.p + 12, 2, H->s);
+            fetch_host(H->h_scal + 16, H->scal.p + 16, 6, H->s);
         }
         CATHODE_HIP(hipStreamSynchronize(H->s));
         if (rep) {
@@ -1562,6 +1652,12 @@ static int ndd_run(NddHandle* H, int nsteps, cathode_ndd_report* rep)
                 rep->Ii_anode = H->h_scal[0];
                 rep->Ie_anode = H->h_scal[1];
                 rep->I_anode = H->h_scal[0] + H->h_scal[1];
+                rep->dt = H->h_scal[16];
+                rep->CFL = H->h_scal[17];
+                rep->dt_min = H->h_scal[18];
+                rep->rhoe_max = H->h_scal[19];
+                rep->Te_max = H->h_scal[20];
+                rep->ue_max = H->h_scal[21];
             }
             rep->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         }

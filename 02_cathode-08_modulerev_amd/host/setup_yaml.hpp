@@ -280,7 +280,7 @@ inline SetupParams setup_params_from_yaml(const SetupYaml& y)
 // domain (ni, nj, xL, xR, rmin, rmax -> dx, dr), operating_condition.V_bias,
 // physical_parameter (masse, massi, Ti), secondary_electron_emission,
 // solver_setting (maxITR_SOR_*, error_cnv_SOR_*, maxITR_HES_*,
-// error_cnv_HES_*) and simulation_control (dt_ini, flag_error).
+// error_cnv_HES_*) and simulation_control (dt_ini, flag_error, flag_adp_dt, CFL).
 // Mobilities, the energy switch, the precision mode and the SEE table are not
 // setup.yaml keys (the reference computes mobilities from BOLSIG tables) and
 // keep the values already in *p.
@@ -321,6 +321,8 @@ inline void ndd_params_from_setup(const SetupYaml& y, cathode_ndd_params* p)
     r.read(p->maxITR_HES_rhoeps, ss, "maxITR_HES_rhoeps");
     r.read(p->dt, "simulation_control", "dt_ini");
     r.read(p->flag_error, "simulation_control", "flag_error");
+    r.read(p->flag_adp_dt, "simulation_control", "flag_adp_dt");
+    r.read(p->CFL, "simulation_control", "CFL");
     r.finish();
     if (p->ni <= 0 || p->nj <= 0) throw std::runtime_error("[setup.yaml] domain.ni / domain.nj must be positive");
     p->dx = (xR - xL) / p->ni;

@@ -47,6 +47,9 @@ def default_params(ni, nj, dx, dr, dt=1e-10, V_bias=30.0, energy_on=False,
     p.error_cnv_SOR_phi = p.error_cnv_SOR_rhoe = p.error_cnv_SOR_rhoeps = p.error_cnv_SOR_rhoi = 1e-10
     p.mode = mode
     p.learned = 0
+    # input/setup.yaml simulation_control: fixed dt (the report carries the CFL number)
+    p.flag_adp_dt = 0
+    p.CFL = 1.0
     return p
 
 

@@ -209,6 +209,13 @@ typedef struct cathode_ndd_params {
     double see_eb[CATHODE_SEE_TABLE_MAX];
     double see_rd[CATHODE_SEE_TABLE_MAX];
     double see_ts[CATHODE_SEE_TABLE_MAX];
+    /* time-step control, FluidModule::check_CFL (fluidModule.hpp:57-115),
+     * evaluated before every step as main.cpp:174 does: the dielectric
+     * relaxation bound dt_min = eps0 / (e max(n_e) max(mu_e)) over every cell.
+     * flag_adp_dt = 1 (setup.yaml simulation_control.flag_adp_dt): each step
+     * takes dt = CFL * dt_min; 0: dt stays and the report gives CFL = dt / dt_min. */
+    int flag_adp_dt;
+    double CFL;
 } cathode_ndd_params;
 
 /* Host field set, each (ni+2)*(nj+2) doubles, ld = nj+2.  Face arrays use
@@ -233,6 +240,11 @@ typedef struct cathode_ndd_report {
            error_rhoUer, error_rhoeps, error_Gx, error_Gr;
     double I_anode, Ii_anode, Ie_anode;   /* right-electrode current [A] (fluidModule.hpp:3942-3965) */
     double seconds;             /* host wall time of the call */
+    /* check_CFL before the last step (Params::dt, CFL): the step's dt, the CFL
+     * number, dt_min and the maxima it is built from -- max n_e and max Te
+     * over every cell, max |Gamma_e| / n_e over the interior plus the thermal
+     * speed at max Te (computed as the reference does; it sets nothing) */
+    double dt, CFL, dt_min, rhoe_max, Te_max, ue_max;
 } cathode_ndd_report;
 
 int cathode_ndd_create(const cathode_ndd_params* p, void** handle);

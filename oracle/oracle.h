@@ -34,6 +34,9 @@
  *   orc_ndd_assemble (energy RHS)         J.E source = 3/5 (-e Gamma E) dx dr for uniform
  *                                         fields and flux; 3/5 (Pabs - n_e eloss) dx dr
  *                                         without them
+ *   orc_check_cfl                         uniform n_e: dt_min = eps0/(e n mu) exactly; uniform
+ *                                         fluxes: ue = |Gamma|/n + vth (closed forms); the
+ *                                         adaptive step's dt = CFL dt_min
  *   orc_ndd_step (Te update)              Te0 / 1.001 (the 1e-3 max n_e floor) and the
  *                                         1000 eV cap after a step that barely moves a
  *                                         uniform state
@@ -140,6 +143,10 @@ int orc_ndd_step(const cathode_ndd_params* p, cathode_ndd_fields* f,
 int orc_ndd_assemble(const cathode_ndd_params* p, const cathode_ndd_fields* f,
                      int which, double* aP, double* aE, double* aW, double* aN,
                      double* aS, double* b, double* fx, double* fr);
+
+/* check_CFL (fluidModule.hpp:57-115): out = {dt_min, max n_e, max mu, max Te,
+ * max |Gamma_e|/n_e + vth(max Te)} of the state */
+void orc_check_cfl(const cathode_ndd_params* p, const cathode_ndd_fields* f, double* out);
 
 /* calcRes (miniFuncs.hpp:292-375) over an index box, mode 0..3 */
 double orc_calc_res(const double* vnew, const double* vold, int ld, int i0,

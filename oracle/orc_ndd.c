@@ -633,10 +633,48 @@ static void set_eq(const cathode_ndd_params* p, cathode_ndd_fields* f, int sp, c
 }
 
 /* ------------------------------------------------------------------ */
-int orc_ndd_step(const cathode_ndd_params* p, cathode_ndd_fields* f, int nsteps, int solver,
+/* check_CFL, fluidModule.hpp:57-115: maxima of n_e and Te over every cell
+ * (ghosts included), of mu_para (uniform mu_e here, B = 0) and of the
+ * electron speed |Gamma_e| / n_e over the interior (the x flux averaged over
+ * the cell's two faces, the r flux r-weighted), the thermal speed at max Te
+ * added; dt_min = eps0 / (e max n_e max mu).  out = {dt_min, rhoe_max,
+ * mu_max, Te_max, ue_max}. */
+void orc_check_cfl(const cathode_ndd_params* p, const cathode_ndd_fields* f, double* out)
+{
+    const int ni = p->ni, nj = p->nj, ld = nj + 2;
+    double rhoe_max = 0.0, Te_max = 0.0, ue_max = 0.0;
+    const double mu_max = fmax(0.0, p->mu_e);
+    for (int i = 0; i < ni + 2; i++)
+        for (int j = 0; j < nj + 2; j++) {
+            if (rhoe_max < f->rhoe[i * ld + j]) rhoe_max = f->rhoe[i * ld + j];
+            if (Te_max < f->Te[i * ld + j]) Te_max = f->Te[i * ld + j];
+        }
+    for (int i = 1; i < ni + 1; i++)
+        for (int j = 1; j < nj + 1; j++) {
+            double rj = p->rmin + (j - 0.5) * p->dr, rjm = p->rmin + (j - 1.5) * p->dr,
+                   rjp = p->rmin + (j + 0.5) * p->dr;
+            double rL = (rj + rjm) / 2.0, rR = (rj + rjp) / 2.0;
+            double n = f->rhoe[i * ld + j] + 1e-100;
+            double ux = (f->rhoUex[(i + 1) * ld + j] + f->rhoUex[i * ld + j]) / 2.0 / n;
+            double ur = (rR * f->rhoUer[i * ld + j + 1] + rL * f->rhoUer[i * ld + j]) / (2.0 * rj) / n;
+            double u = sqrt(ux * ux + ur * ur);
+            if (ue_max < u) ue_max = u;
+        }
+    double vth = sqrt(8.0 * ORC_BOLTZ * Te_max / (M_PI * p->masse));
+    out[0] = ORC_EPS0 / (ORC_E0 * rhoe_max * mu_max);
+    out[1] = rhoe_max;
+    out[2] = mu_max;
+    out[3] = Te_max;
+    out[4] = ue_max + vth;
+}
+
+int orc_ndd_step(const cathode_ndd_params* pin, cathode_ndd_fields* f, int nsteps, int solver,
                  cathode_ndd_report* rep)
 {
     ctx_t Cx, *C = &Cx;
+    /* a copy whose dt check_CFL may set (flag_adp_dt = 1) */
+    cathode_ndd_params pl = *pin;
+    const cathode_ndd_params* p = &pl;
     if (rep) memset(rep, 0, sizeof(*rep));
     struct timespec t0, t1;
     clock_gettime(CLOCK_MONOTONIC, &t0);
@@ -650,6 +688,19 @@ int orc_ndd_step(const cathode_ndd_params* p, cathode_ndd_fields* f, int nsteps,
     double err3[3];
     sg_eq_t q;
     for (int step = 0; step < nsteps; step++) {
+        /* 0. time step (main.cpp:174): adaptive dt, or the CFL number of the fixed one */
+        double cfl[5];
+        orc_check_cfl(p, f, cfl);
+        if (pl.flag_adp_dt == 1) pl.dt = cfl[0] * pl.CFL;
+        else pl.CFL = pl.dt / cfl[0];
+        if (rep) {
+            rep->dt = pl.dt;
+            rep->CFL = pl.CFL;
+            rep->dt_min = cfl[0];
+            rep->rhoe_max = cfl[1];
+            rep->Te_max = cfl[3];
+            rep->ue_max = cfl[4];
+        }
         /* 1. ions */
         set_eq(p, f, SP_ION, NULL, &q);
         sg_routine(C, f, SP_ION, &q, rhoi_old, sx_old, sr_old, err3);

@@ -38,6 +38,7 @@ _SIGS = [
     ("orc_bern_bm", ctypes.c_double, [ctypes.c_double] * 3 + [ctypes.c_int]),
     ("orc_set_literal_bernoulli", None, [ctypes.c_int]),
     ("orc_plant", None, [ctypes.c_int]),
+    ("orc_check_cfl", None, [ctypes.POINTER(abi.NddParams), ctypes.POINTER(abi.NddFields), _D]),
     ("orc_struct5_direct", ctypes.c_int, [_D] * 6 + [_I] + [ctypes.c_int] * 4 + [_D]),
     ("orc_helm5_direct", ctypes.c_int, [_D] * 6 + [_I] + [ctypes.c_int] * 4 + [_D]),
     ("orc_lu9_direct", ctypes.c_int, [_D] * 10 + [_I] + [ctypes.c_int] * 4 + [_D]),
@@ -236,3 +237,11 @@ ORC_PLANT_SEE_DENSITY = 6
 ORC_PLANT_SEE_ION_SIGN = 7
 ORC_PLANT_SEE_RD_RATIO = 8
 ORC_PLANT_SEE_EMIT_ENERGY = 9
+
+
+def check_cfl(params, state):
+    """{dt_min, rhoe_max, mu_max, Te_max, ue_max} of the state (check_CFL)"""
+    out = np.zeros(5)
+    f = abi.fields_struct(state)
+    lib().orc_check_cfl(ctypes.byref(params), ctypes.byref(f), P(out))
+    return out

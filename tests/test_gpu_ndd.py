@@ -131,3 +131,33 @@ def test_pc_loop_depth_and_early_stop(hiplib, mode, learned, cnv):
     rep = gpu_steps(hiplib, p, st, nsteps)
     assert rep.nonconverged == 0
     compare(st, ref, FIELDS)
+
+
+def test_cfl_report_matches_oracle(hiplib):
+    """check_CFL before the last step (fixed dt): CFL number, dt_min and the
+    maxima it is built from, against the oracle's (fluidModule.hpp:57-115)."""
+    p, st = W.config0()
+    print("ndd config0 96x64 check_CFL report, 3 steps", flush=True)
+    ref = W.copy_state(st)
+    rr = O.ndd_step(p, ref, 3, solver=0)
+    rep = gpu_steps(hiplib, p, st, 3)
+    assert rep.dt == p.dt and rr.dt == p.dt
+    for k in ("CFL", "dt_min", "rhoe_max", "Te_max", "ue_max"):
+        g, o = getattr(rep, k), getattr(rr, k)
+        assert abs(g - o) <= 1e-9 * abs(o), (k, g, o)
+    assert rep.ue_max > 1e5  # the electron fluxes of the earlier steps enter
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_adaptive_dt_steps(hiplib, mode):
+    """flag_adp_dt = 1: every step takes dt = CFL eps0 / (e max n_e mu_e)."""
+    p, st = W.config0(mode=mode)
+    p.flag_adp_dt, p.CFL = 1, 0.3
+    print(f"ndd config0 96x64 adaptive dt CFL 0.3 mode={mode}, 3 steps", flush=True)
+    ref = W.copy_state(st)
+    rr = O.ndd_step(p, ref, 3, solver=0)
+    rep = gpu_steps(hiplib, p, st, 3)
+    assert rep.nonconverged == 0
+    assert abs(rep.dt - rr.dt) <= 1e-12 * rr.dt and rep.CFL == 0.3
+    assert abs(rr.dt / (0.3 * W.EPS0 / (W.E0 * rr.rhoe_max * p.mu_e)) - 1) < 1e-14
+    compare(st, ref, FIELDS)
