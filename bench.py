@@ -244,8 +244,8 @@ def main():
     ap.add_argument("--paths", default="fp64,mixed,learned",
                     help="fp64, mixed (FP32 V-cycles + FP64 refinement), learned (mixed + learned initial guess), "
                          "warm (mixed + previous-solution warm starts, no network: the learned path's ablation)")
-    ap.add_argument("--sweep", action="store_true",
-                    help="also run the 64-instance bias sweep (batched vs one at a time)")
+    ap.add_argument("--no-sweep", dest="sweep", action="store_false",
+                    help="skip the 64-instance bias sweep (batched vs one at a time), run by default")
     ap.add_argument("--sweep-steps", type=int, default=50)
     ap.add_argument("--sweep-workers", type=int, default=8)
     ap.add_argument("--calibrate-copy", action="store_true",

@@ -83,3 +83,55 @@ def test_batch_bitwise_matches_sequential(hiplib):
             assert rep.total_iterations == its, (trial, k, rep.total_iterations, its)
             for name in ("phi", "rhoe", "rhoi", "Te", "rhoeps"):
                 assert np.array_equal(out[name], ref[name]), (trial, k, name)
+
+
+def test_config3_full_sweep(hiplib):
+    """The bias-sweep workload as specified: 64 instances with grids drawn
+    from {64x32, 128x64, 256x128, 512x256} (0.1 mm cells), bias U[0, 60] V,
+    config-0 initial states, FP32 V-cycles with FP64 refinement.  Four steps
+    batched (8 workers) against one-at-a-time stepping for every instance
+    (bit for bit: fields, currents, iteration counts), and the collector
+    current and fields of every 64x32 and 128x64 member against the oracle
+    (its own multigrid to the same 1e-10 relative residual)."""
+    import oracle_lib as O
+
+    nsteps = 4
+    inst = W.config3()
+    sizes = [(p.ni, p.nj) for p, _, _ in inst]
+    assert len(inst) == 64 and set(sizes) <= set(W.SWEEP_SIZES) and len(set(sizes)) == 4
+    assert all(0.0 <= b <= 60.0 for _, _, b in inst)
+    print(f"config3: 64 instances, sizes {sorted(set(sizes))}, {nsteps} steps", flush=True)
+    seq = []
+    for p, st, _ in inst:
+        h = G.ndd_handle(hiplib, p, st)
+        rep = abi.NddReport()
+        abi.check(hiplib.cathode_ndd_step(h, nsteps, ctypes.byref(rep)), hiplib)
+        seq.append((G.ndd_state(hiplib, h, st), rep.I_anode, rep.total_iterations))
+        hiplib.cathode_ndd_destroy(h)
+    hs = [G.ndd_handle(hiplib, p, st) for p, st, _ in inst]
+    try:
+        reps = G.ndd_batch(hiplib, hs, nsteps, 8)
+        outs = [G.ndd_state(hiplib, h, st) for h, (_, st, _) in zip(hs, inst)]
+    finally:
+        for h in hs:
+            hiplib.cathode_ndd_destroy(h)
+    for k, ((ref, I_ref, its), out, rep) in enumerate(zip(seq, outs, reps)):
+        assert rep.steps == nsteps and rep.nonconverged == 0 and rep.max_rel_residual <= 1e-10, k
+        assert rep.I_anode == I_ref and rep.total_iterations == its, (k, rep.I_anode, I_ref)
+        for name in ("phi", "rhoe", "rhoi", "Ex", "Er", "rhoUex", "rhoUix"):
+            assert np.array_equal(out[name], ref[name]), (k, name)
+    O.lib().orc_mg_set_threads(8)
+    checked = 0
+    for k, (p, st, bias) in enumerate(inst):
+        if (p.ni, p.nj) not in ((64, 32), (128, 64)):
+            continue
+        ref = W.copy_state(st)
+        rr = O.ndd_step(p, ref, nsteps, solver=1)
+        assert rr.nonconverged == 0
+        e = max(rel_l2(outs[k][n], ref[n]) for n in ("phi", "rhoe", "rhoi", "rhoUex", "rhoUix"))
+        dI = abs(reps[k].I_anode - rr.I_anode) / abs(rr.I_anode)
+        assert e <= 1e-9 and dI <= 1e-9, (k, p.ni, p.nj, bias, e, dI)
+        checked += 1
+    print(f"config3: batched == one at a time for 64 instances; {checked} small members within 1e-9 "
+          f"of the oracle", flush=True)
+    assert checked >= 16
