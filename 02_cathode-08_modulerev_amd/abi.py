@@ -81,6 +81,7 @@ class NddReport(ctypes.Structure):
         ("seconds", ctypes.c_double),
         ("dt", ctypes.c_double), ("CFL", ctypes.c_double), ("dt_min", ctypes.c_double),
         ("rhoe_max", ctypes.c_double), ("Te_max", ctypes.c_double), ("ue_max", ctypes.c_double),
+        ("iterations_by_solve", ctypes.c_int * 8),
     ]
 
 

@@ -408,6 +408,9 @@ struct SetupLevels {
     WarmSet ws;
     const double* b;
     double* wpart;
+    // the operator is the one of the previous setup (only b changed): the
+    // scalars and the warm start's products, no level data
+    int rhs_only;
 };
 
 __global__ void __launch_bounds__(SETUP_NT) k_setup_levels(SetupLevels a, double* __restrict__ sc, int mixed)
@@ -451,6 +454,40 @@ __global__ void __launch_bounds__(SETUP_NT) k_setup_levels(SetupLevels a, double
         const int n = (l & 1) ? SETUP_STAGE1 : SETUP_STAGE0;
         return LevelStage{b, b + n, b + 2 * n, b + 3 * n, b + 4 * n, b + 5 * n, (l & 1) ? sa1 : sa0, i0, j0, cols};
     };
+    if (a.rhs_only) {
+        if (a.ws.k == 0) return;
+        double wd[WARM_NP];
+#pragma unroll
+        for (int q = 0; q < WARM_NP; q++) wd[q] = 0.0;
+        const int rows = a.ri * a.pi[0], cols = a.rj * a.pj[0];
+        const int i0 = 1 + (int)blockIdx.y * rows, j0 = 1 + (int)blockIdx.x * cols;
+        for (int q = tid; q < rows * cols; q += SETUP_NT) {
+            const int i = i0 + q / cols, j = j0 + q % cols;
+            if (i > a.ni || j > a.nj) continue;
+            const size_t k = (size_t)i * a.ld + j;
+            if (!a.act[k]) continue;
+            double ay[WARM_MAX];
+#pragma unroll
+            for (int q2 = 0; q2 < WARM_MAX; q2++)
+                ay[q2] = q2 < a.ws.k ? warm_ay(a.ws, q2, i, j, k, a.aP, a.aE, a.aW, a.aN, a.aS) : 0.0;
+            const double bk = a.b[k];
+#pragma unroll
+            for (int q2 = 0; q2 < WARM_MAX; q2++) {
+                wd[q2] += ay[q2] * bk;
+#pragma unroll
+                for (int p = 0; p <= q2; p++) wd[WARM_MAX + warm_gidx(p, q2)] += ay[p] * ay[q2];
+            }
+            wd[WARM_NP - 1] += bk * bk;
+        }
+        __shared__ double wl0[WARM_NP * (SETUP_NT / 64)];
+        block_sum_vec_bcast<SETUP_NT, WARM_NP>(wd, wl0);
+        if (tid == 0) {
+            const int nsb = (int)(gridDim.x * gridDim.y), bid = (int)(blockIdx.y * gridDim.x + blockIdx.x);
+#pragma unroll
+            for (int q = 0; q < WARM_NP; q++) a.wpart[q * nsb + bid] = wd[q];
+        }
+        return;
+    }
     bool staged = a.rj * a.pj[0] <= SETUP_STAGE_COLS;
     for (int l = 0; l <= a.nf; l++)
         staged = staged && a.ri * a.pi[l] * a.rj * a.pj[l] <= ((l & 1) ? SETUP_STAGE1 : SETUP_STAGE0);
@@ -1151,6 +1188,7 @@ void MGSolver::zero_buffers(hipStream_t s)
 void MGSolver::build(int ni, int nj)
 {
     drop_graphs();  // they hold the old buffers
+    setup_mode = setup_ni = setup_nj = -1;  // no hierarchy to reuse
     cfg_ni = ni;
     cfg_nj = nj;
     for (int q = 0; q < 2; q++) {
@@ -1279,11 +1317,20 @@ void MGSolver::setup(int mode, hipStream_t s, const WarmSet& ws)
     const size_t n0 = level_off[1] - level_off[0];
     const int mixed = mode == CATHODE_MODE_MIXED ? 1 : 0;
     const int nb = cdiv(F.nj, BX) * cdiv(F.ni, BY);
+    const bool act_changed = act_dirty;
     // level-0 activity lives in act0 (filled by the caller); mirror into actall
     if (act_dirty) {
         CATHODE_HIP(hipMemcpyAsync(actall.p, act0.p, n0, hipMemcpyDeviceToDevice, s));
         act_dirty = false;
     }
+    // the caller's promise that only b changed since the last setup, for the
+    // same precision: the levels, FP32 copies and coarsest inverse stay
+    const bool rhs_only = same_operator && setup_mode == mode && !act_changed && cfg_ni == setup_ni &&
+                          cfg_nj == setup_nj;
+    same_operator = false;
+    setup_mode = mode;
+    setup_ni = cfg_ni;
+    setup_nj = cfg_nj;
     double* sbuf = F.b;  // row excess of level 0 (F.b is rewritten by the outer loop)
     if (!prepared) {
         hipLaunchKernelGGL(k_l0_prepare, grid2(F.ni, F.nj), dim3(BX, BY), 0, s, F.ni, F.nj, F.ld, act0.p, F.aP, F.aE,
@@ -1292,12 +1339,12 @@ void MGSolver::setup(int mode, hipStream_t s, const WarmSet& ws)
     }
     prepared = false;
     const double* w = user_w;
-    if (!w) {
+    if (!w && !rhs_only) {
         hipLaunchKernelGGL(k_l0_weights, dim3(1), dim3(256), (F.nj + 2) * sizeof(double), s, F.ni, F.nj, F.ld, F.aN,
                            F.aS, w0.p);
         CATHODE_LAUNCH_CHECK();
-        w = w0.p;
     }
+    if (!w) w = w0.p;
     // scalars, level-0 mass and the first coarsenings: one launch; a
     // workgroup owns a 16 x 32 block of level 0 (fewer rows / columns where a
     // direction stops coarsening)
@@ -1327,9 +1374,11 @@ void MGSolver::setup(int mode, hipStream_t s, const WarmSet& ws)
             wpart.alloc((size_t)WARM_NP * warm_nb + 64);
             a.wpart = wpart.p;
         }
+        a.rhs_only = rhs_only ? 1 : 0;
         hipLaunchKernelGGL(k_setup_levels, sg, dim3(SETUP_NT), 0, s, a, scal.p, mixed);
         CATHODE_LAUNCH_CHECK();
     }
+    if (rhs_only) return;  // the levels and the coarsest inverse of the last setup stand
     // further fine levels: one launch each; the rest in one workgroup
     for (int l = nf; l + 1 < nl && l < lt; l++) {
         Level<double>& C = L64[l + 1];

@@ -163,6 +163,12 @@ public:
         return old;
     }
     bool act_dirty = true;
+    // The caller's promise that the next solve's operator (aP..aS and the
+    // activity) is bitwise the one of the last setup, in the same precision,
+    // and only the right-hand side changed (the later passes of a
+    // predictor-corrector loop): that setup's levels, FP32 copies and
+    // coarsest inverse are reused.  Clears itself.
+    bool same_operator = false;
     // The caller's kernel already did the level-0 preparation for the next
     // solve (couplings to inactive cells zeroed, row excess into prep_s(),
     // max|aP| and |b|^2 block partials into prep_part() in the 64 x 4 block
@@ -259,6 +265,7 @@ private:
     double* h_scal = nullptr;  // pinned mirror of scal (arena mode: the caller's host block)
     size_t h_scal_n = 0;
     bool h_owned = true;       // h_scal came from hipHostMalloc
+    int setup_mode = -1, setup_ni = -1, setup_nj = -1;  // precision and shape of the last full setup
     // arena mode (configure_in): buffers are pieces of the caller's region
     Arena* arena = nullptr;    // &region in arena mode; a counting arena while sizing a shape
     Arena region;

@@ -1124,6 +1124,7 @@ static bool solve(NddHandle& H, double gate, int maxITR, double tol, int slot)
     H.mg.side_cnv = -INFINITY;
     if (H.mg.aborted) return false;
     H.last_its[slot] = st.iterations;
+    H.rep.iterations_by_solve[slot] += st.iterations;
     account(H, st);
     return true;
 }
@@ -1304,6 +1305,9 @@ static void poisson_routine(NddHandle& H, bool report)
         const bool learn = ncount == 1 && (use_net || H.cap_feat);
         if (learn) learned_guess(H, use_net);
         else warm_guess(H, warm_slot(EQ_POISSON, ncount));
+        // the later passes' operator depends only on n_e, mu and dt, which the
+        // routine does not change: the first pass's hierarchy is reused
+        H.mg.same_operator = ncount > 1;
         if (!solve(H, ncount > 1 ? H.p.error_cnv_HES_phi : -INFINITY, H.p.maxITR_SOR_phi, H.p.error_cnv_SOR_phi,
                    warm_slot(EQ_POISSON, ncount))) {
             // calcRes of the previous iteration met the PC criterion: undo the prediction
@@ -1387,6 +1391,10 @@ static void sg_routine(NddHandle& H, int sp, bool report, double* err3)
         CATHODE_LAUNCH_CHECK();
         const int eq = sp == SP_ELEC ? EQ_ELEC : (sp == SP_ION ? EQ_ION : EQ_ENERGY);
         warm_guess(H, warm_slot(eq, ncount));
+        // the later passes read back the first pass's face weights (the
+        // fields and temperatures they depend on do not change in the
+        // routine), so their operator is bitwise the first pass's
+        H.mg.same_operator = ncount > 1;
         if (!solve(H, ncount > 1 ? cnv : -INFINITY, lin_maxit, lin_tol, warm_slot(eq, ncount))) {
             d2d(H, {{A.Fx, S[1]}, {A.Fr, S[2]}});
             break;
@@ -1749,6 +1757,7 @@ int cathode_ndd_batch_step(void** handles, int n, int nsteps, int workers, catho
                             r.steps += first[k].steps;
                             r.solves += first[k].solves;
                             r.total_iterations += first[k].total_iterations;
+                            for (int q = 0; q < 8; q++) r.iterations_by_solve[q] += first[k].iterations_by_solve[q];
                             r.max_iterations = std::max(r.max_iterations, first[k].max_iterations);
                             r.nonconverged += first[k].nonconverged;
                             r.max_rel_residual = std::max(r.max_rel_residual, first[k].max_rel_residual);

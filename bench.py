@@ -69,6 +69,8 @@ def run_path(lib, mode, steps, warmup, seed, net=None, warm=False):
     t1 = time.perf_counter()
     wall = t1 - t0
     its = rep.total_iterations
+    names = ["electrons", "electrons_pc", "ions", "ions_pc", "energy", "energy_pc", "poisson", "poisson_pc"]
+    by_solve = {nm: rep.iterations_by_solve[q] / steps for q, nm in enumerate(names)}
     solves = rep.solves
     nonconv = rep.nonconverged
     maxres = rep.max_rel_residual
@@ -104,6 +106,7 @@ def run_path(lib, mode, steps, warmup, seed, net=None, warm=False):
         "smooth_GBps": bpl.value / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0,
         "smooth_tile": f"{ti.value}x{tj.value}/{nt.value}",
         "steps_total": warmup + steps + nprof,
+        "iterations_per_step_by_solve": {k: round(v, 3) for k, v in by_solve.items()},
     }, st
 
 

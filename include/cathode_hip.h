@@ -245,6 +245,10 @@ typedef struct cathode_ndd_report {
      * over every cell, max |Gamma_e| / n_e over the interior plus the thermal
      * speed at max Te (computed as the reference does; it sets nothing) */
     double dt, CFL, dt_min, rhoe_max, Te_max, ue_max;
+    /* iterations summed over the call's steps per solve of a step:
+     * [2 eq + pass], eq 0 electrons, 1 ions, 2 energy, 3 Poisson; pass 0 the
+     * first PC pass, 1 the later ones */
+    int iterations_by_solve[8];
 } cathode_ndd_report;
 
 int cathode_ndd_create(const cathode_ndd_params* p, void** handle);
