@@ -85,6 +85,64 @@ class NddReport(ctypes.Structure):
     ]
 
 
+MW_MAX_BLOCKS = 16
+
+
+class MwParams(ctypes.Structure):
+    _fields_ = [
+        ("ni", ctypes.c_int), ("nj", ctypes.c_int),
+        ("dx", ctypes.c_double), ("dr", ctypes.c_double), ("rmin", ctypes.c_double),
+        ("omega", ctypes.c_double), ("masse", ctypes.c_double),
+        ("n_bl", ctypes.c_int),
+        ("cell_box", ctypes.c_int * (4 * MW_MAX_BLOCKS)), ("xface_box", ctypes.c_int * (4 * MW_MAX_BLOCKS)),
+        ("rface_box", ctypes.c_int * (4 * MW_MAX_BLOCKS)),
+        ("flag_mwRef", ctypes.c_int), ("Pmw", ctypes.c_double), ("Z0", ctypes.c_double),
+        ("r1", ctypes.c_double), ("r2", ctypes.c_double),
+        ("S11", ctypes.c_double * 2), ("S12", ctypes.c_double * 2), ("S21", ctypes.c_double * 2),
+        ("S22", ctypes.c_double * 2),
+        ("J1r_exc", ctypes.c_double),
+        ("i_ref", ctypes.c_int), ("j_out", ctypes.c_int), ("i_out_lo", ctypes.c_int), ("i_out_hi", ctypes.c_int),
+        ("j_in_lo", ctypes.c_int), ("j_in_hi", ctypes.c_int),
+    ]
+
+
+MW_FIELD_NAMES = ["rhoe", "nu_m", "Bx", "Br", "E1x", "E1r", "E1p", "J1x", "J1r", "J1p",
+                  "J1x_exc", "J1r_exc", "J1p_exc", "Pabsx", "Pabsr", "Pabsp", "Pabs"]
+
+
+class MwFields(ctypes.Structure):
+    _fields_ = [(n, c_double_p) for n in MW_FIELD_NAMES]
+
+
+class MwReport(ctypes.Structure):
+    _fields_ = [
+        ("Px", ctypes.c_double), ("Pr", ctypes.c_double), ("Pp", ctypes.c_double),
+        ("power_sum", ctypes.c_double), ("P_out", ctypes.c_double), ("P_in", ctypes.c_double),
+        ("V_ref", ctypes.c_double * 2), ("I_ref", ctypes.c_double * 2), ("Z_ref", ctypes.c_double * 2),
+        ("P_fwd", ctypes.c_double), ("P_ref", ctypes.c_double),
+        ("P_fwd_all", ctypes.c_double), ("P_ref_all", ctypes.c_double), ("P_err", ctypes.c_double),
+        ("ratio", ctypes.c_double), ("ratio_eff", ctypes.c_double),
+        ("Gamma_p", ctypes.c_double * 2), ("Zp", ctypes.c_double * 2), ("Gamma_all", ctypes.c_double * 2),
+        ("Zall", ctypes.c_double * 2),
+        ("J1r_exc", ctypes.c_double),
+    ]
+
+
+def mw_fields_struct(arrays):
+    """MwFields pointing into a dict of numpy arrays (complex arrays as
+    complex128; missing keys -> NULL)."""
+    f = MwFields()
+    for n in MW_FIELD_NAMES:
+        a = arrays.get(n)
+        if a is None:
+            setattr(f, n, None)
+            continue
+        if np.iscomplexobj(a):
+            a = a.view(np.float64)
+        setattr(f, n, dptr(a))
+    return f
+
+
 class EthetaParams(ctypes.Structure):
     _fields_ = [("ni", ctypes.c_int), ("nj", ctypes.c_int), ("dx", ctypes.c_double),
                 ("dr", ctypes.c_double), ("rmin", ctypes.c_double), ("omega", ctypes.c_double),
@@ -126,6 +184,8 @@ SIGNATURES = [
     ("cathode_struct5_solve_dev", ctypes.c_int,
      _SOLVE5 + [ctypes.c_int, ctypes.c_double, ctypes.c_int, _D, ctypes.c_void_p,
                 ctypes.POINTER(Status)]),
+    ("cathode_mw_energy_profile", ctypes.c_int,
+     [ctypes.POINTER(MwParams), ctypes.POINTER(MwFields), ctypes.POINTER(MwReport)]),
     ("cathode_struct5_solve_dev_noalloc", ctypes.c_int,
      _SOLVE5 + [ctypes.c_int, ctypes.c_double, ctypes.c_int, _D, ctypes.c_void_p,
                 ctypes.POINTER(Status)]),

@@ -230,6 +230,7 @@ int cathode_finalize(void)
     g_actbuf.release();
     sor_release();
     helm_release();
+    mw_release();
     if (c.stream) (void)hipStreamDestroy(c.stream);
     c.stream = nullptr;
     c.ready = false;

@@ -5,5 +5,7 @@ namespace cathode {
 
 // Frees the complex solver and its staging buffers (cathode_finalize).
 void helm_release();
+// frees the microwave power-profile buffers (mw.hip)
+void mw_release();
 
 }  // namespace cathode

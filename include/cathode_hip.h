@@ -176,6 +176,81 @@ int cathode_etheta_solve(const cathode_etheta_params* p, const double* ne,
                          double* P_total, cathode_status* st);
 
 /* ------------------------------------------------------------------ */
+/* microwave power profile and port normalisation                     */
+/* (EmfieldModule::update_energy_profile, include/emfieldModule.hpp:  */
+/*  7630-8105, with mWLoss_at_boundary_xL / rR, :8488-8668)           */
+/* ------------------------------------------------------------------ */
+#define CATHODE_MW_MAX_BLOCKS 16
+
+/* Inclusive index boxes of the blocks (the reference's gc.i/j_flc_bl,
+ * gx.i/j_flx_bl, gr.i/j_flr_bl: cells, x faces, r faces), 4 ints per block:
+ * {ilo, ihi, jlo, jhi}.  Every loop of the reference runs over the blocks in
+ * order, so a cell or face in two boxes counts twice in the power sums, as
+ * there. */
+typedef struct cathode_mw_params {
+    int ni, nj;
+    double dx, dr, rmin;
+    double omega;                /* Params::omegam = 2 pi fmw [rad/s] */
+    double masse;                /* [kg] */
+    int n_bl;                    /* blocks; the last one is the coaxial (dielectric) feed */
+    int cell_box[4 * CATHODE_MW_MAX_BLOCKS];
+    int xface_box[4 * CATHODE_MW_MAX_BLOCKS];
+    int rface_box[4 * CATHODE_MW_MAX_BLOCKS];
+    int flag_mwRef;              /* 1: normalise the forward power at the amplifier through the
+                                    port's S-parameters; 0: normalise the absorbed power */
+    double Pmw;                  /* microwave power [W] */
+    double Z0;                   /* line impedance [ohm] */
+    double r1, r2;               /* inner / outer radius of the coaxial feed [m] */
+    double S11[2], S12[2], S21[2], S22[2];   /* port S-parameters (re, im), Params::S11..S22 */
+    double J1r_exc;              /* excitation amplitude, rescaled like the reference's pm.J1r_exc */
+    int i_ref;                   /* x face of the reference plane (int(x_ref/dx + 0.5) + 1, x_ref = 0) */
+    int j_out, i_out_lo, i_out_hi;  /* cells (i_out_lo..i_out_hi, j_out) under the outer boundary
+                                       whose Poynting flux is the loss P_out (gc.j_flc_bl[4][1] - 1,
+                                       gc.i_flc_bl[4][0..1]) */
+    int j_in_lo, j_in_hi;        /* cells of the reference plane for the Poynting input P_in
+                                    (gc.j_flc_bl[n_bl-1][0..1]) */
+} cathode_mw_params;
+
+/* Host arrays, each (ni+2)*(nj+2) values, ld = nj+2; complex arrays are
+ * interleaved (re, im) doubles.  In: rhoe, nu_m (electron momentum
+ * collision frequency nu_m1), Bx, Br (cells) and the field E1x (x faces),
+ * E1r (r faces), E1p (cells).  In/out, written inside the blocks' boxes and
+ * kept elsewhere as the reference's persistent members: J1x, J1r, J1p =
+ * sigma E with the magnetised cold-plasma conductivity tensor, Pabsx, Pabsr,
+ * Pabsp = 0.5 Re(J E*) per component, Pabs (cells) = the face averages plus
+ * Pabsp.  Rescaled on the whole grid: Pabs (both branches); with
+ * flag_mwRef = 1 also E1x, E1r, E1p, J1x/r/p and the excitation currents
+ * J1x_exc, J1r_exc, J1p_exc (may be NULL). */
+typedef struct cathode_mw_fields {
+    const double *rhoe, *nu_m, *Bx, *Br;
+    double *E1x, *E1r, *E1p;
+    double *J1x, *J1r, *J1p;
+    double *J1x_exc, *J1r_exc, *J1p_exc;
+    double *Pabsx, *Pabsr, *Pabsp, *Pabs;
+} cathode_mw_fields;
+
+typedef struct cathode_mw_report {
+    double Px, Pr, Pp;           /* absorbed power per component [W] (before normalisation) */
+    double power_sum;            /* absorbed power after normalisation [W] */
+    double P_out, P_in;          /* Poynting loss through the outer boundary (after normalisation);
+                                    Poynting input at the reference plane (diagnostic, before) [W] */
+    double V_ref[2], I_ref[2], Z_ref[2];   /* voltage, current, impedance at the reference plane */
+    double P_fwd, P_ref;         /* forward / reflected power at the cathode port [W] */
+    double P_fwd_all, P_ref_all; /* the same at the amplifier, through the S-parameters [W] */
+    double P_err;                /* P_fwd - P_ref - absorbed - P_out, folded into the absorbed power */
+    double ratio, ratio_eff;     /* power ratio Pmw / P_fwd_all (or Pmw / absorbed) and the field scale^2 */
+    double Gamma_p[2], Zp[2], Gamma_all[2], Zall[2];  /* reflection coefficients and impedances */
+    double J1r_exc;              /* the rescaled excitation amplitude */
+} cathode_mw_report;
+
+/* Replaces EmfieldModule::update_energy_profile (emfieldModule.hpp:7630-8105):
+ * J = sigma E and the absorbed power on the device, then the normalisation of
+ * the flag_mwRef branch (reference-plane V, I and Poynting flux, S-parameter
+ * transfer to the amplifier, Pmw / P_fwd_all with the power error folded into
+ * the absorbed power) or of the plain branch (Pmw / absorbed power). */
+int cathode_mw_energy_profile(const cathode_mw_params* p, cathode_mw_fields* f, cathode_mw_report* rep);
+
+/* ------------------------------------------------------------------ */
 /* device-resident implicit NDD step                                  */
 /* (FluidModule::solve_phi_couple_wdTe_wSEE_PC, solve_rhoe_wdTe_wSEE_PC,
  *  solve_Te_wdTe_wSEE_PC, include/fluidModule.hpp:2080-4957, with the ion

@@ -43,8 +43,28 @@
  *   orc_ndd_assemble (SEE)                constant yields: emitted particle / energy wall
  *                                         flux = yield x primary (ion flux, thermal flux of
  *                                         n_e at the step start, thermal energy flux)
+ *   orc_mw_energy_profile                 J = sigma E against the 3x3 solve of the cold
+ *                                         electron momentum equation m(nu + i w) v = -e(E +
+ *                                         v x B); no absorption when nu = 0; B = 0 absorbed
+ *                                         power = Drude Re(sigma) |E|^2 / 2 over the exact
+ *                                         volumes; TEM coax wave: the reference-plane
+ *                                         Poynting input, V, I and Z in closed form
+ *                                         (discrete sin / tan factors, -> |V|^2 / 2Z and
+ *                                         eta ln(b/a) / 2 pi); smooth fields: both Poynting
+ *                                         fluxes converge at second order to the quadrature
+ *                                         of the continuous flux (H from Faraday's law); port:
+ *                                         P_fwd - P_ref = Re(V conj(I)) / 2, Zp = V / I,
+ *                                         Gamma_all = S11 + S12 S21 Gp / (1 - S22 Gp), the
+ *                                         amplifier's forward power = Pmw after scaling and
+ *                                         absorbed + boundary loss = P_fwd - P_ref
+ *                                         (tests/test_oracle_mw.py).  The phi term of the
+ *                                         x-face Poynting flux follows the reference's
+ *                                         E1p_tmp * conj(H1p_tmp) (emfieldModule.hpp:8518) as
+ *                                         written; it is pinned only by the device matching
+ *                                         it, every pin above holds E_phi = 0 there
  *   orc_plant                             plants one named mistake; each check above that
- *                                         covers it then fails (tests/test_oracle_pins.py)
+ *                                         covers it then fails (tests/test_oracle_pins.py,
+ *                                         tests/test_oracle_mw.py)
  *   orc_abcd_*                            S -> ABCD -> S round trip is the identity
  * See tests/test_oracle_*.py for each check.
  */
@@ -86,9 +106,14 @@ enum {
     ORC_PLANT_SEE_DENSITY = 6,        /* EISEE from n_e instead of n_e at the step start */
     ORC_PLANT_SEE_ION_SIGN = 7,       /* IISEE with the wrong sign */
     ORC_PLANT_SEE_RD_RATIO = 8,       /* energy EISEE without the rediffused energy ratio */
-    ORC_PLANT_SEE_EMIT_ENERGY = 9     /* emitted energy e T_emit instead of 2 e T_emit */
+    ORC_PLANT_SEE_EMIT_ENERGY = 9,    /* emitted energy e T_emit instead of 2 e T_emit */
+    ORC_PLANT_MW_HALL_SIGN = 10,      /* x-row phi entry of the conductivity with -h_r */
+    ORC_PLANT_MW_POYNTING_CONJ = 11,  /* Poynting fluxes with E H instead of E conj(H) */
+    ORC_PLANT_MW_PORT_WAVES = 12,     /* forward and reflected waves at the port swapped */
+    ORC_PLANT_MW_RFACE_WEIGHT = 13    /* r-face power volume weighted with the cell radius */
 };
 void orc_plant(int which);
+int orc_planted(void);
 
 /* ---- 5-point solves with the HYPRE box semantics (solverFuncs.hpp:26-162) ---- */
 /* Exact direct solve (banded LU, no pivoting: the SG/Poisson matrices are
@@ -143,6 +168,10 @@ int orc_ndd_step(const cathode_ndd_params* p, cathode_ndd_fields* f,
 int orc_ndd_assemble(const cathode_ndd_params* p, const cathode_ndd_fields* f,
                      int which, double* aP, double* aE, double* aW, double* aN,
                      double* aS, double* b, double* fx, double* fr);
+
+/* EmfieldModule::update_energy_profile (emfieldModule.hpp:7630-8105): J = sigma E,
+ * absorbed power, Poynting fluxes and the port normalisation (orc_mw.c) */
+int orc_mw_energy_profile(const cathode_mw_params* p, cathode_mw_fields* f, cathode_mw_report* rep);
 
 /* check_CFL (fluidModule.hpp:57-115): out = {dt_min, max n_e, max mu, max Te,
  * max |Gamma_e|/n_e + vth(max Te)} of the state */

@@ -37,6 +37,7 @@ void orc_set_literal_bernoulli(int on) { g_literal = on; }
  * the error it guards against (ORC_PLANT_*, oracle.h); 0 = the restatement. */
 static int g_plant = 0;
 void orc_plant(int which) { g_plant = which; }
+int orc_planted(void) { return g_plant; }
 
 double orc_bern_bp(double vd, double z, double D_over_h, int literal)
 {

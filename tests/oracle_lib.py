@@ -38,6 +38,8 @@ _SIGS = [
     ("orc_bern_bm", ctypes.c_double, [ctypes.c_double] * 3 + [ctypes.c_int]),
     ("orc_set_literal_bernoulli", None, [ctypes.c_int]),
     ("orc_plant", None, [ctypes.c_int]),
+    ("orc_mw_energy_profile", ctypes.c_int,
+     [ctypes.POINTER(abi.MwParams), ctypes.POINTER(abi.MwFields), ctypes.POINTER(abi.MwReport)]),
     ("orc_check_cfl", None, [ctypes.POINTER(abi.NddParams), ctypes.POINTER(abi.NddFields), _D]),
     ("orc_struct5_direct", ctypes.c_int, [_D] * 6 + [_I] + [ctypes.c_int] * 4 + [_D]),
     ("orc_helm5_direct", ctypes.c_int, [_D] * 6 + [_I] + [ctypes.c_int] * 4 + [_D]),
@@ -237,6 +239,10 @@ ORC_PLANT_SEE_DENSITY = 6
 ORC_PLANT_SEE_ION_SIGN = 7
 ORC_PLANT_SEE_RD_RATIO = 8
 ORC_PLANT_SEE_EMIT_ENERGY = 9
+ORC_PLANT_MW_HALL_SIGN = 10
+ORC_PLANT_MW_POYNTING_CONJ = 11
+ORC_PLANT_MW_PORT_WAVES = 12
+ORC_PLANT_MW_RFACE_WEIGHT = 13
 
 
 def check_cfl(params, state):
@@ -245,3 +251,12 @@ def check_cfl(params, state):
     f = abi.fields_struct(state)
     lib().orc_check_cfl(ctypes.byref(params), ctypes.byref(f), P(out))
     return out
+
+
+def mw_energy_profile(params, arrays):
+    """the oracle's update_energy_profile on a dict of arrays (modified in place)"""
+    f = abi.mw_fields_struct(arrays)
+    rep = abi.MwReport()
+    rc = lib().orc_mw_energy_profile(ctypes.byref(params), ctypes.byref(f), ctypes.byref(rep))
+    assert rc == 0
+    return rep
