@@ -67,6 +67,26 @@ class NddFields(ctypes.Structure):
     _fields_ = [(n, c_double_p) for n in FIELD_NAMES]
 
 
+NDD_MAX_BLOCKS = 8
+NDD_MAX_SEGS = 24
+SEG_X_LOW, SEG_X_HIGH, SEG_R_LOW, SEG_R_HIGH, SEG_R_OPEN, SEG_R_AXIS = range(6)
+
+
+class NddSegment(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("cell", ctypes.c_int), ("lo", ctypes.c_int), ("hi", ctypes.c_int),
+                ("flag", ctypes.c_int)]
+
+
+class NddGeometry(ctypes.Structure):
+    _fields_ = [
+        ("n_bl", ctypes.c_int),
+        ("cell_box", ctypes.c_int * (4 * NDD_MAX_BLOCKS)), ("xface_box", ctypes.c_int * (4 * NDD_MAX_BLOCKS)),
+        ("rface_box", ctypes.c_int * (4 * NDD_MAX_BLOCKS)),
+        ("n_seg", ctypes.c_int), ("n_ne", ctypes.c_int), ("n_te", ctypes.c_int),
+        ("seg", NddSegment * NDD_MAX_SEGS), ("ne", NddSegment * NDD_MAX_SEGS), ("te", NddSegment * NDD_MAX_SEGS),
+    ]
+
+
 class NddReport(ctypes.Structure):
     _fields_ = [
         ("steps", ctypes.c_int), ("solves", ctypes.c_int), ("total_iterations", ctypes.c_int),
@@ -219,6 +239,10 @@ SIGNATURES = [
                                                   ctypes.c_void_p, ctypes.c_void_p, ctypes.c_float,
                                                   ctypes.POINTER(ctypes.c_double)]),
     ("cathode_ndd_set_learned", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    ("cathode_ndd_set_geometry", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(NddGeometry)]),
+    ("cathode_ndd_reference_geometry", ctypes.c_int,
+     [c_double_p, c_double_p, ctypes.c_double, ctypes.c_double, ctypes.POINTER(NddGeometry), c_int_p, c_int_p,
+      c_int_p]),
     ("cathode_ndd_capture", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     ("cathode_ndd_profile", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("cathode_ndd_profile_read", ctypes.c_int,

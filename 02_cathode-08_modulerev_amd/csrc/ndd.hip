@@ -33,6 +33,7 @@
 #include <memory>
 #include <vector>
 
+#include "geometry.hpp"
 #include "learned.hpp"
 #include "mg.hpp"
 
@@ -170,90 +171,180 @@ __device__ inline double ion_wall_gpn(const NddView& V, double U, double sign)
     const int j = (j0) + (int)(blockIdx.x * NB_X + threadIdx.x); \
     const int i = (i0) + (int)(blockIdx.y * NB_Y + threadIdx.y);
 
+// ================================================================== geometry
+// Face type codes (geometry.hpp) of x face (i, j) and r face (i, j), the
+// plasma cells, the potential ghost rules (-1 or (source << 2) | (Dirichlet
+// << 1) | bias, source a field index) and the ghost writes after the
+// electron and energy routines.  GeoRect: the rectangle 1..ni x 1..nj with
+// the block-4 boundary set (left wall phi = 0, right electrode at V_bias, open
+// top, axis below), from the indices.  GeoMask: the tables of a block
+// geometry (cathode_ndd_set_geometry), pitch ld.
+struct GeoRect {
+    __device__ __forceinline__ int xf(const NddView& V, int i, int j) const
+    {
+        return i == 1 ? GF_WLO : (i == V.ni + 1 ? (GF_WHI | GF_BIAS) : GF_INNER);
+    }
+    __device__ __forceinline__ int rf(const NddView& V, int i, int j) const
+    {
+        return j == 1 ? GF_AXIS : (j == V.nj + 1 ? GF_OPEN : GF_INNER);
+    }
+    // cell (i, j) of the interior 1..ni x 1..nj is plasma
+    __device__ __forceinline__ bool act(const NddView&, int, int) const { return true; }
+    // any cell of the extended grid
+    __device__ __forceinline__ bool act_nb(const NddView& V, int i, int j) const
+    {
+        return i >= 1 && i <= V.ni && j >= 1 && j <= V.nj;
+    }
+    __device__ __forceinline__ int ghost(const NddView& V, int i, int j) const
+    {
+        const bool in_i = i >= 1 && i <= V.ni, in_j = j >= 1 && j <= V.nj;
+        if (i == 0 && in_j) return (int)((IX(1, j) << 2) | 2);
+        if (i == V.ni + 1 && in_j) return (int)((IX(V.ni, j) << 2) | 3);
+        if (j == V.nj + 1 && in_i) return (int)(IX(i, V.nj) << 2);
+        return -1;
+    }
+    __device__ __forceinline__ int te_bits(const NddView& V, int i, int j) const
+    {
+        return (i == 1 ? (GW_XLO | (GW_XLO << 4)) : 0) | (i == V.ni ? (GW_XHI | (GW_XHI << 4)) : 0) |
+               (j == 1 ? (GW_RLO | (GW_RLO << 4)) : 0) | (j == V.nj ? (GW_RHI | (GW_RHI << 4)) : 0);
+    }
+    __device__ __forceinline__ int ne_bits(const NddView&, int, int) const { return 0; }
+};
+
+struct GeoMask {
+    const uint8_t *xc, *rc, *ac, *teb, *neb;
+    const int* gs;
+    __device__ __forceinline__ int xf(const NddView& V, int i, int j) const { return xc[IX(i, j)]; }
+    __device__ __forceinline__ int rf(const NddView& V, int i, int j) const { return rc[IX(i, j)]; }
+    __device__ __forceinline__ bool act(const NddView& V, int i, int j) const { return ac[IX(i, j)] != 0; }
+    __device__ __forceinline__ bool act_nb(const NddView& V, int i, int j) const { return ac[IX(i, j)] != 0; }
+    __device__ __forceinline__ int ghost(const NddView& V, int i, int j) const { return gs[IX(i, j)]; }
+    __device__ __forceinline__ int te_bits(const NddView& V, int i, int j) const { return teb[IX(i, j)]; }
+    __device__ __forceinline__ int ne_bits(const NddView& V, int i, int j) const { return neb[IX(i, j)]; }
+};
+
+// masks of the transverse-flux interpolation: 0 across a wall (and the axis
+// for r faces), as the reference's bLr / bRr / bLx / bRx flags
+__device__ __forceinline__ double tmask_r(int code)
+{
+    const int t = code & GF_TYPE;
+    return (double)(t != GF_WLO && t != GF_WHI && t != GF_AXIS);
+}
+__device__ __forceinline__ double tmask_x(int code)
+{
+    const int t = code & GF_TYPE;
+    return (double)(t != GF_WLO && t != GF_WHI);
+}
+__device__ __forceinline__ bool is_wall(int t) { return t == GF_WLO || t == GF_WHI; }
+
 // ================================================================== Poisson
 struct PoissonArgs {
-    const double *phi, *rhoe, *rhoi, *Te, *rhoe_old, *Ex, *Er, *Ex_old, *rhoUix, *S;
+    const double *phi, *rhoe, *rhoi, *Te, *rhoe_old, *Ex, *Er, *Ex_old, *Er_old, *rhoUix, *rhoUir, *S;
     const double *nUex_t, *nUer_t;
     double *nUex, *nUer;
     Mob m;
 };
 
-// predicted x flux on face (i, j), i = 1..ni+1 (fluidModule.hpp:2266-2462)
-__device__ __forceinline__ double poisson_flux_x(const NddView& V, const PoissonArgs& A, int i, int j)
+// predicted x flux on face (i, j) (fluidModule.hpp:2266-2462): inner faces
+// the drift-diffusion flux plus the mobility-tensor correction, wall faces
+// the thermal + emission wall flux of the plasma cell beside the wall
+template <class G>
+__device__ __forceinline__ double poisson_flux_x(const G& g, const NddView& V, const PoissonArgs& A, int i, int j)
 {
 #pragma clang fp contract(off)
     const double dx = V.dx;
     const double* r = V.r;
-    if (i >= 2 && i <= V.ni) {
+    const int t = g.xf(V, i, j) & GF_TYPE;
+    if (t == GF_INNER) {
         const double rL = (r[j] + r[j - 1]) / 2.0, rR = (r[j] + r[j + 1]) / 2.0;
         const double n_f = (A.rhoe[IX(i, j)] + A.rhoe[IX(i - 1, j)]) / 2.0;
         const double dphidx = (A.phi[IX(i, j)] - A.phi[IX(i - 1, j)]) / dx;
         const double dpedx =
             K_BOLTZ * (A.rhoe[IX(i, j)] * A.Te[IX(i, j)] - A.rhoe[IX(i - 1, j)] * A.Te[IX(i - 1, j)]) / dx;
-        const double bLr = (double)(j != 1), bLrD = (double)(j != 1), bRr = 1.0, bRrD = 1.0;
+        const double bLr = tmask_r(g.rf(V, i, j)), bLrD = tmask_r(g.rf(V, i - 1, j));
+        const double bRr = tmask_r(g.rf(V, i, j + 1)), bRrD = tmask_r(g.rf(V, i - 1, j + 1));
         const double tr = (rR * A.nUer_t[IX(i, j + 1)] * bRr + rL * A.nUer_t[IX(i, j)] * bLr +
                            rR * A.nUer_t[IX(i - 1, j + 1)] * bRrD + rL * A.nUer_t[IX(i - 1, j)] * bLrD) /
                           (4.0 * r[j]);
         return A.nUex_t[IX(i, j)] - (A.m.inv_par * A.nUex_t[IX(i, j)] + A.m.inv_cross * tr) * A.m.mu_star +
                            A.m.mu_star * (n_f * dphidx - dpedx / K_E0);
-    } else if (i == 1) {
-        const double Te = A.Te[IX(1, j)];
-        const double dphi = fmax(-A.Ex_old[IX(1, j)] * dx / 2.0, 0.0);
+    } else if (t == GF_WLO) {
+        const double Te = A.Te[IX(i, j)];
+        const double dphi = fmax(-A.Ex_old[IX(i, j)] * dx / 2.0, 0.0);
         const double gpn = wall_gpn(V, Te, dphi, -1.0);
-        const double see = -V.coefIISEE_ts * A.rhoUix[IX(1, j)] - eisee_sum(V, Te) * A.rhoe_old[IX(1, j)] * gpn;
-        return A.rhoe[IX(1, j)] * gpn + see;
+        const double see = -V.coefIISEE_ts * A.rhoUix[IX(i, j)] - eisee_sum(V, Te) * A.rhoe_old[IX(i, j)] * gpn;
+        return A.rhoe[IX(i, j)] * gpn + see;
     } else {
-        const int c = V.ni;
+        const int c = i - 1;
         const double Te = A.Te[IX(c, j)];
-        const double dphi = fmax(A.Ex_old[IX(c + 1, j)] * dx / 2.0, 0.0);
+        const double dphi = fmax(A.Ex_old[IX(i, j)] * dx / 2.0, 0.0);
         const double gpn = wall_gpn(V, Te, dphi, 1.0);
-        const double see = -V.coefIISEE_ts * A.rhoUix[IX(c + 1, j)] - eisee_sum(V, Te) * A.rhoe_old[IX(c, j)] * gpn;
+        const double see = -V.coefIISEE_ts * A.rhoUix[IX(i, j)] - eisee_sum(V, Te) * A.rhoe_old[IX(c, j)] * gpn;
         return A.rhoe[IX(c, j)] * gpn + see;
     }
 }
 
-// predicted r flux on face (i, j), j = 2..nj+1 (fluidModule.hpp:2466-2621)
-__device__ __forceinline__ double poisson_flux_r(const NddView& V, const PoissonArgs& A, int i, int j)
+// predicted r flux on face (i, j) (fluidModule.hpp:2466-2621): inner faces
+// as for x, wall faces the wall flux, the open face the ion flux
+template <class G>
+__device__ __forceinline__ double poisson_flux_r(const G& g, const NddView& V, const PoissonArgs& A, int i, int j)
 {
 #pragma clang fp contract(off)
     const double dr = V.dr;
     const double* r = V.r;
-    if (j <= V.nj) {
+    const int t = g.rf(V, i, j) & GF_TYPE;
+    if (t == GF_INNER) {
         const double r_t = (r[j] + r[j - 1]) / 2.0;
         const double n_f = (A.rhoe[IX(i, j)] + A.rhoe[IX(i, j - 1)]) / 2.0;
         const double dphidr = (A.phi[IX(i, j)] - A.phi[IX(i, j - 1)]) / dr;
         const double dpedr =
             K_BOLTZ * (A.rhoe[IX(i, j)] * A.Te[IX(i, j)] - A.rhoe[IX(i, j - 1)] * A.Te[IX(i, j - 1)]) / dr;
-        const double bLx = (double)(i != 1), bLxD = (double)(i != 1);
-        const double bRx = (double)(i != V.ni), bRxD = (double)(i != V.ni);
+        const double bLx = tmask_x(g.xf(V, i, j)), bLxD = tmask_x(g.xf(V, i, j - 1));
+        const double bRx = tmask_x(g.xf(V, i + 1, j)), bRxD = tmask_x(g.xf(V, i + 1, j - 1));
         const double tx = (r[j] * A.nUex_t[IX(i + 1, j)] * bRx + r[j] * A.nUex_t[IX(i, j)] * bLx +
                            r[j - 1] * A.nUex_t[IX(i + 1, j - 1)] * bRxD + r[j - 1] * A.nUex_t[IX(i, j - 1)] * bLxD) /
                           (4.0 * r_t);
         return A.nUer_t[IX(i, j)] - (A.m.inv_cross * tx + A.m.inv_par * A.nUer_t[IX(i, j)]) * A.m.mu_star +
                            A.m.mu_star * (n_f * dphidr - dpedr / K_E0);
+    } else if (t == GF_OPEN) {
+        const double uir = V.mu_i * A.Er[IX(i, j)] + K_EPS_VD;
+        return A.rhoi[IX(i, j - 1)] * uir;
+    } else if (t == GF_WLO) {
+        const double Te = A.Te[IX(i, j)];
+        const double dphi = fmax(-A.Er_old[IX(i, j)] * dr / 2.0, 0.0);
+        const double gpn = wall_gpn(V, Te, dphi, -1.0);
+        const double see = -V.coefIISEE_ts * A.rhoUir[IX(i, j)] - eisee_sum(V, Te) * A.rhoe_old[IX(i, j)] * gpn;
+        return A.rhoe[IX(i, j)] * gpn + see;
     } else {
-        const double uir = V.mu_i * A.Er[IX(i, V.nj + 1)] + K_EPS_VD;
-        return A.rhoi[IX(i, V.nj)] * uir;
+        const int c = j - 1;
+        const double Te = A.Te[IX(i, c)];
+        const double dphi = fmax(A.Er_old[IX(i, j)] * dr / 2.0, 0.0);
+        const double gpn = wall_gpn(V, Te, dphi, 1.0);
+        const double see = -V.coefIISEE_ts * A.rhoUir[IX(i, j)] - eisee_sum(V, Te) * A.rhoe_old[IX(i, c)] * gpn;
+        return A.rhoe[IX(i, c)] * gpn + see;
     }
 }
 
 
-// The multigrid's level-0 preparation, done by the assembly kernels for the
-// NDD mask (every interior cell active, the ghost ring inactive): couplings
-// to ghost cells zeroed (HYPRE's off-grid convention), the row excess
-// s = max(aP - sum a, 0), and the block partials of max|aP| (slot 0) and
-// |b|^2 (slot 1) in the multigrid's 2-D block layout (this grid's).
+// The multigrid's level-0 preparation, done by the assembly kernels: the
+// couplings to cells outside the plasma zeroed (HYPRE's off-grid
+// convention), the row excess s = max(aP - sum a, 0), and the block partials
+// of max|aP| (slot 0) and |b|^2 (slot 1) in the multigrid's 2-D block layout
+// (this grid's); rows of non-plasma cells get the identity, as the
+// multigrid's own preparation gives them.
 // one assembled row: the five coefficients and the right-hand side
 struct Row5 {
     double P, E, W, N, S, b;
 };
 
-__device__ inline void prep_row(const NddView& V, int i, int j, Row5& R, double* s_out, double& amax, double& bb)
+template <class G>
+__device__ inline void prep_row(const G& g, const NddView& V, int i, int j, Row5& R, double* s_out, double& amax,
+                                double& bb)
 {
-    R.E = i < V.ni ? R.E : 0.0;
-    R.W = i > 1 ? R.W : 0.0;
-    R.N = j < V.nj ? R.N : 0.0;
-    R.S = j > 1 ? R.S : 0.0;
+    R.E = g.act_nb(V, i + 1, j) ? R.E : 0.0;
+    R.W = g.act_nb(V, i - 1, j) ? R.W : 0.0;
+    R.N = g.act_nb(V, i, j + 1) ? R.N : 0.0;
+    R.S = g.act_nb(V, i, j - 1) ? R.S : 0.0;
     const double ex = R.P - (R.E + R.W + R.N + R.S);
     s_out[IX0(i, j)] = ex > 0.0 ? ex : 0.0;
     amax = fabs(R.P);
@@ -272,6 +363,20 @@ __device__ inline void store_row(const NddView& V, int i, int j, const Row5& R, 
     b[k] = R.b;
 }
 
+// a cell outside the plasma: the identity row, b = 0
+__device__ inline void store_idle_row(const NddView& V, int i, int j, double* aP, double* aE, double* aW, double* aN,
+                                      double* aS, double* b, double* s_out)
+{
+    const size_t k = IX0(i, j);
+    aP[k] = 1.0;
+    aE[k] = 0.0;
+    aW[k] = 0.0;
+    aN[k] = 0.0;
+    aS[k] = 0.0;
+    b[k] = 0.0;
+    if (s_out) s_out[k] = INFINITY;
+}
+
 __device__ inline void prep_partials(double amax, double bb, double* part)
 {
     __shared__ double lds[8];
@@ -288,8 +393,12 @@ __device__ inline void prep_partials(double amax, double bb, double* part)
 
 // 5-point Poisson correction system and its BC folds (fluidModule.hpp:2627-2824)
 // (the predicted face fluxes are stored after every load of the cell, so the
-// loads need not wait for the stores)
-__device__ __forceinline__ Row5 poisson_assemble_cell(const NddView& V, const PoissonArgs& A, int i, int j)
+// loads need not wait for the stores).  The plasma conduction term of a face
+// counts only between two plasma cells; Dirichlet walls fold aP += a_nb, the
+// open face aP -= a_nb, in the reference's order.
+template <class G>
+__device__ __forceinline__ Row5 poisson_assemble_cell(const G& g, const NddView& V, const PoissonArgs& A, int i,
+                                                      int j)
 {
 #pragma clang fp contract(off)
     const double dx = V.dx, dr = V.dr, dt = V.dt;
@@ -297,12 +406,14 @@ __device__ __forceinline__ Row5 poisson_assemble_cell(const NddView& V, const Po
     const double qR = (r[j] + r[j + 1]) / 2.0 / r[j];
     const double qL = (r[j] + r[j - 1]) / 2.0 / r[j];
     const double mu = A.m.mu_star;
+    const int tL = g.xf(V, i, j) & GF_TYPE, tR = g.xf(V, i + 1, j) & GF_TYPE;
+    const int tS = g.rf(V, i, j) & GF_TYPE, tN = g.rf(V, i, j + 1) & GF_TYPE;
     const double nLx = (A.rhoe[IX(i, j)] + A.rhoe[IX(i - 1, j)]) / 2.0;
     const double nRx = (A.rhoe[IX(i, j)] + A.rhoe[IX(i + 1, j)]) / 2.0;
     const double nLr = (A.rhoe[IX(i, j)] + A.rhoe[IX(i, j - 1)]) / 2.0;
     const double nRr = (A.rhoe[IX(i, j)] + A.rhoe[IX(i, j + 1)]) / 2.0;
-    const double bLx = (double)(i != 1), bRx = (double)(i != V.ni);
-    const double bLr = (double)(j != 1), bRr = (double)(j != V.nj);
+    const double bLx = (double)(tL == GF_INNER), bRx = (double)(tR == GF_INNER);
+    const double bLr = (double)(tS == GF_INNER), bRr = (double)(tN == GF_INNER);
     const double eRx = K_EPS0 + K_E0 * dt * nRx * mu * bRx;
     const double eLx = K_EPS0 + K_E0 * dt * nLx * mu * bLx;
     const double eRr = K_EPS0 + K_E0 * dt * nRr * mu * bRr;
@@ -311,85 +422,97 @@ __device__ __forceinline__ Row5 poisson_assemble_cell(const NddView& V, const Po
     double vE = eRx * dr * dr;
     double vW = eLx * dr * dr;
     double vN = qR * eRr * dx * dx;
-    const double vS = qL * eLr * dx * dx;
-    // predicted fluxes of the four faces, stored as in the SG assembly
-    const double fxL = poisson_flux_x(V, A, i, j), fxR = poisson_flux_x(V, A, i + 1, j);
-    const double frL = j >= 2 ? poisson_flux_r(V, A, i, j) : A.nUer[IX(i, 1)];
-    const double frR = poisson_flux_r(V, A, i, j + 1);
+    double vS = qL * eLr * dx * dx;
+    // predicted fluxes of the four faces, stored as in the SG assembly; the
+    // axis face is not predicted
+    const double fxL = poisson_flux_x(g, V, A, i, j), fxR = poisson_flux_x(g, V, A, i + 1, j);
+    const double frL = tS != GF_AXIS ? poisson_flux_r(g, V, A, i, j) : A.nUer[IX(i, j)];
+    const double frR = poisson_flux_r(g, V, A, i, j + 1);
     const double div = (fxR - fxL) / dx + (qR * frR - qL * frL) / dr;
     const double lap = -(A.Ex[IX(i + 1, j)] - A.Ex[IX(i, j)]) / dx - (qR * A.Er[IX(i, j + 1)] - qL * A.Er[IX(i, j)]) / dr;
     const double S = A.S ? A.S[IX(i, j)] : 0.0;
     const double vb = (K_E0 * A.rhoi[IX(i, j)] - K_E0 * A.rhoe[IX(i, j)] - K_E0 * dt * S + K_EPS0 * lap + K_E0 * dt * div) *
                       dx * dx * dr * dr;
-    if (i == 1) {
+    if (tL == GF_WLO) {
         vP = vP + vW;
         vW = 0.0;
     }
-    if (i == V.ni) {
+    if (tR == GF_WHI) {
         vP = vP + vE;
         vE = 0.0;
     }
-    if (j == V.nj) {
+    if (tS == GF_WLO) {
+        vP = vP + vS;
+        vS = 0.0;
+    }
+    if (tN == GF_WHI) {
+        vP = vP + vN;
+        vN = 0.0;
+    }
+    if (tN == GF_OPEN) {
         vP = vP - vN;
         vN = 0.0;
     }
     A.nUex[IX(i, j)] = fxL;
-    if (i == V.ni) A.nUex[IX(i + 1, j)] = fxR;
-    if (j >= 2) A.nUer[IX(i, j)] = frL;
-    if (j == V.nj) A.nUer[IX(i, j + 1)] = frR;
+    if (tR == GF_WHI) A.nUex[IX(i + 1, j)] = fxR;
+    if (tS != GF_AXIS) A.nUer[IX(i, j)] = frL;
+    if (tN == GF_WHI || tN == GF_OPEN) A.nUer[IX(i, j + 1)] = frR;
     return Row5{vP, vE, vW, vN, vS, vb};
 }
 
 // with s_out set, also the multigrid's level-0 preparation (prep_row)
-__global__ void k_poisson_assemble(NddView V, PoissonArgs A, double* aP, double* aE, double* aW, double* aN,
+template <class G>
+__global__ void k_poisson_assemble(G g, NddView V, PoissonArgs A, double* aP, double* aE, double* aW, double* aN,
                                    double* aS, double* b, double* s_out, double* part)
 {
     THREAD_IJ(1, 1);
     double amax = 0.0, bb = 0.0;
     if (i <= V.ni && j <= V.nj) {
-        Row5 R = poisson_assemble_cell(V, A, i, j);
-        if (s_out) prep_row(V, i, j, R, s_out, amax, bb);
-        store_row(V, i, j, R, aP, aE, aW, aN, aS, b);
+        if (g.act(V, i, j)) {
+            Row5 R = poisson_assemble_cell(g, V, A, i, j);
+            if (s_out) prep_row(g, V, i, j, R, s_out, amax, bb);
+            store_row(V, i, j, R, aP, aE, aW, aN, aS, b);
+        } else {
+            store_idle_row(V, i, j, aP, aE, aW, aN, aS, b, s_out);
+        }
     }
     if (s_out) prep_partials(amax, bb, part);
 }
 
-// phi ghosts before the PC loop (fluidModule.hpp:2160-2253): thread per edge cell
-// (the same values also into cp, the first pass's copy of phi)
-__global__ void k_poisson_ghost(NddView V, double* __restrict__ phi, double* __restrict__ cp)
+// phi ghosts before the PC loop (fluidModule.hpp:2152-2249): every cell of
+// the extended grid with a ghost rule takes 2 V - phi (V = 0 or V_bias) or
+// phi of its source cell (the same values also into cp, the first pass's
+// copy of phi)
+template <class G>
+__global__ void k_poisson_ghost(G g, NddView V, double* __restrict__ phi, double* __restrict__ cp)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < V.nj) {
-        const int j = t + 1;
-        const double w = -phi[IX(1, j)], e = 2.0 * V.V_bias - phi[IX(V.ni, j)];
-        phi[IX(0, j)] = w;
-        phi[IX(V.ni + 1, j)] = e;
-        cp[IX(0, j)] = w;
-        cp[IX(V.ni + 1, j)] = e;
-    }
-    if (t < V.ni) {
-        const int i = t + 1;
-        const double n = phi[IX(i, V.nj)];
-        phi[IX(i, V.nj + 1)] = n;
-        cp[IX(i, V.nj + 1)] = n;
-    }
+#pragma clang fp contract(off)
+    THREAD_IJ(0, 0);
+    if (i > V.ni + 1 || j > V.nj + 1) return;
+    const int q = g.ghost(V, i, j);
+    if (q < 0) return;
+    const double ps = phi[q >> 2];
+    const double v = (q & 2) ? ((q & 1) ? 2.0 * V.V_bias - ps : -ps) : ps;
+    phi[IX(i, j)] = v;
+    cp[IX(i, j)] = v;
 }
 
-// the correction at (i, j) of the extended grid: the solution inside, its
-// odd mirror on the z-wall ghosts (Dirichlet) and its copy on the open top
-__device__ inline double dphi_at(const NddView& V, const double* __restrict__ sol, int i, int j)
+// the correction at (i, j) of the extended grid: the solution in plasma
+// cells, its odd mirror on Dirichlet ghosts and its copy on open ghosts
+template <class G>
+__device__ inline double dphi_at(const G& g, const NddView& V, const double* __restrict__ sol, int i, int j)
 {
-    const int ni = V.ni, nj = V.nj;
-    const bool in_i = i >= 1 && i <= ni, in_j = j >= 1 && j <= nj;
-    if (in_i && in_j) return sol[IX0(i, j)];
-    if (i == 0 && in_j) return -sol[IX0(1, j)];
-    if (i == ni + 1 && in_j) return -sol[IX0(ni, j)];
-    if (j == nj + 1 && in_i) return sol[IX0(i, nj)];
-    return 0.0;
+    if (g.act_nb(V, i, j)) return sol[IX0(i, j)];
+    const int q = g.ghost(V, i, j);
+    if (q < 0) return 0.0;
+    const int src = q >> 2;
+    const int si = src / V.ld, sj = src - si * V.ld;
+    const double d = sol[IX0(si, sj)];
+    return (q & 2) ? -d : d;
 }
 
 // phi += dphi on the extended grid with the ghost reflections of dphi, the
-// wall-face field update (fluidModule.hpp:2850-2958), then the interior face
+// wall-face field update (fluidModule.hpp:2850-2958), then the inner face
 // fields E = -grad phi and the flux correction mu* n_f grad dphi
 // (fluidModule.hpp:2959-3033).  phi0 is phi at the start of the PC pass, so
 // every thread can form the updated phi of its neighbours.  The new phi and
@@ -397,7 +520,8 @@ __device__ inline double dphi_at(const NddView& V, const double* __restrict__ so
 // start-of-pass copies nc, nx, nr (their other entries were copied at the
 // routine's start and do not change).  Every load is issued before the first
 // store.
-__global__ void k_poisson_update(NddView V, const double* __restrict__ sol, const double* __restrict__ phi0,
+template <class G>
+__global__ void k_poisson_update(G g, NddView V, const double* __restrict__ sol, const double* __restrict__ phi0,
                                  double* __restrict__ dphi, double* __restrict__ phi, double* __restrict__ Ex,
                                  double* __restrict__ Er, const double* __restrict__ rhoe, double* __restrict__ nUex,
                                  double* __restrict__ nUer, Mob m, double* __restrict__ nc, double* __restrict__ nx,
@@ -406,61 +530,69 @@ __global__ void k_poisson_update(NddView V, const double* __restrict__ sol, cons
 #pragma clang fp contract(off)
     THREAD_IJ(0, 0);
     if (i > V.ni + 1 || j > V.nj + 1) return;
-    const int ni = V.ni, nj = V.nj;
-    const bool cell = i >= 1 && i <= ni && j >= 1 && j <= nj;
-    const double d = dphi_at(V, sol, i, j);
+    const bool cell = g.act_nb(V, i, j);
+    const double d = dphi_at(g, V, sol, i, j);
     const double p = phi0[IX(i, j)] + d;
+    int tL = GF_NONE, tR = GF_NONE, tS = GF_NONE, tN = GF_NONE;
     double dl = 0.0, pl = 0.0, ds = 0.0, ps = 0.0, rc = 0.0, rl = 0.0, rs = 0.0;
-    double ux = 0.0, ur = 0.0, uxE = 0.0, urN = 0.0, ex1 = 0.0, exN = 0.0;
+    double ux = 0.0, ur = 0.0, uxE = 0.0, urN = 0.0, ex1 = 0.0, exN = 0.0, er1 = 0.0, erN = 0.0;
     if (cell) {
+        tL = g.xf(V, i, j) & GF_TYPE;
+        tR = g.xf(V, i + 1, j) & GF_TYPE;
+        tS = g.rf(V, i, j) & GF_TYPE;
+        tN = g.rf(V, i, j + 1) & GF_TYPE;
         ux = nUex[IX(i, j)];
         ur = nUer[IX(i, j)];
         rc = rhoe[IX(i, j)];
-        if (i >= 2) {
-            dl = dphi_at(V, sol, i - 1, j);
+        if (tL == GF_INNER) {
+            dl = dphi_at(g, V, sol, i - 1, j);
             pl = phi0[IX(i - 1, j)] + dl;
             rl = rhoe[IX(i - 1, j)];
         }
-        if (j >= 2) {
-            ds = dphi_at(V, sol, i, j - 1);
+        if (tS == GF_INNER) {
+            ds = dphi_at(g, V, sol, i, j - 1);
             ps = phi0[IX(i, j - 1)] + ds;
             rs = rhoe[IX(i, j - 1)];
         }
-        if (i == 1) ex1 = Ex[IX(1, j)];
-        if (i == ni) {
-            exN = Ex[IX(ni + 1, j)];
-            uxE = nUex[IX(ni + 1, j)];
+        if (tL == GF_WLO) ex1 = Ex[IX(i, j)];
+        if (tR == GF_WHI) {
+            exN = Ex[IX(i + 1, j)];
+            uxE = nUex[IX(i + 1, j)];
         }
-        if (j == nj) urN = nUer[IX(i, nj + 1)];
+        if (tS == GF_WLO) er1 = Er[IX(i, j)];
+        if (tN == GF_WHI) erN = Er[IX(i, j + 1)];
+        if (tN == GF_WHI || tN == GF_OPEN) urN = nUer[IX(i, j + 1)];
     }
     dphi[IX(i, j)] = d;
     phi[IX(i, j)] = p;
     nc[IX(i, j)] = p;
     if (!cell) return;
-    // wall faces: d is the solution at (1, j) or (ni, j) there
-    if (i == 1) Ex[IX(1, j)] = ex1 - (d - (-d)) / V.dx;
-    if (i == ni) Ex[IX(ni + 1, j)] = exN - ((-d) - d) / V.dx;
-    if (j == nj) Er[IX(i, nj + 1)] = 0.0;
-    if (i >= 2) {
+    // wall faces: the ghost across them holds -d
+    if (tL == GF_WLO) Ex[IX(i, j)] = ex1 - (d - (-d)) / V.dx;
+    if (tR == GF_WHI) Ex[IX(i + 1, j)] = exN - ((-d) - d) / V.dx;
+    if (tS == GF_WLO) Er[IX(i, j)] = er1 - (d - (-d)) / V.dr;
+    if (tN == GF_WHI) Er[IX(i, j + 1)] = erN - ((-d) - d) / V.dr;
+    if (tN == GF_OPEN) Er[IX(i, j + 1)] = 0.0;
+    if (tL == GF_INNER) {
         Ex[IX(i, j)] = -(p - pl) / V.dx;
         const double n_f = (rc + rl) / 2.0;
         const double u = ux + m.mu_star * n_f * ((d - dl) / V.dx);
         nUex[IX(i, j)] = u;
         nx[IX(i, j)] = u;
     } else {
-        nx[IX(1, j)] = ux;
+        nx[IX(i, j)] = ux;
     }
-    if (i == ni) nx[IX(ni + 1, j)] = uxE;
-    if (j >= 2) {
+    if (tR == GF_WHI) nx[IX(i + 1, j)] = uxE;
+    if (tS == GF_INNER) {
         Er[IX(i, j)] = -(p - ps) / V.dr;
         const double n_f = (rc + rs) / 2.0;
         const double u = ur + m.mu_star * n_f * ((d - ds) / V.dr);
         nUer[IX(i, j)] = u;
         nr[IX(i, j)] = u;
     } else {
-        nr[IX(i, 1)] = ur;
+        nr[IX(i, j)] = ur;
     }
-    if (j == nj) nr[IX(i, nj + 1)] = urN;
+    if (tN == GF_WHI || tN == GF_OPEN) nr[IX(i, j + 1)] = urN;
 }
 
 // ================================================================== SG equations
@@ -469,7 +601,7 @@ struct SgArgs {
     Mob m;
     double tfac;
     const double *n_old, *T;  // start-of-routine density, temperature of the SG faces
-    const double *Ex, *Er, *Te, *rhoi, *rhoe, *rhoe_old, *rhoUix, *rhoUex, *rhoUer;
+    const double *Ex, *Er, *Te, *rhoi, *rhoe, *rhoe_old, *rhoUix, *rhoUir, *rhoUex, *rhoUer;
     const double *S, *Pabs, *eloss;
     const double *Fx_t, *Fr_t;
     double *n, *Fx, *Fr;
@@ -481,19 +613,63 @@ struct SgArgs {
     int reuse;  // the weights are already in wgt (a later PC pass of the routine)
 };
 
-// predicted SG flux on x face (i, j), i = 1..ni+1: the Scharfetter-Gummel
-// flux plus the mobility-tensor correction of the previous fluxes inside,
-// the wall fluxes on faces 1 and ni+1 (fluidModule.hpp:3239-3367, energy
-// :4161-4316).  bp, bm: the face's weights bern_bp / bern_bm (inner faces).
-__device__ __forceinline__ double sg_flux_x(const NddView& V, const SgArgs& A, int i, int j, double bp, double bm)
+// wall flux of species sp through a wall face whose plasma cell is c = (ci,
+// cj), face field E and ion flux Gi at the face, sgn -1 for a wall on the
+// cell's low side, +1 on its high side (fluidModule.hpp:3250-3367 and
+// 3453-3532, energy :4161-4316 and :4403-4506; ions: the kinetic flux of
+// :1502-1526 at the drift mu_i E); h is the spacing across the face.  The
+// electrons' sheath barrier is the cell potential above the wall's,
+// phi_cell - phi_wall = sgn E h / 2 (fluidModule.hpp:3255, 3335).
+__device__ __forceinline__ double sg_wall_flux(const NddView& V, const SgArgs& A, int ci, int cj, double E, double Gi,
+                                               double sgn, double h)
 {
 #pragma clang fp contract(off)
-    const double dx = V.dx;
-    const double* r = V.r;
     const double* T = A.T;
-    if (i >= 2 && i <= V.ni) {
+    if (A.sp == SP_ION) {
+        const double U = V.mu_i * E;
+        return A.n[IX(ci, cj)] * ion_wall_gpn(V, U, sgn);
+    } else if (A.sp == SP_ELEC) {
+        const double Te = A.Te[IX(ci, cj)];
+        const double dphi = fmax(sgn * E * h / 2.0, 0.0);
+        const double gpn = wall_gpn(V, Te, dphi, sgn);
+        const double see = -V.coefIISEE_ts * Gi - eisee_sum(V, Te) * A.rhoe_old[IX(ci, cj)] * gpn;
+        return A.n[IX(ci, cj)] * gpn + see;
+    } else {
+        const double Teo = T[IX(ci, cj)];
+        const double dphi = fmax(sgn * E * h / 2.0, 0.0);
+        const double vth = sqrt(8.0 * K_BOLTZ * Teo / (M_PI * V.masse));
+        const double ex = exp(fmin(-K_E0 * dphi / (K_BOLTZ * Teo + 1e-100), K_LIM_EXP));
+        const double gpne = sgn * vth * ex / 3.0 * 1.0;
+        const double eps_emit = 2.0 * K_E0 * V.Te_emitSEE_eV;
+        const double gpn = sgn * 0.25 * 1.0 * vth * ex;
+        const double Te = A.Te[IX(ci, cj)];
+        double eb = 0.0, rd = 0.0, ts = 0.0;
+        if (V.n_see > 1) {
+            eb = func_Te(V.see_eb, V.n_see, Te, V.dTe_SEE);
+            rd = func_Te(V.see_rd, V.n_see, Te, V.dTe_SEE);
+            ts = func_Te(V.see_ts, V.n_see, Te, V.dTe_SEE);
+        }
+        const double see = -eps_emit * V.coefIISEE_ts * Gi - (eb + V.ratioEngy_EISEE_rd * rd) * A.n_old[IX(ci, cj)] * gpne -
+                           eps_emit * ts * A.rhoe_old[IX(ci, cj)] * gpn;
+        return A.n[IX(ci, cj)] * gpne + see;
+    }
+}
+
+// predicted SG flux on x face (i, j): the Scharfetter-Gummel flux plus the
+// mobility-tensor correction of the previous fluxes on inner faces, the wall
+// flux on wall faces (fluidModule.hpp:3239-3367, energy :4161-4316).  bp, bm:
+// the face's weights bern_bp / bern_bm (inner faces).
+template <class G>
+__device__ __forceinline__ double sg_flux_x(const G& g, const NddView& V, const SgArgs& A, int i, int j, double bp,
+                                            double bm)
+{
+#pragma clang fp contract(off)
+    const double* r = V.r;
+    const int t = g.xf(V, i, j) & GF_TYPE;
+    if (t == GF_INNER) {
         const double rL = (r[j] + r[j - 1]) / 2.0, rR = (r[j] + r[j + 1]) / 2.0;
-        const double bLr = (double)(j != 1), bLrD = (double)(j != 1), bRr = 1.0, bRrD = 1.0;
+        const double bLr = tmask_r(g.rf(V, i, j)), bLrD = tmask_r(g.rf(V, i - 1, j));
+        const double bRr = tmask_r(g.rf(V, i, j + 1)), bRrD = tmask_r(g.rf(V, i - 1, j + 1));
         const double tr = (rR * A.Fr_t[IX(i, j + 1)] * bRr + rL * A.Fr_t[IX(i, j)] * bLr +
                            rR * A.Fr_t[IX(i - 1, j + 1)] * bRrD + rL * A.Fr_t[IX(i - 1, j)] * bLrD) /
                           (4.0 * r[j]);
@@ -501,72 +677,46 @@ __device__ __forceinline__ double sg_flux_x(const NddView& V, const SgArgs& A, i
         if (A.sp == SP_ENERGY) sg = sg * 5.0 / 3.0;
         return sg + A.Fx_t[IX(i, j)] - A.m.mu_star * (A.m.inv_par * A.Fx_t[IX(i, j)] + A.m.inv_cross * tr);
     }
-    const int side = i == 1 ? 0 : 1;
-    const int c = side ? V.ni : 1;  // wall cell
-    const int fi = side ? V.ni + 1 : 1;
-    const double sgn = side ? 1.0 : -1.0;
-    if (A.sp == SP_ION) {
-        const double U = V.mu_i * A.Ex[IX(fi, j)];
-        return A.n[IX(c, j)] * ion_wall_gpn(V, U, sgn);
-    } else if (A.sp == SP_ELEC) {
-        const double Te = A.Te[IX(c, j)];
-        const double dphi = fmax(-sgn * A.Ex[IX(fi, j)] * dx / 2.0, 0.0);
-        const double gpn = wall_gpn(V, Te, dphi, sgn);
-        const double see = -V.coefIISEE_ts * A.rhoUix[IX(fi, j)] - eisee_sum(V, Te) * A.rhoe_old[IX(c, j)] * gpn;
-        return A.n[IX(c, j)] * gpn + see;
-    } else {
-        const double Teo = T[IX(c, j)];
-        const double dphi = fmax(-sgn * A.Ex[IX(fi, j)] * dx / 2.0, 0.0);
-        const double vth = sqrt(8.0 * K_BOLTZ * Teo / (M_PI * V.masse));
-        const double ex = exp(fmin(-K_E0 * dphi / (K_BOLTZ * Teo + 1e-100), K_LIM_EXP));
-        const double gpne = sgn * vth * ex / 3.0 * 1.0;
-        const double eps_emit = 2.0 * K_E0 * V.Te_emitSEE_eV;
-        const double gpn = sgn * 0.25 * 1.0 * vth * ex;
-        const double Te = A.Te[IX(c, j)];
-        double eb = 0.0, rd = 0.0, ts = 0.0;
-        if (V.n_see > 1) {
-            eb = func_Te(V.see_eb, V.n_see, Te, V.dTe_SEE);
-            rd = func_Te(V.see_rd, V.n_see, Te, V.dTe_SEE);
-            ts = func_Te(V.see_ts, V.n_see, Te, V.dTe_SEE);
-        }
-        const double see = -eps_emit * V.coefIISEE_ts * A.rhoUix[IX(fi, j)] -
-                           (eb + V.ratioEngy_EISEE_rd * rd) * A.n_old[IX(c, j)] * gpne -
-                           eps_emit * ts * A.rhoe_old[IX(c, j)] * gpn;
-        return A.n[IX(c, j)] * gpne + see;
-    }
+    const bool high = t == GF_WHI;  // the plasma cell is i - 1
+    return sg_wall_flux(V, A, high ? i - 1 : i, j, A.Ex[IX(i, j)], A.rhoUix[IX(i, j)], high ? 1.0 : -1.0, V.dx);
 }
 
-// predicted SG flux on r face (i, j), j = 2..nj+1; face nj+1 is the open
-// upper boundary (fluidModule.hpp:3443-3542, energy :4403-4506).  bp, bm as
-// for sg_flux_x.
-__device__ __forceinline__ double sg_flux_r(const NddView& V, const SgArgs& A, int i, int j, double bp, double bm)
+// predicted SG flux on r face (i, j): inner faces as for x, wall faces the
+// wall flux, the open face the ion flux (electrons, ions) or the convected
+// energy 5/2 k T max(Gamma_e, 0) (fluidModule.hpp:3443-3542, energy
+// :4403-4506).  bp, bm as for sg_flux_x.
+template <class G>
+__device__ __forceinline__ double sg_flux_r(const G& g, const NddView& V, const SgArgs& A, int i, int j, double bp,
+                                            double bm)
 {
 #pragma clang fp contract(off)
-    const double dr = V.dr;
     const double* r = V.r;
     const double* T = A.T;
-    if (j <= V.nj) {
+    const int t = g.rf(V, i, j) & GF_TYPE;
+    if (t == GF_INNER) {
         const double r_t = (r[j] + r[j - 1]) / 2.0;
-        const double bLx = (double)(i != 1), bLxD = (double)(i != 1);
-        const double bRx = (double)(i != V.ni), bRxD = (double)(i != V.ni);
+        const double bLx = tmask_x(g.xf(V, i, j)), bLxD = tmask_x(g.xf(V, i, j - 1));
+        const double bRx = tmask_x(g.xf(V, i + 1, j)), bRxD = tmask_x(g.xf(V, i + 1, j - 1));
         const double tx = (r[j] * A.Fx_t[IX(i + 1, j)] * bRx + r[j] * A.Fx_t[IX(i, j)] * bLx +
                            r[j - 1] * A.Fx_t[IX(i + 1, j - 1)] * bRxD + r[j - 1] * A.Fx_t[IX(i, j - 1)] * bLxD) /
                           (4.0 * r_t);
         double sg = bp * A.n[IX(i, j - 1)] - bm * A.n[IX(i, j)];
         if (A.sp == SP_ENERGY) sg = sg * 5.0 / 3.0;
         return sg + A.Fr_t[IX(i, j)] - A.m.mu_star * (A.m.inv_cross * tx + A.m.inv_par * A.Fr_t[IX(i, j)]);
-    } else if (A.sp == SP_ENERGY) {
-        return 5.0 / 2.0 * K_BOLTZ * T[IX(i, V.nj)] * fmax(A.rhoUer[IX(i, V.nj + 1)], 0.0);
-    } else {
-        const double uir = V.mu_i * A.Er[IX(i, V.nj + 1)] + K_EPS_VD;
-        return A.rhoi[IX(i, V.nj)] * uir;
+    } else if (t == GF_OPEN) {
+        if (A.sp == SP_ENERGY) return 5.0 / 2.0 * K_BOLTZ * T[IX(i, j - 1)] * fmax(A.rhoUer[IX(i, j)], 0.0);
+        const double uir = V.mu_i * A.Er[IX(i, j)] + K_EPS_VD;
+        return A.rhoi[IX(i, j - 1)] * uir;
     }
+    const bool high = t == GF_WHI;  // the plasma cell is j - 1
+    return sg_wall_flux(V, A, i, high ? j - 1 : j, A.Er[IX(i, j)], A.rhoUir[IX(i, j)], high ? 1.0 : -1.0, V.dr);
 }
 
 
 // the weights of each face are formed once, for the coefficients and the
 // predicted flux; the fluxes are stored after every load of the cell
-__device__ __forceinline__ Row5 sg_assemble_cell(const NddView& V, const SgArgs& A, int i, int j)
+template <class G>
+__device__ __forceinline__ Row5 sg_assemble_cell(const G& g, const NddView& V, const SgArgs& A, int i, int j)
 {
 #pragma clang fp contract(off)
     const double dx = V.dx, dr = V.dr, dt = V.dt;
@@ -576,12 +726,14 @@ __device__ __forceinline__ Row5 sg_assemble_cell(const NddView& V, const SgArgs&
     const double qR = (r[j] + r[j + 1]) / 2.0 / r[j];
     const double qL = (r[j] + r[j - 1]) / 2.0 / r[j];
     const double mu = A.m.mu_star;
-    const double bLx = (double)(i != 1), bRx = (double)(i != V.ni);
-    const double bLr = (double)(j != 1), bRr = (double)(j != V.nj);
+    const int tL = g.xf(V, i, j) & GF_TYPE, tR = g.xf(V, i + 1, j) & GF_TYPE;
+    const int tS = g.rf(V, i, j) & GF_TYPE, tN = g.rf(V, i, j + 1) & GF_TYPE;
+    const double bLx = (double)(tL == GF_INNER), bRx = (double)(tR == GF_INNER);
+    const double bLr = (double)(tS == GF_INNER), bRr = (double)(tN == GF_INNER);
     // the weights of the four faces: formed here on the routine's first PC
     // pass, read back on later passes (the fields and temperatures they
-    // depend on do not change within the routine); the east face of the last
-    // row and the north face of the last column belong to no cell's store
+    // depend on do not change within the routine); the east and north faces
+    // of a cell whose neighbour there is not plasma belong to no cell's store
     const size_t k = IX0(i, j), wn = A.wn;
     double bpLx, bmLx, bpRx, bmRx, bpLr, bmLr, bpRr, bmRr;
     if (A.reuse) {
@@ -593,13 +745,13 @@ __device__ __forceinline__ Row5 sg_assemble_cell(const NddView& V, const SgArgs&
         sg_weights(V, A.sp, mu, A.Ex[IX(i, j)], T[IX(i - 1, j)], T[IX(i, j)], dx, bpLx, bmLx);
         sg_weights(V, A.sp, mu, A.Er[IX(i, j)], T[IX(i, j - 1)], T[IX(i, j)], dr, bpLr, bmLr);
     }
-    if (A.reuse && i < V.ni) {
+    if (A.reuse && g.act_nb(V, i + 1, j)) {
         bpRx = A.wgt[k + V.ld0];
         bmRx = A.wgt[wn + k + V.ld0];
     } else {
         sg_weights(V, A.sp, mu, A.Ex[IX(i + 1, j)], T[IX(i, j)], T[IX(i + 1, j)], dx, bpRx, bmRx);
     }
-    if (A.reuse && j < V.nj) {
+    if (A.reuse && g.act_nb(V, i, j + 1)) {
         bpRr = A.wgt[2 * wn + k + 1];
         bmRr = A.wgt[3 * wn + k + 1];
     } else {
@@ -613,16 +765,18 @@ __device__ __forceinline__ Row5 sg_assemble_cell(const NddView& V, const SgArgs&
     R.N = qR * bmRr * bRr * dx;
     R.S = qL * bpLr * bLr * dx;
     // the predicted fluxes of the cell's four faces; the cell stores its west
-    // and south faces (east and north too on the last row and column), the
-    // axis face j = 1 is not predicted
-    const double fxL = sg_flux_x(V, A, i, j, bpLx, bmLx), fxR = sg_flux_x(V, A, i + 1, j, bpRx, bmRx);
-    const double frL = j >= 2 ? sg_flux_r(V, A, i, j, bpLr, bmLr) : A.Fr[IX(i, 1)];
-    const double frR = sg_flux_r(V, A, i, j + 1, bpRr, bmRr);
+    // and south faces (east and north too where they are boundaries), the
+    // axis face is not predicted
+    const double fxL = sg_flux_x(g, V, A, i, j, bpLx, bmLx), fxR = sg_flux_x(g, V, A, i + 1, j, bpRx, bmRx);
+    const double frL = tS != GF_AXIS ? sg_flux_r(g, V, A, i, j, bpLr, bmLr) : A.Fr[IX(i, j)];
+    const double frR = sg_flux_r(g, V, A, i, j + 1, bpRr, bmRr);
     const double div = (fxR - fxL) / dx + (qR * frR - qL * frL) / dr;
     if (A.sp == SP_ENERGY) {
-        const double bLxw = (double)(i != 1), bRxw = (double)(i != V.ni);
+        // J.E with the wall faces left out (fluidModule.hpp:4624-4625)
+        const double bLxw = (double)!is_wall(tL), bRxw = (double)!is_wall(tR);
+        const double bLrw = (double)!is_wall(tS), bRrw = (double)!is_wall(tN);
         const double JEx = -K_E0 * (A.rhoUex[IX(i + 1, j)] * A.Ex[IX(i + 1, j)] * bRxw + A.rhoUex[IX(i, j)] * A.Ex[IX(i, j)] * bLxw) / 2.0;
-        const double JEr = -K_E0 * (rR * A.rhoUer[IX(i, j + 1)] * A.Er[IX(i, j + 1)] * 1.0 + rL * A.rhoUer[IX(i, j)] * A.Er[IX(i, j)] * 1.0) /
+        const double JEr = -K_E0 * (rR * A.rhoUer[IX(i, j + 1)] * A.Er[IX(i, j + 1)] * bRrw + rL * A.rhoUer[IX(i, j)] * A.Er[IX(i, j)] * bLrw) /
                            (2.0 * r[j]);
         const double R1 = (-A.n[IX(i, j)] + A.n_old[IX(i, j)]) / dt;
         const double R2 = JEx + JEr;
@@ -636,9 +790,9 @@ __device__ __forceinline__ Row5 sg_assemble_cell(const NddView& V, const SgArgs&
         R.b = (-div + S - (A.n[IX(i, j)] - A.n_old[IX(i, j)]) / dt) * dx * dr;
     }
     A.Fx[IX(i, j)] = fxL;
-    if (i == V.ni) A.Fx[IX(i + 1, j)] = fxR;
-    if (j >= 2) A.Fr[IX(i, j)] = frL;
-    if (j == V.nj) A.Fr[IX(i, j + 1)] = frR;
+    if (tR == GF_WHI) A.Fx[IX(i + 1, j)] = fxR;
+    if (tS != GF_AXIS) A.Fr[IX(i, j)] = frL;
+    if (tN == GF_WHI || tN == GF_OPEN) A.Fr[IX(i, j + 1)] = frR;
     if (!A.reuse) {
         A.wgt[k] = bpLx;
         A.wgt[wn + k] = bmLx;
@@ -649,47 +803,56 @@ __device__ __forceinline__ Row5 sg_assemble_cell(const NddView& V, const SgArgs&
 }
 
 // with s_out set, also the multigrid's level-0 preparation (prep_row)
-__global__ void k_sg_assemble(NddView V, SgArgs A, double* aP, double* aE, double* aW, double* aN, double* aS,
+template <class G>
+__global__ void k_sg_assemble(G g, NddView V, SgArgs A, double* aP, double* aE, double* aW, double* aN, double* aS,
                               double* b, double* s_out, double* part)
 {
     THREAD_IJ(1, 1);
     double amax = 0.0, bb = 0.0;
     if (i <= V.ni && j <= V.nj) {
-        Row5 R = sg_assemble_cell(V, A, i, j);
-        if (s_out) prep_row(V, i, j, R, s_out, amax, bb);
-        store_row(V, i, j, R, aP, aE, aW, aN, aS, b);
+        if (g.act(V, i, j)) {
+            Row5 R = sg_assemble_cell(g, V, A, i, j);
+            if (s_out) prep_row(g, V, i, j, R, s_out, amax, bb);
+            store_row(V, i, j, R, aP, aE, aW, aN, aS, b);
+        } else {
+            store_idle_row(V, i, j, aP, aE, aW, aN, aS, b, s_out);
+        }
     }
     if (s_out) prep_partials(amax, bb, part);
 }
 
-// flux correction on interior faces and the positivity-guarded density
+// flux correction on inner faces and the positivity-guarded density
 // update (fluidModule.hpp:3682-3757); thread (i, j) owns x-face (i, j),
 // r-face (i, j) and cell (i, j).  The faces' SG weights are the ones the
 // assembly of this PC pass formed (same fields, same temperatures).  The
 // corrected density and every face flux the assembly predicted also go to
 // the next pass's start-of-pass copies nc, nx, nr (their other entries were
 // copied at the routine's first pass and do not change).
-__global__ void k_sg_correct(NddView V, SgArgs A, const double* __restrict__ dn, double* __restrict__ nc,
+template <class G>
+__global__ void k_sg_correct(G g, NddView V, SgArgs A, const double* __restrict__ dn, double* __restrict__ nc,
                              double* __restrict__ nx, double* __restrict__ nr)
 {
 #pragma clang fp contract(off)
     THREAD_IJ(1, 1);
     if (i > V.ni || j > V.nj) return;
+    if (!g.act(V, i, j)) return;
+    const int tL = g.xf(V, i, j) & GF_TYPE, tR = g.xf(V, i + 1, j) & GF_TYPE;
+    const int tS = g.rf(V, i, j) & GF_TYPE, tN = g.rf(V, i, j + 1) & GF_TYPE;
     const size_t k = IX0(i, j), wn = A.wn;
     // every load before the first store
     const double fx = A.Fx[IX(i, j)], fr = A.Fr[IX(i, j)], n0 = A.n[IX(i, j)];
-    const double fxE = i == V.ni ? A.Fx[IX(i + 1, j)] : 0.0;
-    const double frN = j == V.nj ? A.Fr[IX(i, j + 1)] : 0.0;
+    const double fxE = tR == GF_WHI ? A.Fx[IX(i + 1, j)] : 0.0;
+    const double frN = (tN == GF_WHI || tN == GF_OPEN) ? A.Fr[IX(i, j + 1)] : 0.0;
     const double dc = dn[k], dW = dn[IX0(i - 1, j)], dS = dn[IX0(i, j - 1)];
     const double bpx = A.wgt[k], bmx = A.wgt[wn + k], bpr = A.wgt[2 * wn + k], bmr = A.wgt[3 * wn + k];
     double fx1 = fx, fr1 = fr;
-    if (i >= 2) {
+    if (tL == GF_INNER) {
         double d = bpx * dW - bmx * dc;
         if (A.sp == SP_ENERGY) d = d * 5.0 / 3.0;
         fx1 = fx + d;
         A.Fx[IX(i, j)] = fx1;
     }
-    if (j >= 2) {
+    if (tS == GF_INNER) {
         double d = bpr * dS - bmr * dc;
         if (A.sp == SP_ENERGY) d = d * 5.0 / 3.0;
         fr1 = fr + d;
@@ -700,19 +863,20 @@ __global__ void k_sg_correct(NddView V, SgArgs A, const double* __restrict__ dn,
     nx[IX(i, j)] = fx1;
     nr[IX(i, j)] = fr1;
     nc[IX(i, j)] = nn > 0.0 ? nn : n0;
-    if (i == V.ni) nx[IX(i + 1, j)] = fxE;
-    if (j == V.nj) nr[IX(i, j + 1)] = frN;
+    if (tR == GF_WHI) nx[IX(i + 1, j)] = fxE;
+    if (tN == GF_WHI || tN == GF_OPEN) nr[IX(i, j + 1)] = frN;
 }
 
 // ================================================================== reductions
-// calcRes (miniFuncs.hpp:292-375) of three (new, old) pairs over the cell
-// box [1..ni]x[1..nj], the x-face box [2..ni]x[1..nj] and the r-face box
-// [1..ni]x[2..nj]; six partial values per block.
+// calcRes (miniFuncs.hpp:292-375) of three (new, old) pairs over the plasma
+// cells, the inner x faces and the inner r faces (the reference's cell and
+// face boxes); six partial values per block.
 struct ResPairs {
     const double *n0, *o0, *n1, *o1, *n2, *o2;
 };
 
-__global__ void k_calcres(NddView V, ResPairs P, int mode, double eps, double* __restrict__ part, int nb)
+template <class G>
+__global__ void k_calcres(G g, NddView V, ResPairs P, int mode, double eps, double* __restrict__ part, int nb)
 {
     __shared__ double lds[6 * NB_X * NB_Y / 64];
     THREAD_IJ(1, 1);
@@ -722,7 +886,8 @@ __global__ void k_calcres(NddView V, ResPairs P, int mode, double eps, double* _
         // the first face's value is loaded and not used)
         const double vnq[3] = {P.n0[IX(i, j)], P.n1[IX(i, j)], P.n2[IX(i, j)]};
         const double voq[3] = {P.o0[IX(i, j)], P.o1[IX(i, j)], P.o2[IX(i, j)]};
-        const bool in[3] = {true, i >= 2, j >= 2};
+        const bool in[3] = {g.act(V, i, j), (g.xf(V, i, j) & GF_TYPE) == GF_INNER,
+                            (g.rf(V, i, j) & GF_TYPE) == GF_INNER};
 #pragma unroll
         for (int q = 0; q < 3; q++) {
             if (!in[q]) continue;
@@ -800,13 +965,14 @@ __global__ void k_calcres_finish(const double* __restrict__ part, int nb, int mo
 }
 
 // energy post-processing (fluidModule.hpp:4868-4905)
-__global__ void k_energy_guard_max(NddView V, double* __restrict__ rhoeps, const double* __restrict__ rhoeps_old,
+template <class G>
+__global__ void k_energy_guard_max(G g, NddView V, double* __restrict__ rhoeps, const double* __restrict__ rhoeps_old,
                                    const double* __restrict__ rhoe, double* __restrict__ part)
 {
     __shared__ double lds[8];
     THREAD_IJ(1, 1);
     double v = 0.0;
-    if (i <= V.ni && j <= V.nj) {
+    if (i <= V.ni && j <= V.nj && g.act(V, i, j)) {
         if (rhoeps[IX(i, j)] < 0.0) rhoeps[IX(i, j)] = fmax(rhoeps_old[IX(i, j)], 0.0);
         v = rhoe[IX(i, j)];
     }
@@ -814,7 +980,8 @@ __global__ void k_energy_guard_max(NddView V, double* __restrict__ rhoeps, const
     if (lin_tid() == 0) part[blockIdx.y * gridDim.x + blockIdx.x] = m;
 }
 
-__global__ void k_te_update(NddView V, const double* __restrict__ rhoeps, const double* __restrict__ rhoe,
+template <class G>
+__global__ void k_te_update(G g, NddView V, const double* __restrict__ rhoeps, const double* __restrict__ rhoe,
                             const double* __restrict__ part, int nb, double* __restrict__ Te)
 {
 #pragma clang fp contract(off)
@@ -825,15 +992,52 @@ __global__ void k_te_update(NddView V, const double* __restrict__ rhoeps, const 
     double mx = 0.0;
     for (int k = lin_tid(); k < nb; k += NB_X * NB_Y) mx = fmax(mx, part[k]);
     const double nmax = block_max_bcast<NB_X * NB_Y>(mx, lds);
-    if (i > V.ni || j > V.nj) return;
+    if (i > V.ni || j > V.nj || !g.act(V, i, j)) return;
     const double Te_max = 1000.0 * K_E0 / K_BOLTZ, ratio = 0.001;
     const double t = fmin(rhoeps[IX(i, j)] / (3.0 / 2.0 * (rhoe[IX(i, j)] + nmax * ratio) * K_BOLTZ + 1e-100), Te_max);
     Te[IX(i, j)] = t;
-    // the ghost ring copies its neighbour (no corners)
-    if (i == 1) Te[IX(0, j)] = t;
-    if (i == V.ni) Te[IX(V.ni + 1, j)] = t;
-    if (j == 1) Te[IX(i, 0)] = t;
-    if (j == V.nj) Te[IX(i, V.nj + 1)] = t;
+    // the ghosts of the geometry's Te list copy their cell (the rectangle:
+    // the whole ghost ring, no corners)
+    const int w = g.te_bits(V, i, j);
+    if (w & GW_XLO) Te[IX(i - 1, j)] = (w & (GW_XLO << 4)) ? t : 0.0;
+    if (w & GW_XHI) Te[IX(i + 1, j)] = (w & (GW_XHI << 4)) ? t : 0.0;
+    if (w & GW_RLO) Te[IX(i, j - 1)] = (w & (GW_RLO << 4)) ? t : 0.0;
+    if (w & GW_RHI) Te[IX(i, j + 1)] = (w & (GW_RHI << 4)) ? t : 0.0;
+}
+
+// the electron-density ghosts of a block geometry after the electron routine
+// (fluidModule.hpp:3873-3913): 0 behind walls, a copy across open sides
+template <class G>
+__global__ void k_ne_ghost(G g, NddView V, double* __restrict__ rhoe)
+{
+    THREAD_IJ(1, 1);
+    if (i > V.ni || j > V.nj || !g.act(V, i, j)) return;
+    const int w = g.ne_bits(V, i, j);
+    if (!w) return;
+    const double t = rhoe[IX(i, j)];
+    if (w & GW_XLO) rhoe[IX(i - 1, j)] = (w & (GW_XLO << 4)) ? t : 0.0;
+    if (w & GW_XHI) rhoe[IX(i + 1, j)] = (w & (GW_XHI << 4)) ? t : 0.0;
+    if (w & GW_RLO) rhoe[IX(i, j - 1)] = (w & (GW_RLO << 4)) ? t : 0.0;
+    if (w & GW_RHI) rhoe[IX(i, j + 1)] = (w & (GW_RHI << 4)) ? t : 0.0;
+}
+
+// the ion density across the open faces of a block geometry (the rectangle
+// uses k_top_copy)
+template <class G>
+__global__ void k_open_copy(G g, NddView V, double* __restrict__ a)
+{
+    THREAD_IJ(1, 1);
+    if (i > V.ni || j > V.nj || !g.act(V, i, j)) return;
+    if ((g.rf(V, i, j + 1) & GF_TYPE) == GF_OPEN) a[IX(i, j + 1)] = a[IX(i, j)];
+}
+
+// the multigrid's level-0 activity from the plasma table (pitches ld, ld0)
+__global__ void k_act_from_table(NddView V, const uint8_t* __restrict__ ac, unsigned char* __restrict__ act0, int n0)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n0) return;
+    const int i = k / V.ld0, j = k - i * V.ld0;
+    act0[k] = (i <= V.ni + 1 && j <= V.nj + 1) ? ac[IX(i, j)] : 0;
 }
 
 __global__ void k_top_copy(NddView V, double* __restrict__ a)
@@ -842,16 +1046,26 @@ __global__ void k_top_copy(NddView V, double* __restrict__ a)
     if (i <= V.ni) a[IX(i, V.nj + 1)] = a[IX(i, V.nj)];
 }
 
+// faces of the electrode at V_bias: (x face i, rows j0..j1) per segment
+constexpr int ANODE_MAX = 4;
+struct AnodeSegs {
+    int n;
+    int fi[ANODE_MAX], j0[ANODE_MAX], j1[ANODE_MAX];
+};
+
 // right-electrode current (fluidModule.hpp:3942-3965): out = {Ii, Ie}
-__global__ void k_anode(NddView V, const double* __restrict__ rhoUix, const double* __restrict__ rhoUex,
+__global__ void k_anode(NddView V, AnodeSegs E, const double* __restrict__ rhoUix, const double* __restrict__ rhoUex,
                         double* __restrict__ out)
 {
 #pragma clang fp contract(off)
     __shared__ double lds[8];
     double si = 0.0, se = 0.0;
-    for (int j = 1 + threadIdx.x; j <= V.nj; j += blockDim.x) {
-        si += rhoUix[IX(V.ni + 1, j)] * 2.0 * M_PI * V.r[j] * V.dr * (K_E0);
-        se += rhoUex[IX(V.ni + 1, j)] * 2.0 * M_PI * V.r[j] * V.dr * (-K_E0);
+    for (int q = 0; q < E.n; q++) {
+        const int fi = E.fi[q];
+        for (int j = E.j0[q] + threadIdx.x; j <= E.j1[q]; j += blockDim.x) {
+            si += rhoUix[IX(fi, j)] * 2.0 * M_PI * V.r[j] * V.dr * (K_E0);
+            se += rhoUex[IX(fi, j)] * 2.0 * M_PI * V.r[j] * V.dr * (-K_E0);
+        }
     }
     const double ti = block_sum<256>(si, lds);
     const double te = block_sum<256>(se, lds);
@@ -1044,7 +1258,23 @@ struct NddHandle {
     // first Poisson solves start from the warm start alone, then it is tried again
     bool net_useful = true, gate_pending = false;
     int net_wait = 0;
+    // block geometry (cathode_ndd_set_geometry); the rectangle when !masked
+    bool masked = false;
+    GeoMask gm{};
+    DevBuf<uint8_t> geo_u8;  // x codes, r codes, plasma cells, Te and n_e ghost bits
+    DevBuf<int> geo_gs;      // potential ghost rules
+    AnodeSegs anode{};
 };
+
+// launches kernel template K with the handle's geometry policy as the first
+// argument
+#define NDD_GEO_LAUNCH(H, K, grid, block, ...)                                                   \
+    do {                                                                                          \
+        if ((H).masked)                                                                           \
+            hipLaunchKernelGGL(K<GeoMask>, grid, block, 0, (H).s, (H).gm, __VA_ARGS__);           \
+        else                                                                                      \
+            hipLaunchKernelGGL(K<GeoRect>, grid, block, 0, (H).s, GeoRect{}, __VA_ARGS__);        \
+    } while (0)
 
 enum {
     F_PHI, F_RHOE, F_RHOI, F_TE, F_RHOEPS, F_RHOE_OLD, F_EX, F_RHOUEX, F_NUEX, F_GX, F_RHOUIX,
@@ -1134,7 +1364,7 @@ static bool solve(NddHandle& H, double gate, int maxITR, double tol, int slot)
 static void calcres3_async(NddHandle& H, const ResPairs& P)
 {
     const NddView& V = H.V;
-    hipLaunchKernelGGL(k_calcres, ggrid(V.ni, V.nj), gblk(), 0, H.s, V, P, H.p.flag_error, 1e-100, H.part.p, H.nb);
+    NDD_GEO_LAUNCH(H, k_calcres, ggrid(V.ni, V.nj), gblk(), V, P, H.p.flag_error, 1e-100, H.part.p, H.nb);
     hipLaunchKernelGGL(k_calcres_finish, dim3(1), dim3(256), 0, H.s, H.part.p, H.nb, H.p.flag_error, H.scal.p,
                        H.mg.side_slot());
     CATHODE_LAUNCH_CHECK();
@@ -1144,7 +1374,7 @@ static void calcres3_async(NddHandle& H, const ResPairs& P)
 static double calcres3(NddHandle& H, const ResPairs& P, double* err_out)
 {
     const NddView& V = H.V;
-    hipLaunchKernelGGL(k_calcres, ggrid(V.ni, V.nj), gblk(), 0, H.s, V, P, H.p.flag_error, 1e-100, H.part.p, H.nb);
+    NDD_GEO_LAUNCH(H, k_calcres, ggrid(V.ni, V.nj), gblk(), V, P, H.p.flag_error, 1e-100, H.part.p, H.nb);
     hipLaunchKernelGGL(k_calcres_finish, dim3(1), dim3(256), 0, H.s, H.part.p, H.nb, H.p.flag_error, H.scal.p,
                        (double*)nullptr);
     CATHODE_LAUNCH_CHECK();
@@ -1271,11 +1501,11 @@ static void poisson_routine(NddHandle& H, bool report)
     d2d(H, {{H.phi_old, F[F_PHI]}, {H.Ex_old, F[F_EX]}, {H.Er_old, F[F_ER]}, {H.sx_old, F[F_NUEX]},
             {H.sr_old, F[F_NUER]}, {H.tmp_c, F[F_PHI]}, {H.tmp_x, F[F_NUEX]}, {H.tmp_r, F[F_NUER]},
             {H.tmp2_x, F[F_NUEX]}, {H.tmp2_r, F[F_NUER]}});
-    hipLaunchKernelGGL(k_poisson_ghost, dim3(cdiv(std::max(V.ni, V.nj), 256)), dim3(256), 0, H.s, V, F[F_PHI],
-                       H.tmp_c);
+    NDD_GEO_LAUNCH(H, k_poisson_ghost, ggrid(V.ni + 2, V.nj + 2), gblk(), V, F[F_PHI], H.tmp_c);
     PoissonArgs A{};
     A.phi = F[F_PHI]; A.rhoe = F[F_RHOE]; A.rhoi = F[F_RHOI]; A.Te = F[F_TE]; A.rhoe_old = F[F_RHOE_OLD];
-    A.Ex = F[F_EX]; A.Er = F[F_ER]; A.Ex_old = H.Ex_old; A.rhoUix = F[F_RHOUIX];
+    A.Ex = F[F_EX]; A.Er = F[F_ER]; A.Ex_old = H.Ex_old; A.Er_old = H.Er_old; A.rhoUix = F[F_RHOUIX];
+    A.rhoUir = F[F_RHOUIR];
     A.S = H.has_src[0] ? F[F_S] : nullptr;
     A.nUex = F[F_NUEX]; A.nUer = F[F_NUER];
     A.m = H.me;
@@ -1286,12 +1516,12 @@ static void poisson_routine(NddHandle& H, bool report)
         double* const* Sn = snap[ncount & 1];
         A.nUex_t = S[1];
         A.nUer_t = S[2];
-        hipLaunchKernelGGL(k_poisson_assemble, ggrid(V.ni, V.nj), gblk(), 0, H.s, V, A, H.mg.aP(), H.mg.aE(),
-                           H.mg.aW(), H.mg.aN(), H.mg.aS(), H.mg.rhs(), H.mg.prep_s(), H.mg.prep_part());
+        NDD_GEO_LAUNCH(H, k_poisson_assemble, ggrid(V.ni, V.nj), gblk(), V, A, H.mg.aP(), H.mg.aE(), H.mg.aW(),
+                       H.mg.aN(), H.mg.aS(), H.mg.rhs(), H.mg.prep_s(), H.mg.prep_part());
         H.mg.prepared = true;
         CATHODE_LAUNCH_CHECK();
         bool use_net = false;
-        if (ncount == 1 && H.p.learned && H.cnn) {
+        if (ncount == 1 && H.p.learned && H.cnn && !H.masked) {
             // the previous use's residual ratio (written by the device, read
             // here after the syncs of the steps since)
             if (H.gate_pending) {
@@ -1315,9 +1545,9 @@ static void poisson_routine(NddHandle& H, bool report)
             break;
         }
         if (learn && (H.cap_feat || !H.p.learned)) learned_store(H);  // learned mode reads its warm slot
-        hipLaunchKernelGGL(k_poisson_update, ggrid(V.ni + 2, V.nj + 2), gblk(), 0, H.s, V, H.mg.sol(),
-                           (const double*)S[0], H.dphi, F[F_PHI], F[F_EX], F[F_ER], (const double*)F[F_RHOE],
-                           F[F_NUEX], F[F_NUER], H.me, Sn[0], Sn[1], Sn[2]);
+        NDD_GEO_LAUNCH(H, k_poisson_update, ggrid(V.ni + 2, V.nj + 2), gblk(), V, (const double*)H.mg.sol(),
+                       (const double*)S[0], H.dphi, F[F_PHI], F[F_EX], F[F_ER], (const double*)F[F_RHOE],
+                       F[F_NUEX], F[F_NUER], H.me, Sn[0], Sn[1], Sn[2]);
         warm_store(H, warm_slot(EQ_POISSON, ncount));
         CATHODE_LAUNCH_CHECK();
         if (ncount >= H.p.maxITR_HES_phi) break;
@@ -1341,7 +1571,8 @@ static void sg_routine(NddHandle& H, int sp, bool report, double* err3)
     SgArgs A{};
     A.sp = sp;
     A.Ex = F[F_EX]; A.Er = F[F_ER]; A.Te = F[F_TE]; A.rhoi = F[F_RHOI]; A.rhoe = F[F_RHOE];
-    A.rhoe_old = F[F_RHOE_OLD]; A.rhoUix = F[F_RHOUIX]; A.rhoUex = F[F_RHOUEX]; A.rhoUer = F[F_RHOUER];
+    A.rhoe_old = F[F_RHOE_OLD]; A.rhoUix = F[F_RHOUIX]; A.rhoUir = F[F_RHOUIR]; A.rhoUex = F[F_RHOUEX];
+    A.rhoUer = F[F_RHOUER];
     A.S = H.has_src[0] ? F[F_S] : nullptr;
     A.Pabs = H.has_src[1] ? F[F_PABS] : nullptr;
     A.eloss = H.has_src[2] ? F[F_ELOSS] : nullptr;
@@ -1385,8 +1616,8 @@ static void sg_routine(NddHandle& H, int sp, bool report, double* err3)
         A.Fx_t = S[1];
         A.Fr_t = S[2];
         A.reuse = ncount > 1;
-        hipLaunchKernelGGL(k_sg_assemble, ggrid(V.ni, V.nj), gblk(), 0, H.s, V, A, H.mg.aP(), H.mg.aE(), H.mg.aW(),
-                           H.mg.aN(), H.mg.aS(), H.mg.rhs(), H.mg.prep_s(), H.mg.prep_part());
+        NDD_GEO_LAUNCH(H, k_sg_assemble, ggrid(V.ni, V.nj), gblk(), V, A, H.mg.aP(), H.mg.aE(), H.mg.aW(),
+                       H.mg.aN(), H.mg.aS(), H.mg.rhs(), H.mg.prep_s(), H.mg.prep_part());
         H.mg.prepared = true;
         CATHODE_LAUNCH_CHECK();
         const int eq = sp == SP_ELEC ? EQ_ELEC : (sp == SP_ION ? EQ_ION : EQ_ENERGY);
@@ -1399,7 +1630,8 @@ static void sg_routine(NddHandle& H, int sp, bool report, double* err3)
             d2d(H, {{A.Fx, S[1]}, {A.Fr, S[2]}});
             break;
         }
-        hipLaunchKernelGGL(k_sg_correct, ggrid(V.ni, V.nj), gblk(), 0, H.s, V, A, H.mg.sol(), Sn[0], Sn[1], Sn[2]);
+        NDD_GEO_LAUNCH(H, k_sg_correct, ggrid(V.ni, V.nj), gblk(), V, A, (const double*)H.mg.sol(), Sn[0], Sn[1],
+                       Sn[2]);
         CATHODE_LAUNCH_CHECK();
         warm_store(H, warm_slot(eq, ncount));
         if (ncount >= maxit) break;
@@ -1441,11 +1673,18 @@ static void ndd_one_step(NddHandle& H, bool report)
     // 1. ions
     sg_routine(H, SP_ION, report, e3);
     if (report) H.rep.error_rhoi = e3[0];
-    hipLaunchKernelGGL(k_top_copy, dim3(cdiv(V.ni, 256)), dim3(256), 0, H.s, V, F[F_RHOI]);
+    if (H.masked)
+        hipLaunchKernelGGL(k_open_copy<GeoMask>, ggrid(V.ni, V.nj), gblk(), 0, H.s, H.gm, V, F[F_RHOI]);
+    else
+        hipLaunchKernelGGL(k_top_copy, dim3(cdiv(V.ni, 256)), dim3(256), 0, H.s, V, F[F_RHOI]);
     // 2. Poisson
     poisson_routine(H, report);
     // 3. electrons
     sg_routine(H, SP_ELEC, report, e3);
+    if (H.masked) {
+        hipLaunchKernelGGL(k_ne_ghost<GeoMask>, ggrid(V.ni, V.nj), gblk(), 0, H.s, H.gm, V, F[F_RHOE]);
+        CATHODE_LAUNCH_CHECK();
+    }
     if (report) {
         H.rep.error_rhoe = e3[0];
         H.rep.error_rhoUex = e3[1];
@@ -1460,10 +1699,10 @@ static void ndd_one_step(NddHandle& H, bool report)
             H.rep.error_Gx = e3[1];
             H.rep.error_Gr = e3[2];
         }
-        hipLaunchKernelGGL(k_energy_guard_max, ggrid(V.ni, V.nj), gblk(), 0, H.s, V, F[F_RHOEPS], H.s_old, F[F_RHOE],
-                           H.part.p);
-        hipLaunchKernelGGL(k_te_update, ggrid(V.ni, V.nj), gblk(), 0, H.s, V, F[F_RHOEPS], F[F_RHOE],
-                           (const double*)H.part.p, H.nb, F[F_TE]);
+        NDD_GEO_LAUNCH(H, k_energy_guard_max, ggrid(V.ni, V.nj), gblk(), V, F[F_RHOEPS], (const double*)H.s_old,
+                       (const double*)F[F_RHOE], H.part.p);
+        NDD_GEO_LAUNCH(H, k_te_update, ggrid(V.ni, V.nj), gblk(), V, (const double*)F[F_RHOEPS],
+                       (const double*)F[F_RHOE], (const double*)H.part.p, H.nb, F[F_TE]);
         CATHODE_LAUNCH_CHECK();
     }
     H.rep.steps++;
@@ -1529,6 +1768,10 @@ int cathode_ndd_create(const cathode_ndd_params* p, void** handle)
         const int n0 = (ni + 2) * V.ld0;
         hipLaunchKernelGGL(k_fill_one, dim3(cdiv(n0, 256)), dim3(256), 0, H->s, n0, H->mg.act(), ni, nj, V.ld0);
         CATHODE_LAUNCH_CHECK();
+        H->anode.n = 1;
+        H->anode.fi[0] = ni + 1;
+        H->anode.j0[0] = 1;
+        H->anode.j1[0] = nj;
         H->mg.set_weights(H->radii.p);  // conservative rows: w_j = r_j
         H->nb = cdiv(nj, NB_X) * cdiv(ni, NB_Y);
         H->part.alloc(6 * H->nb + 64);
@@ -1563,6 +1806,69 @@ int cathode_ndd_destroy(void* handle)
     delete H;
     if (s) (void)hipStreamDestroy(s);
     return CATHODE_OK;
+}
+
+int cathode_ndd_set_geometry(void* handle, const cathode_ndd_geometry* g)
+{
+    CATHODE_GUARD_BEGIN
+    NddHandle* H = (NddHandle*)handle;
+    if (!H) {
+        set_error("ndd_set_geometry: null handle");
+        return CATHODE_ERR_ARG;
+    }
+    NddView& V = H->V;
+    const int n0 = (V.ni + 2) * V.ld0;
+    if (!g) {
+        H->masked = false;
+        hipLaunchKernelGGL(k_fill_one, dim3(cdiv(n0, 256)), dim3(256), 0, H->s, n0, H->mg.act(), V.ni, V.nj, V.ld0);
+        CATHODE_LAUNCH_CHECK();
+        H->anode.n = 1;
+        H->anode.fi[0] = V.ni + 1;
+        H->anode.j0[0] = 1;
+        H->anode.j1[0] = V.nj;
+        CATHODE_HIP(hipStreamSynchronize(H->s));
+        return CATHODE_OK;
+    }
+    GeoTables T;
+    std::string err;
+    if (!build_geo_tables(V.ni, V.nj, *g, T, err)) {
+        set_error(err.c_str());
+        return CATHODE_ERR_ARG;
+    }
+    if ((int)T.anode.size() / 3 > ANODE_MAX) {
+        set_error("ndd_set_geometry: more than 4 biased high-x wall segments");
+        return CATHODE_ERR_ARG;
+    }
+    const size_t n = T.xc.size();
+    H->geo_u8.alloc(5 * n);
+    H->geo_gs.alloc(n);
+    std::vector<uint8_t> u8(5 * n);
+    std::memcpy(u8.data(), T.xc.data(), n);
+    std::memcpy(u8.data() + n, T.rc.data(), n);
+    std::memcpy(u8.data() + 2 * n, T.act.data(), n);
+    std::memcpy(u8.data() + 3 * n, T.te.data(), n);
+    std::memcpy(u8.data() + 4 * n, T.ne.data(), n);
+    CATHODE_HIP(hipMemcpyAsync(H->geo_u8.p, u8.data(), 5 * n, hipMemcpyHostToDevice, H->s));
+    CATHODE_HIP(hipMemcpyAsync(H->geo_gs.p, T.gs.data(), n * sizeof(int), hipMemcpyHostToDevice, H->s));
+    H->gm.xc = H->geo_u8.p;
+    H->gm.rc = H->geo_u8.p + n;
+    H->gm.ac = H->geo_u8.p + 2 * n;
+    H->gm.teb = H->geo_u8.p + 3 * n;
+    H->gm.neb = H->geo_u8.p + 4 * n;
+    H->gm.gs = H->geo_gs.p;
+    hipLaunchKernelGGL(k_act_from_table, dim3(cdiv(n0, 256)), dim3(256), 0, H->s, V, (const uint8_t*)H->gm.ac,
+                       H->mg.act(), n0);
+    CATHODE_LAUNCH_CHECK();
+    H->anode.n = (int)T.anode.size() / 3;
+    for (int q = 0; q < H->anode.n; q++) {
+        H->anode.fi[q] = T.anode[3 * q];
+        H->anode.j0[q] = T.anode[3 * q + 1];
+        H->anode.j1[q] = T.anode[3 * q + 2];
+    }
+    H->masked = true;
+    CATHODE_HIP(hipStreamSynchronize(H->s));  // the host tables go out of scope
+    return CATHODE_OK;
+    CATHODE_GUARD_END
 }
 
 int cathode_ndd_set_learned(void* handle, void* net)
@@ -1648,7 +1954,8 @@ static int ndd_run(NddHandle* H, int nsteps, cathode_ndd_report* rep)
         std::memset(&H->rep, 0, sizeof(H->rep));
         for (int s = 0; s < nsteps; s++) ndd_one_step(*H, rep != nullptr && s == nsteps - 1);
         if (rep && nsteps > 0) {
-            hipLaunchKernelGGL(k_anode, dim3(1), dim3(256), 0, H->s, H->V, H->F[F_RHOUIX], H->F[F_RHOUEX], H->scal.p + 12);
+            hipLaunchKernelGGL(k_anode, dim3(1), dim3(256), 0, H->s, H->V, H->anode, (const double*)H->F[F_RHOUIX],
+                               (const double*)H->F[F_RHOUEX], H->scal.p + 12);
             CATHODE_LAUNCH_CHECK();
             fetch_host(H->h_scal, H->scal.p + 12, 2, H->s);
             fetch_host(H->h_scal + 16, H->scal.p + 16, 6, H->s);

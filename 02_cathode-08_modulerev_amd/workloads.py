@@ -196,3 +196,70 @@ def config3(seed=3, count=64, h=1e-4, mode=abi.CATHODE_MODE_MIXED, sizes=SWEEP_S
 
 def copy_state(st):
     return {k: v.copy() for k, v in st.items()}
+
+
+# ------------------------------------------------------------------ five-block cathode
+# block corners of the reference's cathode (include/initialModule.hpp:222-232)
+# and its domain (include/params.hpp:16-19): x1..x6, xR and r1..r5, rmax [m]
+REF_XS = (0.0e-3, 5.6e-3 + 7.8e-3, 10.0e-3 + 7.8e-3, 15.0e-3 + 7.8e-3, 20.0e-3 + 7.8e-3, 0.0e-3 + 7.8e-3, 0.0388)
+REF_RS = (0.6e-3, 2.0e-3, 4.0e-3, 9.0e-3, 2.0e-3, 0.014)
+
+
+def reference_geometry(h, lib=None):
+    """The reference's five-block cathode at spacing h in both directions:
+    (ni, nj, abi.NddGeometry, six cell boxes, six x-face boxes, six r-face
+    boxes) from cathode_ndd_reference_geometry (a host-only call)."""
+    import ctypes
+    lib = lib or abi.load()
+    g = abi.NddGeometry()
+    xs = np.ascontiguousarray(REF_XS, dtype=np.float64)
+    rs = np.ascontiguousarray(REF_RS, dtype=np.float64)
+    c6, x6, r6 = (np.zeros(24, dtype=np.int32) for _ in range(3))
+    rc = lib.cathode_ndd_reference_geometry(abi.dptr(xs), abi.dptr(rs), h, h, ctypes.byref(g),
+                                            c6.ctypes.data_as(abi.c_int_p), x6.ctypes.data_as(abi.c_int_p),
+                                            r6.ctypes.data_as(abi.c_int_p))
+    abi.check(rc, lib)
+    ni = int(round(REF_XS[6] / h))
+    nj = int(round(REF_RS[5] / h))
+    return ni, nj, g, c6.reshape(6, 4), x6.reshape(6, 4), r6.reshape(6, 4)
+
+
+def plasma_mask(g, ni, nj):
+    """plasma cells of a geometry on the (ni+2, nj+2) grid"""
+    m = np.zeros((ni + 2, nj + 2), dtype=bool)
+    for b in range(g.n_bl):
+        i0, i1, j0, j1 = (g.cell_box[4 * b + q] for q in range(4))
+        m[i0:i1 + 1, j0:j1 + 1] = True
+    return m
+
+
+def cathode5(seed=5, h=4e-4, energy_on=True, mode=abi.CATHODE_MODE_FP64, dt=1e-10):
+    """The reference's five-block cathode geometry at spacing h (the
+    reference's own grid is h = 0.2 mm, 194 x 70 cells), electrode at 30 V:
+    n_e = n_i = 1e15 + two seeded Gaussians (peaks log-uniform in 1e16..1e17,
+    sigma 1..2 mm) in the plasma blocks and 0 outside, Te = 3..6 eV smooth,
+    phi = 0, fields and fluxes 0.  Returns (params, state, geometry)."""
+    ni, nj, g, _, _, _ = reference_geometry(h)
+    if abs(ni * h - REF_XS[6]) > 1e-9 or abs(nj * h - REF_RS[5]) > 1e-9:
+        raise ValueError("h must divide the domain 38.8 mm x 14 mm")
+    p = default_params(ni, nj, h, h, dt=dt, V_bias=30.0, energy_on=energy_on, mode=mode)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    mask = plasma_mask(g, ni, nj)
+    Z = np.broadcast_to(axial(ni, h)[:, None], (ni + 2, nj + 2))
+    R = np.broadcast_to(radii(nj, h)[None, :], (ni + 2, nj + 2))
+    n = np.full((ni + 2, nj + 2), 1e15)
+    pts = np.argwhere(mask)
+    for _ in range(2):
+        ci, cj = pts[rng.integers(len(pts))]
+        peak = 10.0 ** rng.uniform(16.0, 17.0)
+        sig = rng.uniform(1e-3, 2e-3)
+        n += peak * np.exp(-((Z - Z[ci, cj]) ** 2 + (R - R[ci, cj]) ** 2) / (2 * sig * sig))
+    n = np.where(mask, n, 0.0)
+    Te = (3.0 + 3.0 * rng.uniform(0.0, 1.0)) * E0 / BOLTZ * np.ones((ni + 2, nj + 2))
+    st = {k: np.zeros((ni + 2, nj + 2)) for k in abi.STATE_NAMES}
+    st["rhoe"][...] = n
+    st["rhoi"][...] = n
+    st["rhoe_old"][...] = n
+    st["Te"][...] = Te
+    st["rhoeps"][...] = 1.5 * n * BOLTZ * Te
+    return p, st, g

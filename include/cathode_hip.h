@@ -327,6 +327,68 @@ typedef struct cathode_ndd_report {
 } cathode_ndd_report;
 
 int cathode_ndd_create(const cathode_ndd_params* p, void** handle);
+
+/* ------------------------------------------------------------------ */
+/* block geometry of the plasma region (include/initialModule.hpp:239-399 */
+/* and the boundary loops of the PC routines in fluidModule.hpp)       */
+/* ------------------------------------------------------------------ */
+/* A handle starts on the rectangle 1..ni x 1..nj with the reference's
+ * block-4 boundary set (left wall phi = 0, right electrode phi = V_bias,
+ * open top, axis below).  cathode_ndd_set_geometry replaces it by a union of
+ * plasma blocks and lists of boundary segments, as the reference's
+ * multi-block cathode does:
+ *  - cell_box / xface_box / rface_box: gc.i/j_flc_bl, gx.i/j_flx_bl,
+ *    gr.i/j_flr_bl of the plasma blocks (the reference's blocks 0 ..
+ *    n_bl - 2), inclusive {i0, i1, j0, j1}; the face boxes hold the faces
+ *    between two plasma cells, where the Scharfetter-Gummel fluxes, field
+ *    updates and calcRes run;
+ *  - seg: the boundary faces in the reference's order (z0, z1, z2, z4, z3,
+ *    z5, x0, x2, x1, x4, x5, x6, the order of its ghost writes,
+ *    fluidModule.hpp:2152-2249); a segment names plasma cells (cell, lo..hi)
+ *    and the side of their face; flag = 1 puts the wall at V_bias;
+ *  - ne: the electron-density ghosts set after the electron routine
+ *    (fluidModule.hpp:3873-3913): flag 0 writes 0, flag 1 copies the cell;
+ *  - te: the Te ghosts copied after the energy routine (:4906-4946).
+ * Every face of a plasma cell must be an inner face or lie in exactly one
+ * segment of its side; the call rejects a geometry that breaks this,
+ * overlapping boxes, or boxes outside 1..ni x 1..nj. */
+#define CATHODE_NDD_MAX_BLOCKS 8
+#define CATHODE_NDD_MAX_SEGS 24
+enum {
+    CATHODE_SEG_X_LOW = 0,   /* wall on the low-x face (face i) of cells (i = cell, j = lo..hi) */
+    CATHODE_SEG_X_HIGH = 1,  /* wall on the high-x face (face i + 1) */
+    CATHODE_SEG_R_LOW = 2,   /* wall on the low-r face (face j) of cells (i = lo..hi, j = cell) */
+    CATHODE_SEG_R_HIGH = 3,  /* wall on the high-r face (face j + 1) */
+    CATHODE_SEG_R_OPEN = 4,  /* open high-r face: grad phi = 0, electron flux = ion flux */
+    CATHODE_SEG_R_AXIS = 5   /* symmetry axis below the cells: no flux, no ghost */
+};
+typedef struct cathode_ndd_segment {
+    int kind, cell, lo, hi, flag;
+} cathode_ndd_segment;
+typedef struct cathode_ndd_geometry {
+    int n_bl;
+    int cell_box[4 * CATHODE_NDD_MAX_BLOCKS];
+    int xface_box[4 * CATHODE_NDD_MAX_BLOCKS];
+    int rface_box[4 * CATHODE_NDD_MAX_BLOCKS];
+    int n_seg, n_ne, n_te;
+    cathode_ndd_segment seg[CATHODE_NDD_MAX_SEGS];
+    cathode_ndd_segment ne[CATHODE_NDD_MAX_SEGS];
+    cathode_ndd_segment te[CATHODE_NDD_MAX_SEGS];
+} cathode_ndd_geometry;
+
+/* NULL restores the rectangle.  Call before cathode_ndd_set_state; the
+ * learned network is not used on a block geometry (the warm starts are). */
+int cathode_ndd_set_geometry(void* handle, const cathode_ndd_geometry* g);
+
+/* The reference's five-block cathode: the block boxes of
+ * initialModule.hpp:239-343 from xs = {x1, x2, x3, x4, x5, x6, xR} and
+ * rs = {r1, r2, r3, r4, r5, rmax} [m] at spacing dx, dr, and the segment
+ * lists of the PC routines.  cells6 / xfaces6 / rfaces6 (24 ints each, may be
+ * NULL) receive all six block boxes (the five plasma blocks and the
+ * dielectric), as the microwave profile takes them.  Host only. */
+int cathode_ndd_reference_geometry(const double* xs, const double* rs, double dx, double dr,
+                                   cathode_ndd_geometry* g, int* cells6, int* xfaces6, int* rfaces6);
+
 int cathode_ndd_set_state(void* handle, const cathode_ndd_fields* host);
 int cathode_ndd_get_state(void* handle, cathode_ndd_fields* host);
 int cathode_ndd_step(void* handle, int nsteps, cathode_ndd_report* rep);

@@ -110,7 +110,9 @@ enum {
     ORC_PLANT_MW_HALL_SIGN = 10,      /* x-row phi entry of the conductivity with -h_r */
     ORC_PLANT_MW_POYNTING_CONJ = 11,  /* Poynting fluxes with E H instead of E conj(H) */
     ORC_PLANT_MW_PORT_WAVES = 12,     /* forward and reflected waves at the port swapped */
-    ORC_PLANT_MW_RFACE_WEIGHT = 13    /* r-face power volume weighted with the cell radius */
+    ORC_PLANT_MW_RFACE_WEIGHT = 13,   /* r-face power volume weighted with the cell radius */
+    ORC_PLANT_GEO_FOLD = 14,          /* Poisson: the folds of the high-r walls left out */
+    ORC_PLANT_GEO_WALL_COEF = 15      /* SG: wall faces keep their implicit coefficient */
 };
 void orc_plant(int which);
 int orc_planted(void);
@@ -165,6 +167,14 @@ int orc_ndd_step(const cathode_ndd_params* p, cathode_ndd_fields* f,
  * which = 0 Poisson (first PC iteration), 1 electrons, 2 ions, 3 energy.
  * Fills aP..aS, b ((ni+2)*(nj+2) each, ld = nj+2) after the BC folds, and
  * the predicted face fluxes into fx / fr. */
+/* The plasma geometry of the following orc_ndd_step / orc_ndd_assemble calls
+ * (blocks, boundary segments and ghost lists, cathode_hip.h); NULL: the
+ * rectangle 1..ni x 1..nj with the block-4 boundary set (the default). */
+void orc_ndd_set_geometry(const cathode_ndd_geometry* g);
+/* calcRes over the union of nbox boxes {i0, i1, j0, j1}, block by block */
+double orc_calc_res_boxes(const double* vn, const double* vo, int ld, const int* boxes, int nbox, double eps,
+                          int mode);
+
 int orc_ndd_assemble(const cathode_ndd_params* p, const cathode_ndd_fields* f,
                      int which, double* aP, double* aE, double* aW, double* aN,
                      double* aS, double* b, double* fx, double* fr);

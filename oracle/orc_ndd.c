@@ -63,54 +63,93 @@ double orc_func_Te(const double* v, int n, double Te, double dTe)
     return v[n - 1];
 }
 
-/* calcRes, miniFuncs.hpp:292-375 */
-double orc_calc_res(const double* vn, const double* vo, int ld, int i0, int i1,
-                    int j0, int j1, double eps, int mode)
+/* calcRes, miniFuncs.hpp:292-375, over the union of nbox boxes {i0, i1, j0, j1}
+ * (block by block, as the reference loops) */
+double orc_calc_res_boxes(const double* vn, const double* vo, int ld, const int* boxes, int nbox, double eps,
+                          int mode)
 {
-    double err = 0.0;
-    if (mode == 0) {
-        for (int i = i0; i <= i1; i++)
-            for (int j = j0; j <= j1; j++) {
-                double e = fabs(vn[i * ld + j] - vo[i * ld + j]);
-                if (e > err) err = e;
-            }
-    } else if (mode == 1) {
-        int n = 0;
-        for (int i = i0; i <= i1; i++)
-            for (int j = j0; j <= j1; j++) {
+    double err = 0.0, nrm = 0.0;
+    int n = 0;
+    for (int b = 0; b < nbox; b++) {
+        const int* B = boxes + 4 * b;
+        for (int i = B[0]; i <= B[1]; i++)
+            for (int j = B[2]; j <= B[3]; j++) {
                 double d = vn[i * ld + j] - vo[i * ld + j];
-                err += (d * d) / (vo[i * ld + j] * vo[i * ld + j] + eps);
-                n++;
+                if (mode == 0) {
+                    double e = fabs(d);
+                    if (e > err) err = e;
+                } else if (mode == 1) {
+                    err += (d * d) / (vo[i * ld + j] * vo[i * ld + j] + eps);
+                    n++;
+                } else if (mode == 2) {
+                    double e = fabs(d);
+                    if (e > err) err = e;
+                    double a = fabs(vo[i * ld + j]);
+                    if (a > nrm) nrm = a;
+                } else {
+                    err += d * d;
+                    nrm += vo[i * ld + j] * vo[i * ld + j];
+                }
             }
-        err = sqrt(err / (double)n);
-    } else if (mode == 2) {
-        double nrm = 0.0;
-        for (int i = i0; i <= i1; i++)
-            for (int j = j0; j <= j1; j++) {
-                double e = fabs(vn[i * ld + j] - vo[i * ld + j]);
-                if (e > err) err = e;
-                double a = fabs(vo[i * ld + j]);
-                if (a > nrm) nrm = a;
-            }
-        err = err / nrm;
-    } else {
-        double nrm = 0.0;
-        for (int i = i0; i <= i1; i++)
-            for (int j = j0; j <= j1; j++) {
-                double d = vn[i * ld + j] - vo[i * ld + j];
-                err += d * d;
-                nrm += vo[i * ld + j] * vo[i * ld + j];
-            }
-        err = sqrt(err / nrm);
     }
+    if (mode == 1) err = sqrt(err / (double)n);
+    else if (mode == 2) err = err / nrm;
+    else if (mode == 3) err = sqrt(err / nrm);
     return err;
 }
 
+/* calcRes over one box */
+double orc_calc_res(const double* vn, const double* vo, int ld, int i0, int i1,
+                    int j0, int j1, double eps, int mode)
+{
+    int box[4] = {i0, i1, j0, j1};
+    return orc_calc_res_boxes(vn, vo, ld, box, 1, eps, mode);
+}
+
 /* ------------------------------------------------------------------ */
+/* ------------------------------------------------------------------ */
+/* Plasma geometry: the reference's plasma blocks (cell boxes, inner face  */
+/* boxes) and its boundary segments (cathode_hip.h).  NULL: the rectangle */
+/* with the block-4 boundary set.  Face categories of the masks below.    */
+/* ------------------------------------------------------------------ */
+enum { FC_NONE = 0, FC_INNER = 1, FC_WALL = 2, FC_OPEN = 3, FC_AXIS = 4 };
+static int g_have_geo = 0;
+static cathode_ndd_geometry g_geo;
+void orc_ndd_set_geometry(const cathode_ndd_geometry* g)
+{
+    g_have_geo = g != NULL;
+    if (g) g_geo = *g;
+}
+
+static void rect_geo(int ni, int nj, cathode_ndd_geometry* g)
+{
+    memset(g, 0, sizeof(*g));
+    g->n_bl = 1;
+    int c[4] = {1, ni, 1, nj}, x[4] = {2, ni, 1, nj}, r[4] = {1, ni, 2, nj};
+    for (int q = 0; q < 4; q++) {
+        g->cell_box[q] = c[q];
+        g->xface_box[q] = x[q];
+        g->rface_box[q] = r[q];
+    }
+    cathode_ndd_segment seg[4] = {{CATHODE_SEG_X_LOW, 1, 1, nj, 0}, {CATHODE_SEG_X_HIGH, ni, 1, nj, 1},
+                                  {CATHODE_SEG_R_OPEN, nj, 1, ni, 0}, {CATHODE_SEG_R_AXIS, 1, 1, ni, 0}};
+    cathode_ndd_segment te[4] = {{CATHODE_SEG_X_LOW, 1, 1, nj, 1}, {CATHODE_SEG_X_HIGH, ni, 1, nj, 1},
+                                 {CATHODE_SEG_R_LOW, 1, 1, ni, 1}, {CATHODE_SEG_R_HIGH, nj, 1, ni, 1}};
+    g->n_seg = 4;
+    g->n_te = 4;
+    for (int q = 0; q < 4; q++) {
+        g->seg[q] = seg[q];
+        g->te[q] = te[q];
+    }
+}
+
 typedef struct {
     const cathode_ndd_params* p;
     int ni, nj, ld;
     size_t n;
+    const cathode_ndd_geometry* G;
+    cathode_ndd_geometry Gbuf;
+    unsigned char *xc, *rc; /* face categories */
     double* r; /* cell-centre radii r[0..nj+1] */
     double *aP, *aE, *aW, *aN, *aS, *b;
     double *x, *tmp_c, *tmp_x, *tmp_r, *Ex_old, *Er_old;
@@ -137,11 +176,39 @@ static void ctx_init(ctx_t* C, const cathode_ndd_params* p, int solver, cathode_
     for (size_t k = 0; k < sizeof(bufs) / sizeof(bufs[0]); k++) *bufs[k] = (double*)calloc(C->n, sizeof(double));
     C->solver = solver;
     C->rep = rep;
+    if (g_have_geo) C->Gbuf = g_geo;
+    else rect_geo(p->ni, p->nj, &C->Gbuf);
+    C->G = &C->Gbuf;
+    const cathode_ndd_geometry* G = C->G;
+    C->xc = (unsigned char*)calloc(C->n, 1);
+    C->rc = (unsigned char*)calloc(C->n, 1);
+    for (int b = 0; b < G->n_bl; b++) {
+        const int *X = G->xface_box + 4 * b, *R = G->rface_box + 4 * b;
+        for (int i = X[0]; i <= X[1]; i++)
+            for (int j = X[2]; j <= X[3]; j++) C->xc[IX(i, j)] = FC_INNER;
+        for (int i = R[0]; i <= R[1]; i++)
+            for (int j = R[2]; j <= R[3]; j++) C->rc[IX(i, j)] = FC_INNER;
+    }
+    for (int s = 0; s < G->n_seg; s++) {
+        const cathode_ndd_segment* q = &G->seg[s];
+        for (int t = q->lo; t <= q->hi; t++) {
+            switch (q->kind) {
+            case CATHODE_SEG_X_LOW: C->xc[IX(q->cell, t)] = FC_WALL; break;
+            case CATHODE_SEG_X_HIGH: C->xc[IX(q->cell + 1, t)] = FC_WALL; break;
+            case CATHODE_SEG_R_LOW: C->rc[IX(t, q->cell)] = FC_WALL; break;
+            case CATHODE_SEG_R_HIGH: C->rc[IX(t, q->cell + 1)] = FC_WALL; break;
+            case CATHODE_SEG_R_OPEN: C->rc[IX(t, q->cell + 1)] = FC_OPEN; break;
+            default: C->rc[IX(t, q->cell)] = FC_AXIS; break;
+            }
+        }
+    }
 }
 
 static void ctx_free(ctx_t* C)
 {
     free(C->r);
+    free(C->xc);
+    free(C->rc);
     double* bufs[] = {C->aP, C->aE, C->aW, C->aN, C->aS, C->b, C->x,
                       C->tmp_c, C->tmp_x, C->tmp_r, C->Ex_old, C->Er_old, C->zero};
     for (size_t k = 0; k < sizeof(bufs) / sizeof(bufs[0]); k++) free(bufs[k]);
@@ -150,15 +217,17 @@ static void ctx_free(ctx_t* C)
 /* the linear solve handed to solver_SMG in the reference */
 static void solve5(ctx_t* C, double* x, int maxITR, double tol)
 {
-    int box[4] = {1, C->ni, 1, C->nj};
+    /* solver_SMG over the plasma blocks' cell boxes */
+    const int* box = C->G->cell_box;
+    const int nbox = C->G->n_bl;
     double rel = 0.0;
     int its = 0;
     if (C->solver == 0) {
-        orc_struct5_direct(C->aP, C->aE, C->aW, C->aN, C->aS, C->b, box, 1, C->ni + 2, C->nj + 2, C->ld, x);
-        rel = orc_rel_residual(C->aP, C->aE, C->aW, C->aN, C->aS, C->b, box, 1, C->ni + 2, C->nj + 2, C->ld, x);
+        orc_struct5_direct(C->aP, C->aE, C->aW, C->aN, C->aS, C->b, box, nbox, C->ni + 2, C->nj + 2, C->ld, x);
+        rel = orc_rel_residual(C->aP, C->aE, C->aW, C->aN, C->aS, C->b, box, nbox, C->ni + 2, C->nj + 2, C->ld, x);
         its = 1;
     } else {
-        its = orc_mg_solve(C->aP, C->aE, C->aW, C->aN, C->aS, C->b, box, 1, C->ni + 2, C->nj + 2, C->ld,
+        its = orc_mg_solve(C->aP, C->aE, C->aW, C->aN, C->aS, C->b, box, nbox, C->ni + 2, C->nj + 2, C->ld,
                            maxITR, tol, x, &rel);
     }
     if (C->rep) {
@@ -213,118 +282,191 @@ static double ion_wall_gpn(const cathode_ndd_params* p, double U, double sign)
 /* ================================================================== */
 /* Poisson, fluidModule.hpp:2080-3075                                  */
 /* ================================================================== */
+/* masks of the transverse-flux interpolation (the reference's bLr, bRr, bLx,
+ * bRx flags): 0 across a wall, and across the axis for r faces */
+static double tmask_r(ctx_t* C, int i, int j) { int c = C->rc[IX(i, j)]; return (double)(c != FC_WALL && c != FC_AXIS); }
+static double tmask_x(ctx_t* C, int i, int j) { return (double)(C->xc[IX(i, j)] != FC_WALL); }
+
+/* electron thermal + emission wall flux through a wall face whose plasma
+ * cell is (ci, cj): E_face the field at the face (the routine's start value
+ * in the Poisson routine), Gi the ion flux at the face, h the spacing,
+ * sgn -1 for a wall on the cell's low side, +1 on its high side
+ * (fluidModule.hpp:3250-3367, 3453-3532) */
+static double elec_wall_flux(ctx_t* C, const cathode_ndd_fields* f, const double* n, int ci, int cj, double E_face,
+                             double Gi, double h, double sgn)
+{
+    const cathode_ndd_params* p = C->p;
+    double Te = f->Te[IX(ci, cj)];
+    /* the cell potential above the wall's: phi_cell - phi_wall = sgn E h / 2 */
+    double dphi = fmax(sgn * E_face * h / 2.0, 0.0);
+    double gpn = wall_gpn(p, Te, dphi, sgn);
+    const double* nold = g_plant == ORC_PLANT_SEE_DENSITY ? n : f->rhoe_old;
+    const double gi = g_plant == ORC_PLANT_SEE_ION_SIGN ? -p->coefIISEE_ts : p->coefIISEE_ts;
+    double see = -gi * Gi - eisee_sum(p, Te) * nold[IX(ci, cj)] * gpn;
+    return n[IX(ci, cj)] * gpn + see;
+}
+
 static void poisson_predict_assemble(ctx_t* C, cathode_ndd_fields* f, const double* nUex_t,
                                      const double* nUer_t)
 {
     const cathode_ndd_params* p = C->p;
-    const int ni = C->ni, nj = C->nj;
+    const cathode_ndd_geometry* G = C->G;
     const double dx = p->dx, dr = p->dr, dt = p->dt;
     const double* r = C->r;
     mob_t m = mobility(p->mu_e);
     double* S = src_or_zero(C, f->rate_ionize);
 
-    /* predicted x-flux, interior faces (:2266-2342) */
-    for (int i = 2; i <= ni; i++)
-        for (int j = 1; j <= nj; j++) {
-            double rL = (r[j] + r[j - 1]) / 2.0, rR = (r[j] + r[j + 1]) / 2.0;
-            double n_f = (f->rhoe[IX(i, j)] + f->rhoe[IX(i - 1, j)]) / 2.0;
-            double dphidx = (f->phi[IX(i, j)] - f->phi[IX(i - 1, j)]) / dx;
-            double dpedx = ORC_BOLTZ * (f->rhoe[IX(i, j)] * f->Te[IX(i, j)] - f->rhoe[IX(i - 1, j)] * f->Te[IX(i - 1, j)]) / dx;
-            double bLr = (double)(j != 1), bLrD = (double)(j != 1), bRr = 1.0, bRrD = 1.0;
-            double tr = (rR * nUer_t[IX(i, j + 1)] * bRr + rL * nUer_t[IX(i, j)] * bLr +
-                         rR * nUer_t[IX(i - 1, j + 1)] * bRrD + rL * nUer_t[IX(i - 1, j)] * bLrD) /
-                        (4.0 * r[j]);
-            f->nUex[IX(i, j)] = nUex_t[IX(i, j)] - (m.inv_par * nUex_t[IX(i, j)] + m.inv_cross * tr) * m.mu_star +
-                                m.mu_star * (n_f * dphidx - dpedx / ORC_E0);
+    /* predicted x-flux on the blocks' inner x faces (:2266-2342) */
+    for (int bl = 0; bl < G->n_bl; bl++) {
+        const int* B = G->xface_box + 4 * bl;
+        for (int i = B[0]; i <= B[1]; i++)
+            for (int j = B[2]; j <= B[3]; j++) {
+                double rL = (r[j] + r[j - 1]) / 2.0, rR = (r[j] + r[j + 1]) / 2.0;
+                double n_f = (f->rhoe[IX(i, j)] + f->rhoe[IX(i - 1, j)]) / 2.0;
+                double dphidx = (f->phi[IX(i, j)] - f->phi[IX(i - 1, j)]) / dx;
+                double dpedx = ORC_BOLTZ * (f->rhoe[IX(i, j)] * f->Te[IX(i, j)] - f->rhoe[IX(i - 1, j)] * f->Te[IX(i - 1, j)]) / dx;
+                double bLr = tmask_r(C, i, j), bLrD = tmask_r(C, i - 1, j);
+                double bRr = tmask_r(C, i, j + 1), bRrD = tmask_r(C, i - 1, j + 1);
+                double tr = (rR * nUer_t[IX(i, j + 1)] * bRr + rL * nUer_t[IX(i, j)] * bLr +
+                             rR * nUer_t[IX(i - 1, j + 1)] * bRrD + rL * nUer_t[IX(i - 1, j)] * bLrD) /
+                            (4.0 * r[j]);
+                f->nUex[IX(i, j)] = nUex_t[IX(i, j)] - (m.inv_par * nUex_t[IX(i, j)] + m.inv_cross * tr) * m.mu_star +
+                                    m.mu_star * (n_f * dphidx - dpedx / ORC_E0);
+            }
+    }
+    /* wall x faces (:2346-2462), segment by segment, with the routine's
+     * starting field */
+    for (int s = 0; s < G->n_seg; s++) {
+        const cathode_ndd_segment* q = &G->seg[s];
+        for (int t = q->lo; t <= q->hi; t++) {
+            if (q->kind == CATHODE_SEG_X_LOW)
+                f->nUex[IX(q->cell, t)] = elec_wall_flux(C, f, f->rhoe, q->cell, t, C->Ex_old[IX(q->cell, t)],
+                                                         f->rhoUix[IX(q->cell, t)], dx, -1.0);
+            else if (q->kind == CATHODE_SEG_X_HIGH)
+                f->nUex[IX(q->cell + 1, t)] = elec_wall_flux(C, f, f->rhoe, q->cell, t, C->Ex_old[IX(q->cell + 1, t)],
+                                                             f->rhoUix[IX(q->cell + 1, t)], dx, 1.0);
         }
-    /* wall x-faces (:2346-2462) */
-    for (int j = 1; j <= nj; j++) {
-        int i = 1;
-        double Te = f->Te[IX(i, j)];
-        double dphi = fmax(-C->Ex_old[IX(i, j)] * dx / 2.0, 0.0);
-        double gpn = wall_gpn(p, Te, dphi, -1.0);
-        double see = -p->coefIISEE_ts * f->rhoUix[IX(i, j)] - eisee_sum(p, Te) * f->rhoe_old[IX(i, j)] * gpn;
-        f->nUex[IX(i, j)] = f->rhoe[IX(i, j)] * gpn + see;
-        i = ni;
-        Te = f->Te[IX(i, j)];
-        dphi = fmax(C->Ex_old[IX(i + 1, j)] * dx / 2.0, 0.0);
-        gpn = wall_gpn(p, Te, dphi, 1.0);
-        see = -p->coefIISEE_ts * f->rhoUix[IX(i + 1, j)] - eisee_sum(p, Te) * f->rhoe_old[IX(i, j)] * gpn;
-        f->nUex[IX(i + 1, j)] = f->rhoe[IX(i, j)] * gpn + see;
     }
-    /* predicted r-flux, interior faces (:2466-2545) */
-    for (int i = 1; i <= ni; i++)
-        for (int j = 2; j <= nj; j++) {
-            double r_t = (r[j] + r[j - 1]) / 2.0;
-            double n_f = (f->rhoe[IX(i, j)] + f->rhoe[IX(i, j - 1)]) / 2.0;
-            double dphidr = (f->phi[IX(i, j)] - f->phi[IX(i, j - 1)]) / dr;
-            double dpedr = ORC_BOLTZ * (f->rhoe[IX(i, j)] * f->Te[IX(i, j)] - f->rhoe[IX(i, j - 1)] * f->Te[IX(i, j - 1)]) / dr;
-            double bLx = (double)(i != 1), bLxD = (double)(i != 1), bRx = (double)(i != ni), bRxD = (double)(i != ni);
-            double tx = (r[j] * nUex_t[IX(i + 1, j)] * bRx + r[j] * nUex_t[IX(i, j)] * bLx +
-                         r[j - 1] * nUex_t[IX(i + 1, j - 1)] * bRxD + r[j - 1] * nUex_t[IX(i, j - 1)] * bLxD) /
-                        (4.0 * r_t);
-            f->nUer[IX(i, j)] = nUer_t[IX(i, j)] - (m.inv_cross * tx + m.inv_par * nUer_t[IX(i, j)]) * m.mu_star +
-                                m.mu_star * (n_f * dphidr - dpedr / ORC_E0);
+    /* predicted r-flux on the blocks' inner r faces (:2466-2545) */
+    for (int bl = 0; bl < G->n_bl; bl++) {
+        const int* B = G->rface_box + 4 * bl;
+        for (int i = B[0]; i <= B[1]; i++)
+            for (int j = B[2]; j <= B[3]; j++) {
+                double r_t = (r[j] + r[j - 1]) / 2.0;
+                double n_f = (f->rhoe[IX(i, j)] + f->rhoe[IX(i, j - 1)]) / 2.0;
+                double dphidr = (f->phi[IX(i, j)] - f->phi[IX(i, j - 1)]) / dr;
+                double dpedr = ORC_BOLTZ * (f->rhoe[IX(i, j)] * f->Te[IX(i, j)] - f->rhoe[IX(i, j - 1)] * f->Te[IX(i, j - 1)]) / dr;
+                double bLx = tmask_x(C, i, j), bLxD = tmask_x(C, i, j - 1);
+                double bRx = tmask_x(C, i + 1, j), bRxD = tmask_x(C, i + 1, j - 1);
+                double tx = (r[j] * nUex_t[IX(i + 1, j)] * bRx + r[j] * nUex_t[IX(i, j)] * bLx +
+                             r[j - 1] * nUex_t[IX(i + 1, j - 1)] * bRxD + r[j - 1] * nUex_t[IX(i, j - 1)] * bLxD) /
+                            (4.0 * r_t);
+                f->nUer[IX(i, j)] = nUer_t[IX(i, j)] - (m.inv_cross * tx + m.inv_par * nUer_t[IX(i, j)]) * m.mu_star +
+                                    m.mu_star * (n_f * dphidr - dpedr / ORC_E0);
+            }
+    }
+    /* wall r faces (:2549-2613) and the open face (:2614-2621): electron
+     * flux = ion flux there */
+    for (int s = 0; s < G->n_seg; s++) {
+        const cathode_ndd_segment* q = &G->seg[s];
+        for (int t = q->lo; t <= q->hi; t++) {
+            if (q->kind == CATHODE_SEG_R_LOW) {
+                f->nUer[IX(t, q->cell)] = elec_wall_flux(C, f, f->rhoe, t, q->cell, C->Er_old[IX(t, q->cell)],
+                                                         f->rhoUir[IX(t, q->cell)], dr, -1.0);
+            } else if (q->kind == CATHODE_SEG_R_HIGH) {
+                f->nUer[IX(t, q->cell + 1)] = elec_wall_flux(C, f, f->rhoe, t, q->cell, C->Er_old[IX(t, q->cell + 1)],
+                                                             f->rhoUir[IX(t, q->cell + 1)], dr, 1.0);
+            } else if (q->kind == CATHODE_SEG_R_OPEN) {
+                double uir = p->mu_i * f->Er[IX(t, q->cell + 1)] + ORC_EPS_VD;
+                f->nUer[IX(t, q->cell + 1)] = f->rhoi[IX(t, q->cell)] * uir;
+            }
         }
-    /* open upper face (:2614-2621): electron flux = ion flux */
-    for (int i = 1; i <= ni; i++) {
-        double uir = p->mu_i * f->Er[IX(i, nj + 1)] + ORC_EPS_VD;
-        f->nUer[IX(i, nj + 1)] = f->rhoi[IX(i, nj)] * uir;
     }
-    /* coefficients (:2627-2719) */
-    for (int i = 1; i <= ni; i++)
-        for (int j = 1; j <= nj; j++) {
-            double qR = (r[j] + r[j + 1]) / 2.0 / r[j];
-            double qL = (r[j] + r[j - 1]) / 2.0 / r[j];
-            double mu = m.mu_star; /* (mu_perp[i][j] + mu_perp[nb]) / 2 * c_mu */
-            const double* ne = f->rhoe;
-            double dte = dt;
-            if (g_plant == ORC_PLANT_POISSON_DT) dte = 2.0 * dt;
-            if (g_plant == ORC_PLANT_POISSON_MU) mu = mobility(p->mu_i).mu_star;
-            if (g_plant == ORC_PLANT_POISSON_DENSITY) ne = f->rhoi;
-            double nLx = (ne[IX(i, j)] + ne[IX(i - 1, j)]) / 2.0;
-            double nRx = (ne[IX(i, j)] + ne[IX(i + 1, j)]) / 2.0;
-            double nLr = (ne[IX(i, j)] + ne[IX(i, j - 1)]) / 2.0;
-            double nRr = (ne[IX(i, j)] + ne[IX(i, j + 1)]) / 2.0;
-            double bLx = (double)(i != 1), bRx = (double)(i != ni);
-            double bLr = (double)(j != 1), bRr = (double)(j != nj);
-            double eRx = ORC_EPS0 + ORC_E0 * dte * nRx * mu * bRx;
-            double eLx = ORC_EPS0 + ORC_E0 * dte * nLx * mu * bLx;
-            double eRr = ORC_EPS0 + ORC_E0 * dte * nRr * mu * bRr;
-            double eLr = ORC_EPS0 + ORC_E0 * dte * nLr * mu * bLr;
-            C->aP[IX(i, j)] = (eRx + eLx) * dr * dr + (qR * eRr + qL * eLr) * dx * dx;
-            C->aE[IX(i, j)] = eRx * dr * dr;
-            C->aW[IX(i, j)] = eLx * dr * dr;
-            C->aN[IX(i, j)] = qR * eRr * dx * dx;
-            C->aS[IX(i, j)] = qL * eLr * dx * dx;
-            double div = (f->nUex[IX(i + 1, j)] - f->nUex[IX(i, j)]) / dx +
-                         (qR * f->nUer[IX(i, j + 1)] - qL * f->nUer[IX(i, j)]) / dr;
-            double lap = -(f->Ex[IX(i + 1, j)] - f->Ex[IX(i, j)]) / dx - (qR * f->Er[IX(i, j + 1)] - qL * f->Er[IX(i, j)]) / dr;
-            const double s_on = g_plant == ORC_PLANT_POISSON_SOURCE ? 0.0 : 1.0;
-            C->b[IX(i, j)] = (ORC_E0 * f->rhoi[IX(i, j)] - ORC_E0 * f->rhoe[IX(i, j)] - s_on * ORC_E0 * dt * S[IX(i, j)] +
-                              ORC_EPS0 * lap + ORC_E0 * dt * div) *
-                             dx * dx * dr * dr;
+    /* coefficients on the blocks' cells (:2627-2719); the plasma term of a
+     * face counts between two plasma cells only */
+    for (int bl = 0; bl < G->n_bl; bl++) {
+        const int* B = G->cell_box + 4 * bl;
+        for (int i = B[0]; i <= B[1]; i++)
+            for (int j = B[2]; j <= B[3]; j++) {
+                double qR = (r[j] + r[j + 1]) / 2.0 / r[j];
+                double qL = (r[j] + r[j - 1]) / 2.0 / r[j];
+                double mu = m.mu_star; /* (mu_perp[i][j] + mu_perp[nb]) / 2 * c_mu */
+                const double* ne = f->rhoe;
+                double dte = dt;
+                if (g_plant == ORC_PLANT_POISSON_DT) dte = 2.0 * dt;
+                if (g_plant == ORC_PLANT_POISSON_MU) mu = mobility(p->mu_i).mu_star;
+                if (g_plant == ORC_PLANT_POISSON_DENSITY) ne = f->rhoi;
+                double nLx = (ne[IX(i, j)] + ne[IX(i - 1, j)]) / 2.0;
+                double nRx = (ne[IX(i, j)] + ne[IX(i + 1, j)]) / 2.0;
+                double nLr = (ne[IX(i, j)] + ne[IX(i, j - 1)]) / 2.0;
+                double nRr = (ne[IX(i, j)] + ne[IX(i, j + 1)]) / 2.0;
+                double bLx = (double)(C->xc[IX(i, j)] == FC_INNER), bRx = (double)(C->xc[IX(i + 1, j)] == FC_INNER);
+                double bLr = (double)(C->rc[IX(i, j)] == FC_INNER), bRr = (double)(C->rc[IX(i, j + 1)] == FC_INNER);
+                double eRx = ORC_EPS0 + ORC_E0 * dte * nRx * mu * bRx;
+                double eLx = ORC_EPS0 + ORC_E0 * dte * nLx * mu * bLx;
+                double eRr = ORC_EPS0 + ORC_E0 * dte * nRr * mu * bRr;
+                double eLr = ORC_EPS0 + ORC_E0 * dte * nLr * mu * bLr;
+                C->aP[IX(i, j)] = (eRx + eLx) * dr * dr + (qR * eRr + qL * eLr) * dx * dx;
+                C->aE[IX(i, j)] = eRx * dr * dr;
+                C->aW[IX(i, j)] = eLx * dr * dr;
+                C->aN[IX(i, j)] = qR * eRr * dx * dx;
+                C->aS[IX(i, j)] = qL * eLr * dx * dx;
+                double div = (f->nUex[IX(i + 1, j)] - f->nUex[IX(i, j)]) / dx +
+                             (qR * f->nUer[IX(i, j + 1)] - qL * f->nUer[IX(i, j)]) / dr;
+                double lap = -(f->Ex[IX(i + 1, j)] - f->Ex[IX(i, j)]) / dx - (qR * f->Er[IX(i, j + 1)] - qL * f->Er[IX(i, j)]) / dr;
+                const double s_on = g_plant == ORC_PLANT_POISSON_SOURCE ? 0.0 : 1.0;
+                C->b[IX(i, j)] = (ORC_E0 * f->rhoi[IX(i, j)] - ORC_E0 * f->rhoe[IX(i, j)] - s_on * ORC_E0 * dt * S[IX(i, j)] +
+                                  ORC_EPS0 * lap + ORC_E0 * dt * div) *
+                                 dx * dx * dr * dr;
+            }
+    }
+    /* BC folds (:2721-2824), segment by segment: Dirichlet walls by ghost
+     * mirror aP += a_nb, the open face grad phi = 0: aP -= a_nb */
+    for (int s = 0; s < G->n_seg; s++) {
+        const cathode_ndd_segment* q = &G->seg[s];
+        for (int t = q->lo; t <= q->hi; t++) {
+            if (q->kind == CATHODE_SEG_X_LOW) {
+                size_t k = IX(q->cell, t);
+                C->aP[k] = C->aP[k] + C->aW[k];
+                C->aW[k] = 0.0;
+            } else if (q->kind == CATHODE_SEG_X_HIGH) {
+                size_t k = IX(q->cell, t);
+                C->aP[k] = C->aP[k] + C->aE[k];
+                C->aE[k] = 0.0;
+            } else if (q->kind == CATHODE_SEG_R_LOW) {
+                size_t k = IX(t, q->cell);
+                C->aP[k] = C->aP[k] + C->aS[k];
+                C->aS[k] = 0.0;
+            } else if (q->kind == CATHODE_SEG_R_HIGH) {
+                if (g_plant == ORC_PLANT_GEO_FOLD) continue;
+                size_t k = IX(t, q->cell);
+                C->aP[k] = C->aP[k] + C->aN[k];
+                C->aN[k] = 0.0;
+            } else if (q->kind == CATHODE_SEG_R_OPEN) {
+                size_t k = IX(t, q->cell);
+                C->aP[k] = C->aP[k] - C->aN[k];
+                C->aN[k] = 0.0;
+            }
         }
-    /* BC folds (:2721-2824): Dirichlet walls by ghost mirror, open top Neumann */
-    for (int j = 1; j <= nj; j++) {
-        C->aP[IX(1, j)] = C->aP[IX(1, j)] + C->aW[IX(1, j)];
-        C->aW[IX(1, j)] = 0.0;
     }
-    for (int j = 1; j <= nj; j++) {
-        C->aP[IX(ni, j)] = C->aP[IX(ni, j)] + C->aE[IX(ni, j)];
-        C->aE[IX(ni, j)] = 0.0;
-    }
-    for (int i = 1; i <= ni; i++) {
-        C->aP[IX(i, nj)] = C->aP[IX(i, nj)] - C->aN[IX(i, nj)];
-        C->aN[IX(i, nj)] = 0.0;
-    }
+}
+
+/* the calcRes of the three planes of a routine over the blocks' boxes */
+static void calc_res3(ctx_t* C, const double* n, const double* n0, const double* fx, const double* fx0,
+                      const double* fr, const double* fr0, double* e)
+{
+    const cathode_ndd_geometry* G = C->G;
+    int mode = C->p->flag_error;
+    e[0] = orc_calc_res_boxes(n, n0, C->ld, G->cell_box, G->n_bl, 1e-100, mode);
+    e[1] = orc_calc_res_boxes(fx, fx0, C->ld, G->xface_box, G->n_bl, 1e-100, mode);
+    e[2] = orc_calc_res_boxes(fr, fr0, C->ld, G->rface_box, G->n_bl, 1e-100, mode);
 }
 
 static void poisson_routine(ctx_t* C, cathode_ndd_fields* f, double* phi_old, double* nUex_old,
                             double* nUer_old)
 {
     const cathode_ndd_params* p = C->p;
-    const int ni = C->ni, nj = C->nj;
+    const cathode_ndd_geometry* G = C->G;
     const double dx = p->dx, dr = p->dr;
     mob_t m = mobility(p->mu_e);
     double* dphi = C->x;
@@ -334,10 +476,23 @@ static void poisson_routine(ctx_t* C, cathode_ndd_fields* f, double* phi_old, do
     memcpy(nUex_old, f->nUex, C->n * sizeof(double));
     memcpy(nUer_old, f->nUer, C->n * sizeof(double));
     for (size_t k = 0; k < C->n; k++) dphi[k] = f->phi[k] - phi_old[k];
-    /* ghost potential (:2160-2253) */
-    for (int j = 1; j <= nj; j++) f->phi[IX(0, j)] = -f->phi[IX(1, j)];
-    for (int j = 1; j <= nj; j++) f->phi[IX(ni + 1, j)] = 2.0 * p->V_bias - f->phi[IX(ni, j)];
-    for (int i = 1; i <= ni; i++) f->phi[IX(i, nj + 1)] = f->phi[IX(i, nj)];
+    /* ghost potential (:2152-2249), segment by segment: phi = 0 or V_bias
+     * walls, grad phi = 0 on the open face */
+    for (int s = 0; s < G->n_seg; s++) {
+        const cathode_ndd_segment* q = &G->seg[s];
+        for (int t = q->lo; t <= q->hi; t++) {
+            if (q->kind == CATHODE_SEG_X_LOW)
+                f->phi[IX(q->cell - 1, t)] = q->flag ? 2.0 * p->V_bias - f->phi[IX(q->cell, t)] : -f->phi[IX(q->cell, t)];
+            else if (q->kind == CATHODE_SEG_X_HIGH)
+                f->phi[IX(q->cell + 1, t)] = q->flag ? 2.0 * p->V_bias - f->phi[IX(q->cell, t)] : -f->phi[IX(q->cell, t)];
+            else if (q->kind == CATHODE_SEG_R_LOW)
+                f->phi[IX(t, q->cell - 1)] = -f->phi[IX(t, q->cell)];
+            else if (q->kind == CATHODE_SEG_R_HIGH)
+                f->phi[IX(t, q->cell + 1)] = -f->phi[IX(t, q->cell)];
+            else if (q->kind == CATHODE_SEG_R_OPEN)
+                f->phi[IX(t, q->cell + 1)] = f->phi[IX(t, q->cell)];
+        }
+    }
 
     double* phi_t = C->tmp_c;
     double* nUex_t = C->tmp_x;
@@ -350,46 +505,73 @@ static void poisson_routine(ctx_t* C, cathode_ndd_fields* f, double* phi_old, do
         memcpy(nUer_t, f->nUer, C->n * sizeof(double));
         poisson_predict_assemble(C, f, nUex_t, nUer_t);
         solve5(C, dphi, p->maxITR_SOR_phi, p->error_cnv_SOR_phi);
-        /* ghost correction and wall-face field (:2850-2943) */
-        for (int j = 1; j <= nj; j++) {
-            dphi[IX(0, j)] = -dphi[IX(1, j)];
-            f->Ex[IX(1, j)] = f->Ex[IX(1, j)] - (dphi[IX(1, j)] - dphi[IX(0, j)]) / dx;
-        }
-        for (int j = 1; j <= nj; j++) {
-            dphi[IX(ni + 1, j)] = -dphi[IX(ni, j)];
-            f->Ex[IX(ni + 1, j)] = f->Ex[IX(ni + 1, j)] - (dphi[IX(ni + 1, j)] - dphi[IX(ni, j)]) / dx;
-        }
-        for (int i = 1; i <= ni; i++) {
-            dphi[IX(i, nj + 1)] = dphi[IX(i, nj)];
-            f->Er[IX(i, nj + 1)] = 0.0;
+        /* ghost correction and wall-face field (:2840-2943), segment by segment */
+        for (int s = 0; s < G->n_seg; s++) {
+            const cathode_ndd_segment* q = &G->seg[s];
+            for (int t = q->lo; t <= q->hi; t++) {
+                if (q->kind == CATHODE_SEG_X_LOW) {
+                    int i = q->cell, j = t;
+                    dphi[IX(i - 1, j)] = -dphi[IX(i, j)];
+                    f->Ex[IX(i, j)] = f->Ex[IX(i, j)] - (dphi[IX(i, j)] - dphi[IX(i - 1, j)]) / dx;
+                } else if (q->kind == CATHODE_SEG_X_HIGH) {
+                    int i = q->cell, j = t;
+                    dphi[IX(i + 1, j)] = -dphi[IX(i, j)];
+                    f->Ex[IX(i + 1, j)] = f->Ex[IX(i + 1, j)] - (dphi[IX(i + 1, j)] - dphi[IX(i, j)]) / dx;
+                } else if (q->kind == CATHODE_SEG_R_LOW) {
+                    int i = t, j = q->cell;
+                    dphi[IX(i, j - 1)] = -dphi[IX(i, j)];
+                    f->Er[IX(i, j)] = f->Er[IX(i, j)] - (dphi[IX(i, j)] - dphi[IX(i, j - 1)]) / dr;
+                } else if (q->kind == CATHODE_SEG_R_HIGH) {
+                    int i = t, j = q->cell;
+                    dphi[IX(i, j + 1)] = -dphi[IX(i, j)];
+                    f->Er[IX(i, j + 1)] = f->Er[IX(i, j + 1)] - (dphi[IX(i, j + 1)] - dphi[IX(i, j)]) / dr;
+                } else if (q->kind == CATHODE_SEG_R_OPEN) {
+                    int i = t, j = q->cell;
+                    dphi[IX(i, j + 1)] = dphi[IX(i, j)];
+                    f->Er[IX(i, j + 1)] = 0.0;
+                }
+            }
         }
         for (size_t k = 0; k < C->n; k++) f->phi[k] = f->phi[k] + dphi[k];
-        for (int i = 2; i <= ni; i++)
-            for (int j = 1; j <= nj; j++) f->Ex[IX(i, j)] = -(f->phi[IX(i, j)] - f->phi[IX(i - 1, j)]) / dx;
-        for (int i = 1; i <= ni; i++)
-            for (int j = 2; j <= nj; j++) f->Er[IX(i, j)] = -(f->phi[IX(i, j)] - f->phi[IX(i, j - 1)]) / dr;
-        /* flux correction (:2980-3033) */
+        for (int bl = 0; bl < G->n_bl; bl++) {
+            const int* B = G->xface_box + 4 * bl;
+            for (int i = B[0]; i <= B[1]; i++)
+                for (int j = B[2]; j <= B[3]; j++) f->Ex[IX(i, j)] = -(f->phi[IX(i, j)] - f->phi[IX(i - 1, j)]) / dx;
+        }
+        for (int bl = 0; bl < G->n_bl; bl++) {
+            const int* B = G->rface_box + 4 * bl;
+            for (int i = B[0]; i <= B[1]; i++)
+                for (int j = B[2]; j <= B[3]; j++) f->Er[IX(i, j)] = -(f->phi[IX(i, j)] - f->phi[IX(i, j - 1)]) / dr;
+        }
+        /* flux correction on the inner faces (:2980-3033) */
         if (g_plant == ORC_PLANT_POISSON_FLUX_MU) m = mobility(p->mu_i);
-        for (int i = 2; i <= ni; i++)
-            for (int j = 1; j <= nj; j++) {
-                double n_f = (f->rhoe[IX(i, j)] + f->rhoe[IX(i - 1, j)]) / 2.0;
-                f->nUex[IX(i, j)] = f->nUex[IX(i, j)] + m.mu_star * n_f * ((dphi[IX(i, j)] - dphi[IX(i - 1, j)]) / dx);
-            }
-        for (int i = 1; i <= ni; i++)
-            for (int j = 2; j <= nj; j++) {
-                double n_f = (f->rhoe[IX(i, j)] + f->rhoe[IX(i, j - 1)]) / 2.0;
-                f->nUer[IX(i, j)] = f->nUer[IX(i, j)] + m.mu_star * n_f * ((dphi[IX(i, j)] - dphi[IX(i, j - 1)]) / dr);
-            }
-        double e1 = orc_calc_res(f->phi, phi_t, C->ld, 1, ni, 1, nj, 1e-100, p->flag_error);
-        double e2 = orc_calc_res(f->nUex, nUex_t, C->ld, 2, ni, 1, nj, 1e-100, p->flag_error);
-        double e3 = orc_calc_res(f->nUer, nUer_t, C->ld, 1, ni, 2, nj, 1e-100, p->flag_error);
-        double eg = fmax(fmax(e1, e2), e3);
+        for (int bl = 0; bl < G->n_bl; bl++) {
+            const int* B = G->xface_box + 4 * bl;
+            for (int i = B[0]; i <= B[1]; i++)
+                for (int j = B[2]; j <= B[3]; j++) {
+                    double n_f = (f->rhoe[IX(i, j)] + f->rhoe[IX(i - 1, j)]) / 2.0;
+                    f->nUex[IX(i, j)] = f->nUex[IX(i, j)] + m.mu_star * n_f * ((dphi[IX(i, j)] - dphi[IX(i - 1, j)]) / dx);
+                }
+        }
+        for (int bl = 0; bl < G->n_bl; bl++) {
+            const int* B = G->rface_box + 4 * bl;
+            for (int i = B[0]; i <= B[1]; i++)
+                for (int j = B[2]; j <= B[3]; j++) {
+                    double n_f = (f->rhoe[IX(i, j)] + f->rhoe[IX(i, j - 1)]) / 2.0;
+                    f->nUer[IX(i, j)] = f->nUer[IX(i, j)] + m.mu_star * n_f * ((dphi[IX(i, j)] - dphi[IX(i, j - 1)]) / dr);
+                }
+        }
+        double e[3];
+        calc_res3(C, f->phi, phi_t, f->nUex, nUex_t, f->nUer, nUer_t, e);
+        double eg = fmax(fmax(e[0], e[1]), e[2]);
         if (eg < p->error_cnv_HES_phi || ncount >= p->maxITR_HES_phi) break;
     }
     if (C->rep) {
-        C->rep->error_phi = orc_calc_res(f->phi, phi_old, C->ld, 1, ni, 1, nj, 1e-100, p->flag_error);
-        C->rep->error_nUex = orc_calc_res(f->nUex, nUex_old, C->ld, 2, ni, 1, nj, 1e-100, p->flag_error);
-        C->rep->error_nUer = orc_calc_res(f->nUer, nUer_old, C->ld, 1, ni, 2, nj, 1e-100, p->flag_error);
+        double e[3];
+        calc_res3(C, f->phi, phi_old, f->nUex, nUex_old, f->nUer, nUer_old, e);
+        C->rep->error_phi = e[0];
+        C->rep->error_nUex = e[1];
+        C->rep->error_nUer = e[2];
     }
 }
 
@@ -434,11 +616,48 @@ typedef struct {
     double cnv, lin_tol;    /* PC threshold; linear-solve tolerance (error_cnv_SOR_*) */
 } sg_eq_t;
 
+/* wall flux of species sp through a wall face whose plasma cell is (ci, cj):
+ * E the face field, Gi the ion flux at the face, h the spacing, sgn -1 for a
+ * wall on the cell's low side, +1 on its high side (electrons
+ * fluidModule.hpp:3250-3367, 3453-3532; energy :4161-4316, :4403-4506; ions
+ * the kinetic flux of :1502-1526 at the drift mu_i E) */
+static double sg_wall_flux(ctx_t* C, const cathode_ndd_fields* f, int sp, const sg_eq_t* q, const double* n_old,
+                           int ci, int cj, double E, double Gi, double h, double sgn)
+{
+    const cathode_ndd_params* p = C->p;
+    const double* n = q->n;
+    if (sp == SP_ION) {
+        double U = p->mu_i * E;
+        return n[IX(ci, cj)] * ion_wall_gpn(p, U, sgn);
+    } else if (sp == SP_ELEC) {
+        return elec_wall_flux(C, f, n, ci, cj, E, Gi, h, sgn);
+    }
+    const double* T = q->T;
+    double Teo = T[IX(ci, cj)];
+    double dphi = fmax(sgn * E * h / 2.0, 0.0);
+    double vth = sqrt(8.0 * ORC_BOLTZ * Teo / (M_PI * p->masse));
+    double ex = exp(fmin(-ORC_E0 * dphi / (ORC_BOLTZ * Teo + 1e-100), ORC_LIM_EXP));
+    double gpne = sgn * vth * ex / 3.0 * 1.0;
+    double eps_emit = (g_plant == ORC_PLANT_SEE_EMIT_ENERGY ? 1.0 : 2.0) * ORC_E0 * p->Te_emitSEE_eV;
+    double gpn = sgn * 0.25 * 1.0 * vth * ex;
+    double Te = f->Te[IX(ci, cj)];
+    double eb = 0.0, rd = 0.0, ts = 0.0;
+    if (p->n_see > 1) {
+        eb = orc_func_Te(p->see_eb, p->n_see, Te, p->dTe_SEE);
+        rd = orc_func_Te(p->see_rd, p->n_see, Te, p->dTe_SEE);
+        ts = orc_func_Te(p->see_ts, p->n_see, Te, p->dTe_SEE);
+    }
+    double see = -eps_emit * p->coefIISEE_ts * Gi -
+                 (eb + (g_plant == ORC_PLANT_SEE_RD_RATIO ? 1.0 : p->ratioEngy_EISEE_rd) * rd) * n_old[IX(ci, cj)] * gpne -
+                 eps_emit * ts * f->rhoe_old[IX(ci, cj)] * gpn;
+    return n[IX(ci, cj)] * gpne + see;
+}
+
 static void sg_predict_assemble(ctx_t* C, cathode_ndd_fields* f, int sp, const sg_eq_t* q,
                                 const double* Fx_t, const double* Fr_t, const double* n_old)
 {
     const cathode_ndd_params* p = C->p;
-    const int ni = C->ni, nj = C->nj;
+    const cathode_ndd_geometry* G = C->G;
     const double dx = p->dx, dr = p->dr, dt = p->dt;
     const double* r = C->r;
     const int lit = g_literal;
@@ -446,125 +665,130 @@ static void sg_predict_assemble(ctx_t* C, cathode_ndd_fields* f, int sp, const s
     const double* T = q->T;
     double* n = q->n;
 
-    /* interior x faces */
-    for (int i = 2; i <= ni; i++)
-        for (int j = 1; j <= nj; j++) {
-            double rL = (r[j] + r[j - 1]) / 2.0, rR = (r[j] + r[j + 1]) / 2.0;
-            face_t F = sg_face(p, sp, m.mu_star, f->Ex[IX(i, j)], T[IX(i - 1, j)], T[IX(i, j)], dx);
-            double bLr = (double)(j != 1), bLrD = (double)(j != 1), bRr = 1.0, bRrD = 1.0;
-            double tr = (rR * Fr_t[IX(i, j + 1)] * bRr + rL * Fr_t[IX(i, j)] * bLr +
-                         rR * Fr_t[IX(i - 1, j + 1)] * bRrD + rL * Fr_t[IX(i - 1, j)] * bLrD) /
-                        (4.0 * r[j]);
-            double sg = orc_bern_bp(F.vd, F.z, F.D / dx, lit) * n[IX(i - 1, j)] - orc_bern_bm(F.vd, F.z, F.D / dx, lit) * n[IX(i, j)];
-            if (sp == SP_ENERGY) sg = sg * 5.0 / 3.0;
-            q->Fx[IX(i, j)] = sg + Fx_t[IX(i, j)] - m.mu_star * (m.inv_par * Fx_t[IX(i, j)] + m.inv_cross * tr);
+    /* the blocks' inner x faces (:3160-3244) */
+    for (int bl = 0; bl < G->n_bl; bl++) {
+        const int* B = G->xface_box + 4 * bl;
+        for (int i = B[0]; i <= B[1]; i++)
+            for (int j = B[2]; j <= B[3]; j++) {
+                double rL = (r[j] + r[j - 1]) / 2.0, rR = (r[j] + r[j + 1]) / 2.0;
+                face_t F = sg_face(p, sp, m.mu_star, f->Ex[IX(i, j)], T[IX(i - 1, j)], T[IX(i, j)], dx);
+                double bLr = tmask_r(C, i, j), bLrD = tmask_r(C, i - 1, j);
+                double bRr = tmask_r(C, i, j + 1), bRrD = tmask_r(C, i - 1, j + 1);
+                double tr = (rR * Fr_t[IX(i, j + 1)] * bRr + rL * Fr_t[IX(i, j)] * bLr +
+                             rR * Fr_t[IX(i - 1, j + 1)] * bRrD + rL * Fr_t[IX(i - 1, j)] * bLrD) /
+                            (4.0 * r[j]);
+                double sg = orc_bern_bp(F.vd, F.z, F.D / dx, lit) * n[IX(i - 1, j)] - orc_bern_bm(F.vd, F.z, F.D / dx, lit) * n[IX(i, j)];
+                if (sp == SP_ENERGY) sg = sg * 5.0 / 3.0;
+                q->Fx[IX(i, j)] = sg + Fx_t[IX(i, j)] - m.mu_star * (m.inv_par * Fx_t[IX(i, j)] + m.inv_cross * tr);
+            }
+    }
+    /* wall x faces (:3250-3367), segment by segment */
+    for (int s = 0; s < G->n_seg; s++) {
+        const cathode_ndd_segment* g = &G->seg[s];
+        for (int t = g->lo; t <= g->hi; t++) {
+            if (g->kind == CATHODE_SEG_X_LOW)
+                q->Fx[IX(g->cell, t)] = sg_wall_flux(C, f, sp, q, n_old, g->cell, t, f->Ex[IX(g->cell, t)],
+                                                     f->rhoUix[IX(g->cell, t)], dx, -1.0);
+            else if (g->kind == CATHODE_SEG_X_HIGH)
+                q->Fx[IX(g->cell + 1, t)] = sg_wall_flux(C, f, sp, q, n_old, g->cell, t, f->Ex[IX(g->cell + 1, t)],
+                                                         f->rhoUix[IX(g->cell + 1, t)], dx, 1.0);
         }
-    /* interior r faces */
-    for (int i = 1; i <= ni; i++)
-        for (int j = 2; j <= nj; j++) {
-            double r_t = (r[j] + r[j - 1]) / 2.0;
-            face_t F = sg_face(p, sp, m.mu_star, f->Er[IX(i, j)], T[IX(i, j - 1)], T[IX(i, j)], dr);
-            double bLx = (double)(i != 1), bLxD = (double)(i != 1), bRx = (double)(i != ni), bRxD = (double)(i != ni);
-            double tx = (r[j] * Fx_t[IX(i + 1, j)] * bRx + r[j] * Fx_t[IX(i, j)] * bLx +
-                         r[j - 1] * Fx_t[IX(i + 1, j - 1)] * bRxD + r[j - 1] * Fx_t[IX(i, j - 1)] * bLxD) /
-                        (4.0 * r_t);
-            double sg = orc_bern_bp(F.vd, F.z, F.D / dr, lit) * n[IX(i, j - 1)] - orc_bern_bm(F.vd, F.z, F.D / dr, lit) * n[IX(i, j)];
-            if (sp == SP_ENERGY) sg = sg * 5.0 / 3.0;
-            q->Fr[IX(i, j)] = sg + Fr_t[IX(i, j)] - m.mu_star * (m.inv_cross * tx + m.inv_par * Fr_t[IX(i, j)]);
-        }
-    /* wall x faces and open top */
-    for (int j = 1; j <= nj; j++) {
-        for (int side = 0; side < 2; side++) {
-            int i = side ? ni : 1;     /* wall cell */
-            int fi = side ? ni + 1 : 1; /* wall face */
-            double sgn = side ? 1.0 : -1.0;
-            if (sp == SP_ION) {
-                double U = p->mu_i * f->Ex[IX(fi, j)];
-                q->Fx[IX(fi, j)] = n[IX(i, j)] * ion_wall_gpn(p, U, sgn);
-            } else if (sp == SP_ELEC) {
-                double Te = f->Te[IX(i, j)];
-                double dphi = fmax(-sgn * f->Ex[IX(fi, j)] * dx / 2.0, 0.0);
-                double gpn = wall_gpn(p, Te, dphi, sgn);
-                const double* nold = g_plant == ORC_PLANT_SEE_DENSITY ? n : f->rhoe_old;
-                const double gi = g_plant == ORC_PLANT_SEE_ION_SIGN ? -p->coefIISEE_ts : p->coefIISEE_ts;
-                double see = -gi * f->rhoUix[IX(fi, j)] - eisee_sum(p, Te) * nold[IX(i, j)] * gpn;
-                q->Fx[IX(fi, j)] = n[IX(i, j)] * gpn + see;
-            } else {
-                double Teo = T[IX(i, j)];
-                double dphi = fmax(-sgn * f->Ex[IX(fi, j)] * dx / 2.0, 0.0);
-                double vth = sqrt(8.0 * ORC_BOLTZ * Teo / (M_PI * p->masse));
-                double ex = exp(fmin(-ORC_E0 * dphi / (ORC_BOLTZ * Teo + 1e-100), ORC_LIM_EXP));
-                double gpne = sgn * vth * ex / 3.0 * 1.0;
-                double eps_emit = (g_plant == ORC_PLANT_SEE_EMIT_ENERGY ? 1.0 : 2.0) * ORC_E0 * p->Te_emitSEE_eV;
-                double gpn = sgn * 0.25 * 1.0 * vth * ex;
-                double Te = f->Te[IX(i, j)];
-                double eb = 0.0, rd = 0.0, ts = 0.0;
-                if (p->n_see > 1) {
-                    eb = orc_func_Te(p->see_eb, p->n_see, Te, p->dTe_SEE);
-                    rd = orc_func_Te(p->see_rd, p->n_see, Te, p->dTe_SEE);
-                    ts = orc_func_Te(p->see_ts, p->n_see, Te, p->dTe_SEE);
+    }
+    /* the blocks' inner r faces (:3370-3450) */
+    for (int bl = 0; bl < G->n_bl; bl++) {
+        const int* B = G->rface_box + 4 * bl;
+        for (int i = B[0]; i <= B[1]; i++)
+            for (int j = B[2]; j <= B[3]; j++) {
+                double r_t = (r[j] + r[j - 1]) / 2.0;
+                face_t F = sg_face(p, sp, m.mu_star, f->Er[IX(i, j)], T[IX(i, j - 1)], T[IX(i, j)], dr);
+                double bLx = tmask_x(C, i, j), bLxD = tmask_x(C, i, j - 1);
+                double bRx = tmask_x(C, i + 1, j), bRxD = tmask_x(C, i + 1, j - 1);
+                double tx = (r[j] * Fx_t[IX(i + 1, j)] * bRx + r[j] * Fx_t[IX(i, j)] * bLx +
+                             r[j - 1] * Fx_t[IX(i + 1, j - 1)] * bRxD + r[j - 1] * Fx_t[IX(i, j - 1)] * bLxD) /
+                            (4.0 * r_t);
+                double sg = orc_bern_bp(F.vd, F.z, F.D / dr, lit) * n[IX(i, j - 1)] - orc_bern_bm(F.vd, F.z, F.D / dr, lit) * n[IX(i, j)];
+                if (sp == SP_ENERGY) sg = sg * 5.0 / 3.0;
+                q->Fr[IX(i, j)] = sg + Fr_t[IX(i, j)] - m.mu_star * (m.inv_cross * tx + m.inv_par * Fr_t[IX(i, j)]);
+            }
+    }
+    /* wall r faces (:3453-3532) and the open face (:3533-3542): the ion
+     * flux, or for the energy 5/2 k T max(Gamma_e, 0) (:4492-4499) */
+    for (int s = 0; s < G->n_seg; s++) {
+        const cathode_ndd_segment* g = &G->seg[s];
+        for (int t = g->lo; t <= g->hi; t++) {
+            if (g->kind == CATHODE_SEG_R_LOW) {
+                q->Fr[IX(t, g->cell)] = sg_wall_flux(C, f, sp, q, n_old, t, g->cell, f->Er[IX(t, g->cell)],
+                                                     f->rhoUir[IX(t, g->cell)], dr, -1.0);
+            } else if (g->kind == CATHODE_SEG_R_HIGH) {
+                q->Fr[IX(t, g->cell + 1)] = sg_wall_flux(C, f, sp, q, n_old, t, g->cell, f->Er[IX(t, g->cell + 1)],
+                                                         f->rhoUir[IX(t, g->cell + 1)], dr, 1.0);
+            } else if (g->kind == CATHODE_SEG_R_OPEN) {
+                int i = t, j = g->cell;
+                if (sp == SP_ENERGY) {
+                    q->Fr[IX(i, j + 1)] = 5.0 / 2.0 * ORC_BOLTZ * T[IX(i, j)] * fmax(f->rhoUer[IX(i, j + 1)], 0.0);
+                } else {
+                    double uir = p->mu_i * f->Er[IX(i, j + 1)] + ORC_EPS_VD;
+                    q->Fr[IX(i, j + 1)] = f->rhoi[IX(i, j)] * uir;
                 }
-                double see = -eps_emit * p->coefIISEE_ts * f->rhoUix[IX(fi, j)] -
-                             (eb + (g_plant == ORC_PLANT_SEE_RD_RATIO ? 1.0 : p->ratioEngy_EISEE_rd) * rd) *
-                                 n_old[IX(i, j)] * gpne -
-                             eps_emit * ts * f->rhoe_old[IX(i, j)] * gpn;
-                q->Fx[IX(fi, j)] = n[IX(i, j)] * gpne + see;
             }
         }
     }
-    for (int i = 1; i <= ni; i++) {
-        if (sp == SP_ENERGY) {
-            q->Fr[IX(i, nj + 1)] = 5.0 / 2.0 * ORC_BOLTZ * T[IX(i, nj)] * fmax(f->rhoUer[IX(i, nj + 1)], 0.0);
-        } else {
-            double uir = p->mu_i * f->Er[IX(i, nj + 1)] + ORC_EPS_VD;
-            q->Fr[IX(i, nj + 1)] = f->rhoi[IX(i, nj)] * uir;
-        }
-    }
-    /* coefficients (:3544-3664, :4508-4642) */
+    /* coefficients on the blocks' cells (:3544-3664, :4508-4642) */
     double* S = src_or_zero(C, f->rate_ionize);
     double* Pabs = src_or_zero(C, f->Pabs);
     double* eloss = src_or_zero(C, f->rate_eloss);
-    for (int i = 1; i <= ni; i++)
-        for (int j = 1; j <= nj; j++) {
-            double rR = (r[j] + r[j + 1]) / 2.0, rL = (r[j] + r[j - 1]) / 2.0;
-            double qR = (r[j] + r[j + 1]) / 2.0 / r[j];
-            double qL = (r[j] + r[j - 1]) / 2.0 / r[j];
-            face_t Lx = sg_face(p, sp, m.mu_star, f->Ex[IX(i, j)], T[IX(i - 1, j)], T[IX(i, j)], dx);
-            face_t Rx = sg_face(p, sp, m.mu_star, f->Ex[IX(i + 1, j)], T[IX(i, j)], T[IX(i + 1, j)], dx);
-            face_t Lr = sg_face(p, sp, m.mu_star, f->Er[IX(i, j)], T[IX(i, j - 1)], T[IX(i, j)], dr);
-            face_t Rr = sg_face(p, sp, m.mu_star, f->Er[IX(i, j + 1)], T[IX(i, j)], T[IX(i, j + 1)], dr);
-            double bLx = (double)(i != 1), bRx = (double)(i != ni);
-            double bLr = (double)(j != 1), bRr = (double)(j != nj);
-            double aP = orc_bern_bp(Rx.vd, Rx.z, Rx.D / dx, lit) * bRx * dr + orc_bern_bm(Lx.vd, Lx.z, Lx.D / dx, lit) * bLx * dr +
-                        qR * orc_bern_bp(Rr.vd, Rr.z, Rr.D / dr, lit) * bRr * dx +
-                        qL * orc_bern_bm(Lr.vd, Lr.z, Lr.D / dr, lit) * bLr * dx;
-            C->aP[IX(i, j)] = aP + q->tfac * dx * dr / dt;
-            C->aE[IX(i, j)] = orc_bern_bm(Rx.vd, Rx.z, Rx.D / dx, lit) * bRx * dr;
-            C->aW[IX(i, j)] = orc_bern_bp(Lx.vd, Lx.z, Lx.D / dx, lit) * bLx * dr;
-            C->aN[IX(i, j)] = qR * orc_bern_bm(Rr.vd, Rr.z, Rr.D / dr, lit) * bRr * dx;
-            C->aS[IX(i, j)] = qL * orc_bern_bp(Lr.vd, Lr.z, Lr.D / dr, lit) * bLr * dx;
-            double div = (q->Fx[IX(i + 1, j)] - q->Fx[IX(i, j)]) / dx + (qR * q->Fr[IX(i, j + 1)] - qL * q->Fr[IX(i, j)]) / dr;
-            if (sp == SP_ENERGY) {
-                double bLxw = (double)(i != 1), bRxw = (double)(i != ni); /* wall flags only */
-                double JEx = -ORC_E0 * (f->rhoUex[IX(i + 1, j)] * f->Ex[IX(i + 1, j)] * bRxw + f->rhoUex[IX(i, j)] * f->Ex[IX(i, j)] * bLxw) / 2.0;
-                double JEr = -ORC_E0 * (rR * f->rhoUer[IX(i, j + 1)] * f->Er[IX(i, j + 1)] * 1.0 + rL * f->rhoUer[IX(i, j)] * f->Er[IX(i, j)] * 1.0) /
-                             (2.0 * r[j]);
-                double R1 = (-n[IX(i, j)] + n_old[IX(i, j)]) / dt;
-                double R2 = JEx + JEr;
-                double R3 = Pabs[IX(i, j)] - f->rhoe[IX(i, j)] * eloss[IX(i, j)];
-                double R4 = -div;
-                C->b[IX(i, j)] = 3.0 / 5.0 * (R1 + R2 + R3 + R4) * dx * dr;
-            } else {
-                C->b[IX(i, j)] = (-div + S[IX(i, j)] - (n[IX(i, j)] - n_old[IX(i, j)]) / dt) * dx * dr;
+    for (int bl = 0; bl < G->n_bl; bl++) {
+        const int* B = G->cell_box + 4 * bl;
+        for (int i = B[0]; i <= B[1]; i++)
+            for (int j = B[2]; j <= B[3]; j++) {
+                double rR = (r[j] + r[j + 1]) / 2.0, rL = (r[j] + r[j - 1]) / 2.0;
+                double qR = (r[j] + r[j + 1]) / 2.0 / r[j];
+                double qL = (r[j] + r[j - 1]) / 2.0 / r[j];
+                face_t Lx = sg_face(p, sp, m.mu_star, f->Ex[IX(i, j)], T[IX(i - 1, j)], T[IX(i, j)], dx);
+                face_t Rx = sg_face(p, sp, m.mu_star, f->Ex[IX(i + 1, j)], T[IX(i, j)], T[IX(i + 1, j)], dx);
+                face_t Lr = sg_face(p, sp, m.mu_star, f->Er[IX(i, j)], T[IX(i, j - 1)], T[IX(i, j)], dr);
+                face_t Rr = sg_face(p, sp, m.mu_star, f->Er[IX(i, j + 1)], T[IX(i, j)], T[IX(i, j + 1)], dr);
+                /* the face coefficients count between two plasma cells only
+                 * (walls, open face and axis are explicit fluxes) */
+                double bLx = (double)(C->xc[IX(i, j)] == FC_INNER), bRx = (double)(C->xc[IX(i + 1, j)] == FC_INNER);
+                double bLr = (double)(C->rc[IX(i, j)] == FC_INNER), bRr = (double)(C->rc[IX(i, j + 1)] == FC_INNER);
+                if (g_plant == ORC_PLANT_GEO_WALL_COEF) {
+                    bLx = (double)(C->xc[IX(i, j)] != FC_NONE);
+                    bRx = (double)(C->xc[IX(i + 1, j)] != FC_NONE);
+                }
+                double aP = orc_bern_bp(Rx.vd, Rx.z, Rx.D / dx, lit) * bRx * dr + orc_bern_bm(Lx.vd, Lx.z, Lx.D / dx, lit) * bLx * dr +
+                            qR * orc_bern_bp(Rr.vd, Rr.z, Rr.D / dr, lit) * bRr * dx +
+                            qL * orc_bern_bm(Lr.vd, Lr.z, Lr.D / dr, lit) * bLr * dx;
+                C->aP[IX(i, j)] = aP + q->tfac * dx * dr / dt;
+                C->aE[IX(i, j)] = orc_bern_bm(Rx.vd, Rx.z, Rx.D / dx, lit) * bRx * dr;
+                C->aW[IX(i, j)] = orc_bern_bp(Lx.vd, Lx.z, Lx.D / dx, lit) * bLx * dr;
+                C->aN[IX(i, j)] = qR * orc_bern_bm(Rr.vd, Rr.z, Rr.D / dr, lit) * bRr * dx;
+                C->aS[IX(i, j)] = qL * orc_bern_bp(Lr.vd, Lr.z, Lr.D / dr, lit) * bLr * dx;
+                double div = (q->Fx[IX(i + 1, j)] - q->Fx[IX(i, j)]) / dx + (qR * q->Fr[IX(i, j + 1)] - qL * q->Fr[IX(i, j)]) / dr;
+                if (sp == SP_ENERGY) {
+                    /* J.E without the wall faces (:4624-4625) */
+                    double bLxw = (double)(C->xc[IX(i, j)] != FC_WALL), bRxw = (double)(C->xc[IX(i + 1, j)] != FC_WALL);
+                    double bLrw = (double)(C->rc[IX(i, j)] != FC_WALL), bRrw = (double)(C->rc[IX(i, j + 1)] != FC_WALL);
+                    double JEx = -ORC_E0 * (f->rhoUex[IX(i + 1, j)] * f->Ex[IX(i + 1, j)] * bRxw + f->rhoUex[IX(i, j)] * f->Ex[IX(i, j)] * bLxw) / 2.0;
+                    double JEr = -ORC_E0 * (rR * f->rhoUer[IX(i, j + 1)] * f->Er[IX(i, j + 1)] * bRrw + rL * f->rhoUer[IX(i, j)] * f->Er[IX(i, j)] * bLrw) /
+                                 (2.0 * r[j]);
+                    double R1 = (-n[IX(i, j)] + n_old[IX(i, j)]) / dt;
+                    double R2 = JEx + JEr;
+                    double R3 = Pabs[IX(i, j)] - f->rhoe[IX(i, j)] * eloss[IX(i, j)];
+                    double R4 = -div;
+                    C->b[IX(i, j)] = 3.0 / 5.0 * (R1 + R2 + R3 + R4) * dx * dr;
+                } else {
+                    C->b[IX(i, j)] = (-div + S[IX(i, j)] - (n[IX(i, j)] - n_old[IX(i, j)]) / dt) * dx * dr;
+                }
             }
-        }
+    }
 }
 
 static void sg_routine(ctx_t* C, cathode_ndd_fields* f, int sp, const sg_eq_t* q, double* n_old,
                        double* Fx_old, double* Fr_old, double* err3)
 {
     const cathode_ndd_params* p = C->p;
-    const int ni = C->ni, nj = C->nj;
+    const cathode_ndd_geometry* G = C->G;
     const double dx = p->dx, dr = p->dr;
     const int lit = g_literal;
     mob_t m = mobility(q->mu);
@@ -584,36 +808,59 @@ static void sg_routine(ctx_t* C, cathode_ndd_fields* f, int sp, const sg_eq_t* q
         memcpy(Fr_t, q->Fr, C->n * sizeof(double));
         sg_predict_assemble(C, f, sp, q, Fx_t, Fr_t, n_old);
         solve5(C, dn, q->lin_maxit, q->lin_tol);
-        /* flux correction (:3682-3743) */
-        for (int i = 2; i <= ni; i++)
-            for (int j = 1; j <= nj; j++) {
-                face_t F = sg_face(p, sp, m.mu_star, f->Ex[IX(i, j)], q->T[IX(i - 1, j)], q->T[IX(i, j)], dx);
-                double d = orc_bern_bp(F.vd, F.z, F.D / dx, lit) * dn[IX(i - 1, j)] - orc_bern_bm(F.vd, F.z, F.D / dx, lit) * dn[IX(i, j)];
-                if (sp == SP_ENERGY) d = d * 5.0 / 3.0;
-                q->Fx[IX(i, j)] = q->Fx[IX(i, j)] + d;
-            }
-        for (int i = 1; i <= ni; i++)
-            for (int j = 2; j <= nj; j++) {
-                face_t F = sg_face(p, sp, m.mu_star, f->Er[IX(i, j)], q->T[IX(i, j - 1)], q->T[IX(i, j)], dr);
-                double d = orc_bern_bp(F.vd, F.z, F.D / dr, lit) * dn[IX(i, j - 1)] - orc_bern_bm(F.vd, F.z, F.D / dr, lit) * dn[IX(i, j)];
-                if (sp == SP_ENERGY) d = d * 5.0 / 3.0;
-                q->Fr[IX(i, j)] = q->Fr[IX(i, j)] + d;
-            }
-        /* positivity-guarded update (:3748-3757) */
-        for (int i = 1; i <= ni; i++)
-            for (int j = 1; j <= nj; j++) {
-                double nn = q->n[IX(i, j)] + dn[IX(i, j)];
-                if (nn > 0.0) q->n[IX(i, j)] = nn;
-            }
-        double e1 = orc_calc_res(q->n, n_t, C->ld, 1, ni, 1, nj, 1e-100, p->flag_error);
-        double e2 = orc_calc_res(q->Fx, Fx_t, C->ld, 2, ni, 1, nj, 1e-100, p->flag_error);
-        double e3 = orc_calc_res(q->Fr, Fr_t, C->ld, 1, ni, 2, nj, 1e-100, p->flag_error);
-        double eg = fmax(fmax(e1, e2), e3);
+        /* flux correction on the inner faces (:3682-3743) */
+        for (int bl = 0; bl < G->n_bl; bl++) {
+            const int* B = G->xface_box + 4 * bl;
+            for (int i = B[0]; i <= B[1]; i++)
+                for (int j = B[2]; j <= B[3]; j++) {
+                    face_t F = sg_face(p, sp, m.mu_star, f->Ex[IX(i, j)], q->T[IX(i - 1, j)], q->T[IX(i, j)], dx);
+                    double d = orc_bern_bp(F.vd, F.z, F.D / dx, lit) * dn[IX(i - 1, j)] - orc_bern_bm(F.vd, F.z, F.D / dx, lit) * dn[IX(i, j)];
+                    if (sp == SP_ENERGY) d = d * 5.0 / 3.0;
+                    q->Fx[IX(i, j)] = q->Fx[IX(i, j)] + d;
+                }
+        }
+        for (int bl = 0; bl < G->n_bl; bl++) {
+            const int* B = G->rface_box + 4 * bl;
+            for (int i = B[0]; i <= B[1]; i++)
+                for (int j = B[2]; j <= B[3]; j++) {
+                    face_t F = sg_face(p, sp, m.mu_star, f->Er[IX(i, j)], q->T[IX(i, j - 1)], q->T[IX(i, j)], dr);
+                    double d = orc_bern_bp(F.vd, F.z, F.D / dr, lit) * dn[IX(i, j - 1)] - orc_bern_bm(F.vd, F.z, F.D / dr, lit) * dn[IX(i, j)];
+                    if (sp == SP_ENERGY) d = d * 5.0 / 3.0;
+                    q->Fr[IX(i, j)] = q->Fr[IX(i, j)] + d;
+                }
+        }
+        /* positivity-guarded update on the blocks' cells (:3748-3757) */
+        for (int bl = 0; bl < G->n_bl; bl++) {
+            const int* B = G->cell_box + 4 * bl;
+            for (int i = B[0]; i <= B[1]; i++)
+                for (int j = B[2]; j <= B[3]; j++) {
+                    double nn = q->n[IX(i, j)] + dn[IX(i, j)];
+                    if (nn > 0.0) q->n[IX(i, j)] = nn;
+                }
+        }
+        double e[3];
+        calc_res3(C, q->n, n_t, q->Fx, Fx_t, q->Fr, Fr_t, e);
+        double eg = fmax(fmax(e[0], e[1]), e[2]);
         if (eg < q->cnv || ncount >= q->maxit) break;
     }
-    err3[0] = orc_calc_res(q->n, n_old, C->ld, 1, ni, 1, nj, 1e-100, p->flag_error);
-    err3[1] = orc_calc_res(q->Fx, Fx_old, C->ld, 2, ni, 1, nj, 1e-100, p->flag_error);
-    err3[2] = orc_calc_res(q->Fr, Fr_old, C->ld, 1, ni, 2, nj, 1e-100, p->flag_error);
+    calc_res3(C, q->n, n_old, q->Fx, Fx_old, q->Fr, Fr_old, err3);
+}
+
+/* ghost writes after a routine: the neighbour across each listed face
+ * takes the cell's value (flag 1) or 0 (flag 0), in list order */
+static void ghost_list(ctx_t* C, double* a, const cathode_ndd_segment* list, int cnt)
+{
+    for (int s = 0; s < cnt; s++) {
+        const cathode_ndd_segment* g = &list[s];
+        for (int t = g->lo; t <= g->hi; t++) {
+            int ci, cj, gi, gj;
+            if (g->kind == CATHODE_SEG_X_LOW) { ci = g->cell; cj = t; gi = ci - 1; gj = cj; }
+            else if (g->kind == CATHODE_SEG_X_HIGH) { ci = g->cell; cj = t; gi = ci + 1; gj = cj; }
+            else if (g->kind == CATHODE_SEG_R_LOW) { ci = t; cj = g->cell; gi = ci; gj = cj - 1; }
+            else { ci = t; cj = g->cell; gi = ci; gj = cj + 1; }
+            a[IX(gi, gj)] = g->flag ? a[IX(ci, cj)] : 0.0;
+        }
+    }
 }
 
 static void set_eq(const cathode_ndd_params* p, cathode_ndd_fields* f, int sp, const double* Te_old, sg_eq_t* q)
@@ -680,7 +927,7 @@ int orc_ndd_step(const cathode_ndd_params* pin, cathode_ndd_fields* f, int nstep
     struct timespec t0, t1;
     clock_gettime(CLOCK_MONOTONIC, &t0);
     ctx_init(C, p, solver, rep);
-    const int ni = C->ni, nj = C->nj;
+    const cathode_ndd_geometry* G = C->G;
     double* s_old = (double*)calloc(C->n, sizeof(double));
     double* sx_old = (double*)calloc(C->n, sizeof(double));
     double* sr_old = (double*)calloc(C->n, sizeof(double));
@@ -705,23 +952,34 @@ int orc_ndd_step(const cathode_ndd_params* pin, cathode_ndd_fields* f, int nstep
         /* 1. ions */
         set_eq(p, f, SP_ION, NULL, &q);
         sg_routine(C, f, SP_ION, &q, rhoi_old, sx_old, sr_old, err3);
-        for (int i = 1; i <= ni; i++) f->rhoi[IX(i, nj + 1)] = f->rhoi[IX(i, nj)];
+        /* the ion density across the open faces */
+        for (int s = 0; s < G->n_seg; s++)
+            if (G->seg[s].kind == CATHODE_SEG_R_OPEN)
+                for (int i = G->seg[s].lo; i <= G->seg[s].hi; i++)
+                    f->rhoi[IX(i, G->seg[s].cell + 1)] = f->rhoi[IX(i, G->seg[s].cell)];
         if (rep) rep->error_rhoi = err3[0];
         /* 2. Poisson */
         poisson_routine(C, f, s_old, sx_old, sr_old);
         /* 3. electrons */
         set_eq(p, f, SP_ELEC, NULL, &q);
         sg_routine(C, f, SP_ELEC, &q, f->rhoe_old, sx_old, sr_old, err3);
+        /* electron-density ghosts (:3873-3913): 0 behind walls, copies across open sides */
+        ghost_list(C, f->rhoe, G->ne, G->n_ne);
         if (rep) {
             rep->error_rhoe = err3[0];
             rep->error_rhoUex = err3[1];
             rep->error_rhoUer = err3[2];
+            /* the electrode current over the faces at V_bias (:3942-3965) */
             double Ii = 0.0, Ie = 0.0;
-            for (int j = 1; j <= nj; j++) {
-                double di = f->rhoUix[IX(ni + 1, j)] * 2.0 * M_PI * C->r[j] * p->dr * (ORC_E0);
-                double de = f->rhoUex[IX(ni + 1, j)] * 2.0 * M_PI * C->r[j] * p->dr * (-ORC_E0);
-                Ii += di;
-                Ie += de;
+            for (int s = 0; s < G->n_seg; s++) {
+                const cathode_ndd_segment* g = &G->seg[s];
+                if (g->kind != CATHODE_SEG_X_HIGH || !g->flag) continue;
+                for (int j = g->lo; j <= g->hi; j++) {
+                    double di = f->rhoUix[IX(g->cell + 1, j)] * 2.0 * M_PI * C->r[j] * p->dr * (ORC_E0);
+                    double de = f->rhoUex[IX(g->cell + 1, j)] * 2.0 * M_PI * C->r[j] * p->dr * (-ORC_E0);
+                    Ii += di;
+                    Ie += de;
+                }
             }
             rep->Ii_anode = Ii;
             rep->Ie_anode = Ie;
@@ -737,25 +995,29 @@ int orc_ndd_step(const cathode_ndd_params* pin, cathode_ndd_fields* f, int nstep
                 rep->error_Gx = err3[1];
                 rep->error_Gr = err3[2];
             }
-            for (int i = 1; i <= ni; i++)
-                for (int j = 1; j <= nj; j++)
-                    if (f->rhoeps[IX(i, j)] < 0.0) f->rhoeps[IX(i, j)] = fmax(s_old[IX(i, j)], 0.0);
+            /* positivity guard, max n_e and Te on the blocks' cells (:4866-4901) */
+            for (int bl = 0; bl < G->n_bl; bl++) {
+                const int* B = G->cell_box + 4 * bl;
+                for (int i = B[0]; i <= B[1]; i++)
+                    for (int j = B[2]; j <= B[3]; j++)
+                        if (f->rhoeps[IX(i, j)] < 0.0) f->rhoeps[IX(i, j)] = fmax(s_old[IX(i, j)], 0.0);
+            }
             double nmax = 0.0;
-            for (int i = 1; i <= ni; i++)
-                for (int j = 1; j <= nj; j++)
-                    if (nmax < f->rhoe[IX(i, j)]) nmax = f->rhoe[IX(i, j)];
+            for (int bl = 0; bl < G->n_bl; bl++) {
+                const int* B = G->cell_box + 4 * bl;
+                for (int i = B[0]; i <= B[1]; i++)
+                    for (int j = B[2]; j <= B[3]; j++)
+                        if (nmax < f->rhoe[IX(i, j)]) nmax = f->rhoe[IX(i, j)];
+            }
             const double Te_max = 1000.0 * ORC_E0 / ORC_BOLTZ, ratio = 0.001;
-            for (int i = 1; i <= ni; i++)
-                for (int j = 1; j <= nj; j++)
-                    f->Te[IX(i, j)] = fmin(f->rhoeps[IX(i, j)] / (3.0 / 2.0 * (f->rhoe[IX(i, j)] + nmax * ratio) * ORC_BOLTZ + 1e-100), Te_max);
-            for (int j = 1; j <= nj; j++) {
-                f->Te[IX(0, j)] = f->Te[IX(1, j)];
-                f->Te[IX(ni + 1, j)] = f->Te[IX(ni, j)];
+            for (int bl = 0; bl < G->n_bl; bl++) {
+                const int* B = G->cell_box + 4 * bl;
+                for (int i = B[0]; i <= B[1]; i++)
+                    for (int j = B[2]; j <= B[3]; j++)
+                        f->Te[IX(i, j)] = fmin(f->rhoeps[IX(i, j)] / (3.0 / 2.0 * (f->rhoe[IX(i, j)] + nmax * ratio) * ORC_BOLTZ + 1e-100), Te_max);
             }
-            for (int i = 1; i <= ni; i++) {
-                f->Te[IX(i, 0)] = f->Te[IX(i, 1)];
-                f->Te[IX(i, nj + 1)] = f->Te[IX(i, nj)];
-            }
+            /* Te ghosts (:4906-4946) */
+            ghost_list(C, f->Te, G->te, G->n_te);
         }
         if (rep) rep->steps++;
     }
@@ -787,6 +1049,7 @@ int orc_ndd_assemble(const cathode_ndd_params* p, const cathode_ndd_fields* fin,
         f.nUex = fx;
         f.nUer = fr;
         memcpy(C->Ex_old, fin->Ex, nb);
+        memcpy(C->Er_old, fin->Er, nb);
         memcpy(C->tmp_x, fin->nUex, nb);
         memcpy(C->tmp_r, fin->nUer, nb);
         poisson_predict_assemble(C, &f, C->tmp_x, C->tmp_r);

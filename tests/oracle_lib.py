@@ -38,6 +38,9 @@ _SIGS = [
     ("orc_bern_bm", ctypes.c_double, [ctypes.c_double] * 3 + [ctypes.c_int]),
     ("orc_set_literal_bernoulli", None, [ctypes.c_int]),
     ("orc_plant", None, [ctypes.c_int]),
+    ("orc_ndd_set_geometry", None, [ctypes.POINTER(abi.NddGeometry)]),
+    ("orc_calc_res_boxes", ctypes.c_double,
+     [_D, _D, ctypes.c_int, _I, ctypes.c_int, ctypes.c_double, ctypes.c_int]),
     ("orc_mw_energy_profile", ctypes.c_int,
      [ctypes.POINTER(abi.MwParams), ctypes.POINTER(abi.MwFields), ctypes.POINTER(abi.MwReport)]),
     ("orc_check_cfl", None, [ctypes.POINTER(abi.NddParams), ctypes.POINTER(abi.NddFields), _D]),
@@ -243,6 +246,8 @@ ORC_PLANT_MW_HALL_SIGN = 10
 ORC_PLANT_MW_POYNTING_CONJ = 11
 ORC_PLANT_MW_PORT_WAVES = 12
 ORC_PLANT_MW_RFACE_WEIGHT = 13
+ORC_PLANT_GEO_FOLD = 14
+ORC_PLANT_GEO_WALL_COEF = 15
 
 
 def check_cfl(params, state):
@@ -260,3 +265,8 @@ def mw_energy_profile(params, arrays):
     rc = lib().orc_mw_energy_profile(ctypes.byref(params), ctypes.byref(f), ctypes.byref(rep))
     assert rc == 0
     return rep
+
+
+def set_geometry(g):
+    """the oracle's plasma geometry for the following steps (None: the rectangle)"""
+    lib().orc_ndd_set_geometry(ctypes.byref(g) if g is not None else None)

@@ -140,6 +140,10 @@ def _see_state(see_on):
     st["rhoUix"] = rng.uniform(1e19, 3e19, shp)
     st["rhoUix"][1, :] *= -1.0
     st["rhoe_old"] = st["rhoe"] * rng.uniform(0.7, 1.3, shp)
+    # a retarding sheath on both walls (the wall 30 V below the cell), so the
+    # thermal flux does not dwarf the ion-induced emission
+    st["Ex"][1, :] = -3e5
+    st["Ex"][p.ni + 1, :] = 3e5
     p.Te_emitSEE_eV = T_EMIT
     p.ratioEngy_EISEE_rd = R_RD
     if see_on:

@@ -6,7 +6,10 @@ distribution fixes them independently of the closed forms the oracle codes:
 - ions (fluidModule.hpp:1502-1526): the flux of a Maxwellian at the ion
   temperature drifting at mu_i E, through the wall face, on both walls;
 - electrons (:3250-3367): the flux of a Maxwellian at rest over a retarding
-  barrier e dphi, dphi = max(-+E h/2, 0);
+  barrier e dphi, dphi = phi_cell - phi_wall when positive: with E = -dphi/dx
+  that is max(-E h/2, 0) on a wall below the cell and max(+E h/2, 0) on a
+  wall above it (the reference's :3255, :3335), so electrons are held back
+  where the wall sits below the plasma potential;
 - electron energy (:4174-4316): the energy those electrons carry, the
   parallel energy left after the barrier plus kT of the two other directions.
 Secondary emission is off (no coefficients, no EISEE table), as in the
@@ -55,14 +58,14 @@ def test_ion_wall_flux_is_drifting_maxwellian_half_range():
 
 def test_electron_wall_flux_is_maxwellian_over_the_sheath_barrier():
     p, st = W.config0(ni=8, nj=4, h=1e-4)
-    Exs = np.array([-3e4, -1e2, 0.0, 2e4])  # retarding on the left wall when E > 0
+    Exs = np.array([-3e4, -1e2, 0.0, 2e4])  # retarding on the left wall when E < 0
     st["Ex"][1, 1:5] = Exs
     st["Ex"][p.ni + 1, 1:5] = Exs
     fx = O.ndd_assemble(p, st, 1)[6]
     for j in range(1, 5):
         for face, cell, sgn in ((1, 1, -1.0), (p.ni + 1, p.ni, 1.0)):
             Te = st["Te"][cell, j]
-            dphi = max(-sgn * st["Ex"][face, j] * p.dx / 2.0, 0.0)
+            dphi = max(sgn * st["Ex"][face, j] * p.dx / 2.0, 0.0)
             vmin = np.sqrt(2.0 * E0 * dphi / p.masse)
             expect = sgn * st["rhoe"][cell, j] * moment(p.masse, Te, 0.0, vmin, np.inf)
             assert fx[face, j] == pytest.approx(expect, rel=1e-9), (face, j)
@@ -82,7 +85,7 @@ def test_electron_wall_energy_flux_is_carried_energy_over_the_barrier():
     for j in range(1, 5):
         for face, cell, sgn in ((1, 1, -1.0), (ni + 1, ni, 1.0)):
             T = st["Te"][cell, j]
-            dphi = max(-sgn * st["Ex"][face, j] * p.dx / 2.0, 0.0)
+            dphi = max(sgn * st["Ex"][face, j] * p.dx / 2.0, 0.0)
             vmin = np.sqrt(2.0 * E0 * dphi / p.masse)
             # parallel kinetic energy after the barrier, plus kT of the two other directions
             energy = lambda v: 0.5 * p.masse * (v * v - vmin * vmin) + KB * T
