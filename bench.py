@@ -49,8 +49,10 @@ def sig(v):
     return float(f"{v:.6g}") if isinstance(v, float) else v
 
 
-def run_path(lib, mode, steps, warmup, seed, net=None, warm=False):
-    p, st = W.config1(seed=seed, mode=mode)
+def run_path(lib, mode, steps, warmup, seed, net=None, warm=False, grid=(512, 256)):
+    ni, nj = grid
+    # the same 25.6 mm x 12.8 mm domain at a finer or coarser spacing
+    p, st = W.config1(seed=seed, ni=ni, nj=nj, h=5e-5 * 512 / ni, mode=mode)
     # learned mode: previous-solution warm starts, plus the network's guess
     # for the first Poisson correction solve when a network is attached
     p.learned = 1 if (net is not None or warm) else 0
@@ -247,6 +249,9 @@ def main():
     ap.add_argument("--paths", default="fp64,mixed,learned",
                     help="fp64, mixed (FP32 V-cycles + FP64 refinement), learned (mixed + learned initial guess), "
                          "warm (mixed + previous-solution warm starts, no network: the learned path's ablation)")
+    ap.add_argument("--grid", default="512x256",
+                    help="NIxNJ: the same domain on another grid (the metric is quoted at 512x256; larger grids "
+                         "show the stencil kernels away from the latency-bound regime)")
     ap.add_argument("--no-sweep", dest="sweep", action="store_false",
                     help="skip the 64-instance bias sweep (batched vs one at a time), run by default")
     ap.add_argument("--sweep-steps", type=int, default=50)
@@ -261,6 +266,10 @@ def main():
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
               "the single-instance implicit step does not shard across GPUs here", file=sys.stderr)
         sys.exit(2)
+    grid = tuple(int(v) for v in args.grid.lower().split("x"))
+    if len(grid) != 2 or grid[0] < 8 or grid[1] < 8 or grid[0] * 256 != grid[1] * 512:
+        print("bench.py: --grid must be NIxNJ with NI = 2 NJ (the 512x256 domain's aspect)", file=sys.stderr)
+        sys.exit(2)
     lib = abi.load()
     abi.check(lib.cathode_init(0), lib)
     paths = [s.strip() for s in args.paths.split(",") if s.strip()]
@@ -272,7 +281,7 @@ def main():
         if name == "learned" and net is None:
             net = L.load_net(lib)  # weights trained by learned.py, committed with the package
         res[name], states[name] = run_path(lib, mode, args.steps, args.warmup, args.seed,
-                                           net if name == "learned" else None, warm=name == "warm")
+                                           net if name == "learned" else None, warm=name == "warm", grid=grid)
         print(f"# {name}: {res[name]['steps_per_s']:.2f} steps/s, "
               f"{res[name]['iterations_per_solve']:.2f} BiCGStab it/solve, "
               f"smooth {res[name]['smooth_avg_us']:.2f} us ({res[name]['smooth_GBps']:.0f} GB/s)",
@@ -290,7 +299,8 @@ def main():
     b = res[best]
     roof = {"bound": "hbm", "achieved": round(b["smooth_GBps"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(b["smooth_GBps"] / HBM_PEAK_GBS, 4),
-            "traffic": traffic_per_launch("mixed" if best == "learned" else best, b["smooth_tile"]),
+            "traffic": (traffic_per_launch("mixed" if best == "learned" else best, b["smooth_tile"])
+                        if grid == (512, 256) else None),
             "kernel": "k_vtile finest-level down kernel (BiCGStab p/s update + 2 red-black sweeps + residual + "
                       "restriction), "
                       + ("FP64" if best == "fp64" else "FP32") + ", tile " + b["smooth_tile"],
@@ -311,8 +321,9 @@ def main():
         "vs_baseline": None,
         "dtype": {"fp64": "f64", "mixed": "f32+f64", "learned": "f32+f64 (bf16 network)", "warm": "f32+f64"}[best],
         "data": "synthetic",
-        "config": {"workload": "implicit NDD step, 512x256 cells of 0.05 mm, ions+Poisson+electrons+energy, "
-                               "each solve to relative residual 1e-10", "grid": [512, 256], "seed": args.seed,
+        "config": {"workload": f"implicit NDD step, {grid[0]}x{grid[1]} cells of {0.05 * 512 / grid[0]:.4g} mm, "
+                               "ions+Poisson+electrons+energy, each solve to relative residual 1e-10",
+                   "grid": list(grid), "seed": args.seed,
                    "path": {"fp64": "plain FP64", "mixed": "FP32 V-cycles + FP64 refinement",
                             "learned": "learned initial guess + FP32 V-cycles + FP64 refinement",
                             "warm": "previous-solution warm starts + FP32 V-cycles + FP64 refinement"}[best]},
