@@ -9,9 +9,10 @@
 // with M = 16 pixels of one row, N = 16 output channels, K = 9 taps x 16
 // input channels per step.  Training runs on the device with the library's
 // own kernels: the backward data pass is the same MFMA convolution with the
-// weights transposed and flipped, the weight gradient is an FP32 reduction
-// over pixels (fixed order, so gradients are reproducible), and the
-// optimiser is Adam on FP32 master weights.
+// weights transposed and flipped, the weight gradient is a GEMM over pixels
+// on the matrix cores (per-band partials summed in a fixed order, so
+// gradients are reproducible), and the optimiser is Adam on FP32 master
+// weights.
 //
 // Activations are NHWC over the whole (ni+2) x (nj+2) grid including the
 // ghost ring, which stays zero (the convolution's zero padding).
@@ -261,6 +262,89 @@ __global__ void __launch_bounds__((wgrad_threads<CIN, COUT>()))
     }
 }
 
+// The same partials on the matrix cores: dW is a GEMM with M = COUT output
+// channels, N = 9 CIN (tap, input channel) columns and K = the band's pixels.
+// A chunk of 64 consecutive pixels of one row is staged in LDS with the
+// pixel index contiguous: sA[co][px] = dY, sB[tap * CIN + ci][px] = X at the
+// tap's neighbour (zero beyond the row's end).  Each of the 4 wavefronts owns
+// every 4th 16 x 16 output tile and runs v_mfma_f32_16x16x16_bf16 over the
+// chunk's 4 steps of 16 pixels; the products of BF16 values are exact in
+// FP32, so only the summation order differs from k_cnn_wgrad.  The biases
+// are FP32 sums of sA.  Same partial layout as k_cnn_wgrad.
+constexpr int WG_PX = 64;
+template <int CIN, int COUT>
+__global__ void __launch_bounds__(256)
+    k_cnn_wgrad_mfma(int ni, int nj, int rows_per_block, const bf16_t* __restrict__ X, const bf16_t* __restrict__ dY,
+                     float* __restrict__ part)
+{
+    constexpr int NCOL = 9 * CIN, MT = COUT / 16, NTL = NCOL / 16, T = MT * NTL, TPW = (T + 3) / 4;
+    constexpr int NW = 9 * CIN * COUT, NO = NW + COUT;
+    constexpr int LD = WG_PX + 4;  // row pitch of the staged tiles (bf16), 8-byte aligned rows
+    __shared__ __align__(16) bf16_t sA[COUT * LD];
+    __shared__ __align__(16) bf16_t sB[NCOL * LD];
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int W2 = nj + 2;
+    const int i0 = 1 + blockIdx.x * rows_per_block, i1 = min(ni, i0 + rows_per_block - 1);
+    const int m = lane & 15, g = lane >> 4;
+    f32x4 acc[TPW];
+#pragma unroll
+    for (int q = 0; q < TPW; q++) acc[q] = f32x4{0.f, 0.f, 0.f, 0.f};
+    float bacc = 0.f;
+    const int chunks_per_row = (nj + WG_PX - 1) / WG_PX;
+    const int nchunks = (i1 >= i0) ? (i1 - i0 + 1) * chunks_per_row : 0;
+    for (int c = 0; c < nchunks; c++) {
+        const int i = i0 + c / chunks_per_row, jb = 1 + (c % chunks_per_row) * WG_PX;
+        __syncthreads();
+        // dY: thread per (pixel, 4 channels)
+        for (int q = t; q < WG_PX * (COUT / 4); q += 256) {
+            const int px = q / (COUT / 4), c4 = (q % (COUT / 4)) * 4, j = jb + px;
+            bf16x4 v = bf16x4{0, 0, 0, 0};
+            if (j <= nj) v = *reinterpret_cast<const bf16x4*>(dY + ((size_t)i * W2 + j) * COUT + c4);
+#pragma unroll
+            for (int e = 0; e < 4; e++) sA[(c4 + e) * LD + px] = (bf16_t)v[e];
+        }
+        // X neighbourhoods: thread per (pixel, tap, 4 channels)
+        for (int q = t; q < WG_PX * 9 * (CIN / 4); q += 256) {
+            const int px = q / (9 * (CIN / 4)), rem = q % (9 * (CIN / 4));
+            const int tap = rem / (CIN / 4), c4 = (rem % (CIN / 4)) * 4, j = jb + px;
+            bf16x4 v = bf16x4{0, 0, 0, 0};
+            if (j <= nj) {
+                const int ii = i + tap / 3 - 1, jj = j + tap % 3 - 1;
+                v = *reinterpret_cast<const bf16x4*>(X + ((size_t)ii * W2 + jj) * CIN + c4);
+            }
+#pragma unroll
+            for (int e = 0; e < 4; e++) sB[(tap * CIN + c4 + e) * LD + px] = (bf16_t)v[e];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < TPW; q++) {
+            const int tile = wv + 4 * q;
+            if (tile >= T) break;
+            const int mt = tile / NTL, nt = tile % NTL;
+#pragma unroll
+            for (int ks = 0; ks < WG_PX / 16; ks++) {
+                // A[co][pixel], B[pixel][column]: lane supplies 4 consecutive pixels
+                const bf16x4 a = *reinterpret_cast<const bf16x4*>(&sA[(mt * 16 + m) * LD + ks * 16 + 4 * g]);
+                const bf16x4 b = *reinterpret_cast<const bf16x4*>(&sB[(nt * 16 + m) * LD + ks * 16 + 4 * g]);
+                acc[q] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[q], 0, 0, 0);
+            }
+        }
+        if (t < COUT)
+            for (int px = 0; px < WG_PX; px++) bacc += bf2f(sA[t * LD + px]);
+    }
+    float* out = part + (size_t)blockIdx.x * NO;
+#pragma unroll
+    for (int q = 0; q < TPW; q++) {
+        const int tile = wv + 4 * q;
+        if (tile >= T) break;
+        const int mt = tile / NTL, nt = tile % NTL;
+        // D[row 4g + r][column m]: row = output channel, column = (tap, ci)
+#pragma unroll
+        for (int r = 0; r < 4; r++) out[(size_t)(mt * 16 + 4 * g + r) * NCOL + nt * 16 + m] = acc[q][r];
+    }
+    if (t < COUT) out[NW + t] = bacc;
+}
+
 // grad[o] (+)= sum over blocks of part[b][o], fixed order; o < NW are the
 // weights (same order as the parameter layout), then the biases
 __global__ void k_cnn_wgrad_reduce(const float* __restrict__ part, int nblocks, int no, float* __restrict__ grad,
@@ -375,14 +459,22 @@ static void cnn_forward(CnnNet& N, int ni, int nj, const double* feat, hipStream
     cnn_forward(N, ni, nj, feat, act, N.y.p, s);
 }
 
+// the weight gradient on the matrix cores (default) or on the VALU
+// (cathode_cnn_set_wgrad_valu, for comparison)
+static bool g_wgrad_valu = false;
+
 template <int CIN, int COUT>
 static void wgrad(CnnNet& N, int l, int ni, int nj, const bf16_t* X, const bf16_t* dY, int accumulate, hipStream_t s)
 {
     constexpr int NO = 9 * CIN * COUT + COUT;
     const int rows = 2;  // N.part holds cdiv(ni, 2) bands (cnn_shape)
     const int nblocks = cdiv(ni, rows);
-    hipLaunchKernelGGL((k_cnn_wgrad<CIN, COUT>), dim3(nblocks), dim3(wgrad_threads<CIN, COUT>()), 0, s, ni, nj, rows,
-                       X, dY, N.part.p);
+    if (g_wgrad_valu)
+        hipLaunchKernelGGL((k_cnn_wgrad<CIN, COUT>), dim3(nblocks), dim3(wgrad_threads<CIN, COUT>()), 0, s, ni, nj,
+                           rows, X, dY, N.part.p);
+    else
+        hipLaunchKernelGGL((k_cnn_wgrad_mfma<CIN, COUT>), dim3(nblocks), dim3(256), 0, s, ni, nj, rows, X, dY,
+                           N.part.p);
     // weights and biases are contiguous in the layout (W then b)
     hipLaunchKernelGGL(k_cnn_wgrad_reduce, dim3(cdiv(NO, 256)), dim3(256), 0, s, (const float*)N.part.p, nblocks, NO,
                        N.g.p + g_lay.w_off[l], accumulate);
@@ -460,6 +552,12 @@ void cnn_infer(void* net, int ni, int nj, const double* feat, double* out, CnnSc
 extern "C" {
 
 int cathode_cnn_num_params(void) { return (int)g_lay.total; }
+
+int cathode_cnn_set_wgrad_valu(int on)
+{
+    g_wgrad_valu = on != 0;
+    return CATHODE_OK;
+}
 
 int cathode_cnn_create(int seed, void** net)
 {

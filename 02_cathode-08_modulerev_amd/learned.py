@@ -109,9 +109,10 @@ def load_net(lib, path=WEIGHTS):
 
 
 VARIANTS = ("plain", "warm", "learned")
+FP64_VARIANT = "plain_fp64"  # every solve from x = 0 in FP64: the reference's method on the device
 
 
-def evaluate(lib, net, seeds, steps, ni=512, nj=256, h=5e-5, mode=abi.CATHODE_MODE_MIXED):
+def evaluate(lib, net, seeds, steps, ni=512, nj=256, h=5e-5, mode=abi.CATHODE_MODE_MIXED, fp64=False):
     """Same held-out trajectories three ways: plain (every solve from x = 0,
     as the reference), warm (learned mode without a network: every solve
     from its previous-step solution, least-squares scaled) and learned (the
@@ -119,13 +120,14 @@ def evaluate(lib, net, seeds, steps, ni=512, nj=256, h=5e-5, mode=abi.CATHODE_MO
     Poisson correction solve, warm starts for the others).  BiCGStab
     iterations over all solves, wall time, and the largest field difference
     to the plain run (relative L2)."""
-    out = {v: {"iterations": 0, "seconds": 0.0, "steps": 0} for v in VARIANTS}
+    variants = VARIANTS + ((FP64_VARIANT,) if fp64 else ())
+    out = {v: {"iterations": 0, "seconds": 0.0, "steps": 0} for v in variants}
     out["max_field_rel_l2"] = 0.0
     for s in seeds:
         finals = {}
-        for name in VARIANTS:
-            p, st = sweep_state(s, ni, nj, h, mode)
-            p.learned = 0 if name == "plain" else 1
+        for name in variants:
+            p, st = sweep_state(s, ni, nj, h, abi.CATHODE_MODE_FP64 if name == FP64_VARIANT else mode)
+            p.learned = 0 if name in ("plain", FP64_VARIANT) else 1
             hd = ctypes.c_void_p()
             abi.check(lib.cathode_ndd_create(ctypes.byref(p), ctypes.byref(hd)), lib)
             if name == "learned":
@@ -142,12 +144,12 @@ def evaluate(lib, net, seeds, steps, ni=512, nj=256, h=5e-5, mode=abi.CATHODE_MO
             abi.check(lib.cathode_ndd_get_state(hd, ctypes.byref(abi.fields_struct(fin))), lib)
             finals[name] = fin
             lib.cathode_ndd_destroy(hd)
-        for v in ("warm", "learned"):
+        for v in [x for x in variants if x != "plain"]:
             for k in ("phi", "rhoe", "rhoi", "rhoeps"):
                 a, b = finals[v][k], finals["plain"][k]
                 d = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
                 out["max_field_rel_l2"] = max(out["max_field_rel_l2"], float(d))
-    for name in VARIANTS:
+    for name in variants:
         o = out[name]
         o["steps_per_s"] = o["steps"] / o["seconds"] if o["seconds"] > 0 else 0.0
     return out
@@ -162,10 +164,25 @@ def main():
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--eval-seeds", type=int, default=4)
+    ap.add_argument("--evaluate-only", action="store_true",
+                    help="evaluate the committed weights on held-out seeds (seeds 900..) without training")
+    ap.add_argument("--grid", default="512x256",
+                    help="NIxNJ of the evaluation (the same domain; 1024x512 is the learned-initialisation "
+                         "workload of 1024 x 512 cells)")
     args = ap.parse_args()
     lib = abi.load()
     abi.check(lib.cathode_init(0), lib)
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+    if args.evaluate_only:
+        ni, nj = (int(v) for v in args.grid.lower().split("x"))
+        net = load_net(lib)
+        ev = evaluate(lib, net, list(range(900, 900 + args.eval_seeds)), args.steps, ni=ni, nj=nj,
+                      h=5e-5 * 512 / ni, fp64=True)
+        print(json.dumps({"grid": [ni, nj], "seeds": args.eval_seeds, "steps_per_seed": args.steps,
+                          "weights": os.path.basename(WEIGHTS), "eval": ev}))
+        lib.cathode_cnn_destroy(net)
+        lib.cathode_finalize()
+        return
     t0 = time.perf_counter()
     feat, target = generate_pairs(lib, list(range(100, 100 + args.train_seeds)), args.steps)
     log(f"{feat.shape[0]} pairs in {time.perf_counter() - t0:.1f} s")

@@ -434,6 +434,10 @@ int cathode_cnn_grad_dev(void* net, int ni, int nj, const double* feat,
  * batch: feat = batch x 3 planes, target = batch planes; *loss = mean loss */
 int cathode_cnn_train_step_dev(void* net, int ni, int nj, int batch, const double* feat,
                                const double* target, float lr, double* loss);
+/* Weight gradients on the matrix cores (on = 0, the default: per-band GEMMs
+ * of v_mfma_f32_16x16x16_bf16 over the pixels) or as FP32 VALU reductions
+ * (on = 1); both sum the per-band partials in the same fixed order. */
+int cathode_cnn_set_wgrad_valu(int on);
 /* Attach a network to an NDD handle (NULL detaches); used when
  * params.learned = 1.  The network must outlive its use by the handle. */
 int cathode_ndd_set_learned(void* handle, void* net);

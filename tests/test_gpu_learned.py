@@ -83,6 +83,32 @@ def test_cnn_gradients_match_cpu(hiplib):
         hiplib.cathode_cnn_destroy(net)
 
 
+def test_cnn_mfma_weight_gradient_matches_valu(hiplib):
+    """the matrix-core weight gradient against the FP32 VALU one: the same
+    exact BF16 products, summed in another order"""
+    ni, nj = 50, 70  # a partial 64-pixel chunk per row and an odd band count
+    net = _net(hiplib, 7)
+    try:
+        feat, tgt, fd, td = _case(hiplib, ni, nj, 3)
+        n = hiplib.cathode_cnn_num_params()
+        g = {}
+        for valu in (0, 1):
+            abi.check(hiplib.cathode_cnn_set_wgrad_valu(valu), hiplib)
+            g[valu] = np.zeros(n, dtype=np.float32)
+            loss = ctypes.c_double()
+            abi.check(hiplib.cathode_cnn_grad_dev(net, ni, nj, fd.ptr, td.ptr, ctypes.byref(loss),
+                                                  g[valu].ctypes.data_as(ctypes.POINTER(ctypes.c_float))), hiplib)
+        off, _ = R.layout()
+        for l, (w, b) in enumerate(off):
+            e = rel_l2(g[0][w:b].astype(np.float64), g[1][w:b].astype(np.float64))
+            eb = rel_l2(g[0][b:b + R.COUT[l]].astype(np.float64), g[1][b:b + R.COUT[l]].astype(np.float64))
+            print(f"layer {l}: MFMA vs VALU weight grad rel L2 {e:.2e}, bias {eb:.2e}", flush=True)
+            assert e <= 1e-5 and eb <= 1e-5, (l, e, eb)
+    finally:
+        hiplib.cathode_cnn_set_wgrad_valu(0)
+        hiplib.cathode_cnn_destroy(net)
+
+
 def test_cnn_training_lowers_loss(hiplib):
     ni, nj = 32, 32
     rng = np.random.default_rng(5)
@@ -147,3 +173,8 @@ def test_learned_guess_ablation(hiplib):
     # guess; over 10 steps the measured gain is 5-7% (12% on the bench's
     # steady 100-step run)
     assert gain >= 0.03
+    # the network's term joins the warm start's least-squares span, so the
+    # learned path never needs more iterations than the warm starts alone
+    # (a local 13 x 13 receptive field cannot add more than about a quarter
+    # of an iteration beyond them; the gate keeps its cost off)
+    assert it["learned"] <= it["warm"]
