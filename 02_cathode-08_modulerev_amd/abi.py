@@ -70,6 +70,7 @@ class NddFields(ctypes.Structure):
 NDD_MAX_BLOCKS = 8
 NDD_MAX_SEGS = 24
 SEG_X_LOW, SEG_X_HIGH, SEG_R_LOW, SEG_R_HIGH, SEG_R_OPEN, SEG_R_AXIS = range(6)
+SEG_BIAS, SEG_INLET = 1, 2  # segment flag bits
 
 
 class NddSegment(ctypes.Structure):
@@ -103,6 +104,38 @@ class NddReport(ctypes.Structure):
         ("rhoe_max", ctypes.c_double), ("Te_max", ctypes.c_double), ("ue_max", ctypes.c_double),
         ("iterations_by_solve", ctypes.c_int * 8),
     ]
+
+
+class NeutralParams(ctypes.Structure):
+    _fields_ = [
+        ("which", ctypes.c_int), ("dt_n", ctypes.c_double), ("dt_m", ctypes.c_double),
+        ("Tn", ctypes.c_double), ("ri", ctypes.c_double), ("massi", ctypes.c_double), ("DmN", ctypes.c_double),
+        ("fn_In", ctypes.c_double), ("x_in", ctypes.c_double), ("xL", ctypes.c_double),
+        ("guard_blocks", ctypes.c_int), ("maxITR", ctypes.c_int), ("tol", ctypes.c_double),
+        ("mode", ctypes.c_int), ("flag_error", ctypes.c_int),
+    ]
+
+
+NEUTRAL_FIELD_NAMES = ["rhon", "rhom", "rhoe", "nu_excStep", "nu_super", "nu_excMeta", "nu_exc", "nu_ionz",
+                       "nu_ionzStep", "rhoUix_wall", "rhoUir_wall", "rhoUnx", "rhoUnr", "rhoUnx_wall", "rhoUnr_wall",
+                       "rhoUmx", "rhoUmr", "rhoUmx_wall", "rhoUmr_wall"]
+
+
+class NeutralFields(ctypes.Structure):
+    _fields_ = [(n, c_double_p) for n in NEUTRAL_FIELD_NAMES]
+
+
+class NeutralReport(ctypes.Structure):
+    _fields_ = [("solve_n", Status), ("solve_m", Status), ("error_rhon", ctypes.c_double),
+                ("error_rhom", ctypes.c_double), ("neg_rhon", ctypes.c_int), ("neg_rhom", ctypes.c_int)]
+
+
+def neutral_fields_struct(arrays):
+    """NeutralFields pointing into a dict of numpy arrays (missing keys -> NULL)."""
+    f = NeutralFields()
+    for n in NEUTRAL_FIELD_NAMES:
+        setattr(f, n, dptr(arrays.get(n)))
+    return f
 
 
 MW_MAX_BLOCKS = 16
@@ -245,6 +278,9 @@ SIGNATURES = [
      [c_double_p, c_double_p, ctypes.c_double, ctypes.c_double, ctypes.POINTER(NddGeometry), c_int_p, c_int_p,
       c_int_p]),
     ("cathode_ndd_capture", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    ("cathode_ndd_neutral_update", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.POINTER(NeutralParams), ctypes.POINTER(NeutralFields),
+      ctypes.POINTER(NeutralReport)]),
     ("cathode_ndd_profile", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("cathode_ndd_profile_read", ctypes.c_int,
      [ctypes.c_void_p, c_int_p, c_double_p, c_double_p]),

@@ -98,6 +98,7 @@ bool build_geo_tables(int ni, int nj, const cathode_ndd_geometry& g, GeoTables& 
     T.gs.assign(n, -1);
     T.ne.assign(n, 0);
     T.te.assign(n, 0);
+    T.blk.assign(n, 0);
     T.anode.clear();
     auto at = [&](int i, int j) { return (size_t)i * ld + j; };
     // boxes: inside the grid and disjoint within each kind
@@ -118,7 +119,10 @@ bool build_geo_tables(int ni, int nj, const cathode_ndd_geometry& g, GeoTables& 
                 return fail("r-face boxes overlap");
         }
         for (int i = c[0]; i <= c[1]; i++)
-            for (int j = c[2]; j <= c[3]; j++) T.act[at(i, j)] = 1;
+            for (int j = c[2]; j <= c[3]; j++) {
+                T.act[at(i, j)] = 1;
+                T.blk[at(i, j)] = (uint8_t)(b + 1);
+            }
     }
     T.ncells = 0;
     for (size_t k = 0; k < n; k++) T.ncells += T.act[k];
@@ -150,6 +154,10 @@ bool build_geo_tables(int ni, int nj, const cathode_ndd_geometry& g, GeoTables& 
         const bool xk = q.kind == CATHODE_SEG_X_LOW || q.kind == CATHODE_SEG_X_HIGH;
         if (xk ? (q.cell < 1 || q.cell > ni || q.lo < 1 || q.hi > nj) : (q.cell < 1 || q.cell > nj || q.lo < 1 || q.hi > ni))
             return fail("segment " + std::to_string(s) + " outside the grid");
+        if ((q.flag & ~(CATHODE_SEG_BIAS | CATHODE_SEG_INLET)) || ((q.flag & CATHODE_SEG_BIAS) && !xk) ||
+            ((q.flag & CATHODE_SEG_INLET) && q.kind != CATHODE_SEG_R_HIGH))
+            return fail("segment " + std::to_string(s) +
+                        ": flag bits are CATHODE_SEG_BIAS on x walls and CATHODE_SEG_INLET on high-r walls");
         uint8_t code;
         switch (q.kind) {
         case CATHODE_SEG_X_LOW:
@@ -159,7 +167,8 @@ bool build_geo_tables(int ni, int nj, const cathode_ndd_geometry& g, GeoTables& 
         case CATHODE_SEG_R_OPEN: code = GF_OPEN; break;
         default: code = GF_AXIS; break;
         }
-        if (q.flag && xk) code |= GF_BIAS;
+        if (q.flag & CATHODE_SEG_BIAS) code |= GF_BIAS;
+        if (q.flag & CATHODE_SEG_INLET) code |= GF_INLET;
         for (int t = q.lo; t <= q.hi; t++) {
             const SegPoint P = seg_point(q.kind, q.cell, t);
             std::vector<uint8_t>& fc = xk ? T.xc : T.rc;
@@ -170,10 +179,10 @@ bool build_geo_tables(int ni, int nj, const cathode_ndd_geometry& g, GeoTables& 
             fc[at(P.fi, P.fj)] = code;
             if (q.kind == CATHODE_SEG_R_AXIS) continue;
             const int dir = q.kind == CATHODE_SEG_R_OPEN ? 0 : 1;
-            const int bias = (q.flag && xk) ? 1 : 0;
+            const int bias = (q.flag & CATHODE_SEG_BIAS) ? 1 : 0;
             T.gs[at(P.gi, P.gj)] = (int32_t)((at(P.ci, P.cj) << 2) | (dir << 1) | bias);
         }
-        if (q.kind == CATHODE_SEG_X_HIGH && q.flag) {
+        if (q.kind == CATHODE_SEG_X_HIGH && (q.flag & CATHODE_SEG_BIAS)) {
             T.anode.push_back(q.cell + 1);
             T.anode.push_back(q.lo);
             T.anode.push_back(q.hi);
@@ -287,10 +296,10 @@ extern "C" int cathode_ndd_reference_geometry(const double* xs, const double* rs
         {CATHODE_SEG_X_LOW, C[3][I0], C[3][J0], C[2][J0] - 1, 0},   // z2
         {CATHODE_SEG_X_LOW, C[4][I0], C[1][J0], C[4][J1], 0},       // z4
         {CATHODE_SEG_X_HIGH, C[1][I1], C[1][J0], C[1][J1], 0},      // z3
-        {CATHODE_SEG_X_HIGH, C[4][I1], C[4][J0], C[4][J1], 1},      // z5, the electrode at V_bias
+        {CATHODE_SEG_X_HIGH, C[4][I1], C[4][J0], C[4][J1], CATHODE_SEG_BIAS},  // z5, the electrode at V_bias
         {CATHODE_SEG_R_LOW, C[0][J0], C[0][I0], C[0][I1], 0},       // x0
         {CATHODE_SEG_R_LOW, C[2][J0], C[2][I0], C[2][I1], 0},       // x2
-        {CATHODE_SEG_R_HIGH, C[0][J1], C[0][I0], C[1][I1], 0},      // x1
+        {CATHODE_SEG_R_HIGH, C[0][J1], C[0][I0], C[1][I1], CATHODE_SEG_INLET},  // x1, holds the gas inlet
         {CATHODE_SEG_R_HIGH, C[3][J1], C[1][I1] + 1, C[3][I1], 0},  // x4
         {CATHODE_SEG_R_OPEN, C[4][J1], C[4][I0], C[4][I1], 0},      // x5
         {CATHODE_SEG_R_AXIS, C[3][J0], C[3][I0], C[4][I1], 0},      // x6, the axis row of blocks 3 and 4

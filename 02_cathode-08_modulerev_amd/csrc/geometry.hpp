@@ -22,7 +22,8 @@ enum : uint8_t {
     GF_OPEN = 4,
     GF_AXIS = 5,
     GF_TYPE = 0x0f,
-    GF_BIAS = 0x10
+    GF_BIAS = 0x10,
+    GF_INLET = 0x20  // a high-r wall face of the neutral-gas inlet segment
 };
 
 // per-cell ghost writes after a routine (bit d: write the neighbour on side
@@ -36,6 +37,7 @@ struct GeoTables {
     // (source cell << 2) | (Dirichlet << 1) | bias, the last segment's
     std::vector<int32_t> gs;
     std::vector<uint8_t> ne, te;        // ghost writes of n_e and Te per plasma cell
+    std::vector<uint8_t> blk;           // 1 + the block index of each plasma cell, 0 elsewhere
     std::vector<int> anode;             // biased high-x walls: (face i, j0, j1) triples
     int ncells = 0;                     // plasma cells
 };

@@ -36,32 +36,10 @@
 #include "geometry.hpp"
 #include "learned.hpp"
 #include "mg.hpp"
+#include "ndd_core.hpp"
 
 namespace cathode {
 
-constexpr double K_BOLTZ = 1.380649e-23;   // include/constants.hpp:5
-constexpr double K_E0 = 1.60217662e-19;    // :6
-constexpr double K_EPS0 = 8.8541878128e-12;  // :7
-constexpr double K_LIM_EXP = 1.0e2;        // fluidModule.hpp:3098
-constexpr double K_EPS_VD = 1.0e-3;        // fluidModule.hpp:3099
-constexpr int NB_X = 64, NB_Y = 4;
-
-enum { SP_ELEC = 0, SP_ION = 1, SP_ENERGY = 2 };
-
-// kernel-side view of the problem (passed by value)
-struct NddView {
-    int ni, nj, ld, ld0;
-    double dx, dr, dt;
-    double V_bias, mu_i, Ti, masse, massi;
-    double coefIISEE_ts, Te_emitSEE_eV, ratioEngy_EISEE_rd, dTe_SEE;
-    int n_see;
-    const double* r;  // cell-centre radii r[0..nj+1]
-    const double *see_eb, *see_rd, *see_ts;
-};
-
-struct Mob {
-    double mu_star, inv_par, inv_cross;
-};
 
 // isotropic mobility evaluated through the tensor formulas of
 // fluidModule.hpp:2314-2328 (B = 0)
@@ -75,9 +53,6 @@ static Mob mobility(double mu)
     m.inv_cross = -mu_xr / (mu_para * mu_perp);
     return m;
 }
-
-#define IX(i, j) ((size_t)(i) * (size_t)V.ld + (size_t)(j))
-#define IX0(i, j) ((size_t)(i) * (size_t)V.ld0 + (size_t)(j))
 
 // Scharfetter-Gummel weights: bp = vd/(1 - exp(-z)) (upstream density),
 // bm = vd/(exp(z) - 1) (downstream density), with the exponent clamped at
@@ -167,76 +142,6 @@ __device__ inline double ion_wall_gpn(const NddView& V, double U, double sign)
     return sign * 0.25 * vth * (exp(-u * u) + u * sqrt(M_PI) * (erf(u) + sign));
 }
 
-#define THREAD_IJ(i0, j0)                                        \
-    const int j = (j0) + (int)(blockIdx.x * NB_X + threadIdx.x); \
-    const int i = (i0) + (int)(blockIdx.y * NB_Y + threadIdx.y);
-
-// ================================================================== geometry
-// Face type codes (geometry.hpp) of x face (i, j) and r face (i, j), the
-// plasma cells, the potential ghost rules (-1 or (source << 2) | (Dirichlet
-// << 1) | bias, source a field index) and the ghost writes after the
-// electron and energy routines.  GeoRect: the rectangle 1..ni x 1..nj with
-// the block-4 boundary set (left wall phi = 0, right electrode at V_bias, open
-// top, axis below), from the indices.  GeoMask: the tables of a block
-// geometry (cathode_ndd_set_geometry), pitch ld.
-struct GeoRect {
-    __device__ __forceinline__ int xf(const NddView& V, int i, int j) const
-    {
-        return i == 1 ? GF_WLO : (i == V.ni + 1 ? (GF_WHI | GF_BIAS) : GF_INNER);
-    }
-    __device__ __forceinline__ int rf(const NddView& V, int i, int j) const
-    {
-        return j == 1 ? GF_AXIS : (j == V.nj + 1 ? GF_OPEN : GF_INNER);
-    }
-    // cell (i, j) of the interior 1..ni x 1..nj is plasma
-    __device__ __forceinline__ bool act(const NddView&, int, int) const { return true; }
-    // any cell of the extended grid
-    __device__ __forceinline__ bool act_nb(const NddView& V, int i, int j) const
-    {
-        return i >= 1 && i <= V.ni && j >= 1 && j <= V.nj;
-    }
-    __device__ __forceinline__ int ghost(const NddView& V, int i, int j) const
-    {
-        const bool in_i = i >= 1 && i <= V.ni, in_j = j >= 1 && j <= V.nj;
-        if (i == 0 && in_j) return (int)((IX(1, j) << 2) | 2);
-        if (i == V.ni + 1 && in_j) return (int)((IX(V.ni, j) << 2) | 3);
-        if (j == V.nj + 1 && in_i) return (int)(IX(i, V.nj) << 2);
-        return -1;
-    }
-    __device__ __forceinline__ int te_bits(const NddView& V, int i, int j) const
-    {
-        return (i == 1 ? (GW_XLO | (GW_XLO << 4)) : 0) | (i == V.ni ? (GW_XHI | (GW_XHI << 4)) : 0) |
-               (j == 1 ? (GW_RLO | (GW_RLO << 4)) : 0) | (j == V.nj ? (GW_RHI | (GW_RHI << 4)) : 0);
-    }
-    __device__ __forceinline__ int ne_bits(const NddView&, int, int) const { return 0; }
-};
-
-struct GeoMask {
-    const uint8_t *xc, *rc, *ac, *teb, *neb;
-    const int* gs;
-    __device__ __forceinline__ int xf(const NddView& V, int i, int j) const { return xc[IX(i, j)]; }
-    __device__ __forceinline__ int rf(const NddView& V, int i, int j) const { return rc[IX(i, j)]; }
-    __device__ __forceinline__ bool act(const NddView& V, int i, int j) const { return ac[IX(i, j)] != 0; }
-    __device__ __forceinline__ bool act_nb(const NddView& V, int i, int j) const { return ac[IX(i, j)] != 0; }
-    __device__ __forceinline__ int ghost(const NddView& V, int i, int j) const { return gs[IX(i, j)]; }
-    __device__ __forceinline__ int te_bits(const NddView& V, int i, int j) const { return teb[IX(i, j)]; }
-    __device__ __forceinline__ int ne_bits(const NddView& V, int i, int j) const { return neb[IX(i, j)]; }
-};
-
-// masks of the transverse-flux interpolation: 0 across a wall (and the axis
-// for r faces), as the reference's bLr / bRr / bLx / bRx flags
-__device__ __forceinline__ double tmask_r(int code)
-{
-    const int t = code & GF_TYPE;
-    return (double)(t != GF_WLO && t != GF_WHI && t != GF_AXIS);
-}
-__device__ __forceinline__ double tmask_x(int code)
-{
-    const int t = code & GF_TYPE;
-    return (double)(t != GF_WLO && t != GF_WHI);
-}
-__device__ __forceinline__ bool is_wall(int t) { return t == GF_WLO || t == GF_WHI; }
-
 // ================================================================== Poisson
 struct PoissonArgs {
     const double *phi, *rhoe, *rhoi, *Te, *rhoe_old, *Ex, *Er, *Ex_old, *Er_old, *rhoUix, *rhoUir, *S;
@@ -325,71 +230,6 @@ __device__ __forceinline__ double poisson_flux_r(const G& g, const NddView& V, c
     }
 }
 
-
-// The multigrid's level-0 preparation, done by the assembly kernels: the
-// couplings to cells outside the plasma zeroed (HYPRE's off-grid
-// convention), the row excess s = max(aP - sum a, 0), and the block partials
-// of max|aP| (slot 0) and |b|^2 (slot 1) in the multigrid's 2-D block layout
-// (this grid's); rows of non-plasma cells get the identity, as the
-// multigrid's own preparation gives them.
-// one assembled row: the five coefficients and the right-hand side
-struct Row5 {
-    double P, E, W, N, S, b;
-};
-
-template <class G>
-__device__ inline void prep_row(const G& g, const NddView& V, int i, int j, Row5& R, double* s_out, double& amax,
-                                double& bb)
-{
-    R.E = g.act_nb(V, i + 1, j) ? R.E : 0.0;
-    R.W = g.act_nb(V, i - 1, j) ? R.W : 0.0;
-    R.N = g.act_nb(V, i, j + 1) ? R.N : 0.0;
-    R.S = g.act_nb(V, i, j - 1) ? R.S : 0.0;
-    const double ex = R.P - (R.E + R.W + R.N + R.S);
-    s_out[IX0(i, j)] = ex > 0.0 ? ex : 0.0;
-    amax = fabs(R.P);
-    bb = R.b * R.b;
-}
-
-__device__ inline void store_row(const NddView& V, int i, int j, const Row5& R, double* aP, double* aE, double* aW,
-                                 double* aN, double* aS, double* b)
-{
-    const size_t k = IX0(i, j);
-    aP[k] = R.P;
-    aE[k] = R.E;
-    aW[k] = R.W;
-    aN[k] = R.N;
-    aS[k] = R.S;
-    b[k] = R.b;
-}
-
-// a cell outside the plasma: the identity row, b = 0
-__device__ inline void store_idle_row(const NddView& V, int i, int j, double* aP, double* aE, double* aW, double* aN,
-                                      double* aS, double* b, double* s_out)
-{
-    const size_t k = IX0(i, j);
-    aP[k] = 1.0;
-    aE[k] = 0.0;
-    aW[k] = 0.0;
-    aN[k] = 0.0;
-    aS[k] = 0.0;
-    b[k] = 0.0;
-    if (s_out) s_out[k] = INFINITY;
-}
-
-__device__ inline void prep_partials(double amax, double bb, double* part)
-{
-    __shared__ double lds[8];
-    double v[2] = {amax, bb};
-    const bool mx[2] = {true, false};
-    block_reduce_vec<NB_X * NB_Y, 2>(v, mx, lds);
-    const double t1 = v[0], t2 = v[1];
-    if (lin_tid() == 0) {
-        const int nb = (int)(gridDim.x * gridDim.y), bid = (int)(blockIdx.y * gridDim.x + blockIdx.x);
-        part[bid] = t1;
-        part[nb + bid] = t2;
-    }
-}
 
 // 5-point Poisson correction system and its BC folds (fluidModule.hpp:2627-2824)
 // (the predicted face fluxes are stored after every load of the cell, so the
@@ -871,9 +711,6 @@ __global__ void k_sg_correct(G g, NddView V, SgArgs A, const double* __restrict_
 // calcRes (miniFuncs.hpp:292-375) of three (new, old) pairs over the plasma
 // cells, the inner x faces and the inner r faces (the reference's cell and
 // face boxes); six partial values per block.
-struct ResPairs {
-    const double *n0, *o0, *n1, *o1, *n2, *o2;
-};
 
 template <class G>
 __global__ void k_calcres(G g, NddView V, ResPairs P, int mode, double eps, double* __restrict__ part, int nb)
@@ -1046,12 +883,6 @@ __global__ void k_top_copy(NddView V, double* __restrict__ a)
     if (i <= V.ni) a[IX(i, V.nj + 1)] = a[IX(i, V.nj)];
 }
 
-// faces of the electrode at V_bias: (x face i, rows j0..j1) per segment
-constexpr int ANODE_MAX = 4;
-struct AnodeSegs {
-    int n;
-    int fi[ANODE_MAX], j0[ANODE_MAX], j1[ANODE_MAX];
-};
 
 // right-electrode current (fluidModule.hpp:3942-3965): out = {Ii, Ie}
 __global__ void k_anode(NddView V, AnodeSegs E, const double* __restrict__ rhoUix, const double* __restrict__ rhoUex,
@@ -1216,71 +1047,6 @@ __global__ void k_learn_store(NddView V, const double* __restrict__ x, const dou
     if (target) target[IX(i, j)] = rms[0] > 0.0 ? v / rms[0] : 0.0;
 }
 
-struct NddHandle {
-    cathode_ndd_params p{};
-    NddView V{};
-    Mob me{}, mi{};
-    hipStream_t s = nullptr;
-    size_t n = 0;
-    DevBuf<double> fields;  // 19 state/source arrays
-    DevBuf<double> work;    // scratch arrays
-    DevBuf<double> radii, see;
-    DevBuf<double> part, scal;
-    DevBuf<double> cfl_part;  // check_CFL block maxima (3 per block of the extended grid)
-    DevBuf<double> sgw;     // SG face weights of one PC pass (SgArgs::wgt)
-    double* h_scal = nullptr;
-    double* F[19] = {nullptr};
-    bool has_src[3] = {false, false, false};
-    // work arrays
-    double *phi_old, *Ex_old, *Er_old, *tmp_c, *tmp_x, *tmp_r, *dphi, *s_old, *sx_old, *sr_old, *Te_old, *rhoi_old;
-    double *tmp2_c, *tmp2_x, *tmp2_r;  // the other set of start-of-pass copies (SG passes alternate)
-    MGSolver mg;
-    cathode_ndd_report rep{};
-    int nb = 0;
-    // learned initial guess of the first Poisson correction solve of a step
-    void* cnn = nullptr;              // network (cathode_ndd_set_learned), not owned
-    std::unique_ptr<CnnScratch, void (*)(CnnScratch*)> cnn_sc{nullptr, cnn_scratch_free};  // this handle's activations
-    DevBuf<double> lfeat, lguess, xprev, lpart, lscal;
-    bool have_xprev = false;
-    // learned mode: every solve starts from its own previous-step solution
-    // (slot = equation x PC iteration), scaled by the least-squares alpha
-    // the last WARM_HIST solutions of each solve, newest first; one
-    // allocation each: they trade places with the solver's solution buffer
-    static constexpr int WARM_HIST = 3;
-    DevBuf<double> wbuf[8][WARM_HIST];
-    int nhist[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    int last_its[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // each solve's iteration count one step earlier
-    double* cap_feat = nullptr;       // capture for training (device, caller-owned)
-    double* cap_target = nullptr;
-    int guesses = 0, guesses_used = 0;
-    // the network is used while it lowers the warm-started residual: after
-    // a use that did not (residual ratio >= NET_USEFUL), the next NET_PROBE - 1
-    // first Poisson solves start from the warm start alone, then it is tried again
-    bool net_useful = true, gate_pending = false;
-    int net_wait = 0;
-    // block geometry (cathode_ndd_set_geometry); the rectangle when !masked
-    bool masked = false;
-    GeoMask gm{};
-    DevBuf<uint8_t> geo_u8;  // x codes, r codes, plasma cells, Te and n_e ghost bits
-    DevBuf<int> geo_gs;      // potential ghost rules
-    AnodeSegs anode{};
-};
-
-// launches kernel template K with the handle's geometry policy as the first
-// argument
-#define NDD_GEO_LAUNCH(H, K, grid, block, ...)                                                   \
-    do {                                                                                          \
-        if ((H).masked)                                                                           \
-            hipLaunchKernelGGL(K<GeoMask>, grid, block, 0, (H).s, (H).gm, __VA_ARGS__);           \
-        else                                                                                      \
-            hipLaunchKernelGGL(K<GeoRect>, grid, block, 0, (H).s, GeoRect{}, __VA_ARGS__);        \
-    } while (0)
-
-enum {
-    F_PHI, F_RHOE, F_RHOI, F_TE, F_RHOEPS, F_RHOE_OLD, F_EX, F_RHOUEX, F_NUEX, F_GX, F_RHOUIX,
-    F_ER, F_RHOUER, F_NUER, F_GR, F_RHOUIR, F_S, F_PABS, F_ELOSS
-};
-
 static double** field_slot(cathode_ndd_fields* f, int k)
 {
     double** a[19] = {&f->phi, &f->rhoe, &f->rhoi, &f->Te, &f->rhoeps, &f->rhoe_old, &f->Ex, &f->rhoUex, &f->nUex,
@@ -1289,8 +1055,6 @@ static double** field_slot(cathode_ndd_fields* f, int k)
     return a[k];
 }
 
-static inline dim3 gblk() { return dim3(NB_X, NB_Y); }
-static inline dim3 ggrid(int ni, int nj) { return dim3(cdiv(nj, NB_X), cdiv(ni, NB_Y)); }
 
 // up to 8 whole-plane copies in one launch (blockIdx.y picks the pair)
 constexpr int COPY_MAX = 12;
@@ -1308,7 +1072,7 @@ __global__ void k_copy_planes(CopyList L, size_t n)
 }
 
 // dst_q = src_q for each listed pair, in one launch
-static void d2d(NddHandle& H, std::initializer_list<std::pair<double*, const double*>> pairs)
+void d2d(NddHandle& H, std::initializer_list<std::pair<double*, const double*>> pairs)
 {
     CopyList L{};
     int q = 0;
@@ -1324,7 +1088,7 @@ static void d2d(NddHandle& H, std::initializer_list<std::pair<double*, const dou
     CATHODE_LAUNCH_CHECK();
 }
 
-static double read_dev(NddHandle& H, const double* d, int count = 1)
+double read_dev(NddHandle& H, const double* d, int count)
 {
     fetch_host(H.h_scal, d, count, H.s);
     CATHODE_HIP(hipStreamSynchronize(H.s));
@@ -1371,11 +1135,12 @@ static void calcres3_async(NddHandle& H, const ResPairs& P)
 }
 
 // calcRes of three pairs; returns the max error (host sync); errors also in scal[0..3]
-static double calcres3(NddHandle& H, const ResPairs& P, double* err_out)
+double calcres3(NddHandle& H, const ResPairs& P, double* err_out, int mode)
 {
     const NddView& V = H.V;
-    NDD_GEO_LAUNCH(H, k_calcres, ggrid(V.ni, V.nj), gblk(), V, P, H.p.flag_error, 1e-100, H.part.p, H.nb);
-    hipLaunchKernelGGL(k_calcres_finish, dim3(1), dim3(256), 0, H.s, H.part.p, H.nb, H.p.flag_error, H.scal.p,
+    if (mode < 0) mode = H.p.flag_error;
+    NDD_GEO_LAUNCH(H, k_calcres, ggrid(V.ni, V.nj), gblk(), V, P, mode, 1e-100, H.part.p, H.nb);
+    hipLaunchKernelGGL(k_calcres_finish, dim3(1), dim3(256), 0, H.s, H.part.p, H.nb, mode, H.scal.p,
                        (double*)nullptr);
     CATHODE_LAUNCH_CHECK();
     fetch_host(H.h_scal, H.scal.p, 4, H.s);
@@ -1840,21 +1605,23 @@ int cathode_ndd_set_geometry(void* handle, const cathode_ndd_geometry* g)
         return CATHODE_ERR_ARG;
     }
     const size_t n = T.xc.size();
-    H->geo_u8.alloc(5 * n);
+    H->geo_u8.alloc(6 * n);
     H->geo_gs.alloc(n);
-    std::vector<uint8_t> u8(5 * n);
+    std::vector<uint8_t> u8(6 * n);
     std::memcpy(u8.data(), T.xc.data(), n);
     std::memcpy(u8.data() + n, T.rc.data(), n);
     std::memcpy(u8.data() + 2 * n, T.act.data(), n);
     std::memcpy(u8.data() + 3 * n, T.te.data(), n);
     std::memcpy(u8.data() + 4 * n, T.ne.data(), n);
-    CATHODE_HIP(hipMemcpyAsync(H->geo_u8.p, u8.data(), 5 * n, hipMemcpyHostToDevice, H->s));
+    std::memcpy(u8.data() + 5 * n, T.blk.data(), n);
+    CATHODE_HIP(hipMemcpyAsync(H->geo_u8.p, u8.data(), 6 * n, hipMemcpyHostToDevice, H->s));
     CATHODE_HIP(hipMemcpyAsync(H->geo_gs.p, T.gs.data(), n * sizeof(int), hipMemcpyHostToDevice, H->s));
     H->gm.xc = H->geo_u8.p;
     H->gm.rc = H->geo_u8.p + n;
     H->gm.ac = H->geo_u8.p + 2 * n;
     H->gm.teb = H->geo_u8.p + 3 * n;
     H->gm.neb = H->geo_u8.p + 4 * n;
+    H->gm.bk = H->geo_u8.p + 5 * n;
     H->gm.gs = H->geo_gs.p;
     hipLaunchKernelGGL(k_act_from_table, dim3(cdiv(n0, 256)), dim3(256), 0, H->s, V, (const uint8_t*)H->gm.ac,
                        H->mg.act(), n0);

@@ -263,3 +263,91 @@ def cathode5(seed=5, h=4e-4, energy_on=True, mode=abi.CATHODE_MODE_FP64, dt=1e-1
     st["Te"][...] = Te
     st["rhoeps"][...] = 1.5 * n * BOLTZ * Te
     return p, st, g
+
+
+# ------------------------------------------------------------------ neutrals and metastables
+# neutral-gas constants of the reference (include/params.hpp:35-39) and its
+# input/setup.yaml (width_neutIn, Q_neutIn_mgs)
+TN = 300.0
+RI = 216e-12
+DMN = 1.66e20
+WIDTH_NEUT_IN = 0.0055
+Q_NEUT_IN_MGS = 0.069
+
+
+def inlet_flux(q_mgs=Q_NEUT_IN_MGS, width=WIDTH_NEUT_IN, r4=REF_RS[3], massi=MASSI):
+    """fn_In = (Q [mg/s] 1e-6 / m_i) / (2 pi r4 width_neutIn) (initialModule.hpp:235-237)"""
+    return q_mgs * 1e-6 / massi / (2.0 * np.pi * r4 * width)
+
+
+def neutral_params(which=3, dt_n=1e-7, dt_m=1e-7, mode=abi.CATHODE_MODE_FP64, tol=1e-13, maxITR=30,
+                   flag_error=3, guard_blocks=2):
+    p = abi.NeutralParams()
+    p.which, p.dt_n, p.dt_m = which, dt_n, dt_m
+    p.Tn, p.ri, p.massi, p.DmN = TN, RI, MASSI, DMN
+    p.fn_In = inlet_flux()
+    p.x_in = WIDTH_NEUT_IN + REF_XS[5]
+    p.xL = 0.0
+    p.guard_blocks = guard_blocks
+    p.maxITR, p.tol, p.mode, p.flag_error = maxITR, tol, mode, flag_error
+    return p
+
+
+def segment_faces(g):
+    """(segment index, kind, 'x' or 'r', face i, face j, cell i, cell j) of every boundary face"""
+    out = []
+    for s in range(g.n_seg):
+        q = g.seg[s]
+        for t in range(q.lo, q.hi + 1):
+            if q.kind == abi.SEG_X_LOW:
+                out.append((s, q.kind, "x", q.cell, t, q.cell, t))
+            elif q.kind == abi.SEG_X_HIGH:
+                out.append((s, q.kind, "x", q.cell + 1, t, q.cell, t))
+            elif q.kind in (abi.SEG_R_LOW, abi.SEG_R_AXIS):
+                out.append((s, q.kind, "r", t, q.cell, t, q.cell))
+            else:
+                out.append((s, q.kind, "r", t, q.cell + 1, t, q.cell))
+    return out
+
+
+def neutral_state(g, ni, nj, h, seed=7, noise=True):
+    """A neutral / metastable state on geometry g: n ~ 1e20 and m ~ 1e17
+    (smooth, positive in every cell), n_e = 1e15 + a Gaussian of 1e17,
+    rate coefficients 1e5..4e5 1/s, outward ion wall fluxes of 1e20..3e20
+    and metastable wall fluxes 0.25 m vth on the wall faces, old fluxes of
+    either sign below the open face.  noise: the flux arrays also carry
+    random values on the faces the updates must not read or write."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    shp = (ni + 2, nj + 2)
+    a = {}
+    a["rhon"] = 1e20 * (0.7 + 0.6 * smooth_field(rng, ni, nj, h))
+    a["rhom"] = 1e17 * (0.5 + smooth_field(rng, ni, nj, h))
+    Z = axial(ni, h)[:, None]
+    R = radii(nj, h)[None, :]
+    a["rhoe"] = 1e15 + 1e17 * np.exp(-((Z - 0.6 * ni * h) ** 2 + R ** 2) / (2 * (3e-3) ** 2))
+    for k, scale in (("nu_excStep", 3e5), ("nu_super", 1e5), ("nu_excMeta", 1e5), ("nu_exc", 4e5),
+                     ("nu_ionz", 2e5), ("nu_ionzStep", 2e5)):
+        a[k] = scale * (0.5 + smooth_field(rng, ni, nj, h))
+    names = ["rhoUix_wall", "rhoUir_wall", "rhoUnx", "rhoUnr", "rhoUnx_wall", "rhoUnr_wall", "rhoUmx", "rhoUmr",
+             "rhoUmx_wall", "rhoUmr_wall"]
+    for k in names:
+        a[k] = rng.uniform(-1e19, 1e19, shp) if noise else np.zeros(shp)
+    vth = np.sqrt(8.0 * BOLTZ * TN / (np.pi * MASSI))
+    for s, kind, ax, fi, fj, ci, cj in segment_faces(g):
+        if kind == abi.SEG_R_AXIS:
+            continue
+        if kind == abi.SEG_R_OPEN:
+            # the old flux of the face below the open face, either sign
+            a["rhoUnr"][ci, cj] = rng.uniform(-0.5e21, 2e21)
+            a["rhoUmr"][ci, cj] = rng.uniform(-0.5e19, 2e19)
+            continue
+        sgn = -1.0 if kind in (abi.SEG_X_LOW, abi.SEG_R_LOW) else 1.0
+        gi = "rhoUix_wall" if ax == "x" else "rhoUir_wall"
+        gm = "rhoUmx_wall" if ax == "x" else "rhoUmr_wall"
+        a[gi][fi, fj] = sgn * rng.uniform(1e20, 3e20)
+        a[gm][fi, fj] = sgn * 0.25 * vth * a["rhom"][ci, cj]
+    return {k: np.ascontiguousarray(v, dtype=np.float64) for k, v in a.items()}
+
+
+def copy_arrays(a):
+    return {k: (v.copy() if v is not None else None) for k, v in a.items()}

@@ -345,7 +345,10 @@ int cathode_ndd_create(const cathode_ndd_params* p, void** handle);
  *  - seg: the boundary faces in the reference's order (z0, z1, z2, z4, z3,
  *    z5, x0, x2, x1, x4, x5, x6, the order of its ghost writes,
  *    fluidModule.hpp:2152-2249); a segment names plasma cells (cell, lo..hi)
- *    and the side of their face; flag = 1 puts the wall at V_bias;
+ *    and the side of their face; flag bit 0 (CATHODE_SEG_BIAS) puts an x
+ *    wall at V_bias, flag bit 1 (CATHODE_SEG_INLET) marks a high-r wall as
+ *    the neutral-gas inlet of cathode_ndd_neutral_update (the reference's x1
+ *    wall, whose cells at x <= width_neutIn + x6 feed the gas);
  *  - ne: the electron-density ghosts set after the electron routine
  *    (fluidModule.hpp:3873-3913): flag 0 writes 0, flag 1 copies the cell;
  *  - te: the Te ghosts copied after the energy routine (:4906-4946).
@@ -362,6 +365,8 @@ enum {
     CATHODE_SEG_R_OPEN = 4,  /* open high-r face: grad phi = 0, electron flux = ion flux */
     CATHODE_SEG_R_AXIS = 5   /* symmetry axis below the cells: no flux, no ghost */
 };
+#define CATHODE_SEG_BIAS 1
+#define CATHODE_SEG_INLET 2
 typedef struct cathode_ndd_segment {
     int kind, cell, lo, hi, flag;
 } cathode_ndd_segment;
@@ -388,6 +393,70 @@ int cathode_ndd_set_geometry(void* handle, const cathode_ndd_geometry* g);
  * dielectric), as the microwave profile takes them.  Host only. */
 int cathode_ndd_reference_geometry(const double* xs, const double* rs, double dx, double dr,
                                    cathode_ndd_geometry* g, int* cells6, int* xfaces6, int* rfaces6);
+
+/* ------------------------------------------------------------------ */
+/* neutral and metastable densities (FluidModule::update_rhon,          */
+/* update_rhom, include/fluidModule.hpp:4963-5411, 5416-5792)           */
+/* ------------------------------------------------------------------ */
+/* Each is one implicit r-weighted diffusion solve on the handle's plasma
+ * blocks (the same 5-point system shape and solver_SMG boundary as the NDD
+ * step, zero initial guess), followed by the face-flux update.
+ *  - ground-state neutrals n: D = DnN / n_face with DnN = 2 / (3 sigma)
+ *    sqrt(k Tn / (pi m_i)), sigma = pi (2 ri)^2, n_face the mean of the two
+ *    old cell values; every wall returns the ion and metastable wall fluxes
+ *    (Gamma = -Gamma_i,wall - Gamma_m,wall, explicit); on the inlet wall
+ *    (CATHODE_SEG_INLET) the cells with x <= x_in also take -fn_In (the flux
+ *    update uses x < x_in, as the reference); the open face passes
+ *    max(rL / rR Gamma_old, 0) of the face below; source (nu_excStep +
+ *    nu_super) n_e - (nu_excMeta + nu_exc / 2 + nu_ionz) n_e; after the flux
+ *    update negative n are reset to max(n_old, 0) in every plasma cell.
+ *  - metastables m: D = DmN / n_face of the current neutral density;
+ *    every wall (the inlet one too) loses 0.25 vth m implicitly, vth =
+ *    sqrt(8 k Tn / (pi m_i)); the open face passes rL / rR Gamma_old; source
+ *    n_e (nu_excMeta + nu_exc / 2) - n_e (nu_ionzStep + nu_excStep +
+ *    nu_super); negative m are reset to max(m_old, 0) in the cells of the
+ *    first guard_blocks blocks before the flux update.
+ * The reference calls update_rhon every ndt_n steps and update_rhom every
+ * ndt_m steps (src/main.cpp:238-247), neutrals first: `which` selects them. */
+typedef struct cathode_neutral_params {
+    int which;              /* bit 0: neutrals, bit 1: metastables */
+    double dt_n, dt_m;      /* dt * ndt_n, dt * ndt_m [s] */
+    double Tn, ri, massi;   /* neutral temperature [K], atom radius [m], atom mass [kg] */
+    double DmN;             /* metastable diffusion coefficient x neutral density [1/(m s)] */
+    double fn_In;           /* inlet particle flux density [1/(m^2 s)] */
+    double x_in;            /* width_neutIn + x6 [m] */
+    double xL;              /* low-x domain edge: cell x_i = xL + (i - 1/2) dx */
+    int guard_blocks;       /* blocks whose negative m are reset (the reference: 2) */
+    int maxITR;             /* each solve (maxITR_SOR_phi) */
+    double tol;             /* relative residual (error_cnv_SOR_phi) */
+    int mode;               /* CATHODE_MODE_FP64 or CATHODE_MODE_MIXED */
+    int flag_error;         /* calcRes norm of error_rhon / error_rhom (0..3) */
+} cathode_neutral_params;
+
+/* Host arrays of (ni+2) x (nj+2) doubles, [i][j] with ld = nj + 2 (x-face
+ * arrays indexed by the face's i, r-face arrays by the face's j).  In/out:
+ * rhon, rhom, the fluxes and the wall fluxes of the species updated (the
+ * metastable wall fluxes are also read by the neutral update).  In: rhoe, the
+ * rate coefficients (NULL = 0) and the ion wall fluxes rhoUix_wall (x faces),
+ * rhoUir_wall (r faces).  A species not selected may leave its arrays NULL. */
+typedef struct cathode_neutral_fields {
+    double *rhon, *rhom;
+    const double *rhoe, *nu_excStep, *nu_super, *nu_excMeta, *nu_exc, *nu_ionz, *nu_ionzStep;
+    const double *rhoUix_wall, *rhoUir_wall;
+    double *rhoUnx, *rhoUnr, *rhoUnx_wall, *rhoUnr_wall;
+    double *rhoUmx, *rhoUmr, *rhoUmx_wall, *rhoUmr_wall;
+} cathode_neutral_fields;
+
+typedef struct cathode_neutral_report {
+    cathode_status solve_n, solve_m;  /* the two linear solves */
+    double error_rhon, error_rhom;    /* calcRes of the new against the old density (pm.error_rhon / _rhom) */
+    int neg_rhon, neg_rhom;           /* cells reset from a negative value */
+} cathode_neutral_report;
+
+/* update_rhon and / or update_rhom on the handle's grid and geometry; the
+ * handle's NDD state is not touched.  rep may be NULL. */
+int cathode_ndd_neutral_update(void* handle, const cathode_neutral_params* p, cathode_neutral_fields* f,
+                               cathode_neutral_report* rep);
 
 int cathode_ndd_set_state(void* handle, const cathode_ndd_fields* host);
 int cathode_ndd_get_state(void* handle, cathode_ndd_fields* host);

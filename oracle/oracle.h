@@ -112,7 +112,11 @@ enum {
     ORC_PLANT_MW_PORT_WAVES = 12,     /* forward and reflected waves at the port swapped */
     ORC_PLANT_MW_RFACE_WEIGHT = 13,   /* r-face power volume weighted with the cell radius */
     ORC_PLANT_GEO_FOLD = 14,          /* Poisson: the folds of the high-r walls left out */
-    ORC_PLANT_GEO_WALL_COEF = 15      /* SG: wall faces keep their implicit coefficient */
+    ORC_PLANT_GEO_WALL_COEF = 15,     /* SG: wall faces keep their implicit coefficient */
+    ORC_PLANT_NEUT_SIGMA = 16,        /* neutral D with the cross-section pi ri^2 */
+    ORC_PLANT_NEUT_INLET_SIGN = 17,   /* neutral inlet feed with the wrong sign in the system */
+    ORC_PLANT_META_VTH = 18,          /* metastable wall loss with sqrt(2 k Tn / m) for the mean speed */
+    ORC_PLANT_META_RWALL_Q = 19       /* metastable high-r wall loss without the face weight qR */
 };
 void orc_plant(int which);
 int orc_planted(void);
@@ -171,6 +175,8 @@ int orc_ndd_step(const cathode_ndd_params* p, cathode_ndd_fields* f,
  * (blocks, boundary segments and ghost lists, cathode_hip.h); NULL: the
  * rectangle 1..ni x 1..nj with the block-4 boundary set (the default). */
 void orc_ndd_set_geometry(const cathode_ndd_geometry* g);
+/* the geometry those calls use on an ni x nj grid */
+void orc_ndd_geometry(int ni, int nj, cathode_ndd_geometry* out);
 /* calcRes over the union of nbox boxes {i0, i1, j0, j1}, block by block */
 double orc_calc_res_boxes(const double* vn, const double* vo, int ld, const int* boxes, int nbox, double eps,
                           int mode);
@@ -178,6 +184,13 @@ double orc_calc_res_boxes(const double* vn, const double* vo, int ld, const int*
 int orc_ndd_assemble(const cathode_ndd_params* p, const cathode_ndd_fields* f,
                      int which, double* aP, double* aE, double* aW, double* aN,
                      double* aS, double* b, double* fx, double* fr);
+
+/* FluidModule::update_rhon / update_rhom (fluidModule.hpp:4963-5792, orc_neutral.c)
+ * on the grid of `grid` (ni, nj, dx, dr, rmin) and the geometry set by
+ * orc_ndd_set_geometry.  solver: 0 = exact direct solves, 1 = CPU multigrid
+ * to p->tol.  Same arguments and outputs as cathode_ndd_neutral_update. */
+int orc_neutral_update(const cathode_ndd_params* grid, const cathode_neutral_params* p, cathode_neutral_fields* f,
+                       cathode_neutral_report* rep, int solver);
 
 /* EmfieldModule::update_energy_profile (emfieldModule.hpp:7630-8105): J = sigma E,
  * absorbed power, Poynting fluxes and the port normalisation (orc_mw.c) */

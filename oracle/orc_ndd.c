@@ -143,6 +143,13 @@ static void rect_geo(int ni, int nj, cathode_ndd_geometry* g)
     }
 }
 
+/* the geometry of the following calls on an ni x nj grid (the rectangle when none is set) */
+void orc_ndd_geometry(int ni, int nj, cathode_ndd_geometry* out)
+{
+    if (g_have_geo) *out = g_geo;
+    else rect_geo(ni, nj, out);
+}
+
 typedef struct {
     const cathode_ndd_params* p;
     int ni, nj, ld;
@@ -482,9 +489,9 @@ static void poisson_routine(ctx_t* C, cathode_ndd_fields* f, double* phi_old, do
         const cathode_ndd_segment* q = &G->seg[s];
         for (int t = q->lo; t <= q->hi; t++) {
             if (q->kind == CATHODE_SEG_X_LOW)
-                f->phi[IX(q->cell - 1, t)] = q->flag ? 2.0 * p->V_bias - f->phi[IX(q->cell, t)] : -f->phi[IX(q->cell, t)];
+                f->phi[IX(q->cell - 1, t)] = (q->flag & CATHODE_SEG_BIAS) ? 2.0 * p->V_bias - f->phi[IX(q->cell, t)] : -f->phi[IX(q->cell, t)];
             else if (q->kind == CATHODE_SEG_X_HIGH)
-                f->phi[IX(q->cell + 1, t)] = q->flag ? 2.0 * p->V_bias - f->phi[IX(q->cell, t)] : -f->phi[IX(q->cell, t)];
+                f->phi[IX(q->cell + 1, t)] = (q->flag & CATHODE_SEG_BIAS) ? 2.0 * p->V_bias - f->phi[IX(q->cell, t)] : -f->phi[IX(q->cell, t)];
             else if (q->kind == CATHODE_SEG_R_LOW)
                 f->phi[IX(t, q->cell - 1)] = -f->phi[IX(t, q->cell)];
             else if (q->kind == CATHODE_SEG_R_HIGH)
@@ -973,7 +980,7 @@ int orc_ndd_step(const cathode_ndd_params* pin, cathode_ndd_fields* f, int nstep
             double Ii = 0.0, Ie = 0.0;
             for (int s = 0; s < G->n_seg; s++) {
                 const cathode_ndd_segment* g = &G->seg[s];
-                if (g->kind != CATHODE_SEG_X_HIGH || !g->flag) continue;
+                if (g->kind != CATHODE_SEG_X_HIGH || !(g->flag & CATHODE_SEG_BIAS)) continue;
                 for (int j = g->lo; j <= g->hi; j++) {
                     double di = f->rhoUix[IX(g->cell + 1, j)] * 2.0 * M_PI * C->r[j] * p->dr * (ORC_E0);
                     double de = f->rhoUex[IX(g->cell + 1, j)] * 2.0 * M_PI * C->r[j] * p->dr * (-ORC_E0);

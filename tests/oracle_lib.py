@@ -62,6 +62,9 @@ _SIGS = [
     ("orc_func_Te", ctypes.c_double, [_D, ctypes.c_int, ctypes.c_double, ctypes.c_double]),
     ("orc_helm_assemble", ctypes.c_int, [ctypes.POINTER(HelmParams), _D, _D] + [_D] * 6),
     ("orc_helm_post", ctypes.c_double, [ctypes.POINTER(HelmParams), _D, _D, _D]),
+    ("orc_neutral_update", ctypes.c_int,
+     [ctypes.POINTER(abi.NddParams), ctypes.POINTER(abi.NeutralParams), ctypes.POINTER(abi.NeutralFields),
+      ctypes.POINTER(abi.NeutralReport), ctypes.c_int]),
     ("orc_s_to_abcd", None, [_D, ctypes.c_double, _D]),
     ("orc_abcd_to_s", None, [_D, ctypes.c_double, _D]),
 ]
@@ -248,6 +251,10 @@ ORC_PLANT_MW_PORT_WAVES = 12
 ORC_PLANT_MW_RFACE_WEIGHT = 13
 ORC_PLANT_GEO_FOLD = 14
 ORC_PLANT_GEO_WALL_COEF = 15
+ORC_PLANT_NEUT_SIGMA = 16
+ORC_PLANT_NEUT_INLET_SIGN = 17
+ORC_PLANT_META_VTH = 18
+ORC_PLANT_META_RWALL_Q = 19
 
 
 def check_cfl(params, state):
@@ -270,3 +277,13 @@ def mw_energy_profile(params, arrays):
 def set_geometry(g):
     """the oracle's plasma geometry for the following steps (None: the rectangle)"""
     lib().orc_ndd_set_geometry(ctypes.byref(g) if g is not None else None)
+
+
+def neutral_update(grid, params, arrays, solver=0):
+    """the oracle's update_rhon / update_rhom on a dict of arrays (modified in place)"""
+    f = abi.neutral_fields_struct(arrays)
+    rep = abi.NeutralReport()
+    rc = lib().orc_neutral_update(ctypes.byref(grid), ctypes.byref(params), ctypes.byref(f), ctypes.byref(rep),
+                                  solver)
+    assert rc == 0
+    return rep

@@ -85,7 +85,9 @@ def test_entry_points_refuse_before_init():
     ("cathode_status", "Status"), ("cathode_ndd_params", "NddParams"), ("cathode_ndd_fields", "NddFields"),
     ("cathode_ndd_report", "NddReport"), ("cathode_etheta_params", "EthetaParams"),
     ("cathode_ndd_segment", "NddSegment"), ("cathode_ndd_geometry", "NddGeometry"),
-    ("cathode_mw_params", "MwParams"), ("cathode_mw_fields", "MwFields"), ("cathode_mw_report", "MwReport")])
+    ("cathode_mw_params", "MwParams"), ("cathode_mw_fields", "MwFields"), ("cathode_mw_report", "MwReport"),
+    ("cathode_neutral_params", "NeutralParams"), ("cathode_neutral_fields", "NeutralFields"),
+    ("cathode_neutral_report", "NeutralReport")])
 def test_ctypes_struct_layout_matches_header(cname, pyname, tmp_path):
     """Every ctypes mirror has the C struct's size and field offsets (gcc on the header)."""
     py = getattr(abi, pyname)
