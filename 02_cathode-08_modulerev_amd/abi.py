@@ -70,7 +70,7 @@ class NddFields(ctypes.Structure):
 NDD_MAX_BLOCKS = 8
 NDD_MAX_SEGS = 24
 SEG_X_LOW, SEG_X_HIGH, SEG_R_LOW, SEG_R_HIGH, SEG_R_OPEN, SEG_R_AXIS = range(6)
-SEG_BIAS, SEG_INLET = 1, 2  # segment flag bits
+SEG_BIAS, SEG_INLET, SEG_UIX_BELOW = 1, 2, 4  # segment flag bits
 
 
 class NddSegment(ctypes.Structure):
@@ -134,6 +134,36 @@ def neutral_fields_struct(arrays):
     """NeutralFields pointing into a dict of numpy arrays (missing keys -> NULL)."""
     f = NeutralFields()
     for n in NEUTRAL_FIELD_NAMES:
+        setattr(f, n, dptr(arrays.get(n)))
+    return f
+
+
+class IonParams(ctypes.Structure):
+    _fields_ = [
+        ("dt_i", ctypes.c_double), ("Ti", ctypes.c_double), ("massi", ctypes.c_double), ("V_bias", ctypes.c_double),
+        ("Te_rep_eV", ctypes.c_double), ("rmax", ctypes.c_double), ("maxITR_Ui", ctypes.c_int),
+        ("tol_Ui", ctypes.c_double), ("maxITR_rhoi", ctypes.c_int), ("tol_rhoi", ctypes.c_double),
+        ("mode", ctypes.c_int),
+    ]
+
+
+ION_FIELD_NAMES = ["rhoi", "Uix", "Uir", "Uip", "rhoUix", "rhoUir", "rhoUix_wall", "rhoUir_wall", "phi", "Ex", "Er",
+                   "rate_ionize", "nui_m", "Bx", "Br"]
+
+
+class IonFields(ctypes.Structure):
+    _fields_ = [(n, c_double_p) for n in ION_FIELD_NAMES]
+
+
+class IonReport(ctypes.Structure):
+    _fields_ = [("solve_Uix", Status), ("solve_Uir", Status), ("solve_Uip", Status), ("solve_rhoi", Status),
+                ("neg_rhoi", ctypes.c_int)]
+
+
+def ion_fields_struct(arrays):
+    """IonFields pointing into a dict of numpy arrays (missing keys -> NULL)."""
+    f = IonFields()
+    for n in ION_FIELD_NAMES:
         setattr(f, n, dptr(arrays.get(n)))
     return f
 
@@ -278,6 +308,8 @@ SIGNATURES = [
      [c_double_p, c_double_p, ctypes.c_double, ctypes.c_double, ctypes.POINTER(NddGeometry), c_int_p, c_int_p,
       c_int_p]),
     ("cathode_ndd_capture", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    ("cathode_ndd_ion_update", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.POINTER(IonParams), ctypes.POINTER(IonFields), ctypes.POINTER(IonReport)]),
     ("cathode_ndd_neutral_update", ctypes.c_int,
      [ctypes.c_void_p, ctypes.POINTER(NeutralParams), ctypes.POINTER(NeutralFields),
       ctypes.POINTER(NeutralReport)]),

@@ -154,10 +154,12 @@ bool build_geo_tables(int ni, int nj, const cathode_ndd_geometry& g, GeoTables& 
         const bool xk = q.kind == CATHODE_SEG_X_LOW || q.kind == CATHODE_SEG_X_HIGH;
         if (xk ? (q.cell < 1 || q.cell > ni || q.lo < 1 || q.hi > nj) : (q.cell < 1 || q.cell > nj || q.lo < 1 || q.hi > ni))
             return fail("segment " + std::to_string(s) + " outside the grid");
-        if ((q.flag & ~(CATHODE_SEG_BIAS | CATHODE_SEG_INLET)) || ((q.flag & CATHODE_SEG_BIAS) && !xk) ||
-            ((q.flag & CATHODE_SEG_INLET) && q.kind != CATHODE_SEG_R_HIGH))
+        if ((q.flag & ~(CATHODE_SEG_BIAS | CATHODE_SEG_INLET | CATHODE_SEG_UIX_BELOW)) ||
+            ((q.flag & CATHODE_SEG_BIAS) && !xk) ||
+            ((q.flag & (CATHODE_SEG_INLET | CATHODE_SEG_UIX_BELOW)) && q.kind != CATHODE_SEG_R_HIGH))
             return fail("segment " + std::to_string(s) +
-                        ": flag bits are CATHODE_SEG_BIAS on x walls and CATHODE_SEG_INLET on high-r walls");
+                        ": flag bits are CATHODE_SEG_BIAS on x walls, CATHODE_SEG_INLET and "
+                        "CATHODE_SEG_UIX_BELOW on high-r walls");
         uint8_t code;
         switch (q.kind) {
         case CATHODE_SEG_X_LOW:
@@ -299,7 +301,7 @@ extern "C" int cathode_ndd_reference_geometry(const double* xs, const double* rs
         {CATHODE_SEG_X_HIGH, C[4][I1], C[4][J0], C[4][J1], CATHODE_SEG_BIAS},  // z5, the electrode at V_bias
         {CATHODE_SEG_R_LOW, C[0][J0], C[0][I0], C[0][I1], 0},       // x0
         {CATHODE_SEG_R_LOW, C[2][J0], C[2][I0], C[2][I1], 0},       // x2
-        {CATHODE_SEG_R_HIGH, C[0][J1], C[0][I0], C[1][I1], CATHODE_SEG_INLET},  // x1, holds the gas inlet
+        {CATHODE_SEG_R_HIGH, C[0][J1], C[0][I0], C[1][I1], CATHODE_SEG_INLET | CATHODE_SEG_UIX_BELOW},  // x1
         {CATHODE_SEG_R_HIGH, C[3][J1], C[1][I1] + 1, C[3][I1], 0},  // x4
         {CATHODE_SEG_R_OPEN, C[4][J1], C[4][I0], C[4][I1], 0},      // x5
         {CATHODE_SEG_R_AXIS, C[3][J0], C[3][I0], C[4][I1], 0},      // x6, the axis row of blocks 3 and 4

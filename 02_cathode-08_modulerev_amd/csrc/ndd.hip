@@ -1528,6 +1528,7 @@ int cathode_ndd_create(const cathode_ndd_params* p, void** handle)
         V.see_eb = H->see.p; V.see_rd = H->see.p + CATHODE_SEE_TABLE_MAX; V.see_ts = H->see.p + 2 * CATHODE_SEE_TABLE_MAX;
         H->me = mobility(p->mu_e);
         H->mi = mobility(p->mu_i);
+        H->geo_src = rect_geometry(ni, nj);
         H->mg.configure(ni, nj, H->s);
         V.ld0 = H->mg.ld0();
         const int n0 = (ni + 2) * V.ld0;
@@ -1583,8 +1584,10 @@ int cathode_ndd_set_geometry(void* handle, const cathode_ndd_geometry* g)
     }
     NddView& V = H->V;
     const int n0 = (V.ni + 2) * V.ld0;
+    H->geo_version++;
     if (!g) {
         H->masked = false;
+        H->geo_src = rect_geometry(V.ni, V.nj);
         hipLaunchKernelGGL(k_fill_one, dim3(cdiv(n0, 256)), dim3(256), 0, H->s, n0, H->mg.act(), V.ni, V.nj, V.ld0);
         CATHODE_LAUNCH_CHECK();
         H->anode.n = 1;
@@ -1633,6 +1636,7 @@ int cathode_ndd_set_geometry(void* handle, const cathode_ndd_geometry* g)
         H->anode.j1[q] = T.anode[3 * q + 2];
     }
     H->masked = true;
+    H->geo_src = *g;
     CATHODE_HIP(hipStreamSynchronize(H->s));  // the host tables go out of scope
     return CATHODE_OK;
     CATHODE_GUARD_END

@@ -242,9 +242,21 @@ struct NddHandle {
     DevBuf<uint8_t> geo_u8;  // x codes, r codes, plasma cells, Te and n_e ghost bits, block indices
     DevBuf<int> geo_gs;      // potential ghost rules
     AnodeSegs anode{};
+    // the geometry in segment form (the rectangle's when none is set) and a
+    // counter that changes with every cathode_ndd_set_geometry
+    cathode_ndd_geometry geo_src{};
+    int geo_version = 0;
     // neutral / metastable update (neutral.hip): its planes, allocated on first use
     DevBuf<double> neut;
     DevBuf<int> neut_cnt;
+    // ion block (ion.hip): planes, the per-face operation tables and ghost
+    // lists of the current geometry (built for geo_version ion_plan_version)
+    DevBuf<double> ionp;
+    DevBuf<uint8_t> ion_ops;
+    DevBuf<int> ion_ghosts;
+    int ion_plan_version = -1;
+    int ion_ng[3] = {0, 0, 0};
+    DevBuf<int> ion_cnt;
 };
 
 // launches kernel template K with the handle's geometry policy as the first

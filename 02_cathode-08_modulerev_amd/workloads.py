@@ -351,3 +351,53 @@ def neutral_state(g, ni, nj, h, seed=7, noise=True):
 
 def copy_arrays(a):
     return {k: (v.copy() if v is not None else None) for k, v in a.items()}
+
+
+# ------------------------------------------------------------------ ions
+TE_REP_EV = 0.1  # input/setup.yaml Te_rep_eV
+
+
+def ion_params(dt_i=1e-10, mode=abi.CATHODE_MODE_FP64, V_bias=30.0, tol_Ui=1e-13, tol_rhoi=1e-13, maxITR=30):
+    p = abi.IonParams()
+    p.dt_i, p.Ti, p.massi, p.V_bias = dt_i, TI, MASSI, V_bias
+    p.Te_rep_eV, p.rmax = TE_REP_EV, REF_RS[5]
+    p.maxITR_Ui, p.tol_Ui, p.maxITR_rhoi, p.tol_rhoi = maxITR, tol_Ui, maxITR, tol_rhoi
+    p.mode = mode
+    return p
+
+
+def ion_state(g, ni, nj, h, seed=17, V_bias=30.0, noise=True):
+    """An ion state on geometry g: n_i = 1e15 + two Gaussians of 1e16..1e17
+    in the plasma (0 outside), a potential rising smoothly from 0 at the
+    left to V_bias at the right with E = -grad phi on the faces, velocities
+    of a few hundred m/s, S = 1e22..1e23 1/(m^3 s), nui_m = 1e5..1e6 1/s,
+    B = 10..50 mT.  noise: the flux arrays carry random values on the faces
+    the update must not read or write."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    shp = (ni + 2, nj + 2)
+    m = plasma_mask(g, ni, nj)
+    Z = np.broadcast_to(axial(ni, h)[:, None], shp)
+    R = np.broadcast_to(radii(nj, h)[None, :], shp)
+    n = np.full(shp, 1e15)
+    pts = np.argwhere(m)
+    for _ in range(2):
+        ci, cj = pts[rng.integers(len(pts))]
+        n = n + 10.0 ** rng.uniform(16.0, 17.0) * np.exp(-((Z - Z[ci, cj]) ** 2 + (R - R[ci, cj]) ** 2) / (2 * 1.5e-3 ** 2))
+    a = {"rhoi": np.where(m, n, 0.0)}
+    L = ni * h
+    phi = V_bias * (Z / L) ** 2 + 2.0 * np.sin(np.pi * R / (nj * h)) * smooth_field(rng, ni, nj, h)
+    a["phi"] = phi
+    Ex = np.zeros(shp)
+    Ex[1:, :] = -(phi[1:, :] - phi[:-1, :]) / h
+    Er = np.zeros(shp)
+    Er[:, 1:] = -(phi[:, 1:] - phi[:, :-1]) / h
+    a["Ex"], a["Er"] = Ex, Er
+    for k in ("Uix", "Uir", "Uip"):
+        a[k] = rng.uniform(-300.0, 300.0, shp)
+    a["rate_ionize"] = 1e22 * (1.0 + 9.0 * smooth_field(rng, ni, nj, h))
+    a["nui_m"] = 1e5 * (1.0 + 9.0 * smooth_field(rng, ni, nj, h))
+    a["Bx"] = 0.01 + 0.04 * smooth_field(rng, ni, nj, h)
+    a["Br"] = 0.01 + 0.04 * smooth_field(rng, ni, nj, h)
+    for k in ("rhoUix", "rhoUir", "rhoUix_wall", "rhoUir_wall"):
+        a[k] = rng.uniform(-1e19, 1e19, shp) if noise else np.zeros(shp)
+    return {k: np.ascontiguousarray(v, dtype=np.float64) for k, v in a.items()}

@@ -348,7 +348,9 @@ int cathode_ndd_create(const cathode_ndd_params* p, void** handle);
  *    and the side of their face; flag bit 0 (CATHODE_SEG_BIAS) puts an x
  *    wall at V_bias, flag bit 1 (CATHODE_SEG_INLET) marks a high-r wall as
  *    the neutral-gas inlet of cathode_ndd_neutral_update (the reference's x1
- *    wall, whose cells at x <= width_neutIn + x6 feed the gas);
+ *    wall, whose cells at x <= width_neutIn + x6 feed the gas), flag bit 2
+ *    (CATHODE_SEG_UIX_BELOW) moves a high-r wall's tangential Uix reflect
+ *    of cathode_ndd_ion_update to the row below its cells (that x1 wall);
  *  - ne: the electron-density ghosts set after the electron routine
  *    (fluidModule.hpp:3873-3913): flag 0 writes 0, flag 1 copies the cell;
  *  - te: the Te ghosts copied after the energy routine (:4906-4946).
@@ -367,6 +369,7 @@ enum {
 };
 #define CATHODE_SEG_BIAS 1
 #define CATHODE_SEG_INLET 2
+#define CATHODE_SEG_UIX_BELOW 4
 typedef struct cathode_ndd_segment {
     int kind, cell, lo, hi, flag;
 } cathode_ndd_segment;
@@ -457,6 +460,67 @@ typedef struct cathode_neutral_report {
  * handle's NDD state is not touched.  rep may be NULL. */
 int cathode_ndd_neutral_update(void* handle, const cathode_neutral_params* p, cathode_neutral_fields* f,
                                cathode_neutral_report* rep);
+
+/* ------------------------------------------------------------------ */
+/* the ion block of the reference's time step (src/main.cpp:194-213):  */
+/* FluidModule::solve_Uix_tmp, solve_Uir_tmp, solve_Uip_tmp            */
+/* (include/fluidModule.hpp:123-1398), solve_rhoi_constTe (:1405-1755) */
+/* and correct_Ui_constTe (:1763-2072)                                 */
+/* ------------------------------------------------------------------ */
+/* Three implicit upwind momentum solves give the temporary ion velocity:
+ * Uix on the inner x faces, Uir on the inner r faces, Uip in the cells
+ * (ionization drag S / n_i, ion-neutral collisions nui_m, e E, the Lorentz
+ * force of (Bx, Br), the ion pressure k Ti grad n_i / (m_i n_i) and the
+ * centrifugal terms).  At a wall the normal velocity is the energy-
+ * conserving sign sqrt(U_old^2 + max(2 e dphi / m_i, 0)) of the adjacent
+ * unknown (dphi the potential at that face, minus V_bias at a biased wall),
+ * the tangential velocities are 0 (Uip: continuity), the open face passes
+ * rL / rR max(Uir_old, 0).  The density then solves an upwind continuity
+ * with those velocities and a numerical diffusion D = k (Ti + Te_rep) /
+ * (m_i (nui_m + nu_star)), nu_star = sqrt(k Te_rep / m_i) / rmax, and
+ * the drifting-Maxwellian wall flux (implicit), zero gradient at the open
+ * face; the wall fluxes rhoUix_wall / rhoUir_wall are recorded, negative
+ * densities reset to max(old, 0).  Finally the velocities and the fluxes
+ * rhoUix / rhoUir of the inner faces (and the open face) take the
+ * correction -D grad(n_new - n_old).
+ * The boundary operations follow the geometry's segments; with the
+ * CATHODE_SEG_UIX_BELOW flag a high-r wall's tangential Uix reflect writes
+ * the row below its cells instead of the ghost row above, as the reference
+ * does for its x1 wall (fluidModule.hpp:520-525).  Every wall needs an
+ * inner face next to it (blocks at least two cells across). */
+typedef struct cathode_ion_params {
+    double dt_i;            /* dt * ndt_i [s] */
+    double Ti, massi;       /* ion temperature [K], ion mass [kg] */
+    double V_bias;          /* potential of the biased walls [V] */
+    double Te_rep_eV;       /* representative electron temperature of the numerical diffusion [eV] */
+    double rmax;            /* domain height [m] */
+    int maxITR_Ui;          /* the velocity solves (maxITR_SOR_Ui, error_cnv_SOR_Ui) */
+    double tol_Ui;
+    int maxITR_rhoi;        /* the density solve (maxITR_SOR_rhoi, error_cnv_SOR_rhoi) */
+    double tol_rhoi;
+    int mode;               /* CATHODE_MODE_FP64 or CATHODE_MODE_MIXED */
+} cathode_ion_params;
+
+/* Host arrays of (ni+2) x (nj+2) doubles, [i][j] with ld = nj + 2.  In/out:
+ * rhoi, Uix (x faces), Uir (r faces), Uip; out: rhoUix, rhoUir (inner and
+ * open faces), rhoUix_wall, rhoUir_wall (wall faces; other entries keep
+ * their values).  In: phi, Ex (x faces), Er (r faces), rate_ionize, nui_m,
+ * Bx, Br (cells; NULL = 0). */
+typedef struct cathode_ion_fields {
+    double *rhoi, *Uix, *Uir, *Uip;
+    double *rhoUix, *rhoUir, *rhoUix_wall, *rhoUir_wall;
+    const double *phi, *Ex, *Er, *rate_ionize, *nui_m, *Bx, *Br;
+} cathode_ion_fields;
+
+typedef struct cathode_ion_report {
+    cathode_status solve_Uix, solve_Uir, solve_Uip, solve_rhoi;
+    int neg_rhoi;                      /* cells reset from a negative density */
+} cathode_ion_report;
+
+/* the ion block on the handle's grid and geometry; the handle's NDD state is
+ * not touched.  rep may be NULL. */
+int cathode_ndd_ion_update(void* handle, const cathode_ion_params* p, cathode_ion_fields* f,
+                           cathode_ion_report* rep);
 
 int cathode_ndd_set_state(void* handle, const cathode_ndd_fields* host);
 int cathode_ndd_get_state(void* handle, cathode_ndd_fields* host);

@@ -116,7 +116,9 @@ enum {
     ORC_PLANT_NEUT_SIGMA = 16,        /* neutral D with the cross-section pi ri^2 */
     ORC_PLANT_NEUT_INLET_SIGN = 17,   /* neutral inlet feed with the wrong sign in the system */
     ORC_PLANT_META_VTH = 18,          /* metastable wall loss with sqrt(2 k Tn / m) for the mean speed */
-    ORC_PLANT_META_RWALL_Q = 19       /* metastable high-r wall loss without the face weight qR */
+    ORC_PLANT_META_RWALL_Q = 19,      /* metastable high-r wall loss without the face weight qR */
+    ORC_PLANT_ION_LORENTZ_SIGN = 20,  /* azimuthal ion momentum with the Lorentz force reversed */
+    ORC_PLANT_ION_CORR_TE = 21        /* velocity correction's diffusion without Te_rep */
 };
 void orc_plant(int which);
 int orc_planted(void);
@@ -191,6 +193,14 @@ int orc_ndd_assemble(const cathode_ndd_params* p, const cathode_ndd_fields* f,
  * to p->tol.  Same arguments and outputs as cathode_ndd_neutral_update. */
 int orc_neutral_update(const cathode_ndd_params* grid, const cathode_neutral_params* p, cathode_neutral_fields* f,
                        cathode_neutral_report* rep, int solver);
+
+/* the ion block (fluidModule.hpp:123-2072, orc_ion.c), literally over the
+ * five plasma blocks of cathode_ndd_reference_geometry set with
+ * orc_ndd_set_geometry (returns -1 for another geometry).  solver: 0 =
+ * exact direct solves, 1 = CPU multigrid.  Same arguments and outputs as
+ * cathode_ndd_ion_update. */
+int orc_ion_update(const cathode_ndd_params* grid, const cathode_ion_params* p, cathode_ion_fields* f,
+                   cathode_ion_report* rep, int solver);
 
 /* EmfieldModule::update_energy_profile (emfieldModule.hpp:7630-8105): J = sigma E,
  * absorbed power, Poynting fluxes and the port normalisation (orc_mw.c) */

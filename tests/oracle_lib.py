@@ -65,6 +65,9 @@ _SIGS = [
     ("orc_neutral_update", ctypes.c_int,
      [ctypes.POINTER(abi.NddParams), ctypes.POINTER(abi.NeutralParams), ctypes.POINTER(abi.NeutralFields),
       ctypes.POINTER(abi.NeutralReport), ctypes.c_int]),
+    ("orc_ion_update", ctypes.c_int,
+     [ctypes.POINTER(abi.NddParams), ctypes.POINTER(abi.IonParams), ctypes.POINTER(abi.IonFields),
+      ctypes.POINTER(abi.IonReport), ctypes.c_int]),
     ("orc_s_to_abcd", None, [_D, ctypes.c_double, _D]),
     ("orc_abcd_to_s", None, [_D, ctypes.c_double, _D]),
 ]
@@ -255,6 +258,8 @@ ORC_PLANT_NEUT_SIGMA = 16
 ORC_PLANT_NEUT_INLET_SIGN = 17
 ORC_PLANT_META_VTH = 18
 ORC_PLANT_META_RWALL_Q = 19
+ORC_PLANT_ION_LORENTZ_SIGN = 20
+ORC_PLANT_ION_CORR_TE = 21
 
 
 def check_cfl(params, state):
@@ -286,4 +291,14 @@ def neutral_update(grid, params, arrays, solver=0):
     rc = lib().orc_neutral_update(ctypes.byref(grid), ctypes.byref(params), ctypes.byref(f), ctypes.byref(rep),
                                   solver)
     assert rc == 0
+    return rep
+
+
+def ion_update(grid, params, arrays, solver=0):
+    """the oracle's ion block on a dict of arrays (modified in place); needs the
+    five-block geometry set with set_geometry"""
+    f = abi.ion_fields_struct(arrays)
+    rep = abi.IonReport()
+    rc = lib().orc_ion_update(ctypes.byref(grid), ctypes.byref(params), ctypes.byref(f), ctypes.byref(rep), solver)
+    assert rc == 0, "orc_ion_update needs the five-block geometry"
     return rep
