@@ -310,6 +310,11 @@ SIGNATURES = [
     ("cathode_ndd_capture", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     ("cathode_ndd_ion_update", ctypes.c_int,
      [ctypes.c_void_p, ctypes.POINTER(IonParams), ctypes.POINTER(IonFields), ctypes.POINTER(IonReport)]),
+    ("cathode_ndd_ion_update_dev", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.POINTER(IonParams), ctypes.POINTER(IonFields), ctypes.POINTER(IonReport)]),
+    ("cathode_ndd_neutral_update_dev", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.POINTER(NeutralParams), ctypes.POINTER(NeutralFields),
+      ctypes.POINTER(NeutralReport)]),
     ("cathode_ndd_neutral_update", ctypes.c_int,
      [ctypes.c_void_p, ctypes.POINTER(NeutralParams), ctypes.POINTER(NeutralFields),
       ctypes.POINTER(NeutralReport)]),
@@ -386,6 +391,26 @@ class DeviceArray:
         if self.ptr.value:
             self.lib.cathode_dev_free(self.ptr)
             self.ptr = ctypes.c_void_p()
+
+
+def device_fields(lib, arrays, names, struct_cls):
+    """The present arrays of a dict copied into one device allocation: (a
+    struct_cls of device pointers, missing names NULL; the DeviceArray; the
+    names in it).  download_fields brings them back."""
+    present = [n for n in names if arrays.get(n) is not None]
+    shape = arrays[present[0]].shape
+    D = DeviceArray(lib, (len(present),) + tuple(shape))
+    f = struct_cls()
+    for n in names:
+        setattr(f, n, None)
+    for k, n in enumerate(present):
+        D.upload(arrays[n], k)
+        setattr(f, n, ctypes.cast(D.at(k), c_double_p))
+    return f, D, present
+
+
+def download_fields(D, present):
+    return {n: D.download(k) for k, n in enumerate(present)}
 
 
 def check(rc, lib=None):

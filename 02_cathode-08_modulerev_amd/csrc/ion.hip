@@ -586,8 +586,10 @@ bool build_ion_plan(int ni, int nj, const cathode_ndd_geometry& g, IonPlan& P, s
 
 using namespace cathode;
 
-extern "C" int cathode_ndd_ion_update(void* handle, const cathode_ion_params* p, cathode_ion_fields* f,
-                                      cathode_ion_report* rep)
+// the block on host arrays (dev = false: staged through the handle's planes)
+// or on the caller's device arrays (dev = true)
+static int ion_entry(void* handle, const cathode_ion_params* p, cathode_ion_fields* f, cathode_ion_report* rep,
+                     bool dev)
 {
     try {
         NddHandle* H = (NddHandle*)handle;
@@ -647,6 +649,10 @@ extern "C" int cathode_ndd_ion_update(void* handle, const cathode_ion_params* p,
         CATHODE_HIP(hipMemsetAsync(H->ion_cnt.p, 0, sizeof(int), H->s));
         auto up = [&](int q, const double* src) -> const double* {
             if (!src) return Q[IP_ZERO];
+            if (dev) {
+                Q[q] = const_cast<double*>(src);
+                return src;
+            }
             CATHODE_HIP(hipMemcpyAsync(Q[q], src, n * sizeof(double), hipMemcpyHostToDevice, H->s));
             return Q[q];
         };
@@ -735,7 +741,7 @@ extern "C" int cathode_ndd_ion_update(void* handle, const cathode_ion_params* p,
         NDD_GEO_LAUNCH(*H, k_ion_act, dim3(cdiv(n0, 256)), dim3(256), V, H->mg.act(), n0, 2);
         CATHODE_LAUNCH_CHECK();
         auto down = [&](double* dst, int q) {
-            CATHODE_HIP(hipMemcpyAsync(dst, Q[q], n * sizeof(double), hipMemcpyDeviceToHost, H->s));
+            if (!dev) CATHODE_HIP(hipMemcpyAsync(dst, Q[q], n * sizeof(double), hipMemcpyDeviceToHost, H->s));
         };
         down(f->rhoi, IP_RHOI);
         down(f->Uix, IP_UIX);
@@ -761,4 +767,16 @@ extern "C" int cathode_ndd_ion_update(void* handle, const cathode_ion_params* p,
         }
         return CATHODE_OK;
     CATHODE_CATCH_END
+}
+
+extern "C" int cathode_ndd_ion_update(void* handle, const cathode_ion_params* p, cathode_ion_fields* f,
+                                      cathode_ion_report* rep)
+{
+    return ion_entry(handle, p, f, rep, false);
+}
+
+extern "C" int cathode_ndd_ion_update_dev(void* handle, const cathode_ion_params* p, cathode_ion_fields* f,
+                                          cathode_ion_report* rep)
+{
+    return ion_entry(handle, p, f, rep, true);
 }

@@ -222,8 +222,10 @@ bool bad_number(double v) { return !std::isfinite(v); }
 
 }  // namespace
 
-extern "C" int cathode_ndd_neutral_update(void* handle, const cathode_neutral_params* p, cathode_neutral_fields* f,
-                                          cathode_neutral_report* rep)
+// the update on host arrays (dev = false: staged through the handle's
+// planes) or on the caller's device arrays (dev = true)
+static int neutral_entry(void* handle, const cathode_neutral_params* p, cathode_neutral_fields* f,
+                         cathode_neutral_report* rep, bool dev)
 {
     try {
         NddHandle* H = (NddHandle*)handle;
@@ -255,10 +257,15 @@ extern "C" int cathode_ndd_neutral_update(void* handle, const cathode_neutral_pa
         for (int q = 0; q < NP_COUNT; q++) P[q] = H->neut.p + q * n;
         CATHODE_HIP(hipMemsetAsync(P[NP_ZERO], 0, n * sizeof(double), H->s));
         CATHODE_HIP(hipMemsetAsync(H->neut_cnt.p, 0, 2 * sizeof(int), H->s));
-        // host -> device; an absent input reads as the zero plane
+        // host -> device, or the caller's device arrays; an absent input
+        // reads as the zero plane
         const double* zero = P[NP_ZERO];
         auto up = [&](int q, const double* src) -> const double* {
             if (!src) return zero;
+            if (dev) {
+                P[q] = const_cast<double*>(src);
+                return src;
+            }
             CATHODE_HIP(hipMemcpyAsync(P[q], src, n * sizeof(double), hipMemcpyHostToDevice, H->s));
             return P[q];
         };
@@ -338,7 +345,7 @@ extern "C" int cathode_ndd_neutral_update(void* handle, const cathode_neutral_pa
         }
         // device -> host
         auto down = [&](double* dst, int q) {
-            CATHODE_HIP(hipMemcpyAsync(dst, P[q], n * sizeof(double), hipMemcpyDeviceToHost, H->s));
+            if (!dev) CATHODE_HIP(hipMemcpyAsync(dst, P[q], n * sizeof(double), hipMemcpyDeviceToHost, H->s));
         };
         if (do_n) {
             down(f->rhon, NP_N);
@@ -370,4 +377,16 @@ extern "C" int cathode_ndd_neutral_update(void* handle, const cathode_neutral_pa
         }
         return CATHODE_OK;
     CATHODE_CATCH_END
+}
+
+extern "C" int cathode_ndd_neutral_update(void* handle, const cathode_neutral_params* p, cathode_neutral_fields* f,
+                                          cathode_neutral_report* rep)
+{
+    return neutral_entry(handle, p, f, rep, false);
+}
+
+extern "C" int cathode_ndd_neutral_update_dev(void* handle, const cathode_neutral_params* p,
+                                              cathode_neutral_fields* f, cathode_neutral_report* rep)
+{
+    return neutral_entry(handle, p, f, rep, true);
 }

@@ -178,6 +178,54 @@ def run_sweep(lib, steps, workers):
             "collector_current_A": I_batch, "max_rel_dev_vs_one_at_a_time": dev, "nonconverged": nc}
 
 
+ROWS_H = 2e-4  # the reference's own grid: the five-block cathode at 0.2 mm, 194 x 70 cells
+
+
+def rows_workload(lib):
+    ni, nj, g, *_ = W.reference_geometry(ROWS_H, lib)
+    grid = W.default_params(ni, nj, ROWS_H, ROWS_H)
+    ion = W.ion_state(g, ni, nj, ROWS_H, noise=False)
+    neu = W.neutral_state(g, ni, nj, ROWS_H, noise=False)
+    return ni, nj, g, grid, ion, neu
+
+
+def run_rows(lib, reps=20):
+    """The ion block (solve_Uix/Uir/Uip_tmp, solve_rhoi_constTe,
+    correct_Ui_constTe: 4 solves) and the neutral + metastable update
+    (update_rhon, update_rhom: 2 solves) on the reference's own cathode grid
+    with device-resident arrays (cathode_ndd_*_update_dev): wall time per
+    call after 2 untimed calls, each solve to relative residual 1e-10."""
+    ni, nj, g, grid, ion, neu = rows_workload(lib)
+    hd = ctypes.c_void_p()
+    abi.check(lib.cathode_ndd_create(ctypes.byref(grid), ctypes.byref(hd)), lib)
+    out = {}
+    try:
+        abi.check(lib.cathode_ndd_set_geometry(hd, ctypes.byref(g)), lib)
+        cases = (("ion_block", ion, abi.ION_FIELD_NAMES, abi.IonFields, W.ion_params(tol_Ui=1e-10, tol_rhoi=1e-10),
+                  lib.cathode_ndd_ion_update_dev, abi.IonReport,
+                  ("solve_Uix", "solve_Uir", "solve_Uip", "solve_rhoi")),
+                 ("neutral_update", neu, abi.NEUTRAL_FIELD_NAMES, abi.NeutralFields, W.neutral_params(tol=1e-10),
+                  lib.cathode_ndd_neutral_update_dev, abi.NeutralReport, ("solve_n", "solve_m")))
+        for name, state, names, cls, prm, fn, rcls, solves in cases:
+            f, D, present = abi.device_fields(lib, state, names, cls)
+            rep = rcls()
+            for _ in range(2):
+                abi.check(fn(hd, ctypes.byref(prm), ctypes.byref(f), ctypes.byref(rep)), lib)
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                abi.check(fn(hd, ctypes.byref(prm), ctypes.byref(f), ctypes.byref(rep)), lib)
+            t1 = time.perf_counter()
+            D.free()
+            out[name] = {"ms_per_call": sig(1e3 * (t1 - t0) / reps), "calls": reps,
+                         "iterations": [getattr(rep, k).iterations for k in solves],
+                         "max_rel_residual": max(getattr(rep, k).rel_residual for k in solves)}
+    finally:
+        lib.cathode_ndd_destroy(hd)
+    out["workload"] = (f"five-block cathode at 0.2 mm ({ni}x{nj} cells, the reference's grid), "
+                       "device-resident arrays, FP64 solves to 1e-10")
+    return out
+
+
 def cpu_model():
     """The host CPU's model name (lscpu, else /proc/cpuinfo)."""
     import subprocess
@@ -232,7 +280,34 @@ def cpu_baseline(seed, budget_s=12.0):
 
     v1, n1 = timed(1)
     va, na = timed(threads_all)
-    return {"value": va, "unit": "steps/s", "cores": threads_all, "kind": "port",
+    # the ion block and the neutral update on the same cathode workload as
+    # the device rows (multigrid solves to 1e-10), one thread
+    lib.orc_mg_set_threads(1)
+    lib.orc_ndd_set_geometry.argtypes = [ctypes.POINTER(abi.NddGeometry)]
+    lib.orc_ion_update.argtypes = [ctypes.POINTER(abi.NddParams), ctypes.POINTER(abi.IonParams),
+                                   ctypes.POINTER(abi.IonFields), ctypes.POINTER(abi.IonReport), ctypes.c_int]
+    lib.orc_neutral_update.argtypes = [ctypes.POINTER(abi.NddParams), ctypes.POINTER(abi.NeutralParams),
+                                       ctypes.POINTER(abi.NeutralFields), ctypes.POINTER(abi.NeutralReport),
+                                       ctypes.c_int]
+    ni, nj, g, grid, ion, neu = rows_workload(abi.load())
+    lib.orc_ndd_set_geometry(ctypes.byref(g))
+    rows = {}
+    for name, state, fn, prm, fs, rcls in (
+            ("ion_block", ion, lib.orc_ion_update, W.ion_params(tol_Ui=1e-10, tol_rhoi=1e-10), abi.ion_fields_struct,
+             abi.IonReport),
+            ("neutral_update", neu, lib.orc_neutral_update, W.neutral_params(tol=1e-10), abi.neutral_fields_struct,
+             abi.NeutralReport)):
+        st = W.copy_arrays(state)
+        f = fs(st)
+        rep = rcls()
+        assert fn(ctypes.byref(grid), ctypes.byref(prm), ctypes.byref(f), ctypes.byref(rep), 1) == 0
+        reps = 10
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            assert fn(ctypes.byref(grid), ctypes.byref(prm), ctypes.byref(f), ctypes.byref(rep), 1) == 0
+        rows[name] = {"ms_per_call": 1e3 * (time.perf_counter() - t0) / reps, "calls": reps, "cores": 1}
+    lib.orc_ndd_set_geometry(None)
+    return {"value": va, "unit": "steps/s", "cores": threads_all, "kind": "port", "rows": rows,
             "sample": f"{na} steps of the same 512x256 workload after 1 untimed step, {threads_all} threads",
             "one_core": {"value": v1, "cores": 1, "sample": f"{n1} steps after 1 untimed step"},
             "build": "oracle/orc_*.c, gcc -O3 -march=x86-64-v3 -fopenmp (oracle/liboracle_fast.so)",
@@ -338,6 +413,9 @@ def main():
         print(f"# sweep: {s['instance_steps_per_s']} instance-steps/s batched ({s['workers']} workers), "
               f"{s['instance_steps_per_s_one_at_a_time']} one at a time, max dev {s['max_rel_dev_vs_one_at_a_time']:.2e}",
               file=sys.stderr, flush=True)
+    out["rows"] = run_rows(lib)
+    print(f"# rows: ion block {out['rows']['ion_block']['ms_per_call']} ms/call, neutral update "
+          f"{out['rows']['neutral_update']['ms_per_call']} ms/call", file=sys.stderr, flush=True)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.seed)
     if net is not None:

@@ -460,6 +460,10 @@ typedef struct cathode_neutral_report {
  * handle's NDD state is not touched.  rep may be NULL. */
 int cathode_ndd_neutral_update(void* handle, const cathode_neutral_params* p, cathode_neutral_fields* f,
                                cathode_neutral_report* rep);
+/* The same on device arrays of that layout (no PCIe traffic); no reference
+ * counterpart, as cathode_ndd_ion_update_dev. */
+int cathode_ndd_neutral_update_dev(void* handle, const cathode_neutral_params* p, cathode_neutral_fields* f,
+                                   cathode_neutral_report* rep);
 
 /* ------------------------------------------------------------------ */
 /* the ion block of the reference's time step (src/main.cpp:194-213):  */
@@ -521,6 +525,12 @@ typedef struct cathode_ion_report {
  * not touched.  rep may be NULL. */
 int cathode_ndd_ion_update(void* handle, const cathode_ion_params* p, cathode_ion_fields* f,
                            cathode_ion_report* rep);
+/* The same on device arrays of that layout (cathode_dev_alloc or any HIP
+ * allocation on the handle's device), for a caller that keeps the fields
+ * resident: nothing crosses PCIe.  No reference counterpart (the reference
+ * holds its fields on the host). */
+int cathode_ndd_ion_update_dev(void* handle, const cathode_ion_params* p, cathode_ion_fields* f,
+                               cathode_ion_report* rep);
 
 int cathode_ndd_set_state(void* handle, const cathode_ndd_fields* host);
 int cathode_ndd_get_state(void* handle, cathode_ndd_fields* host);

@@ -103,3 +103,27 @@ def test_rectangle_runs_and_closes_the_balance(hiplib):
         assert np.isfinite(dev[k]).all(), k
     res = TI.balance_residual(g, ni, nj, h, p, a, dev)
     assert res < 1e-9, res
+
+
+def test_device_arrays_match_host_arrays(hiplib):
+    """cathode_ndd_ion_update_dev on device copies gives the host entry's bits"""
+    h = 4e-4
+    ni, nj, g, *_ = W.reference_geometry(h)
+    grid = W.default_params(ni, nj, h, h)
+    a = W.ion_state(g, ni, nj, h, seed=31)
+    p = W.ion_params()
+    print("ion device arrays vs host arrays", flush=True)
+    host, _ = device(hiplib, grid, g, p, a)
+    hd = ctypes.c_void_p()
+    abi.check(hiplib.cathode_ndd_create(ctypes.byref(grid), ctypes.byref(hd)), hiplib)
+    try:
+        abi.check(hiplib.cathode_ndd_set_geometry(hd, ctypes.byref(g)), hiplib)
+        f, D, names = abi.device_fields(hiplib, a, abi.ION_FIELD_NAMES, abi.IonFields)
+        rep = abi.IonReport()
+        abi.check(hiplib.cathode_ndd_ion_update_dev(hd, ctypes.byref(p), ctypes.byref(f), ctypes.byref(rep)), hiplib)
+        out = abi.download_fields(D, names)
+        D.free()
+    finally:
+        hiplib.cathode_ndd_destroy(hd)
+    for k in abi.ION_FIELD_NAMES:
+        assert np.array_equal(out[k], host[k]), k

@@ -145,3 +145,28 @@ def test_bad_arguments_are_refused(hiplib):
                                                  ctypes.byref(rep)) == abi.CATHODE_ERR_ARG
     finally:
         hiplib.cathode_ndd_destroy(hd)
+
+
+def test_device_arrays_match_host_arrays(hiplib):
+    """cathode_ndd_neutral_update_dev on device copies gives the host entry's bits"""
+    h = 4e-4
+    ni, nj, g, *_ = W.reference_geometry(h)
+    grid = W.default_params(ni, nj, h, h)
+    a = W.neutral_state(g, ni, nj, h, seed=29, noise=True)
+    p = W.neutral_params(which=3)
+    print("neutral device arrays vs host arrays", flush=True)
+    host, _ = device(hiplib, grid, g, p, a)
+    hd = ctypes.c_void_p()
+    abi.check(hiplib.cathode_ndd_create(ctypes.byref(grid), ctypes.byref(hd)), hiplib)
+    try:
+        abi.check(hiplib.cathode_ndd_set_geometry(hd, ctypes.byref(g)), hiplib)
+        f, D, names = abi.device_fields(hiplib, a, abi.NEUTRAL_FIELD_NAMES, abi.NeutralFields)
+        rep = abi.NeutralReport()
+        abi.check(hiplib.cathode_ndd_neutral_update_dev(hd, ctypes.byref(p), ctypes.byref(f), ctypes.byref(rep)),
+                  hiplib)
+        out = abi.download_fields(D, names)
+        D.free()
+    finally:
+        hiplib.cathode_ndd_destroy(hd)
+    for k in abi.NEUTRAL_FIELD_NAMES:
+        assert np.array_equal(out[k], host[k]), k
