@@ -168,6 +168,52 @@ def ion_fields_struct(arrays):
     return f
 
 
+RATE_TABLE_NAMES = ["nu_elas", "nu_exc", "nu_excReso", "nu_excMeta", "nu_ionz", "nu_ionzStep", "nu_excStep",
+                    "nu_super", "eloss_n", "eloss_m"]
+
+
+class RateTables(ctypes.Structure):
+    _fields_ = ([("n_e", ctypes.c_int)] + [(n, c_double_p) for n in RATE_TABLE_NAMES] +
+                [("dTe", ctypes.c_double), ("n_i", ctypes.c_int), ("nui_m", c_double_p), ("dEi", ctypes.c_double)])
+
+
+def rate_tables_struct(tabs):
+    """RateTables pointing into a dict of numpy tables (keys of RATE_TABLE_NAMES
+    plus nui_m, dTe, dEi)"""
+    t = RateTables()
+    t.n_e = len(tabs["nu_elas"])
+    for n in RATE_TABLE_NAMES:
+        setattr(t, n, dptr(tabs[n]))
+    t.dTe = tabs["dTe"]
+    t.n_i = len(tabs["nui_m"])
+    t.nui_m = dptr(tabs["nui_m"])
+    t.dEi = tabs["dEi"]
+    return t
+
+
+class TransportParams(ctypes.Structure):
+    _fields_ = [("masse", ctypes.c_double), ("massi", ctypes.c_double), ("Ti", ctypes.c_double),
+                ("nu_eff", ctypes.c_double), ("flag_Bohm", ctypes.c_int), ("flag_Sagdeev", ctypes.c_int)]
+
+
+TRANSPORT_IN = ["rhoe", "rhoi", "rhon", "rhom", "Te", "Bx", "Br", "nu_ano", "Uix", "Uir", "Uip", "rhoUex", "rhoUer"]
+TRANSPORT_OUT = ["nu_elas", "nu_exc", "nu_excReso", "nu_excMeta", "nu_ionz", "nu_ionzStep", "nu_excStep", "nu_super",
+                 "nu_m", "nu_m1", "rate_ionize", "nui_m", "rate_eloss", "delossdeps", "mu_para", "mu_xx", "mu_rr",
+                 "mu_xr", "mu_perp", "mu_min", "Mache"]
+TRANSPORT_FIELD_NAMES = TRANSPORT_IN + ["nu_ano_IAT"] + TRANSPORT_OUT
+
+
+class TransportFields(ctypes.Structure):
+    _fields_ = [(n, c_double_p) for n in TRANSPORT_FIELD_NAMES]
+
+
+def transport_fields_struct(arrays):
+    f = TransportFields()
+    for n in TRANSPORT_FIELD_NAMES:
+        setattr(f, n, dptr(arrays.get(n)))
+    return f
+
+
 MW_MAX_BLOCKS = 16
 
 
@@ -308,6 +354,11 @@ SIGNATURES = [
      [c_double_p, c_double_p, ctypes.c_double, ctypes.c_double, ctypes.POINTER(NddGeometry), c_int_p, c_int_p,
       c_int_p]),
     ("cathode_ndd_capture", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    ("cathode_ndd_set_rate_tables", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(RateTables)]),
+    ("cathode_ndd_transport_update", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.POINTER(TransportParams), ctypes.POINTER(TransportFields)]),
+    ("cathode_ndd_transport_update_dev", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.POINTER(TransportParams), ctypes.POINTER(TransportFields)]),
     ("cathode_ndd_ion_update", ctypes.c_int,
      [ctypes.c_void_p, ctypes.POINTER(IonParams), ctypes.POINTER(IonFields), ctypes.POINTER(IonReport)]),
     ("cathode_ndd_ion_update_dev", ctypes.c_int,

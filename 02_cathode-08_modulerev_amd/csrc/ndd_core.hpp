@@ -257,6 +257,13 @@ struct NddHandle {
     int ion_plan_version = -1;
     int ion_ng[3] = {0, 0, 0};
     DevBuf<int> ion_cnt;
+    // transport coefficients (transport.hip): the rate tables (10 electron
+    // tables of rate_n[0] entries, then the ion table of rate_n[1]), their
+    // steps dTe and dEi, the staging planes and the n_e maximum
+    DevBuf<double> rate_tab;
+    int rate_n[2] = {0, 0};
+    double rate_d[2] = {0.0, 0.0};
+    DevBuf<double> trp, tr_scal;
 };
 
 // launches kernel template K with the handle's geometry policy as the first

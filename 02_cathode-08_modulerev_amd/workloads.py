@@ -401,3 +401,41 @@ def ion_state(g, ni, nj, h, seed=17, V_bias=30.0, noise=True):
     for k in ("rhoUix", "rhoUir", "rhoUix_wall", "rhoUir_wall"):
         a[k] = rng.uniform(-1e19, 1e19, shp) if noise else np.zeros(shp)
     return {k: np.ascontiguousarray(v, dtype=np.float64) for k, v in a.items()}
+
+
+# ------------------------------------------------------------------ transport coefficients
+def transport_params(flag_Bohm=0, flag_Sagdeev=1, nu_eff=1e7):
+    p = abi.TransportParams()
+    p.masse, p.massi, p.Ti, p.nu_eff = MASSE, MASSI, TI, nu_eff
+    p.flag_Bohm, p.flag_Sagdeev = flag_Bohm, flag_Sagdeev
+    return p
+
+
+def transport_state(g, ni, nj, h, seed=41, Te_max_eV=9.0):
+    """Inputs of the transport update on geometry g: n_e, n_i (Gaussians over
+    1e15), n_n ~ 1e20, n_m ~ 1e17, Te = 0.5 .. Te_max_eV eV (a few cells
+    beyond the fixture table's end), B of 10..80 mT, nu_ano, ion velocities
+    and electron fluxes of either sign; nu_ano_IAT and Mache carry random
+    values (kept where the update does not write them)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    shp = (ni + 2, nj + 2)
+    sm = lambda: smooth_field(rng, ni, nj, h)
+    a = {}
+    a["rhoe"] = 1e15 + 1e17 * sm() ** 2
+    a["rhoi"] = a["rhoe"] * (1.0 + 0.1 * sm())
+    a["rhon"] = 1e20 * (0.5 + sm())
+    a["rhom"] = 1e17 * (0.5 + sm())
+    a["Te"] = (0.5 + (Te_max_eV - 0.5) * sm()) * E0 / BOLTZ
+    a["Te"][rng.integers(1, ni + 1, 5), rng.integers(1, nj + 1, 5)] = 30.0 * E0 / BOLTZ
+    a["Bx"] = 0.01 + 0.07 * sm()
+    a["Br"] = -0.04 + 0.08 * sm()
+    a["nu_ano"] = 1e6 * sm()
+    for k, sc in (("Uix", 2e3), ("Uir", 2e3), ("Uip", 5e2), ("rhoUex", 1e22), ("rhoUer", 1e22)):
+        a[k] = rng.uniform(-sc, sc, shp)
+    a["Uix"][rng.integers(1, ni + 1, 5), rng.integers(1, nj + 1, 5)] = 2e5   # beyond the ion table
+    a["nu_ano_IAT"] = rng.uniform(0.0, 1e5, shp)
+    a["Mache"] = rng.uniform(0.0, 1.0, shp)
+    for k in abi.TRANSPORT_OUT:
+        if k not in a:
+            a[k] = np.zeros(shp)
+    return {k: np.ascontiguousarray(v, dtype=np.float64) for k, v in a.items()}

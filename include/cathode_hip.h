@@ -532,6 +532,61 @@ int cathode_ndd_ion_update(void* handle, const cathode_ion_params* p, cathode_io
 int cathode_ndd_ion_update_dev(void* handle, const cathode_ion_params* p, cathode_ion_fields* f,
                                cathode_ion_report* rep);
 
+/* ------------------------------------------------------------------ */
+/* transport coefficients from the rate tables                         */
+/* (FluidModule::update_transport_coef, include/fluidModule.hpp:       */
+/*  5799-5976; the tables of InputModule::input_BOLSIG_data,           */
+/*  include/inputModule.hpp:1407-1654)                                  */
+/* ------------------------------------------------------------------ */
+/* Electron tables: per neutral (or metastable) density, one entry per
+ * mean-energy step of the BOLSIG+ file, looked up by Te [K] with the step
+ * dTe = 2/3 of the energy step in K (miniFuncs.hpp func_Te: linear between
+ * entries, the last entry beyond the table; func_dTe: the slope, the last
+ * interval beyond).  Ion table nui_m by m_i |U_i|^2 with the step dEi [J]
+ * (func_Ei: beyond the table the cross-section of the last entry is kept,
+ * rate proportional to the speed).  host/tables.hpp reads the files. */
+typedef struct cathode_rate_tables {
+    int n_e;                                                  /* entries of each electron table */
+    const double *nu_elas, *nu_exc, *nu_excReso, *nu_excMeta, *nu_ionz, *nu_ionzStep, *nu_excStep, *nu_super;
+    const double *eloss_n, *eloss_m;                          /* energy-loss rates per neutral / metastable [J m^3/s] */
+    double dTe;                                               /* [K] */
+    int n_i;                                                  /* entries of the ion table */
+    const double* nui_m;
+    double dEi;                                               /* [J] */
+} cathode_rate_tables;
+
+/* copies the tables to the handle's device (host arrays) */
+int cathode_ndd_set_rate_tables(void* handle, const cathode_rate_tables* t);
+
+typedef struct cathode_transport_params {
+    double masse, massi, Ti;    /* [kg], [kg], [K] */
+    double nu_eff;              /* added to the microwave collision frequency nu_m1 [1/s] */
+    int flag_Bohm, flag_Sagdeev;
+} cathode_transport_params;
+
+/* Arrays of (ni+2) x (nj+2) doubles, [i][j], ld = nj + 2.  In: rhoe, rhoi,
+ * rhon, rhom, Te [K], Bx, Br [T], nu_ano (Bohm), Uix (x faces), Uir (r
+ * faces), Uip, rhoUex (x faces), rhoUer (r faces); NULL = 0.  In/out:
+ * nu_ano_IAT (written on the plasma cells, read everywhere).  Out (every
+ * cell of the extended grid; Mache on the plasma cells): the collision
+ * frequencies, nu_m, nu_m1, rate_ionize, nui_m, rate_eloss, delossdeps and
+ * the mobility tensor mu_para, mu_xx, mu_rr, mu_xr, mu_perp, mu_min.  An
+ * output left NULL is not written. */
+typedef struct cathode_transport_fields {
+    const double *rhoe, *rhoi, *rhon, *rhom, *Te, *Bx, *Br, *nu_ano, *Uix, *Uir, *Uip, *rhoUex, *rhoUer;
+    double *nu_ano_IAT;
+    double *nu_elas, *nu_exc, *nu_excReso, *nu_excMeta, *nu_ionz, *nu_ionzStep, *nu_excStep, *nu_super;
+    double *nu_m, *nu_m1, *rate_ionize, *nui_m, *rate_eloss, *delossdeps;
+    double *mu_para, *mu_xx, *mu_rr, *mu_xr, *mu_perp, *mu_min, *Mache;
+} cathode_transport_fields;
+
+/* update_transport_coef on the handle's grid and geometry with the tables
+ * of cathode_ndd_set_rate_tables (host arrays) */
+int cathode_ndd_transport_update(void* handle, const cathode_transport_params* p, cathode_transport_fields* f);
+/* The same on device arrays of that layout; no reference counterpart (for
+ * callers that keep the fields resident). */
+int cathode_ndd_transport_update_dev(void* handle, const cathode_transport_params* p, cathode_transport_fields* f);
+
 int cathode_ndd_set_state(void* handle, const cathode_ndd_fields* host);
 int cathode_ndd_get_state(void* handle, cathode_ndd_fields* host);
 int cathode_ndd_step(void* handle, int nsteps, cathode_ndd_report* rep);

@@ -118,7 +118,9 @@ enum {
     ORC_PLANT_META_VTH = 18,          /* metastable wall loss with sqrt(2 k Tn / m) for the mean speed */
     ORC_PLANT_META_RWALL_Q = 19,      /* metastable high-r wall loss without the face weight qR */
     ORC_PLANT_ION_LORENTZ_SIGN = 20,  /* azimuthal ion momentum with the Lorentz force reversed */
-    ORC_PLANT_ION_CORR_TE = 21        /* velocity correction's diffusion without Te_rep */
+    ORC_PLANT_ION_CORR_TE = 21,       /* velocity correction's diffusion without Te_rep */
+    ORC_PLANT_TR_DENO = 22,           /* mobility tensor over 2 + W^2 (the unused deno1) */
+    ORC_PLANT_TR_IAT_DENSITY = 23     /* ion plasma frequency of the IAT rate from n_e */
 };
 void orc_plant(int which);
 int orc_planted(void);
@@ -201,6 +203,16 @@ int orc_neutral_update(const cathode_ndd_params* grid, const cathode_neutral_par
  * cathode_ndd_ion_update. */
 int orc_ion_update(const cathode_ndd_params* grid, const cathode_ion_params* p, cathode_ion_fields* f,
                    cathode_ion_report* rep, int solver);
+
+/* FluidModule::update_transport_coef (fluidModule.hpp:5799-5976) and the
+ * table lookups of miniFuncs.hpp:35-91 (orc_transport.c), over the geometry
+ * of orc_ndd_set_geometry.  Same arguments and outputs as
+ * cathode_ndd_transport_update, the tables as host arrays. */
+int orc_transport_update(const cathode_ndd_params* grid, const cathode_transport_params* p,
+                         const cathode_rate_tables* T, cathode_transport_fields* f);
+double orc_func_Te_tab(const double* v, int n, double Te, double dTe);
+double orc_func_dTe_tab(const double* v, int n, double Te, double dTe);
+double orc_func_Ei_tab(const double* v, int n, double Ei, double dEi, double massi);
 
 /* EmfieldModule::update_energy_profile (emfieldModule.hpp:7630-8105): J = sigma E,
  * absorbed power, Poynting fluxes and the port normalisation (orc_mw.c) */

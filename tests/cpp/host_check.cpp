@@ -1,6 +1,9 @@
 // host_check.cpp -- checks of the C++ host headers (02_cathode-08_modulerev_amd/host).
 //
 //   host_check cpu <tmpdir> <setup.yaml>   CmpMat2 / S<->ABCD, setup.yaml reader, restart CSV
+//   host_check tables <rateCoef_e> <rateCoef_i> <coefEISEE> <out>
+//                             the rate and yield tables read by host/tables.hpp,
+//                             written to <out> as "name n v0 v1 ..." lines (%.17g)
 //   host_check gpu            solver_SMG / solver_SOR / solver_LU_9p through the
 //                             drop-in header against the oracle (liboracle.so)
 // Prints one line per check and exits non-zero on the first failure.
@@ -18,6 +21,7 @@
 #include "../../02_cathode-08_modulerev_amd/host/restart_csv.hpp"
 #include "../../02_cathode-08_modulerev_amd/host/setup_yaml.hpp"
 #include "../../02_cathode-08_modulerev_amd/host/solverFuncs.hpp"
+#include "../../02_cathode-08_modulerev_amd/host/tables.hpp"
 #include "../../oracle/oracle.h"
 
 using namespace cathode_host;
@@ -301,6 +305,36 @@ static void check_dropin_gpu()
     std::printf("ok dropin\n");
 }
 
+static void dump_tables(const char* pe, const char* pi, const char* psee, const char* out)
+{
+    const RateTables T = read_rate_tables(pe, pi);
+    cathode_ndd_params p{};
+    read_eisee_table(psee, p);
+    FILE* f = std::fopen(out, "w");
+    auto put = [&](const char* name, const double* v, size_t n) {
+        std::fprintf(f, "%s %zu", name, n);
+        for (size_t k = 0; k < n; k++) std::fprintf(f, " %.17g", v[k]);
+        std::fprintf(f, "\n");
+    };
+    const std::vector<double>* tabs[11] = {&T.nu_elas, &T.nu_exc, &T.nu_excReso, &T.nu_excMeta, &T.nu_ionz,
+                                           &T.nu_ionzStep, &T.nu_excStep, &T.nu_super, &T.eloss_n, &T.eloss_m,
+                                           &T.nui_m};
+    const char* names[11] = {"nu_elas", "nu_exc", "nu_excReso", "nu_excMeta", "nu_ionz", "nu_ionzStep", "nu_excStep",
+                             "nu_super", "eloss_n", "eloss_m", "nui_m"};
+    for (int q = 0; q < 11; q++) put(names[q], tabs[q]->data(), tabs[q]->size());
+    put("dTe", &T.dTe, 1);
+    put("dEi", &T.dEi, 1);
+    put("see_eb", p.see_eb, (size_t)p.n_see);
+    put("see_rd", p.see_rd, (size_t)p.n_see);
+    put("see_ts", p.see_ts, (size_t)p.n_see);
+    put("dTe_SEE", &p.dTe_SEE, 1);
+    std::fclose(f);
+    const cathode_rate_tables v = T.view();
+    CHECK(v.n_e == (int)T.nu_elas.size() && v.n_i == (int)T.nui_m.size() && v.nu_super == T.nu_super.data(),
+          "view");
+    std::printf("ok tables\n");
+}
+
 int main(int argc, char** argv)
 {
     const std::string mode = argc > 1 ? argv[1] : "cpu";
@@ -308,6 +342,8 @@ int main(int argc, char** argv)
         check_abcd();
         check_yaml(argc > 3 ? argv[3] : "tests/golden/setup_reference_layout.yaml");
         check_restart(argc > 2 ? argv[2] : ".");
+    } else if (mode == "tables" && argc > 5) {
+        dump_tables(argv[2], argv[3], argv[4], argv[5]);
     } else if (mode == "gpu") {
         check_dropin_gpu();
     } else if (mode == "noinit") {

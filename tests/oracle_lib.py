@@ -68,6 +68,12 @@ _SIGS = [
     ("orc_ion_update", ctypes.c_int,
      [ctypes.POINTER(abi.NddParams), ctypes.POINTER(abi.IonParams), ctypes.POINTER(abi.IonFields),
       ctypes.POINTER(abi.IonReport), ctypes.c_int]),
+    ("orc_transport_update", ctypes.c_int,
+     [ctypes.POINTER(abi.NddParams), ctypes.POINTER(abi.TransportParams), ctypes.POINTER(abi.RateTables),
+      ctypes.POINTER(abi.TransportFields)]),
+    ("orc_func_Te_tab", ctypes.c_double, [_D, ctypes.c_int, ctypes.c_double, ctypes.c_double]),
+    ("orc_func_dTe_tab", ctypes.c_double, [_D, ctypes.c_int, ctypes.c_double, ctypes.c_double]),
+    ("orc_func_Ei_tab", ctypes.c_double, [_D, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
     ("orc_s_to_abcd", None, [_D, ctypes.c_double, _D]),
     ("orc_abcd_to_s", None, [_D, ctypes.c_double, _D]),
 ]
@@ -260,6 +266,8 @@ ORC_PLANT_META_VTH = 18
 ORC_PLANT_META_RWALL_Q = 19
 ORC_PLANT_ION_LORENTZ_SIGN = 20
 ORC_PLANT_ION_CORR_TE = 21
+ORC_PLANT_TR_DENO = 22
+ORC_PLANT_TR_IAT_DENSITY = 23
 
 
 def check_cfl(params, state):
@@ -302,3 +310,11 @@ def ion_update(grid, params, arrays, solver=0):
     rc = lib().orc_ion_update(ctypes.byref(grid), ctypes.byref(params), ctypes.byref(f), ctypes.byref(rep), solver)
     assert rc == 0, "orc_ion_update needs the five-block geometry"
     return rep
+
+
+def transport_update(grid, params, tabs, arrays):
+    """the oracle's update_transport_coef on a dict of arrays (modified in place)"""
+    t = abi.rate_tables_struct(tabs)
+    f = abi.transport_fields_struct(arrays)
+    rc = lib().orc_transport_update(ctypes.byref(grid), ctypes.byref(params), ctypes.byref(t), ctypes.byref(f))
+    assert rc == 0

@@ -88,7 +88,9 @@ def test_entry_points_refuse_before_init():
     ("cathode_mw_params", "MwParams"), ("cathode_mw_fields", "MwFields"), ("cathode_mw_report", "MwReport"),
     ("cathode_neutral_params", "NeutralParams"), ("cathode_neutral_fields", "NeutralFields"),
     ("cathode_neutral_report", "NeutralReport"), ("cathode_ion_params", "IonParams"),
-    ("cathode_ion_fields", "IonFields"), ("cathode_ion_report", "IonReport")])
+    ("cathode_ion_fields", "IonFields"), ("cathode_ion_report", "IonReport"),
+    ("cathode_rate_tables", "RateTables"), ("cathode_transport_params", "TransportParams"),
+    ("cathode_transport_fields", "TransportFields")])
 def test_ctypes_struct_layout_matches_header(cname, pyname, tmp_path):
     """Every ctypes mirror has the C struct's size and field offsets (gcc on the header)."""
     py = getattr(abi, pyname)
