@@ -223,7 +223,56 @@ def run_rows(lib, reps=20):
         lib.cathode_ndd_destroy(hd)
     out["workload"] = (f"five-block cathode at 0.2 mm ({ni}x{nj} cells, the reference's grid), "
                        "device-resident arrays, FP64 solves to 1e-10")
+    out["transport_update"] = run_transport(lib)
     return out
+
+
+TRANSPORT_BYTES_PER_CELL = 288  # 14 planes read (13 inputs + nu_ano_IAT), 22 written (21 outputs + nu_ano_IAT)
+
+
+def run_transport(lib, reps=50, grid=(512, 256)):
+    """update_transport_coef on the bench's 512x256 rectangle (every cell
+    plasma), device-resident arrays, synthetic tables of the reference's size
+    (1800 electron entries, 39 ion entries): wall time per call (a max
+    reduction of n_e and the pointwise kernel)."""
+    import numpy as np
+    ni, nj = grid
+    h = 0.05e-3 * 512 / ni
+    p = W.default_params(ni, nj, h, h)
+    rng = np.random.Generator(np.random.PCG64(7))
+    tabs = {n: np.ascontiguousarray(1e-14 * (1.0 + rng.uniform(0, 1, 1800))) for n in abi.RATE_TABLE_NAMES}
+    tabs["nui_m"] = np.ascontiguousarray(1e-15 * (1.0 + rng.uniform(0, 1, 39)))
+    tabs["dTe"], tabs["dEi"] = 0.1 * 2.0 / 3.0 * W.E0 / W.BOLTZ, 0.1 * W.E0
+    a = W.transport_state(W_rect(ni, nj), ni, nj, h)
+    hd = ctypes.c_void_p()
+    abi.check(lib.cathode_ndd_create(ctypes.byref(p), ctypes.byref(hd)), lib)
+    try:
+        t = abi.rate_tables_struct(tabs)
+        abi.check(lib.cathode_ndd_set_rate_tables(hd, ctypes.byref(t)), lib)
+        f, D, names = abi.device_fields(lib, a, abi.TRANSPORT_FIELD_NAMES, abi.TransportFields)
+        prm = W.transport_params()
+        for _ in range(3):
+            abi.check(lib.cathode_ndd_transport_update_dev(hd, ctypes.byref(prm), ctypes.byref(f)), lib)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            abi.check(lib.cathode_ndd_transport_update_dev(hd, ctypes.byref(prm), ctypes.byref(f)), lib)
+        t1 = time.perf_counter()
+        D.free()
+    finally:
+        lib.cathode_ndd_destroy(hd)
+    cells = (ni + 2) * (nj + 2)
+    return {"ms_per_call": sig(1e3 * (t1 - t0) / reps), "calls": reps, "cells": cells,
+            "bytes_per_call": cells * TRANSPORT_BYTES_PER_CELL,
+            "workload": f"{ni}x{nj} rectangle, device-resident arrays, synthetic tables of 1800 / 39 entries"}
+
+
+def W_rect(ni, nj):
+    """the built-in rectangle in segment form (for the workload builders)"""
+    g = abi.NddGeometry()
+    g.n_bl = 1
+    for q, v in enumerate((1, ni, 1, nj)):
+        g.cell_box[q] = v
+    return g
 
 
 def cpu_model():
@@ -331,6 +380,8 @@ def main():
                     help="skip the 64-instance bias sweep (batched vs one at a time), run by default")
     ap.add_argument("--sweep-steps", type=int, default=50)
     ap.add_argument("--sweep-workers", type=int, default=8)
+    ap.add_argument("--rows-only", action="store_true",
+                    help="only time the ion block, the neutral update and the transport update (one JSON line)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run the copy kernel used to calibrate the HBM traffic counters")
     args = ap.parse_args()
@@ -347,6 +398,10 @@ def main():
         sys.exit(2)
     lib = abi.load()
     abi.check(lib.cathode_init(0), lib)
+    if args.rows_only:
+        print(json.dumps({"rows": run_rows(lib)}))
+        lib.cathode_finalize()
+        return
     paths = [s.strip() for s in args.paths.split(",") if s.strip()]
     res = {}
     states = {}
@@ -415,7 +470,8 @@ def main():
               file=sys.stderr, flush=True)
     out["rows"] = run_rows(lib)
     print(f"# rows: ion block {out['rows']['ion_block']['ms_per_call']} ms/call, neutral update "
-          f"{out['rows']['neutral_update']['ms_per_call']} ms/call", file=sys.stderr, flush=True)
+          f"{out['rows']['neutral_update']['ms_per_call']} ms/call, transport update "
+          f"{out['rows']['transport_update']['ms_per_call']} ms/call", file=sys.stderr, flush=True)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.seed)
     if net is not None:
