@@ -272,6 +272,81 @@ def mw_fields_struct(arrays):
     return f
 
 
+MWBC_DIRICHLET, MWBC_OPEN, MWBC_GAUSS, MWBC_CONCAVE_A, MWBC_CONCAVE_B, MWBC_CONVEX_A, MWBC_CONVEX_B, \
+    MWBC_EXCITE = range(8)
+MW_COUPLED, MW_IMPEDANCE = 0, 1
+
+
+class MwBcList(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int), ("i", c_int_p), ("j", c_int_p), ("s", c_int_p)]
+
+
+class MwBc(ctypes.Structure):
+    _fields_ = [("x", MwBcList * 4), ("r", MwBcList * 4), ("p", MwBcList * 4)]
+
+
+class MwSolveOpts(ctypes.Structure):
+    _fields_ = [("which", ctypes.c_int), ("j_flc0", ctypes.c_int), ("nu_eff", ctypes.c_double),
+                ("max_refine", ctypes.c_int), ("tol", ctypes.c_double)]
+
+
+MW_SOLVE_FIELD_NAMES = ["rhoe", "nu_m", "Bx", "Br", "epsr", "Er_dc", "act_x", "act_r", "act_p",
+                        "J1x_exc", "J1r_exc", "J1p_exc", "E1x", "E1r", "E1p"]
+MW_SOLVE_INT_FIELDS = ("act_x", "act_r", "act_p")
+
+
+class MwSolveFields(ctypes.Structure):
+    _fields_ = [(n, c_int_p if n in MW_SOLVE_INT_FIELDS else c_double_p) for n in MW_SOLVE_FIELD_NAMES]
+
+
+class MwSolveReport(ctypes.Structure):
+    _fields_ = [
+        ("n_x", ctypes.c_int), ("n_r", ctypes.c_int), ("n_p", ctypes.c_int), ("block", ctypes.c_int),
+        ("refine", ctypes.c_int), ("rel_residual", ctypes.c_double),
+        ("V_ref", ctypes.c_double * 2), ("I_ref", ctypes.c_double * 2), ("Z_ref", ctypes.c_double * 2),
+        ("P_fwd", ctypes.c_double), ("P_ref", ctypes.c_double),
+        ("P_fwd_all", ctypes.c_double), ("P_ref_all", ctypes.c_double),
+        ("P_out", ctypes.c_double), ("P_err", ctypes.c_double), ("ratio", ctypes.c_double),
+        ("J1r_exc", ctypes.c_double),
+    ]
+
+
+def mw_bc_struct(lists):
+    """MwBc from {"x": [W, E, S, N], "r": [...], "p": [...]}, each side a list
+    of (i, j, s) triples.  Returns (struct, keep) -- keep holds the int arrays
+    the struct points into."""
+    bc, keep = MwBc(), []
+    for comp in ("x", "r", "p"):
+        sides = lists.get(comp, [[], [], [], []])
+        for k in range(4):
+            e = np.ascontiguousarray(np.array(sides[k], dtype=np.int32).reshape(-1, 3).T.copy())
+            keep.append(e)
+            L = getattr(bc, comp)[k]
+            L.n = e.shape[1]
+            L.i = e[0].ctypes.data_as(c_int_p)
+            L.j = e[1].ctypes.data_as(c_int_p)
+            L.s = e[2].ctypes.data_as(c_int_p)
+    return bc, keep
+
+
+def mw_solve_fields_struct(arrays):
+    """MwSolveFields pointing into a dict of numpy arrays (complex arrays as
+    complex128, masks as int32; missing keys -> NULL)."""
+    f = MwSolveFields()
+    for n in MW_SOLVE_FIELD_NAMES:
+        a = arrays.get(n)
+        if a is None:
+            setattr(f, n, None)
+        elif n in MW_SOLVE_INT_FIELDS:
+            assert a.dtype == np.int32 and a.flags["C_CONTIGUOUS"]
+            setattr(f, n, a.ctypes.data_as(c_int_p))
+        else:
+            if np.iscomplexobj(a):
+                a = a.view(np.float64)
+            setattr(f, n, dptr(a))
+    return f
+
+
 class EthetaParams(ctypes.Structure):
     _fields_ = [("ni", ctypes.c_int), ("nj", ctypes.c_int), ("dx", ctypes.c_double),
                 ("dr", ctypes.c_double), ("rmin", ctypes.c_double), ("omega", ctypes.c_double),
@@ -315,6 +390,12 @@ SIGNATURES = [
                 ctypes.POINTER(Status)]),
     ("cathode_mw_energy_profile", ctypes.c_int,
      [ctypes.POINTER(MwParams), ctypes.POINTER(MwFields), ctypes.POINTER(MwReport)]),
+    ("cathode_mw_solve", ctypes.c_int,
+     [ctypes.POINTER(MwParams), ctypes.POINTER(MwBc), ctypes.POINTER(MwSolveOpts), ctypes.POINTER(MwSolveFields),
+      ctypes.POINTER(MwSolveReport)]),
+    ("cathode_mw_solve_dev", ctypes.c_int,
+     [ctypes.POINTER(MwParams), ctypes.POINTER(MwBc), ctypes.POINTER(MwSolveOpts), ctypes.POINTER(MwSolveFields),
+      ctypes.POINTER(MwSolveReport)]),
     ("cathode_struct5_solve_dev_noalloc", ctypes.c_int,
      _SOLVE5 + [ctypes.c_int, ctypes.c_double, ctypes.c_int, _D, ctypes.c_void_p,
                 ctypes.POINTER(Status)]),

@@ -231,6 +231,7 @@ int cathode_finalize(void)
     sor_release();
     helm_release();
     mw_release();
+    mws_release();
     if (c.stream) (void)hipStreamDestroy(c.stream);
     c.stream = nullptr;
     c.ready = false;

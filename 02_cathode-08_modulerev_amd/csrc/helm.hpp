@@ -7,5 +7,7 @@ namespace cathode {
 void helm_release();
 // frees the microwave power-profile buffers (mw.hip)
 void mw_release();
+// frees the coupled microwave solve's workspace (mwsolve.hip)
+void mws_release();
 
 }  // namespace cathode

@@ -439,3 +439,144 @@ def transport_state(g, ni, nj, h, seed=41, Te_max_eV=9.0):
         if k not in a:
             a[k] = np.zeros(shp)
     return {k: np.ascontiguousarray(v, dtype=np.float64) for k, v in a.items()}
+
+
+# ---- coupled microwave solve (cathode_mw_solve) ----
+
+def _mw_params(ni, nj, dx, dr, rmin, fmw):
+    p = abi.MwParams()
+    p.ni, p.nj, p.dx, p.dr, p.rmin = ni, nj, dx, dr, rmin
+    p.omega = 2.0 * np.pi * fmw
+    p.masse = MASSE
+    return p
+
+
+def _set_box(arr, b, box):
+    for k in range(4):
+        arr[4 * b + k] = box[k]
+
+
+def _mask(ni, nj, box):
+    m = np.zeros((ni + 2, nj + 2), dtype=np.int32)
+    m[box[0]:box[1] + 1, box[2]:box[3] + 1] = 1
+    return m
+
+
+def _sparam(mag, deg):
+    return mag * np.cos(deg / 180.0 * np.pi), mag * np.sin(deg / 180.0 * np.pi)
+
+
+def coax_impedance_test():
+    """The reference's coaxial-feed impedance test (test/校正/setup.yaml with
+    flag_impTest = 1; InitialModule::makeBoundary_impedanceTest,
+    include/initialModule.hpp:1061-1324): 194 x 7 cells of 0.2 mm between
+    r = 0.6 and 2 mm, dielectric eps_r 2.0838, 4.25 GHz, 8 W, TEM excitation
+    of 0.212687 at the west end, open east end, Gauss conditions on both
+    conductors.  The stub and cable corrections of that setup are zero
+    (dh_stab = dl_cable = 0), so the port S-parameters are the measured ones.
+    Returns (MwParams, lists, MwSolveOpts, arrays)."""
+    ni, nj, xL, xR, rmin, rmax = 194, 7, 0.0, 0.0388, 0.0006, 0.002
+    dx, dr = (xR - xL) / ni, (rmax - rmin) / nj
+    p = _mw_params(ni, nj, dx, dr, rmin, 4.25e9)
+    p.n_bl = 1
+    cell, xf, rf = (1, ni, 1, nj), (2, ni, 1, nj), (1, ni, 2, nj)
+    _set_box(p.cell_box, 0, cell)
+    _set_box(p.xface_box, 0, xf)
+    _set_box(p.rface_box, 0, rf)
+    p.flag_mwRef, p.Pmw, p.Z0 = 1, 8.0, 50.0
+    p.r1, p.r2 = rmin, rmax
+    p.S11[:] = _sparam(0.24403462, -552.4800212)
+    p.S21[:] = _sparam(0.797613913, -619.4434408)
+    p.S12[:] = _sparam(0.813217659, -619.9129232)
+    p.S22[:] = _sparam(0.126026294, -850.3272909)
+    p.J1r_exc = 0.212687
+    p.i_ref = int(0.0 / dx + 1.5)
+    lists = {
+        "x": [[(xf[0], j, abi.MWBC_DIRICHLET) for j in range(xf[2], xf[3] + 1)],
+              [(xf[1], j, abi.MWBC_GAUSS) for j in range(xf[2], xf[3] + 1)],
+              [(i, xf[2], abi.MWBC_DIRICHLET) for i in range(xf[0], xf[1] + 1)],
+              [(i, xf[3], abi.MWBC_DIRICHLET) for i in range(xf[0], xf[1] + 1)]],
+        "r": [[(rf[0], j, abi.MWBC_EXCITE) for j in range(rf[2], rf[3] + 1)],
+              [(rf[1], j, abi.MWBC_OPEN) for j in range(rf[2], rf[3] + 1)],
+              [(i, rf[2], abi.MWBC_GAUSS) for i in range(rf[0], rf[1] + 1)],
+              [(i, rf[3], abi.MWBC_GAUSS) for i in range(rf[0], rf[1] + 1)]],
+        "p": [[], [], [], []],
+    }
+    o = abi.MwSolveOpts()
+    o.which, o.j_flc0, o.nu_eff, o.max_refine, o.tol = abi.MW_IMPEDANCE, cell[2], 2.0e9, 3, 1e-13
+    shp = (ni + 2, nj + 2)
+    a = {"rhoe": np.full(shp, 1e17), "nu_m": np.full(shp, 5e8), "Bx": np.zeros(shp), "Br": np.zeros(shp),
+         "epsr": np.ones(shp)}
+    a["epsr"][cell[0]:cell[1] + 1, cell[2]:cell[3] + 1] = 2.083825280349
+    a["act_p"], a["act_x"], a["act_r"] = _mask(ni, nj, cell), _mask(ni, nj, xf), _mask(ni, nj, rf)
+    for k in ("J1x_exc", "J1r_exc", "J1p_exc", "E1x", "E1r", "E1p"):
+        a[k] = np.zeros(shp, dtype=np.complex128)
+    return p, lists, o, a
+
+
+def mw_coupled_rect(ni=24, nj=12, h=2e-4, seed=11, n0=3e17, B0=0.12, fmw=4.25e9, rmin=6e-4):
+    """A coupled, magnetised microwave case on one rectangular block that
+    runs every fold and ghost rule of solve_Microwave: x faces 2..ni x 1..nj,
+    r faces 1..ni x 2..nj, cells 1..ni x 1..nj; the entries of each list cycle
+    through that list's types (plus one type no rule names); a Gaussian
+    plasma (peak n0) in a field of about B0 (both components), collision
+    frequency 2e9, eps_r 1 .. 2.1, excitation currents on all three
+    components and a TEM excitation on the west E_r faces.  rmin > 0 (an
+    inner conductor, as in the coaxial feed): the south Gauss and corner rules
+    divide by the radius of the face below the first cell.  Returns
+    (MwParams, lists, MwSolveOpts, arrays)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    p = _mw_params(ni, nj, h, h, rmin, fmw)
+    p.n_bl = 1
+    cell, xf, rf = (1, ni, 1, nj), (2, ni, 1, nj), (1, ni, 2, nj)
+    _set_box(p.cell_box, 0, cell)
+    _set_box(p.xface_box, 0, xf)
+    _set_box(p.rface_box, 0, rf)
+    p.flag_mwRef, p.Pmw, p.Z0 = 1, 8.0, 50.0
+    p.r1, p.r2 = max(rmin, 0.5 * h), rmin + nj * h
+    p.J1r_exc = 0.2
+    p.i_ref = 1
+
+    def cyc(entries, types):
+        return [(i, j, types[k % len(types)]) for k, (i, j) in enumerate(entries)]
+
+    xW = [(xf[0], j) for j in range(xf[2], xf[3] + 1)]
+    xE = [(xf[1], j) for j in range(xf[2], xf[3] + 1)]
+    xS = [(i, xf[2]) for i in range(xf[0], xf[1] + 1)]
+    xN = [(i, xf[3]) for i in range(xf[0], xf[1] + 1)]
+    rW = [(rf[0], j) for j in range(rf[2], rf[3] + 1)]
+    rE = [(rf[1], j) for j in range(rf[2], rf[3] + 1)]
+    rS = [(i, rf[2]) for i in range(rf[0], rf[1] + 1)]
+    rN = [(i, rf[3]) for i in range(rf[0], rf[1] + 1)]
+    pW = [(cell[0], j) for j in range(cell[2], cell[3] + 1)]
+    pE = [(cell[1], j) for j in range(cell[2], cell[3] + 1)]
+    pS = [(i, cell[2]) for i in range(cell[0], cell[1] + 1)]
+    pN = [(i, cell[3]) for i in range(cell[0], cell[1] + 1)]
+    lists = {
+        "x": [cyc(xW, (2, 3, 4, 0, 2, 9)), cyc(xE, (2, 4, 0, 2)), cyc(xS, (0, 6, 0, 0)), cyc(xN, (0, 1, 5, 6, 1))],
+        "r": [cyc(rW, (7, 7, 0, 1, 5, 6, 7)), cyc(rE, (1, 0, 5, 1)), cyc(rS, (2, 3, 2, 2)),
+              cyc(rN, (2, 3, 4, 2))],
+        "p": [cyc(pW, (0, 1, 1)), cyc(pE, (0,)), cyc(pS, (0,)), cyc(pN, (1, 0, 1))],
+    }
+    o = abi.MwSolveOpts()
+    o.which, o.j_flc0, o.nu_eff, o.max_refine, o.tol = abi.MW_COUPLED, 0, 2.0e9, 4, 1e-13
+    shp = (ni + 2, nj + 2)
+    x = (np.arange(ni + 2) - 0.5) * h
+    r = rmin + (np.arange(nj + 2) - 0.5) * h
+    X, R = np.meshgrid(x, r, indexing="ij")
+    xc, rc, s = x[ni // 2], r[nj // 3], 0.25 * ni * h
+    a = {}
+    a["rhoe"] = n0 * np.exp(-((X - xc) ** 2 + (R - rc) ** 2) / (2 * s * s)) + 1e14
+    a["nu_m"] = 2e9 * (1.0 + 0.2 * rng.uniform(-1, 1, shp))
+    a["Bx"] = B0 * (0.8 + 0.4 * np.cos(X / (ni * h) * np.pi))
+    a["Br"] = 0.5 * B0 * np.sin(2.0 * X / (ni * h) * np.pi) * R / (nj * h)
+    a["epsr"] = 1.0 + 1.1 * rng.uniform(0, 1, shp)
+    a["Er_dc"] = rng.uniform(-1e3, 1e3, shp)
+    a["act_p"], a["act_x"], a["act_r"] = _mask(ni, nj, cell), _mask(ni, nj, xf), _mask(ni, nj, rf)
+
+    def cx(scale):
+        return scale * (rng.uniform(-1, 1, shp) + 1j * rng.uniform(-1, 1, shp))
+
+    a["J1x_exc"], a["J1r_exc"], a["J1p_exc"] = cx(10.0), cx(10.0), cx(10.0)
+    a["E1x"], a["E1r"], a["E1p"] = cx(1.0), cx(1.0), cx(1.0)   # ghost rules read values they do not write
+    return p, lists, o, {k: np.ascontiguousarray(v) for k, v in a.items()}

@@ -251,6 +251,106 @@ typedef struct cathode_mw_report {
 int cathode_mw_energy_profile(const cathode_mw_params* p, cathode_mw_fields* f, cathode_mw_report* rep);
 
 /* ------------------------------------------------------------------ */
+/* coupled (E_x, E_r, E_phi) magnetised microwave solve               */
+/* (EmfieldModule::solve_Microwave, include/emfieldModule.hpp:2249-3321; */
+/*  EmfieldModule::solve_Microwave_impedanceTest, :5528-6141)          */
+/* ------------------------------------------------------------------ */
+
+/* Types of the boundary-list entries (MicrowaveBC::sBnd??, the values that
+ * makeBoundary writes, include/initialModule.hpp:207-1060).  A type a fold or
+ * ghost rule does not name is left alone there, as in the reference. */
+#define CATHODE_MWBC_DIRICHLET 0   /* conducting wall (ghost mirror or zero) */
+#define CATHODE_MWBC_OPEN 1        /* first-order absorbing boundary */
+#define CATHODE_MWBC_GAUSS 2       /* div E = 0 in the boundary cell */
+#define CATHODE_MWBC_CONCAVE_A 3   /* concave corner (lower / left wall) */
+#define CATHODE_MWBC_CONCAVE_B 4   /* concave corner (upper / right wall) */
+#define CATHODE_MWBC_CONVEX_A 5    /* convex corner (lower / left side open) */
+#define CATHODE_MWBC_CONVEX_B 6    /* convex corner (upper / right side open) */
+#define CATHODE_MWBC_EXCITE 7      /* TEM excitation of the coaxial feed (E_r, west side) */
+
+/* One boundary list (MicrowaveBC iBnd??, jBnd??, sBnd??): entry k is the
+ * face or cell (i[k], j[k]) with type s[k]. */
+typedef struct cathode_mw_bc_list {
+    int n;
+    const int* i;
+    const int* j;
+    const int* s;
+} cathode_mw_bc_list;
+
+/* The twelve lists, side index 0 = west (i - 1), 1 = east (i + 1), 2 = south
+ * (j - 1), 3 = north (j + 1): x[] for E_x (x faces), r[] for E_r (r faces),
+ * p[] for E_phi (cells); MicrowaveBC's iBndWx .. iBndNp.  Folds and ghost
+ * rules run in the reference's order: x W, E, S, N, then r, then p. */
+typedef struct cathode_mw_bc {
+    cathode_mw_bc_list x[4], r[4], p[4];
+} cathode_mw_bc;
+
+#define CATHODE_MW_COUPLED 0       /* solve_Microwave: all three components, coupled */
+#define CATHODE_MW_IMPEDANCE 1     /* solve_Microwave_impedanceTest: E_r alone (5-point), rhoe = 0,
+                                      nu_m = nu_eff, then the reference-plane normalisation */
+
+typedef struct cathode_mw_solve_opts {
+    int which;          /* CATHODE_MW_COUPLED or CATHODE_MW_IMPEDANCE */
+    int j_flc0;         /* gc.j_flc_bl[0][0]: the r-face type-3 south fold and ghost keep their
+                           scale only where r[j] >= j_flc0, the reference's comparison of a
+                           radius with this index (emfieldModule.hpp:2673, :3181) */
+    double nu_eff;      /* CATHODE_MW_IMPEDANCE: the collision frequency written into nu_m */
+    int max_refine;     /* refinement steps after the direct solve, 0..8 */
+    double tol;         /* refinement stops at ||b - A E|| / ||b|| <= tol */
+} cathode_mw_solve_opts;
+
+/* Arrays (ni+2)*(nj+2), ld = nj+2, complex ones interleaved (re, im).
+ * rhoe, nu_m, Bx, Br, epsr: cells (gc.rhoe, nu_m1, Bx, Br, epsr); with
+ * CATHODE_MW_IMPEDANCE rhoe and nu_m are overwritten with 0 and nu_eff on the
+ * whole grid, as the reference does (:5531-5536).  Er_dc: the static radial
+ * field gr.Er (r faces), read only by the east open ghost of E_r (:3162;
+ * NULL reads 0).  act_x, act_r, act_p: 1 marks an unknown (jdgBnd_Ex, _Er,
+ * _Ep).  J1x_exc, J1r_exc, J1p_exc: excitation currents (complex).  E1x,
+ * E1r, E1p (complex, in/out): the unknowns are written, then the ghost values
+ * the lists name; every other value is kept (the ghost rules read them).  With
+ * CATHODE_MW_IMPEDANCE all three fields and the excitation currents are
+ * rescaled on the whole grid by the power ratio. */
+typedef struct cathode_mw_solve_fields {
+    double *rhoe, *nu_m;
+    const double *Bx, *Br, *epsr, *Er_dc;
+    const int *act_x, *act_r, *act_p;
+    double *J1x_exc, *J1r_exc, *J1p_exc;
+    double *E1x, *E1r, *E1p;
+} cathode_mw_solve_fields;
+
+typedef struct cathode_mw_solve_report {
+    int n_x, n_r, n_p;           /* unknowns per component */
+    int block;                   /* dense block size of the column chain */
+    int refine;                  /* refinement steps taken */
+    double rel_residual;         /* ||b - A E||_2 / ||b||_2 of the solved field (before scaling) */
+    /* CATHODE_MW_IMPEDANCE (solve_Microwave_impedanceTest, :5917-6139) */
+    double V_ref[2], I_ref[2], Z_ref[2];   /* reference plane (x face p->i_ref) */
+    double P_fwd, P_ref;         /* at the cathode port, after scaling [W] */
+    double P_fwd_all, P_ref_all; /* at the amplifier, through the S-parameters, after scaling [W] */
+    double P_out;                /* Poynting flux through the east face of block 0 (mWLoss_at_boundary_xR) */
+    double P_err;                /* P_fwd - P_ref - P_out */
+    double ratio;                /* Pmw / P_fwd before scaling; the fields scale by sqrt(ratio) */
+    double J1r_exc;              /* the rescaled excitation amplitude */
+} cathode_mw_solve_report;
+
+/* Replaces EmfieldModule::solve_Microwave (emfieldModule.hpp:2249-3321) or,
+ * with which = CATHODE_MW_IMPEDANCE, solve_Microwave_impedanceTest
+ * (:5528-6141).  Of p it reads ni, nj, dx, dr, rmin, omega, masse, the
+ * blocks' boxes (the rows are assembled on the boxes' union; a box is
+ * x faces 1..ni+1 x 1..nj, r faces 0..ni+1 x 2..nj, cells 0..ni+1 x 1..nj),
+ * J1r_exc, r1, r2 (the excitation); with CATHODE_MW_IMPEDANCE also i_ref,
+ * Z0, Pmw and the S-parameters.  The system (Eigen SparseLU in the
+ * reference) is solved directly on the device: block cyclic reduction over
+ * the x columns with dense blocks, then refinement.  Errors: a list entry
+ * outside 1..ni x 1..nj (x 2..nj for E_r), two entries of one list on the
+ * same face or cell, or a singular block. */
+int cathode_mw_solve(const cathode_mw_params* p, const cathode_mw_bc* bc, const cathode_mw_solve_opts* o,
+                     cathode_mw_solve_fields* f, cathode_mw_solve_report* rep);
+/* The same with every array of f a device pointer (the lists stay on the host). */
+int cathode_mw_solve_dev(const cathode_mw_params* p, const cathode_mw_bc* bc, const cathode_mw_solve_opts* o,
+                         cathode_mw_solve_fields* f, cathode_mw_solve_report* rep);
+
+/* ------------------------------------------------------------------ */
 /* device-resident implicit NDD step                                  */
 /* (FluidModule::solve_phi_couple_wdTe_wSEE_PC, solve_rhoe_wdTe_wSEE_PC,
  *  solve_Te_wdTe_wSEE_PC, include/fluidModule.hpp:2080-4957, with the ion

@@ -120,7 +120,10 @@ enum {
     ORC_PLANT_ION_LORENTZ_SIGN = 20,  /* azimuthal ion momentum with the Lorentz force reversed */
     ORC_PLANT_ION_CORR_TE = 21,       /* velocity correction's diffusion without Te_rep */
     ORC_PLANT_TR_DENO = 22,           /* mobility tensor over 2 + W^2 (the unused deno1) */
-    ORC_PLANT_TR_IAT_DENSITY = 23     /* ion plasma frequency of the IAT rate from n_e */
+    ORC_PLANT_TR_IAT_DENSITY = 23,    /* ion plasma frequency of the IAT rate from n_e */
+    ORC_PLANT_MWS_RX_WEIGHT = 24,     /* coupled microwave: E_x entries of the r rows weighted 1/2 */
+    ORC_PLANT_MWS_EXCITE_SIGN = 25,   /* coupled microwave: excitation source with -i */
+    ORC_PLANT_MWS_GAUSS_RADII = 26    /* coupled microwave: south Gauss fold of E_r with rL / rR */
 };
 void orc_plant(int which);
 int orc_planted(void);
@@ -217,6 +220,25 @@ double orc_func_Ei_tab(const double* v, int n, double Ei, double dEi, double mas
 /* EmfieldModule::update_energy_profile (emfieldModule.hpp:7630-8105): J = sigma E,
  * absorbed power, Poynting fluxes and the port normalisation (orc_mw.c) */
 int orc_mw_energy_profile(const cathode_mw_params* p, cathode_mw_fields* f, cathode_mw_report* rep);
+
+/* EmfieldModule::solve_Microwave (emfieldModule.hpp:2249-3321) and
+ * solve_Microwave_impedanceTest (:5528-6141), orc_mwsolve.c: the same
+ * arguments and outputs as cathode_mw_solve (o->max_refine and o->tol unused:
+ * the direct solve is banded LU with partial pivoting).  Returns 0, or a
+ * negative code for an argument error or a zero pivot. */
+int orc_mw_solve(const cathode_mw_params* p, const cathode_mw_bc* bc, const cathode_mw_solve_opts* o,
+                 cathode_mw_solve_fields* f, cathode_mw_solve_report* rep);
+/* The folded rows of that system, for tests: for component c (0 = E_x,
+ * 1 = E_r, 2 = E_phi) the coefficients of slot k (ORC_MWS_*) at face or cell
+ * (i, j) are coef[((c * ORC_MWS_NSLOT + k) * (ni+2) + i) * (nj+2) + j]
+ * (complex, interleaved); sizes 3 * ORC_MWS_NSLOT * (ni+2)*(nj+2) complex. */
+enum { ORC_MWS_P = 0, ORC_MWS_E, ORC_MWS_W, ORC_MWS_N, ORC_MWS_S, ORC_MWS_NE, ORC_MWS_NW, ORC_MWS_SE,
+       ORC_MWS_SW, ORC_MWS_EE, ORC_MWS_WW, ORC_MWS_NN, ORC_MWS_SS, ORC_MWS_B, ORC_MWS_NSLOT };
+int orc_mw_rows(const cathode_mw_params* p, const cathode_mw_bc* bc, const cathode_mw_solve_opts* o,
+                const cathode_mw_solve_fields* f, double* coef);
+/* y = A E over the unknowns of that system (rows of inactive faces and cells 0) */
+int orc_mw_apply(const cathode_mw_params* p, const cathode_mw_solve_fields* f, int which, const double* coef,
+                 const double* Ex, const double* Er, const double* Ep, double* yx, double* yr, double* yp);
 
 /* check_CFL (fluidModule.hpp:57-115): out = {dt_min, max n_e, max mu, max Te,
  * max |Gamma_e|/n_e + vth(max Te)} of the state */

@@ -74,6 +74,14 @@ _SIGS = [
     ("orc_func_Te_tab", ctypes.c_double, [_D, ctypes.c_int, ctypes.c_double, ctypes.c_double]),
     ("orc_func_dTe_tab", ctypes.c_double, [_D, ctypes.c_int, ctypes.c_double, ctypes.c_double]),
     ("orc_func_Ei_tab", ctypes.c_double, [_D, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
+    ("orc_mw_solve", ctypes.c_int,
+     [ctypes.POINTER(abi.MwParams), ctypes.POINTER(abi.MwBc), ctypes.POINTER(abi.MwSolveOpts),
+      ctypes.POINTER(abi.MwSolveFields), ctypes.POINTER(abi.MwSolveReport)]),
+    ("orc_mw_rows", ctypes.c_int,
+     [ctypes.POINTER(abi.MwParams), ctypes.POINTER(abi.MwBc), ctypes.POINTER(abi.MwSolveOpts),
+      ctypes.POINTER(abi.MwSolveFields), _D]),
+    ("orc_mw_apply", ctypes.c_int,
+     [ctypes.POINTER(abi.MwParams), ctypes.POINTER(abi.MwSolveFields), ctypes.c_int, _D] + [_D] * 6),
     ("orc_s_to_abcd", None, [_D, ctypes.c_double, _D]),
     ("orc_abcd_to_s", None, [_D, ctypes.c_double, _D]),
 ]
@@ -268,6 +276,12 @@ ORC_PLANT_ION_LORENTZ_SIGN = 20
 ORC_PLANT_ION_CORR_TE = 21
 ORC_PLANT_TR_DENO = 22
 ORC_PLANT_TR_IAT_DENSITY = 23
+ORC_PLANT_MWS_RX_WEIGHT = 24
+ORC_PLANT_MWS_EXCITE_SIGN = 25
+ORC_PLANT_MWS_GAUSS_RADII = 26
+MWS_NSLOT = 14
+(MWS_P, MWS_E, MWS_W, MWS_N, MWS_S, MWS_NE, MWS_NW, MWS_SE, MWS_SW, MWS_EE, MWS_WW, MWS_NN, MWS_SS,
+ MWS_B) = range(14)
 
 
 def check_cfl(params, state):
@@ -318,3 +332,35 @@ def transport_update(grid, params, tabs, arrays):
     f = abi.transport_fields_struct(arrays)
     rc = lib().orc_transport_update(ctypes.byref(grid), ctypes.byref(params), ctypes.byref(t), ctypes.byref(f))
     assert rc == 0
+
+
+def mw_solve(params, lists, opts, arrays):
+    """the oracle's solve_Microwave / impedance test on a dict of arrays (modified in place)"""
+    bc, keep = abi.mw_bc_struct(lists)
+    f = abi.mw_solve_fields_struct(arrays)
+    rep = abi.MwSolveReport()
+    rc = lib().orc_mw_solve(ctypes.byref(params), ctypes.byref(bc), ctypes.byref(opts), ctypes.byref(f),
+                            ctypes.byref(rep))
+    assert rc == 0, rc
+    return rep
+
+
+def mw_rows(params, lists, opts, arrays):
+    """folded row coefficients, complex array [3, MWS_NSLOT, ni+2, nj+2]"""
+    bc, keep = abi.mw_bc_struct(lists)
+    f = abi.mw_solve_fields_struct(arrays)
+    out = np.zeros((3, MWS_NSLOT, params.ni + 2, params.nj + 2), dtype=np.complex128)
+    rc = lib().orc_mw_rows(ctypes.byref(params), ctypes.byref(bc), ctypes.byref(opts), ctypes.byref(f),
+                           P(out.view(np.float64)))
+    assert rc == 0, rc
+    return out
+
+
+def mw_apply(params, arrays, which, coef, Ex, Er, Ep):
+    """(A E) per component for the rows coef"""
+    f = abi.mw_solve_fields_struct(arrays)
+    ys = [np.zeros_like(Ex) for _ in range(3)]
+    E = [np.ascontiguousarray(e, dtype=np.complex128) for e in (Ex, Er, Ep)]
+    lib().orc_mw_apply(ctypes.byref(params), ctypes.byref(f), which, P(coef.view(np.float64)),
+                       *[P(e.view(np.float64)) for e in E], *[P(y.view(np.float64)) for y in ys])
+    return ys

@@ -90,7 +90,9 @@ def test_entry_points_refuse_before_init():
     ("cathode_neutral_report", "NeutralReport"), ("cathode_ion_params", "IonParams"),
     ("cathode_ion_fields", "IonFields"), ("cathode_ion_report", "IonReport"),
     ("cathode_rate_tables", "RateTables"), ("cathode_transport_params", "TransportParams"),
-    ("cathode_transport_fields", "TransportFields")])
+    ("cathode_transport_fields", "TransportFields"), ("cathode_mw_bc_list", "MwBcList"), ("cathode_mw_bc", "MwBc"),
+    ("cathode_mw_solve_opts", "MwSolveOpts"), ("cathode_mw_solve_fields", "MwSolveFields"),
+    ("cathode_mw_solve_report", "MwSolveReport")])
 def test_ctypes_struct_layout_matches_header(cname, pyname, tmp_path):
     """Every ctypes mirror has the C struct's size and field offsets (gcc on the header)."""
     py = getattr(abi, pyname)

@@ -1,0 +1,164 @@
+"""The oracle's coupled microwave solve (oracle/orc_mwsolve.c:
+EmfieldModule::solve_Microwave, include/emfieldModule.hpp:2249-3321, and
+solve_Microwave_impedanceTest, :5528-6141), without a GPU.
+
+Pins:
+- the reference's own impedance test (tests/golden/coax_impedance_er.csv, from
+  test/校正/result_deg0.csv and result_deg4.csv): the normalised, cell-centred
+  E_r of the coaxial feed at phases 0 and 80 degrees to the six printed
+  digits, the excitation amplitude implied by its analytic column, and the
+  amplifier's forward power of 校正後FWD_power.txt (12.57480 W).  This covers
+  the E_r rows, the excitation, open and Gauss folds, the ghost rules, the
+  reference-plane V and I and the power ratio;
+- the plasma part of every row (with the rows of rhoe = 0 subtracted) applied
+  to any field is i omega mu0 dx^2 dr^2 J, with J = sigma E computed by
+  update_energy_profile's restatement (oracle/orc_mw.c, itself pinned by the
+  cold-plasma momentum equation): the conductivity tensor and the averaging
+  weights of the cross-component couplings;
+- with B = 0 the E_phi rows decouple, and the coupled solve's E_phi equals the
+  E_theta system of orc_helm (pinned by manufactured solutions) with the same
+  conducting walls;
+- the banded LU leaves ||b - A E|| / ||b|| at roundoff.
+Each planted mistake (ORC_PLANT_MWS_*) fails its check.
+"""
+import csv
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+W = O.workloads
+abi = O.abi
+HERE = os.path.dirname(os.path.abspath(__file__))
+FIX = os.path.join(HERE, "golden", "coax_impedance_er.csv")
+FWD_POWER_AMP = 12.57480   # test/校正/校正後FWD_power.txt
+
+
+@pytest.fixture
+def clean():
+    yield
+    O.lib().orc_plant(0)
+
+
+def coax_errors():
+    p, lists, o, a = W.coax_impedance_test()
+    rep = O.mw_solve(p, lists, o, a)
+    r = p.rmin + (np.arange(p.nj + 2) - 0.5) * p.dr
+    E = a["E1r"]
+    err = 0.0
+    with open(FIX, newline="") as f:
+        rows = list(csv.reader(f))[1:]
+    for row in rows:
+        d, i, j = int(row[0]), int(row[1]), int(row[2])
+        rL, rR = (r[j - 1] + r[j]) / 2.0, (r[j + 1] + r[j]) / 2.0
+        Ec = (rR * E[i, j + 1] + rL * E[i, j]) / (2.0 * r[j]) * np.exp(1j * 2.0 * np.pi / 18.0 * d)
+        ref = complex(float(row[3]), float(row[4]))
+        err = max(err, abs(Ec - ref) / abs(ref))
+    # analytic column: J1r_exc(new) / r * 1e3 / ln(r2 / r1) * exp(-i omega x / c0) at deg 0
+    i, j = int(rows[0][1]), int(rows[0][2])
+    c0 = 1.0 / np.sqrt(W.EPS0 * 2.083825280349 * W.MU0)
+    x = (i - 0.5) * p.dx
+    anly = complex(float(rows[0][5]), float(rows[0][6]))
+    J1_fix = abs(anly) * r[j] * np.log(p.r2 / p.r1) / 1e3
+    assert abs(np.angle(anly) + p.omega / c0 * x) < 1e-5
+    return err, rep, J1_fix
+
+
+def test_coax_impedance_fixture(clean):
+    err, rep, J1_fix = coax_errors()
+    assert err < 1e-5, err
+    assert rep.rel_residual < 1e-12
+    assert abs(rep.J1r_exc / J1_fix - 1.0) < 1e-5
+    assert abs(rep.P_fwd - 8.0) < 1e-12                      # Pmw at the cathode port
+    assert abs(rep.P_fwd_all / FWD_POWER_AMP - 1.0) < 5e-6   # the reference's recorded value
+    assert rep.n_x == 0 and rep.n_p == 0 and rep.n_r == 194 * 6
+
+
+@pytest.mark.parametrize("plant", [O.ORC_PLANT_MWS_EXCITE_SIGN, O.ORC_PLANT_MWS_GAUSS_RADII])
+def test_coax_fixture_catches(clean, plant):
+    O.lib().orc_plant(plant)
+    err, _, _ = coax_errors()
+    assert err > 1e-3, err
+
+
+def sigma_error():
+    p, lists, o, a = W.mw_coupled_rect(ni=20, nj=10, seed=5)
+    rng = np.random.default_rng(2)
+    shp = a["E1x"].shape
+    E = []
+    for k in ("act_x", "act_r", "act_p"):
+        e = rng.uniform(-1, 1, shp) + 1j * rng.uniform(-1, 1, shp)
+        E.append(np.ascontiguousarray(e * (a[k] == 1)))
+    coefP = O.mw_rows(p, lists, o, a)
+    vac = W.copy_arrays(a)
+    vac["rhoe"] = np.zeros_like(a["rhoe"])
+    coef0 = O.mw_rows(p, lists, o, vac)
+    yP = O.mw_apply(p, a, o.which, coefP, *E)
+    y0 = O.mw_apply(p, a, o.which, coef0, *E)
+    # J = sigma E by update_energy_profile's restatement
+    m = {"rhoe": a["rhoe"], "nu_m": a["nu_m"], "Bx": a["Bx"], "Br": a["Br"],
+         "E1x": E[0].copy(), "E1r": E[1].copy(), "E1p": E[2].copy()}
+    for k in ("J1x", "J1r", "J1p", "J1x_exc", "J1r_exc", "J1p_exc"):
+        m[k] = np.zeros(shp, dtype=np.complex128)
+    for k in ("Pabsx", "Pabsr", "Pabsp", "Pabs"):
+        m[k] = np.zeros(shp)
+    p.flag_mwRef = 0
+    p.j_out, p.i_out_lo, p.i_out_hi, p.j_in_lo, p.j_in_hi = p.nj - 1, 1, p.ni, 1, p.nj
+    O.mw_energy_profile(p, m)
+    h4 = p.dx * p.dx * p.dr * p.dr
+    iwm = 1j * p.omega * W.MU0
+    worst = 0.0
+    for c, (J, act) in enumerate(((m["J1x"], a["act_x"]), (m["J1r"], a["act_r"]), (m["J1p"], a["act_p"]))):
+        got = (yP[c] - y0[c])[act == 1]
+        want = (iwm * h4 * J)[act == 1]
+        worst = max(worst, np.abs(got - want).max() / np.abs(want).max())
+    return worst
+
+
+def test_plasma_rows_are_the_conductivity_current(clean):
+    assert sigma_error() < 1e-10   # the vacuum parts cancel to roundoff of the larger rows
+
+
+def test_conductivity_check_catches_the_weight(clean):
+    O.lib().orc_plant(O.ORC_PLANT_MWS_RX_WEIGHT)
+    assert sigma_error() > 1e-3
+
+
+def test_bzero_ephi_equals_the_etheta_system(clean):
+    ni, nj, h = 24, 12, 2e-4
+    p, lists, o, a = W.mw_coupled_rect(ni=ni, nj=nj, h=h, B0=0.0, seed=9)
+    nu, epsr = 2e9, 1.7
+    a["nu_m"][:] = nu
+    a["epsr"][:] = epsr
+    a["act_x"][:] = 0
+    a["act_r"][:] = 0
+    lists = {"x": [[], [], [], []], "r": [[], [], [], []],
+             "p": [[(1, j, 0) for j in range(1, nj + 1)], [(ni, j, 0) for j in range(1, nj + 1)],
+                   [(i, 1, 0) for i in range(1, ni + 1)], [(i, nj, 0) for i in range(1, ni + 1)]]}
+    rep = O.mw_solve(p, lists, o, a)
+    assert rep.n_x == 0 and rep.n_r == 0 and rep.n_p == ni * nj and rep.rel_residual < 1e-12
+    hp = O.HelmParams(ni=ni, nj=nj, dx=h, dr=h, rmin=p.rmin, omega=p.omega, nu=nu, epsr=epsr,
+                      masse=W.MASSE, pec_z=1, pec_rmax=1)
+    shp = (ni + 2, nj + 2)
+    co = [np.zeros(shp, dtype=np.complex128) for _ in range(6)]
+    J = np.ascontiguousarray(a["J1p_exc"])
+    ne = np.ascontiguousarray(a["rhoe"])
+    O.lib().orc_helm_assemble(hp, O.P(ne), O.P(J.view(np.float64)), *[O.P(c.view(np.float64)) for c in co])
+    x = np.zeros(shp, dtype=np.complex128)
+    boxes = O.boxes_arr([1, ni, 1, nj])
+    rc = O.lib().orc_helm5_direct(*[O.P(c.view(np.float64)) for c in co], boxes.ctypes.data_as(abi.c_int_p), 1,
+                                  ni + 2, nj + 2, nj + 2, O.P(x.view(np.float64)))
+    assert rc == 0
+    inner = (slice(1, ni + 1), slice(1, nj + 1))
+    err = np.abs(a["E1p"][inner] - x[inner]).max() / np.abs(x[inner]).max()
+    assert err < 1e-10, err
+
+
+def test_banded_lu_residual():
+    p, lists, o, a = W.mw_coupled_rect(ni=30, nj=14, seed=3)
+    rep = O.mw_solve(p, lists, o, a)
+    assert rep.n_x == 29 * 14 and rep.n_r == 30 * 13 and rep.n_p == 30 * 14
+    assert rep.rel_residual < 1e-12, rep.rel_residual
+    assert np.isfinite(a["E1x"]).all() and np.isfinite(a["E1r"]).all() and np.isfinite(a["E1p"]).all()
