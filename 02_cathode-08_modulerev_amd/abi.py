@@ -545,6 +545,42 @@ def download_fields(D, present):
     return {n: D.download(k) for k, n in enumerate(present)}
 
 
+class MwSolveDevice:
+    """The arrays of a cathode_mw_solve_dev call in device memory, one
+    allocation per array (complex as interleaved float64, masks int32)."""
+
+    def __init__(self, lib, arrays):
+        self.lib, self.bufs, self.f = lib, {}, MwSolveFields()
+        try:
+            for n in MW_SOLVE_FIELD_NAMES:
+                v = arrays.get(n)
+                if v is None:
+                    setattr(self.f, n, None)
+                    continue
+                raw = np.ascontiguousarray(v.view(np.float64) if np.iscomplexobj(v) else v)
+                ptr = ctypes.c_void_p()
+                check(lib.cathode_dev_alloc(raw.nbytes, ctypes.byref(ptr)), lib)
+                self.bufs[n] = (ptr, v.dtype, v.shape, raw.nbytes)
+                check(lib.cathode_dev_copy(ptr, raw.ctypes.data_as(ctypes.c_void_p), raw.nbytes, 0), lib)
+                setattr(self.f, n, ctypes.cast(ptr, c_int_p if n in MW_SOLVE_INT_FIELDS else c_double_p))
+        except Exception:
+            self.free()
+            raise
+
+    def download(self):
+        out = {}
+        for n, (ptr, dt, shape, nb) in self.bufs.items():
+            a = np.zeros(shape, dtype=dt)
+            check(self.lib.cathode_dev_copy(a.ctypes.data_as(ctypes.c_void_p), ptr, nb, 1), self.lib)
+            out[n] = a
+        return out
+
+    def free(self):
+        for ptr, *_ in self.bufs.values():
+            self.lib.cathode_dev_free(ptr)
+        self.bufs = {}
+
+
 def check(rc, lib=None):
     if rc != CATHODE_OK:
         lib = lib or _LIB

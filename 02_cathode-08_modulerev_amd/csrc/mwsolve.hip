@@ -572,26 +572,46 @@ __global__ __launch_bounds__(256) void k_mws_blocks(MwsView V, Fields F, const c
     }
 }
 
-// In-place inverse of one m x m block per workgroup: Gauss-Jordan with partial
-// (row) pivoting, the row interchanges undone as column interchanges at the
-// end.  A zero pivot sets *bad.
-__global__ __launch_bounds__(256) void k_gj_inverse(cxd* const* __restrict__ mats, int m, int* __restrict__ bad)
+// ---------------------------------------------------------------- dense block inverses
+// In-place inverses of a batch of m x m blocks: blocked Gauss-Jordan with
+// partial (row) pivoting.  For each panel of nb columns K = [k0, k0 + nb):
+//  k_gj_panel   one workgroup per block holds the m x nb panel in LDS and runs
+//               the Gauss-Jordan steps of its columns (pivot = largest |a_ik|,
+//               i >= k; row interchange; elimination above and below).  The
+//               panel then holds the columns K of the step transform T, which
+//               differs from the identity only there; the panel's row
+//               interchanges are composed into a gather map of rows k0..m-1.
+//  k_gj_rows    the other columns: rows gathered through the map, rows K
+//               copied out (tmp) and set to 0;
+//  k_gj_update  those columns += T[:, K] tmp (off K a row keeps its own value,
+//               on K it is T[K, K] tmp).
+// At the end the row interchanges come back as column interchanges
+// (k_gj_cols), as in the unblocked in-place inverse.  A zero pivot sets *bad.
+__global__ __launch_bounds__(256) void k_gj_panel(cxd* const* __restrict__ mats, int m, int k0, int nbk, int nb,
+                                                 int* __restrict__ gmap, int* __restrict__ perm, int* __restrict__ bad)
 {
     extern __shared__ unsigned char lds_raw[];
-    cxd* srow = reinterpret_cast<cxd*>(lds_raw);      // scaled pivot row
-    cxd* fcol = srow + m;                             // column k before the step
-    int* perm = reinterpret_cast<int*>(fcol + m);
+    cxd* P = reinterpret_cast<cxd*>(lds_raw);         // m x nb panel
+    cxd* fcol = P + (size_t)m * nb;                   // column kk before the step
+    cxd* srow = fcol + m;                             // scaled pivot row (nb)
+    int* map = reinterpret_cast<int*>(srow + nb);     // gather map of rows k0..m-1
     __shared__ double best_v[4];
     __shared__ int best_i[4];
     __shared__ int piv_row;
     cxd* a = mats[blockIdx.x];
+    int* gm = gmap + (size_t)blockIdx.x * m;
+    int* pm = perm + (size_t)blockIdx.x * m;
     const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
-    for (int k = 0; k < m; k++) {
-        // pivot: largest |a_ik|^2 over i >= k (first index on ties)
+    for (int i = wid; i < m; i += 4)
+        for (int c = lane; c < nbk; c += 64) P[(size_t)i * nb + c] = a[(size_t)i * m + k0 + c];
+    for (int r = k0 + t; r < m; r += 256) map[r - k0] = r;
+    __syncthreads();
+    for (int kk = 0; kk < nbk; kk++) {
+        const int k = k0 + kk;
         double bv = -1.0;
         int bi = m;
         for (int i = k + t; i < m; i += 256) {
-            const double v = abs2v(a[(size_t)i * m + k]);
+            const double v = abs2v(P[(size_t)i * nb + kk]);
             if (v > bv) {
                 bv = v;
                 bi = i;
@@ -623,47 +643,139 @@ __global__ __launch_bounds__(256) void k_gj_inverse(cxd* const* __restrict__ mat
                 p = k;
             }
             piv_row = p;
-            perm[k] = p;
+            pm[k] = p;
+            const int tmp = map[k - k0];
+            map[k - k0] = map[p - k0];
+            map[p - k0] = tmp;
         }
         __syncthreads();
         const int p = piv_row;
-        if (p != k)
-            for (int c = t; c < m; c += 256) {
-                const cxd x = a[(size_t)k * m + c];
-                a[(size_t)k * m + c] = a[(size_t)p * m + c];
-                a[(size_t)p * m + c] = x;
-            }
-        __syncthreads();
-        const cxd akk = a[(size_t)k * m + k];
-        const cxd piv = abs2v(akk) > 0.0 ? cxd(1.0) / akk : cxd(0.0);
-        for (int c = t; c < m; c += 256) {
-            srow[c] = c == k ? piv : a[(size_t)k * m + c] * piv;
-            fcol[c] = a[(size_t)c * m + k];
+        if (p != k && t < nbk) {
+            const cxd x = P[(size_t)k * nb + t];
+            P[(size_t)k * nb + t] = P[(size_t)p * nb + t];
+            P[(size_t)p * nb + t] = x;
         }
         __syncthreads();
-        for (int i = wid; i < m; i += 4) {
-            cxd* ai = a + (size_t)i * m;
-            const cxd fi = fcol[i];
-            for (int c = lane; c < m; c += 64) {
-                if (i == k)
-                    ai[c] = srow[c];
-                else if (c == k)
-                    ai[c] = -(fi * piv);
-                else
-                    ai[c] = ai[c] - fi * srow[c];
-            }
+        const cxd akk = P[(size_t)k * nb + kk];
+        const cxd piv = abs2v(akk) > 0.0 ? cxd(1.0) / akk : cxd(0.0);
+        if (t < nbk) srow[t] = t == kk ? piv : P[(size_t)k * nb + t] * piv;
+        for (int i = t; i < m; i += 256) fcol[i] = P[(size_t)i * nb + kk];
+        __syncthreads();
+        for (int e = t; e < m * nbk; e += 256) {
+            const int i = e / nbk, c = e - i * nbk;
+            cxd& x = P[(size_t)i * nb + c];
+            if (i == k)
+                x = srow[c];
+            else if (c == kk)
+                x = -(fcol[i] * piv);
+            else
+                x = x - fcol[i] * srow[c];
         }
         __syncthreads();
     }
-    for (int k = m - 1; k >= 0; k--) {
-        const int p = perm[k];
-        if (p != k)
-            for (int i = t; i < m; i += 256) {
-                const cxd x = a[(size_t)i * m + k];
-                a[(size_t)i * m + k] = a[(size_t)i * m + p];
-                a[(size_t)i * m + p] = x;
-            }
+    for (int i = wid; i < m; i += 4)
+        for (int c = lane; c < nbk; c += 64) a[(size_t)i * m + k0 + c] = P[(size_t)i * nb + c];
+    for (int r = k0 + t; r < m; r += 256) gm[r] = map[r - k0];
+}
+
+// the panel's row interchanges on the other columns; rows K into tmp (nb x m) and zeroed
+__global__ __launch_bounds__(256) void k_gj_rows(cxd* const* __restrict__ mats, int m, int k0, int nbk, int cw,
+                                                const int* __restrict__ gmap, cxd* __restrict__ tmp, int nb)
+{
+    extern __shared__ unsigned char lds_raw[];
+    cxd* G = reinterpret_cast<cxd*>(lds_raw);   // (m - k0) x cw gathered
+    cxd* a = mats[blockIdx.y];
+    const int* gm = gmap + (size_t)blockIdx.y * m;
+    cxd* tb = tmp + (size_t)blockIdx.y * nb * m;
+    const int c0 = blockIdx.x * cw;
+    const int rows = m - k0;
+    for (int e = threadIdx.x; e < rows * cw; e += 256) {
+        const int r = e / cw, c = c0 + (e - r * cw);
+        if (c < m && (c < k0 || c >= k0 + nbk)) G[e] = a[(size_t)gm[k0 + r] * m + c];
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < rows * cw; e += 256) {
+        const int r = e / cw, c = c0 + (e - r * cw);
+        if (c >= m) continue;
+        if (c >= k0 && c < k0 + nbk) {
+            if (r < nbk) tb[(size_t)r * m + c] = cxd(0.0);
+            continue;
+        }
+        if (r < nbk) {
+            tb[(size_t)r * m + c] = G[e];
+            a[(size_t)(k0 + r) * m + c] = cxd(0.0);
+        } else {
+            a[(size_t)(k0 + r) * m + c] = G[e];
+        }
+    }
+}
+
+// a[:, c] += a[:, K] tmp[:, c] for the columns c outside K; 32 x 32 outputs per workgroup
+__global__ __launch_bounds__(256) void k_gj_update(cxd* const* __restrict__ mats, int m, int k0, int nbk,
+                                                  const cxd* __restrict__ tmp, int nb)
+{
+    __shared__ cxd As[32][17];
+    __shared__ cxd Bs[16][33];
+    cxd* a = mats[blockIdx.z];
+    const cxd* tb = tmp + (size_t)blockIdx.z * nb * m;
+    const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
+    const int row0 = blockIdx.y * 32, col0 = blockIdx.x * 32;
+    if (col0 >= k0 && col0 + 32 <= k0 + nbk) return;   // the whole tile lies in the panel
+    cxd acc[2][2] = {{cxd(0.0), cxd(0.0)}, {cxd(0.0), cxd(0.0)}};
+    for (int q0 = 0; q0 < nbk; q0 += 16) {
+        for (int e = t; e < 512; e += 256) {
+            const int rr = e >> 4, kk = e & 15;
+            const int gr = row0 + rr, gk = q0 + kk;
+            As[rr][kk] = (gr < m && gk < nbk) ? a[(size_t)gr * m + k0 + gk] : cxd(0.0);
+            const int kb = e >> 5, cc = e & 31;
+            const int gkb = q0 + kb, gc = col0 + cc;
+            Bs[kb][cc] = (gkb < nbk && gc < m) ? tb[(size_t)gkb * m + gc] : cxd(0.0);
+        }
         __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < 16; kk++) {
+            const cxd a0 = As[ty * 2][kk], a1 = As[ty * 2 + 1][kk];
+            const cxd b0 = Bs[kk][tx * 2], b1 = Bs[kk][tx * 2 + 1];
+            acc[0][0] = acc[0][0] + a0 * b0;
+            acc[0][1] = acc[0][1] + a0 * b1;
+            acc[1][0] = acc[1][0] + a1 * b0;
+            acc[1][1] = acc[1][1] + a1 * b1;
+        }
+        __syncthreads();
+    }
+    for (int x = 0; x < 2; x++)
+        for (int y = 0; y < 2; y++) {
+            const int gr = row0 + ty * 2 + x, gc = col0 + tx * 2 + y;
+            if (gr >= m || gc >= m || (gc >= k0 && gc < k0 + nbk)) continue;
+            cxd* c = a + (size_t)gr * m + gc;
+            *c = *c + acc[x][y];
+        }
+}
+
+// the row interchanges of all panels undone as column interchanges, last first
+__global__ __launch_bounds__(256) void k_gj_cols(cxd* const* __restrict__ mats, int m, const int* __restrict__ perm)
+{
+    extern __shared__ unsigned char lds_raw[];
+    int* idx = reinterpret_cast<int*>(lds_raw);                        // m
+    cxd* rowbuf = reinterpret_cast<cxd*>(lds_raw + ((m * sizeof(int) + 15) / 16) * 16);   // 4 x m
+    cxd* a = mats[blockIdx.x];
+    const int* pm = perm + (size_t)blockIdx.x * m;
+    const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+    for (int c = t; c < m; c += 256) idx[c] = c;
+    __syncthreads();
+    if (t == 0)
+        for (int k = m - 1; k >= 0; k--) {
+            const int p = pm[k], x = idx[k];
+            idx[k] = idx[p];
+            idx[p] = x;
+        }
+    __syncthreads();
+    cxd* buf = rowbuf + (size_t)wid * m;
+    for (int i = wid; i < m; i += 4) {
+        cxd* ai = a + (size_t)i * m;
+        // each lane writes back only the entries it gathered
+        for (int c = lane; c < m; c += 64) buf[c] = ai[idx[c]];
+        for (int c = lane; c < m; c += 64) ai[c] = buf[c];
     }
 }
 
@@ -987,9 +1099,27 @@ int mws_run(const cathode_mw_params* p, const cathode_mw_bc* bc, const cathode_m
     for (int l = 0; l < L; l++)
         for (size_t t = 1; t < plan.R[l].size(); t += 2) elim[l].push_back(plan.R[l][t]);
     Arena ar;
+    size_t inv_max = 1;
+    for (int l = 0; l < L; l++) inv_max = std::max(inv_max, elim[l].size());
+    // Gauss-Jordan panel width and gather width: the panel (plus pivot buffers) and
+    // the gathered column strip each within 64 KiB of LDS
+    int nb = 32, cw = 32;
+    while (nb > 1 && ((size_t)m * nb + m + nb) * sizeof(cxd) + (size_t)m * sizeof(int) > 65536) nb /= 2;
+    while (cw > 1 && (size_t)m * cw * sizeof(cxd) > 65536) cw /= 2;
+    const size_t panel_lds = ((size_t)m * nb + m + nb) * sizeof(cxd) + (size_t)m * sizeof(int);
+    const size_t cols_lds = ((m * sizeof(int) + 15) / 16) * 16 + 4 * (size_t)m * sizeof(cxd);
+    if (panel_lds > 65536 || (size_t)m * cw * sizeof(cxd) > 65536 || cols_lds > 65536) {
+        set_error("mw_solve: column block of " + std::to_string(m) + " slots exceeds the inverse's LDS buffers");
+        return CATHODE_ERR_ARG;
+    }
+    cxd* tmpb;
+    int *gmapb, *permb;
     auto layout = [&](Arena& a, cxd** co, cxd** Bm, cxd** GA, cxd** GC, cxd** AC, cxd** fv, cxd** gv, cxd** xv,
                       double** part, int** bad, GemmOp** gops, GemvOp** vops, cxd*** mptr, size_t ngemm, size_t ngemv,
                       size_t nparts) {
+        tmpb = a.take<cxd>(inv_max * nb * m);
+        gmapb = a.take<int>(inv_max * m);
+        permb = a.take<int>(inv_max * m);
         *co = a.take<cxd>(3 * NSLOT * n2);
         *Bm = a.take<cxd>(ncol * mm);
         *GA = a.take<cxd>(ncol * mm);
@@ -1197,16 +1327,22 @@ int mws_run(const cathode_mw_params* p, const cathode_mw_bc* bc, const cathode_m
     if (!hv.empty()) CATHODE_HIP(hipMemcpyAsync(vops, hv.data(), hv.size() * sizeof(GemvOp), hipMemcpyHostToDevice, st));
     if (!hm.empty()) CATHODE_HIP(hipMemcpyAsync(mptr, hm.data(), hm.size() * sizeof(cxd*), hipMemcpyHostToDevice, st));
     CATHODE_HIP(hipMemsetAsync(bad, 0, 16 * sizeof(int), st));
-    const size_t gj_lds = (size_t)2 * m * sizeof(cxd) + (size_t)m * sizeof(int);
-    if (gj_lds > 160 * 1024) {
-        set_error("mw_solve: column block of " + std::to_string(m) + " slots exceeds the pivot buffers");
-        return CATHODE_ERR_ARG;
-    }
     auto run = [&](const std::vector<Launch>& seq) {
         for (const Launch& Lc : seq) {
             if (Lc.kind == 0) {
-                hipLaunchKernelGGL(k_gj_inverse, dim3((unsigned)Lc.count), dim3(256), gj_lds, st,
-                                   (cxd* const*)(mptr + Lc.first), m, bad);
+                cxd* const* mats = mptr + Lc.first;
+                const unsigned cnt = (unsigned)Lc.count;
+                const int tiles = cdiv(m, 32);
+                for (int k0 = 0; k0 < m; k0 += nb) {
+                    const int nbk = std::min(nb, m - k0);
+                    hipLaunchKernelGGL(k_gj_panel, dim3(cnt), dim3(256), panel_lds, st, mats, m, k0, nbk, nb, gmapb,
+                                       permb, bad);
+                    hipLaunchKernelGGL(k_gj_rows, dim3(cdiv(m, cw), cnt), dim3(256), (size_t)m * cw * sizeof(cxd), st,
+                                       mats, m, k0, nbk, cw, (const int*)gmapb, tmpb, nb);
+                    hipLaunchKernelGGL(k_gj_update, dim3(tiles, tiles, cnt), dim3(256), 0, st, mats, m, k0, nbk,
+                                       (const cxd*)tmpb, nb);
+                }
+                hipLaunchKernelGGL(k_gj_cols, dim3(cnt), dim3(256), cols_lds, st, mats, m, (const int*)permb);
             } else if (Lc.kind == 1) {
                 const int tiles = cdiv(m, 32);
                 hipLaunchKernelGGL(k_zgemm_batched, dim3(tiles, tiles, (unsigned)Lc.count), dim3(256), 0, st,

@@ -224,6 +224,49 @@ def run_rows(lib, reps=20):
     out["workload"] = (f"five-block cathode at 0.2 mm ({ni}x{nj} cells, the reference's grid), "
                        "device-resident arrays, FP64 solves to 1e-10")
     out["transport_update"] = run_transport(lib)
+    out["mw_solve"] = run_mwsolve(lib)
+    return out
+
+
+def run_mwsolve(lib, reps=5, grid=(194, 70)):
+    """The coupled (E_x, E_r, E_phi) magnetised microwave solve
+    (solve_Microwave) on a grid of the reference cathode's size (194 x 70
+    cells of 0.2 mm) with every boundary rule in its lists, device-resident
+    arrays (cathode_mw_solve_dev): wall time per call after one untimed call
+    (assembly, folds, block cyclic reduction, refinement to 1e-13, ghosts);
+    and the reference's coaxial impedance test (194 x 7, E_r alone)."""
+    ni, nj = grid
+    p, lists, o, a = W.mw_coupled_rect(ni=ni, nj=nj, h=2e-4, seed=7)
+    bc, keep = abi.mw_bc_struct(lists)
+    D = abi.MwSolveDevice(lib, a)
+    rep = abi.MwSolveReport()
+    try:
+        abi.check(lib.cathode_mw_solve_dev(ctypes.byref(p), ctypes.byref(bc), ctypes.byref(o), ctypes.byref(D.f),
+                                           ctypes.byref(rep)), lib)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            abi.check(lib.cathode_mw_solve_dev(ctypes.byref(p), ctypes.byref(bc), ctypes.byref(o),
+                                               ctypes.byref(D.f), ctypes.byref(rep)), lib)
+        t1 = time.perf_counter()
+    finally:
+        D.free()
+    out = {"ms_per_call": sig(1e3 * (t1 - t0) / reps), "calls": reps, "unknowns": rep.n_x + rep.n_r + rep.n_p,
+           "block": rep.block, "columns": ni + 2, "refine_steps": rep.refine, "rel_residual": rep.rel_residual,
+           "workload": f"coupled magnetised microwave solve, {ni}x{nj} cells of 0.2 mm (the reference cathode's "
+                       "grid size), every list rule, device-resident arrays"}
+    p, lists, o, a = W.coax_impedance_test()
+    bc, keep = abi.mw_bc_struct(lists)
+    D = abi.MwSolveDevice(lib, a)
+    try:
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            abi.check(lib.cathode_mw_solve_dev(ctypes.byref(p), ctypes.byref(bc), ctypes.byref(o),
+                                               ctypes.byref(D.f), ctypes.byref(rep)), lib)
+        t1 = time.perf_counter()
+    finally:
+        D.free()
+    out["impedance_test"] = {"ms_per_call": sig(1e3 * (t1 - t0) / reps), "P_fwd_all_W": sig(rep.P_fwd_all),
+                             "workload": "the reference's coaxial impedance test, 194x7, E_r alone"}
     return out
 
 

@@ -39,27 +39,13 @@ def device(lib, p, lists, o, a, dev=False):
         abi.check(lib.cathode_mw_solve(ctypes.byref(p), ctypes.byref(bc), ctypes.byref(o), ctypes.byref(f),
                                        ctypes.byref(rep)), lib)
         return g, rep
-    bufs, f = {}, abi.MwSolveFields()
+    D = abi.MwSolveDevice(lib, g)
     try:
-        for n in abi.MW_SOLVE_FIELD_NAMES:
-            v = g.get(n)
-            if v is None:
-                setattr(f, n, None)
-                continue
-            raw = v.view(np.float64) if np.iscomplexobj(v) else v
-            ptr = ctypes.c_void_p()
-            abi.check(lib.cathode_dev_alloc(raw.nbytes, ctypes.byref(ptr)), lib)
-            bufs[n] = (ptr, raw)
-            abi.check(lib.cathode_dev_copy(ptr, raw.ctypes.data_as(ctypes.c_void_p), raw.nbytes, 0), lib)
-            typ = abi.c_int_p if n in abi.MW_SOLVE_INT_FIELDS else abi.c_double_p
-            setattr(f, n, ctypes.cast(ptr, typ))
-        abi.check(lib.cathode_mw_solve_dev(ctypes.byref(p), ctypes.byref(bc), ctypes.byref(o), ctypes.byref(f),
+        abi.check(lib.cathode_mw_solve_dev(ctypes.byref(p), ctypes.byref(bc), ctypes.byref(o), ctypes.byref(D.f),
                                            ctypes.byref(rep)), lib)
-        for n, (ptr, raw) in bufs.items():
-            abi.check(lib.cathode_dev_copy(raw.ctypes.data_as(ctypes.c_void_p), ptr, raw.nbytes, 1), lib)
+        g = D.download()
     finally:
-        for ptr, _ in bufs.values():
-            lib.cathode_dev_free(ptr)
+        D.free()
     return g, rep
 
 
