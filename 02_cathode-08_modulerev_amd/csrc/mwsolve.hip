@@ -678,6 +678,132 @@ __global__ __launch_bounds__(256) void k_gj_panel(cxd* const* __restrict__ mats,
     for (int r = k0 + t; r < m; r += 256) gm[r] = map[r - k0];
 }
 
+// The same panel with the values in registers: 16 columns, thread t holds
+// column t % 16 of the rows t / 16 + 16 q (q < QMAX, so m <= 16 QMAX); LDS
+// carries only the pivot row, the pivot column and the partial maxima.
+template <int QMAX>
+__global__ __launch_bounds__(256) void k_gj_panel_reg(cxd* const* __restrict__ mats, int m, int k0, int nbk,
+                                                     int* __restrict__ gmap, int* __restrict__ perm,
+                                                     int* __restrict__ bad)
+{
+    constexpr int NB = 16, G = 256 / NB;
+    extern __shared__ unsigned char lds_raw[];
+    cxd* fcol = reinterpret_cast<cxd*>(lds_raw);   // m
+    int* map = reinterpret_cast<int*>(fcol + m);    // m - k0
+    __shared__ cxd rk[NB], rp[NB];
+    __shared__ double best_v[4];
+    __shared__ int best_i[4];
+    __shared__ int piv_row;
+    cxd* a = mats[blockIdx.x];
+    int* gm = gmap + (size_t)blockIdx.x * m;
+    int* pm = perm + (size_t)blockIdx.x * m;
+    const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+    const int c = t % NB, rg = t / NB;
+    const bool col_ok = c < nbk;
+    cxd v[QMAX];
+#pragma unroll
+    for (int q = 0; q < QMAX; q++) {
+        const int i = rg + G * q;
+        v[q] = (i < m && col_ok) ? a[(size_t)i * m + k0 + c] : cxd(0.0);
+    }
+    for (int r = k0 + t; r < m; r += 256) map[r - k0] = r;
+    __syncthreads();
+    for (int kk = 0; kk < nbk; kk++) {
+        const int k = k0 + kk;
+        // pivot: largest |a_ik|^2, i >= k, first index on ties
+        double bv = -1.0;
+        int bi = m;
+        if (c == kk) {
+#pragma unroll
+            for (int q = 0; q < QMAX; q++) {
+                const int i = rg + G * q;
+                if (i >= k && i < m) {
+                    const double x = abs2v(v[q]);
+                    if (x > bv || (x == bv && i < bi)) {
+                        bv = x;
+                        bi = i;
+                    }
+                }
+            }
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            const double ov = __shfl_xor(bv, off, 64);
+            const int oi = __shfl_xor(bi, off, 64);
+            if (ov > bv || (ov == bv && oi < bi)) {
+                bv = ov;
+                bi = oi;
+            }
+        }
+        if (lane == 0) {
+            best_v[wid] = bv;
+            best_i[wid] = bi;
+        }
+        __syncthreads();
+        if (t == 0) {
+            double x = best_v[0];
+            int p = best_i[0];
+            for (int w = 1; w < 4; w++)
+                if (best_v[w] > x || (best_v[w] == x && best_i[w] < p)) {
+                    x = best_v[w];
+                    p = best_i[w];
+                }
+            if (!(x > 0.0)) {
+                *bad = 1;
+                p = k;
+            }
+            piv_row = p;
+            pm[k] = p;
+            const int tmp = map[k - k0];
+            map[k - k0] = map[p - k0];
+            map[p - k0] = tmp;
+        }
+        __syncthreads();
+        const int p = piv_row;
+        // rows k and p through LDS
+#pragma unroll
+        for (int q = 0; q < QMAX; q++) {
+            const int i = rg + G * q;
+            if (i == k) rk[c] = v[q];
+            if (i == p) rp[c] = v[q];
+        }
+        __syncthreads();
+        const cxd nk_kk = p != k ? rp[kk] : rk[kk];   // new pivot row, column kk
+        const cxd nk_c = p != k ? rp[c] : rk[c];
+        const cxd piv = abs2v(nk_kk) > 0.0 ? cxd(1.0) / nk_kk : cxd(0.0);
+        const cxd srow_c = c == kk ? piv : nk_c * piv;
+#pragma unroll
+        for (int q = 0; q < QMAX; q++) {
+            const int i = rg + G * q;
+            if (p != k) {
+                if (i == k) v[q] = rp[c];
+                if (i == p) v[q] = rk[c];
+            }
+            if (c == kk && i < m) fcol[i] = v[q];
+        }
+        __syncthreads();
+        if (col_ok) {
+#pragma unroll
+            for (int q = 0; q < QMAX; q++) {
+                const int i = rg + G * q;
+                if (i >= m) continue;
+                if (i == k)
+                    v[q] = srow_c;
+                else if (c == kk)
+                    v[q] = -(fcol[i] * piv);
+                else
+                    v[q] = v[q] - fcol[i] * srow_c;
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < QMAX; q++) {
+        const int i = rg + G * q;
+        if (i < m && col_ok) a[(size_t)i * m + k0 + c] = v[q];
+    }
+    __syncthreads();
+    for (int r = k0 + t; r < m; r += 256) gm[r] = map[r - k0];
+}
+
 // the panel's row interchanges on the other columns; rows K into tmp (nb x m) and zeroed
 __global__ __launch_bounds__(256) void k_gj_rows(cxd* const* __restrict__ mats, int m, int k0, int nbk, int cw,
                                                 const int* __restrict__ gmap, cxd* __restrict__ tmp, int nb)
@@ -828,7 +954,7 @@ __global__ __launch_bounds__(256) void k_zgemm_batched(const GemmOp* __restrict_
 }
 
 // y = s0 v0 + s1 M1 x1 + s2 M2 x2 over a batch (M2 / v0 may be NULL); one
-// workgroup per op, a wave per row, lanes across the row.
+// workgroup per 16 rows of an op.
 struct GemvOp {
     cxd* y;
     const cxd* v0;
@@ -841,27 +967,36 @@ struct GemvOp {
 
 __global__ __launch_bounds__(256) void k_zgemv_batched(const GemvOp* __restrict__ ops, int m)
 {
-    const GemvOp op = ops[blockIdx.x];
+    // 16 rows per workgroup, 4 per wave; lanes across the row
+    const GemvOp op = ops[blockIdx.y];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    for (int rw = wid; rw < m; rw += 4) {
-        double r1 = 0.0, i1 = 0.0, r2 = 0.0, i2 = 0.0;
-        for (int c = lane; c < m; c += 64) {
-            const cxd a = op.M1[(size_t)rw * m + c] * op.x1[c];
-            r1 += a.re;
-            i1 += a.im;
+    const int r0 = blockIdx.x * 16 + wid * 4;
+    double re1[4] = {0, 0, 0, 0}, im1[4] = {0, 0, 0, 0}, re2[4] = {0, 0, 0, 0}, im2[4] = {0, 0, 0, 0};
+    for (int c = lane; c < m; c += 64) {
+        const cxd xa = op.x1[c];
+        const cxd xb = op.M2 ? op.x2[c] : cxd(0.0);
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int rw = r0 + q;
+            if (rw >= m) continue;
+            const cxd a = op.M1[(size_t)rw * m + c] * xa;
+            re1[q] += a.re;
+            im1[q] += a.im;
             if (op.M2) {
-                const cxd b = op.M2[(size_t)rw * m + c] * op.x2[c];
-                r2 += b.re;
-                i2 += b.im;
+                const cxd b = op.M2[(size_t)rw * m + c] * xb;
+                re2[q] += b.re;
+                im2[q] += b.im;
             }
         }
-        r1 = wave_sum(r1);
-        i1 = wave_sum(i1);
-        r2 = wave_sum(r2);
-        i2 = wave_sum(i2);
-        if (lane == 0) {
-            cxd v = sc(cxd(r1, i1), op.s1);
-            if (op.M2) v = v + sc(cxd(r2, i2), op.s2);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const double a_re = wave_sum(re1[q]), a_im = wave_sum(im1[q]);
+        const double b_re = wave_sum(re2[q]), b_im = wave_sum(im2[q]);
+        const int rw = r0 + q;
+        if (lane == 0 && rw < m) {
+            cxd v = sc(cxd(a_re, a_im), op.s1);
+            if (op.M2) v = v + sc(cxd(b_re, b_im), op.s2);
             if (op.v0) v = v + sc(op.v0[rw], op.s0);
             op.y[rw] = v;
         }
@@ -1103,7 +1238,7 @@ int mws_run(const cathode_mw_params* p, const cathode_mw_bc* bc, const cathode_m
     for (int l = 0; l < L; l++) inv_max = std::max(inv_max, elim[l].size());
     // Gauss-Jordan panel width and gather width: the panel (plus pivot buffers) and
     // the gathered column strip each within 64 KiB of LDS
-    int nb = 32, cw = 32;
+    int nb = m <= 16 * 32 ? 16 : 32, cw = 32;
     while (nb > 1 && ((size_t)m * nb + m + nb) * sizeof(cxd) + (size_t)m * sizeof(int) > 65536) nb /= 2;
     while (cw > 1 && (size_t)m * cw * sizeof(cxd) > 65536) cw /= 2;
     const size_t panel_lds = ((size_t)m * nb + m + nb) * sizeof(cxd) + (size_t)m * sizeof(int);
@@ -1335,8 +1470,16 @@ int mws_run(const cathode_mw_params* p, const cathode_mw_bc* bc, const cathode_m
                 const int tiles = cdiv(m, 32);
                 for (int k0 = 0; k0 < m; k0 += nb) {
                     const int nbk = std::min(nb, m - k0);
-                    hipLaunchKernelGGL(k_gj_panel, dim3(cnt), dim3(256), panel_lds, st, mats, m, k0, nbk, nb, gmapb,
-                                       permb, bad);
+                    const size_t reg_lds = (size_t)m * sizeof(cxd) + (size_t)m * sizeof(int);
+                    if (nb == 16 && m <= 16 * 16)
+                        hipLaunchKernelGGL(k_gj_panel_reg<16>, dim3(cnt), dim3(256), reg_lds, st, mats, m, k0, nbk,
+                                           gmapb, permb, bad);
+                    else if (nb == 16 && m <= 16 * 32)
+                        hipLaunchKernelGGL(k_gj_panel_reg<32>, dim3(cnt), dim3(256), reg_lds, st, mats, m, k0, nbk,
+                                           gmapb, permb, bad);
+                    else
+                        hipLaunchKernelGGL(k_gj_panel, dim3(cnt), dim3(256), panel_lds, st, mats, m, k0, nbk, nb,
+                                           gmapb, permb, bad);
                     hipLaunchKernelGGL(k_gj_rows, dim3(cdiv(m, cw), cnt), dim3(256), (size_t)m * cw * sizeof(cxd), st,
                                        mats, m, k0, nbk, cw, (const int*)gmapb, tmpb, nb);
                     hipLaunchKernelGGL(k_gj_update, dim3(tiles, tiles, cnt), dim3(256), 0, st, mats, m, k0, nbk,
@@ -1348,7 +1491,7 @@ int mws_run(const cathode_mw_params* p, const cathode_mw_bc* bc, const cathode_m
                 hipLaunchKernelGGL(k_zgemm_batched, dim3(tiles, tiles, (unsigned)Lc.count), dim3(256), 0, st,
                                    (const GemmOp*)(gops + Lc.first), m);
             } else {
-                hipLaunchKernelGGL(k_zgemv_batched, dim3((unsigned)Lc.count), dim3(256), 0, st,
+                hipLaunchKernelGGL(k_zgemv_batched, dim3(cdiv(m, 16), (unsigned)Lc.count), dim3(256), 0, st,
                                    (const GemvOp*)(vops + Lc.first), m);
             }
         }

@@ -399,6 +399,21 @@ def cpu_baseline(seed, budget_s=12.0):
             assert fn(ctypes.byref(grid), ctypes.byref(prm), ctypes.byref(f), ctypes.byref(rep), 1) == 0
         rows[name] = {"ms_per_call": 1e3 * (time.perf_counter() - t0) / reps, "calls": reps, "cores": 1}
     lib.orc_ndd_set_geometry(None)
+    # the coupled microwave solve of the device row (194 x 70, every list
+    # rule): the oracle's banded LU with partial pivoting, one call, one thread
+    lib.orc_mw_solve.argtypes = [ctypes.POINTER(abi.MwParams), ctypes.POINTER(abi.MwBc),
+                                 ctypes.POINTER(abi.MwSolveOpts), ctypes.POINTER(abi.MwSolveFields),
+                                 ctypes.POINTER(abi.MwSolveReport)]
+    p, lists, o, a = W.mw_coupled_rect(ni=194, nj=70, h=2e-4, seed=7)
+    bc, keep = abi.mw_bc_struct(lists)
+    f = abi.mw_solve_fields_struct(a)
+    rep = abi.MwSolveReport()
+    t0 = time.perf_counter()
+    assert lib.orc_mw_solve(ctypes.byref(p), ctypes.byref(bc), ctypes.byref(o), ctypes.byref(f),
+                            ctypes.byref(rep)) == 0
+    rows["mw_solve"] = {"ms_per_call": 1e3 * (time.perf_counter() - t0), "calls": 1, "cores": 1,
+                        "algorithm": "banded LU with partial pivoting over the same rows (the reference's Eigen "
+                                     "SparseLU is not installed)", "rel_residual": rep.rel_residual}
     return {"value": va, "unit": "steps/s", "cores": threads_all, "kind": "port", "rows": rows,
             "sample": f"{na} steps of the same 512x256 workload after 1 untimed step, {threads_all} threads",
             "one_core": {"value": v1, "cores": 1, "sample": f"{n1} steps after 1 untimed step"},
