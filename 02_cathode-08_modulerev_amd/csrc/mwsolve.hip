@@ -679,26 +679,26 @@ __global__ __launch_bounds__(256) void k_gj_panel(cxd* const* __restrict__ mats,
 }
 
 // The same panel with the values in registers: 16 columns, thread t holds
-// column t % 16 of the rows t / 16 + 16 q (q < QMAX, so m <= 16 QMAX); LDS
-// carries only the pivot row, the pivot column and the partial maxima.
+// column t / 16 of the rows t % 16 + 16 q (q < QMAX, so m <= 16 QMAX).  The
+// 16 threads of a column are 16 adjacent lanes of one wave, so the pivot
+// search reduces inside that lane group; LDS carries only the pivot row, the
+// pivot column and the chosen pivot.
 template <int QMAX>
 __global__ __launch_bounds__(256) void k_gj_panel_reg(cxd* const* __restrict__ mats, int m, int k0, int nbk,
                                                      int* __restrict__ gmap, int* __restrict__ perm,
                                                      int* __restrict__ bad)
 {
-    constexpr int NB = 16, G = 256 / NB;
+    constexpr int NB = 16, G = 16;
     extern __shared__ unsigned char lds_raw[];
     cxd* fcol = reinterpret_cast<cxd*>(lds_raw);   // m
     int* map = reinterpret_cast<int*>(fcol + m);    // m - k0
     __shared__ cxd rk[NB], rp[NB];
-    __shared__ double best_v[4];
-    __shared__ int best_i[4];
     __shared__ int piv_row;
     cxd* a = mats[blockIdx.x];
     int* gm = gmap + (size_t)blockIdx.x * m;
     int* pm = perm + (size_t)blockIdx.x * m;
-    const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
-    const int c = t % NB, rg = t / NB;
+    const int t = threadIdx.x;
+    const int c = t / G, rg = t % G;
     const bool col_ok = c < nbk;
     cxd v[QMAX];
 #pragma unroll
@@ -710,7 +710,7 @@ __global__ __launch_bounds__(256) void k_gj_panel_reg(cxd* const* __restrict__ m
     __syncthreads();
     for (int kk = 0; kk < nbk; kk++) {
         const int k = k0 + kk;
-        // pivot: largest |a_ik|^2, i >= k, first index on ties
+        // pivot: largest |a_ik|^2, i >= k, first index on ties (column kk's lane group)
         double bv = -1.0;
         int bi = m;
         if (c == kk) {
@@ -719,14 +719,14 @@ __global__ __launch_bounds__(256) void k_gj_panel_reg(cxd* const* __restrict__ m
                 const int i = rg + G * q;
                 if (i >= k && i < m) {
                     const double x = abs2v(v[q]);
-                    if (x > bv || (x == bv && i < bi)) {
+                    if (x > bv) {
                         bv = x;
                         bi = i;
                     }
                 }
             }
         }
-        for (int off = 32; off > 0; off >>= 1) {
+        for (int off = 8; off > 0; off >>= 1) {
             const double ov = __shfl_xor(bv, off, 64);
             const int oi = __shfl_xor(bi, off, 64);
             if (ov > bv || (ov == bv && oi < bi)) {
@@ -734,20 +734,9 @@ __global__ __launch_bounds__(256) void k_gj_panel_reg(cxd* const* __restrict__ m
                 bi = oi;
             }
         }
-        if (lane == 0) {
-            best_v[wid] = bv;
-            best_i[wid] = bi;
-        }
-        __syncthreads();
-        if (t == 0) {
-            double x = best_v[0];
-            int p = best_i[0];
-            for (int w = 1; w < 4; w++)
-                if (best_v[w] > x || (best_v[w] == x && best_i[w] < p)) {
-                    x = best_v[w];
-                    p = best_i[w];
-                }
-            if (!(x > 0.0)) {
+        if (c == kk && rg == 0) {
+            int p = bi;
+            if (!(bv > 0.0)) {
                 *bad = 1;
                 p = k;
             }
@@ -763,8 +752,8 @@ __global__ __launch_bounds__(256) void k_gj_panel_reg(cxd* const* __restrict__ m
 #pragma unroll
         for (int q = 0; q < QMAX; q++) {
             const int i = rg + G * q;
-            if (i == k) rk[c] = v[q];
-            if (i == p) rp[c] = v[q];
+            if (i == k && c < NB) rk[c] = v[q];
+            if (i == p && c < NB) rp[c] = v[q];
         }
         __syncthreads();
         const cxd nk_kk = p != k ? rp[kk] : rk[kk];   // new pivot row, column kk
