@@ -65,6 +65,17 @@
  *   orc_plant                             plants one named mistake; each check above that
  *                                         covers it then fails (tests/test_oracle_pins.py,
  *                                         tests/test_oracle_mw.py)
+ *   orc_mw_solve / orc_mw_rows            the reference's impedance-test output (result_deg0/4.csv,
+ *                                         six digits) and its recorded amplifier forward power;
+ *                                         plasma part of the rows = i omega mu0 dx^2 dr^2 sigma E
+ *                                         of orc_mw_energy_profile; B = 0: E_phi = the orc_helm
+ *                                         E_theta system; every fold with a ghost rule equals that
+ *                                         ghost substituted into the row; Gauss ghosts leave the
+ *                                         boundary cell divergence-free; LU residual at roundoff
+ *                                         (tests/test_oracle_mwsolve.py).  Parity unpinned: the
+ *                                         convex-corner folds (types 5, 6, no ghost rule) and the
+ *                                         E_x slot of the concave E_r folds (one row off their own
+ *                                         ghost rules in the reference, kept as written)
  *   orc_abcd_*                            S -> ABCD -> S round trip is the identity
  * See tests/test_oracle_*.py for each check.
  */
@@ -123,7 +134,8 @@ enum {
     ORC_PLANT_TR_IAT_DENSITY = 23,    /* ion plasma frequency of the IAT rate from n_e */
     ORC_PLANT_MWS_RX_WEIGHT = 24,     /* coupled microwave: E_x entries of the r rows weighted 1/2 */
     ORC_PLANT_MWS_EXCITE_SIGN = 25,   /* coupled microwave: excitation source with -i */
-    ORC_PLANT_MWS_GAUSS_RADII = 26    /* coupled microwave: south Gauss fold of E_r with rL / rR */
+    ORC_PLANT_MWS_GAUSS_RADII = 26,   /* coupled microwave: south Gauss fold of E_r with rL / rR */
+    ORC_PLANT_MWS_CONCAVE_DENO = 27   /* coupled microwave: west concave fold of E_x (type 3) over 1 + dx/dr qR */
 };
 void orc_plant(int which);
 int orc_planted(void);

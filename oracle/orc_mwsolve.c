@@ -233,7 +233,7 @@ static void fold(sys_t* S, const cathode_mw_params* p, const cathode_mw_bc* bc, 
                         *SW -= *W * dx / dr * qL;
                         *W = 0.0;
                     } else if (s == 3) {
-                        double deno = 1.0 + dx / dr * qL;
+                        double deno = 1.0 + dx / dr * (orc_planted() == ORC_PLANT_MWS_CONCAVE_DENO ? qR : qL);
                         *P -= *W / deno;
                         *NW += *W * (dx / dr * qR) / deno;
                         *W = 0.0;

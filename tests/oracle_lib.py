@@ -279,6 +279,7 @@ ORC_PLANT_TR_IAT_DENSITY = 23
 ORC_PLANT_MWS_RX_WEIGHT = 24
 ORC_PLANT_MWS_EXCITE_SIGN = 25
 ORC_PLANT_MWS_GAUSS_RADII = 26
+ORC_PLANT_MWS_CONCAVE_DENO = 27
 MWS_NSLOT = 14
 (MWS_P, MWS_E, MWS_W, MWS_N, MWS_S, MWS_NE, MWS_NW, MWS_SE, MWS_SW, MWS_EE, MWS_WW, MWS_NN, MWS_SS,
  MWS_B) = range(14)

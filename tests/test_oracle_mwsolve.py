@@ -162,3 +162,110 @@ def test_banded_lu_residual():
     assert rep.n_x == 29 * 14 and rep.n_r == 30 * 13 and rep.n_p == 30 * 14
     assert rep.rel_residual < 1e-12, rep.rel_residual
     assert np.isfinite(a["E1x"]).all() and np.isfinite(a["E1r"]).all() and np.isfinite(a["E1p"]).all()
+
+
+# ---- the folds against the ghost rules and div E = 0 ----------------------
+GHOST = {0: (-1, 0), 1: (1, 0), 2: (0, -1), 3: (0, 1)}   # side -> ghost offset (W, E, S, N)
+NO_POST = {("x", 2, 6), ("x", 3, 5), ("x", 3, 6), ("r", 0, 5), ("r", 0, 6), ("r", 1, 5),
+           ("r", 1, 1),   # convex corners (no ghost rule) and the east open E_r ghost (built from gr.Er)
+           ("r", 2, 3), ("r", 3, 3), ("r", 3, 4)}   # concave E_r folds: the E_x slot one row off their
+# ghost rules in the reference (:2678 vs :3185, :2700 vs :3203, :2708 vs :3209), kept as written
+TARGET = {0: {O.MWS_E: (0, 1, 0), O.MWS_W: (0, -1, 0), O.MWS_N: (0, 0, 1), O.MWS_S: (0, 0, -1),
+              O.MWS_NE: (1, 0, 1), O.MWS_NW: (1, -1, 1), O.MWS_SE: (1, 0, 0), O.MWS_SW: (1, -1, 0),
+              O.MWS_EE: (2, 0, 0), O.MWS_WW: (2, -1, 0)},
+          1: {O.MWS_E: (1, 1, 0), O.MWS_W: (1, -1, 0), O.MWS_N: (1, 0, 1), O.MWS_S: (1, 0, -1),
+              O.MWS_NE: (0, 1, 0), O.MWS_SE: (0, 1, -1), O.MWS_NW: (0, 0, 0), O.MWS_SW: (0, 0, -1),
+              O.MWS_NN: (2, 0, 0), O.MWS_SS: (2, 0, -1)},
+          2: {O.MWS_E: (2, 1, 0), O.MWS_W: (2, -1, 0), O.MWS_N: (2, 0, 1), O.MWS_S: (2, 0, -1),
+              O.MWS_EE: (0, 1, 0), O.MWS_WW: (0, 0, 0), O.MWS_NN: (1, 0, 1), O.MWS_SS: (1, 0, 0)}}
+
+
+def fold_identity_error(seed=21):
+    """For every face or cell in the lists whose rules all have a ghost rule:
+    (P_f - P_0) E - sum_k (a_f,k - a_0,k) E_k = b_f - b_0, with the unfolded
+    rows (no lists), the folded ones, and E after the solve and the ghost
+    rules (the fold is the ghost rule substituted into the row)."""
+    p, lists, o, a = W.mw_coupled_rect(ni=20, nj=10, seed=seed)
+    ni, nj = p.ni, p.nj
+    empty = {"x": [[], [], [], []], "r": [[], [], [], []], "p": [[], [], [], []]}
+    c0 = O.mw_rows(p, empty, o, a)
+    cf = O.mw_rows(p, lists, o, a)
+    g = W.copy_arrays(a)
+    O.mw_solve(p, lists, o, g)
+    E = [g["E1x"], g["E1r"], g["E1p"]]
+    act = [a["act_x"], a["act_r"], a["act_p"]]
+    entries = {}
+    for comp, c in (("x", 0), ("r", 1), ("p", 2)):
+        for side in range(4):
+            for (i, j, s) in lists[comp][side]:
+                entries.setdefault((c, i, j), []).append((comp, side, s))
+    worst, n = 0.0, 0
+    for (c, i, j), es in entries.items():
+        if any((comp, side, s) in NO_POST for comp, side, s in es):
+            continue
+        ghosts = {(c, i + GHOST[side][0], j + GHOST[side][1]) for _, side, _ in es}
+        lhs = (cf[c, O.MWS_P, i, j] - c0[c, O.MWS_P, i, j]) * E[c][i, j]
+        ok = True
+        for k, (tc, di, dj) in TARGET[c].items():
+            d = cf[c, k, i, j] - c0[c, k, i, j]
+            if d == 0:
+                continue
+            t = (tc, i + di, j + dj)
+            if t not in ghosts and act[tc][t[1], t[2]] != 1:
+                ok = False   # the reference drops a folded slot whose target is no unknown
+                break
+            lhs -= d * E[tc][t[1], t[2]]
+        if not ok:
+            continue
+        rhs = cf[c, O.MWS_B, i, j] - c0[c, O.MWS_B, i, j]
+        scale = abs(c0[c, O.MWS_P, i, j] * E[c][i, j]) + abs(rhs) + 1e-300
+        worst = max(worst, abs(lhs - rhs) / scale)
+        n += 1
+    return worst, n
+
+
+def test_folds_are_the_ghost_rules_substituted(clean):
+    err, n = fold_identity_error()
+    assert n > 60 and err < 1e-12, (n, err)
+
+
+@pytest.mark.parametrize("plant", [O.ORC_PLANT_MWS_GAUSS_RADII, O.ORC_PLANT_MWS_CONCAVE_DENO])
+def test_fold_identity_catches(clean, plant):
+    O.lib().orc_plant(plant)
+    err, _ = fold_identity_error()
+    assert err > 1e-6, err
+
+
+def test_gauss_ghosts_make_the_boundary_cell_divergence_free():
+    """Type-2 ghosts (:3035-3039, :3061-3065, :3172-3176, :3193-3197): the
+    discrete div E of the cell across the boundary, (E_x(i+1) - E_x(i)) / dx +
+    (r_{j+1/2} E_r(j+1) - r_{j-1/2} E_r(j)) / (r_j dr), is zero."""
+    p, lists, o, a = W.mw_coupled_rect(ni=20, nj=10, seed=8)
+    g = W.copy_arrays(a)
+    O.mw_solve(p, lists, o, g)
+    Ex, Er = g["E1x"], g["E1r"]
+    r = p.rmin + (np.arange(p.nj + 2) - 0.5) * p.dr
+    dx, dr = p.dx, p.dr
+
+    def div(ic, jc):
+        rR, rL = (r[jc] + r[jc + 1]) / 2.0, (r[jc] + r[jc - 1]) / 2.0
+        t1 = (Ex[ic + 1, jc] - Ex[ic, jc]) / dx
+        t2 = (rR * Er[ic, jc + 1] - rL * Er[ic, jc]) / (r[jc] * dr)
+        return abs(t1 + t2) / (abs(t1) + abs(t2) + 1e-300)
+
+    cells = []
+    for (i, j, s) in lists["x"][0]:
+        if s == 2:
+            cells.append((i - 1, j))
+    for (i, j, s) in lists["x"][1]:
+        if s == 2:
+            cells.append((i, j))
+    for (i, j, s) in lists["r"][2]:
+        if s == 2:
+            cells.append((i, j - 1))
+    for (i, j, s) in lists["r"][3]:
+        if s == 2:
+            cells.append((i, j))
+    assert len(cells) > 20
+    worst = max(div(i, j) for i, j in cells)
+    assert worst < 1e-12, worst
