@@ -580,3 +580,150 @@ def mw_coupled_rect(ni=24, nj=12, h=2e-4, seed=11, n0=3e17, B0=0.12, fmw=4.25e9,
     a["J1x_exc"], a["J1r_exc"], a["J1p_exc"] = cx(10.0), cx(10.0), cx(10.0)
     a["E1x"], a["E1r"], a["E1p"] = cx(1.0), cx(1.0), cx(1.0)   # ghost rules read values they do not write
     return p, lists, o, {k: np.ascontiguousarray(v) for k, v in a.items()}
+
+
+# the five-block cathode with its coaxial feed (InitialModule::makeBoundary,
+# include/initialModule.hpp:207-1060): block corners [m]
+MW_XS = {"x1": 0.0e-3, "x2": 5.6e-3 + 7.8e-3, "x3": 10.0e-3 + 7.8e-3, "x4": 15.0e-3 + 7.8e-3,
+         "x5": 20.0e-3 + 7.8e-3, "x6": 0.0e-3 + 7.8e-3}
+MW_RS = {"r1": 0.6e-3, "r2": 2.0e-3, "r3": 4.0e-3, "r4": 9.0e-3, "r5": 2.0e-3, "r6": 14.0e-3}
+EPSR_DIELE = 2.083825280349   # input/setup.yaml epsr_diele
+J1R_EXC = 0.212687            # input/setup.yaml J1r_exc
+
+
+def cathode_mw_boxes(ni, nj, dx, dr, xR=0.0388, rmax=0.014):
+    """The six blocks' inclusive boxes {ilo, ihi, jlo, jhi} of cells, x faces
+    and r faces (:239-316; block 5 is the coaxial feed)."""
+    X, R = MW_XS, MW_RS
+    lo = lambda v, d: int(v / d + 0.5)
+    hi = lambda v, d: int(v / d - 0.5)
+    cell = [(lo(X["x6"], dx) + 1, hi(X["x2"], dx) + 1, lo(R["r3"], dr) + 1, hi(R["r4"], dr) + 1),
+            (lo(X["x2"], dx) + 1, hi(X["x4"], dx) + 1, lo(R["r5"], dr) + 1, hi(R["r4"], dr) + 1),
+            (lo(X["x2"], dx) + 1, hi(X["x3"], dx) + 1, lo(R["r1"], dr) + 1, hi(R["r5"], dr) + 1),
+            (lo(X["x3"], dx) + 1, hi(X["x5"], dx) + 1, lo(0.0, dr) + 1, hi(R["r5"], dr) + 1),
+            (lo(X["x5"], dx) + 1, hi(xR, dx) + 1, lo(0.0, dr) + 1, hi(rmax, dr) + 1),
+            (lo(X["x1"], dx) + 1, hi(X["x2"], dx) + 1, lo(R["r1"], dr) + 1, hi(R["r2"], dr) + 1)]
+    xf = [(lo(X["x6"], dx) + 2, hi(X["x2"], dx) + 2, lo(R["r3"], dr) + 1, hi(R["r4"], dr) + 1),
+          (lo(X["x2"], dx) + 2, hi(X["x4"], dx) + 1, lo(R["r5"], dr) + 1, hi(R["r4"], dr) + 1),
+          (lo(X["x2"], dx) + 2, hi(X["x3"], dx) + 2, lo(R["r1"], dr) + 1, hi(R["r5"], dr) + 1),
+          (lo(X["x3"], dx) + 2, hi(X["x5"], dx) + 2, lo(0.0, dr) + 1, hi(R["r5"], dr) + 1),
+          (lo(X["x5"], dx) + 2, hi(xR, dx) + 1, lo(0.0, dr) + 1, hi(rmax, dr) + 1),
+          (lo(X["x1"], dx) + 2, hi(X["x2"], dx) + 2, lo(R["r1"], dr) + 1, hi(R["r2"], dr) + 1)]
+    rf = [(lo(X["x6"], dx) + 1, hi(X["x2"], dx) + 1, lo(R["r3"], dr) + 2, hi(R["r4"], dr) + 1),
+          (lo(X["x2"], dx) + 1, hi(X["x4"], dx) + 1, lo(R["r5"], dr) + 1, hi(R["r4"], dr) + 1),
+          (lo(X["x2"], dx) + 1, hi(X["x3"], dx) + 1, lo(R["r1"], dr) + 2, hi(R["r5"], dr) + 1),
+          (lo(X["x3"], dx) + 1, hi(X["x5"], dx) + 1, lo(0.0, dr) + 2, hi(R["r5"], dr) + 1),
+          (lo(X["x5"], dx) + 1, hi(xR, dx) + 1, lo(0.0, dr) + 2, hi(rmax, dr) + 1),
+          (lo(X["x1"], dx) + 1, hi(X["x2"], dx) + 1, lo(R["r1"], dr) + 2, hi(R["r2"], dr) + 1)]
+    return cell, xf, rf
+
+
+def cathode_mw_lists(c):
+    """makeBoundary's twelve MicrowaveBC lists (:420-1058) from the cell boxes
+    c[b] = (ilo, ihi, jlo, jhi): {"x": [W, E, S, N], "r": [...], "p": [...]},
+    entries (i, j, type), in the reference's push order."""
+    I0, I1, J0, J1 = 0, 1, 2, 3
+    ci = lambda b, k: c[b][k]
+    G, D, O_, C3, C4, V5, V6, EX = 2, 0, 1, 3, 4, 5, 6, 7
+    xW = [(ci(0, I0) + 1, j, G) for j in range(ci(0, J0) + 1, ci(0, J1))]
+    xW += [(ci(0, I0) + 1, ci(0, J0), C3), (ci(0, I0) + 1, ci(0, J1), C4)]
+    xW += [(ci(2, I0) + 1, j, G) for j in range(ci(5, J1) + 1, ci(0, J0))]
+    xW += [(ci(3, I0) + 1, j, G) for j in range(ci(3, J0), ci(2, J0))]
+    xW += [(ci(4, I0) + 1, j, G) for j in range(ci(1, J0), ci(4, J1))]
+    xW += [(ci(4, I0) + 1, ci(4, J1), C4)]
+    xW += [(ci(5, I0) + 1, j, D) for j in range(ci(5, J0), ci(5, J1) + 1)]
+    xE = [(ci(1, I1), j, G) for j in range(ci(1, J0), ci(1, J1))]
+    xE += [(ci(1, I1), ci(1, J1), C4)]
+    xE += [(ci(4, I1), j, G) for j in range(ci(4, J0), ci(4, J1))]
+    xE += [(ci(4, I1), ci(4, J1), C4)]
+    xS = [(i, ci(0, J0), D) for i in range(ci(0, I0) + 1, ci(0, I1) + 1)]
+    xS += [(ci(0, I1) + 1, ci(0, J0), V6)]
+    xS += [(i, ci(5, J0), D) for i in range(ci(5, I0) + 1, ci(2, I1) + 1)]
+    xS += [(ci(2, I1) + 1, ci(5, J0), V6)]
+    xN = [(i, ci(0, J1), D) for i in range(ci(0, I0) + 1, ci(1, I1) + 1)]
+    xN += [(i, ci(5, J1), D) for i in range(ci(5, I0) + 1, ci(5, I1) + 1)]
+    xN += [(ci(5, I1) + 1, ci(5, J1), V6)]
+    xN += [(i, ci(3, J1), D) for i in range(ci(1, I1) + 2, ci(3, I1) + 1)]
+    xN += [(ci(1, I1) + 1, ci(3, J1), V5), (ci(3, I1) + 1, ci(3, J1), V6)]
+    xN += [(i, ci(4, J1), O_) for i in range(ci(4, I0) + 1, ci(4, I1) + 1)]
+    rW = [(ci(0, I0), j, D) for j in range(ci(0, J0) + 1, ci(0, J1) + 1)]
+    rW += [(ci(1, I0), j, D) for j in range(ci(5, J1) + 2, ci(0, J0))]
+    rW += [(ci(1, I0), ci(5, J1) + 1, V5), (ci(1, I0), ci(0, J0), V6)]
+    rW += [(ci(3, I0), j, D) for j in range(ci(3, J0) + 1, ci(2, J0))]
+    rW += [(ci(3, I0), ci(2, J0), V6)]
+    rW += [(ci(4, I0), j, D) for j in range(ci(1, J0) + 1, ci(4, J1) + 1)]
+    rW += [(ci(4, I0), ci(1, J0), V5)]
+    rW += [(ci(5, I0), j, EX) for j in range(ci(5, J0) + 1, ci(5, J1) + 1)]
+    rE = [(ci(1, I1), j, D) for j in range(ci(1, J0) + 1, ci(1, J1) + 1)]
+    rE += [(ci(1, I1), ci(1, J0), V5)]
+    rE += [(ci(4, I1), j, D) for j in range(ci(4, J0) + 1, ci(4, J1) + 1)]
+    rS = [(i, ci(0, J0) + 1, G) for i in range(ci(0, I0) + 1, ci(0, I1) + 1)]
+    rS += [(ci(0, I0), ci(0, J0) + 1, C3)]
+    rS += [(i, ci(5, J0) + 1, G) for i in range(ci(5, I0), ci(2, I1) + 1)]
+    rN = [(i, ci(0, J1), G) for i in range(ci(0, I0) + 1, ci(1, I1))]
+    rN += [(ci(0, I0), ci(0, J1), C3), (ci(1, I1), ci(0, J1), C4)]
+    rN += [(i, ci(5, J1), G) for i in range(ci(5, I0), ci(5, I1) + 1)]
+    rN += [(i, ci(3, J1), G) for i in range(ci(1, I1) + 1, ci(3, I1) + 1)]
+    rN += [(i, ci(4, J1), G) for i in range(ci(4, I0) + 1, ci(4, I1))]
+    rN += [(ci(4, I0), ci(4, J1), C3), (ci(4, I1), ci(4, J1), C4)]
+    pW = [(ci(0, I0), j, D) for j in range(ci(0, J0), ci(0, J1) + 1)]
+    pW += [(ci(2, I0), j, D) for j in range(ci(5, J1) + 1, ci(0, J0))]
+    pW += [(ci(3, I0), j, D) for j in range(ci(3, J0), ci(2, J0))]
+    pW += [(ci(4, I0), j, D) for j in range(ci(1, J0), ci(4, J1) + 1)]
+    pW += [(ci(5, I0), j, O_) for j in range(ci(5, J0), ci(5, J1) + 1)]
+    pE = [(ci(1, I1), j, D) for j in range(ci(1, J0), ci(1, J1) + 1)]
+    pE += [(ci(4, I1), j, D) for j in range(ci(4, J0), ci(4, J1) + 1)]
+    pS = [(i, ci(0, J0), D) for i in range(ci(0, I0), ci(0, I1) + 1)]
+    pS += [(i, ci(5, J0), D) for i in range(ci(5, I0), ci(2, I1) + 1)]
+    pN = [(i, ci(0, J1), D) for i in range(ci(0, I0), ci(1, I1) + 1)]
+    pN += [(i, ci(5, J1), D) for i in range(ci(5, I0), ci(5, I1) + 1)]
+    pN += [(i, ci(3, J1), D) for i in range(ci(1, I1) + 1, ci(3, I1) + 1)]
+    pN += [(i, ci(4, J1), O_) for i in range(ci(4, I0), ci(4, I1) + 1)]
+    return {"x": [xW, xE, xS, xN], "r": [rW, rE, rS, rN], "p": [pW, pE, pS, pN]}
+
+
+def cathode_mw_case(h=2e-4, seed=12, Bx=None, Br=None, n0=3e17):
+    """The coupled microwave solve on the reference's five-block cathode and
+    coaxial feed (makeBoundary's boxes, masks and lists; input/setup.yaml's
+    4.25 GHz, J1r_exc and dielectric; eps_r of the feed block from
+    makeProfile, include/initialModule.hpp:1595-1600), xL = rmin = 0,
+    38.8 x 14 mm at spacing h.  Bx, Br: the magnet's field on this grid (the
+    reference's input/Bfield_data_0.2mm.csv at h = 0.2 mm, committed as
+    tests/golden/bfield_cathode_0p2mm.npz); None gives a synthetic field of
+    0.02 .. 0.4 T.  The plasma (peak n0, smooth, seeded) fills the five plasma
+    blocks; the feed holds none.  Returns (MwParams, lists, MwSolveOpts, arrays)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    ni, nj = int(round(0.0388 / h)), int(round(0.014 / h))
+    p = _mw_params(ni, nj, h, h, 0.0, 4.25e9)
+    cell, xf, rf = cathode_mw_boxes(ni, nj, h, h)
+    p.n_bl = 6
+    for b in range(6):
+        _set_box(p.cell_box, b, cell[b])
+        _set_box(p.xface_box, b, xf[b])
+        _set_box(p.rface_box, b, rf[b])
+    p.flag_mwRef, p.Pmw, p.Z0 = 1, 15.7024, 50.0
+    p.r1, p.r2 = MW_RS["r1"], MW_RS["r2"]
+    p.J1r_exc = J1R_EXC
+    p.i_ref = int(0.0 / h + 0.5) + 1
+    o = abi.MwSolveOpts()
+    o.which, o.j_flc0, o.nu_eff, o.max_refine, o.tol = abi.MW_COUPLED, cell[0][2], 2.0e9, 3, 1e-13
+    shp = (ni + 2, nj + 2)
+    union = lambda boxes: np.maximum.reduce([_mask(ni, nj, b) for b in boxes]).astype(np.int32)
+    a = {"act_p": union(cell), "act_x": union(xf), "act_r": union(rf)}
+    a["epsr"] = np.ones(shp)
+    c5 = cell[5]
+    a["epsr"][c5[0]:c5[1] + 1, c5[2]:c5[3] + 1] = EPSR_DIELE
+    plasma = np.maximum.reduce([_mask(ni, nj, b) for b in cell[:5]]) > 0
+    a["rhoe"] = np.where(plasma, n0 * (0.05 + smooth_field(rng, ni, nj, h) ** 2), 0.0)
+    a["nu_m"] = 2e9 * (0.5 + smooth_field(rng, ni, nj, h))
+    if Bx is None:
+        x = (np.arange(ni + 2) - 0.5) * h
+        r = (np.arange(nj + 2) - 0.5) * h
+        X, R = np.meshgrid(x, r, indexing="ij")
+        Bx = 0.02 + 0.38 * np.exp(-((X - 0.02) ** 2) / (2 * 0.006 ** 2))
+        Br = 0.1 * (X - 0.02) / 0.006 * np.exp(-((X - 0.02) ** 2) / (2 * 0.006 ** 2)) * R / 0.014
+    a["Bx"], a["Br"] = np.ascontiguousarray(Bx, dtype=np.float64), np.ascontiguousarray(Br, dtype=np.float64)
+    a["Er_dc"] = np.zeros(shp)
+    for k in ("J1x_exc", "J1r_exc", "J1p_exc", "E1x", "E1r", "E1p"):
+        a[k] = np.zeros(shp, dtype=np.complex128)
+    return p, cathode_mw_lists(cell), o, {k: np.ascontiguousarray(v) for k, v in a.items()}

@@ -228,15 +228,20 @@ def run_rows(lib, reps=20):
     return out
 
 
-def run_mwsolve(lib, reps=5, grid=(194, 70)):
-    """The coupled (E_x, E_r, E_phi) magnetised microwave solve
-    (solve_Microwave) on a grid of the reference cathode's size (194 x 70
-    cells of 0.2 mm) with every boundary rule in its lists, device-resident
-    arrays (cathode_mw_solve_dev): wall time per call after one untimed call
-    (assembly, folds, block cyclic reduction, refinement to 1e-13, ghosts);
-    and the reference's coaxial impedance test (194 x 7, E_r alone)."""
-    ni, nj = grid
-    p, lists, o, a = W.mw_coupled_rect(ni=ni, nj=nj, h=2e-4, seed=7)
+BFIELD_0P2MM = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden",
+                            "bfield_cathode_0p2mm.npz")
+
+
+def mw_cathode_case():
+    """The reference cathode's microwave configuration at its 0.2 mm grid with
+    the magnet's field of input/Bfield_data_0.2mm.csv (committed fixture)."""
+    import numpy as np
+    B = np.load(BFIELD_0P2MM)
+    return W.cathode_mw_case(h=2e-4, Bx=B["Bx"], Br=B["Br"])
+
+
+def _time_mwsolve(lib, case, reps):
+    p, lists, o, a = case
     bc, keep = abi.mw_bc_struct(lists)
     D = abi.MwSolveDevice(lib, a)
     rep = abi.MwSolveReport()
@@ -250,13 +255,28 @@ def run_mwsolve(lib, reps=5, grid=(194, 70)):
         t1 = time.perf_counter()
     finally:
         D.free()
-    out = {"ms_per_call": sig(1e3 * (t1 - t0) / reps), "calls": reps, "unknowns": rep.n_x + rep.n_r + rep.n_p,
-           "block": rep.block, "columns": ni + 2, "refine_steps": rep.refine, "rel_residual": rep.rel_residual,
-           "workload": f"coupled magnetised microwave solve, {ni}x{nj} cells of 0.2 mm (the reference cathode's "
-                       "grid size), every list rule, device-resident arrays"}
+    return {"ms_per_call": sig(1e3 * (t1 - t0) / reps), "calls": reps, "unknowns": rep.n_x + rep.n_r + rep.n_p,
+            "block": rep.block, "columns": p.ni + 2, "refine_steps": rep.refine, "rel_residual": rep.rel_residual}
+
+
+def run_mwsolve(lib, reps=5):
+    """The coupled (E_x, E_r, E_phi) magnetised microwave solve
+    (solve_Microwave) with device-resident arrays (cathode_mw_solve_dev): wall
+    time per call after one untimed call (assembly, folds, block cyclic
+    reduction, refinement to 1e-13, ghosts).  Main line: the reference
+    cathode's configuration (makeBoundary's five blocks and coaxial feed,
+    194 x 70 cells of 0.2 mm, the magnet's field); also a 194 x 70 rectangle
+    whose lists run every rule, and the reference's coaxial impedance test
+    (194 x 7, E_r alone)."""
+    out = _time_mwsolve(lib, mw_cathode_case(), reps)
+    out["workload"] = ("coupled magnetised microwave solve on the reference cathode (makeBoundary's blocks and lists, "
+                       "194x70 cells of 0.2 mm, input/Bfield_data_0.2mm.csv), device-resident arrays")
+    out["rectangle"] = _time_mwsolve(lib, W.mw_coupled_rect(ni=194, nj=70, h=2e-4, seed=7), reps)
+    out["rectangle"]["workload"] = "194x70 rectangle, every list rule, synthetic magnetised plasma"
     p, lists, o, a = W.coax_impedance_test()
     bc, keep = abi.mw_bc_struct(lists)
     D = abi.MwSolveDevice(lib, a)
+    rep = abi.MwSolveReport()
     try:
         t0 = time.perf_counter()
         for _ in range(reps):
@@ -399,12 +419,12 @@ def cpu_baseline(seed, budget_s=12.0):
             assert fn(ctypes.byref(grid), ctypes.byref(prm), ctypes.byref(f), ctypes.byref(rep), 1) == 0
         rows[name] = {"ms_per_call": 1e3 * (time.perf_counter() - t0) / reps, "calls": reps, "cores": 1}
     lib.orc_ndd_set_geometry(None)
-    # the coupled microwave solve of the device row (194 x 70, every list
-    # rule): the oracle's banded LU with partial pivoting, one call, one thread
+    # the coupled microwave solve of the device row (the reference cathode at
+    # 0.2 mm): the oracle's banded LU with partial pivoting, one call, one thread
     lib.orc_mw_solve.argtypes = [ctypes.POINTER(abi.MwParams), ctypes.POINTER(abi.MwBc),
                                  ctypes.POINTER(abi.MwSolveOpts), ctypes.POINTER(abi.MwSolveFields),
                                  ctypes.POINTER(abi.MwSolveReport)]
-    p, lists, o, a = W.mw_coupled_rect(ni=194, nj=70, h=2e-4, seed=7)
+    p, lists, o, a = mw_cathode_case()
     bc, keep = abi.mw_bc_struct(lists)
     f = abi.mw_solve_fields_struct(a)
     rep = abi.MwSolveReport()

@@ -127,3 +127,18 @@ def test_argument_errors(hiplib):
     rc = hiplib.cathode_mw_solve(ctypes.byref(p), ctypes.byref(bc), ctypes.byref(o), ctypes.byref(f),
                                  ctypes.byref(rep))
     assert rc == abi.CATHODE_ERR_ARG
+
+
+BFIELD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "bfield_cathode_0p2mm.npz")
+
+
+@pytest.mark.parametrize("h", [4e-4, 2e-4])
+def test_reference_cathode_matches_oracle(hiplib, h):
+    """makeBoundary's geometry and lists; at the reference's 0.2 mm grid with
+    the magnet's field of input/Bfield_data_0.2mm.csv (an ECR surface inside
+    the plasma), at 0.4 mm with a synthetic field."""
+    B = np.load(BFIELD) if h == 2e-4 else None
+    p, lists, o, a = W.cathode_mw_case(h=h, Bx=None if B is None else B["Bx"], Br=None if B is None else B["Br"])
+    print(f"mw_solve reference cathode h={h}", flush=True)
+    got, drep, ref, rrep = compare(hiplib, p, lists, o, a)
+    assert drep.n_x > 1500 and drep.n_r > 1500 and drep.n_p > 1500
