@@ -4,6 +4,10 @@
 //   host_check tables <rateCoef_e> <rateCoef_i> <coefEISEE> <out>
 //                             the rate and yield tables read by host/tables.hpp,
 //                             written to <out> as "name n v0 v1 ..." lines (%.17g)
+//   host_check result <dir> <out.csv>
+//                             the result CSV of host/result_csv.hpp from the fields
+//                             in <dir> (<name>.bin float64, <name>.cbin complex128,
+//                             grid.txt "key value" lines, eloss_n.bin / eloss_m.bin)
 //   host_check gpu            solver_SMG / solver_SOR / solver_LU_9p through the
 //                             drop-in header against the oracle (liboracle.so)
 // Prints one line per check and exits non-zero on the first failure.
@@ -19,6 +23,7 @@
 
 #include "../../02_cathode-08_modulerev_amd/host/cmpMat2.hpp"
 #include "../../02_cathode-08_modulerev_amd/host/restart_csv.hpp"
+#include "../../02_cathode-08_modulerev_amd/host/result_csv.hpp"
 #include "../../02_cathode-08_modulerev_amd/host/setup_yaml.hpp"
 #include "../../02_cathode-08_modulerev_amd/host/solverFuncs.hpp"
 #include "../../02_cathode-08_modulerev_amd/host/tables.hpp"
@@ -335,6 +340,72 @@ static void dump_tables(const char* pe, const char* pi, const char* psee, const 
     std::printf("ok tables\n");
 }
 
+template <class T>
+static std::vector<T> read_bin(const std::string& path)
+{
+    std::ifstream in(path, std::ios::binary);
+    std::vector<T> v;
+    if (!in) return v;
+    in.seekg(0, std::ios::end);
+    const size_t bytes = (size_t)in.tellg();
+    in.seekg(0);
+    v.resize(bytes / sizeof(T));
+    in.read(reinterpret_cast<char*>(v.data()), (std::streamsize)(v.size() * sizeof(T)));
+    return v;
+}
+
+static void write_result(const std::string& dir, const std::string& out)
+{
+    ResultGrid g;
+    std::ifstream gin(dir + "/grid.txt");
+    std::string key;
+    double v;
+    while (gin >> key >> v) {
+        if (key == "ni") g.ni = (int)v;
+        else if (key == "nj") g.nj = (int)v;
+        else if (key == "xL") g.xL = v;
+        else if (key == "dx") g.dx = v;
+        else if (key == "rmin") g.rmin = v;
+        else if (key == "dr") g.dr = v;
+        else if (key == "dt") g.dt = v;
+        else if (key == "ndt_i") g.ndt_i = (int)v;
+        else if (key == "ndt_m") g.ndt_m = (int)v;
+        else if (key == "ndt_n") g.ndt_n = (int)v;
+        else if (key == "masse") g.masse = v;
+        else if (key == "massi") g.massi = v;
+        else if (key == "nu_eff") g.nu_eff = v;
+        else if (key == "deltaECR") g.deltaECR = v;
+        else if (key == "dTe") g.dTe = v;
+        else if (key == "n_bl") g.n_bl = (int)v;
+        else if (key == "box") g.cell_box.push_back((int)v);
+    }
+    std::vector<std::vector<double> > keepr;
+    std::vector<std::vector<std::complex<double> > > keepc;
+    ResultFields F;
+    const std::vector<std::string> real = {
+        "rhoe", "rhoe_old", "rhoi", "rhoi_old", "Uix", "Uir", "Uip", "rhoUix", "rhoUir", "rhoUix_wall",
+        "rhoUir_wall", "rhoUex", "rhoUer", "Te", "rhoeps", "rhoeps_old", "Gx", "Gr", "phi", "Ex", "Er",
+        "rate_ionize", "nu_m", "nu_m1", "nu_ionz", "nu_ionzStep", "nu_excStep", "nu_exc", "nu_excMeta", "nui_m",
+        "mu_para", "mu_perp", "mu_xx", "mu_rr", "mu_xr", "Ap", "Bx", "Br", "Pabs", "rate_eloss", "rhom",
+        "rhom_old", "rhoUmx", "rhoUmr", "rhon", "rhon_old", "rhoUnx", "rhoUnr", "distECR", "scx", "scr", "nUex",
+        "nUer", "nu_super", "nu_ano", "nu_ano_IAT", "jdgBnd_flc", "jdgBnd_Ep"};
+    keepr.reserve(real.size());
+    for (const std::string& n : real) {
+        keepr.push_back(read_bin<double>(dir + "/" + n + ".bin"));
+        if (!keepr.back().empty()) F.real[n] = keepr.back().data();
+    }
+    const std::vector<std::string> cplx = {"E1x", "E1r", "E1p", "J1x", "J1r", "J1p"};
+    keepc.reserve(cplx.size());
+    for (const std::string& n : cplx) {
+        keepc.push_back(read_bin<std::complex<double> >(dir + "/" + n + ".cbin"));
+        if (!keepc.back().empty()) F.cplx[n] = keepc.back().data();
+    }
+    F.eloss_n = read_bin<double>(dir + "/eloss_n.bin");
+    F.eloss_m = read_bin<double>(dir + "/eloss_m.bin");
+    write_result_csv(out, g, F);
+    std::printf("ok result\n");
+}
+
 int main(int argc, char** argv)
 {
     const std::string mode = argc > 1 ? argv[1] : "cpu";
@@ -342,6 +413,8 @@ int main(int argc, char** argv)
         check_abcd();
         check_yaml(argc > 3 ? argv[3] : "tests/golden/setup_reference_layout.yaml");
         check_restart(argc > 2 ? argv[2] : ".");
+    } else if (mode == "result" && argc > 3) {
+        write_result(argv[2], argv[3]);
     } else if (mode == "tables" && argc > 5) {
         dump_tables(argv[2], argv[3], argv[4], argv[5]);
     } else if (mode == "gpu") {
