@@ -103,6 +103,7 @@ class NddReport(ctypes.Structure):
         ("dt", ctypes.c_double), ("CFL", ctypes.c_double), ("dt_min", ctypes.c_double),
         ("rhoe_max", ctypes.c_double), ("Te_max", ctypes.c_double), ("ue_max", ctypes.c_double),
         ("iterations_by_solve", ctypes.c_int * 8),
+        ("cut_by_solve", ctypes.c_int * 8),
     ]
 
 

@@ -113,9 +113,11 @@ struct LevelStage {
 // (an absent child, here = false, reads child (ci0, cj0) and drops out, so
 // every load can be issued before any is tested); writes the coarse level
 // and, with stage_out, the values a further coarsening reads into out
+// Returns the coarse row's excess fraction (aP - sum a_nb) / aP (+inf for an
+// inactive coarse cell).
 template <class Fetch>
-__device__ inline void coarsen_cell_f(const CoarsenArgs& a, int I, int J, double s32, int mixed, Fetch fetch,
-                                      const LevelStage& out, bool stage_out)
+__device__ inline double coarsen_cell_f(const CoarsenArgs& a, int I, int J, double s32, int mixed, Fetch fetch,
+                                        const LevelStage& out, bool stage_out)
 {
     const int i0 = a.fi * (I - 1) + 1, i1 = min(a.fi * I, a.fni);
     const int j0 = a.fj * (J - 1) + 1, j1 = min(a.fj * J, a.fnj);
@@ -149,7 +151,7 @@ __device__ inline void coarsen_cell_f(const CoarsenArgs& a, int I, int J, double
     a.cact[kc] = (unsigned char)any;
     a.cb[kc] = 0.0;
     a.cx[kc] = 0.0;
-    double P;
+    double P, frac = INFINITY;
     if (!any) {
         P = 1.0;
         E = W = N = So = 0.0;
@@ -159,7 +161,9 @@ __device__ inline void coarsen_cell_f(const CoarsenArgs& a, int I, int J, double
         W *= a.hx;
         N *= a.hy;
         So *= a.hy;
-        P = (M + a.hb * (S - M)) + (E + W + N + So);
+        const double ex = M + a.hb * (S - M);
+        P = ex + (E + W + N + So);
+        frac = P > 0.0 ? ex / P : 0.0;
     }
     a.cm[kc] = M;
     a.caP[kc] = P;
@@ -184,6 +188,7 @@ __device__ inline void coarsen_cell_f(const CoarsenArgs& a, int I, int J, double
         out.S[q] = So;
         out.M[q] = M;
     }
+    return frac;
 }
 
 // a child read from the fine level in global memory
@@ -206,9 +211,9 @@ struct GlobalChild {
 };
 
 // the same from the fine level in global memory
-__device__ inline void coarsen_cell(const CoarsenArgs& a, int I, int J, double s32, int mixed)
+__device__ inline double coarsen_cell(const CoarsenArgs& a, int I, int J, double s32, int mixed)
 {
-    coarsen_cell_f(a, I, J, s32, mixed, GlobalChild{a}, LevelStage{}, false);
+    return coarsen_cell_f(a, I, J, s32, mixed, GlobalChild{a}, LevelStage{}, false);
 }
 
 __global__ void k_coarsen(CoarsenArgs a, const double* __restrict__ sc, int mixed)
@@ -411,6 +416,11 @@ struct SetupLevels {
     // the operator is the one of the previous setup (only b changed): the
     // scalars and the warm start's products, no level data
     int rhs_only;
+    // V-cycle cut candidates: per-workgroup minima of the row excess
+    // fraction of level 0 (dpart[bid]) and of level dcl (dpart[dnb + bid];
+    // dcl in 1..nf, else not measured); dpart null: neither
+    double* dpart;
+    int dnb, dcl;
 };
 
 __global__ void __launch_bounds__(SETUP_NT) k_setup_levels(SetupLevels a, double* __restrict__ sc, int mixed)
@@ -496,6 +506,7 @@ __global__ void __launch_bounds__(SETUP_NT) k_setup_levels(SetupLevels a, double
     double wd[WARM_NP];
 #pragma unroll
     for (int q = 0; q < WARM_NP; q++) wd[q] = 0.0;
+    double d0 = INFINITY, dc = INFINITY;  // excess fraction minima of level 0 and level dcl
     {
         const int rows = a.ri * a.pi[0], cols = a.rj * a.pj[0];
         const int i0 = 1 + (int)blockIdx.y * rows, j0 = 1 + (int)blockIdx.x * cols;
@@ -531,6 +542,7 @@ __global__ void __launch_bounds__(SETUP_NT) k_setup_levels(SetupLevels a, double
                 }
                 wd[WARM_NP - 1] += bk * bk;
             }
+            d0 = fmin(d0, a.aP[k] > 0.0 ? a.s[k] / a.aP[k] : 0.0);
             double v = a.s[k];
             if (a.act[k + a.ld]) v = fmin(v, a.s[k + a.ld]);
             if (a.act[k - a.ld]) v = fmin(v, a.s[k - a.ld]);
@@ -571,10 +583,11 @@ __global__ void __launch_bounds__(SETUP_NT) k_setup_levels(SetupLevels a, double
             if (staged) dst.act[q] = 0;
             if (I > c.cni || J > c.cnj) continue;
             if (!staged) {
-                coarsen_cell(c, I, J, s32, mixed);
+                const double f = coarsen_cell(c, I, J, s32, mixed);
+                if (l + 1 == a.dcl) dc = fmin(dc, f);
                 continue;
             }
-            coarsen_cell_f(
+            const double f = coarsen_cell_f(
                 c, I, J, s32, mixed,
                 [&](int i, int j, bool here, int ci0, int cj0) {
                     const int ii = here ? i : ci0, jj = here ? j : cj0;
@@ -591,6 +604,17 @@ __global__ void __launch_bounds__(SETUP_NT) k_setup_levels(SetupLevels a, double
                     return v;
                 },
                 dst, true);
+            if (l + 1 == a.dcl) dc = fmin(dc, f);
+        }
+    }
+    if (a.dpart) {
+        __syncthreads();  // lds is free again
+        d0 = -block_max<SETUP_NT>(-d0, lds);
+        dc = -block_max<SETUP_NT>(-dc, lds);
+        if (tid == 0) {
+            const int bid = (int)(blockIdx.y * gridDim.x + blockIdx.x);
+            a.dpart[bid] = d0;
+            a.dpart[a.dnb + bid] = dc;
         }
     }
 }
@@ -618,6 +642,8 @@ enum {
     SC_TS = 10,
     SC_TT = 11,
     SC_RT = 12,   // |b - A x|^2 (true residual)
+    SC_D0 = 13,   // V-cycle cut candidates: row excess fraction minimum of level 0
+    SC_DC = 14,   // ... and of the second candidate level
     SC_SIDE = 15, // caller's value read with the first round trip (see MGSolver::side_slot)
     SC_COUNT = 16
 };
@@ -754,6 +780,11 @@ struct RestartArgs {
     WarmSet ws;
     const double* wpart;
     int wnb;
+    // the setup launch's excess fraction partials (nullptr: none), reduced
+    // by restart block 0 into sc[SC_D0], sc[SC_DC]
+    const double* dpart;
+    int dnb, dstride;  // workgroups of the setup launch, distance of the second set
+    double* sc;
 };
 
 // NT / (BX BY) blocks of the restart in one workgroup of BX x (NT / BX)
@@ -832,6 +863,19 @@ __device__ __forceinline__ void restart_block(const RestartArgs& A, int blk0)
         A.r0[k] = q;
         A.p[k] = 0.0;
         A.v[k] = 0.0;
+    }
+    if (blk0 == 0 && A.dpart) {
+        double d0 = INFINITY, dc = INFINITY;
+        for (int q = lin_tid(); q < A.dnb; q += NT) {
+            d0 = fmin(d0, A.dpart[q]);
+            dc = fmin(dc, A.dpart[A.dstride + q]);
+        }
+        d0 = -block_max<NT>(-d0, lds);
+        dc = -block_max<NT>(-dc, lds);
+        if (lin_tid() == 0) {
+            A.sc[SC_D0] = d0;
+            A.sc[SC_DC] = dc;
+        }
     }
     // the k_bicg_x layout: (r, r), (r0, r) = (r, r), |b - A x|^2 = (r, r)
     rr = wave_sum(rr);
@@ -1046,6 +1090,8 @@ __global__ void k_bicg_x(int ni, int nj, int ld, const double* __restrict__ aP, 
             if (bid == 0) {
                 host[SC_BB] = sc[SC_BB];
                 host[SC_SIDE] = sc[SC_SIDE];
+                host[SC_D0] = sc[SC_D0];
+                host[SC_DC] = sc[SC_DC];
             }
         }
     }
@@ -1285,6 +1331,14 @@ void MGSolver::build(int ni, int nj)
     grab(kv, 9 * n0);
     grab(cinv64, MG_COARSEST_MAX * MG_COARSEST_MAX);
     if (arena) grab(dmirror, nsc);
+    // V-cycle cut candidates: level 0 and the last level of the first fine
+    // launch unit below it (levels 1 and 2, which run as a pair unless
+    // CATHODE_PAIR=0, or level 1 alone), which the setup launch builds; the
+    // same levels with pairs on or off, so both give the same results
+    tcut = -1;
+    if (lt > 1) tcut = (2 < lt && L64[1].cx && L64[1].cy) ? 2 : 1;
+    dpart_n = (cdiv(ni, 16) + 2) * (cdiv(nj, 32) + 2);  // bounds the setup launch's workgroups
+    grab(dpart, 2 * (size_t)dpart_n);
 }
 
 CoarsenArgs MGSolver::coarsen_args(int l, const double* w)
@@ -1375,6 +1429,16 @@ void MGSolver::setup(int mode, hipStream_t s, const WarmSet& ws)
             a.wpart = wpart.p;
         }
         a.rhs_only = rhs_only ? 1 : 0;
+        // the cut candidates' excess fractions (reduced by the first restart)
+        setup_dnb = (int)(sg.x * sg.y);
+        const bool dmeas = !rhs_only && setup_dnb <= dpart_n;
+        a.dpart = dmeas ? dpart.p : nullptr;
+        a.dnb = dpart_n;
+        a.dcl = (tcut >= 1 && tcut <= nf) ? tcut : -1;
+        if (!rhs_only) {
+            dpart_ok = dmeas;
+            setup_dcl = a.dcl;
+        }
         hipLaunchKernelGGL(k_setup_levels, sg, dim3(SETUP_NT), 0, s, a, scal.p, mixed);
         CATHODE_LAUNCH_CHECK();
     }
@@ -1415,9 +1479,13 @@ void MGSolver::setup(int mode, hipStream_t s, const WarmSet& ws)
     CATHODE_LAUNCH_CHECK();
 }
 
+// cut >= 0: the V-cycle stops at fine level cut (the last level of a launch
+// unit): that level is smoothed from x = 0 by its down and up launches with
+// no coarse correction, and the coarser levels and the tail are skipped.
 template <class T>
-void MGSolver::vcycle(Level<T>* L, hipStream_t s, const VFuse& fdown, const VFuse& fup)
+void MGSolver::vcycle(Level<T>* L, hipStream_t s, const VFuse& fdown, const VFuse& fup, int cut)
 {
+    const int last = cut >= 0 ? cut : lt - 1;  // the last fine level visited
     const double* w = user_w ? user_w : w0.p;
     auto vlevel = [&](int l) {
         Level<T>& A = L[l];
@@ -1426,7 +1494,7 @@ void MGSolver::vcycle(Level<T>* L, hipStream_t s, const VFuse& fdown, const VFus
     // fine levels: fused down kernels; below the finest, a level that
     // coarsens 2x2 into a next fine level runs with it (pair_start[l])
     bool pair_start[MG_MAX_LEVELS] = {};
-    for (int l = 0; l < lt;) {
+    for (int l = 0; l <= last;) {
         Level<T>& A = L[l];
         if (pair_on && l >= 1 && l + 1 < lt && A.cx && A.cy) {
             Level<T>& C = L[l + 1];
@@ -1447,54 +1515,57 @@ void MGSolver::vcycle(Level<T>* L, hipStream_t s, const VFuse& fdown, const VFus
         l++;
     }
     // coarse levels in one workgroup
-    TailArgs<T> ta{};
-    ta.nt = nl - lt;
-    int off = 0;
-    for (int q = 0; q < ta.nt; q++) {
-        Level<T>& A = L[lt + q];
-        ta.ni[q] = A.ni;
-        ta.nj[q] = A.nj;
-        ta.off[q] = off;
-        off += (A.ni + 2) * (A.nj + 2);
-        ta.cx[q] = A.cx;
-        ta.cy[q] = A.cy;
-        ta.gld[q] = A.ld;
-        ta.aP[q] = A.aP;
-        ta.aE[q] = A.aE;
-        ta.aW[q] = A.aW;
-        ta.aN[q] = A.aN;
-        ta.aS[q] = A.aS;
-        ta.act[q] = actall.p + level_off[lt + q];
+    if (cut < 0) {
+        TailArgs<T> ta{};
+        ta.nt = nl - lt;
+        int off = 0;
+        for (int q = 0; q < ta.nt; q++) {
+            Level<T>& A = L[lt + q];
+            ta.ni[q] = A.ni;
+            ta.nj[q] = A.nj;
+            ta.off[q] = off;
+            off += (A.ni + 2) * (A.nj + 2);
+            ta.cx[q] = A.cx;
+            ta.cy[q] = A.cy;
+            ta.gld[q] = A.ld;
+            ta.aP[q] = A.aP;
+            ta.aE[q] = A.aE;
+            ta.aW[q] = A.aW;
+            ta.aN[q] = A.aN;
+            ta.aS[q] = A.aS;
+            ta.act[q] = actall.p + level_off[lt + q];
+        }
+        ta.ncell = off;
+        ta.b_in = L[lt].b;
+        ta.x_out = L[lt].x;
+        ta.w = lt == 0 ? w : nullptr;
+        ta.cn = L[nl - 1].ni * L[nl - 1].nj;
+        // the coarsest level is solved in FP64 in both precisions: the FP32
+        // levels hold s32 A, so their solution is (A^-1 b) / s32
+        ta.cinv = cinv64.p;
+        ta.s32 = sizeof(T) == 4 ? scal.p + SC_S32 : nullptr;
+        const int top = L[lt].ni * L[lt].nj;
+        const size_t lds = tail_lds_bytes<T>(off, top);
+        if (top <= TAILR_NT) launch_tail<T, 1>(ta, lds, s);
+        else launch_tail<T, 2>(ta, lds, s);
     }
-    ta.ncell = off;
-    ta.b_in = L[lt].b;
-    ta.x_out = L[lt].x;
-    ta.w = lt == 0 ? w : nullptr;
-    ta.cn = L[nl - 1].ni * L[nl - 1].nj;
-    // the coarsest level is solved in FP64 in both precisions: the FP32
-    // levels hold s32 A, so their solution is (A^-1 b) / s32
-    ta.cinv = cinv64.p;
-    ta.s32 = sizeof(T) == 4 ? scal.p + SC_S32 : nullptr;
-    const int top = L[lt].ni * L[lt].nj;
-    const size_t lds = tail_lds_bytes<T>(off, top);
-    if (top <= TAILR_NT) launch_tail<T, 1>(ta, lds, s);
-    else launch_tail<T, 2>(ta, lds, s);
     // fine levels: fused up kernels; the coarse correction is the next
     // level's final x (xu of a fine level or of a pair's first level, x of
     // the tail's first level)
-    for (int l = lt - 1; l >= 0;) {
+    for (int l = last; l >= 0;) {
         if (l >= 1 && pair_start[l - 1]) {
             Level<T>& C = L[l];
             Level<T>& D = L[l + 1];
             launch_vpair<T>(pair_cfg, false, s, vlevel(l - 1), vlevel(l), C.cx ? 2 : 1, C.cy ? 2 : 1, D.ni, D.nj, D.ld,
-                            l + 1 < lt ? D.xu : D.x);
+                            l == cut ? nullptr : (l + 1 < lt ? D.xu : D.x));
             l -= 2;
             continue;
         }
         Level<T>& A = L[l];
         Level<T>& C = L[l + 1];
         launch_vtile<T, false>(tile_of<T>(l), l == 0, s, vlevel(l), A.cx ? 2 : 1, A.cy ? 2 : 1, (const double*)nullptr,
-                               C.ni, C.nj, C.ld, l + 1 < lt ? C.xu : C.x, l == 0 ? fup : VFuse());
+                               C.ni, C.nj, C.ld, l == cut ? nullptr : (l + 1 < lt ? C.xu : C.x),
+                               l == 0 ? fup : VFuse());
         l--;
     }
     CATHODE_LAUNCH_CHECK();
@@ -1646,11 +1717,13 @@ void MGSolver::drop_graphs()
     graphs.clear();
 }
 
-const MGSolver::IterGraph* MGSolver::find_graph(int mixed, int parity, const double* xa, const double* pa) const
+const MGSolver::IterGraph* MGSolver::find_graph(int mixed, int parity, const double* xa, const double* pa,
+                                                int cut) const
 {
     const double* w = user_w ? user_w : w0.p;
     for (const auto& gr : graphs) {
-        if (gr.mixed != mixed || gr.parity != parity || gr.xa != xa || gr.pa != pa || gr.w != w) continue;
+        if (gr.mixed != mixed || gr.parity != parity || gr.xa != xa || gr.pa != pa || gr.w != w || gr.cut != cut)
+            continue;
         if (std::memcmp(gr.tiles, tiles[mixed], sizeof(gr.tiles)) != 0) continue;
         return &gr;
     }
@@ -1663,7 +1736,7 @@ const MGSolver::IterGraph* MGSolver::find_graph(int mixed, int parity, const dou
 // the capture has been ended.
 template <class Fn>
 const MGSolver::IterGraph* MGSolver::capture_graph(int mixed, int parity, const double* xa, const double* pa,
-                                                   hipStream_t s, Fn&& enqueue)
+                                                   int cut, hipStream_t s, Fn&& enqueue)
 {
     if (graphs.size() >= 8) drop_graphs();
     if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
@@ -1697,6 +1770,7 @@ const MGSolver::IterGraph* MGSolver::capture_graph(int mixed, int parity, const 
     ig.exec = ex;
     ig.mixed = mixed;
     ig.parity = parity;
+    ig.cut = cut;
     ig.xa = xa;
     ig.pa = pa;
     ig.w = user_w ? user_w : w0.p;
@@ -1718,6 +1792,12 @@ int MGSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_sta
     WarmSet ws = (keep_guess || use_nine) ? WarmSet() : warm;
     warm = WarmSet();
     setup(mode, s, ws);
+    // the V-cycle cut of this solve (the caller's hint, when this hierarchy
+    // offers it)
+    const int cut = cut_valid(cut_hint) ? cut_hint : -1;
+    cut_hint = -1;
+    cut_nx = -1;
+    bool pulled = false;
     {
         const int q = mixed ? 1 : 0;
         const char* force = std::getenv("CATHODE_TILE");
@@ -1798,8 +1878,8 @@ int MGSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_sta
         fu.out = v;
         fu.dotvec = r0;
         fu.want_tt = 0;
-        if (mixed) vcycle<float>(L32, s, fd, fu);
-        else vcycle<double>(L64, s, fd, fu);
+        if (mixed) vcycle<float>(L32, s, fd, fu, cut);
+        else vcycle<double>(L64, s, fd, fu, cut);
         fd.pro = 2;
         fd.first = 0;
         fd.pin = nullptr;
@@ -1811,8 +1891,8 @@ int MGSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_sta
         fu.out = t;
         fu.dotvec = sv;
         fu.want_tt = 1;
-        if (mixed) vcycle<float>(L32, s, fd, fu);
-        else vcycle<double>(L64, s, fd, fu);
+        if (mixed) vcycle<float>(L32, s, fd, fu, cut);
+        else vcycle<double>(L64, s, fd, fu, cut);
         hipLaunchKernelGGL(k_bicg_x, g, blk, 0, s, F.ni, F.nj, F.ld, F.aP, F.aE, F.aW, F.aN, F.aS, n9, n0, bsol.p,
                            act0.p, xa_, xb_, r, r0, ph, sh, sv, t, scal.p, part3, nb, (const double*)part.p, ntiles,
                            host_dst());
@@ -1839,7 +1919,10 @@ int MGSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_sta
             ra.r = r; ra.r0 = r0; ra.p = pa; ra.v = v; ra.part = part3; ra.nb = nb;
             ra.xout = (zero_guess || ws.k > 0) ? xa : nullptr;
             ra.ws = ws; ra.wpart = wpart.p; ra.wnb = warm_nb;
-            if (chain_pending) {
+            ra.dpart = dpart_ok ? dpart.p : nullptr; ra.dnb = setup_dnb; ra.dstride = dpart_n; ra.sc = scal.p;
+            // a cut V-cycle reads no tail level: the chain stays pending
+            // until a solve on this hierarchy needs it
+            if (chain_pending && cut < 0) {
                 chain_pending = false;
                 hipLaunchKernelGGL(k_restart_chain, dim3(1 + cdiv(g.x * g.y, 2)), dim3(BX, RC_NT / BX), 0, s, ra,
                                    pend_ch, (const double*)scal.p, pend_mixed, pend_ci);
@@ -1856,8 +1939,8 @@ int MGSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_sta
             const int first = restart ? 1 : 0;
             if (!first && graphs_on) {
                 // later iterations replay a captured copy of the same launches
-                const IterGraph* gr = find_graph(mixed, it & 1, xa, pa);
-                if (!gr) gr = capture_graph(mixed, it & 1, xa, pa, s, [&] { fused_iter(first, xa, xb, pa, pb); });
+                const IterGraph* gr = find_graph(mixed, it & 1, xa, pa, cut);
+                if (!gr) gr = capture_graph(mixed, it & 1, xa, pa, cut, s, [&] { fused_iter(first, xa, xb, pa, pb); });
                 if (gr) CATHODE_HIP(hipGraphLaunch(gr->exec, s));
                 else fused_iter(first, xa, xb, pa, pb);
             } else {
@@ -1913,6 +1996,7 @@ int MGSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_sta
         // summed here in a fixed order
         CATHODE_HIP(hipStreamSynchronize(s));
         pull_host(SC_COUNT + 3 * (size_t)nb, s);
+        pulled = true;
         {
             double rr = 0.0, rt2 = 0.0;
             const double* hp = h_scal + SC_COUNT;
@@ -1976,7 +2060,15 @@ int MGSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_sta
         const double per_cell = fused ? 7.25 * vb + 1.0 + 8.0 * (2.5 + 1.0) : 7.25 * vb + 1.0;
         prof.bytes_per_launch = per_cell * (double)F.ni * (double)F.nj;
     }
-    flush_chain(s);  // a solve that ran no restart still leaves a whole hierarchy
+    // a solve that ran no restart still leaves a whole hierarchy (a cut
+    // solve leaves the chain to the next solve that needs it)
+    if (cut < 0) flush_chain(s);
+    // the cut the measured excess fractions allow for the next solve of
+    // this kind (the restart reduced them, the x update copied them out)
+    if (pulled && dpart_ok && !aborted) {
+        if (cut_valid(0) && h_scal[SC_D0] >= CUT_DELTA0) cut_nx = 0;
+        else if (setup_dcl > 0 && cut_valid(setup_dcl) && h_scal[SC_DC] >= CUT_DELTAC) cut_nx = setup_dcl;
+    }
     auto t1 = std::chrono::steady_clock::now();
     if (st) {
         st->iterations = it;

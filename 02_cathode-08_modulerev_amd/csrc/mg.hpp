@@ -15,6 +15,9 @@
 //   transfers  w-weighted sum restriction, piecewise-constant prolongation;
 //   coarsest   the first level of <= 32 cells, solved with its dense
 //              inverse (built at setup by Gauss-Jordan elimination);
+//   cut        a mass-dominated operator may stop its V-cycles at level 0
+//              or at level tcut (MGSolver::cut_hint): that level is smoothed
+//              with no coarse correction and no coarser launch runs;
 //   kernels    one fused down kernel (2 sweeps + residual + restriction) and
 //              one fused up kernel (prolongation + 2 sweeps) per fine level,
 //              and one workgroup for all levels of <= 512 cells (vcycle.hpp).
@@ -37,6 +40,11 @@ constexpr int MG_COARSEST_MAX = 32;
 constexpr int CINV_LD = MG_COARSEST_MAX + 1;
 
 constexpr int MG_MAX_LEVELS = 24;
+
+// V-cycle cuts (MGSolver::cut_hint): the excess fraction a level needs for
+// the V-cycle to stop there, level 0 and the coarser candidate
+constexpr double CUT_DELTA0 = 0.9;
+constexpr double CUT_DELTAC = 0.35;
 
 template <class T>
 struct Level {
@@ -153,6 +161,22 @@ public:
     // iterating up to the first check (a more accurate answer, never a
     // less accurate one); one that needs more continues with checks.
     int expected_its = 0;
+    // Where the next solve cuts its V-cycles (the caller's choice, from the
+    // cut_next() of the same kind of solve one time step earlier; clears
+    // itself): -1 visits every level; a cut the solver offers (level 0, or
+    // the second candidate described at tcut) is the last level visited,
+    // smoothed from x = 0 with no coarse-grid correction below it, so the
+    // V-cycle skips the launches of the coarser levels.  Any other value is
+    // ignored.  The converged
+    // answer is that of the full V-cycle to the solve's tolerance; only the
+    // iteration count can change.
+    int cut_hint = -1;
+    // After a solve: the cut its operator allows, from the row excess
+    // fractions delta_l = min over active cells of (aP - sum a_nb) / aP the
+    // setup measured on the candidate levels: level 0 when delta_0 >= 0.9
+    // (a mass-dominated system: smoothing alone converges), else the second
+    // candidate when its delta >= 0.35, else -1.
+    int cut_next() const { return cut_nx; }
     // Hands the solution buffer to the caller in exchange for buf (same
     // size, its own allocation, zero ghosts): no copy.  Kernels already
     // launched with the old sol() pointer still read the solution.
@@ -206,7 +230,7 @@ private:
     void setup(int mode, hipStream_t s, const WarmSet& ws);
     struct CoarsenArgs coarsen_args(int l, const double* w);
     template <class T>
-    void vcycle(Level<T>* L, hipStream_t s, const VFuse& fdown = VFuse(), const VFuse& fup = VFuse());
+    void vcycle(Level<T>* L, hipStream_t s, const VFuse& fdown = VFuse(), const VFuse& fup = VFuse(), int cut = -1);
     template <class T>
     int tile_of(int l) const { return tiles[sizeof(T) == 4 ? 1 : 0][l]; }
     template <class T>
@@ -222,6 +246,19 @@ private:
     CoarsenChain pend_ch{};
     CoarseInv pend_ci{};
     void flush_chain(hipStream_t s);
+    // V-cycle cuts: the candidates are level 0 and tcut (level 2 when level
+    // 1 coarsens in both directions and level 2 is a fine level, else level
+    // 1; -1 when level 1 is in the tail).  The setup launch leaves per-workgroup
+    // minima of delta_0 and delta_tcut in dpart; the restart reduces them
+    // into the scalar slots the host reads.
+    int tcut = -1;
+    int cut_nx = -1;
+    DevBuf<double> dpart;
+    int dpart_n = 0;    // partials per level the buffer holds
+    int setup_dnb = 0;  // workgroups of the last setup launch
+    bool dpart_ok = false;  // the last full setup measured the fractions
+    int setup_dcl = -1;     // the level of the second fraction (-1: not measured)
+    bool cut_valid(int c) const { return !use_nine && lt > 0 && (c == 0 || (c > 0 && c == tcut)); }
 
     int nl = 0;
     int cfg_ni = -1, cfg_nj = -1;
@@ -242,14 +279,14 @@ private:
     // by everything their arguments depend on
     struct IterGraph {
         hipGraphExec_t exec = nullptr;
-        int mixed = 0, parity = 0;
+        int mixed = 0, parity = 0, cut = -1;
         const double *xa = nullptr, *pa = nullptr, *w = nullptr;
         int tiles[MG_MAX_LEVELS] = {};
     };
     std::vector<IterGraph> graphs;
-    const IterGraph* find_graph(int mixed, int parity, const double* xa, const double* pa) const;
+    const IterGraph* find_graph(int mixed, int parity, const double* xa, const double* pa, int cut) const;
     template <class Fn>
-    const IterGraph* capture_graph(int mixed, int parity, const double* xa, const double* pa, hipStream_t s,
+    const IterGraph* capture_graph(int mixed, int parity, const double* xa, const double* pa, int cut, hipStream_t s,
                                    Fn&& enqueue);
     void drop_graphs();
     bool tuned[2] = {false, false};

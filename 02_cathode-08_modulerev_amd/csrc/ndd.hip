@@ -1114,9 +1114,12 @@ static bool solve(NddHandle& H, double gate, int maxITR, double tol, int slot)
     cathode_status st{};
     H.mg.side_cnv = gate;
     if (H.p.learned) H.mg.expected_its = H.last_its[slot];
+    if (H.cut_on) H.mg.cut_hint = H.cut[slot];
+    H.rep.cut_by_solve[slot] = H.cut_on ? H.cut[slot] : -1;
     H.mg.solve(maxITR, tol, H.p.mode, H.s, &st);
     H.mg.side_cnv = -INFINITY;
     if (H.mg.aborted) return false;
+    if (H.cut_on) H.cut[slot] = H.mg.cut_next();
     H.last_its[slot] = st.iterations;
     H.rep.iterations_by_solve[slot] += st.iterations;
     account(H, st);
@@ -1493,6 +1496,10 @@ int cathode_ndd_create(const cathode_ndd_params* p, void** handle)
         }
         std::unique_ptr<NddHandle> H(new NddHandle());
         H->p = *p;
+        {
+            const char* e = std::getenv("CATHODE_CUT");
+            H->cut_on = !(e && e[0] == '0');
+        }
         // each instance has its own stream so independent instances can run
         // side by side (cathode_ndd_batch_step)
         CATHODE_HIP(hipStreamCreateWithFlags(&H->s, hipStreamNonBlocking));
@@ -1677,6 +1684,7 @@ int cathode_ndd_set_state(void* handle, const cathode_ndd_fields* host)
         H->have_xprev = false;  // a new state: no previous Poisson correction
         for (int& v : H->nhist) v = 0;
         for (int& v : H->last_its) v = 0;
+        for (int& v : H->cut) v = -1;
         H->net_useful = true;
         H->gate_pending = false;
         H->net_wait = 0;

@@ -228,6 +228,10 @@ struct NddHandle {
     DevBuf<double> wbuf[8][WARM_HIST];
     int nhist[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     int last_its[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // each solve's iteration count one step earlier
+    // each solve's V-cycle cut, from its operator one step earlier (-1: every
+    // level); CATHODE_CUT=0 keeps every V-cycle whole
+    int cut[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
+    bool cut_on = true;
     double* cap_feat = nullptr;       // capture for training (device, caller-owned)
     double* cap_target = nullptr;
     int guesses = 0, guesses_used = 0;

@@ -11,6 +11,8 @@
 //           red-black sweep produces.
 //   k_up    the same tile with a halo of 4: x = x_down + x_coarse(parent)
 //           (piecewise-constant prolongation), then two red-black sweeps.
+//           In a V-cycle cut at this level (no coarser level visited) the
+//           coarse array is null and x = x_down.
 // Coarse levels (<= TAIL_MAX_CELLS cells each) run in one workgroup, all in
 // LDS (k_tail), down to a single coarsest cell solved by one division.
 #pragma once
@@ -118,8 +120,9 @@ __global__ void __launch_bounds__(NT)
                     cB[k] = F.b[g];
                 }
                 if (!DOWN) {
+                    // no coarse level below (a hierarchy cut at this level): no correction
                     const int I = fi == 2 ? (gi + 1) >> 1 : gi, J = fj == 2 ? (gj + 1) >> 1 : gj;
-                    xv = F.x[g] + cbx[(size_t)I * cld + J];
+                    xv = cbx ? F.x[g] + cbx[(size_t)I * cld + J] : F.x[g];
                 }
             }
         }
@@ -545,7 +548,7 @@ __global__ void __launch_bounds__(NT)
             const size_t g = (size_t)gi * C.ld + gj;
             Gc.cB[k] = C.b[g];
             const int I = fi3 == 2 ? (gi + 1) >> 1 : gi, J = fj3 == 2 ? (gj + 1) >> 1 : gj;
-            xv = C.x[g] + cx3[(size_t)I * c3ld + J];
+            xv = cx3 ? C.x[g] + cx3[(size_t)I * c3ld + J] : C.x[g];  // cx3 null: C is the last level
         }
         xc[c] = xv;
     }

@@ -424,6 +424,11 @@ typedef struct cathode_ndd_report {
      * [2 eq + pass], eq 0 electrons, 1 ions, 2 energy, 3 Poisson; pass 0 the
      * first PC pass, 1 the later ones */
     int iterations_by_solve[8];
+    /* the V-cycle cut each solve of the last step ran with, same order (-1:
+     * every level; l >= 0: the V-cycles stopped at level l, smoothed there
+     * with no coarser correction -- a mass-dominated operator, judged from
+     * the same solve one step earlier).  The oracle leaves it 0. */
+    int cut_by_solve[8];
 } cathode_ndd_report;
 
 int cathode_ndd_create(const cathode_ndd_params* p, void** handle);

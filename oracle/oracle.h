@@ -179,6 +179,10 @@ int orc_mg_solve(const double* aP, const double* aE, const double* aW,
                  const int* boxes, int n_bl, int ni2, int nj2, int ld,
                  int maxITR, double tol, double* x, double* relres);
 void orc_mg_set_threads(int nthreads);
+/* V-cycle cuts of mass-dominated operators on (1, the default) or off */
+void orc_mg_set_cut(int on);
+/* the cut the last orc_mg_solve ran with (-1: none) */
+int orc_mg_last_cut(void);
 
 /* ---- implicit NDD step (fluidModule.hpp:1405-1755 ion (restated as SG at
  * charge +1), :2080-3075 Poisson, :3082-4000 electrons, :4006-4957 energy).

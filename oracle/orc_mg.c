@@ -23,6 +23,16 @@
  *   coarsest   the first level of <= 32 cells, solved by dense LU with
  *              partial pivoting (the device inverts it by Gauss-Jordan
  *              elimination without pivoting: the same solution to rounding);
+ *   cut        a mass-dominated operator stops its V-cycles early: with the
+ *              row excess fraction delta_l = min over active cells of
+ *              (aP - sum a_nb) / aP, level 0 when delta_0 >= 0.9, else the
+ *              device's second candidate level (the last level of its first
+ *              fine launch unit below level 0) when its delta >= 0.35.  The
+ *              cut level is smoothed (2 + 2 sweeps from x = 0) with no
+ *              coarse-grid correction.  The device decides from the same
+ *              solve one time step earlier (its launches are enqueued before
+ *              the setup's fractions reach the host); this copy from the
+ *              solve's own operator;
  *   outer      BiCGStab right-preconditioned by one V-cycle per application,
  *              from x = 0 (solverFuncs.hpp:106-109), stopping when the true
  *              residual ||b-Ax||_2/||b||_2 <= tol or after maxITR iterations.
@@ -50,6 +60,10 @@ typedef struct {
 
 static int g_threads = 1;
 void orc_mg_set_threads(int n) { g_threads = n > 0 ? n : 1; }
+static int g_cut_on = 1;
+static int g_last_cut = -1;
+void orc_mg_set_cut(int on) { g_cut_on = on != 0; }
+int orc_mg_last_cut(void) { return g_last_cut; }
 void orc_mg_set_coarse_max(int n) { MG_COARSE_MAX = n > 0 ? n : 1; }
 
 #define L_AT(L, a, i, j) ((L)->a[(size_t)(i) * (L)->ld + (j)])
@@ -293,11 +307,19 @@ static void dense_solve(level_t* L, const dense_t* D)
     free(v);
 }
 
-static void vcycle(level_t* Ls, int l, int nl, const dense_t* D)
+static void vcycle(level_t* Ls, int l, int nl, const dense_t* D, int cut)
 {
     level_t* L = &Ls[l];
     if (l == nl - 1) {
         dense_solve(L, D);
+        return;
+    }
+    if (l == cut) {
+        /* the last level visited: pre- and post-smoothing, no correction */
+        for (int s = 0; s < 4; s++) {
+            smooth(L, 0);
+            smooth(L, 1);
+        }
         return;
     }
     for (int s = 0; s < 2; s++) {
@@ -306,7 +328,7 @@ static void vcycle(level_t* Ls, int l, int nl, const dense_t* D)
     }
     residual(L);
     restrict_res(L, &Ls[l + 1]);
-    vcycle(Ls, l + 1, nl, D);
+    vcycle(Ls, l + 1, nl, D, cut);
     prolong_add(L, &Ls[l + 1]);
     for (int s = 0; s < 2; s++) {
         smooth(L, 0);
@@ -338,14 +360,48 @@ static void apply_A(level_t* L, const double* in, double* out)
 }
 
 /* out = V-cycle(in) from a zero initial guess */
-static void precond(level_t* Ls, int nl, const dense_t* D, const double* in, double* out)
+static void precond(level_t* Ls, int nl, const dense_t* D, int cut, const double* in, double* out)
 {
     level_t* L0 = &Ls[0];
     size_t n0 = (size_t)(L0->ni + 2) * L0->ld;
     for (size_t k = 0; k < n0; k++) L0->b[k] = L0->act[k] ? in[k] : 0.0;
     memset(L0->x, 0, n0 * sizeof(double));
-    vcycle(Ls, 0, nl, D);
+    vcycle(Ls, 0, nl, D, cut);
     memcpy(out, L0->x, n0 * sizeof(double));
+}
+
+/* minimum row excess fraction (aP - sum a_nb) / aP over the active cells */
+static double excess_fraction(const level_t* L)
+{
+    double d = INFINITY;
+    for (int i = 1; i <= L->ni; i++)
+        for (int j = 1; j <= L->nj; j++) {
+            if (!L_AT(L, act, i, j)) continue;
+            double a = L_AT(L, aP, i, j);
+            double e = a - (L_AT(L, aE, i, j) + L_AT(L, aW, i, j) + L_AT(L, aN, i, j) + L_AT(L, aS, i, j));
+            double f = a > 0.0 ? (e > 0.0 ? e : 0.0) / a : 0.0;
+            if (f < d) d = f;
+        }
+    return d;
+}
+
+/* The V-cycle cut (-1: none) with the device's candidate levels: level 0
+ * when the finest level runs in the fine-level kernels (the one-workgroup
+ * tail takes the levels from lt on: a first level of <= 2048 cells and at
+ * most four further ones of <= 1024), and tcut, the last level of the first
+ * launch unit below level 0 (levels 1 and 2 together when level 1 coarsens
+ * in both directions and 2 < lt, else level 1), when 1 <= tcut < lt. */
+static int mg_cut(const level_t* Ls, int nl)
+{
+    int lt = nl - 1;
+    while (lt > 0 && (long)Ls[lt - 1].ni * Ls[lt - 1].nj <= 2048 && (long)Ls[lt].ni * Ls[lt].nj <= 1024 &&
+           nl - (lt - 1) <= 5)
+        lt--;
+    if (lt <= 0) return -1;
+    if (excess_fraction(&Ls[0]) >= 0.9) return 0;
+    if (lt <= 1) return -1;
+    int tcut = (2 < lt && Ls[1].cx && Ls[1].cy) ? 2 : 1;
+    return excess_fraction(&Ls[tcut]) >= 0.35 ? tcut : -1;
 }
 
 int orc_mg_solve(const double* aP, const double* aE, const double* aW, const double* aN,
@@ -386,6 +442,8 @@ int orc_mg_solve(const double* aP, const double* aE, const double* aW, const dou
     }
     dense_t D;
     dense_setup(&Ls[nl - 1], &D);
+    const int cut = g_cut_on ? mg_cut(Ls, nl) : -1;
+    g_last_cut = cut;
     double bnorm = sqrt(bb);
     int it = 0;
     double res = 0.0;
@@ -420,11 +478,11 @@ int orc_mg_solve(const double* aP, const double* aE, const double* aW, const dou
                 for (size_t k = 0; k < n0; k++) p[k] = r[k] + beta * (p[k] - om * v[k]);
             }
             restart = 0;
-            precond(Ls, nl, &D, p, ph);
+            precond(Ls, nl, &D, cut, p, ph);
             apply_A(L0, ph, v);
             al = rho1 / dotp(r0, v, n0);
             for (size_t k = 0; k < n0; k++) sv[k] = r[k] - al * v[k];
-            precond(Ls, nl, &D, sv, sh);
+            precond(Ls, nl, &D, cut, sv, sh);
             apply_A(L0, sh, t);
             double tt = dotp(t, t, n0);
             om = tt > 0.0 ? dotp(t, sv, n0) / tt : 0.0;

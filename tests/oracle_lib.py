@@ -53,6 +53,8 @@ _SIGS = [
     ("orc_mg_solve", ctypes.c_int,
      [_D] * 6 + [_I] + [ctypes.c_int] * 4 + [ctypes.c_int, ctypes.c_double, _D, _D]),
     ("orc_mg_set_threads", None, [ctypes.c_int]),
+    ("orc_mg_set_cut", None, [ctypes.c_int]),
+    ("orc_mg_last_cut", ctypes.c_int, []),
     ("orc_ndd_step", ctypes.c_int,
      [ctypes.POINTER(abi.NddParams), ctypes.POINTER(abi.NddFields), ctypes.c_int, ctypes.c_int,
       ctypes.POINTER(abi.NddReport)]),
