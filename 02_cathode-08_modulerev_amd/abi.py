@@ -432,6 +432,7 @@ SIGNATURES = [
     ("cathode_ndd_set_learned", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     ("cathode_ndd_set_geometry", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(NddGeometry)]),
     ("cathode_cnn_set_wgrad_valu", ctypes.c_int, [ctypes.c_int]),
+    ("cathode_set_presum_min", ctypes.c_int, [ctypes.c_int]),
     ("cathode_ndd_reference_geometry", ctypes.c_int,
      [c_double_p, c_double_p, ctypes.c_double, ctypes.c_double, ctypes.POINTER(NddGeometry), c_int_p, c_int_p,
       c_int_p]),

@@ -184,6 +184,12 @@ extern "C" {
 
 int cathode_abi_version(void) { return CATHODE_ABI_VERSION; }
 
+int cathode_set_presum_min(int n)
+{
+    set_presum_min(n);
+    return CATHODE_OK;
+}
+
 const char* cathode_last_error(void) { return get_error(); }
 
 void cathode_last_status(cathode_status* st)

@@ -2,6 +2,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <atomic>
 #include <mutex>
 #include <chrono>
 
@@ -1330,6 +1331,7 @@ void MGSolver::build(int ni, int nj)
     grab(scal, nsc);
     grab(kv, 9 * n0);
     grab(cinv64, MG_COARSEST_MAX * MG_COARSEST_MAX);
+    grab(presum, PS_COUNT + 2);
     if (arena) grab(dmirror, nsc);
     // V-cycle cut candidates: level 0 and the last level of the first fine
     // launch unit below it (levels 1 and 2, which run as a pair unless
@@ -1409,7 +1411,8 @@ void MGSolver::setup(int mode, hipStream_t s, const WarmSet& ws)
         a.ni = F.ni; a.nj = F.nj; a.ld = F.ld; a.act = act0.p; a.s = sbuf; a.w = w; a.m = mass.p + level_off[0];
         a.aP = F.aP; a.aE = F.aE; a.aW = F.aW; a.aN = F.aN; a.aS = F.aS;
         a.fP = G.aP; a.fE = G.aE; a.fW = G.aW; a.fN = G.aN; a.fS = G.aS;
-        a.part = part.p; a.nb = nb; a.nact = (double)F.ni * (double)F.nj;
+        a.part = presummed(part.p, nb, 2, 1u, PS_SETUP, a.nb, s);  // max|aP| and |b|^2 partials
+        a.nact = (double)F.ni * (double)F.nj;
         a.nf = nf;
         a.pi[nf] = a.pj[nf] = 1;
         for (int l = nf - 1; l >= 0; l--) {
@@ -1869,8 +1872,7 @@ int MGSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_sta
         fd.first = first;
         fd.pin = pa_;
         fd.ps = pb_;
-        fd.pred = part3 + nb;
-        fd.npred = nb;
+        fd.pred = presummed(part3 + nb, nb, 1, 0u, PS_RHO1, fd.npred, s);  // (r0, r)
         fd.s_rho1 = s_new;
         fd.s_rho = s_prev;
         fu.epi = 1;
@@ -1884,8 +1886,7 @@ int MGSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_sta
         fd.first = 0;
         fd.pin = nullptr;
         fd.ps = sv;
-        fd.pred = part.p;
-        fd.npred = ntiles;
+        fd.pred = presummed(part.p, ntiles, 1, 0u, PS_R0V, fd.npred, s);  // (r0, v)
         fd.s_r0v = SC_R0V;
         fu.y = sh;
         fu.out = t;
@@ -1893,9 +1894,10 @@ int MGSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_sta
         fu.want_tt = 1;
         if (mixed) vcycle<float>(L32, s, fd, fu, cut);
         else vcycle<double>(L64, s, fd, fu, cut);
+        int n2 = 0;
+        const double* p2 = presummed(part.p, ntiles, 2, 0u, PS_OM, n2, s);  // (t, s), (t, t)
         hipLaunchKernelGGL(k_bicg_x, g, blk, 0, s, F.ni, F.nj, F.ld, F.aP, F.aE, F.aW, F.aN, F.aS, n9, n0, bsol.p,
-                           act0.p, xa_, xb_, r, r0, ph, sh, sv, t, scal.p, part3, nb, (const double*)part.p, ntiles,
-                           host_dst());
+                           act0.p, xa_, xb_, r, r0, ph, sh, sv, t, scal.p, part3, nb, p2, n2, host_dst());
         CATHODE_LAUNCH_CHECK();
     };
     // V-cycle output on level 0: the finest up kernel's xu (x when the
@@ -1918,7 +1920,8 @@ int MGSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_sta
             ra.b = bsol.p; ra.x = xa; ra.act = act0.p;
             ra.r = r; ra.r0 = r0; ra.p = pa; ra.v = v; ra.part = part3; ra.nb = nb;
             ra.xout = (zero_guess || ws.k > 0) ? xa : nullptr;
-            ra.ws = ws; ra.wpart = wpart.p; ra.wnb = warm_nb;
+            ra.ws = ws;
+            if (ws.k > 0) ra.wpart = presummed(wpart.p, warm_nb, WARM_NP, 0u, PS_WARM, ra.wnb, s);
             ra.dpart = dpart_ok ? dpart.p : nullptr; ra.dnb = setup_dnb; ra.dstride = dpart_n; ra.sc = scal.p;
             // a cut V-cycle reads no tail level: the chain stays pending
             // until a solve on this hierarchy needs it
@@ -2077,6 +2080,50 @@ int MGSolver::solve(int maxITR, double tol, int mode, hipStream_t s, cathode_sta
         st->seconds = std::chrono::duration<double>(t1 - t0).count();
     }
     return it;
+}
+
+// ------------------------------------------------------------------ partial pre-reduction
+// Fused kernels whose every workgroup needs a global sum (the BiCGStab
+// scalars, the warm start's products, the setup's max|aP| and |b|^2) read
+// the previous launch's per-workgroup partials.  That read is
+// (workgroups) x (partials), so above presum_min() partials one workgroup sums
+// each vector first, in a fixed order, and the consumers read one value.
+constexpr int PRESUM_NT = 1024;
+static std::atomic<int> g_presum_min{PRESUM_MIN_DEFAULT};
+int presum_min() { return g_presum_min.load(std::memory_order_relaxed); }
+void set_presum_min(int n) { g_presum_min.store(n < 0 ? PRESUM_MIN_DEFAULT : n, std::memory_order_relaxed); }
+
+// dst[v] = sum (max where bit v of max_mask is set) of src[v * n + q], q < n
+__global__ void __launch_bounds__(PRESUM_NT) k_presum(const double* __restrict__ src, int n, int nvec,
+                                                      unsigned max_mask, double* __restrict__ dst)
+{
+    __shared__ double lds[PRESUM_NT / 64];
+    for (int v = 0; v < nvec; v++) {
+        const bool mx = (max_mask >> v) & 1u;
+        const double* p = src + (size_t)v * n;
+        double a = mx ? -INFINITY : 0.0;
+        for (int q = threadIdx.x; q < n; q += PRESUM_NT) a = mx ? fmax(a, p[q]) : a + p[q];
+        a = mx ? block_max<PRESUM_NT>(a, lds) : block_sum<PRESUM_NT>(a, lds);
+        if (threadIdx.x == 0) dst[v] = a;
+    }
+}
+
+// the (pointer, count) a consumer of nvec partial vectors of length n
+// (vector v at src + v * n) reads: src and n, or the pre-reduced slots and 1
+const double* MGSolver::presummed(const double* src, int n, int nvec, unsigned max_mask, int slot, int& count,
+                                  hipStream_t s)
+{
+    count = n;
+    if (n <= presum_min()) return src;
+    presum_launch(src, n, nvec, max_mask, presum.p + slot, s);
+    count = 1;
+    return presum.p + slot;
+}
+
+void presum_launch(const double* src, int n, int nvec, unsigned max_mask, double* dst, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_presum, dim3(1), dim3(PRESUM_NT), 0, s, src, n, nvec, max_mask, dst);
+    CATHODE_LAUNCH_CHECK();
 }
 
 // ------------------------------------------------------------------ reductions (common.hpp)

@@ -296,7 +296,16 @@ private:
     int pair_cfg = 0;          // their tile configuration (VP_CFGS; CATHODE_PAIR_TILE picks another)
     int tail_cells = 0;        // LDS cells of the tail levels
     DevBuf<double> part;       // reduction partials (2 slots)
-    DevBuf<double> cinv64;     // dense inverse of the coarsest level (FP64, used by both precisions)
+    // Partial vectors that every workgroup of the next launch sums: above
+    // presum_min() partials one workgroup reduces them first (fixed order) into
+    // these slots, and the consumers read one value per vector, so their
+    // cost stays linear in the grid (below it they sum the partials
+    // themselves, which saves a launch on small grids)
+    enum { PS_RHO1 = 0, PS_R0V = 1, PS_OM = 2, PS_SETUP = 4, PS_WARM = 6, PS_COUNT = 6 + WARM_NP };
+    DevBuf<double> presum;
+    const double* presummed(const double* src, int n, int nvec, unsigned max_mask, int slot, int& count,
+                            hipStream_t s);
+    DevBuf<double> cinv64;    // dense inverse of the coarsest level (FP64, used by both precisions)
     DevBuf<double> kv;         // BiCGStab vectors r, r0, p, v, s, t, ph, sh, p (second buffer)
     DevBuf<double> scal;       // device scalars: [0] |b|^2, [1] |r|^2, [2] scale, [3] f32 matrix scale
     double* h_scal = nullptr;  // pinned mirror of scal (arena mode: the caller's host block)
@@ -320,6 +329,15 @@ private:
     double* host_dst() { return arena ? dmirror.p : h_scal; }
     void pull_host(size_t n, hipStream_t s);
 };
+
+// Above presum_min() partials, kernels whose every workgroup needs a global sum
+// or maximum read it from one value that presum_launch forms first:
+// dst[v] = the fixed-order sum (maximum where bit v of max_mask is set) of
+// src[v * n + q], q < n, in one workgroup
+constexpr int PRESUM_MIN_DEFAULT = 4096;
+int presum_min();             // the current threshold (cathode_set_presum_min)
+void set_presum_min(int n);
+void presum_launch(const double* src, int n, int nvec, unsigned max_mask, double* dst, hipStream_t s);
 
 // FP64 residual-norm helper used by tests of the library itself
 double rel_residual_dev(const double* aP, const double* aE, const double* aW, const double* aN,

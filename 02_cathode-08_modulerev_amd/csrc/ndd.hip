@@ -1469,8 +1469,17 @@ static void ndd_one_step(NddHandle& H, bool report)
         }
         NDD_GEO_LAUNCH(H, k_energy_guard_max, ggrid(V.ni, V.nj), gblk(), V, F[F_RHOEPS], (const double*)H.s_old,
                        (const double*)F[F_RHOE], H.part.p);
+        // max n_e: every workgroup of the update reads the guard's partials
+        // (one pre-reduced value on large grids, slot 24)
+        const double* gp = H.part.p;
+        int gnb = H.nb;
+        if (gnb > presum_min()) {
+            presum_launch(H.part.p, gnb, 1, 1u, H.scal.p + 24, H.s);
+            gp = H.scal.p + 24;
+            gnb = 1;
+        }
         NDD_GEO_LAUNCH(H, k_te_update, ggrid(V.ni, V.nj), gblk(), V, (const double*)F[F_RHOEPS],
-                       (const double*)F[F_RHOE], (const double*)H.part.p, H.nb, F[F_TE]);
+                       (const double*)F[F_RHOE], gp, gnb, F[F_TE]);
         CATHODE_LAUNCH_CHECK();
     }
     H.rep.steps++;

@@ -68,6 +68,16 @@ int cathode_dev_alloc(size_t bytes, void** ptr);
 int cathode_dev_free(void* ptr);
 int cathode_dev_copy(void* dst, const void* src, size_t bytes, int kind);
 
+/* Global sums inside the fused solver kernels: every workgroup of a launch
+ * reads the previous launch's per-workgroup partials, which costs
+ * (workgroups x partials); above n partials one workgroup pre-reduces them
+ * (fixed order) and the launch reads one value per sum.  n < 0 restores the
+ * default (4096: grids above about 1024 x 1024 cells); n = 0 pre-reduces
+ * always.  Library-wide, set between solves; results agree with either
+ * setting to rounding (the summation order differs).  No counterpart in the
+ * reference (HYPRE's reductions are its own). */
+int cathode_set_presum_min(int n);
+
 /* ------------------------------------------------------------------ */
 /* structured 5-point solves (drop-in for include/solverFuncs.hpp)    */
 /* ------------------------------------------------------------------ */
