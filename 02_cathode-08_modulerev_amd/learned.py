@@ -89,6 +89,51 @@ def train(lib, net, feat, target, epochs=4, batch=8, lr=1e-3, seed=0, log=None):
     return losses
 
 
+def train_streaming(lib, net, pairs, chunk, steps_per_seed, ni, nj, h, batch=8, lr=1e-3, seed0=100, log=None):
+    """One pass over `pairs` fresh pairs, generated `chunk` at a time (the
+    trajectories of chunk / steps_per_seed seeds, then trained on, shuffled,
+    and freed: the whole set need not fit in memory); the learning rate
+    falls linearly to a tenth over the pass.  Returns the mean loss of each
+    chunk and the seconds spent generating and training."""
+    per_chunk_seeds = max(1, chunk // steps_per_seed)
+    nchunks = max(1, pairs // (per_chunk_seeds * steps_per_seed))
+    rng = np.random.default_rng(seed0)
+    losses, t_gen, t_train = [], 0.0, 0.0
+    fb = abi.DeviceArray(lib, (batch, 3, ni + 2, nj + 2))
+    tb = abi.DeviceArray(lib, (batch, ni + 2, nj + 2))
+    try:
+        for c in range(nchunks):
+            t0 = time.perf_counter()
+            seeds = list(range(seed0 + c * per_chunk_seeds, seed0 + (c + 1) * per_chunk_seeds))
+            feat, target = generate_pairs(lib, seeds, steps_per_seed, ni, nj, h)
+            t1 = time.perf_counter()
+            S = feat.shape[0]
+            order = rng.permutation(S)
+            lr_c = lr * (1.0 - 0.9 * c / max(1, nchunks - 1))
+            cl = []
+            for b0 in range(0, S - batch + 1, batch):
+                for q in range(batch):
+                    fb.copy_from(q, feat, int(order[b0 + q]))
+                    tb.copy_from(q, target, int(order[b0 + q]))
+                loss = ctypes.c_double()
+                abi.check(lib.cathode_cnn_train_step_dev(net, ni, nj, batch, fb.ptr, tb.ptr, ctypes.c_float(lr_c),
+                                                         ctypes.byref(loss)), lib)
+                cl.append(loss.value)
+            feat.free()
+            target.free()
+            t2 = time.perf_counter()
+            t_gen += t1 - t0
+            t_train += t2 - t1
+            losses.append(float(np.mean(cl)))
+            if log:
+                log(f"chunk {c + 1}/{nchunks}: {S} pairs (seeds {seeds[0]}..{seeds[-1]}), mean loss {losses[-1]:.4f} "
+                    f"(lr {lr_c:.1e}); generate {t1 - t0:.1f} s, train {t2 - t1:.1f} s")
+    finally:
+        fb.free()
+        tb.free()
+    return losses, t_gen, t_train
+
+
 def get_params(lib, net):
     n = lib.cathode_cnn_num_params()
     w = np.zeros(n, dtype=np.float32)
@@ -169,6 +214,11 @@ def main():
     ap.add_argument("--grid", default="512x256",
                     help="NIxNJ of the evaluation (the same domain; 1024x512 is the learned-initialisation "
                          "workload of 1024 x 512 cells)")
+    ap.add_argument("--pairs", type=int, default=0,
+                    help="stream this many fresh pairs at --grid (chunks of --chunk, one pass) instead of "
+                         "training epochs on --train-seeds x --steps pairs at 512x256")
+    ap.add_argument("--chunk", type=int, default=500)
+    ap.add_argument("--steps-per-seed", type=int, default=20)
     args = ap.parse_args()
     lib = abi.load()
     abi.check(lib.cathode_init(0), lib)
@@ -180,6 +230,22 @@ def main():
                       h=5e-5 * 512 / ni, fp64=True)
         print(json.dumps({"grid": [ni, nj], "seeds": args.eval_seeds, "steps_per_seed": args.steps,
                           "weights": os.path.basename(WEIGHTS), "eval": ev}))
+        lib.cathode_cnn_destroy(net)
+        lib.cathode_finalize()
+        return
+    if args.pairs > 0:
+        ni, nj = (int(v) for v in args.grid.lower().split("x"))
+        h = 5e-5 * 512 / ni
+        net = ctypes.c_void_p()
+        abi.check(lib.cathode_cnn_create(1, ctypes.byref(net)), lib)
+        losses, t_gen, t_train = train_streaming(lib, net, args.pairs, args.chunk, args.steps_per_seed, ni, nj, h,
+                                                 args.batch, args.lr, log=log)
+        np.save(args.out, get_params(lib, net), allow_pickle=False)
+        ev = evaluate(lib, net, list(range(900, 900 + args.eval_seeds)), args.steps, ni=ni, nj=nj, h=h, fp64=True)
+        print(json.dumps({"grid": [ni, nj], "pairs": len(losses) * (args.chunk // args.steps_per_seed)
+                          * args.steps_per_seed, "chunk": args.chunk, "generate_seconds": t_gen,
+                          "train_seconds": t_train, "loss_per_chunk": losses, "eval_seeds": args.eval_seeds,
+                          "eval_steps_per_seed": args.steps, "eval": ev}))
         lib.cathode_cnn_destroy(net)
         lib.cathode_finalize()
         return
