@@ -6,8 +6,9 @@ mixed path and the plain FP64 path are stepped one implicit step at a time
 next to the FP64 oracle (its own multigrid to the same 1e-10 relative
 residual, fluidModule.hpp:2080-4957 restated).  After every step each field
 must agree with the oracle and with the FP64 GPU path to <= 1e-9 relative
-L2 (the bar is cond(A) * 1e-10 for these systems); the per-field errors are
-printed for the record.
+L2 (the bar is cond(A) * 1e-10 for these systems), over the workload's 100
+steps; the per-field errors are printed for the record (every tenth step and
+the worst of each path and field).
 """
 import ctypes
 import importlib
@@ -25,7 +26,7 @@ pytestmark = pytest.mark.gpu
 FIELDS = ["phi", "rhoe", "rhoi", "rhoeps", "Te", "Ex", "Er", "rhoUex", "rhoUer", "rhoUix", "rhoUir", "nUex",
           "nUer", "Gx", "Gr"]
 TOL = 1e-9
-STEPS = 5
+STEPS = 100  # the mixed-precision workload's length
 
 
 class Stepper:
@@ -65,6 +66,7 @@ def test_learned_mixed_fp64_per_step_vs_oracle(hiplib):
         ref = W.copy_state(st)
         O.lib().orc_mg_set_threads(8)
         worst = {k: 0.0 for k in runs}
+        worst_f = {k: {f: 0.0 for f in FIELDS} for k in runs}
         for s in range(1, STEPS + 1):
             rr = O.ndd_step(p64, ref, 1, solver=1)
             assert rr.nonconverged == 0
@@ -74,15 +76,21 @@ def test_learned_mixed_fp64_per_step_vs_oracle(hiplib):
                 e_or = {k: O.rel_l2(r.st[k], ref[k]) for k in FIELDS}
                 e_64 = {k: O.rel_l2(r.st[k], runs["fp64"].st[k]) for k in FIELDS}
                 worst[name] = max(worst[name], max(e_or.values()))
-                print(f"step {s} {name}: vs oracle max {max(e_or.values()):.2e} "
-                      f"(phi {e_or['phi']:.1e} ne {e_or['rhoe']:.1e} ni {e_or['rhoi']:.1e} "
-                      f"neps {e_or['rhoeps']:.1e} Te {e_or['Te']:.1e}); vs FP64 GPU max {max(e_64.values()):.2e}",
-                      flush=True)
+                for f in FIELDS:
+                    worst_f[name][f] = max(worst_f[name][f], e_or[f])
+                if s % 10 == 0 or s == 1:
+                    print(f"step {s} {name}: vs oracle max {max(e_or.values()):.2e} "
+                          f"(phi {e_or['phi']:.1e} ne {e_or['rhoe']:.1e} ni {e_or['rhoi']:.1e} "
+                          f"neps {e_or['rhoeps']:.1e} Te {e_or['Te']:.1e}); vs FP64 GPU max {max(e_64.values()):.2e}",
+                          flush=True)
                 bad = {k: v for k, v in e_or.items() if not v <= TOL}
                 assert not bad, (name, s, bad)
                 bad = {k: v for k, v in e_64.items() if not v <= TOL}
                 assert not bad, (name, s, bad)
         print(f"worst per path over {STEPS} steps: {worst}", flush=True)
+        for name in runs:
+            print(f"worst per field, {name}: " + ", ".join(f"{f} {v:.1e}" for f, v in worst_f[name].items()),
+                  flush=True)
     finally:
         for r in runs.values():
             r.close()
