@@ -457,7 +457,7 @@ def main():
     ap.add_argument("--no-sweep", dest="sweep", action="store_false",
                     help="skip the 64-instance bias sweep (batched vs one at a time), run by default")
     ap.add_argument("--sweep-steps", type=int, default=50)
-    ap.add_argument("--sweep-workers", type=int, default=8)
+    ap.add_argument("--sweep-workers", type=int, default=16)
     ap.add_argument("--rows-only", action="store_true",
                     help="only time the ion block, the neutral update and the transport update (one JSON line)")
     ap.add_argument("--calibrate-copy", action="store_true",
