@@ -29,9 +29,10 @@ namespace cathode {
 struct TileCfg {
     int ti, tj, nt;
 };
-constexpr int VT_NCFG = 8;
-constexpr TileCfg VT_CFGS[VT_NCFG] = {{16, 32, 256}, {16, 32, 512}, {8, 32, 256}, {16, 64, 512},
-                                      {32, 32, 512}, {8, 64, 512},  {8, 16, 256}, {16, 16, 256}};
+constexpr int VT_NCFG = 10;
+constexpr TileCfg VT_CFGS[VT_NCFG] = {{16, 32, 256}, {16, 32, 512}, {8, 32, 256},   {16, 64, 512},
+                                      {32, 32, 512}, {8, 64, 512},  {8, 16, 256},   {16, 16, 256},
+                                      {32, 64, 1024}, {64, 32, 1024}};  // the last two: less halo on large grids
 constexpr int TAIL_MAX_CELLS = 512;     // levels with at most this many cells go to the tail
 constexpr int TAIL_NT = 512;            // threads of the tail workgroup
 constexpr int TAIL_MAX_LEVELS = 16;
@@ -312,6 +313,8 @@ inline void launch_vtile(int cfg, bool top, hipStream_t s, const VLevel<T>& V, i
     case 5: launch_vtile_cfg<T, DOWN, 5>(top, s, V, fi, fj, w, cni, cnj, cld, cbx, fu, ev0, ev1); break;
     case 6: launch_vtile_cfg<T, DOWN, 6>(top, s, V, fi, fj, w, cni, cnj, cld, cbx, fu, ev0, ev1); break;
     case 7: launch_vtile_cfg<T, DOWN, 7>(top, s, V, fi, fj, w, cni, cnj, cld, cbx, fu, ev0, ev1); break;
+    case 8: launch_vtile_cfg<T, DOWN, 8>(top, s, V, fi, fj, w, cni, cnj, cld, cbx, fu, ev0, ev1); break;
+    case 9: launch_vtile_cfg<T, DOWN, 9>(top, s, V, fi, fj, w, cni, cnj, cld, cbx, fu, ev0, ev1); break;
     default: launch_vtile_cfg<T, DOWN, 0>(top, s, V, fi, fj, w, cni, cnj, cld, cbx, fu, ev0, ev1); break;
     }
 }

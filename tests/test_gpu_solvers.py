@@ -172,7 +172,7 @@ def test_solverfuncs_dropin_header():
     assert r.returncode == 0 and "ok dropin" in r.stdout, r.stdout + r.stderr
 
 
-@pytest.mark.parametrize("cfg", range(8))
+@pytest.mark.parametrize("cfg", range(10))
 def test_struct5_every_tile_config(hiplib, cfg, monkeypatch):
     """Each fused-tile configuration the autotuner can pick gives a converged,
     oracle-matching solve (odd grid, so partial tiles and odd coarsening occur)."""
