@@ -1312,7 +1312,7 @@ void MGSolver::build(int ni, int nj)
         const char* e = std::getenv("CATHODE_PAIR");
         pair_on = !(e && e[0] == '0');
         const char* c = std::getenv("CATHODE_PAIR_TILE");
-        pair_cfg = (c && *c) ? std::atoi(c) : 2;  // 8 x 16 tiles of the second level, 1024 threads
+        pair_cfg = (c && *c) ? std::atoi(c) : 3;  // 8 x 8 tiles of the second level, 1024 threads
     }
     // levels from lt on run inside the one-workgroup tail kernel
     lt = nl - 1;

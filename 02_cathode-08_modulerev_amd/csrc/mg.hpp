@@ -293,7 +293,7 @@ private:
     bool autotune_on = true;
     int lt = 0;                // first level handled by the one-workgroup tail kernel
     bool pair_on = true;       // fine levels below the finest run two per launch (k_vpair_*)
-    int pair_cfg = 2;          // their tile configuration (VP_CFGS; CATHODE_PAIR_TILE picks another)
+    int pair_cfg = 3;          // their tile configuration (VP_CFGS; CATHODE_PAIR_TILE picks another)
     int tail_cells = 0;        // LDS cells of the tail levels
     DevBuf<double> part;       // reduction partials (2 slots)
     // Partial vectors that every workgroup of the next launch sums: above
