@@ -585,8 +585,9 @@ __global__ void __launch_bounds__(NT)
 }
 
 // the pair kernels' tiles of the second level and workgroup sizes
-constexpr int VP_NCFG = 5;
-constexpr TileCfg VP_CFGS[VP_NCFG] = {{16, 16, 1024}, {8, 16, 512}, {8, 16, 1024}, {8, 8, 1024}, {8, 8, 512}};
+constexpr int VP_NCFG = 6;
+constexpr TileCfg VP_CFGS[VP_NCFG] = {{16, 16, 1024}, {8, 16, 512}, {8, 16, 1024}, {8, 8, 1024}, {8, 8, 512},
+                                      {4, 8, 1024}};
 
 template <class T, int CFG>
 inline void launch_vpair_cfg(bool down, hipStream_t s, const VLevel<T>& F, const VLevel<T>& C, int fi3, int fj3,
@@ -611,6 +612,7 @@ inline void launch_vpair(int cfg, bool down, hipStream_t s, const VLevel<T>& F, 
     case 2: launch_vpair_cfg<T, 2>(down, s, F, C, fi3, fj3, c3ni, c3nj, c3ld, c3); break;
     case 3: launch_vpair_cfg<T, 3>(down, s, F, C, fi3, fj3, c3ni, c3nj, c3ld, c3); break;
     case 4: launch_vpair_cfg<T, 4>(down, s, F, C, fi3, fj3, c3ni, c3nj, c3ld, c3); break;
+    case 5: launch_vpair_cfg<T, 5>(down, s, F, C, fi3, fj3, c3ni, c3nj, c3ld, c3); break;
     default: launch_vpair_cfg<T, 0>(down, s, F, C, fi3, fj3, c3ni, c3nj, c3ld, c3); break;
     }
 }

@@ -43,7 +43,7 @@ def _run(lib, p, st, nsteps, pair, tile=0):
     return out, its
 
 
-@pytest.mark.parametrize("tile", [0, 1, 2, 3, 4])  # the pair tile configurations (VP_CFGS)
+@pytest.mark.parametrize("tile", [0, 1, 2, 3, 4, 5])  # the pair tile configurations (VP_CFGS)
 @pytest.mark.parametrize("mode", [abi.CATHODE_MODE_FP64, abi.CATHODE_MODE_MIXED])
 @pytest.mark.parametrize("shape", [(256, 128), (512, 256), (250, 130)])
 def test_pair_kernels_bitwise(hiplib, mode, shape, tile):
