@@ -1,6 +1,6 @@
 """GPU parity of the device-resident implicit NDD step against the oracle.
 
-Bar (north star): after each implicit step solved to relative residual
+Bar: after each implicit step solved to relative residual
 1e-10, every field agrees with the FP64 oracle to <= 1e-9 relative L2.  The
 oracle solves its systems exactly (banded LU) on the small grids and with its
 own multigrid copy to the same tolerance on the full 512x256 grid.
